@@ -1,0 +1,754 @@
+//#####################################################################
+// mpm_oracle — CPU restatement of PhysBAM's explicit APIC MPM step.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP product
+// path: only the test suite, the smoke check and the CPU-baseline leg of the
+// benchmark may load it.  The product library never links or calls it.
+//
+// It restates, in the same operation order as the reference (serial scatter,
+// threads=1), the functions below.  Citations are relative to the PhysBAM
+// checkout (Public_Library/...).
+//   weights      Hybrid_Methods/Iterators/PARTICLE_GRID_WEIGHTS_SPLINE.cpp:33-56, :93-105, :133-140, :152-158
+//   svd3         Core/Matrices/MATRIX_3X3.cpp:36-236   (Gast et al. 2016 implicit-shift QR)
+//   svd2         Core/Matrices/MATRIX_2X2.h:313-322, SYMMETRIC_MATRIX_2X2.h:364-375
+//   P_hat        Deformables/Constitutive_Models/COROTATED_FIXED.cpp:59-65,
+//                NEO_HOOKEAN.cpp:44-81 (Clamp_To_Hyperbola + P_From_Strain)
+//   stress       Hybrid_Methods/Forces/MPM_FINITE_ELEMENTS.cpp:47-67, :84-99
+//   gravity      Hybrid_Methods/Forces/MPM_GRAVITY.cpp:62-69
+//   P2G          Hybrid_Methods/Examples_And_Drivers/MPM_DRIVER.cpp:245-306
+//   forces       MPM_DRIVER.cpp:959-1060 (symplectic branch)
+//   collisions   Hybrid_Methods/System/MPM_OBJECTIVE.cpp:150-224; Collisions/MPM_COLLISION_OBJECT.cpp:27-75;
+//                Core/Math_Tools/RANGE.cpp:38-63 (box phi and normal)
+//   friction     MPM_DRIVER.cpp:1064-1094
+//   G2P          MPM_DRIVER.cpp:450-515
+//   invalidation MPM_DRIVER.cpp:1219-1239, simulated list :1142-1157
+//   reductions   Examples_And_Drivers/MPM_EXAMPLE.cpp:209-315
+//
+// Each piece is pinned in tests/ against the reference harness outputs
+// (golden vectors under tests/golden/) and against closed forms.
+//#####################################################################
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace oracle{
+
+template<class T,int d> struct Vec
+{
+    T x[d];
+    Vec(){for(int i=0;i<d;i++) x[i]=0;}
+    T& operator[](int i){return x[i];}
+    const T& operator[](int i) const{return x[i];}
+};
+template<class T,int d> Vec<T,d> operator+(const Vec<T,d>& a,const Vec<T,d>& b){Vec<T,d> r;for(int i=0;i<d;i++) r[i]=a[i]+b[i];return r;}
+template<class T,int d> Vec<T,d> operator-(const Vec<T,d>& a,const Vec<T,d>& b){Vec<T,d> r;for(int i=0;i<d;i++) r[i]=a[i]-b[i];return r;}
+template<class T,int d> Vec<T,d> operator*(T s,const Vec<T,d>& a){Vec<T,d> r;for(int i=0;i<d;i++) r[i]=s*a[i];return r;}
+template<class T,int d> Vec<T,d> operator*(const Vec<T,d>& a,T s){Vec<T,d> r;for(int i=0;i<d;i++) r[i]=a[i]*s;return r;}
+template<class T,int d> T dot(const Vec<T,d>& a,const Vec<T,d>& b){T s=a[0]*b[0];for(int i=1;i<d;i++) s+=a[i]*b[i];return s;}
+template<class T,int d> T magnitude(const Vec<T,d>& a){return std::sqrt(dot(a,a));}
+
+// Column-major storage x[i+d*j], as MATRIX<T,d> (Core/Matrices/MATRIX_3X3.h:115-119).
+template<class T,int d> struct Mat
+{
+    T x[d*d];
+    Mat(){for(int i=0;i<d*d;i++) x[i]=0;}
+    static Mat Identity(){Mat m;for(int i=0;i<d;i++) m.x[i*(d+1)]=1;return m;}
+    T& operator()(int i,int j){return x[i+d*j];}
+    const T& operator()(int i,int j) const{return x[i+d*j];}
+    Vec<T,d> Row(int i) const{Vec<T,d> r;for(int j=0;j<d;j++) r[j]=x[i+d*j];return r;}
+    Vec<T,d> Col(int j) const{Vec<T,d> r;for(int i=0;i<d;i++) r[i]=x[i+d*j];return r;}
+    void Set_Row(int i,const Vec<T,d>& v){for(int j=0;j<d;j++) x[i+d*j]=v[j];}
+    void Set_Col(int j,const Vec<T,d>& v){for(int i=0;i<d;i++) x[i+d*j]=v[i];}
+};
+template<class T,int d> Mat<T,d> operator*(const Mat<T,d>& A,const Mat<T,d>& B)
+{Mat<T,d> C;for(int j=0;j<d;j++) for(int i=0;i<d;i++){T s=A(i,0)*B(0,j);for(int k=1;k<d;k++) s+=A(i,k)*B(k,j);C(i,j)=s;}return C;}
+template<class T,int d> Vec<T,d> operator*(const Mat<T,d>& A,const Vec<T,d>& v)
+{Vec<T,d> r;for(int i=0;i<d;i++){T s=A(i,0)*v[0];for(int k=1;k<d;k++) s+=A(i,k)*v[k];r[i]=s;}return r;}
+template<class T,int d> Mat<T,d> Times_Transpose(const Mat<T,d>& A,const Mat<T,d>& B)
+{Mat<T,d> C;for(int j=0;j<d;j++) for(int i=0;i<d;i++){T s=A(i,0)*B(j,0);for(int k=1;k<d;k++) s+=A(i,k)*B(j,k);C(i,j)=s;}return C;}
+template<class T,int d> Mat<T,d> operator*(T s,const Mat<T,d>& A){Mat<T,d> r;for(int i=0;i<d*d;i++) r.x[i]=s*A.x[i];return r;}
+template<class T,int d> Mat<T,d> operator+(const Mat<T,d>& A,const Mat<T,d>& B){Mat<T,d> r;for(int i=0;i<d*d;i++) r.x[i]=A.x[i]+B.x[i];return r;}
+template<class T,int d> Mat<T,d> Outer(const Vec<T,d>& u,const Vec<T,d>& v){Mat<T,d> r;for(int j=0;j<d;j++) for(int i=0;i<d;i++) r(i,j)=u[i]*v[j];return r;}
+template<class T,int d> T Frobenius_Norm_Squared(const Mat<T,d>& A){T s=A.x[0]*A.x[0];for(int i=1;i<d*d;i++) s+=A.x[i]*A.x[i];return s;}
+
+//#####################################################################
+// 3x3 SVD: MATRIX_3X3.cpp:36-236.  U, V rotations; |s0|>=|s1|>=|s2|; only s2 may be negative.
+//#####################################################################
+template<class T> Vec<T,2> Givens(T a,T b)
+{
+    Vec<T,2> v;v[0]=a;v[1]=-b;
+    T m=std::sqrt(a*a+b*b);
+    if(m){T s=1/m;v[0]=a*s;v[1]=-b*s;}
+    else{v[0]=1;v[1]=0;}
+    return v;
+}
+template<class T> void Apply_L(Mat<T,3>& A,const Vec<T,2>& u,int i,int j)
+{
+    Vec<T,3> v=A.Row(i),w=A.Row(j);
+    A.Set_Row(i,u[0]*v+u[1]*w);
+    A.Set_Row(j,u[0]*w-u[1]*v);
+}
+template<class T> void Apply_R(Mat<T,3>& A,const Vec<T,2>& u,int i,int j)
+{
+    Vec<T,3> v=A.Col(i),w=A.Col(j);
+    A.Set_Col(i,u[0]*v+u[1]*w);
+    A.Set_Col(j,u[0]*w-u[1]*v);
+}
+template<class T> void Zero_L(Mat<T,3>& U,Mat<T,3>& A,int r0,int r1,int c)
+{
+    Vec<T,2> g=Givens(A(r0,c),A(r1,c));
+    Apply_L(A,g,r1,r0);
+    Apply_R(U,g,r1,r0);
+    A(r1,c)=0;
+}
+template<class T> void Zero_R(Mat<T,3>& V,Mat<T,3>& A,int c0,int c1,int r)
+{
+    Vec<T,2> g=Givens(A(r,c0),A(r,c1));
+    Apply_R(A,g,c1,c0);
+    Apply_R(V,g,c1,c0);
+    A(r,c1)=0;
+}
+template<class T> void Zero_Chasing(Mat<T,3>& U,Mat<T,3>& A,Mat<T,3>& V)
+{
+    Zero_L(U,A,0,1,0);
+    Zero_R(V,A,1,2,0);
+    Zero_L(U,A,1,2,1);
+}
+template<class T> void Swap_Col(Mat<T,3>& U,int i,int j)
+{
+    Vec<T,3> u=U.Col(i),v=U.Col(j);
+    U.Set_Col(i,v);
+    U.Set_Col(j,(T)-1*u);
+}
+template<class T> void Swap_Row(Mat<T,3>& U,int i,int j)
+{
+    Vec<T,3> u=U.Row(i),v=U.Row(j);
+    U.Set_Row(i,v);
+    U.Set_Row(j,(T)-1*u);
+}
+template<class T> void Swap(Mat<T,3>& U,Vec<T,3>& D,Mat<T,3>& V,int i,int j)
+{
+    Swap_Col(U,i,j);
+    Swap_Col(V,i,j);
+    T t=D[i];D[i]=D[j];D[j]=t;
+}
+template<class T> void Flip_Sign(Mat<T,3>& U,Vec<T,3>& D,int i,int j)
+{
+    D[i]=-D[i];D[j]=-D[j];
+    U.Set_Col(i,(T)-1*U.Col(i));
+    U.Set_Col(j,(T)-1*U.Col(j));
+}
+// argmax(a,b,c) of Core/Math_Tools/argmax.h: ties go to the earlier index.
+template<class T> int Arg_Max3(T a,T b,T c){return a>=c?(a>=b?0:1):(b>=c?1:2);}
+template<class T> void Sort_SV(Mat<T,3>& U,Vec<T,3>& D,Mat<T,3>& V)
+{
+    int i=Arg_Max3(std::abs(D[0]),std::abs(D[1]),std::abs(D[2]));
+    if(i>0) Swap(U,D,V,0,i);
+    if(std::abs(D[1])<std::abs(D[2])) Swap(U,D,V,1,2);
+    if(D[0]<0) Flip_Sign(U,D,0,D[1]<0?1:2);
+    else if(D[1]<0) Flip_Sign(U,D,1,2);
+}
+
+//#####################################################################
+// 2x2 SVD: MATRIX_2X2.h:313-322 (indefinite polar + symmetric eigenproblem).
+//#####################################################################
+template<class T> void Svd2(const Mat<T,2>& A,Mat<T,2>& U,Vec<T,2>& sv,Mat<T,2>& V)
+{
+    const T* x=A.x;
+    T x03=x[0]+x[3],c,s;
+    if(x03==0){c=0;s=1;}
+    else{T t=(x[1]-x[2])/x03;c=1/std::sqrt(1+t*t);s=t*c;}
+    Mat<T,2> Q;Q.x[0]=c;Q.x[1]=s;Q.x[2]=-s;Q.x[3]=c;
+    T s00=Q.x[0]*x[0]+Q.x[1]*x[1],s10=Q.x[0]*x[2]+Q.x[1]*x[3],s11=Q.x[2]*x[2]+Q.x[3]*x[3];
+    // SYMMETRIC_MATRIX_2X2.h:364-375
+    T a=(T).5*(s00+s11),b=(T).5*(s00-s11),cc=s10;
+    T c2=cc*cc,m=std::sqrt(b*b+c2),k=s00*s11-c2;
+    Vec<T,2> ev;
+    if(a>=0){ev[0]=a+m;ev[1]=ev[0]?k/ev[0]:0;}
+    else{ev[1]=a-m;ev[0]=ev[1]?k/ev[1]:0;}
+    if(ev[1]>ev[0]){T t=ev[0];ev[0]=ev[1];ev[1]=t;}
+    Vec<T,2> c0;
+    if(b>=0){c0[0]=m+b;c0[1]=cc;}else{c0[0]=-cc;c0[1]=b-m;}
+    T mag=std::sqrt(c0[0]*c0[0]+c0[1]*c0[1]);
+    if(mag){T inv=1/mag;c0[0]*=inv;c0[1]*=inv;}else{c0[0]=1;c0[1]=0;}
+    V.x[0]=c0[0];V.x[1]=c0[1];V.x[2]=-c0[1];V.x[3]=c0[0];
+    sv=ev;
+    if(sv[1]<0 && std::abs(sv[1])>=std::abs(sv[0])){
+        T s0=-sv[1],s1=-sv[0];sv[0]=s0;sv[1]=s1;
+        for(int i=0;i<4;i++) Q.x[i]=-Q.x[i];
+        Mat<T,2> W;W.x[0]=V.x[2];W.x[1]=V.x[3];W.x[2]=-V.x[0];W.x[3]=-V.x[1];V=W;}
+    U=Q*V;
+}
+template<class T> void Top_Block(Mat<T,3>& U,Mat<T,3>& B,Mat<T,3>& V,Vec<T,3>& D)
+{
+    Mat<T,2> a,u,v;Vec<T,2> dd;
+    a.x[0]=B(0,0);a.x[1]=B(1,0);a.x[2]=B(0,1);a.x[3]=B(1,1);
+    Svd2(a,u,dd,v);
+    Vec<T,2> uc;uc[0]=u.x[0];uc[1]=u.x[1];
+    Vec<T,2> vc;vc[0]=v.x[0];vc[1]=v.x[1];
+    Apply_R(U,uc,0,1);
+    Apply_R(V,vc,0,1);
+    D[0]=dd[0];D[1]=dd[1];D[2]=B(2,2);
+    Sort_SV(U,D,V);
+}
+template<class T> void Bot_Block(Mat<T,3>& U,Mat<T,3>& B,Mat<T,3>& V,Vec<T,3>& D)
+{
+    Mat<T,2> a,u,v;Vec<T,2> dd;
+    a.x[0]=B(1,1);a.x[1]=B(2,1);a.x[2]=B(1,2);a.x[3]=B(2,2);
+    Svd2(a,u,dd,v);
+    Vec<T,2> uc;uc[0]=u.x[0];uc[1]=u.x[1];
+    Vec<T,2> vc;vc[0]=v.x[0];vc[1]=v.x[1];
+    Apply_R(U,uc,1,2);
+    Apply_R(V,vc,1,2);
+    D[0]=B(0,0);D[1]=dd[0];D[2]=dd[1];
+    Sort_SV(U,D,V);
+}
+template<class T> bool Check_Tolerances(Mat<T,3>& B,Mat<T,3>& U,Vec<T,3>& D,Mat<T,3>& V,T tau)
+{
+    if(std::abs(B(1,2))<=tau){Top_Block(U,B,V,D);return true;}
+    if(std::abs(B(0,1))<=tau){Bot_Block(U,B,V,D);return true;}
+    if(std::abs(B(1,1))<=tau){
+        Vec<T,2> g=Givens(B(1,2),B(2,2));
+        Vec<T,2> o;o[0]=-g[1];o[1]=g[0];
+        Apply_L(B,o,2,1);
+        Apply_R(U,o,2,1);
+        Top_Block(U,B,V,D);
+        return true;}
+    if(std::abs(B(2,2))<=tau){
+        Zero_R(V,B,1,2,1);
+        Zero_R(V,B,0,2,0);
+        Top_Block(U,B,V,D);
+        return true;}
+    if(std::abs(B(0,0))<=tau){
+        Zero_L(U,B,1,0,1);
+        Zero_L(U,B,2,0,2);
+        Bot_Block(U,B,V,D);
+        return true;}
+    return false;
+}
+template<class T> bool Lazy_Sort(Mat<T,3>& U,Mat<T,3>& B,Mat<T,3>& V,int i,int j)
+{
+    T flip_ratio=100;
+    if(std::abs(B(i,i))*flip_ratio>=std::abs(B(j,j))) return false;
+    Swap_Row(B,i,j);
+    Swap_Col(U,i,j);
+    Swap_Col(B,i,j);
+    Swap_Col(V,i,j);
+    return true;
+}
+template<class T> void Bidiagonalize(Mat<T,3>& U,Mat<T,3>& B,Mat<T,3>& V)
+{
+    Zero_L(U,B,1,2,0);
+    Zero_Chasing(U,B,V);
+}
+template<class T> int Svd3(const Mat<T,3>& F,Mat<T,3>& U,Vec<T,3>& D,Mat<T,3>& V)
+{
+    Mat<T,3> B=F;
+    U=V=Mat<T,3>::Identity();
+    Bidiagonalize(U,B,V);
+    T nu=16*std::numeric_limits<T>::epsilon();
+    T half_norm=(T).5*std::sqrt(Frobenius_Norm_Squared(B));
+    T tau=nu*(half_norm<(T)1?(T)1:half_norm);
+    for(int it=0;it<100000;it++){
+        if(Check_Tolerances(B,U,D,V,tau)) return it;
+        bool s01=Lazy_Sort(U,B,V,0,1);
+        bool s02=Lazy_Sort(U,B,V,0,2);
+        bool s12=Lazy_Sort(U,B,V,1,2);
+        if(s01|s02|s12){
+            Bidiagonalize(U,B,V);
+            if(Check_Tolerances(B,U,D,V,tau)) return it;}
+        T al0=B(0,0),al1=B(1,1),al2=B(2,2);
+        T be0=B(0,1),be1=B(1,2);
+        T ga0=al0*be0,ga1=al1*be1;
+        T a0=al1*al1+be0*be0;
+        T a1=al2*al2+be1*be1;
+        T b0=ga1;
+        T dd=(T).5*(a0-a1);
+        T sgn=dd>=0?(T)1:(T)-1;
+        T mu=a1-b0*b0/(dd+sgn*std::sqrt(dd*dd+b0*b0));
+        Vec<T,2> g=Givens(al0*al0-mu,ga0);
+        Apply_R(B,g,1,0);
+        Apply_R(V,g,1,0);
+        Zero_Chasing(U,B,V);}
+    return -1;
+}
+template<class T> int Svd(const Mat<T,3>& F,Mat<T,3>& U,Vec<T,3>& D,Mat<T,3>& V){return Svd3(F,U,D,V);}
+template<class T> int Svd(const Mat<T,2>& F,Mat<T,2>& U,Vec<T,2>& D,Mat<T,2>& V){Svd2(F,U,D,V);return 0;}
+
+//#####################################################################
+// Constitutive models in singular-value space.
+//#####################################################################
+// DIAGONAL_MATRIX::Cofactor_Matrix (DIAGONAL_MATRIX.h:204-219): 2D (y,x); 3D (y*z, z*x, y*x).
+template<class T> Vec<T,2> Cofactor(const Vec<T,2>& s){Vec<T,2> r;r[0]=s[1];r[1]=s[0];return r;}
+template<class T> Vec<T,3> Cofactor(const Vec<T,3>& s){Vec<T,3> r;T yz=s[1]*s[2];r[0]=yz;r[1]=s[2]*s[0];r[2]=s[1]*s[0];return r;}
+template<class T,int d> T Product(const Vec<T,d>& s){T p=s[0];for(int i=1;i<d;i++) p*=s[i];return p;}
+
+// COROTATED_FIXED.cpp:59-65
+template<class T,int d> Vec<T,d> P_Corotated(const Vec<T,d>& s,T mu,T la)
+{
+    Vec<T,d> cof=Cofactor(s);
+    T J=Product(s);
+    T a=2*mu,b=la*(J-1);
+    Vec<T,d> r;
+    for(int i=0;i<d;i++) r[i]=(s[i]-1)*a+cof[i]*b;
+    return r;
+}
+// NEO_HOOKEAN.cpp:44-68 Clamp_To_Hyperbola
+template<class T> Vec<T,2> Clamp_To_Hyperbola(const Vec<T,2>& F,T thr,T root)
+{
+    Vec<T,2> r;
+    if(F[0]*F[0]>thr){r[0]=F[0];r[1]=thr/F[0];}
+    else{r[0]=root;r[1]=root;}
+    return r;
+}
+template<class T> Vec<T,3> Clamp_To_Hyperbola(const Vec<T,3>& F,T thr,T root)
+{
+    Vec<T,3> r;
+    if(F[0]*F[1]*F[1]>thr){r[0]=F[0];r[1]=F[1];r[2]=thr/(F[0]*F[1]);}
+    else if(F[0]*F[0]*F[0]>thr){T c=std::sqrt(thr/F[0]);r[0]=F[0];r[1]=c;r[2]=c;}
+    else{r[0]=root;r[1]=root;r[2]=root;}
+    return r;
+}
+// pow<1,d>(t) of the constructor: the d-th root of the threshold.
+template<class T> T Dth_Root(T t,int d){return d==2?std::sqrt(t):std::cbrt(t);}
+// NEO_HOOKEAN.cpp:73-81
+template<class T,int d> Vec<T,d> P_Neo_Hookean(const Vec<T,d>& s,T mu,T la,T thr)
+{
+    T J=Product(s);
+    Vec<T,d> r;
+    if(J>=thr){
+        T c=mu-la*std::log(J);
+        for(int i=0;i<d;i++) r[i]=s[i]*mu-(1/s[i])*c;
+        return r;}
+    Vec<T,d> Fc=Clamp_To_Hyperbola(s,thr,Dth_Root(thr,d)),dF,Fi;
+    for(int i=0;i<d;i++){Fi[i]=1/Fc[i];dF[i]=s[i]-Fc[i];}
+    T c=mu-la*std::log(thr);
+    T ip=Fi[0]*dF[0];for(int i=1;i<d;i++) ip+=Fi[i]*dF[i];
+    T li=la*ip;
+    for(int i=0;i<d;i++){
+        T a=s[i]*mu;
+        T b=(Fi[i]*Fi[i]*dF[i]-Fi[i])*c;
+        r[i]=a+b+Fi[i]*li;}
+    return r;
+}
+
+//#####################################################################
+// Scene and state
+//#####################################################################
+struct Wall{int type;double friction;double lo[3],hi[3];};   // type: 0 stick, 1 slip, 2 separate
+
+template<class T,int d> struct Scene
+{
+    int N[3];
+    Vec<T,d> lo,hi,dX,one_over_dX,gravity;
+    int model;T neo_threshold;T dt;
+    std::vector<Wall> walls;
+    int gn[3];        // nodes per axis including 3 ghost layers each side
+    long long nn;     // total nodes
+};
+
+template<class T,int d> struct Box{Vec<T,d> lo,hi;};
+
+// RANGE<TV>::Signed_Distance, RANGE.cpp:38-45
+template<class T,int d> T Box_Phi(const Box<T,d>& b,const Vec<T,d>& X)
+{
+    Vec<T,d> len=b.hi-b.lo,c=(b.lo+b.hi)*(T).5,phi;
+    bool all_le=true;
+    for(int i=0;i<d;i++){phi[i]=std::abs(X[i]-c[i])-(T).5*len[i];if(!(phi[i]<=0)) all_le=false;}
+    if(!all_le){Vec<T,d> p;for(int i=0;i<d;i++) p[i]=phi[i]>0?phi[i]:0;return magnitude(p);}
+    T m=phi[0];for(int i=1;i<d;i++) if(phi[i]>m) m=phi[i];
+    return m;
+}
+// RANGE<TV>::Normal, RANGE.cpp:50-63
+template<class T,int d> Vec<T,d> Box_Normal(const Box<T,d>& b,const Vec<T,d>& X)
+{
+    bool inside=true;
+    for(int i=0;i<d;i++) if(!(X[i]>=b.lo[i] && X[i]<=b.hi[i])) inside=false;
+    Vec<T,d> pmax=X-b.hi,pmin=b.lo-X,n;
+    if(inside){
+        Vec<T,d> m;for(int i=0;i<d;i++) m[i]=pmin[i]>pmax[i]?pmin[i]:pmax[i];
+        int axis=0;
+        if(d==3) axis=Arg_Max3(m[0],m[1],m[2]);
+        else axis=m[0]>=m[1]?0:1;
+        n[axis]=pmax[axis]>pmin[axis]?(T)1:(T)-1;
+        return n;}
+    for(int i=0;i<d;i++){
+        T phi=pmin[i]>pmax[i]?pmin[i]:pmax[i];
+        n[i]=phi>0?(pmax[i]>pmin[i]?phi:-phi):0;}
+    T m=magnitude(n);
+    if(m) return n*(1/m);
+    Vec<T,d> e;e[0]=1;return e;
+}
+
+template<class T,int d> struct Weights
+{
+    int base[d];
+    T w[3][d],dw[3][d];
+};
+
+// PARTICLE_GRID_WEIGHTS_SPLINE.cpp:133-140 (Compute_Precompute_Data) with :93-105 (degree 2)
+template<class T,int d> void Compute_Weights(const Scene<T,d>& s,const Vec<T,d>& X,Weights<T,d>& pd)
+{
+    Vec<T,d> Y=X-s.dX;  // X-(T).5*degree*DX with degree 2
+    for(int a=0;a<d;a++) pd.base[a]=(int)std::floor((Y[a]-s.lo[a])*s.one_over_dX[a]);
+    for(int a=0;a<d;a++){
+        T center=s.lo[a]+((T)pd.base[a]+(T).5)*s.dX[a];
+        T x=(X[a]-center)*s.one_over_dX[a];
+        T q=x-(T)1.5;
+        pd.w[0][a]=(T).5*(q*q);
+        pd.dw[0][a]=q*s.one_over_dX[a];
+        x-=(T)1;
+        pd.w[1][a]=-x*x+(T).75;
+        pd.dw[1][a]=((T)(-2)*x)*s.one_over_dX[a];
+        x-=(T)1;
+        T r=x+(T)1.5;
+        pd.w[2][a]=(T).5*(r*r);
+        pd.dw[2][a]=r*s.one_over_dX[a];}
+}
+
+// One stencil entry (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:33-56); k enumerates {0,1,2}^d with the last axis fastest.
+template<class T,int d> void Stencil(const Weights<T,d>& pd,int k,int idx[d],T& w,Vec<T,d>& g)
+{
+    int o[d];
+    for(int a=d-1;a>=0;a--){o[a]=k%3;k/=3;}
+    T aw[d];
+    for(int a=0;a<d;a++){idx[a]=pd.base[a]+o[a];aw[a]=pd.w[o[a]][a];}
+    w=aw[0];for(int a=1;a<d;a++) w*=aw[a];
+    for(int i=0;i<d;i++) g[i]=1;
+    for(int j=0;j<d;j++){
+        T dj=pd.dw[o[j]][j];
+        for(int i=0;i<d;i++) g[i]*=(i==j)?dj:aw[j];}
+}
+
+template<class T,int d> long long Flat(const Scene<T,d>& s,const int idx[d])
+{
+    long long f=0;
+    for(int a=0;a<d;a++) f=f*s.gn[a]+(idx[a]+3);
+    return f;
+}
+template<class T,int d> Vec<T,d> Node_Center(const Scene<T,d>& s,long long f)
+{
+    int idx[d];
+    for(int a=d-1;a>=0;a--){idx[a]=(int)(f%s.gn[a])-3;f/=s.gn[a];}
+    Vec<T,d> c;
+    for(int a=0;a<d;a++) c[a]=s.lo[a]+((T)idx[a]+(T).5)*s.dX[a];
+    return c;
+}
+
+template<class T,int d> struct Particles
+{
+    long long n;
+    T* X;T* V;T* F;T* B;
+    const T* mass;const T* volume;const T* mu;const T* lambda;
+    unsigned char* valid;
+    Vec<T,d> x(long long p) const{Vec<T,d> r;for(int a=0;a<d;a++) r[a]=X[p*d+a];return r;}
+    Vec<T,d> v(long long p) const{Vec<T,d> r;for(int a=0;a<d;a++) r[a]=V[p*d+a];return r;}
+    Mat<T,d> f(long long p) const{Mat<T,d> r;for(int k=0;k<d*d;k++) r.x[k]=F[p*d*d+k];return r;}
+    Mat<T,d> b(long long p) const{Mat<T,d> r;for(int k=0;k<d*d;k++) r.x[k]=B[p*d*d+k];return r;}
+};
+
+template<class T,int d> struct Grid
+{
+    std::vector<T> mass;
+    std::vector<Vec<T,d> > velocity,velocity_save,velocity_friction_save,v0,dv,tmp2,tmp0,tmp1,v1;
+    std::vector<long long> valid;
+};
+
+struct Collision{int object;long long p;};
+
+// Per-particle stress A_p = PFT (MPM_FINITE_ELEMENTS.cpp:59-65); volume applied by the caller.
+template<class T,int d> Mat<T,d> Stress_PFT(const Scene<T,d>& s,const Mat<T,d>& F,const Mat<T,d>& Fn,T mu,T la)
+{
+    Mat<T,d> U,V;Vec<T,d> sig;
+    Svd(F,U,sig,V);
+    Mat<T,d> FV=Fn*V;
+    Vec<T,d> P=s.model==0?P_Corotated(sig,mu,la):P_Neo_Hookean(sig,mu,la,s.neo_threshold);
+    Mat<T,d> DT;  // DIAGONAL_MATRIX::Times_Transpose: r(i,j)=P(i)*FV(j,i)
+    for(int i=0;i<d;i++) for(int j=0;j<d;j++) DT(i,j)=P[i]*FV(j,i);
+    return U*DT;
+}
+
+template<class T,int d> void Step(const Scene<T,d>& s,Particles<T,d>& P,Grid<T,d>& G,T time)
+{
+    const T dt=s.dt;
+    // Update_Simulated_Particles (MPM_DRIVER.cpp:1142-1157)
+    std::vector<long long> sim;
+    for(long long p=0;p<P.n;p++) if(P.valid[p]) sim.push_back(p);
+    std::vector<Weights<T,d> > W(P.n);
+    for(long long p=0;p<P.n;p++) Compute_Weights(s,P.x(p),W[p]);
+    T dpinv[d];
+    for(int a=0;a<d;a++) dpinv[a]=((T)4*s.one_over_dX[a])*s.one_over_dX[a];
+    const int K=d==3?27:9;
+
+    // Particle_To_Grid (MPM_DRIVER.cpp:245-306)
+    long long nn=s.nn;
+    G.mass.assign(nn,0);
+    G.velocity.assign(nn,Vec<T,d>());
+    G.velocity_save.assign(nn,Vec<T,d>());
+    G.velocity_friction_save.assign(nn,Vec<T,d>());
+    if((long long)G.dv.size()!=nn){G.dv.assign(nn,Vec<T,d>());G.tmp0=G.tmp1=G.v1=G.dv;}
+    for(long long p:sim){
+        Vec<T,d> Xp=P.x(p),Vp=P.v(p);Mat<T,d> Bp=P.b(p);T mp=P.mass[p];
+        for(int k=0;k<K;k++){
+            int idx[d];T w;Vec<T,d> g;
+            Stencil(W[p],k,idx,w,g);
+            long long f=Flat(s,idx);
+            G.mass[f]+=w*mp;
+            Vec<T,d> Vv=w*Vp;
+            Vec<T,d> Y=Node_Center(s,f)-Xp;
+            Vec<T,d> DY;  // SYMMETRIC_MATRIX * VECTOR with zero off-diagonals
+            for(int i=0;i<d;i++){T acc=(i==0?dpinv[0]:(T)0)*Y[0];for(int j=1;j<d;j++) acc+=(i==j?dpinv[i]:(T)0)*Y[j];DY[i]=acc;}
+            Vv=Vv+(Bp*DY)*w;
+            G.velocity[f]=G.velocity[f]+mp*Vv;}}
+    G.valid.clear();
+    for(long long i=0;i<nn;i++){
+        if(G.mass[i]!=0){
+            G.valid.push_back(i);
+            Vec<T,d> v=G.velocity[i]*((T)1/G.mass[i]);
+            G.velocity[i]=v;G.velocity_save[i]=v;}
+        else G.velocity[i]=Vec<T,d>();}
+
+    // Apply_Forces, symplectic branch (MPM_DRIVER.cpp:959-1060)
+    G.v0=G.velocity;
+    std::vector<Mat<T,d> > PFT(P.n);
+    for(long long p:sim){
+        Mat<T,d> F=P.f(p);
+        PFT[p]=Stress_PFT(s,F,F,P.mu[p],P.lambda[p]);}
+    G.tmp2.assign(nn,Vec<T,d>());
+    for(long long p:sim){  // MPM_FINITE_ELEMENTS::Add_Forces
+        Mat<T,d> A=P.volume[p]*PFT[p];
+        for(int k=0;k<K;k++){
+            int idx[d];T w;Vec<T,d> g;
+            Stencil(W[p],k,idx,w,g);
+            long long f=Flat(s,idx);
+            G.tmp2[f]=G.tmp2[f]-A*g;}}
+    for(long long p:sim){  // MPM_GRAVITY::Add_Forces
+        for(int k=0;k<K;k++){
+            int idx[d];T w;Vec<T,d> g;
+            Stencil(W[p],k,idx,w,g);
+            long long f=Flat(s,idx);
+            G.tmp2[f]=G.tmp2[f]+(w*P.mass[p])*s.gravity;}}
+    for(long long i:G.valid) G.dv[i]=(dt/G.mass[i])*G.tmp2[i];
+    for(long long i:G.valid) G.tmp1[i]=G.dv[i];
+
+    // Adjust_For_Collision / Test_Add_Collision (MPM_OBJECTIVE.cpp:150-224)
+    std::vector<Collision> collisions;
+    std::vector<Vec<T,d> > coll_n;
+    std::vector<long long> stuck;
+    std::vector<Box<T,d> > boxes(s.walls.size());
+    for(size_t o=0;o<s.walls.size();o++) for(int a=0;a<d;a++){boxes[o].lo[a]=(T)s.walls[o].lo[a];boxes[o].hi[a]=(T)s.walls[o].hi[a];}
+    T t1=time+dt;(void)t1;
+    for(long long i:G.valid){
+        for(size_t o=0;o<s.walls.size();o++){
+            Vec<T,d> Vn=G.v0[i]+G.dv[i];
+            Vec<T,d> X0=Node_Center(s,i);
+            Vec<T,d> X=X0+dt*Vn;
+            T phi0=Box_Phi(boxes[o],X0);
+            T phi=Box_Phi(boxes[o],X)-(phi0<0?phi0:(T)0);
+            int type=s.walls[o].type;
+            if(type==0 && phi0>0) continue;
+            if(type==1 && phi0>0) continue;
+            if(type==2 && phi>(T)1e-15) continue;
+            if(type!=0 && phi>std::numeric_limits<float>::max()) continue;
+            Vec<T,d> n=Box_Normal(boxes[o],X);
+            collisions.push_back({(int)o,i});
+            coll_n.push_back(n);
+            X=X-phi*n;
+            Vn=(X-X0)*((T)1/dt);
+            if(type==0){Vn=Vec<T,d>();stuck.push_back(i);}
+            G.dv[i]=(Vn-G.v0[i])*((T)1/(T)1);}}
+    for(long long i:G.valid) G.tmp0[i]=G.dv[i];
+
+    // Apply_Friction (MPM_DRIVER.cpp:1064-1094)
+    for(long long j:G.valid) G.velocity_friction_save[j]=G.dv[j];
+    if(!s.walls.empty()){
+        for(long long j:G.valid) G.v1[j]=(T)1*G.v0[j]+G.dv[j];
+        for(long long j:stuck) G.v1[j]=Vec<T,d>();
+        for(size_t c=0;c<collisions.size();c++){
+            long long p=collisions[c].p;
+            const Vec<T,d>& n=coll_n[c];
+            Vec<T,d>& v=G.v1[p];
+            T normal_force=dot(n,G.tmp0[p]-G.tmp1[p]);
+            Vec<T,d> t=v-dot(v,n)*n;
+            T t_mag=magnitude(t);
+            if(t_mag){T inv=1/t_mag;t=t*inv;}else{t=Vec<T,d>();t[0]=1;}
+            T k=(T)s.walls[collisions[c].object].friction*normal_force;
+            if(t_mag<=k) v=dot(v,n)*n;
+            else v=v-k*t;
+            G.velocity_friction_save[p]=v-G.v0[p];}}
+    for(long long j:G.valid){
+        G.velocity[j]=G.dv[j]+G.v0[j];
+        G.velocity_friction_save[j]=G.velocity_friction_save[j]+G.v0[j];}
+    for(long long j:stuck) G.velocity_friction_save[j]=Vec<T,d>();
+
+    // Grid_To_Particle (MPM_DRIVER.cpp:450-515) + Reflect_Or_Invalidate_Particle (:1219-1239)
+    Vec<T,d> A=s.lo,Bd=s.hi;
+    Vec<T,d> eps=(std::numeric_limits<T>::epsilon()*(T)4)*(Bd-A);
+    A=A+eps;Bd=Bd-eps;
+    for(long long p:sim){
+        Vec<T,d> V_pic,V_weight_old,V_pic_fric;
+        Mat<T,d> Bh,grad;
+        for(int k=0;k<K;k++){
+            int idx[d];T w;Vec<T,d> g;
+            Stencil(W[p],k,idx,w,g);
+            long long f=Flat(s,idx);
+            Vec<T,d> V_new=G.velocity[f],V_old=G.velocity_save[f],V_fric=G.velocity_friction_save[f];
+            V_pic=V_pic+w*V_new;
+            V_weight_old=V_weight_old+w*V_old;
+            V_pic_fric=V_pic_fric+w*V_fric;
+            grad=grad+Outer(V_new,g);
+            Vec<T,d> Z=Node_Center(s,f);
+            Vec<T,d> xi_new=Z+dt*V_new;
+            Bh=Bh+(w/2)*(Outer(V_fric,Z+xi_new)+Outer(Z-xi_new,V_fric));}
+        Mat<T,d> Am=dt*grad;
+        for(int i=0;i<d;i++) Am(i,i)+=1;
+        Mat<T,d> Fn=Am*P.f(p);
+        Vec<T,d> Xp=P.x(p);
+        Vec<T,d> xp_new=Xp+dt*V_pic;
+        Mat<T,d> corr=(T).5*(Outer(V_pic_fric,Xp+xp_new)+Outer(Xp-xp_new,V_pic_fric));
+        for(int q=0;q<d*d;q++) Bh.x[q]-=corr.x[q];
+        Vec<T,d> V_flip=(P.v(p)+V_pic_fric)-V_weight_old;
+        Vec<T,d> Vnew=V_flip*(T)0+V_pic_fric*(T)1;
+        for(int q=0;q<d*d;q++){P.F[p*d*d+q]=Fn.x[q];P.B[p*d*d+q]=Bh.x[q];}
+        for(int a=0;a<d;a++){P.X[p*d+a]=xp_new[a];P.V[p*d+a]=Vnew[a];}
+        for(int a=0;a<d;a++){
+            if(P.X[p*d+a]<A[a]) P.valid[p]=0;
+            else if(P.X[p*d+a]>Bd[a]) P.valid[p]=0;}}
+}
+
+template<class T,int d> Scene<T,d> Make_Scene(const int* N,const double* lo,const double* hi,const double* g,int model,double thr,double dt,
+    int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi)
+{
+    Scene<T,d> s;
+    s.nn=1;
+    for(int a=0;a<3;a++) s.N[a]=a<d?N[a]:1;
+    for(int a=0;a<d;a++){
+        s.lo[a]=(T)lo[a];s.hi[a]=(T)hi[a];s.gravity[a]=(T)g[a];
+        s.dX[a]=(s.hi[a]-s.lo[a])/(T)N[a];
+        s.one_over_dX[a]=(T)1/s.dX[a];
+        s.gn[a]=N[a]+6;s.nn*=s.gn[a];}
+    s.model=model;s.neo_threshold=(T)thr;s.dt=(T)dt;
+    for(int i=0;i<nwalls;i++){
+        Wall w;w.type=wtype[i];w.friction=wfric[i];
+        for(int a=0;a<3;a++){w.lo[a]=wlo[3*i+a];w.hi[a]=whi[3*i+a];}
+        s.walls.push_back(w);}
+    return s;
+}
+
+template<class T,int d> int Run(const int* N,const double* lo,const double* hi,const double* g,int model,double thr,double dt,
+    int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi,
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,const T* mu,const T* la,unsigned char* valid,int nsteps,
+    T* gmass,T* gvel,T* gvsave,T* gvfric)
+{
+    Scene<T,d> s=Make_Scene<T,d>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi);
+    Particles<T,d> P{np,X,V,F,B,mass,volume,mu,la,valid};
+    Grid<T,d> G;
+    T time=0;
+    for(int k=0;k<nsteps;k++){Step(s,P,G,time);time+=s.dt;}
+    if(nsteps>0){
+        for(long long i=0;i<s.nn;i++){
+            if(gmass) gmass[i]=G.mass[i];
+            for(int a=0;a<d;a++){
+                if(gvel) gvel[i*d+a]=G.velocity[i][a];
+                if(gvsave) gvsave[i*d+a]=G.velocity_save[i][a];
+                if(gvfric) gvfric[i*d+a]=G.velocity_friction_save[i][a];}}}
+    return 0;
+}
+}  // namespace oracle
+
+using namespace oracle;
+
+#define ORACLE_API extern "C" __attribute__((visibility("default")))
+
+#define DEFINE_RUN(SUF,T) \
+ORACLE_API int oracle_run_##SUF(int dim,const int* N,const double* lo,const double* hi,const double* g,int model,double thr,double dt, \
+    int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi, \
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,const T* mu,const T* la,unsigned char* valid,int nsteps, \
+    T* gmass,T* gvel,T* gvsave,T* gvfric) \
+{ \
+    if(dim==2) return Run<T,2>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume,mu,la,valid,nsteps,gmass,gvel,gvsave,gvfric); \
+    if(dim==3) return Run<T,3>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume,mu,la,valid,nsteps,gmass,gvel,gvsave,gvfric); \
+    return -1; \
+}
+DEFINE_RUN(f64,double)
+DEFINE_RUN(f32,float)
+
+template<class T,int d> static void Svd_Many(long long n,const T* A,T* U,T* S,T* V,int* iters)
+{
+    for(long long i=0;i<n;i++){
+        Mat<T,d> M,u,v;Vec<T,d> s;
+        for(int k=0;k<d*d;k++) M.x[k]=A[i*d*d+k];
+        int it=Svd(M,u,s,v);
+        if(iters) iters[i]=it;
+        for(int k=0;k<d*d;k++){U[i*d*d+k]=u.x[k];V[i*d*d+k]=v.x[k];}
+        for(int k=0;k<d;k++) S[i*d+k]=s[k];}
+}
+#define DEFINE_SVD(SUF,T) \
+ORACLE_API int oracle_svd_##SUF(int dim,long long n,const T* A,T* U,T* S,T* V,int* iters) \
+{ \
+    if(dim==2){Svd_Many<T,2>(n,A,U,S,V,iters);return 0;} \
+    if(dim==3){Svd_Many<T,3>(n,A,U,S,V,iters);return 0;} \
+    return -1; \
+}
+DEFINE_SVD(f64,double)
+DEFINE_SVD(f32,float)
+
+template<class T,int d> static void Pstrain_Many(int model,T thr,long long n,const T* S,const T* mu,const T* la,T* P)
+{
+    for(long long i=0;i<n;i++){
+        Vec<T,d> s;for(int k=0;k<d;k++) s[k]=S[i*d+k];
+        Vec<T,d> r=model==0?P_Corotated(s,mu[i],la[i]):P_Neo_Hookean(s,mu[i],la[i],thr);
+        for(int k=0;k<d;k++) P[i*d+k]=r[k];}
+}
+#define DEFINE_PSTRAIN(SUF,T) \
+ORACLE_API int oracle_pstrain_##SUF(int dim,int model,double thr,long long n,const T* S,const T* mu,const T* la,T* P) \
+{ \
+    if(dim==2){Pstrain_Many<T,2>(model,(T)thr,n,S,mu,la,P);return 0;} \
+    if(dim==3){Pstrain_Many<T,3>(model,(T)thr,n,S,mu,la,P);return 0;} \
+    return -1; \
+}
+DEFINE_PSTRAIN(f64,double)
+DEFINE_PSTRAIN(f32,float)
+
+// Kirchhoff-type stress V0*P(F)*F^T per particle (what the explicit step scatters).
+template<class T,int d> static void Stress_Many(int model,T thr,long long n,const T* F,const T* vol,const T* mu,const T* la,T* A)
+{
+    int N[3]={1,1,1};double lo[3]={0,0,0},hi[3]={1,1,1},g[3]={0,0,0};
+    Scene<T,d> s=Make_Scene<T,d>(N,lo,hi,g,model,thr,1,0,0,0,0,0);
+    for(long long i=0;i<n;i++){
+        Mat<T,d> M;for(int k=0;k<d*d;k++) M.x[k]=F[i*d*d+k];
+        Mat<T,d> R=vol[i]*Stress_PFT(s,M,M,mu[i],la[i]);
+        for(int k=0;k<d*d;k++) A[i*d*d+k]=R.x[k];}
+}
+#define DEFINE_STRESS(SUF,T) \
+ORACLE_API int oracle_stress_##SUF(int dim,int model,double thr,long long n,const T* F,const T* vol,const T* mu,const T* la,T* A) \
+{ \
+    if(dim==2){Stress_Many<T,2>(model,(T)thr,n,F,vol,mu,la,A);return 0;} \
+    if(dim==3){Stress_Many<T,3>(model,(T)thr,n,F,vol,mu,la,A);return 0;} \
+    return -1; \
+}
+DEFINE_STRESS(f64,double)
+DEFINE_STRESS(f32,float)
+
+template<class T,int d> static void Weights_Many(const int* N,const double* lo,const double* hi,long long n,const T* X,int* base,T* w,T* dw)
+{
+    double g[3]={0,0,0};
+    Scene<T,d> s=Make_Scene<T,d>(N,lo,hi,g,0,0,1,0,0,0,0,0);
+    for(long long i=0;i<n;i++){
+        Vec<T,d> x;for(int a=0;a<d;a++) x[a]=X[i*d+a];
+        Weights<T,d> pd;Compute_Weights(s,x,pd);
+        for(int a=0;a<d;a++){
+            base[i*d+a]=pd.base[a];
+            for(int k=0;k<3;k++){w[(i*3+k)*d+a]=pd.w[k][a];dw[(i*3+k)*d+a]=pd.dw[k][a];}}}
+}
+#define DEFINE_WEIGHTS(SUF,T) \
+ORACLE_API int oracle_weights_##SUF(int dim,const int* N,const double* lo,const double* hi,long long n,const T* X,int* base,T* w,T* dw) \
+{ \
+    if(dim==2){Weights_Many<T,2>(N,lo,hi,n,X,base,w,dw);return 0;} \
+    if(dim==3){Weights_Many<T,3>(N,lo,hi,n,X,base,w,dw);return 0;} \
+    return -1; \
+}
+DEFINE_WEIGHTS(f64,double)
+DEFINE_WEIGHTS(f32,float)

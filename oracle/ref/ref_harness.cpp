@@ -1,0 +1,345 @@
+//#####################################################################
+// Reference harness: drives PhysBAM's own MPM_DRIVER (compiled from the
+// sources under the reference checkout, never copied here) on a scene read
+// from a binary file and dumps particle and grid state.  It is test
+// infrastructure only: it runs in the authoring container to produce golden
+// vectors and, as a binary under oracle/_ref/, to time the reference CPU step.
+//
+// Modes
+//   ref_harness step  <scene.bin> <out.bin>
+//   ref_harness time  <scene.bin> <threads> <warmup> <steps>
+//   ref_harness svd   <prec> <dim> <in.bin> <out.bin>
+//   ref_harness pstrain <prec> <dim> <model> <threshold> <in.bin> <out.bin>
+//   ref_harness weights <prec> <dim> <N> <lo> <hi> <in.bin> <out.bin>
+//
+// Scene file layout (little endian) is written by tests/golden/scene_io.py.
+//#####################################################################
+#include <Core/Log/LOG.h>
+#include <Core/Math_Tools/RANGE_ITERATOR.h>
+#include <Core/Matrices/DIAGONAL_MATRIX.h>
+#include <Core/Matrices/MATRIX.h>
+#include <Grid_Tools/Grids/GRID.h>
+#include <Deformables/Constitutive_Models/COROTATED_FIXED.h>
+#include <Deformables/Constitutive_Models/NEO_HOOKEAN.h>
+#include <Hybrid_Methods/Examples_And_Drivers/MPM_DRIVER.h>
+#include <Hybrid_Methods/Examples_And_Drivers/MPM_EXAMPLE.h>
+#include <Hybrid_Methods/Examples_And_Drivers/MPM_PARTICLES.h>
+#include <Hybrid_Methods/Forces/MPM_FINITE_ELEMENTS.h>
+#include <Hybrid_Methods/Forces/MPM_GRAVITY.h>
+#include <Hybrid_Methods/Forces/MPM_FORCE_HELPER.h>
+#include <Hybrid_Methods/Iterators/GATHER_SCATTER.h>
+#include <Hybrid_Methods/Iterators/PARTICLE_GRID_WEIGHTS_SPLINE.h>
+#include <Hybrid_Methods/System/MPM_KRYLOV_SYSTEM.h>
+#include <Hybrid_Methods/System/MPM_KRYLOV_VECTOR.h>
+#include <Hybrid_Methods/System/MPM_OBJECTIVE.h>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#ifdef USE_OPENMP
+#include <omp.h>
+#endif
+using namespace PhysBAM;
+
+namespace{
+struct READER
+{
+    FILE* f;
+    explicit READER(const char* path):f(fopen(path,"rb")){if(!f){fprintf(stderr,"cannot open %s\n",path);exit(2);}}
+    ~READER(){fclose(f);}
+    template<class X> X get(){X x;if(fread(&x,sizeof(X),1,f)!=1){fprintf(stderr,"short read\n");exit(2);}return x;}
+    template<class X> void get(X* p,size_t n){if(n && fread(p,sizeof(X),n,f)!=n){fprintf(stderr,"short read\n");exit(2);}}
+};
+struct WRITER
+{
+    FILE* f;
+    explicit WRITER(const char* path):f(fopen(path,"wb")){if(!f){fprintf(stderr,"cannot open %s\n",path);exit(2);}}
+    ~WRITER(){fclose(f);}
+    template<class X> void put(const X& x){fwrite(&x,sizeof(X),1,f);}
+    template<class X> void put(const X* p,size_t n){if(n) fwrite(p,sizeof(X),n,f);}
+};
+
+struct WALL {int type;double friction;double lo[3],hi[3];};
+struct SCENE
+{
+    int dim,prec,model,nsteps,dump_every,nwalls,threads;
+    int N[3];
+    double lo[3],hi[3],gravity[3],dt,neo_threshold;
+    long long np;
+    std::vector<WALL> walls;
+    std::vector<double> X,V,F,B,mass,volume,mu,lambda;
+    std::vector<unsigned char> valid;
+};
+
+SCENE Read_Scene(const char* path)
+{
+    READER r(path);
+    char magic[8];r.get(magic,8);
+    if(memcmp(magic,"MPMSCN1",7)){fprintf(stderr,"bad scene magic\n");exit(2);}
+    SCENE s;
+    s.dim=r.get<int>();s.prec=r.get<int>();s.model=r.get<int>();s.nsteps=r.get<int>();
+    s.dump_every=r.get<int>();s.nwalls=r.get<int>();s.threads=r.get<int>();r.get<int>();
+    r.get(s.N,3);r.get(s.lo,3);r.get(s.hi,3);r.get(s.gravity,3);
+    s.dt=r.get<double>();s.neo_threshold=r.get<double>();
+    s.np=r.get<long long>();
+    for(int i=0;i<s.nwalls;i++){
+        WALL w;w.type=r.get<int>();r.get<int>();w.friction=r.get<double>();
+        r.get(w.lo,3);r.get(w.hi,3);s.walls.push_back(w);}
+    size_t n=s.np,d=s.dim;
+    s.X.resize(n*d);s.V.resize(n*d);s.F.resize(n*d*d);s.B.resize(n*d*d);
+    s.mass.resize(n);s.volume.resize(n);s.mu.resize(n);s.lambda.resize(n);s.valid.resize(n);
+    r.get(s.X.data(),n*d);r.get(s.V.data(),n*d);r.get(s.F.data(),n*d*d);r.get(s.B.data(),n*d*d);
+    r.get(s.mass.data(),n);r.get(s.volume.data(),n);r.get(s.mu.data(),n);r.get(s.lambda.data(),n);
+    r.get(s.valid.data(),n);
+    return s;
+}
+
+template<class TV>
+class HARNESS_EXAMPLE:public MPM_EXAMPLE<TV>
+{
+public:
+    explicit HARNESS_EXAMPLE(const STREAM_TYPE st):MPM_EXAMPLE<TV>(st){}
+    void Initialize() override {}
+};
+
+template<class TV>
+struct RUN
+{
+    typedef typename TV::SCALAR T;
+    enum {d=TV::m};
+    typedef VECTOR<int,d> TV_INT;
+    HARNESS_EXAMPLE<TV> ex;
+    MPM_DRIVER<TV>* driver=0;
+    const SCENE& s;
+
+    RUN(const SCENE& s,STREAM_TYPE st):ex(st),s(s)
+    {
+        TV_INT N;TV lo,hi,g;
+        for(int a=0;a<d;a++){N(a)=s.N[a];lo(a)=(T)s.lo[a];hi(a)=(T)s.hi[a];g(a)=(T)s.gravity[a];}
+        ex.grid.Initialize(N,RANGE<TV>(lo,hi),true);
+        ex.threads=s.threads;
+        ex.gather_scatter.threads=s.threads;
+        ex.Set_Weights(2);
+        ex.use_symplectic_euler=true;
+        ex.use_affine=true;
+        ex.flip=0;
+        MPM_PARTICLES<TV>& P=ex.particles;
+        P.Store_B(true);
+        P.Store_Lame(true);
+        P.Add_Elements((int)s.np);
+        for(int p=0;p<(int)s.np;p++){
+            for(int a=0;a<d;a++){P.X(p)(a)=(T)s.X[p*d+a];P.V(p)(a)=(T)s.V[p*d+a];}
+            for(int k=0;k<d*d;k++){P.F(p).x[k]=(T)s.F[p*d*d+k];P.B(p).x[k]=(T)s.B[p*d*d+k];}
+            P.mass(p)=(T)s.mass[p];P.volume(p)=(T)s.volume[p];
+            P.mu(p)=(T)s.mu[p];P.lambda(p)=(T)s.lambda[p];P.valid(p)=s.valid[p]!=0;}
+        ISOTROPIC_CONSTITUTIVE_MODEL<T,d>* cm=0;
+        if(s.model==0) cm=new COROTATED_FIXED<T,d>((T)1e4,(T).3);
+        else cm=new NEO_HOOKEAN<T,d>((T)1e4,(T).3,(T).05,(T)s.neo_threshold);
+        ex.Add_Force(*new MPM_FINITE_ELEMENTS<TV>(ex.force_helper,*cm,ex.gather_scatter,0));
+        ex.Add_Force(*new MPM_GRAVITY<TV>(ex.force_helper,g,ex.gather_scatter,0));
+        for(const WALL& w:s.walls){
+            TV a,b;for(int k=0;k<d;k++){a(k)=(T)w.lo[k];b(k)=(T)w.hi[k];}
+            ex.Add_Collision_Object(RANGE<TV>(a,b),(typename MPM_EXAMPLE<TV>::COLLISION_TYPE)w.type,(T)w.friction);}
+
+        driver=new MPM_DRIVER<TV>(ex);
+        // Same set-up as MPM_DRIVER::Initialize, minus restart handling and frame output.
+        auto dom=ex.grid.Domain_Indices(ex.ghost);
+        ex.mass.Resize(dom);
+        ex.velocity.Resize(dom);
+        ex.velocity_save.Resize(dom);
+        ex.velocity_friction_save.Resize(dom);
+        driver->dv.u.Resize(dom);
+        driver->rhs.u.Resize(dom);
+        driver->objective.system.tmp.u.Resize(dom);
+        P.Store_B(ex.use_affine);
+        P.Store_S(ex.use_oldroyd);
+        ex.Dp_inv.Resize(P.X.m);
+        RANGE<TV_INT> range(ex.grid.Cell_Indices(ex.ghost));
+        ex.location.Resize(range,no_init);
+        for(RANGE_ITERATOR<d> it(range);it.Valid();it.Next())
+            ex.location(it.index)=ex.grid.Center(it.index);
+        driver->Update_Simulated_Particles();
+    }
+    ~RUN(){delete driver;}
+
+    void Step()
+    {
+        ex.dt=(T)s.dt;
+        driver->Advance_One_Time_Step();
+        ex.time+=ex.dt;
+    }
+
+    void Dump(WRITER& w,int step)
+    {
+        w.put(step);
+        MPM_PARTICLES<TV>& P=ex.particles;
+        int n=P.X.m;
+        std::vector<double> buf;
+        buf.resize((size_t)n*d);
+        for(int p=0;p<n;p++) for(int a=0;a<d;a++) buf[p*d+a]=P.X(p)(a);
+        w.put(buf.data(),buf.size());
+        for(int p=0;p<n;p++) for(int a=0;a<d;a++) buf[p*d+a]=P.V(p)(a);
+        w.put(buf.data(),buf.size());
+        buf.resize((size_t)n*d*d);
+        for(int p=0;p<n;p++) for(int k=0;k<d*d;k++) buf[p*d*d+k]=P.F(p).x[k];
+        w.put(buf.data(),buf.size());
+        for(int p=0;p<n;p++) for(int k=0;k<d*d;k++) buf[p*d*d+k]=P.B(p).x[k];
+        w.put(buf.data(),buf.size());
+        std::vector<unsigned char> v(n);
+        for(int p=0;p<n;p++) v[p]=P.valid(p);
+        w.put(v.data(),v.size());
+        int nn=ex.mass.array.m;
+        buf.resize(nn);
+        for(int i=0;i<nn;i++) buf[i]=ex.mass.array(i);
+        w.put(buf.data(),buf.size());
+        const ARRAY<TV,TV_INT>* arrs[3]={&ex.velocity,&ex.velocity_save,&ex.velocity_friction_save};
+        buf.resize((size_t)nn*d);
+        for(int k=0;k<3;k++){
+            for(int i=0;i<nn;i++) for(int a=0;a<d;a++) buf[i*d+a]=arrs[k]->array(i)(a);
+            w.put(buf.data(),buf.size());}
+    }
+};
+
+template<class TV>
+int Run_Step(const SCENE& s,const char* out,STREAM_TYPE st)
+{
+#ifdef USE_OPENMP
+    omp_set_num_threads(s.threads>0?s.threads:1);
+#endif
+    RUN<TV> run(s,st);
+    WRITER w(out);
+    w.put("MPMOUT1",8);
+    int ndumps=s.dump_every>0?s.nsteps/s.dump_every:0;
+    int nn=run.ex.mass.array.m;
+    int np=(int)s.np;
+    w.put(ndumps);w.put(s.dim);w.put(np);w.put(nn);
+    for(int k=1;k<=s.nsteps;k++){
+        run.Step();
+        if(s.dump_every>0 && k%s.dump_every==0) run.Dump(w,k);}
+    return 0;
+}
+
+template<class TV>
+int Run_Time(const SCENE& s,int threads,int warmup,int steps,STREAM_TYPE st)
+{
+#ifdef USE_OPENMP
+    omp_set_num_threads(threads);
+#endif
+    SCENE t=s;t.threads=threads;
+    RUN<TV> run(t,st);
+    for(int k=0;k<warmup;k++) run.Step();
+    auto t0=std::chrono::steady_clock::now();
+    for(int k=0;k<steps;k++) run.Step();
+    auto t1=std::chrono::steady_clock::now();
+    double sec=std::chrono::duration<double>(t1-t0).count();
+    int active=0;
+    for(int p=0;p<run.ex.particles.X.m;p++) active+=run.ex.particles.valid(p);
+    printf("{\"particles\": %lld, \"active\": %d, \"threads\": %d, \"steps\": %d, \"seconds\": %.6f, \"particle_steps_per_s\": %.6e}\n",
+        s.np,active,threads,steps,sec,(double)s.np*steps/sec);
+    return 0;
+}
+
+template<class T,int d>
+int Run_Svd(const char* in,const char* out)
+{
+    READER r(in);
+    long long n=r.get<long long>();
+    std::vector<double> A(n*d*d);
+    r.get(A.data(),A.size());
+    WRITER w(out);
+    std::vector<double> U(n*d*d),S(n*d),V(n*d*d);
+    for(long long i=0;i<n;i++){
+        MATRIX<T,d> M,u,v;DIAGONAL_MATRIX<T,d> D;
+        for(int k=0;k<d*d;k++) M.x[k]=(T)A[i*d*d+k];
+        M.Singular_Value_Decomposition(u,D,v);
+        for(int k=0;k<d*d;k++){U[i*d*d+k]=u.x[k];V[i*d*d+k]=v.x[k];}
+        for(int k=0;k<d;k++) S[i*d+k]=D.x(k);}
+    w.put(U.data(),U.size());w.put(S.data(),S.size());w.put(V.data(),V.size());
+    return 0;
+}
+
+template<class T,int d>
+int Run_Pstrain(int model,double threshold,const char* in,const char* out)
+{
+    READER r(in);
+    long long n=r.get<long long>();
+    std::vector<double> S(n*d),mu(n),la(n);
+    r.get(S.data(),S.size());r.get(mu.data(),n);r.get(la.data(),n);
+    ARRAY<T> amu(n),ala(n);
+    for(long long i=0;i<n;i++){amu(i)=(T)mu[i];ala(i)=(T)la[i];}
+    ISOTROPIC_CONSTITUTIVE_MODEL<T,d>* cm=0;
+    if(model==0) cm=new COROTATED_FIXED<T,d>((T)1e4,(T).3);
+    else cm=new NEO_HOOKEAN<T,d>((T)1e4,(T).3,(T).05,(T)threshold);
+    ARRAY_VIEW<T> vmu(amu),vla(ala);
+    cm->mu=&vmu;cm->lambda=&vla;
+    std::vector<double> P(n*d);
+    for(long long i=0;i<n;i++){
+        DIAGONAL_MATRIX<T,d> D;for(int k=0;k<d;k++) D.x(k)=(T)S[i*d+k];
+        DIAGONAL_MATRIX<T,d> R=cm->P_From_Strain(D,(int)i);
+        for(int k=0;k<d;k++) P[i*d+k]=R.x(k);}
+    cm->mu=0;cm->lambda=0;
+    delete cm;
+    WRITER w(out);
+    w.put(P.data(),P.size());
+    return 0;
+}
+
+template<class T,int d>
+int Run_Weights(int N,double lo,double hi,const char* in,const char* out)
+{
+    typedef VECTOR<T,d> TV;
+    READER r(in);
+    long long n=r.get<long long>();
+    std::vector<double> X(n*d);
+    r.get(X.data(),X.size());
+    GRID<TV> grid(VECTOR<int,d>()+N,RANGE<TV>(TV()+(T)lo,TV()+(T)hi),true);
+    PARTICLE_GRID_WEIGHTS_SPLINE<TV,2> wts(grid,1);
+    std::vector<int> base(n*d);
+    std::vector<double> w(n*3*d),dw(n*3*d);
+    for(long long i=0;i<n;i++){
+        TV x;for(int a=0;a<d;a++) x(a)=(T)X[i*d+a];
+        typename PARTICLE_GRID_WEIGHTS_SPLINE<TV,2>::PRECOMPUTE_DATA pd;
+        wts.Compute_Precompute_Data(pd,x);
+        for(int a=0;a<d;a++){
+            base[i*d+a]=pd.base(a);
+            for(int k=0;k<3;k++){w[(i*3+k)*d+a]=pd.w(k)(a);dw[(i*3+k)*d+a]=pd.dw(k)(a);}}}
+    WRITER wr(out);
+    wr.put(base.data(),base.size());wr.put(w.data(),w.size());wr.put(dw.data(),dw.size());
+    return 0;
+}
+}
+
+int main(int argc,char* argv[])
+{
+    LOG::Initialize_Logging(true,true,0,false);
+    if(argc<2){fprintf(stderr,"usage: ref_harness step|time|svd|pstrain|weights ...\n");return 2;}
+    std::string mode=argv[1];
+    if(mode=="step" || mode=="time"){
+        if((mode=="step" && argc!=4) || (mode=="time" && argc!=6)){fprintf(stderr,"bad arguments\n");return 2;}
+        SCENE s=Read_Scene(argv[2]);
+        int rc=2;
+        if(s.prec==8){
+            STREAM_TYPE st((double()));
+            if(s.dim==2) rc=mode=="step"?Run_Step<VECTOR<double,2> >(s,argv[3],st):Run_Time<VECTOR<double,2> >(s,atoi(argv[3]),atoi(argv[4]),atoi(argv[5]),st);
+            else rc=mode=="step"?Run_Step<VECTOR<double,3> >(s,argv[3],st):Run_Time<VECTOR<double,3> >(s,atoi(argv[3]),atoi(argv[4]),atoi(argv[5]),st);}
+        else{
+            STREAM_TYPE st((float()));
+            if(s.dim==2) rc=mode=="step"?Run_Step<VECTOR<float,2> >(s,argv[3],st):Run_Time<VECTOR<float,2> >(s,atoi(argv[3]),atoi(argv[4]),atoi(argv[5]),st);
+            else rc=mode=="step"?Run_Step<VECTOR<float,3> >(s,argv[3],st):Run_Time<VECTOR<float,3> >(s,atoi(argv[3]),atoi(argv[4]),atoi(argv[5]),st);}
+        LOG::Finish_Logging();
+        return rc;}
+    if(mode=="svd" && argc==6){
+        int prec=atoi(argv[2]),dim=atoi(argv[3]);
+        if(prec==8) return dim==2?Run_Svd<double,2>(argv[4],argv[5]):Run_Svd<double,3>(argv[4],argv[5]);
+        return dim==2?Run_Svd<float,2>(argv[4],argv[5]):Run_Svd<float,3>(argv[4],argv[5]);}
+    if(mode=="pstrain" && argc==8){
+        int prec=atoi(argv[2]),dim=atoi(argv[3]),model=atoi(argv[4]);double thr=atof(argv[5]);
+        if(prec==8) return dim==2?Run_Pstrain<double,2>(model,thr,argv[6],argv[7]):Run_Pstrain<double,3>(model,thr,argv[6],argv[7]);
+        return dim==2?Run_Pstrain<float,2>(model,thr,argv[6],argv[7]):Run_Pstrain<float,3>(model,thr,argv[6],argv[7]);}
+    if(mode=="weights" && argc==9){
+        int prec=atoi(argv[2]),dim=atoi(argv[3]),N=atoi(argv[4]);double lo=atof(argv[5]),hi=atof(argv[6]);
+        if(prec==8) return dim==2?Run_Weights<double,2>(N,lo,hi,argv[7],argv[8]):Run_Weights<double,3>(N,lo,hi,argv[7],argv[8]);
+        return dim==2?Run_Weights<float,2>(N,lo,hi,argv[7],argv[8]):Run_Weights<float,3>(N,lo,hi,argv[7],argv[8]);}
+    fprintf(stderr,"bad arguments\n");
+    return 2;
+}

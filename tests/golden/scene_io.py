@@ -1,0 +1,116 @@
+"""Scene files for the reference harness (oracle/ref/ref_harness.cpp).
+
+Test infrastructure: writes the binary scene layout the harness reads, runs
+the harness binary built under oracle/_ref/, and parses its dumps.  The
+harness exists only where the reference checkout was available at build time;
+callers check ``ref_available()`` first.
+"""
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def ref_available():
+    return os.path.isfile(REF_HARNESS) and os.access(REF_HARNESS, os.X_OK)
+
+
+def write_scene(path, sc, nsteps, dump_every, threads=1):
+    """Write scene dict ``sc`` (see tests/mpmtest.py make_* helpers)."""
+    d = sc["dim"]
+    n = int(sc["X"].shape[0])
+    N = list(sc["N"]) + [1] * (3 - d)
+    lo = list(sc["lo"]) + [0.0] * (3 - d)
+    hi = list(sc["hi"]) + [0.0] * (3 - d)
+    g = list(sc["gravity"]) + [0.0] * (3 - d)
+    walls = sc.get("walls", [])
+    with open(path, "wb") as f:
+        f.write(b"MPMSCN1\0")
+        f.write(struct.pack("<8i", d, sc["prec"], sc["model"], nsteps, dump_every,
+                            len(walls), threads, 0))
+        f.write(struct.pack("<3i", *N))
+        f.write(struct.pack("<3d", *lo))
+        f.write(struct.pack("<3d", *hi))
+        f.write(struct.pack("<3d", *g))
+        f.write(struct.pack("<2d", sc["dt"], sc.get("neo_threshold", 0.25)))
+        f.write(struct.pack("<q", n))
+        for w in walls:
+            wl = list(w["lo"]) + [0.0] * (3 - d)
+            wh = list(w["hi"]) + [0.0] * (3 - d)
+            f.write(struct.pack("<2i", w["type"], 0))
+            f.write(struct.pack("<d", w["friction"]))
+            f.write(struct.pack("<3d", *wl))
+            f.write(struct.pack("<3d", *wh))
+        for key in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda"):
+            f.write(np.ascontiguousarray(sc[key], dtype=np.float64).tobytes())
+        f.write(np.ascontiguousarray(sc["valid"], dtype=np.uint8).tobytes())
+
+
+def read_output(path, dim):
+    with open(path, "rb") as f:
+        buf = f.read()
+    assert buf[:7] == b"MPMOUT1"
+    ndumps, d, n, nn = struct.unpack_from("<4i", buf, 8)
+    assert d == dim
+    off = 24
+    dumps = []
+
+    def take(count, dtype):
+        nonlocal off
+        a = np.frombuffer(buf, dtype=dtype, count=count, offset=off)
+        off += a.nbytes
+        return a
+
+    for _ in range(ndumps):
+        (step,) = struct.unpack_from("<i", buf, off)
+        off += 4
+        r = {"step": step}
+        r["X"] = take(n * d, np.float64).reshape(n, d)
+        r["V"] = take(n * d, np.float64).reshape(n, d)
+        r["F"] = take(n * d * d, np.float64).reshape(n, d * d)
+        r["B"] = take(n * d * d, np.float64).reshape(n, d * d)
+        r["valid"] = take(n, np.uint8)
+        r["grid_mass"] = take(nn, np.float64)
+        r["grid_velocity"] = take(nn * d, np.float64).reshape(nn, d)
+        r["grid_velocity_save"] = take(nn * d, np.float64).reshape(nn, d)
+        r["grid_velocity_friction_save"] = take(nn * d, np.float64).reshape(nn, d)
+        dumps.append(r)
+    return dumps
+
+
+def run_ref(sc, nsteps, dump_every=None, threads=1):
+    """Run the reference MPM_DRIVER on scene ``sc``; returns the list of dumps."""
+    if dump_every is None:
+        dump_every = nsteps
+    with tempfile.TemporaryDirectory() as td:
+        sp = os.path.join(td, "scene.bin")
+        op = os.path.join(td, "out.bin")
+        write_scene(sp, sc, nsteps, dump_every, threads)
+        subprocess.run([REF_HARNESS, "step", sp, op], check=True)
+        return read_output(op, sc["dim"])
+
+
+def ref_component(mode, args, payload, outspec):
+    """Run a component mode (svd, pstrain, weights) of the harness.
+
+    ``payload`` is the input bytes; ``outspec`` a list of (dtype, count).
+    """
+    with tempfile.TemporaryDirectory() as td:
+        ip = os.path.join(td, "in.bin")
+        op = os.path.join(td, "out.bin")
+        with open(ip, "wb") as f:
+            f.write(payload)
+        subprocess.run([REF_HARNESS, mode] + [str(a) for a in args] + [ip, op], check=True)
+        with open(op, "rb") as f:
+            buf = f.read()
+    out, off = [], 0
+    for dt, cnt in outspec:
+        a = np.frombuffer(buf, dtype=dt, count=cnt, offset=off)
+        off += a.nbytes
+        out.append(a.copy())
+    return out
