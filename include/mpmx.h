@@ -1,0 +1,169 @@
+/*
+ * mpmx — MI355X (gfx950) explicit APIC MPM time step behind a C ABI.
+ *
+ * One context holds one MPM scene on one GPU: the grid description
+ * (GRID<TV> with ghost 3), the particle state (MPM_PARTICLES<TV>), the force
+ * set (one MPM_FINITE_ELEMENTS with per-particle mu/lambda plus MPM_GRAVITY)
+ * and the collision objects (axis-aligned boxes), and advances it with the
+ * symplectic-Euler step of MPM_DRIVER<TV>::Advance_One_Time_Step.
+ *
+ * Reference paths below are relative to PhysBAM's Public_Library/.
+ *
+ * Conventions
+ *   - every call returns MPMX_OK (0) or a negative MPMX_E* code; the message
+ *     of the last failure on the calling thread is mpmx_last_error();
+ *   - host arrays are caller-owned and copied in/out; device buffers are owned
+ *     by the context;
+ *   - floating point arrays have the context precision (float for
+ *     MPMX_FP32, double for MPMX_FP64), vectors are d contiguous values per
+ *     particle and d x d matrices are column-major per particle
+ *     (Core/Matrices/MATRIX_3X3.h:115-119), exactly as MPM_PARTICLES stores X,
+ *     V, F and B;
+ *   - a context is externally synchronised and runs on one HIP stream; calls
+ *     return after their GPU work has finished unless named *_async.
+ */
+#ifndef MPMX_H
+#define MPMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPMX_OK 0
+#define MPMX_EINVAL (-1)   /* bad argument */
+#define MPMX_EHIP (-2)     /* HIP runtime failure */
+#define MPMX_ESTATE (-3)   /* call not valid in the current state */
+#define MPMX_ERANGE (-4)   /* particle outside the ghosted grid */
+#define MPMX_EINTERNAL (-5)
+
+#define MPMX_FP32 4
+#define MPMX_FP64 8
+
+/* Constitutive models (Deformables/Constitutive_Models). */
+#define MPMX_FIXED_COROTATED 0 /* COROTATED_FIXED.cpp:59-65 */
+#define MPMX_NEO_HOOKEAN 1     /* NEO_HOOKEAN.cpp:44-81 */
+
+/* Collision types, MPM_COLLISION_OBJECT<TV>::COLLISION_TYPE
+   (Hybrid_Methods/Collisions/MPM_COLLISION_OBJECT.h:21). */
+#define MPMX_STICK 0
+#define MPMX_SLIP 1
+#define MPMX_SEPARATE 2
+
+typedef struct mpmx_ctx mpmx_ctx;
+
+/* GRID<TV>::Initialize(counts, RANGE(lo, hi), MAC=true) with ghost 3
+   (Grid_Tools/Grids/GRID.cpp:13-27; MPM_EXAMPLE.h:86). */
+typedef struct mpmx_grid_desc {
+  int dim;         /* 2 or 3 */
+  int precision;   /* MPMX_FP32 or MPMX_FP64 */
+  int cells[3];    /* cells per axis (unused axes ignored) */
+  double lo[3];    /* domain min corner */
+  double hi[3];    /* domain max corner */
+} mpmx_grid_desc;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* mpmx_last_error(void);
+
+/* Create a context on the current HIP device.  Replaces the construction of
+   MPM_EXAMPLE<TV> + MPM_DRIVER<TV> (Examples_And_Drivers/MPM_EXAMPLE.cpp:21-31,
+   MPM_DRIVER.cpp:47-55) and the grid set-up of MPM_DRIVER::Initialize
+   (MPM_DRIVER.cpp:102-108). */
+int mpmx_create(const mpmx_grid_desc* grid, mpmx_ctx** out);
+int mpmx_destroy(mpmx_ctx* ctx);
+
+/* Particle state in, MPM_PARTICLES<TV> layout (Examples_And_Drivers/MPM_PARTICLES.h:17-44):
+   X, V: n*d; F, B: n*d*d (column-major); mass, volume, mu, lambda: n; valid: n
+   bytes (nonzero = simulated).  mu/lambda are the per-particle Lame parameters
+   bound by MPM_FINITE_ELEMENTS (Forces/MPM_FINITE_ELEMENTS.cpp:30-31).
+   Replaces STANDARD_TESTS_BASE::Add_Particle + Set_Lame_On_Particles
+   (Projects/mpm/STANDARD_TESTS_BASE.cpp:285-310, :567-586).  Fails with
+   MPMX_ERANGE if a valid particle's stencil would leave the ghosted grid. */
+int mpmx_set_particles(mpmx_ctx* ctx, int64_t n, const void* X, const void* V, const void* F, const void* B,
+                       const void* mass, const void* volume, const void* mu, const void* lambda,
+                       const uint8_t* valid);
+
+/* Particle state out (any pointer may be NULL), in the order given to
+   mpmx_set_particles. */
+int mpmx_get_particles(mpmx_ctx* ctx, void* X, void* V, void* F, void* B, uint8_t* valid);
+int64_t mpmx_particle_count(const mpmx_ctx* ctx);
+
+/* Elastic force: one MPM_FINITE_ELEMENTS with the given model over all
+   particles (STANDARD_TESTS_BASE::Add_Fixed_Corotated / Add_Neo_Hookean,
+   STANDARD_TESTS_BASE.cpp:372-392).  neo_threshold is NEO_HOOKEAN's
+   failure_threshold (default 0.25, NEO_HOOKEAN.h:33). */
+int mpmx_set_constitutive(mpmx_ctx* ctx, int model, double neo_threshold);
+
+/* MPM_GRAVITY (Forces/MPM_GRAVITY.cpp:62-69); g has dim entries. */
+int mpmx_set_gravity(mpmx_ctx* ctx, const double* g);
+
+/* MPM_EXAMPLE::Add_Collision_Object with an ANALYTIC_IMPLICIT_OBJECT<RANGE<TV>>
+   (MPM_EXAMPLE.cpp:331-338; MPM_EXAMPLE.h:153-156); box corners have dim
+   entries.  At most 16 objects. */
+int mpmx_add_collision_box(mpmx_ctx* ctx, int type, double friction, const double* lo, const double* hi);
+int mpmx_clear_collision_objects(mpmx_ctx* ctx);
+
+/* nsteps calls of MPM_DRIVER::Advance_One_Time_Step with example.dt = dt
+   (MPM_DRIVER.cpp:141-157), advancing example.time by dt each. */
+int mpmx_step(mpmx_ctx* ctx, double dt, int nsteps);
+/* Same, enqueued on the context stream without waiting. */
+int mpmx_step_async(mpmx_ctx* ctx, double dt, int nsteps);
+int mpmx_synchronize(mpmx_ctx* ctx);
+
+/* Simulation time (example.time). */
+double mpmx_time(const mpmx_ctx* ctx);
+
+/* FP64 grid totals after the last step's grid update: mass and momentum
+   sum_i m_i v_i over nodes with nonzero mass, i.e. MPM_EXAMPLE::
+   Total_Grid_Linear_Momentum of the velocity used by G2P (MPM_EXAMPLE.cpp:
+   226-241).  momentum has dim entries.  Computed only when enabled. */
+int mpmx_enable_grid_stats(mpmx_ctx* ctx, int enable);
+int mpmx_grid_stats(mpmx_ctx* ctx, double* mass, double* momentum);
+
+/* FP64 particle totals over valid particles: mass and sum m_p v_p
+   (MPM_EXAMPLE::Total_Particle_Linear_Momentum, MPM_EXAMPLE.cpp:209-225). */
+int mpmx_particle_stats(mpmx_ctx* ctx, double* mass, double* momentum, int64_t* valid_count);
+
+/* Dense grid arrays of the last step, indexed like MPM_EXAMPLE::mass and
+   ::velocity over Domain_Indices(ghost=3): (cells+6)^dim nodes, row-major
+   with the last axis fastest (Core/Arrays_Nd/ARRAYS_ND_BASE.h:88-93).
+   mass: nodes values; velocity: nodes*dim (the velocity G2P reads).  Nodes
+   with zero mass read as zero. */
+int64_t mpmx_grid_node_count(const mpmx_ctx* ctx);
+int mpmx_get_grid(mpmx_ctx* ctx, void* mass, void* velocity);
+
+/* Tuning: reorder particle storage by tile every `interval` steps (>=1). */
+int mpmx_set_reorder_interval(mpmx_ctx* ctx, int interval);
+
+/* Per-phase GPU timing with HIP events on the context stream.  Phases:
+   "bin" (sort + tile lists), "p2g", "grid", "g2p", "reorder", "step".
+   mpmx_phase_time returns the accumulated milliseconds and launch count since
+   the last reset. */
+int mpmx_enable_timing(mpmx_ctx* ctx, int enable);
+int mpmx_reset_timing(mpmx_ctx* ctx);
+int mpmx_phase_time(mpmx_ctx* ctx, const char* phase, double* total_ms, int64_t* count);
+
+/* Stand-alone stress kernel: tau_p = volume_p * P(F_p) F_p^T for n particles
+   (the per-particle term of MPM_FINITE_ELEMENTS::Precompute + Add_Forces,
+   Forces/MPM_FINITE_ELEMENTS.cpp:59-65, :91-96), host arrays in and out in
+   the context precision.  F, tau: n*d*d column-major. */
+int mpmx_stress(mpmx_ctx* ctx, int64_t n, const void* F, const void* volume, const void* mu,
+                const void* lambda, void* tau);
+
+/* Stand-alone SVD kernel F = U diag(s) V^T for n matrices, the convention of
+   MATRIX<T,3>::Singular_Value_Decomposition (Core/Matrices/MATRIX_3X3.cpp:
+   198-236): U, V rotations, |s0|>=|s1|>=|s2|, only the last may be negative. */
+int mpmx_svd(mpmx_ctx* ctx, int64_t n, const void* F, void* U, void* s, void* V);
+
+/* Stand-alone binning sort: stable sort of n uint32 keys (values = positions
+   0..n-1) on the GPU; keys_out / order_out are host arrays of n entries.
+   Only the low key_bits bits take part, and 0xffffffff sorts last. */
+int mpmx_sort_keys(mpmx_ctx* ctx, int64_t n, const uint32_t* keys, int key_bits, uint32_t* keys_out,
+                   uint32_t* order_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPMX_H */

@@ -1,0 +1,180 @@
+// C ABI of the MI355X MPM step (include/mpmx.h).  Every entry point converts
+// exceptions from the engine into a status code and a thread-local message.
+#include "mpmx.h"
+
+#include <exception>
+#include <new>
+#include <string>
+
+#include "mpmx_engine.h"
+
+struct mpmx_ctx {
+  mpmx::EngineBase* engine;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template<class F> int guard(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return MPMX_OK;
+  } catch (const mpmx::Error& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return MPMX_EINTERNAL;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return MPMX_EINTERNAL;
+  } catch (...) {
+    g_last_error = "unknown failure";
+    return MPMX_EINTERNAL;
+  }
+}
+
+int bad(const char* msg) {
+  g_last_error = msg;
+  return MPMX_EINVAL;
+}
+}  // namespace
+
+extern "C" {
+
+const char* mpmx_last_error(void) { return g_last_error.c_str(); }
+
+int mpmx_create(const mpmx_grid_desc* grid, mpmx_ctx** out) {
+  if (!grid || !out) return bad("null argument");
+  *out = nullptr;
+  return guard([&] {
+    mpmx::EngineBase* e = mpmx::make_engine(*grid);
+    *out = new mpmx_ctx{e};
+  });
+}
+
+int mpmx_destroy(mpmx_ctx* ctx) {
+  if (!ctx) return MPMX_OK;
+  int rc = guard([&] { delete ctx->engine; });
+  delete ctx;
+  return rc;
+}
+
+int mpmx_set_particles(mpmx_ctx* ctx, int64_t n, const void* X, const void* V, const void* F, const void* B,
+                       const void* mass, const void* volume, const void* mu, const void* lambda,
+                       const uint8_t* valid) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->set_particles(n, X, V, F, B, mass, volume, mu, lambda, valid); });
+}
+
+int mpmx_get_particles(mpmx_ctx* ctx, void* X, void* V, void* F, void* B, uint8_t* valid) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->get_particles(X, V, F, B, valid); });
+}
+
+int64_t mpmx_particle_count(const mpmx_ctx* ctx) { return ctx ? ctx->engine->particle_count() : -1; }
+
+int mpmx_set_constitutive(mpmx_ctx* ctx, int model, double neo_threshold) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->set_constitutive(model, neo_threshold); });
+}
+
+int mpmx_set_gravity(mpmx_ctx* ctx, const double* g) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->set_gravity(g); });
+}
+
+int mpmx_add_collision_box(mpmx_ctx* ctx, int type, double friction, const double* lo, const double* hi) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->add_collision_box(type, friction, lo, hi); });
+}
+
+int mpmx_clear_collision_objects(mpmx_ctx* ctx) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->clear_collision_objects(); });
+}
+
+int mpmx_step(mpmx_ctx* ctx, double dt, int nsteps) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->step(dt, nsteps, true); });
+}
+
+int mpmx_step_async(mpmx_ctx* ctx, double dt, int nsteps) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->step(dt, nsteps, false); });
+}
+
+int mpmx_synchronize(mpmx_ctx* ctx) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->synchronize(); });
+}
+
+double mpmx_time(const mpmx_ctx* ctx) { return ctx ? ctx->engine->time() : 0.0; }
+
+int mpmx_enable_grid_stats(mpmx_ctx* ctx, int enable) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->enable_grid_stats(enable != 0); });
+}
+
+int mpmx_grid_stats(mpmx_ctx* ctx, double* mass, double* momentum) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->grid_stats(mass, momentum); });
+}
+
+int mpmx_particle_stats(mpmx_ctx* ctx, double* mass, double* momentum, int64_t* valid_count) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->particle_stats(mass, momentum, valid_count); });
+}
+
+int64_t mpmx_grid_node_count(const mpmx_ctx* ctx) { return ctx ? ctx->engine->grid_node_count() : -1; }
+
+int mpmx_get_grid(mpmx_ctx* ctx, void* mass, void* velocity) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->get_grid(mass, velocity); });
+}
+
+int mpmx_set_reorder_interval(mpmx_ctx* ctx, int interval) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->set_reorder_interval(interval); });
+}
+
+int mpmx_enable_timing(mpmx_ctx* ctx, int enable) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->enable_timing(enable != 0); });
+}
+
+int mpmx_reset_timing(mpmx_ctx* ctx) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->reset_timing(); });
+}
+
+int mpmx_phase_time(mpmx_ctx* ctx, const char* phase, double* total_ms, int64_t* count) {
+  if (!ctx) return bad("null context");
+  bool found = false;
+  int rc = guard([&] { found = ctx->engine->phase_time(phase, total_ms, count); });
+  if (rc == MPMX_OK && !found) return bad("unknown phase");
+  return rc;
+}
+
+int mpmx_stress(mpmx_ctx* ctx, int64_t n, const void* F, const void* volume, const void* mu, const void* lambda,
+                void* tau) {
+  if (!ctx) return bad("null context");
+  if (n > 0 && (!F || !volume || !mu || !lambda || !tau)) return bad("null array");
+  return guard([&] { ctx->engine->stress(n, F, volume, mu, lambda, tau); });
+}
+
+int mpmx_svd(mpmx_ctx* ctx, int64_t n, const void* F, void* U, void* s, void* V) {
+  if (!ctx) return bad("null context");
+  if (n > 0 && (!F || !U || !s || !V)) return bad("null array");
+  return guard([&] { ctx->engine->svd(n, F, U, s, V); });
+}
+
+int mpmx_sort_keys(mpmx_ctx* ctx, int64_t n, const uint32_t* keys, int key_bits, uint32_t* keys_out,
+                   uint32_t* order_out) {
+  if (!ctx) return bad("null context");
+  if (n > 0 && (!keys || !keys_out || !order_out)) return bad("null array");
+  return guard([&] { ctx->engine->sort_keys(n, keys, key_bits, keys_out, order_out); });
+}
+
+}  // extern "C"

@@ -1,0 +1,399 @@
+// Device math for the explicit APIC MPM step on gfx950: quadratic B-spline
+// weights, a Jacobi-based SVD of F held in registers, and the per-particle
+// Kirchhoff stress tau = V0 * P(F) * F^T used by the force scatter.
+//
+// Conventions follow PhysBAM's MPM path:
+//   * grid nodes are MAC cell centres x_i = lo + (i + 1/2) h with three ghost
+//     layers, so node coordinate n in [0, N+6) is reference index n-3
+//     (Grid_Tools/Grids/GRID.h:123-124, :258-259);
+//   * the stencil base is floor((x - h - lo)/h), local coordinate
+//     r = (x - x_base)/h in [0.5, 1.5) (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:133-140);
+//   * the SVD returns rotations U, V with |s0|>=|s1|>=|s2| and only s2 possibly
+//     negative (Core/Matrices/MATRIX_3X3.cpp:198-205, :115-122).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpmx {
+
+#define MPMX_HD __host__ __device__ __forceinline__
+
+template<typename T> MPMX_HD T tsqrt(T x);
+template<> MPMX_HD float tsqrt<float>(float x) { return sqrtf(x); }
+template<> MPMX_HD double tsqrt<double>(double x) { return sqrt(x); }
+template<typename T> MPMX_HD T tabs(T x) { return x < 0 ? -x : x; }
+template<typename T> MPMX_HD T tlog(T x);
+template<> MPMX_HD float tlog<float>(float x) { return logf(x); }
+template<> MPMX_HD double tlog<double>(double x) { return log(x); }
+template<typename T> MPMX_HD T tfloor(T x);
+template<> MPMX_HD float tfloor<float>(float x) { return floorf(x); }
+template<> MPMX_HD double tfloor<double>(double x) { return floor(x); }
+
+// Tile geometry: a tile is 4x4x4 nodes in 3D and 8x8 nodes in 2D (64 nodes).
+// The LDS region of a particle tile covers the stencils of every particle whose
+// base node lies in the tile: 6^3 (3D) or 10^2 (2D) nodes.
+template<int D> struct Tile;
+template<> struct Tile<3> { static constexpr int E = 4, R = 6, NODES = 64, RN = 216, K = 27; };
+template<> struct Tile<2> { static constexpr int E = 8, R = 10, NODES = 64, RN = 100, K = 9; };
+
+constexpr uint32_t INVALID_KEY = 0xffffffffu;
+constexpr int MAX_WALLS = 16;
+
+template<typename T> struct alignas(4 * sizeof(T)) Node4 { T m, x, y, z; };
+
+template<typename T> struct WallDev {
+  int type;  // 0 stick, 1 slip, 2 separate
+  int pad;
+  T friction;
+  T lo[3], hi[3];
+};
+
+template<typename T, int D> struct Geom {
+  T lo[D], hi[D], dx[D], inv_dx[D];
+  int G[D];   // nodes per axis, N + 6
+  int GT[D];  // tiles per axis (storage), ceil(G/E) + 1
+  int ntiles; // prod GT
+};
+
+// Stencil base (node coordinate, ghost offset included) and 1D weights.
+template<typename T, int D> struct Wts {
+  int base[D];
+  T xe[D];              // x - x_base
+  T w[3][D], dw[3][D];
+};
+
+template<typename T, int D>
+MPMX_HD void weights(const Geom<T, D>& g, const T* X, Wts<T, D>& s) {
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    const T inv = g.inv_dx[a];
+    int b = (int)tfloor<T>(((X[a] - g.dx[a]) - g.lo[a]) * inv);
+    T cen = g.lo[a] + ((T)b + (T)0.5) * g.dx[a];
+    T xe = X[a] - cen;
+    T x = xe * inv;
+    s.base[a] = b + 3;
+    s.xe[a] = xe;
+    T q0 = x - (T)1.5, q1 = x - (T)1, q2 = x - (T)0.5;
+    s.w[0][a] = (T)0.5 * q0 * q0;
+    s.w[1][a] = (T)0.75 - q1 * q1;
+    s.w[2][a] = (T)0.5 * q2 * q2;
+    s.dw[0][a] = q0 * inv;
+    s.dw[1][a] = (T)(-2) * q1 * inv;
+    s.dw[2][a] = q2 * inv;
+  }
+}
+
+// Linear tile id of the particle's base node, or INVALID_KEY if the stencil
+// would leave the ghosted grid.
+template<typename T, int D>
+MPMX_HD uint32_t tile_key(const Geom<T, D>& g, const T* X) {
+  uint32_t key = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    int b = (int)tfloor<T>(((X[a] - g.dx[a]) - g.lo[a]) * g.inv_dx[a]) + 3;
+    if (!(b >= 0 && b + 2 < g.G[a])) return INVALID_KEY;
+    key = key * (uint32_t)g.GT[a] + (uint32_t)(b / Tile<D>::E);
+  }
+  return key;
+}
+
+// -------------------------------------------------------------------- SVD --
+// One-sided Jacobi: diagonalise S = F^T F with cyclic exact rotations (fixed
+// sweep count), then B = F V, sort columns by norm, and Givens-QR B = U R.
+// diag(R) are the singular values; R's off-diagonals vanish to rounding.
+
+template<typename T> struct SweepCount;
+template<> struct SweepCount<float> { static constexpr int n = 4; };
+template<> struct SweepCount<double> { static constexpr int n = 6; };
+
+// Rotation (c, s) annihilating S_pq of a symmetric 2x2 block [app apq; apq aqq].
+template<typename T>
+MPMX_HD void jacobi_rot(T app, T aqq, T apq, T& c, T& s) {
+  if (apq == (T)0) { c = 1; s = 0; return; }
+  T tau = (aqq - app) / ((T)2 * apq);
+  T t = (tau >= 0 ? (T)1 : (T)-1) / (tabs(tau) + tsqrt(1 + tau * tau));
+  c = (T)1 / tsqrt(1 + t * t);
+  s = t * c;
+}
+
+// Column-major 3x3 (m[i + 3j]) SVD: F = U diag(s) V^T.
+template<typename T>
+MPMX_HD void svd3(const T* F, T* U, T* s, T* V) {
+  // S = F^T F (symmetric)
+  T S[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      S[i][j] = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2];
+  T Vm[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll
+  for (int sweep = 0; sweep < SweepCount<T>::n; sweep++) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int p = k == 2 ? 1 : 0, q = k == 0 ? 1 : 2;
+      T c, sn;
+      jacobi_rot(S[p][p], S[q][q], S[p][q], c, sn);
+      // S <- J^T S J with J = I except J_pp=c, J_pq=s, J_qp=-s, J_qq=c
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        T sp = S[r][p], sq = S[r][q];
+        S[r][p] = c * sp - sn * sq;
+        S[r][q] = sn * sp + c * sq;
+      }
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        T sp = S[p][r], sq = S[q][r];
+        S[p][r] = c * sp - sn * sq;
+        S[q][r] = sn * sp + c * sq;
+      }
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        T vp = Vm[r][p], vq = Vm[r][q];
+        Vm[r][p] = c * vp - sn * vq;
+        Vm[r][q] = sn * vp + c * vq;
+      }
+    }
+  }
+  // B = F V (Bm[row][col])
+  T Bm[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      Bm[i][j] = F[i] * Vm[0][j] + F[i + 3] * Vm[1][j] + F[i + 6] * Vm[2][j];
+  T n0 = Bm[0][0] * Bm[0][0] + Bm[1][0] * Bm[1][0] + Bm[2][0] * Bm[2][0];
+  T n1 = Bm[0][1] * Bm[0][1] + Bm[1][1] * Bm[1][1] + Bm[2][1] * Bm[2][1];
+  T n2 = Bm[0][2] * Bm[0][2] + Bm[1][2] * Bm[1][2] + Bm[2][2] * Bm[2][2];
+  // Sort columns by decreasing norm; a swap negates one column to keep det V = +1.
+  auto swapc = [&](int a, int b, T& na, T& nb) {
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      T t = Bm[r][a]; Bm[r][a] = -Bm[r][b]; Bm[r][b] = t;
+      T u = Vm[r][a]; Vm[r][a] = -Vm[r][b]; Vm[r][b] = u;
+    }
+    T t = na; na = nb; nb = t;
+  };
+  if (n0 < n1) swapc(0, 1, n0, n1);
+  if (n0 < n2) swapc(0, 2, n0, n2);
+  if (n1 < n2) swapc(1, 2, n1, n2);
+  // Givens QR of B; U accumulates the rotations (det U = +1).
+  T Um[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  auto givens = [&](int r0, int r1, int col) {
+    T a = Bm[r0][col], b = Bm[r1][col];
+    T rr = tsqrt(a * a + b * b);
+    T c = 1, sn = 0;
+    if (rr > (T)0) { c = a / rr; sn = b / rr; }
+    // rows r0, r1 of B <- G B with G = [c s; -s c]; U <- U G^T
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      T x0 = Bm[r0][j], x1 = Bm[r1][j];
+      Bm[r0][j] = c * x0 + sn * x1;
+      Bm[r1][j] = -sn * x0 + c * x1;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      T u0 = Um[i][r0], u1 = Um[i][r1];
+      Um[i][r0] = c * u0 + sn * u1;
+      Um[i][r1] = -sn * u0 + c * u1;
+    }
+  };
+  givens(0, 1, 0);
+  givens(0, 2, 0);
+  givens(1, 2, 1);
+  s[0] = Bm[0][0]; s[1] = Bm[1][1]; s[2] = Bm[2][2];
+  // r00, r11 >= 0 by construction; only s[2] carries the sign of det F.
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) { U[i + 3 * j] = Um[i][j]; V[i + 3 * j] = Vm[i][j]; }
+}
+
+// Column-major 2x2 SVD with the same construction.
+template<typename T>
+MPMX_HD void svd2(const T* F, T* U, T* s, T* V) {
+  T s00 = F[0] * F[0] + F[1] * F[1];
+  T s11 = F[2] * F[2] + F[3] * F[3];
+  T s01 = F[0] * F[2] + F[1] * F[3];
+  T c, sn;
+  jacobi_rot(s00, s11, s01, c, sn);
+  T Vm[2][2] = {{c, sn}, {-sn, c}};
+  T Bm[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) Bm[i][j] = F[i] * Vm[0][j] + F[i + 2] * Vm[1][j];
+  T n0 = Bm[0][0] * Bm[0][0] + Bm[1][0] * Bm[1][0];
+  T n1 = Bm[0][1] * Bm[0][1] + Bm[1][1] * Bm[1][1];
+  if (n0 < n1) {
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      T t = Bm[r][0]; Bm[r][0] = -Bm[r][1]; Bm[r][1] = t;
+      T u = Vm[r][0]; Vm[r][0] = -Vm[r][1]; Vm[r][1] = u;
+    }
+  }
+  T a = Bm[0][0], b = Bm[1][0];
+  T rr = tsqrt(a * a + b * b);
+  T cc = 1, ss = 0;
+  if (rr > (T)0) { cc = a / rr; ss = b / rr; }
+  s[0] = cc * Bm[0][0] + ss * Bm[1][0];
+  s[1] = -ss * Bm[0][1] + cc * Bm[1][1];
+  U[0] = cc; U[1] = ss; U[2] = -ss; U[3] = cc;
+  V[0] = Vm[0][0]; V[1] = Vm[1][0]; V[2] = Vm[0][1]; V[3] = Vm[1][1];
+}
+
+template<typename T, int D> MPMX_HD void svd(const T* F, T* U, T* s, T* V);
+template<> MPMX_HD void svd<float, 3>(const float* F, float* U, float* s, float* V) { svd3(F, U, s, V); }
+template<> MPMX_HD void svd<double, 3>(const double* F, double* U, double* s, double* V) { svd3(F, U, s, V); }
+template<> MPMX_HD void svd<float, 2>(const float* F, float* U, float* s, float* V) { svd2(F, U, s, V); }
+template<> MPMX_HD void svd<double, 2>(const double* F, double* U, double* s, double* V) { svd2(F, U, s, V); }
+
+// ------------------------------------------------------------- stress ------
+// Diagonal first Piola stress P_hat(sigma) in singular-value space.
+//   fixed corotated: 2 mu (s - 1) + lambda (J - 1) cof(s)
+//     (Deformables/Constitutive_Models/COROTATED_FIXED.cpp:59-65)
+//   neo-Hookean:     mu s - (mu - lambda ln J) / s           for J >= t,
+//     otherwise the hyperbola-clamped extrapolation of NEO_HOOKEAN.cpp:44-81.
+template<typename T, int D>
+MPMX_HD void p_hat(int model, T mu, T la, T thr, T root, const T* s, T* P) {
+  T J = s[0];
+#pragma unroll
+  for (int i = 1; i < D; i++) J *= s[i];
+  if (model == 0) {
+    T cof[D];
+    if constexpr (D == 3) { cof[0] = s[1] * s[2]; cof[1] = s[0] * s[2]; cof[2] = s[0] * s[1]; }
+    else { cof[0] = s[1]; cof[1] = s[0]; }
+#pragma unroll
+    for (int i = 0; i < D; i++) P[i] = (T)2 * mu * (s[i] - (T)1) + la * (J - (T)1) * cof[i];
+    return;
+  }
+  if (J >= thr) {
+    T c = mu - la * tlog<T>(J);
+#pragma unroll
+    for (int i = 0; i < D; i++) P[i] = mu * s[i] - c / s[i];
+    return;
+  }
+  // Clamp to the hyperbola prod(s) = t along the largest singular values.
+  T Fc[D];
+  if constexpr (D == 3) {
+    if (s[0] * s[1] * s[1] > thr) { Fc[0] = s[0]; Fc[1] = s[1]; Fc[2] = thr / (s[0] * s[1]); }
+    else if (s[0] * s[0] * s[0] > thr) { T q = tsqrt(thr / s[0]); Fc[0] = s[0]; Fc[1] = q; Fc[2] = q; }
+    else { Fc[0] = root; Fc[1] = root; Fc[2] = root; }
+  } else {
+    if (s[0] * s[0] > thr) { Fc[0] = s[0]; Fc[1] = thr / s[0]; }
+    else { Fc[0] = root; Fc[1] = root; }
+  }
+  T c = mu - la * tlog<T>(thr);
+  T ip = 0, Fi[D], dF[D];
+#pragma unroll
+  for (int i = 0; i < D; i++) { Fi[i] = (T)1 / Fc[i]; dF[i] = s[i] - Fc[i]; ip += Fi[i] * dF[i]; }
+#pragma unroll
+  for (int i = 0; i < D; i++) P[i] = mu * s[i] + c * (Fi[i] * Fi[i] * dF[i] - Fi[i]) + la * ip * Fi[i];
+}
+
+// tau = vol * P F^T = vol * U diag(P_hat_i * s_i) U^T, symmetric; returned as a
+// full column-major D x D matrix.  The FP64 path is used for near-degenerate F
+// (singular values far apart in magnitude or the neo-Hookean clamp branch).
+template<typename T, int D>
+MPMX_HD void kirchhoff_typed(int model, T mu, T la, T thr, T root, T vol, const T* F, T* tau, bool& degenerate) {
+  T U[D * D], V[D * D], s[D], P[D];
+  svd<T, D>(F, U, s, V);
+  T J = s[0];
+#pragma unroll
+  for (int i = 1; i < D; i++) J *= s[i];
+  degenerate = (model == 1 && J < thr) || tabs(s[D - 1]) < (T)1e-3 * tabs(s[0]);
+  p_hat<T, D>(model, mu, la, thr, root, s, P);
+  T k[D];
+#pragma unroll
+  for (int i = 0; i < D; i++) k[i] = vol * P[i] * s[i];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      T acc = 0;
+#pragma unroll
+      for (int q = 0; q < D; q++) acc += U[i + D * q] * k[q] * U[j + D * q];
+      tau[i + D * j] = acc;
+    }
+}
+
+template<typename T, int D>
+__device__ __forceinline__ void kirchhoff(int model, T mu, T la, T thr, T root, T vol, const T* F, T* tau) {
+  bool degenerate;
+  kirchhoff_typed<T, D>(model, mu, la, thr, root, vol, F, tau, degenerate);
+  if (sizeof(T) == 4 && degenerate) {
+    double Fd[D * D], taud[D * D];
+    bool dd;
+#pragma unroll
+    for (int i = 0; i < D * D; i++) Fd[i] = (double)F[i];
+    kirchhoff_typed<double, D>(model, (double)mu, (double)la, (double)thr, (double)root, (double)vol, Fd, taud, dd);
+#pragma unroll
+    for (int i = 0; i < D * D; i++) tau[i] = (T)taud[i];
+  }
+}
+
+// ---------------------------------------------------------------- walls ----
+// Axis-aligned box signed distance and normal (Core/Math_Tools/RANGE.cpp:38-63).
+template<typename T, int D>
+MPMX_HD T box_phi(const WallDev<T>& w, const T* X) {
+  T phi[D];
+  bool inside = true;
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    T c = (w.lo[i] + w.hi[i]) * (T)0.5, hl = (T)0.5 * (w.hi[i] - w.lo[i]);
+    phi[i] = tabs(X[i] - c) - hl;
+    if (!(phi[i] <= 0)) inside = false;
+  }
+  if (!inside) {
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < D; i++) { T p = phi[i] > 0 ? phi[i] : (T)0; s += p * p; }
+    return tsqrt(s);
+  }
+  T m = phi[0];
+#pragma unroll
+  for (int i = 1; i < D; i++) m = phi[i] > m ? phi[i] : m;
+  return m;
+}
+
+template<typename T, int D>
+MPMX_HD void box_normal(const WallDev<T>& w, const T* X, T* n) {
+  bool inside = true;
+  T pmax[D], pmin[D];
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    if (!(X[i] >= w.lo[i] && X[i] <= w.hi[i])) inside = false;
+    pmax[i] = X[i] - w.hi[i];
+    pmin[i] = w.lo[i] - X[i];
+    n[i] = 0;
+  }
+  if (inside) {
+    // Face of least penetration; ties go to the lower axis.
+    int axis = 0;
+    T best = pmin[0] > pmax[0] ? pmin[0] : pmax[0];
+#pragma unroll
+    for (int i = 1; i < D; i++) {
+      T m = pmin[i] > pmax[i] ? pmin[i] : pmax[i];
+      if (m > best) { best = m; axis = i; }
+    }
+    n[axis] = pmax[axis] > pmin[axis] ? (T)1 : (T)-1;
+    return;
+  }
+  T s = 0;
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    T phi = pmin[i] > pmax[i] ? pmin[i] : pmax[i];
+    n[i] = phi > 0 ? (pmax[i] > pmin[i] ? phi : -phi) : (T)0;
+    s += n[i] * n[i];
+  }
+  T m = tsqrt(s);
+  if (m > 0) {
+    T inv = (T)1 / m;
+#pragma unroll
+    for (int i = 0; i < D; i++) n[i] *= inv;
+  } else {
+    n[0] = 1;
+  }
+}
+
+}  // namespace mpmx

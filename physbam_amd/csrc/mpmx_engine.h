@@ -1,0 +1,52 @@
+// Host-side interface between the C ABI (mpmx_api.cpp) and the templated GPU
+// engine (mpmx_step.hip).  No HIP types appear here.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+
+struct mpmx_grid_desc;
+
+namespace mpmx {
+
+struct EngineBase {
+  virtual ~EngineBase() {}
+  virtual int dim() const = 0;
+  virtual int precision() const = 0;
+  virtual void set_particles(int64_t n, const void* X, const void* V, const void* F, const void* B,
+                             const void* mass, const void* volume, const void* mu, const void* lambda,
+                             const uint8_t* valid) = 0;
+  virtual void get_particles(void* X, void* V, void* F, void* B, uint8_t* valid) = 0;
+  virtual int64_t particle_count() const = 0;
+  virtual void set_constitutive(int model, double neo_threshold) = 0;
+  virtual void set_gravity(const double* g) = 0;
+  virtual void add_collision_box(int type, double friction, const double* lo, const double* hi) = 0;
+  virtual void clear_collision_objects() = 0;
+  virtual void step(double dt, int nsteps, bool wait) = 0;
+  virtual void synchronize() = 0;
+  virtual double time() const = 0;
+  virtual void enable_grid_stats(bool on) = 0;
+  virtual void grid_stats(double* mass, double* momentum) = 0;
+  virtual void particle_stats(double* mass, double* momentum, int64_t* valid_count) = 0;
+  virtual int64_t grid_node_count() const = 0;
+  virtual void get_grid(void* mass, void* velocity) = 0;
+  virtual void set_reorder_interval(int interval) = 0;
+  virtual void enable_timing(bool on) = 0;
+  virtual void reset_timing() = 0;
+  virtual bool phase_time(const char* phase, double* total_ms, int64_t* count) = 0;
+  virtual void stress(int64_t n, const void* F, const void* volume, const void* mu, const void* lambda,
+                      void* tau) = 0;
+  virtual void svd(int64_t n, const void* F, void* U, void* s, void* V) = 0;
+  virtual void sort_keys(int64_t n, const uint32_t* keys, int key_bits, uint32_t* keys_out,
+                         uint32_t* order_out) = 0;
+};
+
+// Throws mpmx::Error on failure.
+EngineBase* make_engine(const mpmx_grid_desc& desc);
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+}  // namespace mpmx

@@ -1,0 +1,1347 @@
+// Explicit APIC MPM step on gfx950: binning, P2G with fused stress, grid
+// update with gravity and walls, G2P with the F/B/X/V update.
+//
+// Reference step (Public_Library/Hybrid_Methods/Examples_And_Drivers/MPM_DRIVER.cpp):
+//   Update_Simulated_Particles :1142-1157   -> invalid particles carry INVALID_KEY
+//   Update_Particle_Weights    :233-241     -> recomputed in registers (weights())
+//   Prepare_Scatter            GATHER_SCATTER.cpp:32-86 -> radix sort by tile
+//   Particle_To_Grid           :245-306     -> p2g_kernel
+//   Apply_Forces (symplectic)  :959-1060    -> stress folded into p2g_kernel,
+//                                              gravity/walls/friction in grid_kernel
+//   Grid_To_Particle           :450-515     -> g2p_kernel
+//   Reflect_Or_Invalidate      :1219-1239   -> g2p_kernel
+//
+// Data layout in HBM
+//   particles: structure of arrays, component-major (X[a*n + p], F[k*n + p]
+//              with k the column-major entry), stored in tile order and
+//              re-gathered by the binning permutation every few steps;
+//   grid:      tile-blocked nodes (64 per tile: 4^3 in 3D, 8^2 in 2D), each
+//              node {m, p_x, p_y, p_z} (momentum during P2G, velocity after the
+//              grid update); only tiles touched this step are cleared and used.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "mpmx.h"
+#include "mpmx_device.h"
+#include "mpmx_engine.h"
+#include "mpmx_sort.h"
+
+namespace mpmx {
+
+#define HIPCHECK(x)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) throw Error{MPMX_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+constexpr int BLOCK = 256;
+
+template<typename T, int D> struct Args {
+  Geom<T, D> g;
+  T vlo[D], vhi[D];  // particles outside [vlo, vhi] are invalidated (MPM_DRIVER.cpp:1227-1238)
+  T dt;
+  T grav[D];
+  int model;
+  T thr, root;
+  int nwalls;
+  WallDev<T> walls[MAX_WALLS];
+  T* X; T* V; T* F; T* B;
+  const T* mass; const T* vol; const T* mu; const T* la;
+  uint8_t* valid;
+  uint32_t* keys;
+  uint32_t n;
+  const uint32_t* order;  // sorted position -> slot, null for identity
+  const uint32_t* tile_begin;
+  const uint32_t* tile_end;
+  const uint32_t* ptiles;
+  const uint32_t* gtiles;
+  const uint32_t* counts;  // [0] particle tiles, [1] grid tiles, [2] valid particles
+  Node4<T>* grid;
+  Node4<T>* gridf;
+  uint32_t* err;
+  double* stats_partial;
+};
+
+template<int D> __device__ __forceinline__ void tile_origin(uint32_t t, const int* GT, int* o) {
+#pragma unroll
+  for (int a = D - 1; a >= 0; a--) {
+    o[a] = (int)(t % (uint32_t)GT[a]) * Tile<D>::E;
+    t /= (uint32_t)GT[a];
+  }
+}
+
+template<int D> __device__ __forceinline__ uint32_t node_slot(const int* nc, const int* GT) {
+  uint32_t t = 0, l = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    t = t * (uint32_t)GT[a] + (uint32_t)(nc[a] / Tile<D>::E);
+    l = l * Tile<D>::E + (uint32_t)(nc[a] % Tile<D>::E);
+  }
+  return t * 64u + l;
+}
+
+// Region-local flat index -> offsets (row-major, last axis fastest).
+template<int D> __device__ __forceinline__ void region_offsets(int k, int* off) {
+#pragma unroll
+  for (int a = D - 1; a >= 0; a--) {
+    off[a] = k % Tile<D>::R;
+    k /= Tile<D>::R;
+  }
+}
+
+template<typename T> __device__ __forceinline__ void atomic_add(T* p, T v) { atomicAdd(p, v); }
+
+// ----------------------------------------------------------------- keys ----
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void key_kernel(Args<T, D> a, uint32_t* range_err) {
+  const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
+  if (p >= a.n) return;
+  uint32_t key = INVALID_KEY;
+  if (a.valid[p]) {
+    T x[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+    key = tile_key<T, D>(a.g, x);
+    if (key == INVALID_KEY) atomicAdd(range_err, 1u);
+  }
+  a.keys[p] = key;
+}
+
+__global__ __launch_bounds__(BLOCK) void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Tile ranges of the sorted keys.  A tile is occupied this step iff its stamp
+// equals the step stamp, so nothing has to be cleared between steps.
+__global__ __launch_bounds__(BLOCK) void bounds_kernel(const uint32_t* sk, uint32_t n, uint32_t* tb, uint32_t* te,
+                                                       uint32_t* stamp_arr, uint32_t stamp, uint32_t* counts) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = sk[i];
+  if (k == INVALID_KEY) return;
+  if (i == 0 || sk[i - 1] != k) {
+    tb[k] = i;
+    stamp_arr[k] = stamp;
+  }
+  const bool last = (i == n - 1) || sk[i + 1] != k;
+  if (last) te[k] = i + 1;
+  if (i == n - 1 || sk[i + 1] == INVALID_KEY) counts[2] = i + 1;
+}
+
+// pflag: tile holds particles; gflag: tile is touched by some occupied tile's
+// stencil region (the tile itself or a neighbour in the -x/-y/-z octant).
+template<int D>
+__global__ __launch_bounds__(BLOCK) void flags_kernel(const uint32_t* stamp_arr, uint32_t stamp, uint32_t ntiles,
+                                                      int GT0, int GT1, int GT2, uint32_t* pflag, uint32_t* gflag) {
+  const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= ntiles) return;
+  const int GT[3] = {GT0, GT1, GT2};
+  int c[D];
+  uint32_t r = t;
+#pragma unroll
+  for (int a = D - 1; a >= 0; a--) {
+    c[a] = (int)(r % (uint32_t)GT[a]);
+    r /= (uint32_t)GT[a];
+  }
+  pflag[t] = stamp_arr[t] == stamp ? 1u : 0u;
+  uint32_t g = 0;
+#pragma unroll
+  for (int m = 0; m < (1 << D); m++) {
+    bool ok = true;
+    uint32_t q = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      const int cc = c[a] - ((m >> a) & 1);
+      ok = ok && cc >= 0;
+      q = q * (uint32_t)GT[a] + (uint32_t)(cc < 0 ? 0 : cc);
+    }
+    if (ok && stamp_arr[q] == stamp) g = 1u;
+  }
+  gflag[t] = g;
+}
+
+__global__ __launch_bounds__(BLOCK) void lists_kernel(uint32_t ntiles, const uint32_t* pflag, const uint32_t* pscan,
+                                                      const uint32_t* gflag, const uint32_t* gscan, uint32_t* ptiles,
+                                                      uint32_t* gtiles) {
+  const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= ntiles) return;
+  if (pflag[t]) ptiles[pscan[t]] = t;
+  if (gflag[t]) gtiles[gscan[t]] = t;
+}
+
+template<typename T, bool FRICTION>
+__global__ __launch_bounds__(BLOCK) void zero_kernel(const uint32_t* gtiles, const uint32_t* counts, Node4<T>* grid,
+                                                     Node4<T>* gridf) {
+  const uint64_t total = (uint64_t)counts[1] * 64u;
+  Node4<T> z;
+  z.m = z.x = z.y = z.z = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < total; i += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t slot = gtiles[i >> 6] * 64u + (uint32_t)(i & 63u);
+    grid[slot] = z;
+    if (FRICTION) gridf[slot] = z;
+  }
+}
+
+// ------------------------------------------------------------------ P2G ----
+// One workgroup per occupied tile (grid-stride over the tile list).  Mass and
+// momentum, including the stress impulse -dt * tau_p * grad w_ip of
+// MPM_FINITE_ELEMENTS::Add_Forces (Forces/MPM_FINITE_ELEMENTS.cpp:84-99), are
+// accumulated in LDS for the tile's stencil region and flushed to HBM with
+// one atomic per touched node and component.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void p2g_kernel(Args<T, D> a) {
+  constexpr int E = Tile<D>::E, R = Tile<D>::R, RN = Tile<D>::RN;
+  __shared__ T s_m[RN];
+  __shared__ T s_p[D][RN];
+  const uint32_t ntile = a.counts[0];
+  T dinv[D];
+#pragma unroll
+  for (int c = 0; c < D; c++) dinv[c] = (T)4 * a.g.inv_dx[c] * a.g.inv_dx[c];  // D_p^-1 (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:152-158)
+  for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
+    const uint32_t t = a.ptiles[seg];
+    int org[D];
+    tile_origin<D>(t, a.g.GT, org);
+    for (int k = threadIdx.x; k < RN; k += BLOCK) {
+      s_m[k] = 0;
+#pragma unroll
+      for (int c = 0; c < D; c++) s_p[c][k] = 0;
+    }
+    __syncthreads();
+    const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
+    for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
+      const uint32_t p = a.order ? a.order[i] : i;
+      T x[D], v[D], Bm[D * D], Fm[D * D];
+#pragma unroll
+      for (int c = 0; c < D; c++) { x[c] = a.X[c * a.n + p]; v[c] = a.V[c * a.n + p]; }
+#pragma unroll
+      for (int k = 0; k < D * D; k++) { Bm[k] = a.B[k * a.n + p]; Fm[k] = a.F[k * a.n + p]; }
+      const T m = a.mass[p];
+      Wts<T, D> w;
+      weights<T, D>(a.g, x, w);
+      int lb[D];
+      bool inside = true;
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        lb[c] = w.base[c] - org[c];
+        inside = inside && lb[c] >= 0 && lb[c] < E;
+      }
+      if (!inside) {
+        atomicOr(a.err, 1u);
+        continue;
+      }
+      T tau[D * D];
+      kirchhoff<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
+      // momentum per node: m w (v + B D^-1 (x_i - x_p)) - dt tau grad w
+      T mv[D], mC[D * D], Q[D * D];
+#pragma unroll
+      for (int c = 0; c < D; c++) mv[c] = m * v[c];
+#pragma unroll
+      for (int j = 0; j < D; j++)
+#pragma unroll
+        for (int c = 0; c < D; c++) {
+          mC[c + D * j] = m * Bm[c + D * j] * dinv[j];
+          Q[c + D * j] = a.dt * tau[c + D * j];
+        }
+      T dz[3][D];
+#pragma unroll
+      for (int o = 0; o < 3; o++)
+#pragma unroll
+        for (int c = 0; c < D; c++) dz[o][c] = (T)o * a.g.dx[c] - w.xe[c];
+      if constexpr (D == 3) {
+#pragma unroll
+        for (int ox = 0; ox < 3; ox++) {
+#pragma unroll
+          for (int oy = 0; oy < 3; oy++) {
+            const T wxy = w.w[ox][0] * w.w[oy][1];
+#pragma unroll
+            for (int oz = 0; oz < 3; oz++) {
+              const T wt = wxy * w.w[oz][2];
+              T g[3];
+              g[0] = w.dw[ox][0] * w.w[oy][1] * w.w[oz][2];
+              g[1] = w.w[ox][0] * w.dw[oy][1] * w.w[oz][2];
+              g[2] = wxy * w.dw[oz][2];
+              const int li = ((lb[0] + ox) * R + (lb[1] + oy)) * R + (lb[2] + oz);
+              atomicAdd(&s_m[li], wt * m);
+#pragma unroll
+              for (int c = 0; c < 3; c++) {
+                T aff = mv[c] + mC[c] * dz[ox][0] + mC[c + 3] * dz[oy][1] + mC[c + 6] * dz[oz][2];
+                T f = Q[c] * g[0] + Q[c + 3] * g[1] + Q[c + 6] * g[2];
+                atomicAdd(&s_p[c][li], wt * aff - f);
+              }
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int ox = 0; ox < 3; ox++) {
+#pragma unroll
+          for (int oy = 0; oy < 3; oy++) {
+            const T wt = w.w[ox][0] * w.w[oy][1];
+            T g[2];
+            g[0] = w.dw[ox][0] * w.w[oy][1];
+            g[1] = w.w[ox][0] * w.dw[oy][1];
+            const int li = (lb[0] + ox) * R + (lb[1] + oy);
+            atomicAdd(&s_m[li], wt * m);
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+              T aff = mv[c] + mC[c] * dz[ox][0] + mC[c + 2] * dz[oy][1];
+              T f = Q[c] * g[0] + Q[c + 2] * g[1];
+              atomicAdd(&s_p[c][li], wt * aff - f);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < RN; k += BLOCK) {
+      const T mm = s_m[k];
+      if (mm != (T)0) {
+        int off[D], nc[D];
+        region_offsets<D>(k, off);
+#pragma unroll
+        for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
+        Node4<T>* nd = a.grid + node_slot<D>(nc, a.g.GT);
+        atomic_add(&nd->m, mm);
+        atomic_add(&nd->x, s_p[0][k]);
+        atomic_add(&nd->y, s_p[1][k]);
+        if constexpr (D == 3) atomic_add(&nd->z, s_p[2][k]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------------------- grid update ---
+// v = p/m + dt g, then each collision object in order (MPM_OBJECTIVE.cpp:150-182),
+// then Coulomb friction at colliding nodes (MPM_DRIVER.cpp:1064-1094).
+template<typename T, int D, bool FRICTION>
+__global__ __launch_bounds__(BLOCK) void grid_kernel(Args<T, D> a) {
+  const uint64_t total = (uint64_t)a.counts[1] * 64u;
+  double acc[4] = {0, 0, 0, 0};
+  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total; idx += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t t = a.gtiles[idx >> 6];
+    const uint32_t l = (uint32_t)(idx & 63u);
+    const uint32_t slot = t * 64u + l;
+    Node4<T> nd = a.grid[slot];
+    if (nd.m == (T)0) continue;
+    const T im = (T)1 / nd.m;
+    T v[D];
+    v[0] = nd.x * im + a.dt * a.grav[0];
+    v[1] = nd.y * im + a.dt * a.grav[1];
+    if constexpr (D == 3) v[2] = nd.z * im + a.dt * a.grav[2];
+    T vf[D];
+    if (a.nwalls > 0) {
+      int org[D], nc[D];
+      tile_origin<D>(t, a.g.GT, org);
+      uint32_t r = l;
+#pragma unroll
+      for (int c = D - 1; c >= 0; c--) {
+        nc[c] = org[c] + (int)(r % Tile<D>::E);
+        r /= Tile<D>::E;
+      }
+      T X0[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) X0[c] = a.g.lo[c] + ((T)(nc[c] - 3) + (T)0.5) * a.g.dx[c];
+      T vpre[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) vpre[c] = v[c];
+      bool stuck = false;
+      int ncol = 0;
+      T cn[FRICTION ? MAX_WALLS : 1][D];
+      int cobj[FRICTION ? MAX_WALLS : 1];
+      const T inv_dt = (T)1 / a.dt;
+      for (int o = 0; o < a.nwalls; o++) {
+        const WallDev<T>& wl = a.walls[o];
+        T X[D];
+#pragma unroll
+        for (int c = 0; c < D; c++) X[c] = X0[c] + a.dt * v[c];
+        const T phi0 = box_phi<T, D>(wl, X0);
+        const T phi = box_phi<T, D>(wl, X) - (phi0 < 0 ? phi0 : (T)0);
+        if (wl.type == 0 && phi0 > 0) continue;
+        if (wl.type == 1 && phi0 > 0) continue;
+        if (wl.type == 2 && phi > (T)1e-15) continue;
+        if (wl.type != 0 && phi > (T)3.402823466e38) continue;
+        T nrm[D];
+        box_normal<T, D>(wl, X, nrm);
+#pragma unroll
+        for (int c = 0; c < D; c++) {
+          X[c] -= phi * nrm[c];
+          v[c] = (X[c] - X0[c]) * inv_dt;
+        }
+        if (wl.type == 0) {
+#pragma unroll
+          for (int c = 0; c < D; c++) v[c] = 0;
+          stuck = true;
+        }
+        if constexpr (FRICTION) {
+#pragma unroll
+          for (int c = 0; c < D; c++) cn[ncol][c] = nrm[c];
+          cobj[ncol] = o;
+          ncol++;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < D; c++) vf[c] = v[c];
+      if constexpr (FRICTION) {
+        if (ncol > 0) {
+          T v1[D];
+#pragma unroll
+          for (int c = 0; c < D; c++) v1[c] = stuck ? (T)0 : v[c];
+          for (int q = 0; q < ncol; q++) {
+            T nf = 0, vn = 0;
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+              nf += cn[q][c] * (v[c] - vpre[c]);
+              vn += v1[c] * cn[q][c];
+            }
+            T tv[D], tm = 0;
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+              tv[c] = v1[c] - vn * cn[q][c];
+              tm += tv[c] * tv[c];
+            }
+            tm = tsqrt<T>(tm);
+            const T k = a.walls[cobj[q]].friction * nf;
+            if (tm <= k) {
+#pragma unroll
+              for (int c = 0; c < D; c++) v1[c] = vn * cn[q][c];
+            } else {
+              const T it = tm > 0 ? (T)1 / tm : (T)0;
+#pragma unroll
+              for (int c = 0; c < D; c++) v1[c] -= k * (tm > 0 ? tv[c] * it : (c == 0 ? (T)1 : (T)0));
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < D; c++) vf[c] = v1[c];
+        }
+        if (stuck) {
+#pragma unroll
+          for (int c = 0; c < D; c++) vf[c] = 0;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < D; c++) vf[c] = v[c];
+    }
+    Node4<T> out;
+    out.m = nd.m;
+    out.x = v[0];
+    out.y = v[1];
+    out.z = D == 3 ? v[D - 1] : (T)0;
+    a.grid[slot] = out;
+    if constexpr (FRICTION) {
+      Node4<T> of;
+      of.m = nd.m;
+      of.x = vf[0];
+      of.y = vf[1];
+      of.z = D == 3 ? vf[D - 1] : (T)0;
+      a.gridf[slot] = of;
+    }
+    if (a.stats_partial) {
+      acc[0] += (double)nd.m;
+#pragma unroll
+      for (int c = 0; c < D; c++) acc[1 + c] += (double)nd.m * (double)v[c];
+    }
+  }
+  if (a.stats_partial) {
+    __shared__ double s_acc[BLOCK / 64][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      double x = acc[c];
+      for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+      if (lane == 0) s_acc[wv][c] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      double s = 0;
+      for (int q = 0; q < BLOCK / 64; q++) s += s_acc[q][threadIdx.x];
+      a.stats_partial[blockIdx.x * 4 + threadIdx.x] = s;
+    }
+  }
+}
+
+__global__ void reduce_partials_kernel(const double* partial, int nblocks, double* out) {
+  // one wave; fixed order per lane then a fixed shuffle tree
+  const int c = threadIdx.x & 3, lane = threadIdx.x;
+  double s = 0;
+  for (int b = lane >> 2; b < nblocks; b += 16) s += partial[b * 4 + c];
+  for (int o = 32; o >= 4; o >>= 1) s += __shfl_down(s, o, 64);
+  if (lane < 4) out[lane] = s;
+}
+
+// ------------------------------------------------------------------ G2P ----
+// Gathers v (and the friction velocity when walls have friction) from the
+// tile's stencil region staged in LDS, then updates F, x, v, B and the key.
+// B uses the algebraically equal, cancellation-free form of MPM_DRIVER.cpp:
+// 485, :504-505:  B = sum w vf (x_i - x_p)^T
+//                   + dt/2 [sum w (vf v^T - v vf^T) - (vf_p v_p^T - v_p vf_p^T)].
+template<typename T, int D, bool FRICTION>
+__global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
+  constexpr int R = Tile<D>::R, RN = Tile<D>::RN;
+  __shared__ T s_v[D][RN];
+  __shared__ T s_f[FRICTION ? D : 1][FRICTION ? RN : 1];
+  const uint32_t ntile = a.counts[0];
+  for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
+    const uint32_t t = a.ptiles[seg];
+    int org[D];
+    tile_origin<D>(t, a.g.GT, org);
+    for (int k = threadIdx.x; k < RN; k += BLOCK) {
+      int off[D], nc[D];
+      region_offsets<D>(k, off);
+#pragma unroll
+      for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
+      const uint32_t slot = node_slot<D>(nc, a.g.GT);
+      const Node4<T> nd = a.grid[slot];
+      s_v[0][k] = nd.x;
+      s_v[1][k] = nd.y;
+      if constexpr (D == 3) s_v[2][k] = nd.z;
+      if constexpr (FRICTION) {
+        const Node4<T> nf = a.gridf[slot];
+        s_f[0][k] = nf.x;
+        s_f[1][k] = nf.y;
+        if constexpr (D == 3) s_f[D - 1][k] = nf.z;
+      }
+    }
+    __syncthreads();
+    const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
+    for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
+      const uint32_t p = a.order ? a.order[i] : i;
+      T x[D], Fm[D * D];
+#pragma unroll
+      for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+#pragma unroll
+      for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
+      Wts<T, D> w;
+      weights<T, D>(a.g, x, w);
+      int lb[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) lb[c] = w.base[c] - org[c];
+      T dz[3][D];
+#pragma unroll
+      for (int o = 0; o < 3; o++)
+#pragma unroll
+        for (int c = 0; c < D; c++) dz[o][c] = (T)o * a.g.dx[c] - w.xe[c];
+      T vb[D], G[D * D], Bs[D * D], vfb[D], An[D * D];
+#pragma unroll
+      for (int c = 0; c < D; c++) { vb[c] = 0; vfb[c] = 0; }
+#pragma unroll
+      for (int k = 0; k < D * D; k++) { G[k] = 0; Bs[k] = 0; An[k] = 0; }
+      if constexpr (D == 3) {
+#pragma unroll
+        for (int ox = 0; ox < 3; ox++)
+#pragma unroll
+          for (int oy = 0; oy < 3; oy++) {
+            const T wxy = w.w[ox][0] * w.w[oy][1];
+#pragma unroll
+            for (int oz = 0; oz < 3; oz++) {
+              const T wt = wxy * w.w[oz][2];
+              T g[3];
+              g[0] = w.dw[ox][0] * w.w[oy][1] * w.w[oz][2];
+              g[1] = w.w[ox][0] * w.dw[oy][1] * w.w[oz][2];
+              g[2] = wxy * w.dw[oz][2];
+              const int li = ((lb[0] + ox) * R + (lb[1] + oy)) * R + (lb[2] + oz);
+              T vn[3], vf[3];
+#pragma unroll
+              for (int c = 0; c < 3; c++) {
+                vn[c] = s_v[c][li];
+                vf[c] = FRICTION ? s_f[FRICTION ? c : 0][FRICTION ? li : 0] : vn[c];
+              }
+              const T zz[3] = {dz[ox][0], dz[oy][1], dz[oz][2]};
+#pragma unroll
+              for (int c = 0; c < 3; c++) {
+                vb[c] += wt * vn[c];
+                if (FRICTION) vfb[c] += wt * vf[c];
+                const T wvf = wt * vf[c];
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                  G[c + 3 * j] += vn[c] * g[j];
+                  Bs[c + 3 * j] += wvf * zz[j];
+                  if (FRICTION) An[c + 3 * j] += wt * (vf[c] * vn[j] - vn[c] * vf[j]);
+                }
+              }
+            }
+          }
+      } else {
+#pragma unroll
+        for (int ox = 0; ox < 3; ox++)
+#pragma unroll
+          for (int oy = 0; oy < 3; oy++) {
+            const T wt = w.w[ox][0] * w.w[oy][1];
+            T g[2];
+            g[0] = w.dw[ox][0] * w.w[oy][1];
+            g[1] = w.w[ox][0] * w.dw[oy][1];
+            const int li = (lb[0] + ox) * R + (lb[1] + oy);
+            T vn[2], vf[2];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+              vn[c] = s_v[c][li];
+              vf[c] = FRICTION ? s_f[FRICTION ? c : 0][FRICTION ? li : 0] : vn[c];
+            }
+            const T zz[2] = {dz[ox][0], dz[oy][1]};
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+              vb[c] += wt * vn[c];
+              if (FRICTION) vfb[c] += wt * vf[c];
+              const T wvf = wt * vf[c];
+#pragma unroll
+              for (int j = 0; j < 2; j++) {
+                G[c + 2 * j] += vn[c] * g[j];
+                Bs[c + 2 * j] += wvf * zz[j];
+                if (FRICTION) An[c + 2 * j] += wt * (vf[c] * vn[j] - vn[c] * vf[j]);
+              }
+            }
+          }
+      }
+      if (!FRICTION) {
+#pragma unroll
+        for (int c = 0; c < D; c++) vfb[c] = vb[c];
+      } else {
+        const T h = (T)0.5 * a.dt;
+#pragma unroll
+        for (int c = 0; c < D; c++)
+#pragma unroll
+          for (int j = 0; j < D; j++) Bs[c + D * j] += h * (An[c + D * j] - (vfb[c] * vb[j] - vb[c] * vfb[j]));
+      }
+      // F <- (I + dt grad v) F
+      T Fn[D * D];
+#pragma unroll
+      for (int j = 0; j < D; j++)
+#pragma unroll
+        for (int c = 0; c < D; c++) {
+          T s = Fm[c + D * j];
+#pragma unroll
+          for (int q = 0; q < D; q++) s += a.dt * G[c + D * q] * Fm[q + D * j];
+          Fn[c + D * j] = s;
+        }
+      T xn[D];
+      bool ok = true;
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        xn[c] = x[c] + a.dt * vb[c];
+        if (xn[c] < a.vlo[c] || xn[c] > a.vhi[c]) ok = false;
+      }
+#pragma unroll
+      for (int c = 0; c < D; c++) { a.X[c * a.n + p] = xn[c]; a.V[c * a.n + p] = vfb[c]; }
+#pragma unroll
+      for (int k = 0; k < D * D; k++) { a.F[k * a.n + p] = Fn[k]; a.B[k * a.n + p] = Bs[k]; }
+      if (!ok) a.valid[p] = 0;
+      a.keys[p] = ok ? tile_key<T, D>(a.g, xn) : INVALID_KEY;
+    }
+    __syncthreads();
+  }
+}
+
+// -------------------------------------------------------------- reorder ----
+template<typename T, int D> struct ParticleBufs {
+  T* X; T* V; T* F; T* B; T* mass; T* vol; T* mu; T* la;
+  uint8_t* valid;
+  uint32_t* pid;
+};
+
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void reorder_kernel(ParticleBufs<T, D> src, ParticleBufs<T, D> dst,
+                                                        const uint32_t* order, uint32_t n) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = order[i];
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    dst.X[c * n + i] = src.X[c * n + p];
+    dst.V[c * n + i] = src.V[c * n + p];
+  }
+#pragma unroll
+  for (int k = 0; k < D * D; k++) {
+    dst.F[k * n + i] = src.F[k * n + p];
+    dst.B[k * n + i] = src.B[k * n + p];
+  }
+  dst.mass[i] = src.mass[p];
+  dst.vol[i] = src.vol[p];
+  dst.mu[i] = src.mu[p];
+  dst.la[i] = src.la[p];
+  dst.valid[i] = src.valid[p];
+  dst.pid[i] = src.pid[p];
+}
+
+// Host layout (per particle contiguous, user order) <-> device SoA (slot order).
+template<typename T>
+__global__ __launch_bounds__(BLOCK) void soa_from_aos_kernel(const T* aos, T* soa, uint32_t n, int comps) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < comps; c++) soa[(size_t)c * n + i] = aos[(size_t)i * comps + c];
+}
+
+template<typename T>
+__global__ __launch_bounds__(BLOCK) void aos_from_soa_kernel(const T* soa, T* aos, const uint32_t* pid, uint32_t n,
+                                                             int comps) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t q = pid[i];
+  for (int c = 0; c < comps; c++) aos[(size_t)q * comps + c] = soa[(size_t)c * n + i];
+}
+
+__global__ __launch_bounds__(BLOCK) void bytes_from_slots_kernel(const uint8_t* src, uint8_t* dst, const uint32_t* pid,
+                                                                 uint32_t n) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) dst[pid[i]] = src[i];
+}
+
+__global__ __launch_bounds__(BLOCK) void iota_kernel(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
+// Dense export of the active tiles, MPM_EXAMPLE::mass / ::velocity layout.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void export_grid_kernel(Args<T, D> a, T* dmass, T* dvel) {
+  const uint64_t total = (uint64_t)a.counts[1] * 64u;
+  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total; idx += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t t = a.gtiles[idx >> 6];
+    const uint32_t l = (uint32_t)(idx & 63u);
+    int org[D], nc[D];
+    tile_origin<D>(t, a.g.GT, org);
+    uint32_t r = l;
+    bool in = true;
+#pragma unroll
+    for (int c = D - 1; c >= 0; c--) {
+      nc[c] = org[c] + (int)(r % Tile<D>::E);
+      r /= Tile<D>::E;
+      in = in && nc[c] < a.g.G[c];
+    }
+    if (!in) continue;
+    const Node4<T> nd = a.grid[t * 64u + l];
+    if (nd.m == (T)0) continue;
+    uint64_t f = 0;
+#pragma unroll
+    for (int c = 0; c < D; c++) f = f * (uint64_t)a.g.G[c] + (uint64_t)nc[c];
+    dmass[f] = nd.m;
+    dvel[f * D + 0] = nd.x;
+    dvel[f * D + 1] = nd.y;
+    if constexpr (D == 3) dvel[f * D + 2] = nd.z;
+  }
+}
+
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void particle_stats_kernel(const T* V, const T* mass, const uint8_t* valid,
+                                                               uint32_t n, double* partial) {
+  double acc[4] = {0, 0, 0, 0};
+  double cnt = 0;
+  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < n; p += gridDim.x * BLOCK) {
+    if (!valid[p]) continue;
+    const double m = (double)mass[p];
+    acc[0] += m;
+#pragma unroll
+    for (int c = 0; c < D; c++) acc[1 + c] += m * (double)V[(size_t)c * n + p];
+    cnt += 1;
+  }
+  __shared__ double s[BLOCK / 64][5];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double vals[5] = {acc[0], acc[1], acc[2], acc[3], cnt};
+#pragma unroll
+  for (int c = 0; c < 5; c++) {
+    double x = vals[c];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    if (lane == 0) s[wv][c] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double t = 0;
+    for (int q = 0; q < BLOCK / 64; q++) t += s[q][threadIdx.x];
+    partial[blockIdx.x * 5 + threadIdx.x] = t;
+  }
+}
+
+// Stand-alone kernels for component parity tests.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void stress_kernel(uint32_t n, const T* F, const T* vol, const T* mu, const T* la,
+                                                       T* tau, int model, T thr, T root) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  T Fm[D * D], t[D * D];
+#pragma unroll
+  for (int k = 0; k < D * D; k++) Fm[k] = F[(size_t)i * D * D + k];
+  kirchhoff<T, D>(model, mu[i], la[i], thr, root, vol[i], Fm, t);
+#pragma unroll
+  for (int k = 0; k < D * D; k++) tau[(size_t)i * D * D + k] = t[k];
+}
+
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void svd_kernel(uint32_t n, const T* F, T* U, T* S, T* V) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  T Fm[D * D], u[D * D], s[D], v[D * D];
+#pragma unroll
+  for (int k = 0; k < D * D; k++) Fm[k] = F[(size_t)i * D * D + k];
+  svd<T, D>(Fm, u, s, v);
+#pragma unroll
+  for (int k = 0; k < D * D; k++) { U[(size_t)i * D * D + k] = u[k]; V[(size_t)i * D * D + k] = v[k]; }
+#pragma unroll
+  for (int k = 0; k < D; k++) S[(size_t)i * D + k] = s[k];
+}
+
+// ------------------------------------------------------------------ host ---
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+template<typename X> struct DevBuf {
+  X* p = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), count(o.count) { o.p = nullptr; o.count = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      count = o.count;
+      o.p = nullptr;
+      o.count = 0;
+    }
+    return *this;
+  }
+  void alloc(size_t n) {
+    if (n <= count && p) return;
+    release();
+    if (n == 0) return;
+    HIPCHECK(hipMalloc((void**)&p, n * sizeof(X)));
+    count = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    count = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+struct PhaseTimer {
+  struct Pending {
+    int phase;
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> pool;
+  std::vector<Pending> pending;
+  double total[6] = {0, 0, 0, 0, 0, 0};
+  int64_t count[6] = {0, 0, 0, 0, 0, 0};
+  bool on = false;
+  hipEvent_t get() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      HIPCHECK(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t begin(hipStream_t s) {
+    if (!on) return nullptr;
+    hipEvent_t e = get();
+    HIPCHECK(hipEventRecord(e, s));
+    return e;
+  }
+  void end(int phase, hipEvent_t a, hipStream_t s) {
+    if (!on || !a) return;
+    hipEvent_t b = get();
+    HIPCHECK(hipEventRecord(b, s));
+    pending.push_back({phase, a, b});
+  }
+  void resolve() {
+    for (auto& q : pending) {
+      HIPCHECK(hipEventSynchronize(q.b));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, q.a, q.b));
+      total[q.phase] += ms;
+      count[q.phase] += 1;
+      pool.push_back(q.a);
+      pool.push_back(q.b);
+    }
+    pending.clear();
+  }
+  ~PhaseTimer() {
+    for (auto& q : pending) {
+      (void)hipEventDestroy(q.a);
+      (void)hipEventDestroy(q.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+enum { PH_BIN = 0, PH_P2G, PH_GRID, PH_G2P, PH_REORDER, PH_STEP };
+static const char* kPhaseNames[6] = {"bin", "p2g", "grid", "g2p", "reorder", "step"};
+
+template<typename T, int D>
+class Engine final : public EngineBase {
+ public:
+  explicit Engine(const mpmx_grid_desc& d) {
+    for (int a = 0; a < D; a++) {
+      if (d.cells[a] < 1 || d.cells[a] > (1 << 14)) throw Error{MPMX_EINVAL, "cells out of range"};
+      if (!(d.hi[a] > d.lo[a])) throw Error{MPMX_EINVAL, "empty domain"};
+      g_.lo[a] = (T)d.lo[a];
+      g_.hi[a] = (T)d.hi[a];
+      g_.dx[a] = (g_.hi[a] - g_.lo[a]) / (T)d.cells[a];  // GRID.cpp:23-24
+      g_.inv_dx[a] = (T)1 / g_.dx[a];
+      g_.G[a] = d.cells[a] + 6;
+      g_.GT[a] = (g_.G[a] + Tile<D>::E - 1) / Tile<D>::E + 1;
+      cells_[a] = d.cells[a];
+      // Reflect_Or_Invalidate_Particle bounds: eps = 4 machine-eps * (B - A)
+      const T eps = std::numeric_limits<T>::epsilon() * (T)4 * (g_.hi[a] - g_.lo[a]);
+      vlo_[a] = g_.lo[a] + eps;
+      vhi_[a] = g_.hi[a] - eps;
+    }
+    uint64_t nt = 1;
+    for (int a = 0; a < D; a++) nt *= (uint64_t)g_.GT[a];
+    if (nt >= 0xffffffffull / 64) throw Error{MPMX_EINVAL, "grid too large"};
+    g_.ntiles = (int)nt;
+    key_bits_ = 1;
+    while ((1ull << key_bits_) < nt + 1) key_bits_++;
+    for (int a = 0; a < D; a++) grav_[a] = 0;
+    HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    tile_begin_.alloc(nt);
+    tile_end_.alloc(nt);
+    tile_stamp_.alloc(nt);
+    pflag_.alloc(nt);
+    gflag_.alloc(nt);
+    pscan_.alloc(nt);
+    gscan_.alloc(nt);
+    ptiles_.alloc(nt);
+    gtiles_.alloc(nt);
+    counts_.alloc(8);
+    err_.alloc(4);
+    grid_.alloc(nt * 64);
+    HIPCHECK(hipMemsetAsync(tile_stamp_.p, 0, nt * sizeof(uint32_t), stream_));
+    HIPCHECK(hipMemsetAsync(grid_.p, 0, nt * 64 * sizeof(Node4<T>), stream_));
+    HIPCHECK(hipMemsetAsync(counts_.p, 0, 8 * sizeof(uint32_t), stream_));
+    HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
+    scan_scratch_.alloc(scan_scratch_bytes((uint32_t)nt) + 64);
+    stats_blocks_ = (int)std::min<uint64_t>(std::max<uint64_t>(cdiv(nt * 64, BLOCK), 1), 2048);
+    stats_partial_.alloc((size_t)stats_blocks_ * 5 + 8);
+    stats_out_.alloc(8);
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+  ~Engine() override {
+    if (stream_) {
+      (void)hipStreamSynchronize(stream_);
+      (void)hipStreamDestroy(stream_);
+    }
+  }
+
+  int dim() const override { return D; }
+  int precision() const override { return (int)sizeof(T); }
+  int64_t particle_count() const override { return n_; }
+  double time() const override { return time_; }
+
+  void set_particles(int64_t n, const void* X, const void* V, const void* F, const void* B, const void* mass,
+                     const void* volume, const void* mu, const void* lambda, const uint8_t* valid) override {
+    if (n < 0 || n > (int64_t)0x7fffffff) throw Error{MPMX_EINVAL, "particle count out of range"};
+    if (n > 0 && (!X || !V || !F || !B || !mass || !volume || !mu || !lambda || !valid))
+      throw Error{MPMX_EINVAL, "null particle array"};
+    n_ = (uint32_t)n;
+    allocate_particles();
+    if (n_ > 0) {
+      upload_soa((const T*)X, cur_.X, D);
+      upload_soa((const T*)V, cur_.V, D);
+      upload_soa((const T*)F, cur_.F, D * D);
+      upload_soa((const T*)B, cur_.B, D * D);
+      upload_soa((const T*)mass, cur_.mass, 1);
+      upload_soa((const T*)volume, cur_.vol, 1);
+      upload_soa((const T*)mu, cur_.mu, 1);
+      upload_soa((const T*)lambda, cur_.la, 1);
+      HIPCHECK(hipMemcpyAsync(cur_.valid, valid, n_, hipMemcpyHostToDevice, stream_));
+      iota_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_.pid, n_);
+      HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
+      Args<T, D> a = make_args(0);
+      key_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(a, err_.p + 1);
+      uint32_t errs[4];
+      HIPCHECK(hipMemcpyAsync(errs, err_.p, sizeof(errs), hipMemcpyDeviceToHost, stream_));
+      HIPCHECK(hipStreamSynchronize(stream_));
+      if (errs[1]) {
+        n_ = 0;
+        throw Error{MPMX_ERANGE, std::to_string(errs[1]) + " valid particle(s) outside the ghosted grid"};
+      }
+    }
+    keys_valid_ = true;
+    force_reorder_ = true;
+    has_grid_ = false;
+  }
+
+  void get_particles(void* X, void* V, void* F, void* B, uint8_t* valid) override {
+    if (n_ == 0) return;
+    download_aos(cur_.X, (T*)X, D);
+    download_aos(cur_.V, (T*)V, D);
+    download_aos(cur_.F, (T*)F, D * D);
+    download_aos(cur_.B, (T*)B, D * D);
+    if (valid) {
+      stage_bytes_.alloc(n_);
+      bytes_from_slots_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_.valid, stage_bytes_.p, cur_.pid, n_);
+      HIPCHECK(hipMemcpyAsync(valid, stage_bytes_.p, n_, hipMemcpyDeviceToHost, stream_));
+    }
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+
+  void set_constitutive(int model, double thr) override {
+    if (model != 0 && model != 1) throw Error{MPMX_EINVAL, "unknown constitutive model"};
+    if (model == 1 && !(thr > 0)) throw Error{MPMX_EINVAL, "neo-Hookean threshold must be positive"};
+    model_ = model;
+    thr_ = (T)thr;
+    root_ = D == 2 ? (T)std::sqrt(thr) : (T)std::cbrt(thr);  // pow<1,d>(failure_threshold)
+  }
+  void set_gravity(const double* g) override {
+    if (!g) throw Error{MPMX_EINVAL, "null gravity"};
+    for (int a = 0; a < D; a++) grav_[a] = (T)g[a];
+  }
+  void add_collision_box(int type, double friction, const double* lo, const double* hi) override {
+    if (type < 0 || type > 2) throw Error{MPMX_EINVAL, "unknown collision type"};
+    if (!lo || !hi) throw Error{MPMX_EINVAL, "null box corner"};
+    if (nwalls_ >= MAX_WALLS) throw Error{MPMX_EINVAL, "too many collision objects (max 16)"};
+    WallDev<T> w{};
+    w.type = type;
+    w.friction = (T)friction;
+    for (int a = 0; a < 3; a++) {
+      w.lo[a] = a < D ? (T)lo[a] : 0;
+      w.hi[a] = a < D ? (T)hi[a] : 0;
+    }
+    for (int a = 0; a < D; a++)
+      if (!(w.hi[a] > w.lo[a])) throw Error{MPMX_EINVAL, "degenerate collision box"};
+    walls_[nwalls_++] = w;
+  }
+  void clear_collision_objects() override { nwalls_ = 0; }
+
+  void step(double dt, int nsteps, bool wait) override {
+    if (!(dt > 0)) throw Error{MPMX_EINVAL, "dt must be positive"};
+    if (nsteps < 0) throw Error{MPMX_EINVAL, "negative step count"};
+    for (int s = 0; s < nsteps; s++) step_once((T)dt);
+    if (wait) synchronize();
+  }
+
+  void synchronize() override {
+    HIPCHECK(hipStreamSynchronize(stream_));
+    timer_.resolve();
+    uint32_t errs[4];
+    HIPCHECK(hipMemcpy(errs, err_.p, sizeof(errs), hipMemcpyDeviceToHost));
+    if (errs[0]) throw Error{MPMX_EINTERNAL, "particle outside its binned tile (binning inconsistency)"};
+  }
+
+  void enable_grid_stats(bool on) override { stats_on_ = on; }
+  void grid_stats(double* mass, double* momentum) override {
+    if (!stats_on_ || !has_grid_) throw Error{MPMX_ESTATE, "grid stats need enable_grid_stats and a step"};
+    double out[4];
+    HIPCHECK(hipStreamSynchronize(stream_));
+    HIPCHECK(hipMemcpy(out, stats_out_.p, sizeof(out), hipMemcpyDeviceToHost));
+    if (mass) *mass = out[0];
+    if (momentum)
+      for (int a = 0; a < D; a++) momentum[a] = out[1 + a];
+  }
+  void particle_stats(double* mass, double* momentum, int64_t* valid_count) override {
+    double out[5] = {0, 0, 0, 0, 0};
+    if (n_ > 0) {
+      const int nb = (int)std::min<uint32_t>(cdiv(n_, BLOCK), 1024);
+      DevBuf<double> part;
+      part.alloc((size_t)nb * 5);
+      particle_stats_kernel<T, D><<<nb, BLOCK, 0, stream_>>>(cur_.V, cur_.mass, cur_.valid, n_, part.p);
+      std::vector<double> h((size_t)nb * 5);
+      HIPCHECK(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+      HIPCHECK(hipStreamSynchronize(stream_));
+      for (int b = 0; b < nb; b++)
+        for (int c = 0; c < 5; c++) out[c] += h[(size_t)b * 5 + c];
+    }
+    if (mass) *mass = out[0];
+    if (momentum)
+      for (int a = 0; a < D; a++) momentum[a] = out[1 + a];
+    if (valid_count) *valid_count = (int64_t)out[4];
+  }
+
+  int64_t grid_node_count() const override {
+    int64_t c = 1;
+    for (int a = 0; a < D; a++) c *= g_.G[a];
+    return c;
+  }
+  void get_grid(void* mass, void* velocity) override {
+    if (!has_grid_) throw Error{MPMX_ESTATE, "no grid yet: call mpmx_step first"};
+    const int64_t nn = grid_node_count();
+    DevBuf<T> dm, dv;
+    dm.alloc(nn);
+    dv.alloc(nn * D);
+    HIPCHECK(hipMemsetAsync(dm.p, 0, nn * sizeof(T), stream_));
+    HIPCHECK(hipMemsetAsync(dv.p, 0, nn * D * sizeof(T), stream_));
+    Args<T, D> a = make_args(0);
+    export_grid_kernel<T, D><<<grid_blocks(), BLOCK, 0, stream_>>>(a, dm.p, dv.p);
+    if (mass) HIPCHECK(hipMemcpyAsync(mass, dm.p, nn * sizeof(T), hipMemcpyDeviceToHost, stream_));
+    if (velocity) HIPCHECK(hipMemcpyAsync(velocity, dv.p, nn * D * sizeof(T), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+
+  void set_reorder_interval(int k) override {
+    if (k < 1) throw Error{MPMX_EINVAL, "reorder interval must be >= 1"};
+    reorder_interval_ = k;
+  }
+  void enable_timing(bool on) override {
+    synchronize();
+    timer_.on = on;
+  }
+  void reset_timing() override {
+    synchronize();
+    for (int i = 0; i < 6; i++) {
+      timer_.total[i] = 0;
+      timer_.count[i] = 0;
+    }
+  }
+  bool phase_time(const char* phase, double* total_ms, int64_t* count) override {
+    synchronize();
+    for (int i = 0; i < 6; i++)
+      if (phase && std::strcmp(phase, kPhaseNames[i]) == 0) {
+        if (total_ms) *total_ms = timer_.total[i];
+        if (count) *count = timer_.count[i];
+        return true;
+      }
+    return false;
+  }
+
+  void stress(int64_t n, const void* F, const void* volume, const void* mu, const void* lambda, void* tau) override {
+    if (n <= 0) return;
+    if (n > 0x7fffffff) throw Error{MPMX_EINVAL, "too many matrices"};
+    DevBuf<T> dF, dv, dm, dl, dt;
+    dF.alloc(n * D * D); dv.alloc(n); dm.alloc(n); dl.alloc(n); dt.alloc(n * D * D);
+    HIPCHECK(hipMemcpyAsync(dF.p, F, n * D * D * sizeof(T), hipMemcpyHostToDevice, stream_));
+    HIPCHECK(hipMemcpyAsync(dv.p, volume, n * sizeof(T), hipMemcpyHostToDevice, stream_));
+    HIPCHECK(hipMemcpyAsync(dm.p, mu, n * sizeof(T), hipMemcpyHostToDevice, stream_));
+    HIPCHECK(hipMemcpyAsync(dl.p, lambda, n * sizeof(T), hipMemcpyHostToDevice, stream_));
+    stress_kernel<T, D><<<cdiv(n, BLOCK), BLOCK, 0, stream_>>>((uint32_t)n, dF.p, dv.p, dm.p, dl.p, dt.p, model_,
+                                                                thr_, root_);
+    HIPCHECK(hipMemcpyAsync(tau, dt.p, n * D * D * sizeof(T), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+  void svd(int64_t n, const void* F, void* U, void* s, void* V) override {
+    if (n <= 0) return;
+    if (n > 0x7fffffff) throw Error{MPMX_EINVAL, "too many matrices"};
+    DevBuf<T> dF, dU, dS, dV;
+    dF.alloc(n * D * D); dU.alloc(n * D * D); dS.alloc(n * D); dV.alloc(n * D * D);
+    HIPCHECK(hipMemcpyAsync(dF.p, F, n * D * D * sizeof(T), hipMemcpyHostToDevice, stream_));
+    svd_kernel<T, D><<<cdiv(n, BLOCK), BLOCK, 0, stream_>>>((uint32_t)n, dF.p, dU.p, dS.p, dV.p);
+    HIPCHECK(hipMemcpyAsync(U, dU.p, n * D * D * sizeof(T), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipMemcpyAsync(s, dS.p, n * D * sizeof(T), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipMemcpyAsync(V, dV.p, n * D * D * sizeof(T), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+  void sort_keys(int64_t n, const uint32_t* keys, int key_bits, uint32_t* keys_out, uint32_t* order_out) override {
+    if (n <= 0) return;
+    if (n > 0x7fffffff) throw Error{MPMX_EINVAL, "too many keys"};
+    if (key_bits < 1 || key_bits > 32) throw Error{MPMX_EINVAL, "key_bits must be in [1, 32]"};
+    DevBuf<uint32_t> k, ko, vo, kt, vt;
+    DevBuf<char> scratch;
+    k.alloc(n); ko.alloc(n); vo.alloc(n); kt.alloc(n); vt.alloc(n);
+    scratch.alloc(sort_scratch_bytes((uint32_t)n, key_bits));
+    HIPCHECK(hipMemcpyAsync(k.p, keys, n * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+    radix_sort_pairs(k.p, ko.p, vo.p, kt.p, vt.p, (uint32_t)n, key_bits, scratch.p, stream_);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(keys_out, ko.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipMemcpyAsync(order_out, vo.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+
+ private:
+  void allocate_particles() {
+    auto alloc_set = [&](ParticleBufs<T, D>& pb, std::vector<DevBuf<T>>& fb, DevBuf<uint8_t>& vb,
+                         DevBuf<uint32_t>& ib) {
+      fb.resize(8);
+      const size_t n = std::max<uint32_t>(n_, 1);
+      fb[0].alloc(n * D); fb[1].alloc(n * D); fb[2].alloc(n * D * D); fb[3].alloc(n * D * D);
+      fb[4].alloc(n); fb[5].alloc(n); fb[6].alloc(n); fb[7].alloc(n);
+      vb.alloc(n);
+      ib.alloc(n);
+      pb = ParticleBufs<T, D>{fb[0].p, fb[1].p, fb[2].p, fb[3].p, fb[4].p, fb[5].p, fb[6].p, fb[7].p, vb.p, ib.p};
+    };
+    // Buffers are sized exactly to n so that the SoA stride equals n.
+    for (auto* fbv : {&fa_, &fb_})
+      for (auto& b : *fbv) b.release();
+    va_.release(); vb_.release(); ia_.release(); ib_.release();
+    alloc_set(cur_, fa_, va_, ia_);
+    alloc_set(alt_, fb_, vb_, ib_);
+    const size_t n = std::max<uint32_t>(n_, 1);
+    keys_.release(); skeys_.release(); order_.release(); ktmp_.release(); vtmp_.release();
+    keys_.alloc(n); skeys_.alloc(n); order_.alloc(n); ktmp_.alloc(n); vtmp_.alloc(n);
+    sort_scratch_.release();
+    sort_scratch_.alloc(sort_scratch_bytes((uint32_t)n, key_bits_));
+    stage_.release();
+    stage_.alloc(n * D * D);
+  }
+
+  void upload_soa(const T* host, T* dev, int comps) {
+    HIPCHECK(hipMemcpyAsync(stage_.p, host, (size_t)n_ * comps * sizeof(T), hipMemcpyHostToDevice, stream_));
+    soa_from_aos_kernel<T><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(stage_.p, dev, n_, comps);
+    // the staging buffer is reused: wait before the next upload overwrites it
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+  void download_aos(const T* dev, T* host, int comps) {
+    if (!host) return;
+    aos_from_soa_kernel<T><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(dev, stage_.p, cur_.pid, n_, comps);
+    HIPCHECK(hipMemcpyAsync(host, stage_.p, (size_t)n_ * comps * sizeof(T), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+
+  Args<T, D> make_args(T dt) {
+    Args<T, D> a{};
+    a.g = g_;
+    for (int c = 0; c < D; c++) {
+      a.vlo[c] = vlo_[c];
+      a.vhi[c] = vhi_[c];
+      a.grav[c] = grav_[c];
+    }
+    a.dt = dt;
+    a.model = model_;
+    a.thr = thr_;
+    a.root = root_;
+    a.nwalls = nwalls_;
+    for (int i = 0; i < nwalls_; i++) a.walls[i] = walls_[i];
+    a.X = cur_.X; a.V = cur_.V; a.F = cur_.F; a.B = cur_.B;
+    a.mass = cur_.mass; a.vol = cur_.vol; a.mu = cur_.mu; a.la = cur_.la;
+    a.valid = cur_.valid;
+    a.keys = keys_.p;
+    a.n = n_;
+    a.order = order_.p;
+    a.tile_begin = tile_begin_.p;
+    a.tile_end = tile_end_.p;
+    a.ptiles = ptiles_.p;
+    a.gtiles = gtiles_.p;
+    a.counts = counts_.p;
+    a.grid = grid_.p;
+    a.gridf = gridf_.p;
+    a.err = err_.p;
+    a.stats_partial = nullptr;
+    return a;
+  }
+
+  bool any_friction() const {
+    for (int i = 0; i < nwalls_; i++)
+      if (walls_[i].friction != (T)0 && walls_[i].type != 0) return true;
+    return false;
+  }
+  uint32_t tile_blocks() const { return (uint32_t)std::min<int>(std::max(g_.ntiles, 1), 8192); }
+  uint32_t grid_blocks() const { return (uint32_t)stats_blocks_; }
+
+  void step_once(T dt) {
+    const bool fric = any_friction();
+    if (fric) gridf_.alloc((size_t)g_.ntiles * 64);
+    hipEvent_t e_step = timer_.begin(stream_);
+    if (n_ == 0) {
+      time_ += (double)dt;
+      timer_.end(PH_STEP, e_step, stream_);
+      return;
+    }
+    hipEvent_t e0 = timer_.begin(stream_);
+    Args<T, D> a = make_args(dt);
+    if (!keys_valid_) {
+      key_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(a, err_.p + 1);
+      keys_valid_ = true;
+    }
+    radix_sort_pairs(keys_.p, skeys_.p, order_.p, ktmp_.p, vtmp_.p, n_, key_bits_, sort_scratch_.p, stream_);
+    stamp_++;
+    fill_u32_kernel<<<1, BLOCK, 0, stream_>>>(counts_.p + 2, 1, 0u);
+    bounds_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(skeys_.p, n_, tile_begin_.p, tile_end_.p, tile_stamp_.p,
+                                                          stamp_, counts_.p);
+    timer_.end(PH_BIN, e0, stream_);
+    const uint32_t* order = order_.p;
+    if (force_reorder_ || ++since_reorder_ >= reorder_interval_) {
+      hipEvent_t er = timer_.begin(stream_);
+      reorder_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_, alt_, order_.p, n_);
+      std::swap(cur_, alt_);
+      std::swap(fa_, fb_);
+      std::swap(va_, vb_);
+      std::swap(ia_, ib_);
+      std::swap(keys_, skeys_);
+      order = nullptr;
+      since_reorder_ = 0;
+      force_reorder_ = false;
+      timer_.end(PH_REORDER, er, stream_);
+    }
+    e0 = timer_.begin(stream_);
+    const uint32_t nt = (uint32_t)g_.ntiles;
+    flags_kernel<D><<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(tile_stamp_.p, stamp_, nt, g_.GT[0], D > 1 ? g_.GT[1] : 1,
+                                                            D > 2 ? g_.GT[D - 1] : 1, pflag_.p, gflag_.p);
+    scan_exclusive(pflag_.p, pscan_.p, nt, counts_.p + 0, scan_scratch_.p, stream_);
+    scan_exclusive(gflag_.p, gscan_.p, nt, counts_.p + 1, scan_scratch_.p, stream_);
+    lists_kernel<<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(nt, pflag_.p, pscan_.p, gflag_.p, gscan_.p, ptiles_.p,
+                                                         gtiles_.p);
+    if (fric)
+      zero_kernel<T, true><<<grid_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, grid_.p, gridf_.p);
+    else
+      zero_kernel<T, false><<<grid_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, grid_.p, gridf_.p);
+    timer_.end(PH_BIN, e0, stream_);
+
+    a = make_args(dt);
+    a.order = order;
+    hipEvent_t e1 = timer_.begin(stream_);
+    p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+    timer_.end(PH_P2G, e1, stream_);
+
+    hipEvent_t e2 = timer_.begin(stream_);
+    a.stats_partial = stats_on_ ? stats_partial_.p : nullptr;
+    if (fric)
+      grid_kernel<T, D, true><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
+    else
+      grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
+    if (stats_on_) reduce_partials_kernel<<<1, 64, 0, stream_>>>(stats_partial_.p, grid_blocks(), stats_out_.p);
+    a.stats_partial = nullptr;
+    timer_.end(PH_GRID, e2, stream_);
+
+    hipEvent_t e3 = timer_.begin(stream_);
+    if (fric)
+      g2p_kernel<T, D, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+    else
+      g2p_kernel<T, D, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+    timer_.end(PH_G2P, e3, stream_);
+    HIPCHECK(hipGetLastError());
+    has_grid_ = true;
+    time_ += (double)dt;
+    timer_.end(PH_STEP, e_step, stream_);
+  }
+
+  Geom<T, D> g_{};
+  int cells_[D];
+  T vlo_[D], vhi_[D];
+  T grav_[D];
+  int model_ = 0;
+  T thr_ = (T)0.25, root_ = (T)(D == 2 ? 0.5 : 0.6299605249474366);
+  int nwalls_ = 0;
+  WallDev<T> walls_[MAX_WALLS];
+  int key_bits_ = 1;
+  hipStream_t stream_ = nullptr;
+  uint32_t n_ = 0;
+  uint32_t stamp_ = 0;
+  int since_reorder_ = 0;
+  int reorder_interval_ = 16;
+  bool force_reorder_ = true;
+  bool keys_valid_ = false;
+  bool has_grid_ = false;
+  bool stats_on_ = false;
+  double time_ = 0;
+  int stats_blocks_ = 1;
+  ParticleBufs<T, D> cur_{}, alt_{};
+  std::vector<DevBuf<T>> fa_, fb_;
+  DevBuf<uint8_t> va_, vb_, stage_bytes_;
+  DevBuf<uint32_t> ia_, ib_;
+  DevBuf<uint32_t> keys_, skeys_, order_, ktmp_, vtmp_;
+  DevBuf<char> sort_scratch_, scan_scratch_;
+  DevBuf<uint32_t> tile_begin_, tile_end_, tile_stamp_, pflag_, gflag_, pscan_, gscan_, ptiles_, gtiles_, counts_, err_;
+  DevBuf<Node4<T>> grid_, gridf_;
+  DevBuf<double> stats_partial_, stats_out_;
+  DevBuf<T> stage_;
+  PhaseTimer timer_;
+};
+
+EngineBase* make_engine(const mpmx_grid_desc& d) {
+  if (d.dim == 3 && d.precision == MPMX_FP32) return new Engine<float, 3>(d);
+  if (d.dim == 3 && d.precision == MPMX_FP64) return new Engine<double, 3>(d);
+  if (d.dim == 2 && d.precision == MPMX_FP32) return new Engine<float, 2>(d);
+  if (d.dim == 2 && d.precision == MPMX_FP64) return new Engine<double, 2>(d);
+  throw Error{MPMX_EINVAL, "dim must be 2 or 3 and precision MPMX_FP32 or MPMX_FP64"};
+}
+
+}  // namespace mpmx

@@ -1,0 +1,312 @@
+"""ctypes binding of libmpmx.so (include/mpmx.h and include/mpmx_scene.h).
+
+The binding is thin: every call goes to the C ABI and raises MpmxError on a
+non-zero status.  There is no CPU fallback; importing works without a GPU,
+but creating a context needs the HIP device.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmpmx.so")
+
+FP32, FP64 = 4, 8
+FIXED_COROTATED, NEO_HOOKEAN = 0, 1
+STICK, SLIP, SEPARATE = 0, 1, 2
+SHAPE_BOX, SHAPE_SPHERE = 0, 1
+
+# Every exported symbol of the C ABI, checked by the CPU test suite.
+EXPORTS = (
+    "mpmx_last_error", "mpmx_create", "mpmx_destroy", "mpmx_set_particles", "mpmx_get_particles",
+    "mpmx_particle_count", "mpmx_set_constitutive", "mpmx_set_gravity", "mpmx_add_collision_box",
+    "mpmx_clear_collision_objects", "mpmx_step", "mpmx_step_async", "mpmx_synchronize", "mpmx_time",
+    "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_grid_node_count",
+    "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_enable_timing", "mpmx_reset_timing",
+    "mpmx_phase_time", "mpmx_stress", "mpmx_svd", "mpmx_sort_keys", "mpmx_seed_particles",
+    "mpmx_random_boxes",
+)
+
+
+class MpmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mpmx error {code}: {msg}")
+        self.code = code
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("precision", ctypes.c_int), ("cells", ctypes.c_int * 3),
+                ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3)]
+
+
+class SeedDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("cells", ctypes.c_int * 3), ("lo", ctypes.c_double * 3),
+                ("hi", ctypes.c_double * 3), ("ppc_axis", ctypes.c_int), ("seed", ctypes.c_uint32),
+                ("jitter", ctypes.c_int), ("nshapes", ctypes.c_int),
+                ("shape_type", ctypes.c_void_p), ("shape_a", ctypes.c_void_p), ("shape_b", ctypes.c_void_p),
+                ("density", ctypes.c_void_p), ("youngs", ctypes.c_void_p), ("poisson", ctypes.c_void_p),
+                ("velocity", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIB_PATH):
+            raise MpmxError(-5, f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, c_int, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        L.mpmx_last_error.restype = ctypes.c_char_p
+        L.mpmx_create.argtypes = [ctypes.POINTER(GridDesc), ctypes.POINTER(vp)]
+        L.mpmx_destroy.argtypes = [vp]
+        L.mpmx_set_particles.argtypes = [vp, i64] + [vp] * 9
+        L.mpmx_get_particles.argtypes = [vp] + [vp] * 5
+        L.mpmx_particle_count.argtypes = [vp]
+        L.mpmx_particle_count.restype = i64
+        L.mpmx_set_constitutive.argtypes = [vp, c_int, dbl]
+        L.mpmx_set_gravity.argtypes = [vp, vp]
+        L.mpmx_add_collision_box.argtypes = [vp, c_int, dbl, vp, vp]
+        L.mpmx_clear_collision_objects.argtypes = [vp]
+        L.mpmx_step.argtypes = [vp, dbl, c_int]
+        L.mpmx_step_async.argtypes = [vp, dbl, c_int]
+        L.mpmx_synchronize.argtypes = [vp]
+        L.mpmx_time.argtypes = [vp]
+        L.mpmx_time.restype = dbl
+        L.mpmx_enable_grid_stats.argtypes = [vp, c_int]
+        L.mpmx_grid_stats.argtypes = [vp, vp, vp]
+        L.mpmx_particle_stats.argtypes = [vp, vp, vp, vp]
+        L.mpmx_grid_node_count.argtypes = [vp]
+        L.mpmx_grid_node_count.restype = i64
+        L.mpmx_get_grid.argtypes = [vp, vp, vp]
+        L.mpmx_set_reorder_interval.argtypes = [vp, c_int]
+        L.mpmx_enable_timing.argtypes = [vp, c_int]
+        L.mpmx_reset_timing.argtypes = [vp]
+        L.mpmx_phase_time.argtypes = [vp, ctypes.c_char_p, vp, vp]
+        L.mpmx_stress.argtypes = [vp, i64] + [vp] * 5
+        L.mpmx_svd.argtypes = [vp, i64] + [vp] * 4
+        L.mpmx_sort_keys.argtypes = [vp, i64, vp, c_int, vp, vp]
+        L.mpmx_seed_particles.argtypes = [ctypes.POINTER(SeedDesc), c_int, i64] + [vp] * 9
+        L.mpmx_seed_particles.restype = i64
+        L.mpmx_random_boxes.argtypes = [ctypes.c_uint32, c_int, dbl, dbl, dbl, dbl, dbl, dbl, dbl, dbl, c_int,
+                                        vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise MpmxError(rc, lib().mpmx_last_error().decode())
+
+
+class Context:
+    """One MPM scene on the current GPU (MPM_EXAMPLE + MPM_DRIVER state)."""
+
+    def __init__(self, dim, cells, lo, hi, precision=FP32):
+        self.dim = dim
+        self.precision = precision
+        self.dtype = np.float32 if precision == FP32 else np.float64
+        cells = [cells] * dim if np.isscalar(cells) else list(cells)
+        self.cells = cells
+        d = GridDesc()
+        d.dim, d.precision = dim, precision
+        for a in range(3):
+            d.cells[a] = cells[a] if a < dim else 1
+            d.lo[a] = lo[a] if a < dim else 0.0
+            d.hi[a] = hi[a] if a < dim else 1.0
+        h = ctypes.c_void_p()
+        _check(lib().mpmx_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mpmx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _arr(self, a, shape):
+        return np.ascontiguousarray(np.asarray(a, dtype=self.dtype).reshape(shape))
+
+    def set_particles(self, X, V, F, B, mass, volume, mu, lam, valid=None):
+        d = self.dim
+        n = int(np.asarray(X).shape[0])
+        arrs = [self._arr(X, (n, d)), self._arr(V, (n, d)), self._arr(F, (n, d * d)), self._arr(B, (n, d * d)),
+                self._arr(mass, (n,)), self._arr(volume, (n,)), self._arr(mu, (n,)), self._arr(lam, (n,))]
+        v = np.ones(n, np.uint8) if valid is None else np.ascontiguousarray(valid, np.uint8)
+        _check(lib().mpmx_set_particles(self._h, n, *[_p(a) for a in arrs], _p(v)))
+        self.n = n
+
+    def get_particles(self):
+        n, d = self.n, self.dim
+        X = np.zeros((n, d), self.dtype)
+        V = np.zeros((n, d), self.dtype)
+        F = np.zeros((n, d * d), self.dtype)
+        B = np.zeros((n, d * d), self.dtype)
+        valid = np.zeros(n, np.uint8)
+        _check(lib().mpmx_get_particles(self._h, _p(X), _p(V), _p(F), _p(B), _p(valid)))
+        return dict(X=X, V=V, F=F, B=B, valid=valid)
+
+    def set_constitutive(self, model, neo_threshold=0.25):
+        _check(lib().mpmx_set_constitutive(self._h, model, neo_threshold))
+
+    def set_gravity(self, g):
+        g = np.ascontiguousarray(list(g) + [0.0] * (3 - len(g)), np.float64)
+        _check(lib().mpmx_set_gravity(self._h, _p(g)))
+
+    def add_collision_box(self, type_, friction, lo, hi):
+        lo = np.ascontiguousarray(list(lo) + [0.0] * (3 - len(lo)), np.float64)
+        hi = np.ascontiguousarray(list(hi) + [0.0] * (3 - len(hi)), np.float64)
+        _check(lib().mpmx_add_collision_box(self._h, type_, friction, _p(lo), _p(hi)))
+
+    def clear_collision_objects(self):
+        _check(lib().mpmx_clear_collision_objects(self._h))
+
+    def step(self, dt, nsteps=1):
+        _check(lib().mpmx_step(self._h, dt, nsteps))
+
+    def step_async(self, dt, nsteps=1):
+        _check(lib().mpmx_step_async(self._h, dt, nsteps))
+
+    def synchronize(self):
+        _check(lib().mpmx_synchronize(self._h))
+
+    @property
+    def time(self):
+        return lib().mpmx_time(self._h)
+
+    def enable_grid_stats(self, on=True):
+        _check(lib().mpmx_enable_grid_stats(self._h, 1 if on else 0))
+
+    def grid_stats(self):
+        m = np.zeros(1)
+        p = np.zeros(3)
+        _check(lib().mpmx_grid_stats(self._h, _p(m), _p(p)))
+        return float(m[0]), p[: self.dim].copy()
+
+    def particle_stats(self):
+        m = np.zeros(1)
+        p = np.zeros(3)
+        c = np.zeros(1, np.int64)
+        _check(lib().mpmx_particle_stats(self._h, _p(m), _p(p), _p(c)))
+        return float(m[0]), p[: self.dim].copy(), int(c[0])
+
+    def get_grid(self):
+        nn = lib().mpmx_grid_node_count(self._h)
+        m = np.zeros(nn, self.dtype)
+        v = np.zeros((nn, self.dim), self.dtype)
+        _check(lib().mpmx_get_grid(self._h, _p(m), _p(v)))
+        return m, v
+
+    def set_reorder_interval(self, k):
+        _check(lib().mpmx_set_reorder_interval(self._h, k))
+
+    def enable_timing(self, on=True):
+        _check(lib().mpmx_enable_timing(self._h, 1 if on else 0))
+
+    def reset_timing(self):
+        _check(lib().mpmx_reset_timing(self._h))
+
+    def phase_time(self, phase):
+        t = np.zeros(1)
+        c = np.zeros(1, np.int64)
+        _check(lib().mpmx_phase_time(self._h, phase.encode(), _p(t), _p(c)))
+        return float(t[0]), int(c[0])
+
+    def stress(self, F, volume, mu, lam):
+        d = self.dim
+        F = self._arr(F, (-1, d * d))
+        n = F.shape[0]
+        vol, mu, lam = (self._arr(np.broadcast_to(x, (n,)), (n,)) for x in (volume, mu, lam))
+        tau = np.zeros_like(F)
+        _check(lib().mpmx_stress(self._h, n, _p(F), _p(vol), _p(mu), _p(lam), _p(tau)))
+        return tau
+
+    def svd(self, F):
+        d = self.dim
+        F = self._arr(F, (-1, d * d))
+        n = F.shape[0]
+        U = np.zeros_like(F)
+        V = np.zeros_like(F)
+        S = np.zeros((n, d), self.dtype)
+        _check(lib().mpmx_svd(self._h, n, _p(F), _p(U), _p(S), _p(V)))
+        return U, S, V
+
+    def sort_keys(self, keys, key_bits):
+        keys = np.ascontiguousarray(keys, np.uint32)
+        n = keys.shape[0]
+        ko = np.zeros(n, np.uint32)
+        oo = np.zeros(n, np.uint32)
+        _check(lib().mpmx_sort_keys(self._h, n, _p(keys), key_bits, _p(ko), _p(oo)))
+        return ko, oo
+
+
+def seed_particles(dim, cells, lo, hi, shapes, ppc_axis=2, seed=1234, precision=FP32, jitter=True):
+    """Jittered seeding of shapes, see include/mpmx_scene.h.
+
+    ``shapes``: list of dicts {type: SHAPE_BOX|SHAPE_SPHERE, a: 3 floats,
+    b: 3 floats (box max or [radius]), density, E, nu, v: velocity}.
+    Returns a dict of particle arrays.
+    """
+    cells = [cells] * dim if np.isscalar(cells) else list(cells)
+    ns = len(shapes)
+    st = np.array([s["type"] for s in shapes], np.int32)
+    sa = np.array([list(s["a"]) + [0.0] * (3 - len(s["a"])) for s in shapes], np.float64)
+    sb = np.array([list(s["b"]) + [0.0] * (3 - len(s["b"])) for s in shapes], np.float64)
+    den = np.array([s.get("density", 1000.0) for s in shapes], np.float64)
+    E = np.array([s["E"] for s in shapes], np.float64)
+    nu = np.array([s["nu"] for s in shapes], np.float64)
+    vel = np.array([list(s.get("v", [0.0] * 3)) + [0.0] * (3 - len(s.get("v", [0.0] * 3))) for s in shapes],
+                   np.float64)
+    d = SeedDesc()
+    d.dim = dim
+    for a in range(3):
+        d.cells[a] = cells[a] if a < dim else 1
+        d.lo[a] = lo[a] if a < dim else 0.0
+        d.hi[a] = hi[a] if a < dim else 1.0
+    d.ppc_axis, d.seed, d.jitter, d.nshapes = ppc_axis, seed, 1 if jitter else 0, ns
+    keep = (st, sa, sb, den, E, nu, vel)
+    d.shape_type, d.shape_a, d.shape_b, d.density, d.youngs, d.poisson, d.velocity = [
+        a.ctypes.data for a in keep]
+    n = lib().mpmx_seed_particles(ctypes.byref(d), precision, 0, None, None, None, None, None, None, None, None, None)
+    if n < 0:
+        raise MpmxError(-1, "bad seeding description")
+    dt = np.float32 if precision == FP32 else np.float64
+    out = dict(X=np.zeros((n, dim), dt), V=np.zeros((n, dim), dt), F=np.zeros((n, dim * dim), dt),
+               B=np.zeros((n, dim * dim), dt), mass=np.zeros(n, dt), volume=np.zeros(n, dt), mu=np.zeros(n, dt),
+               lam=np.zeros(n, dt), valid=np.zeros(n, np.uint8))
+    got = lib().mpmx_seed_particles(ctypes.byref(d), precision, n, *[_p(out[k]) for k in
+                                    ("X", "V", "F", "B", "mass", "volume", "mu", "lam", "valid")])
+    if got != n:
+        raise MpmxError(-5, "seeding count mismatch")
+    del keep
+    return out
+
+
+def random_boxes(seed, nboxes=12, side_min=0.08, side_max=0.2, region_lo=0.1, region_hi=0.9,
+                 E_min=1e3, E_max=1e5, nu_min=0.2, nu_max=0.4, max_tries=10000):
+    lo = np.zeros((nboxes, 3))
+    hi = np.zeros((nboxes, 3))
+    E = np.zeros(nboxes)
+    nu = np.zeros(nboxes)
+    k = lib().mpmx_random_boxes(seed, nboxes, side_min, side_max, region_lo, region_hi, E_min, E_max, nu_min,
+                                nu_max, max_tries, _p(lo), _p(hi), _p(E), _p(nu))
+    if k < 0:
+        raise MpmxError(-1, "bad box description")
+    return lo[:k], hi[:k], E[:k], nu[:k]

@@ -1,0 +1,173 @@
+"""Parity of the HIP step (through the C ABI) against the oracle.
+
+Each case builds one seeded scene, runs it on the GPU and in the oracle from
+identical inputs and compares particle X, V, F, B, valid and the grid mass and
+velocity; the error of a quantity is max|a - b| / max|b|.
+
+FP64 runs must match the FP64 oracle to 1e-9 after one step.  FP32 runs must
+match, after one step, both the FP32 oracle and the FP64 oracle (the exact
+answer for the same float inputs) to 1e-5 -- except B against the FP32
+oracle: the reference evaluates B in a form that cancels O(|v||x|) terms to
+produce an O(|v|h) result, so its FP32 B carries ~1e-4 relative rounding
+error, while the kernel's B is checked against the FP64 oracle.  After ten
+steps the FP32 drift from the FP64 oracle is bounded by 1e-4.
+"""
+import numpy as np
+import pytest
+
+import mpmtest as mt
+from physbam_amd import mpmx
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_context(sc, prec=None):
+    prec = prec or sc["prec"]
+    d = sc["dim"]
+    ctx = mpmx.Context(d, sc["N"], sc["lo"], sc["hi"], precision=prec)
+    ctx.set_constitutive(sc["model"], sc.get("neo_threshold", 0.25))
+    ctx.set_gravity(sc["gravity"])
+    for w in sc["walls"]:
+        ctx.add_collision_box(w["type"], w["friction"], w["lo"][:d], w["hi"][:d])
+    ctx.set_particles(sc["X"], sc["V"], sc["F"], sc["B"], sc["mass"], sc["volume"], sc["mu"], sc["lambda"],
+                      sc["valid"])
+    return ctx
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def compare(gpu, orc, tol, tag, keys=("X", "V", "F", "B")):
+    errs = {k: rel(gpu[k], orc[k]) for k in keys}
+    print(tag, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert np.array_equal(gpu["valid"], orc["valid"]), tag
+    for k, v in errs.items():
+        assert v <= tol, (tag, k, v)
+    return errs
+
+
+sq2 = lambda X: (np.abs(X[:, 0] - 0.5) < 0.2) & (np.abs(X[:, 1] - 0.6) < 0.2)
+sph3 = lambda X: ((X - np.array([0.5, 0.45, 0.5])) ** 2).sum(1) < 0.3 ** 2
+
+CASES = [
+    # dim, prec, model, wall type, friction
+    (2, 8, mt.COROTATED, mt.STICK, 0.0),
+    (2, 8, mt.NEO_HOOKEAN, mt.SLIP, 0.3),
+    (2, 4, mt.COROTATED, mt.SEPARATE, 0.0),
+    (3, 8, mt.COROTATED, mt.SLIP, 0.0),
+    (3, 8, mt.NEO_HOOKEAN, mt.SEPARATE, 0.2),
+    (3, 4, mt.COROTATED, mt.STICK, 0.0),
+    (3, 4, mt.NEO_HOOKEAN, mt.SLIP, 0.0),
+    (3, 4, mt.COROTATED, mt.SEPARATE, 0.3),
+]
+
+
+def scene(dim, prec, model, wtype, fric, N=None):
+    N = N or (32 if dim == 2 else 16)
+    walls = mt.wall_boxes(dim, [N] * dim, [0] * dim, [1] * dim, flags=(1 << (2 * dim)) - 1, inset=0.12,
+                          type_=wtype, friction=fric)
+    shape = sq2 if dim == 2 else sph3
+    return mt.make_scene(dim, N, prec=prec, model=model, shape=shape, v0=[0.3, -3.0, 0.2][:dim], vrand=1.0,
+                         brand=1e-3, frand=0.2, walls=walls, dt=1e-3, per_particle_E=True)
+
+
+@pytest.mark.parametrize("dim,prec,model,wtype,fric", CASES)
+def test_one_step_matches_oracle(dim, prec, model, wtype, fric):
+    sc = scene(dim, prec, model, wtype, fric)
+    tag = f"step1 d{dim} p{prec} m{model} w{wtype} f{fric} n={sc['X'].shape[0]}"
+    print(tag, flush=True)
+    orc = mt.oracle_run(sc, 1, prec=prec)
+    with gpu_context(sc) as ctx:
+        ctx.step(sc["dt"], 1)
+        gpu = ctx.get_particles()
+        gm, gv = ctx.get_grid()
+    if prec == 8:
+        compare(gpu, orc, 1e-9, tag)
+    else:
+        orc64 = mt.oracle_run(sc, 1, prec=8)
+        compare(gpu, orc, 1e-5, tag + " vs fp32 oracle", keys=("X", "V", "F"))
+        compare(gpu, orc64, 1e-5, tag + " vs fp64 oracle")
+        print(tag, "fp32 oracle B vs fp64 oracle", f"{rel(orc['B'], orc64['B']):.2e}", flush=True)
+    assert rel(gm, orc["grid_mass"]) <= (1e-12 if prec == 8 else 1e-5)
+    assert rel(gv, orc["grid_velocity"]) <= (1e-9 if prec == 8 else 1e-5)
+
+
+@pytest.mark.parametrize("dim,prec,model,wtype,fric", CASES)
+def test_ten_steps_match_oracle(dim, prec, model, wtype, fric):
+    sc = scene(dim, prec, model, wtype, fric)
+    tag = f"step10 d{dim} p{prec} m{model} w{wtype} f{fric}"
+    print(tag, flush=True)
+    orc64 = mt.oracle_run(sc, 10, prec=8, want_grid=False)
+    with gpu_context(sc) as ctx:
+        ctx.set_reorder_interval(3)
+        ctx.step(sc["dt"], 10)
+        gpu = ctx.get_particles()
+    if prec == 8:
+        compare(gpu, orc64, 1e-8, tag)
+    else:
+        orc = mt.oracle_run(sc, 10, prec=4, want_grid=False)
+        e = compare(gpu, orc64, 1e-4, tag + " vs fp64 oracle")
+        r = {k: rel(orc[k], orc64[k]) for k in e}
+        print(tag, "fp32 oracle vs fp64 oracle", {k: f"{v:.2e}" for k, v in r.items()}, flush=True)
+
+
+def test_sort_is_stable_and_exact():
+    rng = np.random.default_rng(7)
+    with mpmx.Context(3, 8, [0, 0, 0], [1, 1, 1]) as ctx:
+        for n, bits in ((1, 4), (1000, 5), (100003, 19), (262144, 8)):
+            keys = rng.integers(0, (1 << bits) - 1, n).astype(np.uint32)
+            keys[rng.random(n) < 0.05] = 0xFFFFFFFF
+            print("sort", n, bits, flush=True)
+            ko, order = ctx.sort_keys(keys, bits)
+            ref = np.argsort(keys, kind="stable")
+            assert np.array_equal(order, ref)
+            assert np.array_equal(ko, keys[ref])
+
+
+def test_svd_kernel():
+    rng = np.random.default_rng(3)
+    n = 4096
+    F = np.tile(np.eye(3).reshape(-1), (n, 1)) + 0.4 * (rng.random((n, 9)) - 0.5)
+    F[:8] = [[1, 0, 0, 0, 1, 0, 0, 0, 1], [2, 0, 0, 0, 2, 0, 0, 0, 2], [1, 0, 0, 0, 1, 0, 0, 0, -1],
+             [0, 1, 0, 1, 0, 0, 0, 0, 1], [1e-4, 0, 0, 0, 1, 0, 0, 0, 1], [3, 0, 0, 0, 1, 0, 0, 0, 1e-6],
+             [1, 1, 0, 0, 1, 0, 0, 0, 1], [-1, 0, 0, 0, -1, 0, 0, 0, -1]]
+    with mpmx.Context(3, 8, [0, 0, 0], [1, 1, 1], precision=mpmx.FP64) as ctx:
+        U, S, V = ctx.svd(F)
+    Um = U.reshape(n, 3, 3).transpose(0, 2, 1)
+    Vm = V.reshape(n, 3, 3).transpose(0, 2, 1)
+    Fm = F.reshape(n, 3, 3).transpose(0, 2, 1)
+    rec = Um @ (S[:, :, None] * Vm.transpose(0, 2, 1))
+    assert np.abs(rec - Fm).max() < 1e-12
+    assert np.abs(Um @ Um.transpose(0, 2, 1) - np.eye(3)).max() < 1e-12
+    assert np.abs(np.linalg.det(Um) - 1).max() < 1e-12 and np.abs(np.linalg.det(Vm) - 1).max() < 1e-12
+    a = np.abs(S)
+    assert (a[:, 0] >= a[:, 1] - 1e-14).all() and (a[:, 1] >= a[:, 2] - 1e-14).all()
+    assert (S[:, :2] >= 0).all()
+    _, So, _, _ = mt.oracle_svd(F, 3)
+    assert np.abs(S - So).max() < 1e-12
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("model", [0, 1])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_stress_kernel(dim, model, prec):
+    rng = np.random.default_rng(11)
+    n = 2048
+    F = np.tile(np.eye(dim).reshape(-1), (n, 1)) + 0.3 * (rng.random((n, dim * dim)) - 0.5)
+    # compressed particles reach the neo-Hookean clamp branch (J < 0.25)
+    F[: n // 8] *= 0.5
+    vol = rng.uniform(1e-6, 2e-6, n)
+    mu, la = mt.lame(rng.uniform(1e3, 1e5, n), rng.uniform(0.2, 0.4, n))
+    dt = np.float32 if prec == 4 else np.float64
+    F, vol, mu, la = (np.asarray(x, dt).astype(np.float64) for x in (F, vol, mu, la))
+    with mpmx.Context(dim, 8, [0] * dim, [1] * dim, precision=prec) as ctx:
+        ctx.set_constitutive(model, 0.25)
+        tau = ctx.stress(F, vol, mu, la)
+    ref = mt.oracle_stress(F, vol, mu, la, dim, model, 0.25, prec=8)
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    err = (np.abs(tau - ref) / scale).max()
+    print("stress", dim, model, prec, err, flush=True)
+    assert err < (1e-10 if prec == 8 else 2e-5)
