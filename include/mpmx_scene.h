@@ -40,7 +40,8 @@ typedef struct mpmx_seed_desc {
 /* Jittered seeding.  Cells are visited row-major (last axis fastest); each is
    split into ppc_axis^dim sub-cells (row-major), one sample per sub-cell drawn
    with coordinates in axis order; a sample is kept if it lies in a shape (the
-   first containing shape gives its material).  volume = h^d / ppc,
+   first containing shape gives its material).  Samples outside the union
+   bounding box of the shapes are drawn and skipped.  volume = h^d / ppc,
    mass = density * volume, F = I, B = 0, mu/lambda from (E, nu).
    Returns the particle count.  Arrays (precision 4 = float, 8 = double) are
    filled only if X is non-null and capacity >= count. */
@@ -48,14 +49,15 @@ int64_t mpmx_seed_particles(const mpmx_seed_desc* desc, int precision, int64_t c
                             void* F, void* B, void* mass, void* volume, void* mu, void* lambda, uint8_t* valid);
 
 /* Random non-overlapping axis-aligned boxes in [region_lo, region_hi]^3 with
-   independent side lengths uniform in [side_min, side_max], by rejection
-   (at most max_tries per box); Young's modulus log-uniform in [E_min, E_max],
-   Poisson ratio uniform in [nu_min, nu_max].  Draw order per attempt: the
-   three sides, then the three min-corner coordinates; per accepted box: E then
-   nu.  Returns the number of boxes placed. */
+   side lengths uniform in [side_min, side_max] (one draw per box if cubes is
+   nonzero, else one per axis), placed by rejection (at most max_tries per
+   box); Young's modulus log-uniform in [E_min, E_max], Poisson ratio uniform
+   in [nu_min, nu_max].  Draw order per attempt: the side(s), then the three
+   min-corner coordinates; per accepted box: E then nu.  Returns the number of
+   boxes placed. */
 int mpmx_random_boxes(uint32_t seed, int nboxes, double side_min, double side_max, double region_lo,
                       double region_hi, double E_min, double E_max, double nu_min, double nu_max, int max_tries,
-                      double* box_lo, double* box_hi, double* E, double* nu);
+                      int cubes, double* box_lo, double* box_hi, double* E, double* nu);
 
 #ifdef __cplusplus
 }

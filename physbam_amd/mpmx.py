@@ -91,7 +91,7 @@ def lib():
         L.mpmx_seed_particles.argtypes = [ctypes.POINTER(SeedDesc), c_int, i64] + [vp] * 9
         L.mpmx_seed_particles.restype = i64
         L.mpmx_random_boxes.argtypes = [ctypes.c_uint32, c_int, dbl, dbl, dbl, dbl, dbl, dbl, dbl, dbl, c_int,
-                                        vp, vp, vp, vp]
+                                        c_int, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -284,29 +284,36 @@ def seed_particles(dim, cells, lo, hi, shapes, ppc_axis=2, seed=1234, precision=
     keep = (st, sa, sb, den, E, nu, vel)
     d.shape_type, d.shape_a, d.shape_b, d.density, d.youngs, d.poisson, d.velocity = [
         a.ctypes.data for a in keep]
-    n = lib().mpmx_seed_particles(ctypes.byref(d), precision, 0, None, None, None, None, None, None, None, None, None)
+    # Upper bound on the count (cells overlapping each shape's bounding box),
+    # so that one pass of the generator fills the arrays.
+    h = (np.asarray(hi[:dim], float) - np.asarray(lo[:dim], float)) / np.asarray(cells[:dim], float)
+    cap = 0
+    for s in shapes:
+        a = np.asarray(s["a"][:dim], float)
+        ext = np.asarray(s["b"][:dim], float) - a if s["type"] == SHAPE_BOX else np.full(dim, 2 * s["b"][0])
+        cap += int(np.prod(np.ceil(ext / h) + 2)) * ppc_axis ** dim
+    dt = np.float32 if precision == FP32 else np.float64
+    out = dict(X=np.zeros((cap, dim), dt), V=np.zeros((cap, dim), dt), F=np.zeros((cap, dim * dim), dt),
+               B=np.zeros((cap, dim * dim), dt), mass=np.zeros(cap, dt), volume=np.zeros(cap, dt),
+               mu=np.zeros(cap, dt), lam=np.zeros(cap, dt), valid=np.zeros(cap, np.uint8))
+    n = lib().mpmx_seed_particles(ctypes.byref(d), precision, cap, *[_p(out[k]) for k in
+                                  ("X", "V", "F", "B", "mass", "volume", "mu", "lam", "valid")])
+    del keep
     if n < 0:
         raise MpmxError(-1, "bad seeding description")
-    dt = np.float32 if precision == FP32 else np.float64
-    out = dict(X=np.zeros((n, dim), dt), V=np.zeros((n, dim), dt), F=np.zeros((n, dim * dim), dt),
-               B=np.zeros((n, dim * dim), dt), mass=np.zeros(n, dt), volume=np.zeros(n, dt), mu=np.zeros(n, dt),
-               lam=np.zeros(n, dt), valid=np.zeros(n, np.uint8))
-    got = lib().mpmx_seed_particles(ctypes.byref(d), precision, n, *[_p(out[k]) for k in
-                                    ("X", "V", "F", "B", "mass", "volume", "mu", "lam", "valid")])
-    if got != n:
-        raise MpmxError(-5, "seeding count mismatch")
-    del keep
-    return out
+    if n > cap:
+        raise MpmxError(-5, "seeding bound too small")
+    return {k: v[:n] for k, v in out.items()}
 
 
 def random_boxes(seed, nboxes=12, side_min=0.08, side_max=0.2, region_lo=0.1, region_hi=0.9,
-                 E_min=1e3, E_max=1e5, nu_min=0.2, nu_max=0.4, max_tries=10000):
+                 E_min=1e3, E_max=1e5, nu_min=0.2, nu_max=0.4, max_tries=10000, cubes=True):
     lo = np.zeros((nboxes, 3))
     hi = np.zeros((nboxes, 3))
     E = np.zeros(nboxes)
     nu = np.zeros(nboxes)
     k = lib().mpmx_random_boxes(seed, nboxes, side_min, side_max, region_lo, region_hi, E_min, E_max, nu_min,
-                                nu_max, max_tries, _p(lo), _p(hi), _p(E), _p(nu))
+                                nu_max, max_tries, 1 if cubes else 0, _p(lo), _p(hi), _p(E), _p(nu))
     if k < 0:
         raise MpmxError(-1, "bad box description")
     return lo[:k], hi[:k], E[:k], nu[:k]
