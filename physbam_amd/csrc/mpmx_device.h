@@ -83,18 +83,30 @@ MPMX_HD void weights(const Geom<T, D>& g, const T* X, Wts<T, D>& s) {
   }
 }
 
-// Linear tile id of the particle's base node, or INVALID_KEY if the stencil
-// would leave the ghosted grid.
+// Binning key of a particle: (linear id of the tile holding its stencil base
+// node) * 64 + (index of the base node inside the tile, row-major), or
+// INVALID_KEY if the stencil would leave the ghosted grid.  Sorting by this key
+// groups particles by tile and, inside a tile, by base cell.
 template<typename T, int D>
-MPMX_HD uint32_t tile_key(const Geom<T, D>& g, const T* X) {
-  uint32_t key = 0;
+MPMX_HD uint32_t bin_key(const Geom<T, D>& g, const T* X) {
+  uint32_t tile = 0, cell = 0;
 #pragma unroll
   for (int a = 0; a < D; a++) {
     int b = (int)tfloor<T>(((X[a] - g.dx[a]) - g.lo[a]) * g.inv_dx[a]) + 3;
     if (!(b >= 0 && b + 2 < g.G[a])) return INVALID_KEY;
-    key = key * (uint32_t)g.GT[a] + (uint32_t)(b / Tile<D>::E);
+    tile = tile * (uint32_t)g.GT[a] + (uint32_t)(b / Tile<D>::E);
+    cell = cell * (uint32_t)Tile<D>::E + (uint32_t)(b % Tile<D>::E);
   }
-  return key;
+  return (tile << 6) | cell;
+}
+
+// Offsets of cell `l` (0..63) inside its tile, row-major.
+template<int D> MPMX_HD void cell_offsets(uint32_t l, int* o) {
+#pragma unroll
+  for (int a = D - 1; a >= 0; a--) {
+    o[a] = (int)(l % Tile<D>::E);
+    l /= Tile<D>::E;
+  }
 }
 
 // -------------------------------------------------------------------- SVD --
@@ -330,6 +342,155 @@ __device__ __forceinline__ void kirchhoff(int model, T mu, T la, T thr, T root, 
 #pragma unroll
     for (int i = 0; i < D * D; i++) tau[i] = (T)taud[i];
   }
+}
+
+// ------------------------------------------------------ fast stress path ---
+// tau = vol * P(F) F^T evaluated from the small-strain quantities
+//   G = F - I,  E = F F^T - I = G + G^T + G G^T,
+//   J - 1 = tr G + (sum of principal 2x2 minors of G) + det G,
+// which keeps full relative precision when F is close to a rotation.
+//   neo-Hookean (J >= t):  tau = vol (mu E + lambda ln J I)          (no SVD)
+//   fixed corotated:       E = U diag(e) U^T by Jacobi; s_i = sqrt(1 + e_i);
+//                          tau = vol U diag(2 mu s_i (s_i - 1) + lambda (J-1) J) U^T
+// Inverted or nearly degenerate F (J <= 0 or s_min < 1e-2 s_max) and the
+// neo-Hookean clamp branch (J < t) go through the FP64 SVD of kirchhoff().
+template<typename T> __device__ __forceinline__ T fast_rcp(T x) { return (T)1 / x; }
+template<> __device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template<typename T> __device__ __forceinline__ T fast_rsqrt(T x) { return (T)1 / sqrt(x); }
+template<> __device__ __forceinline__ float fast_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template<typename T> __device__ __forceinline__ T tlog1p(T x);
+template<> __device__ __forceinline__ float tlog1p<float>(float x) { return log1pf(x); }
+template<> __device__ __forceinline__ double tlog1p<double>(double x) { return log1p(x); }
+
+template<typename T> struct EigenSweeps;
+template<> struct EigenSweeps<float> { static constexpr int n = 4; };
+template<> struct EigenSweeps<double> { static constexpr int n = 6; };
+
+// Cyclic Jacobi on a symmetric D x D matrix A (in place); U collects the
+// rotations so that A_in = U diag(A_out) U^T.
+template<typename T, int D>
+__device__ __forceinline__ void sym_eigen(T (&A)[D][D], T (&U)[D][D]) {
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) U[i][j] = i == j ? (T)1 : (T)0;
+  constexpr int NP = D == 3 ? 3 : 1;
+  constexpr int sweeps = D == 3 ? EigenSweeps<T>::n : 1;
+#pragma unroll
+  for (int sw = 0; sw < sweeps; sw++) {
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      const int p = (D == 3 && k == 2) ? 1 : 0, q = (D == 3 && k != 0) ? 2 : 1;
+      const T apq = A[p][q];
+      if (apq != (T)0) {
+        const T theta = (A[q][q] - A[p][p]) * (T)0.5 * fast_rcp<T>(apq);
+        T t = fast_rcp<T>(tabs(theta) + tsqrt<T>((T)1 + theta * theta));
+        t = theta < 0 ? -t : t;
+        const T c = fast_rsqrt<T>((T)1 + t * t), s = t * c;
+        A[p][p] -= t * apq;
+        A[q][q] += t * apq;
+        A[p][q] = A[q][p] = 0;
+#pragma unroll
+        for (int r = 0; r < D; r++) {
+          if (r == p || r == q) continue;
+          const T arp = A[r][p], arq = A[r][q];
+          A[r][p] = A[p][r] = c * arp - s * arq;
+          A[r][q] = A[q][r] = s * arp + c * arq;
+        }
+#pragma unroll
+        for (int r = 0; r < D; r++) {
+          const T up = U[r][p], uq = U[r][q];
+          U[r][p] = c * up - s * uq;
+          U[r][q] = s * up + c * uq;
+        }
+      }
+    }
+  }
+}
+
+template<typename T, int D>
+__device__ __forceinline__ void kirchhoff_fast(int model, T mu, T la, T thr, T root, T vol, const T* F, T* tau) {
+  T G[D][D];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) G[i][j] = F[i + D * j] - (i == j ? (T)1 : (T)0);
+  T E[D][D];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = i; j < D; j++) {
+      T e = G[i][j] + G[j][i];
+#pragma unroll
+      for (int k = 0; k < D; k++) e += G[i][k] * G[j][k];
+      E[i][j] = e;
+      E[j][i] = e;
+    }
+  T Jm1;
+  if constexpr (D == 3) {
+    const T tr = G[0][0] + G[1][1] + G[2][2];
+    const T m2 = (G[0][0] * G[1][1] - G[0][1] * G[1][0]) + (G[0][0] * G[2][2] - G[0][2] * G[2][0]) +
+                 (G[1][1] * G[2][2] - G[1][2] * G[2][1]);
+    const T dg = G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1]) - G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0]) +
+                 G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
+    Jm1 = tr + m2 + dg;
+  } else {
+    Jm1 = G[0][0] + G[1][1] + (G[0][0] * G[1][1] - G[0][1] * G[1][0]);
+  }
+  const T J = (T)1 + Jm1;
+  bool slow = J <= (T)0;
+  if (model == 1) {
+    if (!slow && J >= thr) {
+      const T c = la * tlog1p<T>(Jm1);
+#pragma unroll
+      for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) tau[i + D * j] = vol * (mu * E[i][j] + (i == j ? c : (T)0));
+      return;
+    }
+    slow = true;
+  }
+  if (!slow) {
+    T U[D][D];
+    sym_eigen<T, D>(E, U);
+    T emin = E[0][0], emax = E[0][0];
+#pragma unroll
+    for (int i = 1; i < D; i++) {
+      emin = E[i][i] < emin ? E[i][i] : emin;
+      emax = E[i][i] > emax ? E[i][i] : emax;
+    }
+    if ((T)1 + emin < (T)1e-4 * ((T)1 + emax)) {
+      slow = true;
+    } else {
+      const T pj = la * Jm1 * J;
+      T k[D];
+#pragma unroll
+      for (int i = 0; i < D; i++) {
+        const T e = E[i][i];
+        const T s = tsqrt<T>((T)1 + e);
+        const T sm1 = e / ((T)1 + s);
+        k[i] = vol * ((T)2 * mu * s * sm1 + pj);
+      }
+#pragma unroll
+      for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = i; j < D; j++) {
+          T acc = 0;
+#pragma unroll
+          for (int q = 0; q < D; q++) acc += U[i][q] * k[q] * U[j][q];
+          tau[i + D * j] = acc;
+          tau[j + D * i] = acc;
+        }
+      return;
+    }
+  }
+  double Fd[D * D], taud[D * D];
+  bool dd;
+#pragma unroll
+  for (int i = 0; i < D * D; i++) Fd[i] = (double)F[i];
+  kirchhoff_typed<double, D>(model, (double)mu, (double)la, (double)thr, (double)root, (double)vol, Fd, taud, dd);
+#pragma unroll
+  for (int i = 0; i < D * D; i++) tau[i] = (T)taud[i];
 }
 
 // ---------------------------------------------------------------- walls ----

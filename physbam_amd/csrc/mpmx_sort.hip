@@ -19,7 +19,7 @@ constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = THREADS * SCAN_ITEMS;  // 2048
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = THREADS * SORT_ITEMS;  // 4096
-constexpr int MAX_DIGIT_BITS = 8;
+constexpr int MAX_DIGIT_BITS = 9;
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const int lane = threadIdx.x & (WAVE - 1);

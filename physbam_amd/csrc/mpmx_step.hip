@@ -58,6 +58,7 @@ template<typename T, int D> struct Args {
   uint32_t* keys;
   uint32_t n;
   const uint32_t* order;  // sorted position -> slot, null for identity
+  const uint32_t* skeys;  // binning keys in sorted order
   const uint32_t* tile_begin;
   const uint32_t* tile_end;
   const uint32_t* ptiles;
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(BLOCK) void key_kernel(Args<T, D> a, uint32_t* rang
     T x[D];
 #pragma unroll
     for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
-    key = tile_key<T, D>(a.g, x);
+    key = bin_key<T, D>(a.g, x);
     if (key == INVALID_KEY) atomicAdd(range_err, 1u);
   }
   a.keys[p] = key;
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(BLOCK) void fill_u32_kernel(uint32_t* p, uint32_t n
   if (i < n) p[i] = v;
 }
 
-// Tile ranges of the sorted keys.  A tile is occupied this step iff its stamp
+// Tile ranges of the sorted keys (tile = key >> 6).  A tile is occupied this step iff its stamp
 // equals the step stamp, so nothing has to be cleared between steps.
 __global__ __launch_bounds__(BLOCK) void bounds_kernel(const uint32_t* sk, uint32_t n, uint32_t* tb, uint32_t* te,
                                                        uint32_t* stamp_arr, uint32_t stamp, uint32_t* counts) {
@@ -127,12 +128,13 @@ __global__ __launch_bounds__(BLOCK) void bounds_kernel(const uint32_t* sk, uint3
   if (i >= n) return;
   const uint32_t k = sk[i];
   if (k == INVALID_KEY) return;
-  if (i == 0 || sk[i - 1] != k) {
-    tb[k] = i;
-    stamp_arr[k] = stamp;
+  const uint32_t t = k >> 6;
+  if (i == 0 || (sk[i - 1] >> 6) != t) {
+    tb[t] = i;
+    stamp_arr[t] = stamp;
   }
-  const bool last = (i == n - 1) || sk[i + 1] != k;
-  if (last) te[k] = i + 1;
+  const bool last = (i == n - 1) || (sk[i + 1] >> 6) != t;
+  if (last) te[t] = i + 1;
   if (i == n - 1 || sk[i + 1] == INVALID_KEY) counts[2] = i + 1;
 }
 
@@ -194,125 +196,215 @@ __global__ __launch_bounds__(BLOCK) void zero_kernel(const uint32_t* gtiles, con
 // One workgroup per occupied tile (grid-stride over the tile list).  Mass and
 // momentum, including the stress impulse -dt * tau_p * grad w_ip of
 // MPM_FINITE_ELEMENTS::Add_Forces (Forces/MPM_FINITE_ELEMENTS.cpp:84-99), are
-// accumulated in LDS for the tile's stencil region and flushed to HBM with
+// accumulated in LDS over the tile's stencil region and flushed to HBM with
 // one atomic per touched node and component.
+//
+// No LDS atomics (on gfx950 they cost ~40 plain LDS accesses each):
+//   phase A: thread t loads particle r+t of the round (coalesced), computes
+//            its stress and transfer coefficients and stores them in LDS;
+//   phase B: lane l of each wave owns cell l of the tile (64 cells, 64 lanes)
+//            and scatters that cell's particles into the wave's private copy
+//            of the region with ordered read-modify-writes.  The 64 lanes of
+//            one instruction address 64 distinct nodes, a compiler barrier
+//            keeps each read after the previous write in program order, LDS
+//            executes one wave's accesses in order, and no other wave writes
+//            the copy, so every update is race-free;
+//   flush:   the four copies are summed per node and added to HBM.
+template<typename T, int D> struct P2GCoef {
+  // m, mv[D], mC[D*D], Q (symmetric, D(D+1)/2), xe[D]
+  static constexpr int NQ = D * (D + 1) / 2;
+  static constexpr int M = 0, MV = 1, MC = 1 + D, Q = 1 + D + D * D, XE = 1 + D + D * D + NQ;
+  static constexpr int N = XE + D;
+};
+
+template<typename T> struct Vec4Type;
+template<> struct Vec4Type<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template<> struct Vec4Type<double> { typedef double type __attribute__((ext_vector_type(4))); };
+
 template<typename T, int D>
 __global__ __launch_bounds__(BLOCK) void p2g_kernel(Args<T, D> a) {
-  constexpr int E = Tile<D>::E, R = Tile<D>::R, RN = Tile<D>::RN;
-  __shared__ T s_m[RN];
-  __shared__ T s_p[D][RN];
+  constexpr int R = Tile<D>::R, RN = Tile<D>::RN, NW = BLOCK / 64;
+  using C = P2GCoef<T, D>;
+  using T4 = typename Vec4Type<T>::type;
+  __shared__ T4 s_reg[NW][RN];  // per-wave copies of the region: (m, p_x, p_y, p_z)
+  __shared__ T s_c[C::N][BLOCK];
+  __shared__ uint32_t s_cb[64], s_ce[64];
   const uint32_t ntile = a.counts[0];
   T dinv[D];
 #pragma unroll
   for (int c = 0; c < D; c++) dinv[c] = (T)4 * a.g.inv_dx[c] * a.g.inv_dx[c];  // D_p^-1 (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:152-158)
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int lcell[D];
+  cell_offsets<D>((uint32_t)lane, lcell);
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
     const uint32_t t = a.ptiles[seg];
     int org[D];
     tile_origin<D>(t, a.g.GT, org);
-    for (int k = threadIdx.x; k < RN; k += BLOCK) {
-      s_m[k] = 0;
-#pragma unroll
-      for (int c = 0; c < D; c++) s_p[c][k] = 0;
+    for (int k = threadIdx.x; k < NW * RN; k += BLOCK) {
+      T4 z = {0, 0, 0, 0};
+      s_reg[k / RN][k % RN] = z;
+    }
+    if (threadIdx.x < 64) {
+      s_cb[threadIdx.x] = 0;
+      s_ce[threadIdx.x] = 0;
     }
     __syncthreads();
     const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
     for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
-      const uint32_t p = a.order ? a.order[i] : i;
-      T x[D], v[D], Bm[D * D], Fm[D * D];
+      const uint32_t cl = a.skeys[i] & 63u;
+      if (i == b || (a.skeys[i - 1] & 63u) != cl) s_cb[cl] = i;
+      if (i == e - 1 || (a.skeys[i + 1] & 63u) != cl) s_ce[cl] = i + 1;
+    }
+    __syncthreads();
+    for (uint32_t r = b; r < e; r += BLOCK) {
+      // ---- phase A
+      const uint32_t i = r + threadIdx.x;
+      if (i < e) {
+        const uint32_t p = a.order ? a.order[i] : i;
+        T x[D], v[D], Bm[D * D], Fm[D * D];
 #pragma unroll
-      for (int c = 0; c < D; c++) { x[c] = a.X[c * a.n + p]; v[c] = a.V[c * a.n + p]; }
+        for (int c = 0; c < D; c++) { x[c] = a.X[c * a.n + p]; v[c] = a.V[c * a.n + p]; }
 #pragma unroll
-      for (int k = 0; k < D * D; k++) { Bm[k] = a.B[k * a.n + p]; Fm[k] = a.F[k * a.n + p]; }
-      const T m = a.mass[p];
-      Wts<T, D> w;
-      weights<T, D>(a.g, x, w);
-      int lb[D];
-      bool inside = true;
-#pragma unroll
-      for (int c = 0; c < D; c++) {
-        lb[c] = w.base[c] - org[c];
-        inside = inside && lb[c] >= 0 && lb[c] < E;
-      }
-      if (!inside) {
-        atomicOr(a.err, 1u);
-        continue;
-      }
-      T tau[D * D];
-      kirchhoff<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
-      // momentum per node: m w (v + B D^-1 (x_i - x_p)) - dt tau grad w
-      T mv[D], mC[D * D], Q[D * D];
-#pragma unroll
-      for (int c = 0; c < D; c++) mv[c] = m * v[c];
-#pragma unroll
-      for (int j = 0; j < D; j++)
+        for (int k = 0; k < D * D; k++) { Bm[k] = a.B[k * a.n + p]; Fm[k] = a.F[k * a.n + p]; }
+        const T m = a.mass[p];
+        int cl[D];
+        cell_offsets<D>(a.skeys[i] & 63u, cl);
 #pragma unroll
         for (int c = 0; c < D; c++) {
-          mC[c + D * j] = m * Bm[c + D * j] * dinv[j];
-          Q[c + D * j] = a.dt * tau[c + D * j];
+          const int bb = (int)tfloor<T>(((x[c] - a.g.dx[c]) - a.g.lo[c]) * a.g.inv_dx[c]);
+          const T cen = a.g.lo[c] + ((T)bb + (T)0.5) * a.g.dx[c];
+          s_c[C::XE + c][threadIdx.x] = x[c] - cen;
+          if (bb + 3 - org[c] != cl[c]) atomicOr(a.err, 1u);
         }
-      T dz[3][D];
+        T tau[D * D];
+        kirchhoff_fast<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
+        s_c[C::M][threadIdx.x] = m;
 #pragma unroll
-      for (int o = 0; o < 3; o++)
+        for (int c = 0; c < D; c++) s_c[C::MV + c][threadIdx.x] = m * v[c];
 #pragma unroll
-        for (int c = 0; c < D; c++) dz[o][c] = (T)o * a.g.dx[c] - w.xe[c];
-      if constexpr (D == 3) {
+        for (int j = 0; j < D; j++)
 #pragma unroll
-        for (int ox = 0; ox < 3; ox++) {
+          for (int c = 0; c < D; c++) s_c[C::MC + c + D * j][threadIdx.x] = m * Bm[c + D * j] * dinv[j];
+        int q = 0;
 #pragma unroll
-          for (int oy = 0; oy < 3; oy++) {
-            const T wxy = w.w[ox][0] * w.w[oy][1];
+        for (int c = 0; c < D; c++)
 #pragma unroll
-            for (int oz = 0; oz < 3; oz++) {
-              const T wt = wxy * w.w[oz][2];
-              T g[3];
-              g[0] = w.dw[ox][0] * w.w[oy][1] * w.w[oz][2];
-              g[1] = w.w[ox][0] * w.dw[oy][1] * w.w[oz][2];
-              g[2] = wxy * w.dw[oz][2];
-              const int li = ((lb[0] + ox) * R + (lb[1] + oy)) * R + (lb[2] + oz);
-              atomicAdd(&s_m[li], wt * m);
+          for (int j = c; j < D; j++) s_c[C::Q + q++][threadIdx.x] = a.dt * tau[c + D * j];
+      }
+      __syncthreads();
+      // ---- phase B
+      const uint32_t cb = s_cb[lane], ce = s_ce[lane];
+      const uint32_t lo_i = cb > r ? cb : r;
+      const uint32_t hi_i = ce < r + BLOCK ? ce : r + BLOCK;
+      for (uint32_t q = lo_i + wv; q < hi_i; q += NW) {
+        const int sl = (int)(q - r);
+        const T m = s_c[C::M][sl];
+        T mv[D], mC[D][D], Q[D][D], w[3][D], dw[3][D], dz[3][D];
+#pragma unroll
+        for (int c = 0; c < D; c++) mv[c] = s_c[C::MV + c][sl];
+#pragma unroll
+        for (int j = 0; j < D; j++)
+#pragma unroll
+          for (int c = 0; c < D; c++) mC[c][j] = s_c[C::MC + c + D * j][sl];
+        {
+          int qq = 0;
+#pragma unroll
+          for (int c = 0; c < D; c++)
+#pragma unroll
+            for (int j = c; j < D; j++) {
+              const T qv = s_c[C::Q + qq++][sl];
+              Q[c][j] = qv;
+              Q[j][c] = qv;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < D; c++) {
+          const T xe = s_c[C::XE + c][sl];
+          const T inv = a.g.inv_dx[c];
+          const T xx = xe * inv;
+          const T q0 = xx - (T)1.5, q1 = xx - (T)1, q2 = xx - (T)0.5;
+          w[0][c] = (T)0.5 * q0 * q0;
+          w[1][c] = (T)0.75 - q1 * q1;
+          w[2][c] = (T)0.5 * q2 * q2;
+          dw[0][c] = q0 * inv;
+          dw[1][c] = (T)(-2) * q1 * inv;
+          dw[2][c] = q2 * inv;
+#pragma unroll
+          for (int o = 0; o < 3; o++) dz[o][c] = (T)o * a.g.dx[c] - xe;
+        }
+        if constexpr (D == 3) {
+          const int base_li = (lcell[0] * R + lcell[1]) * R + lcell[2];
+#pragma unroll
+          for (int ox = 0; ox < 3; ox++) {
+            T A0[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) A0[c] = mv[c] + mC[c][0] * dz[ox][0];
+#pragma unroll
+            for (int oy = 0; oy < 3; oy++) {
+              T A1[3], Fxy[3];
+              const T wxy = w[ox][0] * w[oy][1];
+              const T gx = dw[ox][0] * w[oy][1], gy = w[ox][0] * dw[oy][1];
 #pragma unroll
               for (int c = 0; c < 3; c++) {
-                T aff = mv[c] + mC[c] * dz[ox][0] + mC[c + 3] * dz[oy][1] + mC[c + 6] * dz[oz][2];
-                T f = Q[c] * g[0] + Q[c + 3] * g[1] + Q[c + 6] * g[2];
-                atomicAdd(&s_p[c][li], wt * aff - f);
+                A1[c] = A0[c] + mC[c][1] * dz[oy][1];
+                Fxy[c] = Q[c][0] * gx + Q[c][1] * gy;
+              }
+#pragma unroll
+              for (int oz = 0; oz < 3; oz++) {
+                const T wz = w[oz][2];
+                const T wt = wxy * wz;
+                const T gz = wxy * dw[oz][2];
+                T4 add;
+                add.x = wt * m;
+                add.y = wt * (A1[0] + mC[0][2] * dz[oz][2]) - (Fxy[0] * wz + Q[0][2] * gz);
+                add.z = wt * (A1[1] + mC[1][2] * dz[oz][2]) - (Fxy[1] * wz + Q[1][2] * gz);
+                add.w = wt * (A1[2] + mC[2][2] * dz[oz][2]) - (Fxy[2] * wz + Q[2][2] * gz);
+                const int li = base_li + (ox * R + oy) * R + oz;
+                const T4 cur = s_reg[wv][li];
+                s_reg[wv][li] = cur + add;
+                __asm__ __volatile__("" ::: "memory");
               }
             }
           }
-        }
-      } else {
+        } else {
+          const int base_li = lcell[0] * R + lcell[D - 1];
 #pragma unroll
-        for (int ox = 0; ox < 3; ox++) {
+          for (int ox = 0; ox < 3; ox++) {
 #pragma unroll
-          for (int oy = 0; oy < 3; oy++) {
-            const T wt = w.w[ox][0] * w.w[oy][1];
-            T g[2];
-            g[0] = w.dw[ox][0] * w.w[oy][1];
-            g[1] = w.w[ox][0] * w.dw[oy][1];
-            const int li = (lb[0] + ox) * R + (lb[1] + oy);
-            atomicAdd(&s_m[li], wt * m);
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-              T aff = mv[c] + mC[c] * dz[ox][0] + mC[c + 2] * dz[oy][1];
-              T f = Q[c] * g[0] + Q[c + 2] * g[1];
-              atomicAdd(&s_p[c][li], wt * aff - f);
+            for (int oy = 0; oy < 3; oy++) {
+              const T wt = w[ox][0] * w[oy][D - 1];
+              const T gx = dw[ox][0] * w[oy][D - 1], gy = w[ox][0] * dw[oy][D - 1];
+              T4 add;
+              add.x = wt * m;
+              add.y = wt * (mv[0] + mC[0][0] * dz[ox][0] + mC[0][D - 1] * dz[oy][D - 1]) -
+                      (Q[0][0] * gx + Q[0][D - 1] * gy);
+              add.z = wt * (mv[D - 1] + mC[D - 1][0] * dz[ox][0] + mC[D - 1][D - 1] * dz[oy][D - 1]) -
+                      (Q[D - 1][0] * gx + Q[D - 1][D - 1] * gy);
+              add.w = 0;
+              const int li = base_li + ox * R + oy;
+              const T4 cur = s_reg[wv][li];
+              s_reg[wv][li] = cur + add;
+              __asm__ __volatile__("" ::: "memory");
             }
           }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
     for (int k = threadIdx.x; k < RN; k += BLOCK) {
-      const T mm = s_m[k];
-      if (mm != (T)0) {
+      T4 sum = s_reg[0][k];
+#pragma unroll
+      for (int w = 1; w < NW; w++) sum += s_reg[w][k];
+      if (sum.x != (T)0) {
         int off[D], nc[D];
         region_offsets<D>(k, off);
 #pragma unroll
         for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
         Node4<T>* nd = a.grid + node_slot<D>(nc, a.g.GT);
-        atomic_add(&nd->m, mm);
-        atomic_add(&nd->x, s_p[0][k]);
-        atomic_add(&nd->y, s_p[1][k]);
-        if constexpr (D == 3) atomic_add(&nd->z, s_p[2][k]);
+        atomic_add(&nd->m, sum.x);
+        atomic_add(&nd->x, sum.y);
+        atomic_add(&nd->y, sum.z);
+        if constexpr (D == 3) atomic_add(&nd->z, sum.w);
       }
     }
     __syncthreads();
@@ -634,7 +726,7 @@ __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
 #pragma unroll
       for (int k = 0; k < D * D; k++) { a.F[k * a.n + p] = Fn[k]; a.B[k * a.n + p] = Bs[k]; }
       if (!ok) a.valid[p] = 0;
-      a.keys[p] = ok ? tile_key<T, D>(a.g, xn) : INVALID_KEY;
+      a.keys[p] = ok ? bin_key<T, D>(a.g, xn) : INVALID_KEY;
     }
     __syncthreads();
   }
@@ -768,7 +860,7 @@ __global__ __launch_bounds__(BLOCK) void stress_kernel(uint32_t n, const T* F, c
   T Fm[D * D], t[D * D];
 #pragma unroll
   for (int k = 0; k < D * D; k++) Fm[k] = F[(size_t)i * D * D + k];
-  kirchhoff<T, D>(model, mu[i], la[i], thr, root, vol[i], Fm, t);
+  kirchhoff_fast<T, D>(model, mu[i], la[i], thr, root, vol[i], Fm, t);
 #pragma unroll
   for (int k = 0; k < D * D; k++) tau[(size_t)i * D * D + k] = t[k];
 }
@@ -902,6 +994,8 @@ class Engine final : public EngineBase {
     g_.ntiles = (int)nt;
     key_bits_ = 1;
     while ((1ull << key_bits_) < nt + 1) key_bits_++;
+    key_bits_ += 6;  // cell inside the tile
+    if (key_bits_ > 32) throw Error{MPMX_EINVAL, "grid too large for 32-bit binning keys"};
     for (int a = 0; a < D; a++) grav_[a] = 0;
     HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     tile_begin_.alloc(nt);
@@ -1277,6 +1371,7 @@ class Engine final : public EngineBase {
 
     a = make_args(dt);
     a.order = order;
+    a.skeys = order ? skeys_.p : keys_.p;
     hipEvent_t e1 = timer_.begin(stream_);
     p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
     timer_.end(PH_P2G, e1, stream_);
