@@ -46,7 +46,7 @@ extern "C" {
 #define MPMX_NEO_HOOKEAN 1     /* NEO_HOOKEAN.cpp:44-81 */
 
 /* Collision types, MPM_COLLISION_OBJECT<TV>::COLLISION_TYPE
-   (Hybrid_Methods/Collisions/MPM_COLLISION_OBJECT.h:21). */
+   (Hybrid_Methods/Collisions/MPM_COLLISION_OBJECT.h:19). */
 #define MPMX_STICK 0
 #define MPMX_SLIP 1
 #define MPMX_SEPARATE 2

@@ -1,0 +1,118 @@
+"""Writes the golden vectors under tests/golden/ from the PhysBAM reference.
+
+Runs the reference harness (oracle/_ref/ref_harness, built from the reference
+checkout by oracle/ref/Makefile) on small seeded scenes and component inputs,
+and stores inputs and outputs as compressed .npz fixtures.  Run in the
+authoring container:  python tests/golden/make_golden.py
+"""
+import json
+import os
+import struct
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.dirname(HERE))
+import mpmtest as mt  # noqa: E402
+import scene_io  # noqa: E402
+
+
+def step_cases():
+    sq = lambda X: (np.abs(X[:, 0] - 0.5) <= 0.2) & (np.abs(X[:, 1] - 0.6) <= 0.2)
+    sph = lambda X: ((X - np.array([0.5, 0.45, 0.5])) ** 2).sum(1) < 0.3 ** 2
+    cases = {}
+    # The 2D FP64 configuration: 64^2, 0.4x0.4 square at (0.5, 0.6), 4 jittered
+    # particles per cell, fixed-corotated E=1e4 nu=0.3, density 1000, v=0,
+    # gravity -9.8, sticky ground at y=0.05, dt=1e-4.
+    walls = mt.wall_boxes(2, [64, 64], [0, 0], [1, 1], flags=8, inset=0.05, type_=mt.STICK, friction=0.0)
+    cases["square2d_f64"] = (mt.make_scene(2, 64, prec=8, model=mt.COROTATED, shape=sq, walls=walls, dt=1e-4,
+                                           seed=1234), (1, 20, 200))
+    walls = mt.wall_boxes(3, [16] * 3, [0] * 3, [1] * 3, flags=63, inset=0.12, type_=mt.SLIP, friction=0.0)
+    cases["sphere3d_neo_slip_f32"] = (mt.make_scene(3, 16, prec=4, model=mt.NEO_HOOKEAN, shape=sph,
+                                                    v0=[0.3, -3.0, 0.2], vrand=1.0, brand=1e-3, frand=0.2,
+                                                    walls=walls, dt=1e-3, per_particle_E=True, seed=5), (1, 10))
+    walls = mt.wall_boxes(3, [16] * 3, [0] * 3, [1] * 3, flags=63, inset=0.12, type_=mt.SEPARATE, friction=0.3)
+    cases["sphere3d_coro_sep_fric_f64"] = (mt.make_scene(3, 16, prec=8, model=mt.COROTATED, shape=sph,
+                                                         v0=[0.3, -3.0, 0.2], vrand=1.0, brand=1e-3, frand=0.2,
+                                                         walls=walls, dt=1e-3, per_particle_E=True, seed=6), (1, 10))
+    # neo-Hookean with compressed particles (J < 0.25 takes the clamp branch)
+    walls = mt.wall_boxes(2, [32, 32], [0, 0], [1, 1], flags=15, inset=0.1, type_=mt.STICK, friction=0.0)
+    sc = mt.make_scene(2, 32, prec=8, model=mt.NEO_HOOKEAN, shape=sq, v0=[0.0, -1.0], walls=walls, dt=1e-4, seed=7)
+    sc["F"][::3] *= 0.45
+    cases["square2d_neo_clamp_f64"] = (sc, (1, 5))
+    return cases
+
+
+def main():
+    if not scene_io.ref_available():
+        sys.exit("reference harness missing: build it with oracle/ref/Makefile")
+    for name, (sc, dumps) in step_cases().items():
+        out = {}
+        for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+            out["in_" + k] = np.asarray(sc[k])
+        meta = {k: sc[k] for k in ("dim", "prec", "model", "N", "lo", "hi", "gravity", "dt", "neo_threshold")}
+        out["meta"] = np.array(json.dumps(meta, default=float))
+        out["walls"] = np.array(json.dumps(sc["walls"], default=float))
+        for j, n in enumerate(dumps):
+            r = scene_io.run_ref(sc, n)[-1]
+            keys = ["X", "V", "F", "B", "valid"]
+            if j == 0:  # grid state after the first step only, to keep the fixture small
+                keys += ["grid_mass", "grid_velocity", "grid_velocity_save", "grid_velocity_friction_save"]
+            for k in keys:
+                out[f"s{n}_{k}"] = r[k]
+        np.savez_compressed(os.path.join(HERE, f"step_{name}.npz"), **out)
+        print("wrote", name, sc["X"].shape[0], "particles, dumps", dumps)
+
+    rng = np.random.default_rng(2024)
+    comp = {}
+    for dim in (2, 3):
+        n = 512
+        A = np.tile(np.eye(dim).reshape(-1), (n, 1)) + 0.6 * (rng.random((n, dim * dim)) - 0.5)
+        A[n // 2:] = rng.normal(size=(n // 2, dim * dim)) * rng.choice([1e-3, 1.0, 30.0], size=(n // 2, 1))
+        special = [np.eye(dim), -np.eye(dim), np.diag([1.0] * (dim - 1) + [-1.0]), 2 * np.eye(dim),
+                   np.diag([3.0] + [1.0] * (dim - 2) + [1e-6]), np.zeros((dim, dim))]
+        sp = np.array([m.T.reshape(-1) for m in special])
+        A = np.vstack([sp, A])
+        for prec, dt in ((8, np.float64), (4, np.float32)):
+            Ain = A.astype(dt).astype(np.float64)
+            U, S, V = scene_io.ref_component(
+                "svd", [prec, dim], struct.pack("<q", len(Ain)) + Ain.tobytes(),
+                [(np.float64, Ain.size), (np.float64, len(Ain) * dim), (np.float64, Ain.size)])
+            comp[f"svd{dim}_p{prec}_A"] = Ain
+            comp[f"svd{dim}_p{prec}_U"] = U.reshape(-1, dim * dim)
+            comp[f"svd{dim}_p{prec}_S"] = S.reshape(-1, dim)
+            comp[f"svd{dim}_p{prec}_V"] = V.reshape(-1, dim * dim)
+        # P_hat(sigma): sorted singular values, some inverted, some compressed
+        m = 400
+        S = np.sort(rng.uniform(0.05, 1.8, (m, dim)), axis=1)[:, ::-1].copy()
+        S[::7, -1] *= -1
+        S[::5] *= 0.4
+        mu, la = mt.lame(rng.uniform(1e3, 1e5, m), rng.uniform(0.2, 0.4, m))
+        for model in (0, 1):
+            for prec, dt in ((8, np.float64), (4, np.float32)):
+                Sin, mui, lai = (x.astype(dt).astype(np.float64) for x in (S, mu, la))
+                (P,) = scene_io.ref_component("pstrain", [prec, dim, model, 0.25],
+                                              struct.pack("<q", m) + Sin.tobytes() + mui.tobytes() + lai.tobytes(),
+                                              [(np.float64, m * dim)])
+                comp[f"pstrain{dim}_m{model}_p{prec}_S"] = Sin
+                comp[f"pstrain{dim}_m{model}_p{prec}_mu"] = mui
+                comp[f"pstrain{dim}_m{model}_p{prec}_la"] = lai
+                comp[f"pstrain{dim}_m{model}_p{prec}_P"] = P.reshape(m, dim)
+        X = rng.uniform(0.0, 1.0, (300, dim))
+        for prec, dt in ((8, np.float64), (4, np.float32)):
+            Xin = X.astype(dt).astype(np.float64)
+            base, w, dw = scene_io.ref_component(
+                "weights", [prec, dim, 16, 0.0, 1.0], struct.pack("<q", len(Xin)) + Xin.tobytes(),
+                [(np.int32, len(Xin) * dim), (np.float64, len(Xin) * 3 * dim), (np.float64, len(Xin) * 3 * dim)])
+            comp[f"weights{dim}_p{prec}_X"] = Xin
+            comp[f"weights{dim}_p{prec}_base"] = base.reshape(-1, dim)
+            comp[f"weights{dim}_p{prec}_w"] = w.reshape(-1, 3, dim)
+            comp[f"weights{dim}_p{prec}_dw"] = dw.reshape(-1, 3, dim)
+    np.savez_compressed(os.path.join(HERE, "components.npz"), **comp)
+    print("wrote components")
+
+
+if __name__ == "__main__":
+    main()

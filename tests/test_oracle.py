@@ -1,0 +1,247 @@
+"""The oracle pinned against the reference and against the mathematics.
+
+CPU only.  Three kinds of checks:
+  * golden vectors written by the PhysBAM reference itself (tests/golden/,
+    made by make_golden.py): the oracle must reproduce them bit for bit,
+    since it restates the reference in the same operation order;
+  * the live reference harness, where it is built (authoring container);
+  * closed forms and invariants that a plausible mistake would break:
+    partition of unity and linear reproduction of the weights, SVD
+    reconstruction / rotations / ordering, P_hat = dPsi/dsigma by finite
+    differences of the textbook energies, mass and momentum conservation of
+    P2G, exact transfer of an affine velocity field.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import mpmtest as mt
+import scene_io
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def scene_from_fixture(z):
+    meta = json.loads(str(z["meta"]))
+    sc = dict(meta)
+    sc["walls"] = json.loads(str(z["walls"]))
+    for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+        sc[k] = z["in_" + k]
+    return sc
+
+
+STEP_FIXTURES = ["step_square2d_f64.npz", "step_sphere3d_neo_slip_f32.npz", "step_sphere3d_coro_sep_fric_f64.npz",
+                 "step_square2d_neo_clamp_f64.npz"]
+
+
+@pytest.mark.parametrize("fixture", STEP_FIXTURES)
+def test_oracle_reproduces_reference_steps_bit_exact(fixture):
+    z = load(fixture)
+    sc = scene_from_fixture(z)
+    steps = sorted({int(k[1:].split("_")[0]) for k in z.files if k.startswith("s") and k[1].isdigit()})
+    for n in steps:
+        orc = mt.oracle_run(sc, n, want_grid=True)
+        for k in ("X", "V", "F", "B", "valid"):
+            assert np.array_equal(orc[k].astype(np.float64) if k != "valid" else orc[k], z[f"s{n}_{k}"]), (fixture, n, k)
+        if f"s{n}_grid_mass" in z.files:
+            assert np.array_equal(orc["grid_mass"].astype(np.float64), z[f"s{n}_grid_mass"])
+            for k in ("grid_velocity", "grid_velocity_save", "grid_velocity_friction_save"):
+                assert np.array_equal(orc[k].astype(np.float64), z[f"s{n}_{k}"]), (fixture, n, k)
+
+
+def test_clamp_fixture_reaches_the_clamp_branch():
+    z = load("step_square2d_neo_clamp_f64.npz")
+    F = z["in_F"].reshape(-1, 2, 2)
+    assert (np.linalg.det(F) < 0.25).sum() > 50
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("prec", [8, 4])
+def test_oracle_svd_matches_reference(dim, prec):
+    z = load("components.npz")
+    A = z[f"svd{dim}_p{prec}_A"]
+    U, S, V, _ = mt.oracle_svd(A, dim, prec=prec)
+    for got, key in ((U, "U"), (S, "S"), (V, "V")):
+        assert np.array_equal(got.astype(np.float64), z[f"svd{dim}_p{prec}_{key}"]), key
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_oracle_svd_properties(dim):
+    rng = np.random.default_rng(1)
+    n = 2000
+    A = rng.normal(size=(n, dim * dim)) * rng.choice([1e-4, 1.0, 50.0], size=(n, 1))
+    U, S, V, it = mt.oracle_svd(A, dim)
+    Um = U.reshape(n, dim, dim).transpose(0, 2, 1)
+    Vm = V.reshape(n, dim, dim).transpose(0, 2, 1)
+    Am = A.reshape(n, dim, dim).transpose(0, 2, 1)
+    rec = Um @ (S[:, :, None] * Vm.transpose(0, 2, 1))
+    # The reference deflates with tau = 16 eps max(|B|_F / 2, 1) (Core/Matrices/MATRIX_3X3.cpp:213-214),
+    # an absolute floor for small matrices, so errors scale with max(|A|, 1).
+    scale = np.maximum(np.abs(Am).max(axis=(1, 2)), 1.0)
+    assert (np.abs(rec - Am).max(axis=(1, 2)) <= 1e-12 * scale).all()
+    assert np.abs(Um @ Um.transpose(0, 2, 1) - np.eye(dim)).max() < 1e-12
+    assert np.abs(np.linalg.det(Um) - 1).max() < 1e-12
+    assert np.abs(np.linalg.det(Vm) - 1).max() < 1e-12
+    a = np.abs(S)
+    for i in range(dim - 1):
+        assert (a[:, i] >= a[:, i + 1]).all()
+        assert (S[:, i] >= 0).all()
+    assert (np.abs(a - np.linalg.svd(Am, compute_uv=False)) <= 1e-12 * scale[:, None]).all()
+    assert (np.sign(S[:, -1]) == np.sign(np.linalg.det(Am))).all()
+    if dim == 3:
+        assert it.max() < 100
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("model", [0, 1])
+@pytest.mark.parametrize("prec", [8, 4])
+def test_oracle_p_hat_matches_reference(dim, model, prec):
+    z = load("components.npz")
+    k = f"pstrain{dim}_m{model}_p{prec}"
+    P = mt.oracle_pstrain(z[k + "_S"], z[k + "_mu"], z[k + "_la"], dim, model, 0.25, prec=prec)
+    assert np.array_equal(P.astype(np.float64), z[k + "_P"])
+
+
+def energy(S, mu, la, model, thr=0.25):
+    """Textbook energy densities: fixed corotated mu sum (s-1)^2 + la/2 (J-1)^2;
+    neo-Hookean mu/2 (sum s^2 - d) - mu ln J + la/2 (ln J)^2 (J >= threshold)."""
+    J = np.prod(S, axis=1)
+    if model == 0:
+        return mu * ((S - 1) ** 2).sum(1) + la / 2 * (J - 1) ** 2
+    d = S.shape[1]
+    lj = np.log(J)
+    return mu / 2 * ((S ** 2).sum(1) - d) - mu * lj + la / 2 * lj ** 2
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("model", [0, 1])
+def test_p_hat_is_energy_gradient(dim, model):
+    rng = np.random.default_rng(5)
+    n = 300
+    S = rng.uniform(0.7, 1.3, (n, dim))
+    mu, la = mt.lame(rng.uniform(1e3, 1e5, n), rng.uniform(0.2, 0.4, n))
+    P = mt.oracle_pstrain(S, mu, la, dim, model)
+    h = 1e-6
+    for i in range(dim):
+        Sp, Sm = S.copy(), S.copy()
+        Sp[:, i] += h
+        Sm[:, i] -= h
+        fd = (energy(Sp, mu, la, model) - energy(Sm, mu, la, model)) / (2 * h)
+        assert np.allclose(P[:, i], fd, rtol=1e-6, atol=1e-6 * (mu + la))
+
+
+def test_neo_hookean_clamp_is_continuous_at_threshold():
+    mu, la = mt.lame(1e4, 0.3)
+    # J = 0.25 exactly along the diagonal: both branches must agree there
+    s = 0.25 ** (1 / 3)
+    eps = 1e-9
+    S = np.array([[s + eps] * 3, [s - eps] * 3])
+    P = mt.oracle_pstrain(S, mu, la, 3, 1)
+    assert np.allclose(P[0], P[1], rtol=1e-6)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("prec", [8, 4])
+def test_oracle_weights_match_reference(dim, prec):
+    z = load("components.npz")
+    X = z[f"weights{dim}_p{prec}_X"]
+    base, w, dw = mt.oracle_weights(X, dim, 16, 0.0, 1.0, prec=prec)
+    assert np.array_equal(base, z[f"weights{dim}_p{prec}_base"])
+    assert np.array_equal(w.astype(np.float64), z[f"weights{dim}_p{prec}_w"])
+    assert np.array_equal(dw.astype(np.float64), z[f"weights{dim}_p{prec}_dw"])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_weights_partition_and_linear_reproduction(dim):
+    rng = np.random.default_rng(9)
+    N = 16
+    h = 1.0 / N
+    X = rng.uniform(0.1, 0.9, (500, dim))
+    base, w, dw = mt.oracle_weights(X, dim, N, 0.0, 1.0)
+    grids = np.stack(np.meshgrid(*[np.arange(3)] * dim, indexing="ij"), -1).reshape(-1, dim)
+    for p in range(len(X)):
+        W = np.prod([w[p, grids[:, a], a] for a in range(dim)], axis=0)
+        G = np.stack([dw[p, grids[:, j], j] * np.prod([w[p, grids[:, a], a] for a in range(dim) if a != j], axis=0)
+                      for j in range(dim)], axis=1)
+        xi = (base[p] + grids + 0.5) * h
+        d = xi - X[p]
+        assert abs(W.sum() - 1) < 1e-14
+        assert np.abs((W[:, None] * d).sum(0)).max() < 1e-15
+        assert np.abs(G.sum(0)).max() < 1e-12
+        assert np.abs(d.T @ G - np.eye(dim)).max() < 1e-12            # sum (x_i - x_p) grad w^T = I
+        assert np.allclose((W[:, None, None] * d[:, :, None] * d[:, None, :]).sum(0), np.eye(dim) * h * h / 4)
+
+
+def test_p2g_conserves_mass_and_momentum():
+    sq = lambda X: (np.abs(X[:, 0] - 0.5) < 0.2) & (np.abs(X[:, 1] - 0.5) < 0.2)
+    sc = mt.make_scene(2, 32, prec=8, shape=sq, vrand=2.0, brand=1e-3, gravity=[0.0, 0.0], dt=1e-4)
+    out = mt.oracle_run(sc, 1)
+    # grid mass after P2G equals particle mass; P2G momentum (velocity_save) equals particle momentum
+    assert np.isclose(out["grid_mass"].sum(), sc["mass"].sum(), rtol=1e-13)
+    gp = (out["grid_mass"][:, None] * out["grid_velocity_save"]).sum(0)
+    pp = (sc["mass"][:, None] * sc["V"]).sum(0)
+    assert np.allclose(gp, pp, rtol=1e-12, atol=1e-14)
+    # without walls or gravity internal forces cancel: the updated grid momentum is unchanged
+    gv = (out["grid_mass"][:, None] * out["grid_velocity"]).sum(0)
+    assert np.allclose(gv, pp, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_affine_velocity_field_is_transferred_exactly(dim):
+    """v(x) = v0 + A x carried by B = A D_p (D_p = h^2/4 I): P2G+G2P returns
+    v_p and grad v = A; with dt -> tiny and F = I, F becomes I + dt A."""
+    rng = np.random.default_rng(3)
+    N = 16
+    h = 1.0 / N
+    A = rng.normal(size=(dim, dim))
+    v0 = rng.normal(size=dim)
+    box = lambda X: (np.abs(X - 0.5) < 0.25).all(1)
+    sc = mt.make_scene(dim, N, prec=8, shape=box, gravity=[0.0] * dim, dt=1e-9, E=1e-30, nu=0.3)
+    X = sc["X"]
+    sc["V"] = v0 + X @ A.T
+    Bm = A * h * h / 4
+    sc["B"] = np.tile(Bm.T.reshape(-1), (len(X), 1))  # column-major
+    out = mt.oracle_run(sc, 1)
+    Xn = X + sc["dt"] * sc["V"]
+    assert np.allclose(out["V"], v0 + X @ A.T, rtol=0, atol=1e-10)
+    assert np.allclose(out["X"], Xn, rtol=0, atol=1e-14)
+    Fexp = (np.eye(dim) + sc["dt"] * A).T.reshape(-1)
+    assert np.allclose(out["F"], Fexp, rtol=0, atol=1e-14)
+    assert np.allclose(out["B"], sc["B"], rtol=1e-8, atol=1e-14)
+
+
+def test_free_fall_is_exact():
+    sq = lambda X: (np.abs(X[:, 0] - 0.5) < 0.1) & (np.abs(X[:, 1] - 0.6) < 0.1)
+    sc = mt.make_scene(2, 32, prec=8, shape=sq, dt=1e-3)
+    out = mt.oracle_run(sc, 5)
+    assert np.allclose(out["V"][:, 1], -9.8 * 5e-3, rtol=1e-12)
+    assert np.allclose(out["V"][:, 0], 0, atol=1e-14)
+
+
+def test_particles_leaving_the_domain_are_invalidated():
+    sq = lambda X: (np.abs(X[:, 0] - 0.5) < 0.1) & (np.abs(X[:, 1] - 0.5) < 0.1)
+    sc = mt.make_scene(2, 16, prec=8, shape=sq, gravity=[0.0, 0.0], dt=1e-3)
+    sc["X"][0] = [0.99999, 0.5]
+    sc["V"][0] = [100.0, 0.0]
+    sc["valid"][1] = 0
+    out = mt.oracle_run(sc, 1)
+    assert out["valid"][0] == 0 and out["valid"][1] == 0 and out["valid"][2:].all()
+    assert np.array_equal(out["X"][1], sc["X"][1])  # invalid particles are not simulated
+
+
+@pytest.mark.skipif(not scene_io.ref_available(), reason="reference harness not built here")
+def test_oracle_matches_live_reference_harness():
+    sph = lambda X: ((X - np.array([0.5, 0.5, 0.5])) ** 2).sum(1) < 0.3 ** 2
+    walls = mt.wall_boxes(3, [12] * 3, [0] * 3, [1] * 3, flags=63, inset=0.15, type_=mt.STICK, friction=0.0)
+    sc = mt.make_scene(3, 12, prec=4, model=mt.NEO_HOOKEAN, shape=sph, v0=[1.0, -2.0, 0.5], vrand=0.5,
+                       brand=1e-3, frand=0.3, walls=walls, dt=5e-4, seed=99)
+    ref = scene_io.run_ref(sc, 8)[-1]
+    orc = mt.oracle_run(sc, 8)
+    for k in ("X", "V", "F", "B"):
+        assert np.array_equal(orc[k].astype(np.float64), ref[k]), k
