@@ -353,7 +353,8 @@ __device__ __forceinline__ void kirchhoff(int model, T mu, T la, T thr, T root, 
 //   fixed corotated:       E = U diag(e) U^T by Jacobi; s_i = sqrt(1 + e_i);
 //                          tau = vol U diag(2 mu s_i (s_i - 1) + lambda (J-1) J) U^T
 // Inverted or nearly degenerate F (J <= 0 or s_min < 1e-2 s_max) and the
-// neo-Hookean clamp branch (J < t) go through the FP64 SVD of kirchhoff().
+// neo-Hookean clamp branch (J < t) are declined (return false) and go through
+// the FP64 SVD of kirchhoff_typed<double>().
 template<typename T> __device__ __forceinline__ T fast_rcp(T x) { return (T)1 / x; }
 template<> __device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template<typename T> __device__ __forceinline__ T fast_rsqrt(T x) { return (T)1 / sqrt(x); }
@@ -408,8 +409,9 @@ __device__ __forceinline__ void sym_eigen(T (&A)[D][D], T (&U)[D][D]) {
   }
 }
 
+// Returns false (tau untouched) when F needs the FP64 SVD path.
 template<typename T, int D>
-__device__ __forceinline__ void kirchhoff_fast(int model, T mu, T la, T thr, T root, T vol, const T* F, T* tau) {
+__device__ __forceinline__ bool kirchhoff_fast(int model, T mu, T la, T thr, T vol, const T* F, T* tau) {
   T G[D][D];
 #pragma unroll
   for (int i = 0; i < D; i++)
@@ -446,9 +448,9 @@ __device__ __forceinline__ void kirchhoff_fast(int model, T mu, T la, T thr, T r
       for (int i = 0; i < D; i++)
 #pragma unroll
         for (int j = 0; j < D; j++) tau[i + D * j] = vol * (mu * E[i][j] + (i == j ? c : (T)0));
-      return;
+      return true;
     }
-    slow = true;
+    return false;
   }
   if (!slow) {
     T U[D][D];
@@ -481,9 +483,15 @@ __device__ __forceinline__ void kirchhoff_fast(int model, T mu, T la, T thr, T r
           tau[i + D * j] = acc;
           tau[j + D * i] = acc;
         }
-      return;
+      return true;
     }
   }
+  return false;
+}
+
+// FP64 SVD path for F that the fast path declines.
+template<typename T, int D>
+__device__ __noinline__ void kirchhoff_slow(int model, T mu, T la, T thr, T root, T vol, const T* F, T* tau) {
   double Fd[D * D], taud[D * D];
   bool dd;
 #pragma unroll
@@ -491,6 +499,11 @@ __device__ __forceinline__ void kirchhoff_fast(int model, T mu, T la, T thr, T r
   kirchhoff_typed<double, D>(model, (double)mu, (double)la, (double)thr, (double)root, (double)vol, Fd, taud, dd);
 #pragma unroll
   for (int i = 0; i < D * D; i++) tau[i] = (T)taud[i];
+}
+
+template<typename T, int D>
+__device__ __forceinline__ void kirchhoff_any(int model, T mu, T la, T thr, T root, T vol, const T* F, T* tau) {
+  if (!kirchhoff_fast<T, D>(model, mu, la, thr, vol, F, tau)) kirchhoff_slow<T, D>(model, mu, la, thr, root, vol, F, tau);
 }
 
 // ---------------------------------------------------------------- walls ----

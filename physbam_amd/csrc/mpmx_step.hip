@@ -63,7 +63,8 @@ template<typename T, int D> struct Args {
   const uint32_t* tile_end;
   const uint32_t* ptiles;
   const uint32_t* gtiles;
-  const uint32_t* counts;  // [0] particle tiles, [1] grid tiles, [2] valid particles
+  uint32_t* counts;  // [0] particle tiles, [1] grid tiles, [2] valid particles, [3] slow-stress particles
+  uint32_t* slow_list;  // particles whose stress needs the FP64 SVD path
   Node4<T>* grid;
   Node4<T>* gridf;
   uint32_t* err;
@@ -199,17 +200,22 @@ __global__ __launch_bounds__(BLOCK) void zero_kernel(const uint32_t* gtiles, con
 // accumulated in LDS over the tile's stencil region and flushed to HBM with
 // one atomic per touched node and component.
 //
-// No LDS atomics (on gfx950 they cost ~40 plain LDS accesses each):
-//   phase A: thread t loads particle r+t of the round (coalesced), computes
-//            its stress and transfer coefficients and stores them in LDS;
-//   phase B: lane l of each wave owns cell l of the tile (64 cells, 64 lanes)
-//            and scatters that cell's particles into the wave's private copy
-//            of the region with ordered read-modify-writes.  The 64 lanes of
-//            one instruction address 64 distinct nodes, a compiler barrier
-//            keeps each read after the previous write in program order, LDS
-//            executes one wave's accesses in order, and no other wave writes
-//            the copy, so every update is race-free;
+// No LDS atomics (on gfx950 they cost ~40 plain LDS accesses each).  Round k
+// takes particles 4k..4k+3 of every cell of the tile (particles are sorted by
+// cell, so a cell is a contiguous range):
+//   phase A: thread t = 4c + j handles particle 4k+j of cell c: it computes
+//            the stress and transfer coefficients and stores them in LDS;
+//   phase B: lane l of wave w owns cell l (64 cells, 64 lanes) and scatters
+//            its particle 4k+w into the wave's private copy of the region
+//            with ordered read-modify-writes.  The 64 lanes of one instruction
+//            address 64 distinct nodes, a compiler barrier keeps each read
+//            after the previous write in program order, LDS executes one
+//            wave's accesses in order, and no other wave writes the copy, so
+//            every update is race-free;
 //   flush:   the four copies are summed per node and added to HBM.
+// Particles whose F needs the FP64 SVD (inverted, nearly degenerate, or in the
+// neo-Hookean clamp branch) are queued; p2g_slow_stress_kernel adds their
+// stress impulse afterwards.
 template<typename T, int D> struct P2GCoef {
   // m, mv[D], mC[D*D], Q (symmetric, D(D+1)/2), xe[D]
   static constexpr int NQ = D * (D + 1) / 2;
@@ -228,12 +234,14 @@ __global__ __launch_bounds__(BLOCK) void p2g_kernel(Args<T, D> a) {
   using T4 = typename Vec4Type<T>::type;
   __shared__ T4 s_reg[NW][RN];  // per-wave copies of the region: (m, p_x, p_y, p_z)
   __shared__ T s_c[C::N][BLOCK];
-  __shared__ uint32_t s_cb[64], s_ce[64];
+  __shared__ uint32_t s_cb[64], s_ce[64], s_max;
   const uint32_t ntile = a.counts[0];
   T dinv[D];
 #pragma unroll
   for (int c = 0; c < D; c++) dinv[c] = (T)4 * a.g.inv_dx[c] * a.g.inv_dx[c];  // D_p^-1 (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:152-158)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int a_cell = threadIdx.x >> 2, a_j = threadIdx.x & 3;  // phase A assignment
+  const int a_slot = a_j * 64 + a_cell;                         // = phase B slot (wave a_j, lane a_cell)
   int lcell[D];
   cell_offsets<D>((uint32_t)lane, lcell);
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
@@ -248,6 +256,7 @@ __global__ __launch_bounds__(BLOCK) void p2g_kernel(Args<T, D> a) {
       s_cb[threadIdx.x] = 0;
       s_ce[threadIdx.x] = 0;
     }
+    if (threadIdx.x == 0) s_max = 0;
     __syncthreads();
     const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
     for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
@@ -256,48 +265,55 @@ __global__ __launch_bounds__(BLOCK) void p2g_kernel(Args<T, D> a) {
       if (i == e - 1 || (a.skeys[i + 1] & 63u) != cl) s_ce[cl] = i + 1;
     }
     __syncthreads();
-    for (uint32_t r = b; r < e; r += BLOCK) {
+    if (threadIdx.x < 64) atomicMax(&s_max, s_ce[threadIdx.x] - s_cb[threadIdx.x]);
+    __syncthreads();
+    const uint32_t a_cb = s_cb[a_cell], a_cnt = s_ce[a_cell] - a_cb;
+    const uint32_t b_cb = s_cb[lane], b_cnt = s_ce[lane] - b_cb;
+    const uint32_t rounds = (s_max + NW - 1) / NW;
+    for (uint32_t k = 0; k < rounds; k++) {
       // ---- phase A
-      const uint32_t i = r + threadIdx.x;
-      if (i < e) {
+      const uint32_t ja = k * NW + a_j;
+      if (ja < a_cnt) {
+        const uint32_t i = a_cb + ja;
         const uint32_t p = a.order ? a.order[i] : i;
         T x[D], v[D], Bm[D * D], Fm[D * D];
 #pragma unroll
         for (int c = 0; c < D; c++) { x[c] = a.X[c * a.n + p]; v[c] = a.V[c * a.n + p]; }
 #pragma unroll
-        for (int k = 0; k < D * D; k++) { Bm[k] = a.B[k * a.n + p]; Fm[k] = a.F[k * a.n + p]; }
+        for (int q = 0; q < D * D; q++) { Bm[q] = a.B[q * a.n + p]; Fm[q] = a.F[q * a.n + p]; }
         const T m = a.mass[p];
         int cl[D];
-        cell_offsets<D>(a.skeys[i] & 63u, cl);
+        cell_offsets<D>((uint32_t)a_cell, cl);
 #pragma unroll
         for (int c = 0; c < D; c++) {
           const int bb = (int)tfloor<T>(((x[c] - a.g.dx[c]) - a.g.lo[c]) * a.g.inv_dx[c]);
           const T cen = a.g.lo[c] + ((T)bb + (T)0.5) * a.g.dx[c];
-          s_c[C::XE + c][threadIdx.x] = x[c] - cen;
+          s_c[C::XE + c][a_slot] = x[c] - cen;
           if (bb + 3 - org[c] != cl[c]) atomicOr(a.err, 1u);
         }
         T tau[D * D];
-        kirchhoff_fast<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
-        s_c[C::M][threadIdx.x] = m;
+        if (!kirchhoff_fast<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.vol[p], Fm, tau)) {
 #pragma unroll
-        for (int c = 0; c < D; c++) s_c[C::MV + c][threadIdx.x] = m * v[c];
+          for (int q = 0; q < D * D; q++) tau[q] = 0;
+          a.slow_list[atomicAdd(&a.counts[3], 1u)] = p;
+        }
+        s_c[C::M][a_slot] = m;
+#pragma unroll
+        for (int c = 0; c < D; c++) s_c[C::MV + c][a_slot] = m * v[c];
 #pragma unroll
         for (int j = 0; j < D; j++)
 #pragma unroll
-          for (int c = 0; c < D; c++) s_c[C::MC + c + D * j][threadIdx.x] = m * Bm[c + D * j] * dinv[j];
+          for (int c = 0; c < D; c++) s_c[C::MC + c + D * j][a_slot] = m * Bm[c + D * j] * dinv[j];
         int q = 0;
 #pragma unroll
         for (int c = 0; c < D; c++)
 #pragma unroll
-          for (int j = c; j < D; j++) s_c[C::Q + q++][threadIdx.x] = a.dt * tau[c + D * j];
+          for (int j = c; j < D; j++) s_c[C::Q + q++][a_slot] = a.dt * tau[c + D * j];
       }
       __syncthreads();
       // ---- phase B
-      const uint32_t cb = s_cb[lane], ce = s_ce[lane];
-      const uint32_t lo_i = cb > r ? cb : r;
-      const uint32_t hi_i = ce < r + BLOCK ? ce : r + BLOCK;
-      for (uint32_t q = lo_i + wv; q < hi_i; q += NW) {
-        const int sl = (int)(q - r);
+      if (k * NW + wv < b_cnt) {
+        const int sl = threadIdx.x;  // = wv * 64 + lane
         const T m = s_c[C::M][sl];
         T mv[D], mC[D][D], Q[D][D], w[3][D], dw[3][D], dz[3][D];
 #pragma unroll
@@ -408,6 +424,53 @@ __global__ __launch_bounds__(BLOCK) void p2g_kernel(Args<T, D> a) {
       }
     }
     __syncthreads();
+  }
+}
+
+// Stress impulse of the queued particles, FP64 SVD path, straight to HBM.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void p2g_slow_stress_kernel(Args<T, D> a) {
+  const uint32_t ns = a.counts[3];
+  for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < ns; q += gridDim.x * BLOCK) {
+    const uint32_t p = a.slow_list[q];
+    T x[D], Fm[D * D], tau[D * D];
+#pragma unroll
+    for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+#pragma unroll
+    for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
+    kirchhoff_slow<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
+    Wts<T, D> w;
+    weights<T, D>(a.g, x, w);
+    for (int k = 0; k < Tile<D>::K; k++) {
+      int o[D], nc[D], r = k;
+      T g[D];
+#pragma unroll
+      for (int c = D - 1; c >= 0; c--) {
+        o[c] = r % 3;
+        r /= 3;
+      }
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        nc[c] = w.base[c] + o[c];
+        T gg = w.dw[o[c]][c];
+#pragma unroll
+        for (int j = 0; j < D; j++)
+          if (j != c) gg *= w.w[o[j]][j];
+        g[c] = gg;
+      }
+      Node4<T>* nd = a.grid + node_slot<D>(nc, a.g.GT);
+      T f[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        T s = 0;
+#pragma unroll
+        for (int j = 0; j < D; j++) s += tau[c + D * j] * g[j];
+        f[c] = -a.dt * s;
+      }
+      atomic_add(&nd->x, f[0]);
+      atomic_add(&nd->y, f[1]);
+      if constexpr (D == 3) atomic_add(&nd->z, f[D - 1]);
+    }
   }
 }
 
@@ -860,7 +923,7 @@ __global__ __launch_bounds__(BLOCK) void stress_kernel(uint32_t n, const T* F, c
   T Fm[D * D], t[D * D];
 #pragma unroll
   for (int k = 0; k < D * D; k++) Fm[k] = F[(size_t)i * D * D + k];
-  kirchhoff_fast<T, D>(model, mu[i], la[i], thr, root, vol[i], Fm, t);
+  kirchhoff_any<T, D>(model, mu[i], la[i], thr, root, vol[i], Fm, t);
 #pragma unroll
   for (int k = 0; k < D * D; k++) tau[(size_t)i * D * D + k] = t[k];
 }
@@ -1265,6 +1328,8 @@ class Engine final : public EngineBase {
     sort_scratch_.alloc(sort_scratch_bytes((uint32_t)n, key_bits_));
     stage_.release();
     stage_.alloc(n * D * D);
+    slow_list_.release();
+    slow_list_.alloc(n);
   }
 
   void upload_soa(const T* host, T* dev, int comps) {
@@ -1305,6 +1370,7 @@ class Engine final : public EngineBase {
     a.ptiles = ptiles_.p;
     a.gtiles = gtiles_.p;
     a.counts = counts_.p;
+    a.slow_list = slow_list_.p;
     a.grid = grid_.p;
     a.gridf = gridf_.p;
     a.err = err_.p;
@@ -1373,7 +1439,9 @@ class Engine final : public EngineBase {
     a.order = order;
     a.skeys = order ? skeys_.p : keys_.p;
     hipEvent_t e1 = timer_.begin(stream_);
+    fill_u32_kernel<<<1, BLOCK, 0, stream_>>>(counts_.p + 3, 1, 0u);
     p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+    p2g_slow_stress_kernel<T, D><<<64, BLOCK, 0, stream_>>>(a);
     timer_.end(PH_P2G, e1, stream_);
 
     hipEvent_t e2 = timer_.begin(stream_);
@@ -1428,6 +1496,7 @@ class Engine final : public EngineBase {
   DevBuf<Node4<T>> grid_, gridf_;
   DevBuf<double> stats_partial_, stats_out_;
   DevBuf<T> stage_;
+  DevBuf<uint32_t> slow_list_;
   PhaseTimer timer_;
 };
 

@@ -1,7 +1,8 @@
 """The C++ host mirror (physbam_amd/host/MPM_AMD.h: MPM_EXAMPLE, MPM_PARTICLES,
 MPM_DRIVER over the C ABI) runs the 2D FP64 elastic-square configuration for
 its full 200 steps on the GPU; the result must match the FP64 oracle run from
-the same initial state to 1e-8 relative per quantity."""
+the same initial state to 1e-8 relative per quantity (B against
+max(|B|, |V| h), as it is round-off while the square is in free fall)."""
 import os
 import subprocess
 
@@ -54,7 +55,11 @@ def test_host_mirror_runs_square_scene_200_steps(tmp_path):
     assert 2500 < sc["X"].shape[0] < 2700
     orc = mt.oracle_run(sc, 200, want_grid=False)
     assert np.array_equal(run["valid"], orc["valid"])
+    h = 1.0 / 64
     for k in ("X", "V", "F", "B"):
-        err = np.abs(run[k] - orc[k]).max() / np.abs(orc[k]).max()
+        # B is velocity x length and round-off while the square falls freely:
+        # measured against max(|B|, |V| h)
+        scale = max(np.abs(orc[k]).max(), np.abs(orc["V"]).max() * h) if k == "B" else np.abs(orc[k]).max()
+        err = np.abs(run[k] - orc[k]).max() / scale
         print(k, f"{err:.2e}", flush=True)
         assert err < 1e-8, (k, err)
