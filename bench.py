@@ -122,6 +122,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the reference CPU timing")
     ap.add_argument("--reorder-interval", type=int, default=16)
+    ap.add_argument("--rebin-fraction", type=float, default=1.0 / 128,
+                    help="rebuild the bins when more than this fraction of particles left their binned tile")
+    ap.add_argument("--rebin-interval", type=int, default=0, help="also rebuild the bins every K steps (0: never)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -153,12 +156,14 @@ def main():
     ctx = make_context(P)
     del P
     ctx.set_reorder_interval(args.reorder_interval)
+    ctx.set_rebin_policy(args.rebin_fraction, args.rebin_interval)
     ctx.step_async(DT, args.warmup)
     ctx.synchronize()
     log(f"warm-up: {args.warmup} steps done")
 
     ctx.enable_timing(True)
     ctx.reset_timing()
+    rebins0, _ = ctx.bin_stats()
     ctx.synchronize()
     t1 = time.perf_counter()
     ctx.step_async(DT, args.steps)
@@ -167,8 +172,9 @@ def main():
     sec = t2 - t1
     log(f"timed: {args.steps} steps in {sec * 1e3:.2f} ms")
 
+    rebins1, movers_last = ctx.bin_stats()
     phases = {}
-    for ph in ("bin", "p2g", "grid", "g2p", "reorder", "step"):
+    for ph in ("rebin", "bin", "p2g", "grid", "g2p", "reorder", "step"):
         tot, cnt = ctx.phase_time(ph)
         phases[ph] = (tot, cnt)
     ctx.enable_timing(False)
@@ -212,7 +218,9 @@ def main():
         "config": {"workload": "12-box 3D APIC MPM, 256^3 grid, 8 ppc, fixed-corotated, sticky floor, dt 2e-5",
                    "particles": n, "valid_particles_after": nvalid, "active_nodes": nodes, "grid": [CELLS] * 3,
                    "box_seed": BOX_SEED, "seed": SEED, "parallelism": "single GPU",
-                   "reorder_interval": args.reorder_interval},
+                   "reorder_interval": args.reorder_interval, "rebin_fraction": args.rebin_fraction,
+                   "rebin_interval": args.rebin_interval, "rebins_timed": rebins1 - rebins0,
+                   "movers_last_step": movers_last},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": kbytes, "avg_launch_ms": avg_s * 1e3},

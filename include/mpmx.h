@@ -133,11 +133,25 @@ int mpmx_particle_stats(mpmx_ctx* ctx, double* mass, double* momentum, int64_t* 
 int64_t mpmx_grid_node_count(const mpmx_ctx* ctx);
 int mpmx_get_grid(mpmx_ctx* ctx, void* mass, void* velocity);
 
-/* Tuning: reorder particle storage by tile every `interval` steps (>=1). */
+/* Binning (replaces GATHER_SCATTER::Prepare_Scatter, Hybrid_Methods/
+   Iterators/GATHER_SCATTER.cpp:32-86).  Particles are binned by the grid tile
+   holding their stencil base; a particle that leaves its binned tile (a
+   "mover") is transferred on a slower path until the bins are rebuilt.  The
+   bins are rebuilt when the mover count two steps earlier exceeds
+   mover_fraction * n, or after max_interval steps (0 = no limit).  Defaults:
+   1/128 and 0.  Results do not depend on the policy beyond floating-point
+   summation order. */
+int mpmx_set_rebin_policy(mpmx_ctx* ctx, double mover_fraction, int max_interval);
+/* Bin rebuilds so far and the mover count of the last step. */
+int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers);
+
+/* Tuning: at a bin rebuild, reorder particle storage by tile if at least
+   `interval` steps (>=1) have passed since the last reorder. */
 int mpmx_set_reorder_interval(mpmx_ctx* ctx, int interval);
 
 /* Per-phase GPU timing with HIP events on the context stream.  Phases:
-   "bin" (sort + tile lists), "p2g", "grid", "g2p", "reorder", "step".
+   "rebin" (keys, sort, tile ranges; only on rebuild steps), "bin" (per-step
+   grid-tile list and clearing), "p2g", "grid", "g2p", "reorder", "step".
    mpmx_phase_time returns the accumulated milliseconds and launch count since
    the last reset. */
 int mpmx_enable_timing(mpmx_ctx* ctx, int enable);

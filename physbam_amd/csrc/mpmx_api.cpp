@@ -139,6 +139,16 @@ int mpmx_set_reorder_interval(mpmx_ctx* ctx, int interval) {
   return guard([&] { ctx->engine->set_reorder_interval(interval); });
 }
 
+int mpmx_set_rebin_policy(mpmx_ctx* ctx, double mover_fraction, int max_interval) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->set_rebin_policy(mover_fraction, max_interval); });
+}
+
+int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->bin_stats(rebins, movers); });
+}
+
 int mpmx_enable_timing(mpmx_ctx* ctx, int enable) {
   if (!ctx) return bad("null context");
   return guard([&] { ctx->engine->enable_timing(enable != 0); });

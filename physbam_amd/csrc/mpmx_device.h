@@ -62,12 +62,21 @@ template<typename T, int D> struct Wts {
   T w[3][D], dw[3][D];
 };
 
+// Stencil base along one axis in node coordinates without the ghost offset:
+// floor((x - dx - lo) / dx) (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:33-56 with
+// degree 2).  Every kernel that bins, classifies or weights a particle calls
+// this, so they all agree on its cell.
+template<typename T, int D>
+MPMX_HD int stencil_base(const Geom<T, D>& g, const T* X, int a) {
+  return (int)tfloor<T>(((X[a] - g.dx[a]) - g.lo[a]) * g.inv_dx[a]);
+}
+
 template<typename T, int D>
 MPMX_HD void weights(const Geom<T, D>& g, const T* X, Wts<T, D>& s) {
 #pragma unroll
   for (int a = 0; a < D; a++) {
     const T inv = g.inv_dx[a];
-    int b = (int)tfloor<T>(((X[a] - g.dx[a]) - g.lo[a]) * inv);
+    int b = stencil_base<T, D>(g, X, a);
     T cen = g.lo[a] + ((T)b + (T)0.5) * g.dx[a];
     T xe = X[a] - cen;
     T x = xe * inv;
@@ -83,21 +92,19 @@ MPMX_HD void weights(const Geom<T, D>& g, const T* X, Wts<T, D>& s) {
   }
 }
 
-// Binning key of a particle: (linear id of the tile holding its stencil base
-// node) * 64 + (index of the base node inside the tile, row-major), or
-// INVALID_KEY if the stencil would leave the ghosted grid.  Sorting by this key
-// groups particles by tile and, inside a tile, by base cell.
+// Binning key of a particle: linear id of the tile holding its stencil base
+// node, or INVALID_KEY if the stencil would leave the ghosted grid.  Sorting by
+// this key groups particles by tile.
 template<typename T, int D>
-MPMX_HD uint32_t bin_key(const Geom<T, D>& g, const T* X) {
-  uint32_t tile = 0, cell = 0;
+MPMX_HD uint32_t tile_key(const Geom<T, D>& g, const T* X) {
+  uint32_t tile = 0;
 #pragma unroll
   for (int a = 0; a < D; a++) {
-    int b = (int)tfloor<T>(((X[a] - g.dx[a]) - g.lo[a]) * g.inv_dx[a]) + 3;
+    const int b = stencil_base<T, D>(g, X, a) + 3;
     if (!(b >= 0 && b + 2 < g.G[a])) return INVALID_KEY;
     tile = tile * (uint32_t)g.GT[a] + (uint32_t)(b / Tile<D>::E);
-    cell = cell * (uint32_t)Tile<D>::E + (uint32_t)(b % Tile<D>::E);
   }
-  return (tile << 6) | cell;
+  return tile;
 }
 
 // Offsets of cell `l` (0..63) inside its tile, row-major.
@@ -350,7 +357,11 @@ __device__ __forceinline__ void kirchhoff(int model, T mu, T la, T thr, T root, 
 //   J - 1 = tr G + (sum of principal 2x2 minors of G) + det G,
 // which keeps full relative precision when F is close to a rotation.
 //   neo-Hookean (J >= t):  tau = vol (mu E + lambda ln J I)          (no SVD)
-//   fixed corotated:       E = U diag(e) U^T by Jacobi; s_i = sqrt(1 + e_i);
+//   fixed corotated:       tau = vol (2 mu ((I + E) - sqrt(I + E)) + lambda (J-1) J I)
+//                          (F F^T - sqrt(F F^T) = (F - R) F^T for the polar R);
+//     |E|_F <= r:          (1+x) - sqrt(1+x) by its Taylor series to x^7
+//                          (corot_series; truncation below 2e-8 relative);
+//     otherwise:           E = U diag(e) U^T by Jacobi; s_i = sqrt(1 + e_i);
 //                          tau = vol U diag(2 mu s_i (s_i - 1) + lambda (J-1) J) U^T
 // Inverted or nearly degenerate F (J <= 0 or s_min < 1e-2 s_max) and the
 // neo-Hookean clamp branch (J < t) are declined (return false) and go through
@@ -359,6 +370,8 @@ template<typename T> __device__ __forceinline__ T fast_rcp(T x) { return (T)1 / 
 template<> __device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template<typename T> __device__ __forceinline__ T fast_rsqrt(T x) { return (T)1 / sqrt(x); }
 template<> __device__ __forceinline__ float fast_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template<typename T> __device__ __forceinline__ T fast_sqrt(T x) { return sqrt(x); }
+template<> __device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template<typename T> __device__ __forceinline__ T tlog1p(T x);
 template<> __device__ __forceinline__ float tlog1p<float>(float x) { return log1pf(x); }
 template<> __device__ __forceinline__ double tlog1p<double>(double x) { return log1p(x); }
@@ -385,7 +398,7 @@ __device__ __forceinline__ void sym_eigen(T (&A)[D][D], T (&U)[D][D]) {
       const T apq = A[p][q];
       if (apq != (T)0) {
         const T theta = (A[q][q] - A[p][p]) * (T)0.5 * fast_rcp<T>(apq);
-        T t = fast_rcp<T>(tabs(theta) + tsqrt<T>((T)1 + theta * theta));
+        T t = fast_rcp<T>(tabs(theta) + fast_sqrt<T>((T)1 + theta * theta));
         t = theta < 0 ? -t : t;
         const T c = fast_rsqrt<T>((T)1 + t * t), s = t * c;
         A[p][p] -= t * apq;
@@ -407,6 +420,50 @@ __device__ __forceinline__ void sym_eigen(T (&A)[D][D], T (&U)[D][D]) {
       }
     }
   }
+}
+
+// Largest |E|_F for the series path: |c_8| r^8 / (1 - r) stays below the
+// precision of T relative to |E| / 2.
+template<typename T> struct SeriesRadius;
+template<> struct SeriesRadius<float> { static constexpr float r2 = 1.0f / 64; };
+template<> struct SeriesRadius<double> { static constexpr double r2 = 1.0 / 65536; };
+
+// (I + E) - sqrt(I + E) for symmetric E: Horner form of
+//   x/2 + x^2/8 - x^3/16 + 5x^4/128 - 7x^5/256 + 21x^6/1024 - 33x^7/2048,
+// the Taylor series of (1+x) - sqrt(1+x) (binomial series of sqrt(1+x)).
+// Every partial product is a polynomial in E, so it commutes with E and is
+// symmetric: only the upper triangle is formed.
+template<typename T, int D>
+__device__ __forceinline__ void corot_series(const T (&E)[D][D], T (&f)[D][D]) {
+  const T c[8] = {(T)0, (T)0.5, (T)0.125, (T)-0.0625, (T)(5.0 / 128), (T)(-7.0 / 256), (T)(21.0 / 1024),
+                  (T)(-33.0 / 2048)};
+  T P[D][D];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) P[i][j] = c[7] * E[i][j] + (i == j ? c[6] : (T)0);
+#pragma unroll
+  for (int k = 5; k >= 0; k--) {
+    T Q[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = i; j < D; j++) {
+        T s = k > 0 && i == j ? c[k] : (T)0;
+#pragma unroll
+        for (int q = 0; q < D; q++) s += E[i][q] * P[q][j];
+        Q[i][j] = s;
+        Q[j][i] = s;
+      }
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) P[i][j] = Q[i][j];
+  }
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) f[i][j] = P[i][j];
 }
 
 // Returns false (tau untouched) when F needs the FP64 SVD path.
@@ -453,6 +510,21 @@ __device__ __forceinline__ bool kirchhoff_fast(int model, T mu, T la, T thr, T v
     return false;
   }
   if (!slow) {
+    T r2 = 0;
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) r2 += E[i][j] * E[i][j];
+    if (r2 <= SeriesRadius<T>::r2) {
+      T f[D][D];
+      corot_series<T, D>(E, f);
+      const T pj = la * Jm1 * J, m2 = (T)2 * mu;
+#pragma unroll
+      for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) tau[i + D * j] = vol * (m2 * f[i][j] + (i == j ? pj : (T)0));
+      return true;
+    }
     T U[D][D];
     sym_eigen<T, D>(E, U);
     T emin = E[0][0], emax = E[0][0];

@@ -31,6 +31,8 @@ struct EngineBase {
   virtual int64_t grid_node_count() const = 0;
   virtual void get_grid(void* mass, void* velocity) = 0;
   virtual void set_reorder_interval(int interval) = 0;
+  virtual void set_rebin_policy(double mover_fraction, int max_interval) = 0;
+  virtual void bin_stats(int64_t* rebins, int64_t* movers) = 0;
   virtual void enable_timing(bool on) = 0;
   virtual void reset_timing() = 0;
   virtual bool phase_time(const char* phase, double* total_ms, int64_t* count) = 0;

@@ -42,6 +42,7 @@ namespace mpmx {
   } while (0)
 
 constexpr int BLOCK = 256;
+constexpr int EXTRA_BLOCKS = 256;  // grid of the mover / slow-path kernels (grid-stride)
 
 template<typename T, int D> struct Args {
   Geom<T, D> g;
@@ -58,13 +59,17 @@ template<typename T, int D> struct Args {
   uint32_t* keys;
   uint32_t n;
   const uint32_t* order;  // sorted position -> slot, null for identity
-  const uint32_t* skeys;  // binning keys in sorted order
   const uint32_t* tile_begin;
   const uint32_t* tile_end;
   const uint32_t* ptiles;
   const uint32_t* gtiles;
-  uint32_t* counts;  // [0] particle tiles, [1] grid tiles, [2] valid particles, [3] slow-stress particles
-  uint32_t* slow_list;  // particles whose stress needs the FP64 SVD path
+  // [0] binned tiles, [1] grid tiles, [3] slow-stress particles, [4] movers
+  uint32_t* counts;
+  uint32_t* slow_list;   // particles whose stress needs the FP64 SVD path
+  uint32_t* mover_list;  // particles whose stencil base left their binned tile
+  uint32_t* mover_tile;  // ... and that tile
+  uint32_t* mstamp;      // per tile: step stamp of the movers whose base is in it
+  uint32_t next_stamp;   // stamp of the next step (written by G2P)
   Node4<T>* grid;
   Node4<T>* gridf;
   uint32_t* err;
@@ -100,7 +105,21 @@ template<int D> __device__ __forceinline__ void region_offsets(int k, int* off) 
 
 template<typename T> __device__ __forceinline__ void atomic_add(T* p, T v) { atomicAdd(p, v); }
 
-// ----------------------------------------------------------------- keys ----
+// Tile holding the stencil base of x (node coordinates, ghost offset included).
+template<typename T, int D> __device__ __forceinline__ uint32_t base_tile(const Geom<T, D>& g, const T* x) {
+  uint32_t t = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) t = t * (uint32_t)g.GT[a] + (uint32_t)((stencil_base<T, D>(g, x, a) + 3) / Tile<D>::E);
+  return t;
+}
+
+// ----------------------------------------------------------------- bins ----
+// Particles are binned by the tile holding their stencil base.  Bins are
+// rebuilt (key -> stable radix sort -> tile ranges) only when enough particles
+// have left their binned tile; in between, such particles ("movers") are listed
+// by P2G each step and transferred through HBM by p2g_extra_kernel and
+// g2p_mover_kernel, and the tiles holding them are marked in mstamp so that the
+// grid tiles they touch are cleared and updated.
 template<typename T, int D>
 __global__ __launch_bounds__(BLOCK) void key_kernel(Args<T, D> a, uint32_t* range_err) {
   const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
@@ -110,7 +129,7 @@ __global__ __launch_bounds__(BLOCK) void key_kernel(Args<T, D> a, uint32_t* rang
     T x[D];
 #pragma unroll
     for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
-    key = bin_key<T, D>(a.g, x);
+    key = tile_key<T, D>(a.g, x);
     if (key == INVALID_KEY) atomicAdd(range_err, 1u);
   }
   a.keys[p] = key;
@@ -121,30 +140,41 @@ __global__ __launch_bounds__(BLOCK) void fill_u32_kernel(uint32_t* p, uint32_t n
   if (i < n) p[i] = v;
 }
 
-// Tile ranges of the sorted keys (tile = key >> 6).  A tile is occupied this step iff its stamp
-// equals the step stamp, so nothing has to be cleared between steps.
+// Tile ranges of the sorted keys.  A tile is binned iff its stamp equals the
+// current bin stamp, so nothing has to be cleared between rebuilds.
 __global__ __launch_bounds__(BLOCK) void bounds_kernel(const uint32_t* sk, uint32_t n, uint32_t* tb, uint32_t* te,
-                                                       uint32_t* stamp_arr, uint32_t stamp, uint32_t* counts) {
+                                                       uint32_t* stamp_arr, uint32_t stamp) {
   const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
-  const uint32_t k = sk[i];
-  if (k == INVALID_KEY) return;
-  const uint32_t t = k >> 6;
-  if (i == 0 || (sk[i - 1] >> 6) != t) {
+  const uint32_t t = sk[i];
+  if (t == INVALID_KEY) return;
+  if (i == 0 || sk[i - 1] != t) {
     tb[t] = i;
     stamp_arr[t] = stamp;
   }
-  const bool last = (i == n - 1) || (sk[i + 1] >> 6) != t;
-  if (last) te[t] = i + 1;
-  if (i == n - 1 || sk[i + 1] == INVALID_KEY) counts[2] = i + 1;
+  if (i == n - 1 || sk[i + 1] != t) te[t] = i + 1;
 }
 
-// pflag: tile holds particles; gflag: tile is touched by some occupied tile's
-// stencil region (the tile itself or a neighbour in the -x/-y/-z octant).
-template<int D>
-__global__ __launch_bounds__(BLOCK) void flags_kernel(const uint32_t* stamp_arr, uint32_t stamp, uint32_t ntiles,
-                                                      int GT0, int GT1, int GT2, uint32_t* pflag, uint32_t* gflag) {
+// pflag: tile is binned (holds a particle range).
+__global__ __launch_bounds__(BLOCK) void pflags_kernel(const uint32_t* bstamp, uint32_t stamp, uint32_t ntiles,
+                                                       uint32_t* pflag) {
   const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t < ntiles) pflag[t] = bstamp[t] == stamp ? 1u : 0u;
+}
+
+// gflag: tile is touched by the stencil of some particle, i.e. the tile itself
+// or a neighbour in its -x/-y/-z octant holds a particle's stencil base: a
+// binned tile or a tile marked by this step's movers.  Thread 0 also clears
+// this step's slow-stress and mover counters.
+template<int D>
+__global__ __launch_bounds__(BLOCK) void gflags_kernel(const uint32_t* bstamp, uint32_t bs, const uint32_t* mstamp,
+                                                       uint32_t ms, uint32_t ntiles, int GT0, int GT1, int GT2,
+                                                       uint32_t* gflag, uint32_t* counts) {
+  const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t == 0) {
+    counts[3] = 0;
+    counts[4] = 0;
+  }
   if (t >= ntiles) return;
   const int GT[3] = {GT0, GT1, GT2};
   int c[D];
@@ -154,7 +184,6 @@ __global__ __launch_bounds__(BLOCK) void flags_kernel(const uint32_t* stamp_arr,
     c[a] = (int)(r % (uint32_t)GT[a]);
     r /= (uint32_t)GT[a];
   }
-  pflag[t] = stamp_arr[t] == stamp ? 1u : 0u;
   uint32_t g = 0;
 #pragma unroll
   for (int m = 0; m < (1 << D); m++) {
@@ -166,18 +195,15 @@ __global__ __launch_bounds__(BLOCK) void flags_kernel(const uint32_t* stamp_arr,
       ok = ok && cc >= 0;
       q = q * (uint32_t)GT[a] + (uint32_t)(cc < 0 ? 0 : cc);
     }
-    if (ok && stamp_arr[q] == stamp) g = 1u;
+    if (ok && (bstamp[q] == bs || mstamp[q] == ms)) g = 1u;
   }
   gflag[t] = g;
 }
 
-__global__ __launch_bounds__(BLOCK) void lists_kernel(uint32_t ntiles, const uint32_t* pflag, const uint32_t* pscan,
-                                                      const uint32_t* gflag, const uint32_t* gscan, uint32_t* ptiles,
-                                                      uint32_t* gtiles) {
+__global__ __launch_bounds__(BLOCK) void list_kernel(uint32_t ntiles, const uint32_t* flag, const uint32_t* scan,
+                                                     uint32_t* list) {
   const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t >= ntiles) return;
-  if (pflag[t]) ptiles[pscan[t]] = t;
-  if (gflag[t]) gtiles[gscan[t]] = t;
+  if (t < ntiles && flag[t]) list[scan[t]] = t;
 }
 
 template<typename T, bool FRICTION>
@@ -194,15 +220,17 @@ __global__ __launch_bounds__(BLOCK) void zero_kernel(const uint32_t* gtiles, con
 }
 
 // ------------------------------------------------------------------ P2G ----
-// One workgroup per occupied tile (grid-stride over the tile list).  Mass and
+// One workgroup per binned tile (grid-stride over the tile list).  Mass and
 // momentum, including the stress impulse -dt * tau_p * grad w_ip of
 // MPM_FINITE_ELEMENTS::Add_Forces (Forces/MPM_FINITE_ELEMENTS.cpp:84-99), are
 // accumulated in LDS over the tile's stencil region and flushed to HBM with
 // one atomic per touched node and component.
 //
-// No LDS atomics (on gfx950 they cost ~40 plain LDS accesses each).  Round k
-// takes particles 4k..4k+3 of every cell of the tile (particles are sorted by
-// cell, so a cell is a contiguous range):
+// The tile's particles are taken in chunks of 512.  Each chunk is grouped by
+// base cell in LDS (wave-ballot ranks, stable), which also sets aside movers
+// (base outside the tile) for p2g_extra_kernel.  No LDS atomics (on gfx950
+// they cost ~40 plain LDS accesses each).  Round k takes particles 4k..4k+3 of
+// every cell:
 //   phase A: thread t = 4c + j handles particle 4k+j of cell c: it computes
 //            the stress and transfer coefficients and stores them in LDS;
 //   phase B: lane l of wave w owns cell l (64 cells, 64 lanes) and scatters
@@ -214,8 +242,8 @@ __global__ __launch_bounds__(BLOCK) void zero_kernel(const uint32_t* gtiles, con
 //            every update is race-free;
 //   flush:   the four copies are summed per node and added to HBM.
 // Particles whose F needs the FP64 SVD (inverted, nearly degenerate, or in the
-// neo-Hookean clamp branch) are queued; p2g_slow_stress_kernel adds their
-// stress impulse afterwards.
+// neo-Hookean clamp branch) are queued; p2g_extra_kernel adds their stress
+// impulse afterwards.
 template<typename T, int D> struct P2GCoef {
   // m, mv[D], mC[D*D], Q (symmetric, D(D+1)/2), xe[D]
   static constexpr int NQ = D * (D + 1) / 2;
@@ -228,13 +256,19 @@ template<> struct Vec4Type<float> { typedef float type __attribute__((ext_vector
 template<> struct Vec4Type<double> { typedef double type __attribute__((ext_vector_type(4))); };
 
 template<typename T, int D>
-__global__ __launch_bounds__(BLOCK) void p2g_kernel(Args<T, D> a) {
-  constexpr int R = Tile<D>::R, RN = Tile<D>::RN, NW = BLOCK / 64;
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void p2g_kernel(Args<T, D> a) {
+  constexpr int R = Tile<D>::R, RN = Tile<D>::RN, NW = BLOCK / 64, E = Tile<D>::E;
+  constexpr int JT = 2, CH = JT * BLOCK;  // particles per chunk
   using C = P2GCoef<T, D>;
   using T4 = typename Vec4Type<T>::type;
   __shared__ T4 s_reg[NW][RN];  // per-wave copies of the region: (m, p_x, p_y, p_z)
-  __shared__ T s_c[C::N][BLOCK];
-  __shared__ uint32_t s_cb[64], s_ce[64], s_max;
+  __shared__ union {
+    T c[C::N][BLOCK];            // phase A -> phase B coefficients
+    uint32_t wc[JT][NW][64];     // grouping: per (pass, wave) cell counts, then offsets
+  } s_u;
+  __shared__ uint32_t s_idx[CH];  // chunk's particle slots grouped by cell
+  __shared__ uint32_t s_cb[64], s_cn[64], s_max;
+  auto& s_c = s_u.c;
   const uint32_t ntile = a.counts[0];
   T dinv[D];
 #pragma unroll
@@ -242,6 +276,7 @@ __global__ __launch_bounds__(BLOCK) void p2g_kernel(Args<T, D> a) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int a_cell = threadIdx.x >> 2, a_j = threadIdx.x & 3;  // phase A assignment
   const int a_slot = a_j * 64 + a_cell;                         // = phase B slot (wave a_j, lane a_cell)
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
   int lcell[D];
   cell_offsets<D>((uint32_t)lane, lcell);
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
@@ -252,160 +287,249 @@ __global__ __launch_bounds__(BLOCK) void p2g_kernel(Args<T, D> a) {
       T4 z = {0, 0, 0, 0};
       s_reg[k / RN][k % RN] = z;
     }
-    if (threadIdx.x < 64) {
-      s_cb[threadIdx.x] = 0;
-      s_ce[threadIdx.x] = 0;
-    }
-    if (threadIdx.x == 0) s_max = 0;
-    __syncthreads();
-    const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
-    for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
-      const uint32_t cl = a.skeys[i] & 63u;
-      if (i == b || (a.skeys[i - 1] & 63u) != cl) s_cb[cl] = i;
-      if (i == e - 1 || (a.skeys[i + 1] & 63u) != cl) s_ce[cl] = i + 1;
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) atomicMax(&s_max, s_ce[threadIdx.x] - s_cb[threadIdx.x]);
-    __syncthreads();
-    const uint32_t a_cb = s_cb[a_cell], a_cnt = s_ce[a_cell] - a_cb;
-    const uint32_t b_cb = s_cb[lane], b_cnt = s_ce[lane] - b_cb;
-    const uint32_t rounds = (s_max + NW - 1) / NW;
-    for (uint32_t k = 0; k < rounds; k++) {
-      // ---- phase A
-      const uint32_t ja = k * NW + a_j;
-      if (ja < a_cnt) {
-        const uint32_t i = a_cb + ja;
-        const uint32_t p = a.order ? a.order[i] : i;
-        T x[D], v[D], Bm[D * D], Fm[D * D];
+    const uint32_t tb = a.tile_begin[t], te = a.tile_end[t];
+    for (uint32_t c0 = tb; c0 < te; c0 += CH) {
+      for (int k = threadIdx.x; k < JT * NW * 64; k += BLOCK) (&s_u.wc[0][0][0])[k] = 0;
+      __syncthreads();
+      // ---- group the chunk by base cell; movers go to the mover list
+      uint32_t cell_j[JT], rank_j[JT], slot_j[JT];
 #pragma unroll
-        for (int c = 0; c < D; c++) { x[c] = a.X[c * a.n + p]; v[c] = a.V[c * a.n + p]; }
+      for (int j = 0; j < JT; j++) {
+        const uint32_t i = c0 + j * BLOCK + threadIdx.x;
+        uint32_t cell = 64, p = 0;
+        if (i < te) {
+          p = a.order ? a.order[i] : i;
+          if (a.valid[p]) {
+            T x[D];
 #pragma unroll
-        for (int q = 0; q < D * D; q++) { Bm[q] = a.B[q * a.n + p]; Fm[q] = a.F[q * a.n + p]; }
-        const T m = a.mass[p];
-        int cl[D];
-        cell_offsets<D>((uint32_t)a_cell, cl);
+            for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+            bool in = true;
+            uint32_t cl = 0;
 #pragma unroll
-        for (int c = 0; c < D; c++) {
-          const int bb = (int)tfloor<T>(((x[c] - a.g.dx[c]) - a.g.lo[c]) * a.g.inv_dx[c]);
-          const T cen = a.g.lo[c] + ((T)bb + (T)0.5) * a.g.dx[c];
-          s_c[C::XE + c][a_slot] = x[c] - cen;
-          if (bb + 3 - org[c] != cl[c]) atomicOr(a.err, 1u);
+            for (int c = 0; c < D; c++) {
+              const int lb = stencil_base<T, D>(a.g, x, c) + 3 - org[c];
+              in = in && lb >= 0 && lb < E;
+              cl = cl * E + (uint32_t)(lb & (E - 1));
+            }
+            if (in) {
+              cell = cl;
+            } else {
+              const uint32_t q = atomicAdd(&a.counts[4], 1u);
+              a.mover_list[q] = p;
+              a.mover_tile[q] = t;
+            }
+          }
         }
-        T tau[D * D];
-        if (!kirchhoff_fast<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.vol[p], Fm, tau)) {
+        const bool act = cell < 64;
+        uint64_t m = __ballot(act);
 #pragma unroll
-          for (int q = 0; q < D * D; q++) tau[q] = 0;
-          a.slow_list[atomicAdd(&a.counts[3], 1u)] = p;
+        for (int bb = 0; bb < 6; bb++) {
+          const bool bit = (cell >> bb) & 1u;
+          const uint64_t bal = __ballot(bit);
+          m &= bit ? bal : ~bal;
         }
-        s_c[C::M][a_slot] = m;
-#pragma unroll
-        for (int c = 0; c < D; c++) s_c[C::MV + c][a_slot] = m * v[c];
-#pragma unroll
-        for (int j = 0; j < D; j++)
-#pragma unroll
-          for (int c = 0; c < D; c++) s_c[C::MC + c + D * j][a_slot] = m * Bm[c + D * j] * dinv[j];
-        int q = 0;
-#pragma unroll
-        for (int c = 0; c < D; c++)
-#pragma unroll
-          for (int j = c; j < D; j++) s_c[C::Q + q++][a_slot] = a.dt * tau[c + D * j];
+        rank_j[j] = (uint32_t)__popcll(m & lt_mask);
+        if (act && lane == __ffsll((unsigned long long)m) - 1) s_u.wc[j][wv][cell] = (uint32_t)__popcll(m);
+        cell_j[j] = cell;
+        slot_j[j] = p;
       }
       __syncthreads();
-      // ---- phase B
-      if (k * NW + wv < b_cnt) {
-        const int sl = threadIdx.x;  // = wv * 64 + lane
-        const T m = s_c[C::M][sl];
-        T mv[D], mC[D][D], Q[D][D], w[3][D], dw[3][D], dz[3][D];
+      if (threadIdx.x < 64) {  // wave 0: per-cell offsets over (pass, wave), then cell starts
+        uint32_t run = 0;
 #pragma unroll
-        for (int c = 0; c < D; c++) mv[c] = s_c[C::MV + c][sl];
+        for (int j = 0; j < JT; j++)
 #pragma unroll
-        for (int j = 0; j < D; j++)
+          for (int w = 0; w < NW; w++) {
+            const uint32_t v = s_u.wc[j][w][threadIdx.x];
+            s_u.wc[j][w][threadIdx.x] = run;
+            run += v;
+          }
+        uint32_t x = run, mx = run;
 #pragma unroll
-          for (int c = 0; c < D; c++) mC[c][j] = s_c[C::MC + c + D * j][sl];
-        {
-          int qq = 0;
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = __shfl_up(x, o, 64);
+          if (lane >= o) x += y;
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o, 64));
+        s_cb[threadIdx.x] = x - run;
+        s_cn[threadIdx.x] = run;
+        if (threadIdx.x == 0) s_max = mx;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < JT; j++)
+        if (cell_j[j] < 64) s_idx[s_cb[cell_j[j]] + s_u.wc[j][wv][cell_j[j]] + rank_j[j]] = slot_j[j];
+      __syncthreads();
+      const uint32_t a_cb = s_cb[a_cell], a_cnt = s_cn[a_cell];
+      const uint32_t b_cnt = s_cn[lane];
+      const uint32_t rounds = (s_max + NW - 1) / NW;
+      for (uint32_t k = 0; k < rounds; k++) {
+        // ---- phase A
+        const uint32_t ja = k * NW + a_j;
+        if (ja < a_cnt) {
+          const uint32_t p = s_idx[a_cb + ja];
+          T x[D], v[D], Bm[D * D], Fm[D * D];
+#pragma unroll
+          for (int c = 0; c < D; c++) { x[c] = a.X[c * a.n + p]; v[c] = a.V[c * a.n + p]; }
+#pragma unroll
+          for (int q = 0; q < D * D; q++) { Bm[q] = a.B[q * a.n + p]; Fm[q] = a.F[q * a.n + p]; }
+          const T m = a.mass[p];
+#pragma unroll
+          for (int c = 0; c < D; c++) {
+            const int bb = stencil_base<T, D>(a.g, x, c);
+            const T cen = a.g.lo[c] + ((T)bb + (T)0.5) * a.g.dx[c];
+            s_c[C::XE + c][a_slot] = x[c] - cen;
+          }
+          T tau[D * D];
+          if (!kirchhoff_fast<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.vol[p], Fm, tau)) {
+#pragma unroll
+            for (int q = 0; q < D * D; q++) tau[q] = 0;
+            a.slow_list[atomicAdd(&a.counts[3], 1u)] = p;
+          }
+          s_c[C::M][a_slot] = m;
+#pragma unroll
+          for (int c = 0; c < D; c++) s_c[C::MV + c][a_slot] = m * v[c];
+#pragma unroll
+          for (int j = 0; j < D; j++)
+#pragma unroll
+            for (int c = 0; c < D; c++) s_c[C::MC + c + D * j][a_slot] = m * Bm[c + D * j] * dinv[j];
+          int q = 0;
 #pragma unroll
           for (int c = 0; c < D; c++)
 #pragma unroll
-            for (int j = c; j < D; j++) {
-              const T qv = s_c[C::Q + qq++][sl];
-              Q[c][j] = qv;
-              Q[j][c] = qv;
+            for (int j = c; j < D; j++) s_c[C::Q + q++][a_slot] = a.dt * tau[c + D * j];
+        }
+        __syncthreads();
+        // ---- phase B
+        if (k * NW + wv < b_cnt) {
+          const int sl = threadIdx.x;  // = wv * 64 + lane
+          const T m = s_c[C::M][sl];
+          T mv[D], mC[D][D], Q[D][D], w[3][D], dw[3][D], dz[3][D];
+#pragma unroll
+          for (int c = 0; c < D; c++) mv[c] = s_c[C::MV + c][sl];
+#pragma unroll
+          for (int j = 0; j < D; j++)
+#pragma unroll
+            for (int c = 0; c < D; c++) mC[c][j] = s_c[C::MC + c + D * j][sl];
+          {
+            int qq = 0;
+#pragma unroll
+            for (int c = 0; c < D; c++)
+#pragma unroll
+              for (int j = c; j < D; j++) {
+                const T qv = s_c[C::Q + qq++][sl];
+                Q[c][j] = qv;
+                Q[j][c] = qv;
+              }
+          }
+#pragma unroll
+          for (int c = 0; c < D; c++) {
+            const T xe = s_c[C::XE + c][sl];
+            const T inv = a.g.inv_dx[c];
+            const T xx = xe * inv;
+            const T q0 = xx - (T)1.5, q1 = xx - (T)1, q2 = xx - (T)0.5;
+            w[0][c] = (T)0.5 * q0 * q0;
+            w[1][c] = (T)0.75 - q1 * q1;
+            w[2][c] = (T)0.5 * q2 * q2;
+            dw[0][c] = q0 * inv;
+            dw[1][c] = (T)(-2) * q1 * inv;
+            dw[2][c] = q2 * inv;
+#pragma unroll
+            for (int o = 0; o < 3; o++) dz[o][c] = (T)o * a.g.dx[c] - xe;
+          }
+          // Separable form: with per-axis factors w, dw and dz = o dx - xe,
+          //   m_i   += w_x w_y w_z m
+          //   p_i,c += w_x w_y w_z (mv_c + sum_j mC_cj dz_j) - sum_j Q_cj (grad w)_j
+          // is built axis by axis so that each node costs one FMA per value.
+          if constexpr (D == 3) {
+            const int base_li = (lcell[0] * R + lcell[1]) * R + lcell[2];
+            T wzd[3], wydz[3];
+#pragma unroll
+            for (int o = 0; o < 3; o++) {
+              wzd[o] = w[o][2] * dz[o][2];
+              wydz[o] = w[o][1] * dz[o][1];
             }
-        }
 #pragma unroll
-        for (int c = 0; c < D; c++) {
-          const T xe = s_c[C::XE + c][sl];
-          const T inv = a.g.inv_dx[c];
-          const T xx = xe * inv;
-          const T q0 = xx - (T)1.5, q1 = xx - (T)1, q2 = xx - (T)0.5;
-          w[0][c] = (T)0.5 * q0 * q0;
-          w[1][c] = (T)0.75 - q1 * q1;
-          w[2][c] = (T)0.5 * q2 * q2;
-          dw[0][c] = q0 * inv;
-          dw[1][c] = (T)(-2) * q1 * inv;
-          dw[2][c] = q2 * inv;
-#pragma unroll
-          for (int o = 0; o < 3; o++) dz[o][c] = (T)o * a.g.dx[c] - xe;
-        }
-        if constexpr (D == 3) {
-          const int base_li = (lcell[0] * R + lcell[1]) * R + lcell[2];
-#pragma unroll
-          for (int ox = 0; ox < 3; ox++) {
-            T A0[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) A0[c] = mv[c] + mC[c][0] * dz[ox][0];
-#pragma unroll
-            for (int oy = 0; oy < 3; oy++) {
-              T A1[3], Fxy[3];
-              const T wxy = w[ox][0] * w[oy][1];
-              const T gx = dw[ox][0] * w[oy][1], gy = w[ox][0] * dw[oy][1];
+            for (int ox = 0; ox < 3; ox++) {
+              const T wx = w[ox][0], dwx = dw[ox][0], dzx = dz[ox][0];
+              T ax[3], bx[3], dx_[3], ex[3], fx[3];
 #pragma unroll
               for (int c = 0; c < 3; c++) {
-                A1[c] = A0[c] + mC[c][1] * dz[oy][1];
-                Fxy[c] = Q[c][0] * gx + Q[c][1] * gy;
+                ax[c] = wx * (mv[c] + mC[c][0] * dzx) - Q[c][0] * dwx;  // coefficient of w_y w_z
+                bx[c] = wx * mC[c][1];                                  // of w_y dz_y w_z
+                dx_[c] = -wx * Q[c][1];                                 // of dw_y w_z
+                ex[c] = wx * mC[c][2];                                  // of w_y w_z dz_z
+                fx[c] = -wx * Q[c][2];                                  // of w_y dw_z
               }
+              const T mx = wx * m;
 #pragma unroll
-              for (int oz = 0; oz < 3; oz++) {
-                const T wz = w[oz][2];
-                const T wt = wxy * wz;
-                const T gz = wxy * dw[oz][2];
-                T4 add;
-                add.x = wt * m;
-                add.y = wt * (A1[0] + mC[0][2] * dz[oz][2]) - (Fxy[0] * wz + Q[0][2] * gz);
-                add.z = wt * (A1[1] + mC[1][2] * dz[oz][2]) - (Fxy[1] * wz + Q[1][2] * gz);
-                add.w = wt * (A1[2] + mC[2][2] * dz[oz][2]) - (Fxy[2] * wz + Q[2][2] * gz);
-                const int li = base_li + (ox * R + oy) * R + oz;
-                const T4 cur = s_reg[wv][li];
-                s_reg[wv][li] = cur + add;
+              for (int oy = 0; oy < 3; oy++) {
+                const T wy = w[oy][1], dwy = dw[oy][1], wyz = wydz[oy];
+                T P[3], S[3], Tz[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                  P[c] = wy * ax[c] + wyz * bx[c] + dwy * dx_[c];  // coefficient of w_z
+                  S[c] = wy * ex[c];                                // of w_z dz_z
+                  Tz[c] = wy * fx[c];                               // of dw_z
+                }
+                const T mxy = wy * mx;
+#pragma unroll
+                for (int oz = 0; oz < 3; oz++) {
+                  const T wz = w[oz][2], dwz = dw[oz][2], wzz = wzd[oz];
+                  const int li = base_li + (ox * R + oy) * R + oz;
+                  T4 cur = s_reg[wv][li];
+                  cur.x += wz * mxy;
+                  cur.y += wz * P[0];
+                  cur.z += wz * P[1];
+                  cur.w += wz * P[2];
+                  cur.y += wzz * S[0];
+                  cur.z += wzz * S[1];
+                  cur.w += wzz * S[2];
+                  cur.y += dwz * Tz[0];
+                  cur.z += dwz * Tz[1];
+                  cur.w += dwz * Tz[2];
+                  s_reg[wv][li] = cur;
+                  __asm__ __volatile__("" ::: "memory");
+                }
+              }
+            }
+          } else {
+            const int base_li = lcell[0] * R + lcell[D - 1];
+            T wyd[3];
+#pragma unroll
+            for (int o = 0; o < 3; o++) wyd[o] = w[o][D - 1] * dz[o][D - 1];
+#pragma unroll
+            for (int ox = 0; ox < 3; ox++) {
+              const T wx = w[ox][0], dwx = dw[ox][0], dzx = dz[ox][0];
+              T ax[2], ex[2], fx[2];
+#pragma unroll
+              for (int c = 0; c < 2; c++) {
+                ax[c] = wx * (mv[c] + mC[c][0] * dzx) - Q[c][0] * dwx;
+                ex[c] = wx * mC[c][D - 1];
+                fx[c] = -wx * Q[c][D - 1];
+              }
+              const T mx = wx * m;
+#pragma unroll
+              for (int oy = 0; oy < 3; oy++) {
+                const T wy = w[oy][D - 1], dwy = dw[oy][D - 1], wyz = wyd[oy];
+                const int li = base_li + ox * R + oy;
+                T4 cur = s_reg[wv][li];
+                cur.x += wy * mx;
+                cur.y += wy * ax[0];
+                cur.z += wy * ax[1];
+                cur.y += wyz * ex[0];
+                cur.z += wyz * ex[1];
+                cur.y += dwy * fx[0];
+                cur.z += dwy * fx[1];
+                s_reg[wv][li] = cur;
                 __asm__ __volatile__("" ::: "memory");
               }
             }
           }
-        } else {
-          const int base_li = lcell[0] * R + lcell[D - 1];
-#pragma unroll
-          for (int ox = 0; ox < 3; ox++) {
-#pragma unroll
-            for (int oy = 0; oy < 3; oy++) {
-              const T wt = w[ox][0] * w[oy][D - 1];
-              const T gx = dw[ox][0] * w[oy][D - 1], gy = w[ox][0] * dw[oy][D - 1];
-              T4 add;
-              add.x = wt * m;
-              add.y = wt * (mv[0] + mC[0][0] * dz[ox][0] + mC[0][D - 1] * dz[oy][D - 1]) -
-                      (Q[0][0] * gx + Q[0][D - 1] * gy);
-              add.z = wt * (mv[D - 1] + mC[D - 1][0] * dz[ox][0] + mC[D - 1][D - 1] * dz[oy][D - 1]) -
-                      (Q[D - 1][0] * gx + Q[D - 1][D - 1] * gy);
-              add.w = 0;
-              const int li = base_li + ox * R + oy;
-              const T4 cur = s_reg[wv][li];
-              s_reg[wv][li] = cur + add;
-              __asm__ __volatile__("" ::: "memory");
-            }
-          }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
     for (int k = threadIdx.x; k < RN; k += BLOCK) {
       T4 sum = s_reg[0][k];
@@ -427,31 +551,64 @@ __global__ __launch_bounds__(BLOCK) void p2g_kernel(Args<T, D> a) {
   }
 }
 
-// Stress impulse of the queued particles, FP64 SVD path, straight to HBM.
+// P2G of what the tile kernel leaves, straight to HBM with atomics:
+//   movers (list [0, counts[4])): the whole APIC transfer and stress impulse;
+//   queued particles (slow list [0, counts[3])): the stress impulse, FP64 SVD path.
 template<typename T, int D>
-__global__ __launch_bounds__(BLOCK) void p2g_slow_stress_kernel(Args<T, D> a) {
-  const uint32_t ns = a.counts[3];
-  for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < ns; q += gridDim.x * BLOCK) {
-    const uint32_t p = a.slow_list[q];
+__global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
+  const uint32_t nm = a.counts[4], ns = a.counts[3];
+  for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < nm + ns; q += gridDim.x * BLOCK) {
+    const bool mover = q < nm;
+    const uint32_t p = mover ? a.mover_list[q] : a.slow_list[q - nm];
     T x[D], Fm[D * D], tau[D * D];
 #pragma unroll
     for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
 #pragma unroll
     for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
-    kirchhoff_slow<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
     Wts<T, D> w;
     weights<T, D>(a.g, x, w);
+    bool in = true;
+#pragma unroll
+    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < a.g.G[c];
+    if (!in) {
+      atomicOr(a.err, 2u);
+      continue;
+    }
+    if (mover)
+      kirchhoff_any<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
+    else
+      kirchhoff_slow<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
+    T m = 0, mv[D], mC[D][D];
+#pragma unroll
+    for (int c = 0; c < D; c++) mv[c] = 0;
+#pragma unroll
+    for (int c = 0; c < D; c++)
+#pragma unroll
+      for (int j = 0; j < D; j++) mC[c][j] = 0;
+    if (mover) {
+      m = a.mass[p];
+#pragma unroll
+      for (int c = 0; c < D; c++) mv[c] = m * a.V[c * a.n + p];
+#pragma unroll
+      for (int j = 0; j < D; j++)
+#pragma unroll
+        for (int c = 0; c < D; c++)
+          mC[c][j] = m * a.B[(c + D * j) * a.n + p] * ((T)4 * a.g.inv_dx[j] * a.g.inv_dx[j]);
+    }
     for (int k = 0; k < Tile<D>::K; k++) {
       int o[D], nc[D], r = k;
-      T g[D];
+      T g[D], dz[D];
 #pragma unroll
       for (int c = D - 1; c >= 0; c--) {
         o[c] = r % 3;
         r /= 3;
       }
+      T wt = 1;
 #pragma unroll
       for (int c = 0; c < D; c++) {
         nc[c] = w.base[c] + o[c];
+        dz[c] = (T)o[c] * a.g.dx[c] - w.xe[c];
+        wt *= w.w[o[c]][c];
         T gg = w.dw[o[c]][c];
 #pragma unroll
         for (int j = 0; j < D; j++)
@@ -462,11 +619,15 @@ __global__ __launch_bounds__(BLOCK) void p2g_slow_stress_kernel(Args<T, D> a) {
       T f[D];
 #pragma unroll
       for (int c = 0; c < D; c++) {
-        T s = 0;
+        T s = 0, ap = mv[c];
 #pragma unroll
-        for (int j = 0; j < D; j++) s += tau[c + D * j] * g[j];
-        f[c] = -a.dt * s;
+        for (int j = 0; j < D; j++) {
+          s += tau[c + D * j] * g[j];
+          ap += mC[c][j] * dz[j];
+        }
+        f[c] = wt * ap - a.dt * s;
       }
+      if (mover) atomic_add(&nd->m, wt * m);
       atomic_add(&nd->x, f[0]);
       atomic_add(&nd->y, f[1]);
       if constexpr (D == 3) atomic_add(&nd->z, f[D - 1]);
@@ -634,14 +795,198 @@ __global__ void reduce_partials_kernel(const double* partial, int nblocks, doubl
 }
 
 // ------------------------------------------------------------------ G2P ----
-// Gathers v (and the friction velocity when walls have friction) from the
-// tile's stencil region staged in LDS, then updates F, x, v, B and the key.
-// B uses the algebraically equal, cancellation-free form of MPM_DRIVER.cpp:
-// 485, :504-505:  B = sum w vf (x_i - x_p)^T
-//                   + dt/2 [sum w (vf v^T - v vf^T) - (vf_p v_p^T - v_p vf_p^T)].
+// Gather of one particle from node velocities nv(ox, oy, oz, c) and friction
+// velocities nf(...) (the same values without friction): v (vb), the
+// friction-save velocity (vfb), grad v (G) and B (Bs), column-major.  B uses
+// the algebraically equal, cancellation-free form of MPM_DRIVER.cpp:485,
+// :504-505:  B = sum w vf (x_i - x_p)^T
+//              + dt/2 [sum w (vf v^T - v vf^T) - (vf_p v_p^T - v_p vf_p^T)].
+template<typename T, int D, bool FRICTION, class NV, class NF>
+__device__ __forceinline__ void g2p_gather(const Geom<T, D>& gm, T dt, const Wts<T, D>& w, NV nv, NF nf, T* vb,
+                                           T* vfb, T* G, T* Bs) {
+  T dz[3][D];
+#pragma unroll
+  for (int o = 0; o < 3; o++)
+#pragma unroll
+    for (int c = 0; c < D; c++) dz[o][c] = (T)o * gm.dx[c] - w.xe[c];
+  T An[D * D];
+#pragma unroll
+  for (int c = 0; c < D; c++) { vb[c] = 0; vfb[c] = 0; }
+#pragma unroll
+  for (int k = 0; k < D * D; k++) { G[k] = 0; Bs[k] = 0; An[k] = 0; }
+  if constexpr (D == 3 && !FRICTION) {
+    // Without friction every sum factors by axis:
+    //   v_p = sum w_x w_y w_z v_i,   (grad v)_c0 = sum dw_x w_y w_z v_c, ...,
+    //   B_cj = sum w v_c dz_j,  dz_j = o_j dx - xe_j,
+    // so the z sums are formed per (x, y) column, the y sums per x plane.
+    T wzd[3], wyd[3], wxd[3];
+#pragma unroll
+    for (int o = 0; o < 3; o++) {
+      wzd[o] = w.w[o][2] * dz[o][2];
+      wyd[o] = w.w[o][1] * dz[o][1];
+      wxd[o] = w.w[o][0] * dz[o][0];
+    }
+#pragma unroll
+    for (int ox = 0; ox < 3; ox++) {
+      // y-level sums of this x plane, per component:
+      // Y00 = sum w_y Z0, Y10 = sum dw_y Z0, Y20 = sum w_y dz_y Z0, Y01 = sum w_y Z1, Y02 = sum w_y Z2
+      T Y00[3], Y10[3], Y20[3], Y01[3], Y02[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) Y00[c] = Y10[c] = Y20[c] = Y01[c] = Y02[c] = 0;
+#pragma unroll
+      for (int oy = 0; oy < 3; oy++) {
+        // z-level sums of this column: Z0 = sum w_z v, Z1 = sum dw_z v, Z2 = sum w_z dz_z v
+        T Z0[3], Z1[3], Z2[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) Z0[c] = Z1[c] = Z2[c] = 0;
+#pragma unroll
+        for (int oz = 0; oz < 3; oz++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const T vn = nv(ox, oy, oz, c);
+            Z0[c] += w.w[oz][2] * vn;
+            Z1[c] += w.dw[oz][2] * vn;
+            Z2[c] += wzd[oz] * vn;
+          }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          Y00[c] += w.w[oy][1] * Z0[c];
+          Y10[c] += w.dw[oy][1] * Z0[c];
+          Y20[c] += wyd[oy] * Z0[c];
+          Y01[c] += w.w[oy][1] * Z1[c];
+          Y02[c] += w.w[oy][1] * Z2[c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        vb[c] += w.w[ox][0] * Y00[c];
+        G[c + 0] += w.dw[ox][0] * Y00[c];
+        Bs[c + 0] += wxd[ox] * Y00[c];
+        G[c + 3] += w.w[ox][0] * Y10[c];
+        Bs[c + 3] += w.w[ox][0] * Y20[c];
+        G[c + 6] += w.w[ox][0] * Y01[c];
+        Bs[c + 6] += w.w[ox][0] * Y02[c];
+      }
+    }
+  } else if constexpr (D == 3) {
+#pragma unroll
+    for (int ox = 0; ox < 3; ox++)
+#pragma unroll
+      for (int oy = 0; oy < 3; oy++) {
+        const T wxy = w.w[ox][0] * w.w[oy][1];
+#pragma unroll
+        for (int oz = 0; oz < 3; oz++) {
+          const T wt = wxy * w.w[oz][2];
+          T g[3];
+          g[0] = w.dw[ox][0] * w.w[oy][1] * w.w[oz][2];
+          g[1] = w.w[ox][0] * w.dw[oy][1] * w.w[oz][2];
+          g[2] = wxy * w.dw[oz][2];
+          T vn[3], vf[3];
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            vn[c] = nv(ox, oy, oz, c);
+            vf[c] = nf(ox, oy, oz, c);
+          }
+          const T zz[3] = {dz[ox][0], dz[oy][1], dz[oz][2]};
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            vb[c] += wt * vn[c];
+            vfb[c] += wt * vf[c];
+            const T wvf = wt * vf[c];
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+              G[c + 3 * j] += vn[c] * g[j];
+              Bs[c + 3 * j] += wvf * zz[j];
+              An[c + 3 * j] += wt * (vf[c] * vn[j] - vn[c] * vf[j]);
+            }
+          }
+        }
+      }
+  } else {
+#pragma unroll
+    for (int ox = 0; ox < 3; ox++)
+#pragma unroll
+      for (int oy = 0; oy < 3; oy++) {
+        const T wt = w.w[ox][0] * w.w[oy][1];
+        T g[2];
+        g[0] = w.dw[ox][0] * w.w[oy][1];
+        g[1] = w.w[ox][0] * w.dw[oy][1];
+        T vn[2], vf[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          vn[c] = nv(ox, oy, 0, c);
+          vf[c] = FRICTION ? nf(ox, oy, 0, c) : vn[c];
+        }
+        const T zz[2] = {dz[ox][0], dz[oy][1]};
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          vb[c] += wt * vn[c];
+          if (FRICTION) vfb[c] += wt * vf[c];
+          const T wvf = wt * vf[c];
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            G[c + 2 * j] += vn[c] * g[j];
+            Bs[c + 2 * j] += wvf * zz[j];
+            if (FRICTION) An[c + 2 * j] += wt * (vf[c] * vn[j] - vn[c] * vf[j]);
+          }
+        }
+      }
+  }
+  if (!FRICTION) {
+#pragma unroll
+    for (int c = 0; c < D; c++) vfb[c] = vb[c];
+  } else {
+    const T h = (T)0.5 * dt;
+#pragma unroll
+    for (int c = 0; c < D; c++)
+#pragma unroll
+      for (int j = 0; j < D; j++) Bs[c + D * j] += h * (An[c + D * j] - (vfb[c] * vb[j] - vb[c] * vfb[j]));
+  }
+}
+
+// F <- (I + dt grad v) F, x <- x + dt v, v <- vf, B, and invalidation outside
+// the domain (MPM_DRIVER.cpp:450-515, :1219-1239).  A particle whose new
+// stencil base leaves its binned tile marks the tile it moves to, so that the
+// next step clears and updates the grid tiles around it.
+template<typename T, int D>
+__device__ __forceinline__ void g2p_store(const Args<T, D>& a, uint32_t p, const T* x, const T* vb, const T* vfb,
+                                          const T* G, const T* Bs, uint32_t bin_tile) {
+  T Fm[D * D], Fn[D * D];
+#pragma unroll
+  for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
+#pragma unroll
+  for (int j = 0; j < D; j++)
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      T s = Fm[c + D * j];
+#pragma unroll
+      for (int q = 0; q < D; q++) s += a.dt * G[c + D * q] * Fm[q + D * j];
+      Fn[c + D * j] = s;
+    }
+  T xn[D];
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    xn[c] = x[c] + a.dt * vb[c];
+    if (xn[c] < a.vlo[c] || xn[c] > a.vhi[c]) ok = false;
+  }
+#pragma unroll
+  for (int c = 0; c < D; c++) { a.X[c * a.n + p] = xn[c]; a.V[c * a.n + p] = vfb[c]; }
+#pragma unroll
+  for (int k = 0; k < D * D; k++) { a.F[k * a.n + p] = Fn[k]; a.B[k * a.n + p] = Bs[k]; }
+  if (!ok) {
+    a.valid[p] = 0;
+  } else {
+    const uint32_t nt = base_tile<T, D>(a.g, xn);
+    if (nt != bin_tile && nt < (uint32_t)a.g.ntiles) a.mstamp[nt] = a.next_stamp;
+  }
+}
+
+// One workgroup per binned tile: the tile's stencil region of v (and of the
+// friction velocity) is staged in LDS; movers are left to g2p_mover_kernel.
 template<typename T, int D, bool FRICTION>
 __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
-  constexpr int R = Tile<D>::R, RN = Tile<D>::RN;
+  constexpr int R = Tile<D>::R, RN = Tile<D>::RN, E = Tile<D>::E;
   __shared__ T s_v[D][RN];
   __shared__ T s_f[FRICTION ? D : 1][FRICTION ? RN : 1];
   const uint32_t ntile = a.counts[0];
@@ -670,128 +1015,67 @@ __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
     const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
     for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
       const uint32_t p = a.order ? a.order[i] : i;
-      T x[D], Fm[D * D];
+      if (!a.valid[p]) continue;
+      T x[D];
 #pragma unroll
       for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
-#pragma unroll
-      for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
       Wts<T, D> w;
       weights<T, D>(a.g, x, w);
+      bool in = true;
       int lb[D];
 #pragma unroll
-      for (int c = 0; c < D; c++) lb[c] = w.base[c] - org[c];
-      T dz[3][D];
-#pragma unroll
-      for (int o = 0; o < 3; o++)
-#pragma unroll
-        for (int c = 0; c < D; c++) dz[o][c] = (T)o * a.g.dx[c] - w.xe[c];
-      T vb[D], G[D * D], Bs[D * D], vfb[D], An[D * D];
-#pragma unroll
-      for (int c = 0; c < D; c++) { vb[c] = 0; vfb[c] = 0; }
-#pragma unroll
-      for (int k = 0; k < D * D; k++) { G[k] = 0; Bs[k] = 0; An[k] = 0; }
-      if constexpr (D == 3) {
-#pragma unroll
-        for (int ox = 0; ox < 3; ox++)
-#pragma unroll
-          for (int oy = 0; oy < 3; oy++) {
-            const T wxy = w.w[ox][0] * w.w[oy][1];
-#pragma unroll
-            for (int oz = 0; oz < 3; oz++) {
-              const T wt = wxy * w.w[oz][2];
-              T g[3];
-              g[0] = w.dw[ox][0] * w.w[oy][1] * w.w[oz][2];
-              g[1] = w.w[ox][0] * w.dw[oy][1] * w.w[oz][2];
-              g[2] = wxy * w.dw[oz][2];
-              const int li = ((lb[0] + ox) * R + (lb[1] + oy)) * R + (lb[2] + oz);
-              T vn[3], vf[3];
-#pragma unroll
-              for (int c = 0; c < 3; c++) {
-                vn[c] = s_v[c][li];
-                vf[c] = FRICTION ? s_f[FRICTION ? c : 0][FRICTION ? li : 0] : vn[c];
-              }
-              const T zz[3] = {dz[ox][0], dz[oy][1], dz[oz][2]};
-#pragma unroll
-              for (int c = 0; c < 3; c++) {
-                vb[c] += wt * vn[c];
-                if (FRICTION) vfb[c] += wt * vf[c];
-                const T wvf = wt * vf[c];
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                  G[c + 3 * j] += vn[c] * g[j];
-                  Bs[c + 3 * j] += wvf * zz[j];
-                  if (FRICTION) An[c + 3 * j] += wt * (vf[c] * vn[j] - vn[c] * vf[j]);
-                }
-              }
-            }
-          }
-      } else {
-#pragma unroll
-        for (int ox = 0; ox < 3; ox++)
-#pragma unroll
-          for (int oy = 0; oy < 3; oy++) {
-            const T wt = w.w[ox][0] * w.w[oy][1];
-            T g[2];
-            g[0] = w.dw[ox][0] * w.w[oy][1];
-            g[1] = w.w[ox][0] * w.dw[oy][1];
-            const int li = (lb[0] + ox) * R + (lb[1] + oy);
-            T vn[2], vf[2];
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-              vn[c] = s_v[c][li];
-              vf[c] = FRICTION ? s_f[FRICTION ? c : 0][FRICTION ? li : 0] : vn[c];
-            }
-            const T zz[2] = {dz[ox][0], dz[oy][1]};
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-              vb[c] += wt * vn[c];
-              if (FRICTION) vfb[c] += wt * vf[c];
-              const T wvf = wt * vf[c];
-#pragma unroll
-              for (int j = 0; j < 2; j++) {
-                G[c + 2 * j] += vn[c] * g[j];
-                Bs[c + 2 * j] += wvf * zz[j];
-                if (FRICTION) An[c + 2 * j] += wt * (vf[c] * vn[j] - vn[c] * vf[j]);
-              }
-            }
-          }
-      }
-      if (!FRICTION) {
-#pragma unroll
-        for (int c = 0; c < D; c++) vfb[c] = vb[c];
-      } else {
-        const T h = (T)0.5 * a.dt;
-#pragma unroll
-        for (int c = 0; c < D; c++)
-#pragma unroll
-          for (int j = 0; j < D; j++) Bs[c + D * j] += h * (An[c + D * j] - (vfb[c] * vb[j] - vb[c] * vfb[j]));
-      }
-      // F <- (I + dt grad v) F
-      T Fn[D * D];
-#pragma unroll
-      for (int j = 0; j < D; j++)
-#pragma unroll
-        for (int c = 0; c < D; c++) {
-          T s = Fm[c + D * j];
-#pragma unroll
-          for (int q = 0; q < D; q++) s += a.dt * G[c + D * q] * Fm[q + D * j];
-          Fn[c + D * j] = s;
-        }
-      T xn[D];
-      bool ok = true;
-#pragma unroll
       for (int c = 0; c < D; c++) {
-        xn[c] = x[c] + a.dt * vb[c];
-        if (xn[c] < a.vlo[c] || xn[c] > a.vhi[c]) ok = false;
+        lb[c] = w.base[c] - org[c];
+        in = in && lb[c] >= 0 && lb[c] < E;
       }
-#pragma unroll
-      for (int c = 0; c < D; c++) { a.X[c * a.n + p] = xn[c]; a.V[c * a.n + p] = vfb[c]; }
-#pragma unroll
-      for (int k = 0; k < D * D; k++) { a.F[k * a.n + p] = Fn[k]; a.B[k * a.n + p] = Bs[k]; }
-      if (!ok) a.valid[p] = 0;
-      a.keys[p] = ok ? bin_key<T, D>(a.g, xn) : INVALID_KEY;
+      if (!in) continue;  // mover
+      const int li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
+      auto nv = [&](int ox, int oy, int oz, int c) -> T {
+        return s_v[c][li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)];
+      };
+      auto nf = [&](int ox, int oy, int oz, int c) -> T {
+        if constexpr (FRICTION) return s_f[c][li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)];
+        else return nv(ox, oy, oz, c);
+      };
+      T vb[D], vfb[D], G[D * D], Bs[D * D];
+      g2p_gather<T, D, FRICTION>(a.g, a.dt, w, nv, nf, vb, vfb, G, Bs);
+      g2p_store<T, D>(a, p, x, vb, vfb, G, Bs, t);
     }
     __syncthreads();
+  }
+}
+
+// G2P of the movers listed by P2G this step, reading the grid from HBM.
+template<typename T, int D, bool FRICTION>
+__global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
+  const uint32_t nm = a.counts[4];
+  for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < nm; q += gridDim.x * BLOCK) {
+    const uint32_t p = a.mover_list[q];
+    T x[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+    Wts<T, D> w;
+    weights<T, D>(a.g, x, w);
+    bool in = true;
+#pragma unroll
+    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < a.g.G[c];
+    if (!in) continue;  // flagged by p2g_extra_kernel
+    auto slot = [&](int ox, int oy, int oz) -> uint32_t {
+      int nc[D];
+      nc[0] = w.base[0] + ox;
+      nc[1] = w.base[1] + oy;
+      if constexpr (D == 3) nc[D - 1] = w.base[D - 1] + oz;
+      return node_slot<D>(nc, a.g.GT);
+    };
+    auto comp = [](const Node4<T>& nd, int c) -> T { return c == 0 ? nd.x : (c == 1 ? nd.y : nd.z); };
+    auto nv = [&](int ox, int oy, int oz, int c) -> T { return comp(a.grid[slot(ox, oy, oz)], c); };
+    auto nf = [&](int ox, int oy, int oz, int c) -> T {
+      if constexpr (FRICTION) return comp(a.gridf[slot(ox, oy, oz)], c);
+      else return nv(ox, oy, oz, c);
+    };
+    T vb[D], vfb[D], G[D * D], Bs[D * D];
+    g2p_gather<T, D, FRICTION>(a.g, a.dt, w, nv, nf, vb, vfb, G, Bs);
+    g2p_store<T, D>(a, p, x, vb, vfb, G, Bs, a.mover_tile[q]);
   }
 }
 
@@ -984,8 +1268,9 @@ struct PhaseTimer {
   };
   std::vector<hipEvent_t> pool;
   std::vector<Pending> pending;
-  double total[6] = {0, 0, 0, 0, 0, 0};
-  int64_t count[6] = {0, 0, 0, 0, 0, 0};
+  static constexpr int NPH = 7;
+  double total[NPH] = {};
+  int64_t count[NPH] = {};
   bool on = false;
   hipEvent_t get() {
     if (pool.empty()) {
@@ -1029,8 +1314,8 @@ struct PhaseTimer {
     for (auto e : pool) (void)hipEventDestroy(e);
   }
 };
-enum { PH_BIN = 0, PH_P2G, PH_GRID, PH_G2P, PH_REORDER, PH_STEP };
-static const char* kPhaseNames[6] = {"bin", "p2g", "grid", "g2p", "reorder", "step"};
+enum { PH_BIN = 0, PH_P2G, PH_GRID, PH_G2P, PH_REORDER, PH_STEP, PH_REBIN };
+static const char* kPhaseNames[PhaseTimer::NPH] = {"bin", "p2g", "grid", "g2p", "reorder", "step", "rebin"};
 
 template<typename T, int D>
 class Engine final : public EngineBase {
@@ -1057,13 +1342,12 @@ class Engine final : public EngineBase {
     g_.ntiles = (int)nt;
     key_bits_ = 1;
     while ((1ull << key_bits_) < nt + 1) key_bits_++;
-    key_bits_ += 6;  // cell inside the tile
-    if (key_bits_ > 32) throw Error{MPMX_EINVAL, "grid too large for 32-bit binning keys"};
     for (int a = 0; a < D; a++) grav_[a] = 0;
     HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     tile_begin_.alloc(nt);
     tile_end_.alloc(nt);
     tile_stamp_.alloc(nt);
+    mstamp_.alloc(nt);
     pflag_.alloc(nt);
     gflag_.alloc(nt);
     pscan_.alloc(nt);
@@ -1074,6 +1358,13 @@ class Engine final : public EngineBase {
     err_.alloc(4);
     grid_.alloc(nt * 64);
     HIPCHECK(hipMemsetAsync(tile_stamp_.p, 0, nt * sizeof(uint32_t), stream_));
+    HIPCHECK(hipMemsetAsync(mstamp_.p, 0, nt * sizeof(uint32_t), stream_));
+    HIPCHECK(hipHostMalloc((void**)&rb_host_, 4 * sizeof(uint32_t), hipHostMallocDefault));
+    for (int i = 0; i < 4; i++) {
+      HIPCHECK(hipEventCreateWithFlags(&rb_ev_[i], hipEventDisableTiming));
+      rb_meta_[i] = Readback{-1, 0};
+      rb_host_[i] = 0;
+    }
     HIPCHECK(hipMemsetAsync(grid_.p, 0, nt * 64 * sizeof(Node4<T>), stream_));
     HIPCHECK(hipMemsetAsync(counts_.p, 0, 8 * sizeof(uint32_t), stream_));
     HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
@@ -1088,6 +1379,9 @@ class Engine final : public EngineBase {
       (void)hipStreamSynchronize(stream_);
       (void)hipStreamDestroy(stream_);
     }
+    for (int i = 0; i < 4; i++)
+      if (rb_ev_[i]) (void)hipEventDestroy(rb_ev_[i]);
+    if (rb_host_) (void)hipHostFree(rb_host_);
   }
 
   int dim() const override { return D; }
@@ -1124,7 +1418,7 @@ class Engine final : public EngineBase {
         throw Error{MPMX_ERANGE, std::to_string(errs[1]) + " valid particle(s) outside the ghosted grid"};
       }
     }
-    keys_valid_ = true;
+    force_rebin_ = true;
     force_reorder_ = true;
     has_grid_ = false;
   }
@@ -1183,7 +1477,7 @@ class Engine final : public EngineBase {
     timer_.resolve();
     uint32_t errs[4];
     HIPCHECK(hipMemcpy(errs, err_.p, sizeof(errs), hipMemcpyDeviceToHost));
-    if (errs[0]) throw Error{MPMX_EINTERNAL, "particle outside its binned tile (binning inconsistency)"};
+    if (errs[0]) throw Error{MPMX_EINTERNAL, "particle stencil outside the grid"};
   }
 
   void enable_grid_stats(bool on) override { stats_on_ = on; }
@@ -1239,20 +1533,33 @@ class Engine final : public EngineBase {
     if (k < 1) throw Error{MPMX_EINVAL, "reorder interval must be >= 1"};
     reorder_interval_ = k;
   }
+  void set_rebin_policy(double mover_fraction, int max_interval) override {
+    if (!(mover_fraction >= 0)) throw Error{MPMX_EINVAL, "mover fraction must be >= 0"};
+    if (max_interval < 0) throw Error{MPMX_EINVAL, "max interval must be >= 0"};
+    rebin_fraction_ = mover_fraction;
+    rebin_max_interval_ = max_interval;
+  }
+  void bin_stats(int64_t* rebins, int64_t* movers) override {
+    synchronize();
+    uint32_t c[8];
+    HIPCHECK(hipMemcpy(c, counts_.p, sizeof(c), hipMemcpyDeviceToHost));
+    if (rebins) *rebins = rebins_;
+    if (movers) *movers = has_grid_ ? (int64_t)c[4] : 0;
+  }
   void enable_timing(bool on) override {
     synchronize();
     timer_.on = on;
   }
   void reset_timing() override {
     synchronize();
-    for (int i = 0; i < 6; i++) {
+    for (int i = 0; i < PhaseTimer::NPH; i++) {
       timer_.total[i] = 0;
       timer_.count[i] = 0;
     }
   }
   bool phase_time(const char* phase, double* total_ms, int64_t* count) override {
     synchronize();
-    for (int i = 0; i < 6; i++)
+    for (int i = 0; i < PhaseTimer::NPH; i++)
       if (phase && std::strcmp(phase, kPhaseNames[i]) == 0) {
         if (total_ms) *total_ms = timer_.total[i];
         if (count) *count = timer_.count[i];
@@ -1330,6 +1637,10 @@ class Engine final : public EngineBase {
     stage_.alloc(n * D * D);
     slow_list_.release();
     slow_list_.alloc(n);
+    mover_list_.release();
+    mover_list_.alloc(n);
+    mover_tile_.release();
+    mover_tile_.alloc(n);
   }
 
   void upload_soa(const T* host, T* dev, int comps) {
@@ -1371,6 +1682,9 @@ class Engine final : public EngineBase {
     a.gtiles = gtiles_.p;
     a.counts = counts_.p;
     a.slow_list = slow_list_.p;
+    a.mover_list = mover_list_.p;
+    a.mover_tile = mover_tile_.p;
+    a.mstamp = mstamp_.p;
     a.grid = grid_.p;
     a.gridf = gridf_.p;
     a.err = err_.p;
@@ -1386,6 +1700,52 @@ class Engine final : public EngineBase {
   uint32_t tile_blocks() const { return (uint32_t)std::min<int>(std::max(g_.ntiles, 1), 8192); }
   uint32_t grid_blocks() const { return (uint32_t)stats_blocks_; }
 
+  // Rebuild the bins when forced, when max_interval steps have passed, or when
+  // the mover count two steps back (read without stalling the stream) exceeds
+  // mover_fraction * n.  Using a fixed lag keeps the decision independent of
+  // host/GPU timing.
+  bool rebin_due() {
+    if (force_rebin_) return true;
+    if (rebin_max_interval_ > 0 && steps_since_rebin_ >= rebin_max_interval_) return true;
+    const int64_t want = step_count_ - 2;
+    if (want < 0) return false;
+    const Readback& r = rb_meta_[want & 3];
+    if (r.step != want || r.gen != bin_stamp_) return false;
+    HIPCHECK(hipEventSynchronize(rb_ev_[want & 3]));
+    return (double)rb_host_[want & 3] > rebin_fraction_ * (double)n_;
+  }
+
+  void rebin() {
+    hipEvent_t e0 = timer_.begin(stream_);
+    Args<T, D> a = make_args(0);
+    key_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(a, err_.p + 1);
+    radix_sort_pairs(keys_.p, skeys_.p, order_.p, ktmp_.p, vtmp_.p, n_, key_bits_, sort_scratch_.p, stream_);
+    bin_stamp_++;
+    bounds_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(skeys_.p, n_, tile_begin_.p, tile_end_.p, tile_stamp_.p,
+                                                          bin_stamp_);
+    const uint32_t nt = (uint32_t)g_.ntiles;
+    pflags_kernel<<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, nt, pflag_.p);
+    scan_exclusive(pflag_.p, pscan_.p, nt, counts_.p + 0, scan_scratch_.p, stream_);
+    list_kernel<<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(nt, pflag_.p, pscan_.p, ptiles_.p);
+    timer_.end(PH_REBIN, e0, stream_);
+    identity_order_ = false;
+    if (force_reorder_ || since_reorder_ >= reorder_interval_) {
+      hipEvent_t er = timer_.begin(stream_);
+      reorder_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_, alt_, order_.p, n_);
+      std::swap(cur_, alt_);
+      std::swap(fa_, fb_);
+      std::swap(va_, vb_);
+      std::swap(ia_, ib_);
+      identity_order_ = true;
+      since_reorder_ = 0;
+      force_reorder_ = false;
+      timer_.end(PH_REORDER, er, stream_);
+    }
+    force_rebin_ = false;
+    steps_since_rebin_ = 0;
+    rebins_++;
+  }
+
   void step_once(T dt) {
     const bool fric = any_friction();
     if (fric) gridf_.alloc((size_t)g_.ntiles * 64);
@@ -1395,54 +1755,32 @@ class Engine final : public EngineBase {
       timer_.end(PH_STEP, e_step, stream_);
       return;
     }
+    if (rebin_due()) rebin();
     hipEvent_t e0 = timer_.begin(stream_);
-    Args<T, D> a = make_args(dt);
-    if (!keys_valid_) {
-      key_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(a, err_.p + 1);
-      keys_valid_ = true;
-    }
-    radix_sort_pairs(keys_.p, skeys_.p, order_.p, ktmp_.p, vtmp_.p, n_, key_bits_, sort_scratch_.p, stream_);
-    stamp_++;
-    fill_u32_kernel<<<1, BLOCK, 0, stream_>>>(counts_.p + 2, 1, 0u);
-    bounds_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(skeys_.p, n_, tile_begin_.p, tile_end_.p, tile_stamp_.p,
-                                                          stamp_, counts_.p);
-    timer_.end(PH_BIN, e0, stream_);
-    const uint32_t* order = order_.p;
-    if (force_reorder_ || ++since_reorder_ >= reorder_interval_) {
-      hipEvent_t er = timer_.begin(stream_);
-      reorder_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_, alt_, order_.p, n_);
-      std::swap(cur_, alt_);
-      std::swap(fa_, fb_);
-      std::swap(va_, vb_);
-      std::swap(ia_, ib_);
-      std::swap(keys_, skeys_);
-      order = nullptr;
-      since_reorder_ = 0;
-      force_reorder_ = false;
-      timer_.end(PH_REORDER, er, stream_);
-    }
-    e0 = timer_.begin(stream_);
     const uint32_t nt = (uint32_t)g_.ntiles;
-    flags_kernel<D><<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(tile_stamp_.p, stamp_, nt, g_.GT[0], D > 1 ? g_.GT[1] : 1,
-                                                            D > 2 ? g_.GT[D - 1] : 1, pflag_.p, gflag_.p);
-    scan_exclusive(pflag_.p, pscan_.p, nt, counts_.p + 0, scan_scratch_.p, stream_);
+    const uint32_t ms = (uint32_t)(step_count_ + 1);  // mover stamp of this step
+    gflags_kernel<D><<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, ms, nt, g_.GT[0],
+                                                             D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gflag_.p,
+                                                             counts_.p);
     scan_exclusive(gflag_.p, gscan_.p, nt, counts_.p + 1, scan_scratch_.p, stream_);
-    lists_kernel<<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(nt, pflag_.p, pscan_.p, gflag_.p, gscan_.p, ptiles_.p,
-                                                         gtiles_.p);
+    list_kernel<<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(nt, gflag_.p, gscan_.p, gtiles_.p);
     if (fric)
       zero_kernel<T, true><<<grid_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, grid_.p, gridf_.p);
     else
       zero_kernel<T, false><<<grid_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, grid_.p, gridf_.p);
     timer_.end(PH_BIN, e0, stream_);
 
-    a = make_args(dt);
-    a.order = order;
-    a.skeys = order ? skeys_.p : keys_.p;
+    Args<T, D> a = make_args(dt);
+    a.order = identity_order_ ? nullptr : order_.p;
+    a.next_stamp = ms + 1;
     hipEvent_t e1 = timer_.begin(stream_);
-    fill_u32_kernel<<<1, BLOCK, 0, stream_>>>(counts_.p + 3, 1, 0u);
     p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-    p2g_slow_stress_kernel<T, D><<<64, BLOCK, 0, stream_>>>(a);
+    p2g_extra_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
     timer_.end(PH_P2G, e1, stream_);
+    const int slot = (int)(step_count_ & 3);
+    HIPCHECK(hipMemcpyAsync(rb_host_ + slot, counts_.p + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipEventRecord(rb_ev_[slot], stream_));
+    rb_meta_[slot] = Readback{step_count_, bin_stamp_};
 
     hipEvent_t e2 = timer_.begin(stream_);
     a.stats_partial = stats_on_ ? stats_partial_.p : nullptr;
@@ -1455,11 +1793,17 @@ class Engine final : public EngineBase {
     timer_.end(PH_GRID, e2, stream_);
 
     hipEvent_t e3 = timer_.begin(stream_);
-    if (fric)
+    if (fric) {
       g2p_kernel<T, D, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-    else
+      g2p_mover_kernel<T, D, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+    } else {
       g2p_kernel<T, D, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+      g2p_mover_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+    }
     timer_.end(PH_G2P, e3, stream_);
+    step_count_++;
+    steps_since_rebin_++;
+    since_reorder_++;
     HIPCHECK(hipGetLastError());
     has_grid_ = true;
     time_ += (double)dt;
@@ -1481,7 +1825,20 @@ class Engine final : public EngineBase {
   int since_reorder_ = 0;
   int reorder_interval_ = 16;
   bool force_reorder_ = true;
-  bool keys_valid_ = false;
+  bool force_rebin_ = true;
+  bool identity_order_ = false;
+  uint32_t bin_stamp_ = 0;
+  int64_t step_count_ = 0, rebins_ = 0;
+  int steps_since_rebin_ = 0;
+  double rebin_fraction_ = 1.0 / 128;
+  int rebin_max_interval_ = 0;
+  struct Readback {
+    int64_t step;
+    uint32_t gen;
+  };
+  Readback rb_meta_[4];
+  hipEvent_t rb_ev_[4] = {};
+  uint32_t* rb_host_ = nullptr;
   bool has_grid_ = false;
   bool stats_on_ = false;
   double time_ = 0;
@@ -1492,11 +1849,12 @@ class Engine final : public EngineBase {
   DevBuf<uint32_t> ia_, ib_;
   DevBuf<uint32_t> keys_, skeys_, order_, ktmp_, vtmp_;
   DevBuf<char> sort_scratch_, scan_scratch_;
-  DevBuf<uint32_t> tile_begin_, tile_end_, tile_stamp_, pflag_, gflag_, pscan_, gscan_, ptiles_, gtiles_, counts_, err_;
+  DevBuf<uint32_t> tile_begin_, tile_end_, tile_stamp_, mstamp_, pflag_, gflag_, pscan_, gscan_, ptiles_, gtiles_,
+      counts_, err_;
   DevBuf<Node4<T>> grid_, gridf_;
   DevBuf<double> stats_partial_, stats_out_;
   DevBuf<T> stage_;
-  DevBuf<uint32_t> slow_list_;
+  DevBuf<uint32_t> slow_list_, mover_list_, mover_tile_;
   PhaseTimer timer_;
 };
 

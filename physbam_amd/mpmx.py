@@ -23,7 +23,8 @@ EXPORTS = (
     "mpmx_particle_count", "mpmx_set_constitutive", "mpmx_set_gravity", "mpmx_add_collision_box",
     "mpmx_clear_collision_objects", "mpmx_step", "mpmx_step_async", "mpmx_synchronize", "mpmx_time",
     "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_grid_node_count",
-    "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_enable_timing", "mpmx_reset_timing",
+    "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_set_rebin_policy", "mpmx_bin_stats",
+    "mpmx_enable_timing", "mpmx_reset_timing",
     "mpmx_phase_time", "mpmx_stress", "mpmx_svd", "mpmx_sort_keys", "mpmx_seed_particles",
     "mpmx_random_boxes",
 )
@@ -82,6 +83,8 @@ def lib():
         L.mpmx_grid_node_count.restype = i64
         L.mpmx_get_grid.argtypes = [vp, vp, vp]
         L.mpmx_set_reorder_interval.argtypes = [vp, c_int]
+        L.mpmx_set_rebin_policy.argtypes = [vp, dbl, c_int]
+        L.mpmx_bin_stats.argtypes = [vp, vp, vp]
         L.mpmx_enable_timing.argtypes = [vp, c_int]
         L.mpmx_reset_timing.argtypes = [vp]
         L.mpmx_phase_time.argtypes = [vp, ctypes.c_char_p, vp, vp]
@@ -216,6 +219,15 @@ class Context:
 
     def set_reorder_interval(self, k):
         _check(lib().mpmx_set_reorder_interval(self._h, k))
+
+    def set_rebin_policy(self, mover_fraction=1.0 / 128, max_interval=0):
+        _check(lib().mpmx_set_rebin_policy(self._h, float(mover_fraction), int(max_interval)))
+
+    def bin_stats(self):
+        """(bin rebuilds so far, movers in the last step)."""
+        r = np.zeros(2, np.int64)
+        _check(lib().mpmx_bin_stats(self._h, _p(r[:1]), _p(r[1:])))
+        return int(r[0]), int(r[1])
 
     def enable_timing(self, on=True):
         _check(lib().mpmx_enable_timing(self._h, 1 if on else 0))

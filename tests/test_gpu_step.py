@@ -114,6 +114,65 @@ def test_ten_steps_match_oracle(dim, prec, model, wtype, fric):
         print(tag, "fp32 oracle vs fp64 oracle", {k: f"{v:.2e}" for k, v in r.items()}, flush=True)
 
 
+# Bins are rebuilt only when enough particles have left their binned tile;
+# until then those particles go through the mover kernels.  Each policy must
+# reproduce the oracle: never rebuilding (every tile change is a mover),
+# rebuilding every step (no movers), and the default threshold.
+POLICIES = [("never", 2.0, 0), ("every_step", 1.0 / 128, 1), ("default", 1.0 / 128, 0)]
+
+
+def mover_scene(dim, prec, walls=True):
+    N = 32 if dim == 2 else 16
+    w = (mt.wall_boxes(dim, [N] * dim, [0] * dim, [1] * dim, flags=(1 << (2 * dim)) - 1, inset=0.12,
+                       type_=mt.SLIP, friction=0.0) if walls else [])
+    shape = sq2 if dim == 2 else sph3
+    return mt.make_scene(dim, N, prec=prec, model=mt.COROTATED, shape=shape, v0=[2.0, -5.0, 1.5][:dim],
+                         vrand=4.0, brand=1e-3, frand=0.1, walls=w, dt=1e-3, per_particle_E=True)
+
+
+@pytest.mark.parametrize("policy,frac,maxint", POLICIES)
+@pytest.mark.parametrize("dim,prec", [(2, 8), (3, 8), (3, 4)])
+def test_movers_between_rebins(dim, prec, policy, frac, maxint):
+    sc = mover_scene(dim, prec)
+    steps = 20
+    tag = f"movers {policy} d{dim} p{prec} n={sc['X'].shape[0]}"
+    print(tag, flush=True)
+    orc64 = mt.oracle_run(sc, steps, prec=8, want_grid=False)
+    with gpu_context(sc) as ctx:
+        ctx.set_rebin_policy(frac, maxint)
+        ctx.set_reorder_interval(4)
+        movers = 0
+        for _ in range(steps):
+            ctx.step(sc["dt"], 1)
+            movers += ctx.bin_stats()[1]
+        rebins = ctx.bin_stats()[0]
+        gpu = ctx.get_particles()
+    print(tag, "rebins", rebins, "mover-steps", movers, flush=True)
+    if policy == "never":
+        assert rebins == 1 and movers > 0
+    if policy == "every_step":
+        assert rebins == steps and movers == 0
+    compare(gpu, orc64, 1e-8 if prec == 8 else 1e-4, tag)
+
+
+@pytest.mark.parametrize("dim,prec", [(2, 8), (3, 4)])
+def test_particles_leaving_the_domain_between_rebins(dim, prec):
+    # no walls: the body falls through y = 0 and its particles are invalidated
+    # (MPM_DRIVER.cpp:1227-1238) while the bins are never rebuilt
+    sc = mover_scene(dim, prec, walls=False)
+    sc["X"][:, 1] -= 0.3 if dim == 2 else 0.25
+    steps = 25
+    orc64 = mt.oracle_run(sc, steps, prec=8, want_grid=False)
+    with gpu_context(sc) as ctx:
+        ctx.set_rebin_policy(2.0, 0)
+        ctx.step(sc["dt"], steps)
+        gpu = ctx.get_particles()
+    lost = int((gpu["valid"] == 0).sum())
+    print("leaving", dim, prec, "invalidated", lost, flush=True)
+    assert lost > 0
+    compare(gpu, orc64, 1e-8 if prec == 8 else 1e-4, f"leaving d{dim} p{prec}")
+
+
 def test_sort_is_stable_and_exact():
     rng = np.random.default_rng(7)
     with mpmx.Context(3, 8, [0, 0, 0], [1, 1, 1]) as ctx:
@@ -148,6 +207,42 @@ def test_svd_kernel():
     assert (S[:, :2] >= 0).all()
     _, So, _, _ = mt.oracle_svd(F, 3)
     assert np.abs(S - So).max() < 1e-12
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_stress_near_rotations(dim, prec):
+    # F = Q (I + e S): strains on both sides of the small-strain series radius
+    rng = np.random.default_rng(5)
+    per = 256
+    Fs = []
+    for e in (1e-7, 1e-4, 1e-2, 0.05, 0.1, 0.12, 0.13, 0.2, 0.4):
+        A = rng.standard_normal((per, dim, dim))
+        Q, _ = np.linalg.qr(A)
+        Q *= np.sign(np.linalg.det(Q))[:, None, None]
+        S = rng.standard_normal((per, dim, dim))
+        S = 0.5 * (S + S.transpose(0, 2, 1))
+        S /= np.linalg.norm(S, axis=(1, 2), keepdims=True)
+        M = Q @ (np.eye(dim) + e * S)
+        Fs.append(M.transpose(0, 2, 1).reshape(per, -1))  # column-major
+    F = np.concatenate(Fs)
+    n = F.shape[0]
+    vol = np.full(n, 1.5e-6)
+    mu, la = mt.lame(rng.uniform(1e3, 1e5, n), rng.uniform(0.2, 0.4, n))
+    dt = np.float32 if prec == 4 else np.float64
+    F, vol, mu, la = (np.asarray(x, dt).astype(np.float64) for x in (F, vol, mu, la))
+    with mpmx.Context(dim, 8, [0] * dim, [1] * dim, precision=prec) as ctx:
+        ctx.set_constitutive(mt.COROTATED, 0.25)
+        tau = ctx.stress(F, vol, mu, la)
+    ref = mt.oracle_stress(F, vol, mu, la, dim, mt.COROTATED, 0.25, prec=8)
+    # Near a rotation F - I is O(1) while the strain is tiny: rounding in F F^T
+    # is O(eps) in units of the modulus, so the scale has a floor of
+    # 5% of vol (2 mu + lambda) (errors below ~1e-6 of the modulus pass).
+    scale = np.maximum(np.abs(ref).max(axis=1), 0.05 * vol * (2 * mu + la))[:, None]
+    err = (np.abs(tau - ref) / scale).max(axis=1)
+    for i, e in enumerate((1e-7, 1e-4, 1e-2, 0.05, 0.1, 0.12, 0.13, 0.2, 0.4)):
+        print("stress near rotation", dim, prec, e, f"{err[i * per:(i + 1) * per].max():.2e}", flush=True)
+    assert err.max() < (1e-10 if prec == 8 else 2e-5)
 
 
 @pytest.mark.parametrize("prec", [4, 8])
