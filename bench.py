@@ -115,12 +115,59 @@ def cpu_baseline(shapes, log):
             "sample": f"one cube of the 12-box scene ({n} particles), 1 step of the serial oracle, FP32"}
 
 
+def adjoint_ratio(log, steps=200, checkpoint=20):
+    """Differentiable configuration: 64^3 grid, one 0.3^3 box of 8 ppc
+    (~56.6k particles), fixed corotated E 1e4, nu 0.3, gravity, dt 1e-4,
+    200 steps, checkpoint every 20, FP32; loss |mean x(T) - (0.7, 0.3, 0.5)|^2.
+    Returns the time of a taped forward + reverse pass over the time of a
+    plain forward pass of the same steps."""
+    cells, dt = 64, 1e-4
+    box = [dict(type=mpmx.SHAPE_BOX, a=[0.35] * 3, b=[0.65] * 3, density=1000.0, E=1e4, nu=0.3, v=[0.0] * 3)]
+    P = mpmx.seed_particles(3, cells, [0.0] * 3, [1.0] * 3, box, ppc_axis=2, seed=SEED, precision=mpmx.FP32)
+    n = int(P["X"].shape[0])
+    ctx = mpmx.Context(3, cells, [0.0] * 3, [1.0] * 3, precision=mpmx.FP32)
+    ctx.set_constitutive(mpmx.FIXED_COROTATED)
+    ctx.set_gravity([0.0, -9.8, 0.0])
+    target = np.array([0.7, 0.3, 0.5])
+
+    def load():
+        ctx.set_particles(P["X"], P["V"], P["F"], P["B"], P["mass"], P["volume"], P["mu"], P["lam"], P["valid"])
+
+    load()
+    ctx.step(dt, 20)  # warm-up
+    load()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(dt, steps)
+    t_fwd = time.perf_counter() - t0
+    load()
+    ctx.step(dt, 1)
+    load()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.tape_begin(checkpoint)
+    ctx.step(dt, steps)
+    X = ctx.get_particles()["X"].astype(np.float64)
+    dX = np.broadcast_to(2 * (X.mean(0) - target) / n, X.shape)
+    g = ctx.backward(dX=dX)
+    t_fa = time.perf_counter() - t0
+    ctx.close()
+    assert np.isfinite(g["V"]).all() and np.isfinite(g["mu"]).all()
+    log(f"adjoint: {n} particles, forward {t_fwd * 1e3:.1f} ms, forward+adjoint {t_fa * 1e3:.1f} ms")
+    return {"ratio": t_fa / t_fwd, "forward_ms_per_step": t_fwd * 1e3 / steps,
+            "forward_adjoint_ms_per_step": t_fa * 1e3 / steps, "particles": n, "steps": steps,
+            "checkpoint_every": checkpoint,
+            "workload": "64^3 grid, one 0.3^3 box at 8 ppc, fixed-corotated, gravity, dt 1e-4, FP32, "
+                        "loss |mean x(T) - (0.7, 0.3, 0.5)|^2, gradients w.r.t. v(0) and Lame parameters"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the reference CPU timing")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint / forward ratio")
     ap.add_argument("--reorder-interval", type=int, default=16)
     ap.add_argument("--rebin-fraction", type=float, default=1.0 / 128,
                     help="rebuild the bins when more than this fraction of particles left their binned tile")
@@ -231,6 +278,11 @@ def main():
         "adjoint_ratio": None,
         "cpu_baseline": None,
     }
+    if not args.no_adjoint:
+        try:
+            out["adjoint_ratio"] = adjoint_ratio(log)
+        except Exception as e:  # report, never hide the GPU number
+            log(f"adjoint ratio failed: {e}")
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(shapes, log)

@@ -149,9 +149,26 @@ int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers);
    `interval` steps (>=1) have passed since the last reorder. */
 int mpmx_set_reorder_interval(mpmx_ctx* ctx, int interval);
 
+/* Differentiation (no counterpart in the reference, whose step has no
+   adjoint).  mpmx_tape_begin records the current particle state and then a
+   checkpoint every `checkpoint_every` steps of the following mpmx_step calls.
+   mpmx_backward takes the adjoints of the final state (dL/dX, dL/dV, dL/dF,
+   dL/dB in the mpmx_set_particles layout; NULL = zero), runs the reverse pass
+   through every recorded step (recomputing each checkpoint segment forward),
+   and returns the adjoints of the state at mpmx_tape_begin (NULL outputs are
+   skipped) and dL/dmu, dL/dlambda per particle (double, accumulated in FP64).
+   The particle state afterwards is the final state again; the tape is
+   consumed.  Not supported: wall friction (rejected by mpmx_tape_begin) and
+   the neo-Hookean clamp branch (MPMX_EINVAL from mpmx_backward).  The
+   collision projection is differentiated with its face normal held fixed. */
+int mpmx_tape_begin(mpmx_ctx* ctx, int checkpoint_every);
+int mpmx_backward(mpmx_ctx* ctx, const void* dX, const void* dV, const void* dF, const void* dB, void* gX, void* gV,
+                  void* gF, void* gB, double* gmu, double* glambda);
+
 /* Per-phase GPU timing with HIP events on the context stream.  Phases:
    "rebin" (keys, sort, tile ranges; only on rebuild steps), "bin" (per-step
-   grid-tile list and clearing), "p2g", "grid", "g2p", "reorder", "step".
+   grid-tile list and clearing), "p2g", "grid", "g2p", "reorder", "step",
+   "adjoint" (a whole mpmx_backward pass).
    mpmx_phase_time returns the accumulated milliseconds and launch count since
    the last reset. */
 int mpmx_enable_timing(mpmx_ctx* ctx, int enable);

@@ -149,6 +149,17 @@ int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers) {
   return guard([&] { ctx->engine->bin_stats(rebins, movers); });
 }
 
+int mpmx_tape_begin(mpmx_ctx* ctx, int checkpoint_every) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->tape_begin(checkpoint_every); });
+}
+
+int mpmx_backward(mpmx_ctx* ctx, const void* dX, const void* dV, const void* dF, const void* dB, void* gX, void* gV,
+                  void* gF, void* gB, double* gmu, double* glambda) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->backward(dX, dV, dF, dB, gX, gV, gF, gB, gmu, glambda); });
+}
+
 int mpmx_enable_timing(mpmx_ctx* ctx, int enable) {
   if (!ctx) return bad("null context");
   return guard([&] { ctx->engine->enable_timing(enable != 0); });

@@ -561,16 +561,33 @@ __device__ __forceinline__ bool kirchhoff_fast(int model, T mu, T la, T thr, T v
   return false;
 }
 
-// FP64 SVD path for F that the fast path declines.
+// FP64 SVD path for F that the fast path declines.  Matrices travel by value
+// (in registers), so the out-of-line call needs no scratch memory.
+template<typename T, int D> struct MatD {
+  T v[D * D];
+};
+
 template<typename T, int D>
-__device__ __noinline__ void kirchhoff_slow(int model, T mu, T la, T thr, T root, T vol, const T* F, T* tau) {
+__device__ __noinline__ MatD<T, D> kirchhoff_slow_v(int model, T mu, T la, T thr, T root, T vol, MatD<T, D> F) {
   double Fd[D * D], taud[D * D];
   bool dd;
 #pragma unroll
-  for (int i = 0; i < D * D; i++) Fd[i] = (double)F[i];
+  for (int i = 0; i < D * D; i++) Fd[i] = (double)F.v[i];
   kirchhoff_typed<double, D>(model, (double)mu, (double)la, (double)thr, (double)root, (double)vol, Fd, taud, dd);
+  MatD<T, D> tau;
 #pragma unroll
-  for (int i = 0; i < D * D; i++) tau[i] = (T)taud[i];
+  for (int i = 0; i < D * D; i++) tau.v[i] = (T)taud[i];
+  return tau;
+}
+
+template<typename T, int D>
+__device__ __forceinline__ void kirchhoff_slow(int model, T mu, T la, T thr, T root, T vol, const T* F, T* tau) {
+  MatD<T, D> f;
+#pragma unroll
+  for (int i = 0; i < D * D; i++) f.v[i] = F[i];
+  const MatD<T, D> t = kirchhoff_slow_v<T, D>(model, mu, la, thr, root, vol, f);
+#pragma unroll
+  for (int i = 0; i < D * D; i++) tau[i] = t.v[i];
 }
 
 template<typename T, int D>

@@ -33,6 +33,9 @@ struct EngineBase {
   virtual void set_reorder_interval(int interval) = 0;
   virtual void set_rebin_policy(double mover_fraction, int max_interval) = 0;
   virtual void bin_stats(int64_t* rebins, int64_t* movers) = 0;
+  virtual void tape_begin(int checkpoint_every) = 0;
+  virtual void backward(const void* dX, const void* dV, const void* dF, const void* dB, void* gX, void* gV, void* gF,
+                        void* gB, double* gmu, double* glambda) = 0;
   virtual void enable_timing(bool on) = 0;
   virtual void reset_timing() = 0;
   virtual bool phase_time(const char* phase, double* total_ms, int64_t* count) = 0;

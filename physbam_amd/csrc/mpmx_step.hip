@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "mpmx.h"
+#include "mpmx_adjoint.h"
 #include "mpmx_device.h"
 #include "mpmx_engine.h"
 #include "mpmx_sort.h"
@@ -72,6 +73,7 @@ template<typename T, int D> struct Args {
   uint32_t next_stamp;   // stamp of the next step (written by G2P)
   Node4<T>* grid;
   Node4<T>* gridf;
+  Node4<T>* gridu;  // if set, the grid update also stores (m, v before the collision objects)
   uint32_t* err;
   double* stats_partial;
 };
@@ -256,7 +258,7 @@ template<> struct Vec4Type<float> { typedef float type __attribute__((ext_vector
 template<> struct Vec4Type<double> { typedef double type __attribute__((ext_vector_type(4))); };
 
 template<typename T, int D>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void p2g_kernel(Args<T, D> a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void p2g_kernel(Args<T, D> a) {
   constexpr int R = Tile<D>::R, RN = Tile<D>::RN, NW = BLOCK / 64, E = Tile<D>::E;
   constexpr int JT = 2, CH = JT * BLOCK;  // particles per chunk
   using C = P2GCoef<T, D>;
@@ -595,6 +597,7 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
         for (int c = 0; c < D; c++)
           mC[c][j] = m * a.B[(c + D * j) * a.n + p] * ((T)4 * a.g.inv_dx[j] * a.g.inv_dx[j]);
     }
+#pragma unroll
     for (int k = 0; k < Tile<D>::K; k++) {
       int o[D], nc[D], r = k;
       T g[D], dz[D];
@@ -635,6 +638,48 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
   }
 }
 
+// Collision objects in order on a node velocity (MPM_OBJECTIVE.cpp:150-182):
+// applies them to v and returns the number of hits; with want >= 0 it stops at
+// hit number `want` (before applying it) and returns its type, normal and
+// object.  Replaying instead of recording keeps per-hit arrays (which would
+// live in scratch memory) out of the kernels.
+template<typename T, int D>
+__device__ __forceinline__ int walls_apply(const Args<T, D>& a, const T* X0, T* v, int want, T* nrm_out, int* obj_out) {
+  int hits = 0;
+  const T inv_dt = (T)1 / a.dt;
+  for (int o = 0; o < a.nwalls; o++) {
+    const WallDev<T>& wl = a.walls[o];
+    T X[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) X[c] = X0[c] + a.dt * v[c];
+    const T phi0 = box_phi<T, D>(wl, X0);
+    const T phi = box_phi<T, D>(wl, X) - (phi0 < 0 ? phi0 : (T)0);
+    if (wl.type == 0 && phi0 > 0) continue;
+    if (wl.type == 1 && phi0 > 0) continue;
+    if (wl.type == 2 && phi > (T)1e-15) continue;
+    if (wl.type != 0 && phi > (T)3.402823466e38) continue;
+    T nrm[D];
+    box_normal<T, D>(wl, X, nrm);
+    if (hits == want) {
+#pragma unroll
+      for (int c = 0; c < D; c++) nrm_out[c] = nrm[c];
+      *obj_out = o;
+      return wl.type;
+    }
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      X[c] -= phi * nrm[c];
+      v[c] = (X[c] - X0[c]) * inv_dt;
+    }
+    if (wl.type == 0) {
+#pragma unroll
+      for (int c = 0; c < D; c++) v[c] = 0;
+    }
+    hits++;
+  }
+  return hits;
+}
+
 // ----------------------------------------------------------- grid update ---
 // v = p/m + dt g, then each collision object in order (MPM_OBJECTIVE.cpp:150-182),
 // then Coulomb friction at colliding nodes (MPM_DRIVER.cpp:1064-1094).
@@ -653,6 +698,14 @@ __global__ __launch_bounds__(BLOCK) void grid_kernel(Args<T, D> a) {
     v[0] = nd.x * im + a.dt * a.grav[0];
     v[1] = nd.y * im + a.dt * a.grav[1];
     if constexpr (D == 3) v[2] = nd.z * im + a.dt * a.grav[2];
+    if (a.gridu) {
+      Node4<T> u;
+      u.m = nd.m;
+      u.x = v[0];
+      u.y = v[1];
+      u.z = D == 3 ? v[D - 1] : (T)0;
+      a.gridu[slot] = u;
+    }
     T vf[D];
     if (a.nwalls > 0) {
       int org[D], nc[D];
@@ -669,66 +722,48 @@ __global__ __launch_bounds__(BLOCK) void grid_kernel(Args<T, D> a) {
       T vpre[D];
 #pragma unroll
       for (int c = 0; c < D; c++) vpre[c] = v[c];
-      bool stuck = false;
-      int ncol = 0;
-      T cn[FRICTION ? MAX_WALLS : 1][D];
-      int cobj[FRICTION ? MAX_WALLS : 1];
-      const T inv_dt = (T)1 / a.dt;
-      for (int o = 0; o < a.nwalls; o++) {
-        const WallDev<T>& wl = a.walls[o];
-        T X[D];
-#pragma unroll
-        for (int c = 0; c < D; c++) X[c] = X0[c] + a.dt * v[c];
-        const T phi0 = box_phi<T, D>(wl, X0);
-        const T phi = box_phi<T, D>(wl, X) - (phi0 < 0 ? phi0 : (T)0);
-        if (wl.type == 0 && phi0 > 0) continue;
-        if (wl.type == 1 && phi0 > 0) continue;
-        if (wl.type == 2 && phi > (T)1e-15) continue;
-        if (wl.type != 0 && phi > (T)3.402823466e38) continue;
-        T nrm[D];
-        box_normal<T, D>(wl, X, nrm);
-#pragma unroll
-        for (int c = 0; c < D; c++) {
-          X[c] -= phi * nrm[c];
-          v[c] = (X[c] - X0[c]) * inv_dt;
-        }
-        if (wl.type == 0) {
-#pragma unroll
-          for (int c = 0; c < D; c++) v[c] = 0;
-          stuck = true;
-        }
-        if constexpr (FRICTION) {
-#pragma unroll
-          for (int c = 0; c < D; c++) cn[ncol][c] = nrm[c];
-          cobj[ncol] = o;
-          ncol++;
-        }
-      }
+      T nd_[D];
+      int ob_;
+      const int ncol = walls_apply<T, D>(a, X0, v, -1, nd_, &ob_);
 #pragma unroll
       for (int c = 0; c < D; c++) vf[c] = v[c];
       if constexpr (FRICTION) {
+        // Apply_Friction (MPM_DRIVER.cpp:1064-1094) over this node's collisions in order
         if (ncol > 0) {
+          bool stuck = false;
+          for (int q = 0; q < ncol; q++) {
+            T vq[D], nq[D];
+            int oq;
+#pragma unroll
+            for (int c = 0; c < D; c++) vq[c] = vpre[c];
+            if (walls_apply<T, D>(a, X0, vq, q, nq, &oq) == 0) stuck = true;
+          }
           T v1[D];
 #pragma unroll
           for (int c = 0; c < D; c++) v1[c] = stuck ? (T)0 : v[c];
           for (int q = 0; q < ncol; q++) {
+            T vq[D], cn[D];
+            int oq;
+#pragma unroll
+            for (int c = 0; c < D; c++) vq[c] = vpre[c];
+            walls_apply<T, D>(a, X0, vq, q, cn, &oq);
             T nf = 0, vn = 0;
 #pragma unroll
             for (int c = 0; c < D; c++) {
-              nf += cn[q][c] * (v[c] - vpre[c]);
-              vn += v1[c] * cn[q][c];
+              nf += cn[c] * (v[c] - vpre[c]);
+              vn += v1[c] * cn[c];
             }
             T tv[D], tm = 0;
 #pragma unroll
             for (int c = 0; c < D; c++) {
-              tv[c] = v1[c] - vn * cn[q][c];
+              tv[c] = v1[c] - vn * cn[c];
               tm += tv[c] * tv[c];
             }
             tm = tsqrt<T>(tm);
-            const T k = a.walls[cobj[q]].friction * nf;
+            const T k = a.walls[oq].friction * nf;
             if (tm <= k) {
 #pragma unroll
-              for (int c = 0; c < D; c++) v1[c] = vn * cn[q][c];
+              for (int c = 0; c < D; c++) v1[c] = vn * cn[c];
             } else {
               const T it = tm > 0 ? (T)1 / tm : (T)0;
 #pragma unroll
@@ -736,11 +771,7 @@ __global__ __launch_bounds__(BLOCK) void grid_kernel(Args<T, D> a) {
             }
           }
 #pragma unroll
-          for (int c = 0; c < D; c++) vf[c] = v1[c];
-        }
-        if (stuck) {
-#pragma unroll
-          for (int c = 0; c < D; c++) vf[c] = 0;
+          for (int c = 0; c < D; c++) vf[c] = stuck ? (T)0 : v1[c];
         }
       }
     } else {
@@ -1079,6 +1110,427 @@ __global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
   }
 }
 
+// Marks, for this step, the tiles holding the stencil base of particles that
+// are outside their binned tile (used after a state is restored while the
+// bins stay: P2G then treats those particles as movers).
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void mark_movers_kernel(Args<T, D> a, uint32_t stamp) {
+  const uint32_t ntile = a.counts[0];
+  for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
+    const uint32_t t = a.ptiles[seg];
+    const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
+    for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
+      const uint32_t p = a.order ? a.order[i] : i;
+      if (!a.valid[p]) continue;
+      T x[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+      const uint32_t nt = base_tile<T, D>(a.g, x);
+      if (nt != t && nt < (uint32_t)a.g.ntiles) a.mstamp[nt] = stamp;
+    }
+  }
+}
+
+// One launch copying X, V, F, B and valid between two particle buffers.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void copy_state_kernel(const T* sX, const T* sV, const T* sF, const T* sB,
+                                                           const uint8_t* sv, T* dX, T* dV, T* dF, T* dB, uint8_t* dv,
+                                                           uint32_t n) {
+  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < n; p += gridDim.x * BLOCK) {
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      dX[c * n + p] = sX[c * n + p];
+      dV[c * n + p] = sV[c * n + p];
+    }
+#pragma unroll
+    for (int k = 0; k < D * D; k++) {
+      dF[k * n + p] = sF[k * n + p];
+      dB[k * n + p] = sB[k * n + p];
+    }
+    dv[p] = sv[p];
+  }
+}
+
+// ------------------------------------------------------------- adjoint ---
+// Reverse of one step (formulas in mpmx_adjoint.h).  Per-particle kernels with
+// HBM atomics for the node scatter; the forward P2G and grid update of the
+// step are recomputed first, with (m, u) kept in gridu.
+template<typename T> struct AdjArgs {
+  T* aX; T* aV; T* aF; T* aB;  // state adjoints, SoA like the state
+  double* amu; double* ala;    // per-particle Lame adjoints, accumulated over steps
+  Node4<T>* agrid;             // vbar (G2P adjoint), then (mbar, Pbar)
+  const Node4<T>* gridu;       // (m, u) of the recomputed step
+};
+
+// G2P adjoint of particle p (formulas in mpmx_adjoint.h): node velocities
+// come from nv(slot-or-local index, c), node adjoints go to add(index, c, value).
+template<typename T, int D, class Index, class NV, class ADD>
+__device__ __forceinline__ void g2p_adjoint_particle(const Args<T, D>& a, const AdjArgs<T>& q, uint32_t p,
+                                                     const Wts<T, D>& w, Index index, NV nv, ADD add) {
+  T Fm[D * D], xb[D], vt[D], Fb[D * D], Bb[D * D], Gb[D * D], G[D * D];
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    xb[c] = q.aX[c * a.n + p];
+    vt[c] = q.aV[c * a.n + p] + a.dt * xb[c];
+  }
+#pragma unroll
+  for (int k = 0; k < D * D; k++) {
+    Fm[k] = a.F[k * a.n + p];
+    Fb[k] = q.aF[k * a.n + p];
+    Bb[k] = q.aB[k * a.n + p];
+    G[k] = 0;
+  }
+#pragma unroll
+  for (int c = 0; c < D; c++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      T s = 0;
+#pragma unroll
+      for (int k = 0; k < D; k++) s += Fb[c + D * k] * Fm[j + D * k];
+      Gb[c + D * j] = a.dt * s;
+    }
+#pragma unroll
+  for (int k = 0; k < Tile<D>::K; k++) {
+    int o[D];
+    T wt, gr[D], H[D][D], dz[D], v[D];
+    stencil_entry<T, D>(a.g, w, k, o, wt, gr, H);
+#pragma unroll
+    for (int c = 0; c < D; c++) dz[c] = (T)o[c] * a.g.dx[c] - w.xe[c];
+    const uint32_t ix = index(o);
+#pragma unroll
+    for (int c = 0; c < D; c++) v[c] = nv(ix, c);
+    T s1 = 0, hv[D], bv[D], vb[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      T bdz = 0, gbg = 0;
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        G[c + D * j] += v[c] * gr[j];
+        bdz += Bb[c + D * j] * dz[j];
+        gbg += Gb[c + D * j] * gr[j];
+      }
+      s1 += v[c] * (vt[c] + bdz);
+      vb[c] = wt * (vt[c] + bdz) + gbg;
+    }
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      T h = 0, b = 0;
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        h += Gb[c + D * j] * v[c];
+        b += Bb[c + D * j] * v[c];
+      }
+      hv[j] = h;
+      bv[j] = b;
+    }
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      T hh = 0;
+#pragma unroll
+      for (int b = 0; b < D; b++) hh += H[c][b] * hv[b];
+      xb[c] += s1 * gr[c] + hh - wt * bv[c];
+    }
+#pragma unroll
+    for (int c = 0; c < D; c++) add(ix, c, vb[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < D; c++) q.aX[c * a.n + p] = xb[c];
+#pragma unroll
+  for (int c = 0; c < D; c++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      T s = Fb[c + D * j];
+#pragma unroll
+      for (int r = 0; r < D; r++) s += a.dt * G[r + D * c] * Fb[r + D * j];
+      q.aF[(c + D * j) * a.n + p] = s;
+    }
+}
+
+// One workgroup per binned tile: the region's velocities are staged in LDS
+// and vbar is accumulated there with LDS atomics (one HBM atomic per node and
+// component at the flush, instead of one per particle and node); movers are
+// left to g2p_adjoint_mover_kernel.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void g2p_adjoint_kernel(Args<T, D> a, AdjArgs<T> q) {
+  constexpr int R = Tile<D>::R, RN = Tile<D>::RN, E = Tile<D>::E;
+  __shared__ T s_v[D][RN];
+  __shared__ T s_b[D][RN];
+  const uint32_t ntile = a.counts[0];
+  for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
+    const uint32_t t = a.ptiles[seg];
+    int org[D];
+    tile_origin<D>(t, a.g.GT, org);
+    for (int k = threadIdx.x; k < RN; k += BLOCK) {
+      int off[D], nc[D];
+      region_offsets<D>(k, off);
+#pragma unroll
+      for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
+      const Node4<T> nd = a.grid[node_slot<D>(nc, a.g.GT)];
+      s_v[0][k] = nd.x;
+      s_v[1][k] = nd.y;
+      if constexpr (D == 3) s_v[2][k] = nd.z;
+#pragma unroll
+      for (int c = 0; c < D; c++) s_b[c][k] = 0;
+    }
+    __syncthreads();
+    const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
+    for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
+      const uint32_t p = a.order ? a.order[i] : i;
+      if (!a.valid[p]) continue;
+      T x[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+      Wts<T, D> w;
+      weights<T, D>(a.g, x, w);
+      bool in = true;
+      int lb[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        lb[c] = w.base[c] - org[c];
+        in = in && lb[c] >= 0 && lb[c] < E;
+      }
+      if (!in) continue;  // mover
+      const int li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
+      auto index = [&](const int* o) -> uint32_t {
+        return (uint32_t)(li0 + (D == 3 ? (o[0] * R + o[1]) * R + o[D - 1] : o[0] * R + o[D - 1]));
+      };
+      auto nv = [&](uint32_t ix, int c) -> T { return s_v[c][ix]; };
+      auto add = [&](uint32_t ix, int c, T val) { atomicAdd(&s_b[c][ix], val); };
+      g2p_adjoint_particle<T, D>(a, q, p, w, index, nv, add);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < RN; k += BLOCK) {
+      int off[D], nc[D];
+      region_offsets<D>(k, off);
+#pragma unroll
+      for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
+      Node4<T>* an = q.agrid + node_slot<D>(nc, a.g.GT);
+      if (s_b[0][k] != (T)0) atomic_add(&an->x, s_b[0][k]);
+      if (s_b[1][k] != (T)0) atomic_add(&an->y, s_b[1][k]);
+      if constexpr (D == 3)
+        if (s_b[2][k] != (T)0) atomic_add(&an->z, s_b[2][k]);
+    }
+    __syncthreads();
+  }
+}
+
+// G2P adjoint of this step's movers, straight from and to HBM.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void g2p_adjoint_mover_kernel(Args<T, D> a, AdjArgs<T> q) {
+  const uint32_t nm = a.counts[4];
+  for (uint32_t m = blockIdx.x * BLOCK + threadIdx.x; m < nm; m += gridDim.x * BLOCK) {
+    const uint32_t p = a.mover_list[m];
+    T x[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+    Wts<T, D> w;
+    weights<T, D>(a.g, x, w);
+    bool in = true;
+#pragma unroll
+    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < a.g.G[c];
+    if (!in) continue;
+    auto index = [&](const int* o) -> uint32_t {
+      int nc[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) nc[c] = w.base[c] + o[c];
+      return node_slot<D>(nc, a.g.GT);
+    };
+    auto nv = [&](uint32_t ix, int c) -> T {
+      const Node4<T>& nd = a.grid[ix];
+      return c == 0 ? nd.x : (c == 1 ? nd.y : nd.z);
+    };
+    auto add = [&](uint32_t ix, int c, T val) {
+      Node4<T>* an = q.agrid + ix;
+      atomic_add(c == 0 ? &an->x : (c == 1 ? &an->y : &an->z), val);
+    };
+    g2p_adjoint_particle<T, D>(a, q, p, w, index, nv, add);
+  }
+}
+
+// Node adjoint: vbar back through the collision objects (replayed from u, in
+// reverse order), then Pbar = ubar / m and mbar = -ubar . (u - dt g) / m.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void grid_adjoint_kernel(Args<T, D> a, AdjArgs<T> q) {
+  const uint64_t total = (uint64_t)a.counts[1] * 64u;
+  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total; idx += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t t = a.gtiles[idx >> 6];
+    const uint32_t l = (uint32_t)(idx & 63u);
+    const uint32_t slot = t * 64u + l;
+    const Node4<T> u = q.gridu[slot];
+    Node4<T> out;
+    out.m = out.x = out.y = out.z = 0;
+    if (u.m == (T)0) {
+      q.agrid[slot] = out;
+      continue;
+    }
+    const Node4<T> vbn = q.agrid[slot];
+    T v[D], vb[D];
+    v[0] = u.x;
+    v[1] = u.y;
+    if constexpr (D == 3) v[2] = u.z;
+    vb[0] = vbn.x;
+    vb[1] = vbn.y;
+    if constexpr (D == 3) vb[2] = vbn.z;
+    if (a.nwalls > 0) {
+      int org[D], nc[D];
+      tile_origin<D>(t, a.g.GT, org);
+      uint32_t r = l;
+#pragma unroll
+      for (int c = D - 1; c >= 0; c--) {
+        nc[c] = org[c] + (int)(r % Tile<D>::E);
+        r /= Tile<D>::E;
+      }
+      T X0[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) X0[c] = a.g.lo[c] + ((T)(nc[c] - 3) + (T)0.5) * a.g.dx[c];
+      T u0[D], dummy[D];
+      int ob;
+#pragma unroll
+      for (int c = 0; c < D; c++) u0[c] = v[c];
+      const int nh = walls_apply<T, D>(a, X0, u0, -1, dummy, &ob);
+      for (int h = nh - 1; h >= 0; h--) {
+        T vq[D], nrm[D];
+#pragma unroll
+        for (int c = 0; c < D; c++) vq[c] = v[c];
+        if (walls_apply<T, D>(a, X0, vq, h, nrm, &ob) == 0) {
+#pragma unroll
+          for (int c = 0; c < D; c++) vb[c] = 0;
+        } else {
+          T dn = 0;
+#pragma unroll
+          for (int c = 0; c < D; c++) dn += nrm[c] * vb[c];
+#pragma unroll
+          for (int c = 0; c < D; c++) vb[c] -= dn * nrm[c];
+        }
+      }
+    }
+    const T im = (T)1 / u.m;
+    T mb = 0;
+    T ub[3] = {0, 0, 0};
+    const T uu[3] = {u.x, u.y, u.z};
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      ub[c] = vb[c] * im;
+      mb -= ub[c] * (uu[c] - a.dt * a.grav[c]);
+    }
+    out.m = mb;
+    out.x = ub[0];
+    out.y = ub[1];
+    out.z = ub[2];
+    q.agrid[slot] = out;
+  }
+}
+
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void p2g_adjoint_kernel(Args<T, D> a, AdjArgs<T> q) {
+  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < a.n; p += gridDim.x * BLOCK) {
+    if (!a.valid[p]) continue;
+    T x[D], v[D], Cm[D * D], Fm[D * D], tau[D * D], Dinv[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      x[c] = a.X[c * a.n + p];
+      v[c] = a.V[c * a.n + p];
+      Dinv[c] = (T)4 * a.g.inv_dx[c] * a.g.inv_dx[c];
+    }
+#pragma unroll
+    for (int c = 0; c < D; c++)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        Cm[c + D * j] = a.B[(c + D * j) * a.n + p] * Dinv[j];
+        Fm[c + D * j] = a.F[(c + D * j) * a.n + p];
+      }
+    const T m = a.mass[p], vol = a.vol[p], mu = a.mu[p], la = a.la[p];
+    kirchhoff_any<T, D>(a.model, mu, la, a.thr, a.root, vol, Fm, tau);
+    Wts<T, D> w;
+    weights<T, D>(a.g, x, w);
+    bool in = true;
+#pragma unroll
+    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < a.g.G[c];
+    if (!in) continue;
+    T vbar[D], Cbar[D * D], tbar[D * D], xbar[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) vbar[c] = xbar[c] = 0;
+#pragma unroll
+    for (int k = 0; k < D * D; k++) Cbar[k] = tbar[k] = 0;
+#pragma unroll
+    for (int k = 0; k < Tile<D>::K; k++) {
+      int o[D], nc[D];
+      T wt, gr[D], H[D][D], dz[D];
+      stencil_entry<T, D>(a.g, w, k, o, wt, gr, H);
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        nc[c] = w.base[c] + o[c];
+        dz[c] = (T)o[c] * a.g.dx[c] - w.xe[c];
+      }
+      const Node4<T> nb = q.agrid[node_slot<D>(nc, a.g.GT)];
+      const T Pb[3] = {nb.x, nb.y, nb.z};
+      T s = nb.m * m, tp[D], ct[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        vbar[c] += m * wt * Pb[c];
+        T ap = v[c];
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+          Cbar[c + D * j] += m * wt * Pb[c] * dz[j];
+          tbar[c + D * j] -= a.dt * Pb[c] * gr[j];
+          ap += Cm[c + D * j] * dz[j];
+        }
+        s += m * Pb[c] * ap;
+      }
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        T t1 = 0, t2 = 0;
+#pragma unroll
+        for (int c = 0; c < D; c++) {
+          t1 += tau[j + D * c] * Pb[c];
+          t2 += Cm[c + D * j] * Pb[c];
+        }
+        tp[j] = t1;
+        ct[j] = t2;
+      }
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        T hh = 0;
+#pragma unroll
+        for (int b = 0; b < D; b++) hh += H[c][b] * tp[b];
+        xbar[c] += s * gr[c] - m * wt * ct[c] - a.dt * hh;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      q.aV[c * a.n + p] = vbar[c];
+      q.aX[c * a.n + p] += xbar[c];
+    }
+#pragma unroll
+    for (int c = 0; c < D; c++)
+#pragma unroll
+      for (int j = 0; j < D; j++) q.aB[(c + D * j) * a.n + p] = Cbar[c + D * j] * Dinv[j];
+    double Fd[D * D], tb[D * D], Fbar[D * D], mub = 0, lab = 0;
+#pragma unroll
+    for (int k = 0; k < D * D; k++) {
+      Fd[k] = (double)Fm[k];
+      tb[k] = (double)tbar[k];
+    }
+    if (!kirchhoff_adjoint<D>(a.model, (double)mu, (double)la, (double)a.thr, (double)vol, Fd, tb, Fbar, mub, lab)) {
+      atomicOr(a.err + 2, 1u);
+      continue;
+    }
+#pragma unroll
+    for (int k = 0; k < D * D; k++) q.aF[k * a.n + p] += (T)Fbar[k];
+    q.amu[p] += mub;
+    q.ala[p] += lab;
+  }
+}
+
+template<typename T>
+__global__ __launch_bounds__(BLOCK) void soa_from_aos_pid_kernel(const T* aos, T* soa, const uint32_t* pid, uint32_t n,
+                                                                 int comps) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t q = pid[i];
+  for (int c = 0; c < comps; c++) soa[(size_t)c * n + i] = aos ? aos[(size_t)q * comps + c] : (T)0;
+}
+
 // -------------------------------------------------------------- reorder ----
 template<typename T, int D> struct ParticleBufs {
   T* X; T* V; T* F; T* B; T* mass; T* vol; T* mu; T* la;
@@ -1268,7 +1720,7 @@ struct PhaseTimer {
   };
   std::vector<hipEvent_t> pool;
   std::vector<Pending> pending;
-  static constexpr int NPH = 7;
+  static constexpr int NPH = 8;
   double total[NPH] = {};
   int64_t count[NPH] = {};
   bool on = false;
@@ -1314,8 +1766,9 @@ struct PhaseTimer {
     for (auto e : pool) (void)hipEventDestroy(e);
   }
 };
-enum { PH_BIN = 0, PH_P2G, PH_GRID, PH_G2P, PH_REORDER, PH_STEP, PH_REBIN };
-static const char* kPhaseNames[PhaseTimer::NPH] = {"bin", "p2g", "grid", "g2p", "reorder", "step", "rebin"};
+enum { PH_BIN = 0, PH_P2G, PH_GRID, PH_G2P, PH_REORDER, PH_STEP, PH_REBIN, PH_ADJOINT };
+static const char* kPhaseNames[PhaseTimer::NPH] = {"bin",  "p2g",  "grid",  "g2p",
+                                                   "reorder", "step", "rebin", "adjoint"};
 
 template<typename T, int D>
 class Engine final : public EngineBase {
@@ -1568,6 +2021,80 @@ class Engine final : public EngineBase {
     return false;
   }
 
+  // ---- differentiation: checkpointed tape and reverse pass
+  void tape_begin(int k) override {
+    if (k < 1) throw Error{MPMX_EINVAL, "checkpoint interval must be >= 1"};
+    if (n_ == 0) throw Error{MPMX_ESTATE, "no particles"};
+    if (any_friction()) throw Error{MPMX_EINVAL, "the adjoint step has no friction: remove wall friction"};
+    if (force_reorder_ || force_rebin_) rebin();  // settle the storage order before recording
+    particle_stats(nullptr, nullptr, &tape_valid_);
+    taping_ = true;
+    tape_k_ = k;
+    tape_dt_.clear();
+    ckpts_.clear();
+    ckpts_.emplace_back();
+    save_state(ckpts_.back());
+  }
+
+  void backward(const void* dX, const void* dV, const void* dF, const void* dB, void* gX, void* gV, void* gF, void* gB,
+                double* gmu, double* gla) override {
+    if (!taping_) throw Error{MPMX_ESTATE, "no tape: call mpmx_tape_begin, then mpmx_step"};
+    const size_t n = n_;
+    aX_.alloc(n * D); aV_.alloc(n * D); aF_.alloc(n * D * D); aB_.alloc(n * D * D);
+    amu_.alloc(n); ala_.alloc(n);
+    agrid_.alloc((size_t)g_.ntiles * 64);
+    gridu_.alloc((size_t)g_.ntiles * 64);
+    upload_adjoint((const T*)dX, aX_.p, D);
+    upload_adjoint((const T*)dV, aV_.p, D);
+    upload_adjoint((const T*)dF, aF_.p, D * D);
+    upload_adjoint((const T*)dB, aB_.p, D * D);
+    HIPCHECK(hipMemsetAsync(amu_.p, 0, n * sizeof(double), stream_));
+    HIPCHECK(hipMemsetAsync(ala_.p, 0, n * sizeof(double), stream_));
+    HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
+    StateBuf fin;
+    save_state(fin);
+    const double time0 = time_;
+    int64_t valid_now = 0;
+    particle_stats(nullptr, nullptr, &valid_now);
+    const bool invalidated = valid_now != tape_valid_;
+    const int ns = (int)tape_dt_.size(), k = tape_k_;
+    std::vector<StateBuf> seg((size_t)std::min(k, std::max(ns, 1)));
+    replaying_ = true;
+    hipEvent_t ea = timer_.begin(stream_);
+    for (int sg = ns > 0 ? (ns - 1) / k : -1; sg >= 0; sg--) {
+      const int s0 = sg * k, s1 = std::min(ns, s0 + k);
+      load_state(ckpts_[(size_t)sg]);
+      save_state(seg[0]);
+      for (int j = s0; j < s1 - 1; j++) {
+        step_once((T)tape_dt_[(size_t)j]);
+        save_state(seg[(size_t)(j - s0 + 1)]);
+      }
+      for (int j = s1 - 1; j >= s0; j--) {
+        // Bins of a later state cover an earlier one (movers are marked) unless
+        // a particle was invalidated in between: then it must be re-binned.
+        load_state(seg[(size_t)(j - s0)], j == s1 - 1 || invalidated);
+        backward_step((T)tape_dt_[(size_t)j]);
+      }
+    }
+    timer_.end(PH_ADJOINT, ea, stream_);
+    replaying_ = false;
+    download_aos(aX_.p, (T*)gX, D);
+    download_aos(aV_.p, (T*)gV, D);
+    download_aos(aF_.p, (T*)gF, D * D);
+    download_aos(aB_.p, (T*)gB, D * D);
+    download_param(amu_.p, gmu);
+    download_param(ala_.p, gla);
+    uint32_t errs[4];
+    HIPCHECK(hipMemcpy(errs, err_.p, sizeof(errs), hipMemcpyDeviceToHost));
+    load_state(fin);
+    time_ = time0;
+    taping_ = false;
+    ckpts_.clear();
+    HIPCHECK(hipStreamSynchronize(stream_));
+    if (errs[2]) throw Error{MPMX_EINVAL, "adjoint: a neo-Hookean particle is in the clamp branch (J < threshold)"};
+    if (errs[0]) throw Error{MPMX_EINTERNAL, "particle stencil outside the grid"};
+  }
+
   void stress(int64_t n, const void* F, const void* volume, const void* mu, const void* lambda, void* tau) override {
     if (n <= 0) return;
     if (n > 0x7fffffff) throw Error{MPMX_EINVAL, "too many matrices"};
@@ -1641,6 +2168,83 @@ class Engine final : public EngineBase {
     mover_list_.alloc(n);
     mover_tile_.release();
     mover_tile_.alloc(n);
+  }
+
+  struct StateBuf {
+    DevBuf<T> s;
+    DevBuf<uint8_t> valid;
+  };
+  // X, V, F, B and valid of the current state, in slot order.
+  void save_state(StateBuf& b) {
+    const size_t n = n_;
+    b.s.alloc(n * (2 * D + 2 * D * D));
+    b.valid.alloc(n);
+    T* q = b.s.p;
+    copy_state_kernel<T, D><<<copy_blocks(), BLOCK, 0, stream_>>>(cur_.X, cur_.V, cur_.F, cur_.B, cur_.valid, q,
+                                                                 q + n * D, q + 2 * n * D, q + n * (2 * D + D * D),
+                                                                 b.valid.p, n_);
+  }
+  // Restores a saved state.  With rebin the positions are re-binned before the
+  // next step; without, the bins stay and backward_step marks the movers.
+  void load_state(const StateBuf& b, bool rebin_next = true) {
+    const size_t n = n_;
+    const T* q = b.s.p;
+    copy_state_kernel<T, D><<<copy_blocks(), BLOCK, 0, stream_>>>(q, q + n * D, q + 2 * n * D, q + n * (2 * D + D * D),
+                                                                 b.valid.p, cur_.X, cur_.V, cur_.F, cur_.B,
+                                                                 cur_.valid, n_);
+    if (rebin_next) force_rebin_ = true;
+  }
+  uint32_t copy_blocks() const { return std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n_, BLOCK), 2048)); }
+  // Host adjoint (user order, NULL = zero) -> device SoA in slot order.
+  void upload_adjoint(const T* host, T* dev, int comps) {
+    if (host)
+      HIPCHECK(hipMemcpyAsync(stage_.p, host, (size_t)n_ * comps * sizeof(T), hipMemcpyHostToDevice, stream_));
+    soa_from_aos_pid_kernel<T><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(host ? stage_.p : nullptr, dev, cur_.pid, n_,
+                                                                        comps);
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+  void download_param(const double* dev, double* host) {
+    if (!host) return;
+    paramstage_.alloc(n_);
+    aos_from_soa_kernel<double><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(dev, paramstage_.p, cur_.pid, n_, 1);
+    HIPCHECK(hipMemcpyAsync(host, paramstage_.p, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+
+  // Reverse of one step from the state it started from (in cur_): recompute
+  // P2G and the grid update, then G2P, grid and P2G adjoints.
+  void backward_step(T dt) {
+    if (rebin_due()) rebin();
+    const uint32_t nt = (uint32_t)g_.ntiles;
+    const uint32_t ms = (uint32_t)(step_count_ + 1);
+    {
+      Args<T, D> a = make_args(dt);
+      a.order = identity_order_ ? nullptr : order_.p;
+      mark_movers_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a, ms);
+    }
+    gflags_kernel<D><<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, ms, nt, g_.GT[0],
+                                                             D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gflag_.p,
+                                                             counts_.p);
+    scan_exclusive(gflag_.p, gscan_.p, nt, counts_.p + 1, scan_scratch_.p, stream_);
+    list_kernel<<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(nt, gflag_.p, gscan_.p, gtiles_.p);
+    zero_kernel<T, false><<<grid_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, grid_.p, gridf_.p);
+    zero_kernel<T, false><<<grid_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, agrid_.p, gridf_.p);
+    Args<T, D> a = make_args(dt);
+    a.order = identity_order_ ? nullptr : order_.p;
+    a.next_stamp = ms + 1;
+    a.gridu = gridu_.p;
+    p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+    p2g_extra_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+    grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
+    AdjArgs<T> q{aX_.p, aV_.p, aF_.p, aB_.p, amu_.p, ala_.p, agrid_.p, gridu_.p};
+    const uint32_t pb = std::min<uint32_t>(cdiv(n_, BLOCK), 4096);
+    g2p_adjoint_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a, q);
+    g2p_adjoint_mover_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a, q);
+    grid_adjoint_kernel<T, D><<<grid_blocks(), BLOCK, 0, stream_>>>(a, q);
+    p2g_adjoint_kernel<T, D><<<pb, BLOCK, 0, stream_>>>(a, q);
+    HIPCHECK(hipGetLastError());
+    step_count_++;
+    steps_since_rebin_++;
   }
 
   void upload_soa(const T* host, T* dev, int comps) {
@@ -1729,7 +2333,7 @@ class Engine final : public EngineBase {
     list_kernel<<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(nt, pflag_.p, pscan_.p, ptiles_.p);
     timer_.end(PH_REBIN, e0, stream_);
     identity_order_ = false;
-    if (force_reorder_ || since_reorder_ >= reorder_interval_) {
+    if (!taping_ && (force_reorder_ || since_reorder_ >= reorder_interval_)) {
       hipEvent_t er = timer_.begin(stream_);
       reorder_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_, alt_, order_.p, n_);
       std::swap(cur_, alt_);
@@ -1804,6 +2408,13 @@ class Engine final : public EngineBase {
     step_count_++;
     steps_since_rebin_++;
     since_reorder_++;
+    if (taping_ && !replaying_) {
+      tape_dt_.push_back((double)dt);
+      if (tape_dt_.size() % (size_t)tape_k_ == 0) {
+        ckpts_.emplace_back();
+        save_state(ckpts_.back());
+      }
+    }
     HIPCHECK(hipGetLastError());
     has_grid_ = true;
     time_ += (double)dt;
@@ -1827,6 +2438,14 @@ class Engine final : public EngineBase {
   bool force_reorder_ = true;
   bool force_rebin_ = true;
   bool identity_order_ = false;
+  bool taping_ = false, replaying_ = false;
+  int tape_k_ = 1;
+  int64_t tape_valid_ = 0;
+  std::vector<double> tape_dt_;
+  std::vector<StateBuf> ckpts_;
+  DevBuf<T> aX_, aV_, aF_, aB_;
+  DevBuf<double> amu_, ala_, paramstage_;
+  DevBuf<Node4<T>> agrid_, gridu_;
   uint32_t bin_stamp_ = 0;
   int64_t step_count_ = 0, rebins_ = 0;
   int steps_since_rebin_ = 0;

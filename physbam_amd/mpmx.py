@@ -24,6 +24,7 @@ EXPORTS = (
     "mpmx_clear_collision_objects", "mpmx_step", "mpmx_step_async", "mpmx_synchronize", "mpmx_time",
     "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_grid_node_count",
     "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_set_rebin_policy", "mpmx_bin_stats",
+    "mpmx_tape_begin", "mpmx_backward",
     "mpmx_enable_timing", "mpmx_reset_timing",
     "mpmx_phase_time", "mpmx_stress", "mpmx_svd", "mpmx_sort_keys", "mpmx_seed_particles",
     "mpmx_random_boxes",
@@ -85,6 +86,8 @@ def lib():
         L.mpmx_set_reorder_interval.argtypes = [vp, c_int]
         L.mpmx_set_rebin_policy.argtypes = [vp, dbl, c_int]
         L.mpmx_bin_stats.argtypes = [vp, vp, vp]
+        L.mpmx_tape_begin.argtypes = [vp, c_int]
+        L.mpmx_backward.argtypes = [vp] + [vp] * 10
         L.mpmx_enable_timing.argtypes = [vp, c_int]
         L.mpmx_reset_timing.argtypes = [vp]
         L.mpmx_phase_time.argtypes = [vp, ctypes.c_char_p, vp, vp]
@@ -228,6 +231,21 @@ class Context:
         r = np.zeros(2, np.int64)
         _check(lib().mpmx_bin_stats(self._h, _p(r[:1]), _p(r[1:])))
         return int(r[0]), int(r[1])
+
+    def tape_begin(self, checkpoint_every=20):
+        _check(lib().mpmx_tape_begin(self._h, int(checkpoint_every)))
+
+    def backward(self, dX=None, dV=None, dF=None, dB=None):
+        """Reverse pass over the recorded steps.  Inputs are adjoints of the
+        final state (None = zero); returns dict X, V, F, B (adjoints of the
+        state at tape_begin) and mu, lambda (float64)."""
+        d, n = self.dim, self.n
+        ins = [None if a is None else self._arr(a, (n, c)) for a, c in ((dX, d), (dV, d), (dF, d * d), (dB, d * d))]
+        outs = [np.zeros((n, c), self.dtype) for c in (d, d, d * d, d * d)]
+        gmu = np.zeros(n)
+        gla = np.zeros(n)
+        _check(lib().mpmx_backward(self._h, *[_p(a) for a in ins], *[_p(a) for a in outs], _p(gmu), _p(gla)))
+        return dict(X=outs[0], V=outs[1], F=outs[2], B=outs[3], mu=gmu, **{"lambda": gla})
 
     def enable_timing(self, on=True):
         _check(lib().mpmx_enable_timing(self._h, 1 if on else 0))

@@ -219,7 +219,7 @@ def test_stress_near_rotations(dim, prec):
     for e in (1e-7, 1e-4, 1e-2, 0.05, 0.1, 0.12, 0.13, 0.2, 0.4):
         A = rng.standard_normal((per, dim, dim))
         Q, _ = np.linalg.qr(A)
-        Q *= np.sign(np.linalg.det(Q))[:, None, None]
+        Q[:, :, -1] *= np.sign(np.linalg.det(Q))[:, None]  # proper rotations (det +1) in 2D and 3D
         S = rng.standard_normal((per, dim, dim))
         S = 0.5 * (S + S.transpose(0, 2, 1))
         S /= np.linalg.norm(S, axis=(1, 2), keepdims=True)
