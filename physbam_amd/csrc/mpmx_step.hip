@@ -157,28 +157,14 @@ __global__ __launch_bounds__(BLOCK) void bounds_kernel(const uint32_t* sk, uint3
   if (i == n - 1 || sk[i + 1] != t) te[t] = i + 1;
 }
 
-// pflag: tile is binned (holds a particle range).
-__global__ __launch_bounds__(BLOCK) void pflags_kernel(const uint32_t* bstamp, uint32_t stamp, uint32_t ntiles,
-                                                       uint32_t* pflag) {
-  const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t < ntiles) pflag[t] = bstamp[t] == stamp ? 1u : 0u;
-}
-
-// gflag: tile is touched by the stencil of some particle, i.e. the tile itself
-// or a neighbour in its -x/-y/-z octant holds a particle's stencil base: a
-// binned tile or a tile marked by this step's movers.  Thread 0 also clears
-// this step's slow-stress and mover counters.
-template<int D>
-__global__ __launch_bounds__(BLOCK) void gflags_kernel(const uint32_t* bstamp, uint32_t bs, const uint32_t* mstamp,
-                                                       uint32_t ms, uint32_t ntiles, int GT0, int GT1, int GT2,
-                                                       uint32_t* gflag, uint32_t* counts) {
-  const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t == 0) {
-    counts[3] = 0;
-    counts[4] = 0;
-  }
-  if (t >= ntiles) return;
-  const int GT[3] = {GT0, GT1, GT2};
+// Grid-tile list in two launches, no device-wide scan: gcount_kernel counts
+// the touched tiles of each 256-tile block; glist_kernel sums the counts of
+// the blocks before its own, ranks its tiles with wave ballots, writes their
+// ids to gtiles and clears their nodes (and those of up to two more node
+// arrays).  The last block writes the total to counts[1].
+template<int D> __device__ __forceinline__ bool grid_tile_flag(const uint32_t* bstamp, uint32_t bs,
+                                                               const uint32_t* mstamp, uint32_t ms, uint32_t t,
+                                                               const int* GT) {
   int c[D];
   uint32_t r = t;
 #pragma unroll
@@ -186,7 +172,7 @@ __global__ __launch_bounds__(BLOCK) void gflags_kernel(const uint32_t* bstamp, u
     c[a] = (int)(r % (uint32_t)GT[a]);
     r /= (uint32_t)GT[a];
   }
-  uint32_t g = 0;
+  bool g = false;
 #pragma unroll
   for (int m = 0; m < (1 << D); m++) {
     bool ok = true;
@@ -197,29 +183,80 @@ __global__ __launch_bounds__(BLOCK) void gflags_kernel(const uint32_t* bstamp, u
       ok = ok && cc >= 0;
       q = q * (uint32_t)GT[a] + (uint32_t)(cc < 0 ? 0 : cc);
     }
-    if (ok && (bstamp[q] == bs || mstamp[q] == ms)) g = 1u;
+    if (ok && (bstamp[q] == bs || mstamp[q] == ms)) g = true;
   }
-  gflag[t] = g;
+  return g;
 }
 
-__global__ __launch_bounds__(BLOCK) void list_kernel(uint32_t ntiles, const uint32_t* flag, const uint32_t* scan,
-                                                     uint32_t* list) {
+// GRID: touched grid tiles (octant rule); otherwise binned tiles (stamp == bs).
+template<int D, bool GRID>
+__global__ __launch_bounds__(BLOCK) void gcount_kernel(const uint32_t* bstamp, uint32_t bs, const uint32_t* mstamp,
+                                                       uint32_t ms, uint32_t ntiles, int GT0, int GT1, int GT2,
+                                                       uint32_t* block_cnt, uint32_t* counts) {
   const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t < ntiles && flag[t]) list[scan[t]] = t;
+  if (GRID && t == 0) {
+    counts[3] = 0;
+    counts[4] = 0;
+  }
+  const int GT[3] = {GT0, GT1, GT2};
+  const bool f = t < ntiles && (GRID ? grid_tile_flag<D>(bstamp, bs, mstamp, ms, t, GT) : bstamp[t] == bs);
+  const int cnt = __syncthreads_count(f ? 1 : 0);
+  if (threadIdx.x == 0) block_cnt[blockIdx.x] = (uint32_t)cnt;
 }
 
-template<typename T, bool FRICTION>
-__global__ __launch_bounds__(BLOCK) void zero_kernel(const uint32_t* gtiles, const uint32_t* counts, Node4<T>* grid,
-                                                     Node4<T>* gridf) {
-  const uint64_t total = (uint64_t)counts[1] * 64u;
-  Node4<T> z;
-  z.m = z.x = z.y = z.z = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < total; i += (uint64_t)gridDim.x * BLOCK) {
-    const uint32_t slot = gtiles[i >> 6] * 64u + (uint32_t)(i & 63u);
-    grid[slot] = z;
-    if (FRICTION) gridf[slot] = z;
+template<typename T, int D, bool GRID>
+__global__ __launch_bounds__(BLOCK) void glist_kernel(const uint32_t* bstamp, uint32_t bs, const uint32_t* mstamp,
+                                                      uint32_t ms, uint32_t ntiles, int GT0, int GT1, int GT2,
+                                                      const uint32_t* block_cnt, uint32_t* gtiles, uint32_t* total,
+                                                      Node4<T>* z0, Node4<T>* z1, Node4<T>* z2) {
+  __shared__ uint32_t s_w[BLOCK / 64], s_tiles[BLOCK];
+  __shared__ uint32_t s_base;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // offset of this block: sum of the counts of the blocks before it
+  uint32_t part = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += BLOCK) part += block_cnt[b];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
+  if (lane == 0) s_w[wv] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t sum = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; w++) sum += s_w[w];
+    s_base = sum;
+    if (blockIdx.x == gridDim.x - 1) *total = sum + block_cnt[blockIdx.x];
+  }
+  __syncthreads();
+  const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+  const int GT[3] = {GT0, GT1, GT2};
+  const bool f = t < ntiles && (GRID ? grid_tile_flag<D>(bstamp, bs, mstamp, ms, t, GT) : bstamp[t] == bs);
+  const uint64_t bal = __ballot(f);
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  if (lane == 0) s_w[wv] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < BLOCK / 64; w++) {
+    off += w < wv ? s_w[w] : 0u;
+    tot += s_w[w];
+  }
+  if (f) {
+    const uint32_t r = off + (uint32_t)__popcll(bal & lt);
+    gtiles[s_base + r] = t;
+    s_tiles[r] = t;
+  }
+  if (!z0) return;
+  __syncthreads();
+  Node4<T> zn;
+  zn.m = zn.x = zn.y = zn.z = 0;
+  for (uint32_t k = threadIdx.x; k < tot * 64u; k += BLOCK) {
+    const uint32_t slot = s_tiles[k >> 6] * 64u + (k & 63u);
+    z0[slot] = zn;
+    if (z1) z1[slot] = zn;
+    if (z2) z2[slot] = zn;
   }
 }
+
 
 // ------------------------------------------------------------------ P2G ----
 // One workgroup per binned tile (grid-stride over the tile list).  Mass and
@@ -1113,7 +1150,8 @@ __global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
 // Marks, for this step, the tiles holding the stencil base of particles that
 // are outside their binned tile (used after a state is restored while the
 // bins stay: P2G then treats those particles as movers).
-template<typename T, int D>
+// With LIST, movers are appended to mover_list (counts[4]) instead.
+template<typename T, int D, bool LIST>
 __global__ __launch_bounds__(BLOCK) void mark_movers_kernel(Args<T, D> a, uint32_t stamp) {
   const uint32_t ntile = a.counts[0];
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
@@ -1126,9 +1164,38 @@ __global__ __launch_bounds__(BLOCK) void mark_movers_kernel(Args<T, D> a, uint32
 #pragma unroll
       for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
       const uint32_t nt = base_tile<T, D>(a.g, x);
-      if (nt != t && nt < (uint32_t)a.g.ntiles) a.mstamp[nt] = stamp;
+      if (nt == t) continue;
+      if (LIST)
+        a.mover_list[atomicAdd(&a.counts[4], 1u)] = p;
+      else if (nt < (uint32_t)a.g.ntiles)
+        a.mstamp[nt] = stamp;
     }
   }
+}
+
+// Per-step grid-tile lists kept for the reverse pass.
+__global__ __launch_bounds__(BLOCK) void save_tiles_kernel(const uint32_t* gtiles, const uint32_t* counts,
+                                                           uint32_t* dst, uint32_t* dst_count) {
+  const uint32_t nt = counts[1];
+  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < nt; i += gridDim.x * BLOCK) dst[i] = gtiles[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst_count = nt;
+}
+
+// Reverse step set-up: counts[1] <- the step's tile count, counts[4] <- 0, and
+// the node adjoints of the step's tiles cleared.
+template<typename T>
+__global__ __launch_bounds__(BLOCK) void restore_tiles_kernel(const uint32_t* tiles, const uint32_t* count,
+                                                              uint32_t* counts, Node4<T>* agrid) {
+  const uint32_t nt = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    counts[1] = nt;
+    counts[4] = 0;
+  }
+  Node4<T> z;
+  z.m = z.x = z.y = z.z = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < (uint64_t)nt * 64u;
+       i += (uint64_t)gridDim.x * BLOCK)
+    agrid[tiles[i >> 6] * 64u + (uint32_t)(i & 63u)] = z;
 }
 
 // One launch copying X, V, F, B and valid between two particle buffers.
@@ -1801,10 +1868,7 @@ class Engine final : public EngineBase {
     tile_end_.alloc(nt);
     tile_stamp_.alloc(nt);
     mstamp_.alloc(nt);
-    pflag_.alloc(nt);
-    gflag_.alloc(nt);
-    pscan_.alloc(nt);
-    gscan_.alloc(nt);
+    gblock_.alloc(cdiv((uint32_t)nt, BLOCK));
     ptiles_.alloc(nt);
     gtiles_.alloc(nt);
     counts_.alloc(8);
@@ -1821,7 +1885,6 @@ class Engine final : public EngineBase {
     HIPCHECK(hipMemsetAsync(grid_.p, 0, nt * 64 * sizeof(Node4<T>), stream_));
     HIPCHECK(hipMemsetAsync(counts_.p, 0, 8 * sizeof(uint32_t), stream_));
     HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
-    scan_scratch_.alloc(scan_scratch_bytes((uint32_t)nt) + 64);
     stats_blocks_ = (int)std::min<uint64_t>(std::max<uint64_t>(cdiv(nt * 64, BLOCK), 1), 2048);
     stats_partial_.alloc((size_t)stats_blocks_ * 5 + 8);
     stats_out_.alloc(8);
@@ -2043,7 +2106,6 @@ class Engine final : public EngineBase {
     aX_.alloc(n * D); aV_.alloc(n * D); aF_.alloc(n * D * D); aB_.alloc(n * D * D);
     amu_.alloc(n); ala_.alloc(n);
     agrid_.alloc((size_t)g_.ntiles * 64);
-    gridu_.alloc((size_t)g_.ntiles * 64);
     upload_adjoint((const T*)dX, aX_.p, D);
     upload_adjoint((const T*)dV, aV_.p, D);
     upload_adjoint((const T*)dF, aF_.p, D * D);
@@ -2058,22 +2120,37 @@ class Engine final : public EngineBase {
     particle_stats(nullptr, nullptr, &valid_now);
     const bool invalidated = valid_now != tape_valid_;
     const int ns = (int)tape_dt_.size(), k = tape_k_;
-    std::vector<StateBuf> seg((size_t)std::min(k, std::max(ns, 1)));
+    const size_t kk = (size_t)std::min(k, std::max(ns, 1));
+    std::vector<StateBuf> seg(kk);
+    std::vector<StepGrid> sgrid(kk);
+    seg_counts_.alloc(kk);
+    const size_t nodes = (size_t)g_.ntiles * 64;
     replaying_ = true;
     hipEvent_t ea = timer_.begin(stream_);
     for (int sg = ns > 0 ? (ns - 1) / k : -1; sg >= 0; sg--) {
       const int s0 = sg * k, s1 = std::min(ns, s0 + k);
+      // forward from the checkpoint, keeping each step's state, grid and tiles
       load_state(ckpts_[(size_t)sg]);
-      save_state(seg[0]);
-      for (int j = s0; j < s1 - 1; j++) {
+      for (int j = s0; j < s1; j++) {
+        const size_t i = (size_t)(j - s0);
+        save_state(seg[i]);
+        sgrid[i].grid.alloc(nodes);
+        sgrid[i].gridu.alloc(nodes);
+        sgrid[i].tiles.alloc((size_t)g_.ntiles);
+        grid_override_ = sgrid[i].grid.p;
+        gridu_override_ = sgrid[i].gridu.p;
         step_once((T)tape_dt_[(size_t)j]);
-        save_state(seg[(size_t)(j - s0 + 1)]);
+        save_tiles_kernel<<<copy_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, sgrid[i].tiles.p,
+                                                                seg_counts_.p + i);
+        grid_override_ = nullptr;
+        gridu_override_ = nullptr;
       }
       for (int j = s1 - 1; j >= s0; j--) {
-        // Bins of a later state cover an earlier one (movers are marked) unless
-        // a particle was invalidated in between: then it must be re-binned.
-        load_state(seg[(size_t)(j - s0)], j == s1 - 1 || invalidated);
-        backward_step((T)tape_dt_[(size_t)j]);
+        // The bins of a later state cover an earlier one (movers are listed)
+        // unless a particle was invalidated in between: then re-bin.
+        const size_t i = (size_t)(j - s0);
+        load_state(seg[i], invalidated);
+        backward_step((T)tape_dt_[(size_t)j], sgrid[i], seg_counts_.p + i);
       }
     }
     timer_.end(PH_ADJOINT, ea, stream_);
@@ -2174,6 +2251,10 @@ class Engine final : public EngineBase {
     DevBuf<T> s;
     DevBuf<uint8_t> valid;
   };
+  struct StepGrid {
+    DevBuf<Node4<T>> grid, gridu;
+    DevBuf<uint32_t> tiles;
+  };
   // X, V, F, B and valid of the current state, in slot order.
   void save_state(StateBuf& b) {
     const size_t n = n_;
@@ -2211,32 +2292,30 @@ class Engine final : public EngineBase {
     HIPCHECK(hipStreamSynchronize(stream_));
   }
 
-  // Reverse of one step from the state it started from (in cur_): recompute
-  // P2G and the grid update, then G2P, grid and P2G adjoints.
-  void backward_step(T dt) {
+  // Grid tiles touched this step (binned tiles and this step's mover tiles,
+  // each with its -octant neighbours) -> gtiles, counts[1]; their nodes in
+  // z0 (and z1, z2 if set) are cleared.  Also clears counts[3], counts[4].
+  void build_grid_tiles(uint32_t ms, Node4<T>* z0, Node4<T>* z1, Node4<T>* z2) {
+    const uint32_t nt = (uint32_t)g_.ntiles, nb = cdiv(nt, BLOCK);
+    gcount_kernel<D, true><<<nb, BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, ms, nt, g_.GT[0],
+                                                      D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gblock_.p,
+                                                      counts_.p);
+    glist_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, ms, nt, g_.GT[0],
+                                                        D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gblock_.p,
+                                                        gtiles_.p, counts_.p + 1, z0, z1, z2);
+  }
+
+  // Reverse of one step from the state it started from (in cur_) and the grid
+  // the forward step built (velocities, (m, u) and touched tiles).
+  void backward_step(T dt, const StepGrid& sgr, const uint32_t* seg_count) {
     if (rebin_due()) rebin();
-    const uint32_t nt = (uint32_t)g_.ntiles;
-    const uint32_t ms = (uint32_t)(step_count_ + 1);
-    {
-      Args<T, D> a = make_args(dt);
-      a.order = identity_order_ ? nullptr : order_.p;
-      mark_movers_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a, ms);
-    }
-    gflags_kernel<D><<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, ms, nt, g_.GT[0],
-                                                             D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gflag_.p,
-                                                             counts_.p);
-    scan_exclusive(gflag_.p, gscan_.p, nt, counts_.p + 1, scan_scratch_.p, stream_);
-    list_kernel<<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(nt, gflag_.p, gscan_.p, gtiles_.p);
-    zero_kernel<T, false><<<grid_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, grid_.p, gridf_.p);
-    zero_kernel<T, false><<<grid_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, agrid_.p, gridf_.p);
     Args<T, D> a = make_args(dt);
     a.order = identity_order_ ? nullptr : order_.p;
-    a.next_stamp = ms + 1;
-    a.gridu = gridu_.p;
-    p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-    p2g_extra_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
-    grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
-    AdjArgs<T> q{aX_.p, aV_.p, aF_.p, aB_.p, amu_.p, ala_.p, agrid_.p, gridu_.p};
+    a.grid = sgr.grid.p;
+    a.gtiles = sgr.tiles.p;
+    restore_tiles_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(sgr.tiles.p, seg_count, counts_.p, agrid_.p);
+    mark_movers_kernel<T, D, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a, 0u);
+    AdjArgs<T> q{aX_.p, aV_.p, aF_.p, aB_.p, amu_.p, ala_.p, agrid_.p, sgr.gridu.p};
     const uint32_t pb = std::min<uint32_t>(cdiv(n_, BLOCK), 4096);
     g2p_adjoint_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a, q);
     g2p_adjoint_mover_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a, q);
@@ -2289,8 +2368,9 @@ class Engine final : public EngineBase {
     a.mover_list = mover_list_.p;
     a.mover_tile = mover_tile_.p;
     a.mstamp = mstamp_.p;
-    a.grid = grid_.p;
+    a.grid = grid_override_ ? grid_override_ : grid_.p;
     a.gridf = gridf_.p;
+    a.gridu = gridu_override_;
     a.err = err_.p;
     a.stats_partial = nullptr;
     return a;
@@ -2327,10 +2407,13 @@ class Engine final : public EngineBase {
     bin_stamp_++;
     bounds_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(skeys_.p, n_, tile_begin_.p, tile_end_.p, tile_stamp_.p,
                                                           bin_stamp_);
-    const uint32_t nt = (uint32_t)g_.ntiles;
-    pflags_kernel<<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, nt, pflag_.p);
-    scan_exclusive(pflag_.p, pscan_.p, nt, counts_.p + 0, scan_scratch_.p, stream_);
-    list_kernel<<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(nt, pflag_.p, pscan_.p, ptiles_.p);
+    const uint32_t nt = (uint32_t)g_.ntiles, nb = cdiv(nt, BLOCK);
+    gcount_kernel<D, false><<<nb, BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, 0u, nt, g_.GT[0],
+                                                       D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gblock_.p,
+                                                       counts_.p);
+    glist_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, 0u, nt, g_.GT[0],
+                                                         D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gblock_.p,
+                                                         ptiles_.p, counts_.p + 0, nullptr, nullptr, nullptr);
     timer_.end(PH_REBIN, e0, stream_);
     identity_order_ = false;
     if (!taping_ && (force_reorder_ || since_reorder_ >= reorder_interval_)) {
@@ -2361,17 +2444,8 @@ class Engine final : public EngineBase {
     }
     if (rebin_due()) rebin();
     hipEvent_t e0 = timer_.begin(stream_);
-    const uint32_t nt = (uint32_t)g_.ntiles;
     const uint32_t ms = (uint32_t)(step_count_ + 1);  // mover stamp of this step
-    gflags_kernel<D><<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, ms, nt, g_.GT[0],
-                                                             D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gflag_.p,
-                                                             counts_.p);
-    scan_exclusive(gflag_.p, gscan_.p, nt, counts_.p + 1, scan_scratch_.p, stream_);
-    list_kernel<<<cdiv(nt, BLOCK), BLOCK, 0, stream_>>>(nt, gflag_.p, gscan_.p, gtiles_.p);
-    if (fric)
-      zero_kernel<T, true><<<grid_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, grid_.p, gridf_.p);
-    else
-      zero_kernel<T, false><<<grid_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, grid_.p, gridf_.p);
+    build_grid_tiles(ms, grid_override_ ? grid_override_ : grid_.p, fric ? gridf_.p : nullptr, nullptr);
     timer_.end(PH_BIN, e0, stream_);
 
     Args<T, D> a = make_args(dt);
@@ -2445,7 +2519,10 @@ class Engine final : public EngineBase {
   std::vector<StateBuf> ckpts_;
   DevBuf<T> aX_, aV_, aF_, aB_;
   DevBuf<double> amu_, ala_, paramstage_;
-  DevBuf<Node4<T>> agrid_, gridu_;
+  DevBuf<Node4<T>> agrid_;
+  DevBuf<uint32_t> seg_counts_;
+  Node4<T>* grid_override_ = nullptr;  // the grid arrays of a recomputed step
+  Node4<T>* gridu_override_ = nullptr;
   uint32_t bin_stamp_ = 0;
   int64_t step_count_ = 0, rebins_ = 0;
   int steps_since_rebin_ = 0;
@@ -2467,9 +2544,8 @@ class Engine final : public EngineBase {
   DevBuf<uint8_t> va_, vb_, stage_bytes_;
   DevBuf<uint32_t> ia_, ib_;
   DevBuf<uint32_t> keys_, skeys_, order_, ktmp_, vtmp_;
-  DevBuf<char> sort_scratch_, scan_scratch_;
-  DevBuf<uint32_t> tile_begin_, tile_end_, tile_stamp_, mstamp_, pflag_, gflag_, pscan_, gscan_, ptiles_, gtiles_,
-      counts_, err_;
+  DevBuf<char> sort_scratch_;
+  DevBuf<uint32_t> tile_begin_, tile_end_, tile_stamp_, mstamp_, gblock_, ptiles_, gtiles_, counts_, err_;
   DevBuf<Node4<T>> grid_, gridf_;
   DevBuf<double> stats_partial_, stats_out_;
   DevBuf<T> stage_;
