@@ -73,6 +73,20 @@ const char* mpmx_last_error(void);
 int mpmx_create(const mpmx_grid_desc* grid, mpmx_ctx** out);
 int mpmx_destroy(mpmx_ctx* ctx);
 
+/* Batch of independent scenes in one context (no counterpart in the reference,
+   which runs one MPM_EXAMPLE per process): scene s has its own grid grids[s]
+   (same dim and precision for all) and its own particles; every step advances
+   all scenes with the same launches.  Forces, gravity, collision boxes and dt
+   are shared and act in each scene's own coordinates.  Particles are set with
+   mpmx_set_batch_particles (arrays concatenated in scene order, counts[s]
+   particles for scene s) and read back by mpmx_get_particles in that order.
+   mpmx_get_grid is for single-scene contexts. */
+int mpmx_create_batch(const mpmx_grid_desc* grids, int nscenes, mpmx_ctx** out);
+int mpmx_scene_count(const mpmx_ctx* ctx);
+int mpmx_set_batch_particles(mpmx_ctx* ctx, int nscenes, const int64_t* counts, const void* X, const void* V,
+                             const void* F, const void* B, const void* mass, const void* volume, const void* mu,
+                             const void* lambda, const uint8_t* valid);
+
 /* Particle state in, MPM_PARTICLES<TV> layout (Examples_And_Drivers/MPM_PARTICLES.h:17-44):
    X, V: n*d; F, B: n*d*d (column-major); mass, volume, mu, lambda: n; valid: n
    bytes (nonzero = simulated).  mu/lambda are the per-particle Lame parameters

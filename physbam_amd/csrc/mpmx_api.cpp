@@ -49,9 +49,27 @@ int mpmx_create(const mpmx_grid_desc* grid, mpmx_ctx** out) {
   if (!grid || !out) return bad("null argument");
   *out = nullptr;
   return guard([&] {
-    mpmx::EngineBase* e = mpmx::make_engine(*grid);
+    mpmx::EngineBase* e = mpmx::make_engine(grid, 1);
     *out = new mpmx_ctx{e};
   });
+}
+
+int mpmx_create_batch(const mpmx_grid_desc* grids, int nscenes, mpmx_ctx** out) {
+  if (!grids || !out) return bad("null argument");
+  *out = nullptr;
+  return guard([&] {
+    mpmx::EngineBase* e = mpmx::make_engine(grids, nscenes);
+    *out = new mpmx_ctx{e};
+  });
+}
+
+int mpmx_scene_count(const mpmx_ctx* ctx) { return ctx ? ctx->engine->scene_count() : MPMX_EINVAL; }
+
+int mpmx_set_batch_particles(mpmx_ctx* ctx, int nscenes, const int64_t* counts, const void* X, const void* V,
+                             const void* F, const void* B, const void* mass, const void* volume, const void* mu,
+                             const void* lambda, const uint8_t* valid) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->set_batch_particles(nscenes, counts, X, V, F, B, mass, volume, mu, lambda, valid); });
 }
 
 int mpmx_destroy(mpmx_ctx* ctx) {

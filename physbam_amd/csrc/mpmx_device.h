@@ -48,11 +48,15 @@ template<typename T> struct WallDev {
   T lo[3], hi[3];
 };
 
+// Geometry of one scene's grid.  Several scenes share one tile array: scene
+// tiles are numbered from tile0 on.
 template<typename T, int D> struct Geom {
   T lo[D], hi[D], dx[D], inv_dx[D];
-  int G[D];   // nodes per axis, N + 6
-  int GT[D];  // tiles per axis (storage), ceil(G/E) + 1
-  int ntiles; // prod GT
+  T vlo[D], vhi[D];  // particles outside [vlo, vhi] are invalidated (MPM_DRIVER.cpp:1227-1238)
+  int G[D];          // nodes per axis, N + 6
+  int GT[D];         // tiles per axis (storage), ceil(G/E) + 1
+  int ntiles;        // prod GT
+  uint32_t tile0;    // first tile of this scene in the shared tile array
 };
 
 // Stencil base (node coordinate, ghost offset included) and 1D weights.
@@ -92,9 +96,9 @@ MPMX_HD void weights(const Geom<T, D>& g, const T* X, Wts<T, D>& s) {
   }
 }
 
-// Binning key of a particle: linear id of the tile holding its stencil base
-// node, or INVALID_KEY if the stencil would leave the ghosted grid.  Sorting by
-// this key groups particles by tile.
+// Binning key of a particle: id of the tile holding its stencil base node (in
+// the shared tile array), or INVALID_KEY if the stencil would leave the ghosted
+// grid.  Sorting by this key groups particles by tile.
 template<typename T, int D>
 MPMX_HD uint32_t tile_key(const Geom<T, D>& g, const T* X) {
   uint32_t tile = 0;
@@ -104,7 +108,7 @@ MPMX_HD uint32_t tile_key(const Geom<T, D>& g, const T* X) {
     if (!(b >= 0 && b + 2 < g.G[a])) return INVALID_KEY;
     tile = tile * (uint32_t)g.GT[a] + (uint32_t)(b / Tile<D>::E);
   }
-  return tile;
+  return g.tile0 + tile;
 }
 
 // Offsets of cell `l` (0..63) inside its tile, row-major.

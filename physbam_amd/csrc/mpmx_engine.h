@@ -16,6 +16,10 @@ struct EngineBase {
   virtual void set_particles(int64_t n, const void* X, const void* V, const void* F, const void* B,
                              const void* mass, const void* volume, const void* mu, const void* lambda,
                              const uint8_t* valid) = 0;
+  virtual void set_batch_particles(int nscenes, const int64_t* counts, const void* X, const void* V, const void* F,
+                                   const void* B, const void* mass, const void* volume, const void* mu,
+                                   const void* lambda, const uint8_t* valid) = 0;
+  virtual int scene_count() const = 0;
   virtual void get_particles(void* X, void* V, void* F, void* B, uint8_t* valid) = 0;
   virtual int64_t particle_count() const = 0;
   virtual void set_constitutive(int model, double neo_threshold) = 0;
@@ -47,7 +51,7 @@ struct EngineBase {
 };
 
 // Throws mpmx::Error on failure.
-EngineBase* make_engine(const mpmx_grid_desc& desc);
+EngineBase* make_engine(const mpmx_grid_desc* descs, int nscenes);
 
 struct Error {
   int code;

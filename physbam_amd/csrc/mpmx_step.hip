@@ -46,8 +46,11 @@ constexpr int BLOCK = 256;
 constexpr int EXTRA_BLOCKS = 256;  // grid of the mover / slow-path kernels (grid-stride)
 
 template<typename T, int D> struct Args {
-  Geom<T, D> g;
-  T vlo[D], vhi[D];  // particles outside [vlo, vhi] are invalidated (MPM_DRIVER.cpp:1227-1238)
+  Geom<T, D> g;                // the geometry when there is one scene
+  const Geom<T, D>* scenes;    // per-scene geometry of a batch
+  const uint16_t* tscene;      // tile -> scene (null: one scene)
+  const uint16_t* pscene;      // particle slot -> scene (null: one scene)
+  uint32_t ntotal;             // tiles of all scenes
   T dt;
   T grav[D];
   int model;
@@ -78,22 +81,32 @@ template<typename T, int D> struct Args {
   double* stats_partial;
 };
 
-template<int D> __device__ __forceinline__ void tile_origin(uint32_t t, const int* GT, int* o) {
+template<typename T, int D> __device__ __forceinline__ Geom<T, D> tile_geom(const Args<T, D>& a, uint32_t t) {
+  return a.tscene ? a.scenes[a.tscene[t]] : a.g;
+}
+template<typename T, int D> __device__ __forceinline__ Geom<T, D> particle_geom(const Args<T, D>& a, uint32_t p) {
+  return a.pscene ? a.scenes[a.pscene[p]] : a.g;
+}
+
+// Node origin of tile t (node coordinates of its scene).
+template<typename T, int D> __device__ __forceinline__ void tile_origin(uint32_t t, const Geom<T, D>& g, int* o) {
+  t -= g.tile0;
 #pragma unroll
   for (int a = D - 1; a >= 0; a--) {
-    o[a] = (int)(t % (uint32_t)GT[a]) * Tile<D>::E;
-    t /= (uint32_t)GT[a];
+    o[a] = (int)(t % (uint32_t)g.GT[a]) * Tile<D>::E;
+    t /= (uint32_t)g.GT[a];
   }
 }
 
-template<int D> __device__ __forceinline__ uint32_t node_slot(const int* nc, const int* GT) {
+// Storage slot of node nc of a scene: tile-blocked, 64 nodes per tile.
+template<typename T, int D> __device__ __forceinline__ uint32_t node_slot(const int* nc, const Geom<T, D>& g) {
   uint32_t t = 0, l = 0;
 #pragma unroll
   for (int a = 0; a < D; a++) {
-    t = t * (uint32_t)GT[a] + (uint32_t)(nc[a] / Tile<D>::E);
+    t = t * (uint32_t)g.GT[a] + (uint32_t)(nc[a] / Tile<D>::E);
     l = l * Tile<D>::E + (uint32_t)(nc[a] % Tile<D>::E);
   }
-  return t * 64u + l;
+  return (g.tile0 + t) * 64u + l;
 }
 
 // Region-local flat index -> offsets (row-major, last axis fastest).
@@ -112,7 +125,7 @@ template<typename T, int D> __device__ __forceinline__ uint32_t base_tile(const 
   uint32_t t = 0;
 #pragma unroll
   for (int a = 0; a < D; a++) t = t * (uint32_t)g.GT[a] + (uint32_t)((stencil_base<T, D>(g, x, a) + 3) / Tile<D>::E);
-  return t;
+  return g.tile0 + t;
 }
 
 // ----------------------------------------------------------------- bins ----
@@ -131,7 +144,7 @@ __global__ __launch_bounds__(BLOCK) void key_kernel(Args<T, D> a, uint32_t* rang
     T x[D];
 #pragma unroll
     for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
-    key = tile_key<T, D>(a.g, x);
+    key = tile_key<T, D>(particle_geom(a, p), x);
     if (key == INVALID_KEY) atomicAdd(range_err, 1u);
   }
   a.keys[p] = key;
@@ -162,51 +175,50 @@ __global__ __launch_bounds__(BLOCK) void bounds_kernel(const uint32_t* sk, uint3
 // the blocks before its own, ranks its tiles with wave ballots, writes their
 // ids to gtiles and clears their nodes (and those of up to two more node
 // arrays).  The last block writes the total to counts[1].
-template<int D> __device__ __forceinline__ bool grid_tile_flag(const uint32_t* bstamp, uint32_t bs,
-                                                               const uint32_t* mstamp, uint32_t ms, uint32_t t,
-                                                               const int* GT) {
+template<typename T, int D>
+__device__ __forceinline__ bool grid_tile_flag(const Args<T, D>& a, const uint32_t* bstamp, uint32_t bs, uint32_t ms,
+                                               uint32_t t) {
+  const Geom<T, D> g = tile_geom(a, t);
   int c[D];
-  uint32_t r = t;
+  uint32_t r = t - g.tile0;
 #pragma unroll
-  for (int a = D - 1; a >= 0; a--) {
-    c[a] = (int)(r % (uint32_t)GT[a]);
-    r /= (uint32_t)GT[a];
+  for (int k = D - 1; k >= 0; k--) {
+    c[k] = (int)(r % (uint32_t)g.GT[k]);
+    r /= (uint32_t)g.GT[k];
   }
-  bool g = false;
+  bool f = false;
 #pragma unroll
   for (int m = 0; m < (1 << D); m++) {
     bool ok = true;
     uint32_t q = 0;
 #pragma unroll
-    for (int a = 0; a < D; a++) {
-      const int cc = c[a] - ((m >> a) & 1);
+    for (int k = 0; k < D; k++) {
+      const int cc = c[k] - ((m >> k) & 1);
       ok = ok && cc >= 0;
-      q = q * (uint32_t)GT[a] + (uint32_t)(cc < 0 ? 0 : cc);
+      q = q * (uint32_t)g.GT[k] + (uint32_t)(cc < 0 ? 0 : cc);
     }
-    if (ok && (bstamp[q] == bs || mstamp[q] == ms)) g = true;
+    q += g.tile0;
+    if (ok && (bstamp[q] == bs || a.mstamp[q] == ms)) f = true;
   }
-  return g;
+  return f;
 }
 
 // GRID: touched grid tiles (octant rule); otherwise binned tiles (stamp == bs).
-template<int D, bool GRID>
-__global__ __launch_bounds__(BLOCK) void gcount_kernel(const uint32_t* bstamp, uint32_t bs, const uint32_t* mstamp,
-                                                       uint32_t ms, uint32_t ntiles, int GT0, int GT1, int GT2,
-                                                       uint32_t* block_cnt, uint32_t* counts) {
+template<typename T, int D, bool GRID>
+__global__ __launch_bounds__(BLOCK) void gcount_kernel(Args<T, D> a, const uint32_t* bstamp, uint32_t bs, uint32_t ms,
+                                                       uint32_t* block_cnt) {
   const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
   if (GRID && t == 0) {
-    counts[3] = 0;
-    counts[4] = 0;
+    a.counts[3] = 0;
+    a.counts[4] = 0;
   }
-  const int GT[3] = {GT0, GT1, GT2};
-  const bool f = t < ntiles && (GRID ? grid_tile_flag<D>(bstamp, bs, mstamp, ms, t, GT) : bstamp[t] == bs);
+  const bool f = t < a.ntotal && (GRID ? grid_tile_flag<T, D>(a, bstamp, bs, ms, t) : bstamp[t] == bs);
   const int cnt = __syncthreads_count(f ? 1 : 0);
   if (threadIdx.x == 0) block_cnt[blockIdx.x] = (uint32_t)cnt;
 }
 
 template<typename T, int D, bool GRID>
-__global__ __launch_bounds__(BLOCK) void glist_kernel(const uint32_t* bstamp, uint32_t bs, const uint32_t* mstamp,
-                                                      uint32_t ms, uint32_t ntiles, int GT0, int GT1, int GT2,
+__global__ __launch_bounds__(BLOCK) void glist_kernel(Args<T, D> a, const uint32_t* bstamp, uint32_t bs, uint32_t ms,
                                                       const uint32_t* block_cnt, uint32_t* gtiles, uint32_t* total,
                                                       Node4<T>* z0, Node4<T>* z1, Node4<T>* z2) {
   __shared__ uint32_t s_w[BLOCK / 64], s_tiles[BLOCK];
@@ -228,8 +240,7 @@ __global__ __launch_bounds__(BLOCK) void glist_kernel(const uint32_t* bstamp, ui
   }
   __syncthreads();
   const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
-  const int GT[3] = {GT0, GT1, GT2};
-  const bool f = t < ntiles && (GRID ? grid_tile_flag<D>(bstamp, bs, mstamp, ms, t, GT) : bstamp[t] == bs);
+  const bool f = t < a.ntotal && (GRID ? grid_tile_flag<T, D>(a, bstamp, bs, ms, t) : bstamp[t] == bs);
   const uint64_t bal = __ballot(f);
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   if (lane == 0) s_w[wv] = (uint32_t)__popcll(bal);
@@ -309,9 +320,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
   __shared__ uint32_t s_cb[64], s_cn[64], s_max;
   auto& s_c = s_u.c;
   const uint32_t ntile = a.counts[0];
-  T dinv[D];
-#pragma unroll
-  for (int c = 0; c < D; c++) dinv[c] = (T)4 * a.g.inv_dx[c] * a.g.inv_dx[c];  // D_p^-1 (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:152-158)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int a_cell = threadIdx.x >> 2, a_j = threadIdx.x & 3;  // phase A assignment
   const int a_slot = a_j * 64 + a_cell;                         // = phase B slot (wave a_j, lane a_cell)
@@ -320,8 +328,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
   cell_offsets<D>((uint32_t)lane, lcell);
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
     const uint32_t t = a.ptiles[seg];
+    const Geom<T, D> g = tile_geom(a, t);
+    T dinv[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) dinv[c] = (T)4 * g.inv_dx[c] * g.inv_dx[c];  // D_p^-1 (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:152-158)
     int org[D];
-    tile_origin<D>(t, a.g.GT, org);
+    tile_origin<T, D>(t, g, org);
     for (int k = threadIdx.x; k < NW * RN; k += BLOCK) {
       T4 z = {0, 0, 0, 0};
       s_reg[k / RN][k % RN] = z;
@@ -346,7 +358,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
             uint32_t cl = 0;
 #pragma unroll
             for (int c = 0; c < D; c++) {
-              const int lb = stencil_base<T, D>(a.g, x, c) + 3 - org[c];
+              const int lb = stencil_base<T, D>(g, x, c) + 3 - org[c];
               in = in && lb >= 0 && lb < E;
               cl = cl * E + (uint32_t)(lb & (E - 1));
             }
@@ -416,8 +428,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
           const T m = a.mass[p];
 #pragma unroll
           for (int c = 0; c < D; c++) {
-            const int bb = stencil_base<T, D>(a.g, x, c);
-            const T cen = a.g.lo[c] + ((T)bb + (T)0.5) * a.g.dx[c];
+            const int bb = stencil_base<T, D>(g, x, c);
+            const T cen = g.lo[c] + ((T)bb + (T)0.5) * g.dx[c];
             s_c[C::XE + c][a_slot] = x[c] - cen;
           }
           T tau[D * D];
@@ -465,7 +477,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
 #pragma unroll
           for (int c = 0; c < D; c++) {
             const T xe = s_c[C::XE + c][sl];
-            const T inv = a.g.inv_dx[c];
+            const T inv = g.inv_dx[c];
             const T xx = xe * inv;
             const T q0 = xx - (T)1.5, q1 = xx - (T)1, q2 = xx - (T)0.5;
             w[0][c] = (T)0.5 * q0 * q0;
@@ -475,7 +487,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
             dw[1][c] = (T)(-2) * q1 * inv;
             dw[2][c] = q2 * inv;
 #pragma unroll
-            for (int o = 0; o < 3; o++) dz[o][c] = (T)o * a.g.dx[c] - xe;
+            for (int o = 0; o < 3; o++) dz[o][c] = (T)o * g.dx[c] - xe;
           }
           // Separable form: with per-axis factors w, dw and dz = o dx - xe,
           //   m_i   += w_x w_y w_z m
@@ -579,7 +591,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
         region_offsets<D>(k, off);
 #pragma unroll
         for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
-        Node4<T>* nd = a.grid + node_slot<D>(nc, a.g.GT);
+        Node4<T>* nd = a.grid + node_slot<T, D>(nc, g);
         atomic_add(&nd->m, sum.x);
         atomic_add(&nd->x, sum.y);
         atomic_add(&nd->y, sum.z);
@@ -599,16 +611,17 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
   for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < nm + ns; q += gridDim.x * BLOCK) {
     const bool mover = q < nm;
     const uint32_t p = mover ? a.mover_list[q] : a.slow_list[q - nm];
+    const Geom<T, D> g = particle_geom(a, p);
     T x[D], Fm[D * D], tau[D * D];
 #pragma unroll
     for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
 #pragma unroll
     for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
     Wts<T, D> w;
-    weights<T, D>(a.g, x, w);
+    weights<T, D>(g, x, w);
     bool in = true;
 #pragma unroll
-    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < a.g.G[c];
+    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < g.G[c];
     if (!in) {
       atomicOr(a.err, 2u);
       continue;
@@ -632,12 +645,12 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
       for (int j = 0; j < D; j++)
 #pragma unroll
         for (int c = 0; c < D; c++)
-          mC[c][j] = m * a.B[(c + D * j) * a.n + p] * ((T)4 * a.g.inv_dx[j] * a.g.inv_dx[j]);
+          mC[c][j] = m * a.B[(c + D * j) * a.n + p] * ((T)4 * g.inv_dx[j] * g.inv_dx[j]);
     }
 #pragma unroll
     for (int k = 0; k < Tile<D>::K; k++) {
       int o[D], nc[D], r = k;
-      T g[D], dz[D];
+      T gr[D], dz[D];
 #pragma unroll
       for (int c = D - 1; c >= 0; c--) {
         o[c] = r % 3;
@@ -647,22 +660,22 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
 #pragma unroll
       for (int c = 0; c < D; c++) {
         nc[c] = w.base[c] + o[c];
-        dz[c] = (T)o[c] * a.g.dx[c] - w.xe[c];
+        dz[c] = (T)o[c] * g.dx[c] - w.xe[c];
         wt *= w.w[o[c]][c];
         T gg = w.dw[o[c]][c];
 #pragma unroll
         for (int j = 0; j < D; j++)
           if (j != c) gg *= w.w[o[j]][j];
-        g[c] = gg;
+        gr[c] = gg;
       }
-      Node4<T>* nd = a.grid + node_slot<D>(nc, a.g.GT);
+      Node4<T>* nd = a.grid + node_slot<T, D>(nc, g);
       T f[D];
 #pragma unroll
       for (int c = 0; c < D; c++) {
         T s = 0, ap = mv[c];
 #pragma unroll
         for (int j = 0; j < D; j++) {
-          s += tau[c + D * j] * g[j];
+          s += tau[c + D * j] * gr[j];
           ap += mC[c][j] * dz[j];
         }
         f[c] = wt * ap - a.dt * s;
@@ -745,8 +758,9 @@ __global__ __launch_bounds__(BLOCK) void grid_kernel(Args<T, D> a) {
     }
     T vf[D];
     if (a.nwalls > 0) {
+      const Geom<T, D> g = tile_geom(a, t);
       int org[D], nc[D];
-      tile_origin<D>(t, a.g.GT, org);
+      tile_origin<T, D>(t, g, org);
       uint32_t r = l;
 #pragma unroll
       for (int c = D - 1; c >= 0; c--) {
@@ -755,7 +769,7 @@ __global__ __launch_bounds__(BLOCK) void grid_kernel(Args<T, D> a) {
       }
       T X0[D];
 #pragma unroll
-      for (int c = 0; c < D; c++) X0[c] = a.g.lo[c] + ((T)(nc[c] - 3) + (T)0.5) * a.g.dx[c];
+      for (int c = 0; c < D; c++) X0[c] = g.lo[c] + ((T)(nc[c] - 3) + (T)0.5) * g.dx[c];
       T vpre[D];
 #pragma unroll
       for (int c = 0; c < D; c++) vpre[c] = v[c];
@@ -1017,8 +1031,8 @@ __device__ __forceinline__ void g2p_gather(const Geom<T, D>& gm, T dt, const Wts
 // stencil base leaves its binned tile marks the tile it moves to, so that the
 // next step clears and updates the grid tiles around it.
 template<typename T, int D>
-__device__ __forceinline__ void g2p_store(const Args<T, D>& a, uint32_t p, const T* x, const T* vb, const T* vfb,
-                                          const T* G, const T* Bs, uint32_t bin_tile) {
+__device__ __forceinline__ void g2p_store(const Args<T, D>& a, const Geom<T, D>& g, uint32_t p, const T* x,
+                                          const T* vb, const T* vfb, const T* G, const T* Bs, uint32_t bin_tile) {
   T Fm[D * D], Fn[D * D];
 #pragma unroll
   for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
@@ -1036,7 +1050,7 @@ __device__ __forceinline__ void g2p_store(const Args<T, D>& a, uint32_t p, const
 #pragma unroll
   for (int c = 0; c < D; c++) {
     xn[c] = x[c] + a.dt * vb[c];
-    if (xn[c] < a.vlo[c] || xn[c] > a.vhi[c]) ok = false;
+    if (xn[c] < g.vlo[c] || xn[c] > g.vhi[c]) ok = false;
   }
 #pragma unroll
   for (int c = 0; c < D; c++) { a.X[c * a.n + p] = xn[c]; a.V[c * a.n + p] = vfb[c]; }
@@ -1045,8 +1059,8 @@ __device__ __forceinline__ void g2p_store(const Args<T, D>& a, uint32_t p, const
   if (!ok) {
     a.valid[p] = 0;
   } else {
-    const uint32_t nt = base_tile<T, D>(a.g, xn);
-    if (nt != bin_tile && nt < (uint32_t)a.g.ntiles) a.mstamp[nt] = a.next_stamp;
+    const uint32_t nt = base_tile<T, D>(g, xn);
+    if (nt != bin_tile && nt < a.ntotal) a.mstamp[nt] = a.next_stamp;
   }
 }
 
@@ -1060,14 +1074,15 @@ __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
   const uint32_t ntile = a.counts[0];
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
     const uint32_t t = a.ptiles[seg];
+    const Geom<T, D> g = tile_geom(a, t);
     int org[D];
-    tile_origin<D>(t, a.g.GT, org);
+    tile_origin<T, D>(t, g, org);
     for (int k = threadIdx.x; k < RN; k += BLOCK) {
       int off[D], nc[D];
       region_offsets<D>(k, off);
 #pragma unroll
       for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
-      const uint32_t slot = node_slot<D>(nc, a.g.GT);
+      const uint32_t slot = node_slot<T, D>(nc, g);
       const Node4<T> nd = a.grid[slot];
       s_v[0][k] = nd.x;
       s_v[1][k] = nd.y;
@@ -1088,7 +1103,7 @@ __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
 #pragma unroll
       for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
       Wts<T, D> w;
-      weights<T, D>(a.g, x, w);
+      weights<T, D>(g, x, w);
       bool in = true;
       int lb[D];
 #pragma unroll
@@ -1106,8 +1121,8 @@ __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
         else return nv(ox, oy, oz, c);
       };
       T vb[D], vfb[D], G[D * D], Bs[D * D];
-      g2p_gather<T, D, FRICTION>(a.g, a.dt, w, nv, nf, vb, vfb, G, Bs);
-      g2p_store<T, D>(a, p, x, vb, vfb, G, Bs, t);
+      g2p_gather<T, D, FRICTION>(g, a.dt, w, nv, nf, vb, vfb, G, Bs);
+      g2p_store<T, D>(a, g, p, x, vb, vfb, G, Bs, t);
     }
     __syncthreads();
   }
@@ -1119,21 +1134,22 @@ __global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
   const uint32_t nm = a.counts[4];
   for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < nm; q += gridDim.x * BLOCK) {
     const uint32_t p = a.mover_list[q];
+    const Geom<T, D> g = particle_geom(a, p);
     T x[D];
 #pragma unroll
     for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
     Wts<T, D> w;
-    weights<T, D>(a.g, x, w);
+    weights<T, D>(g, x, w);
     bool in = true;
 #pragma unroll
-    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < a.g.G[c];
+    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < g.G[c];
     if (!in) continue;  // flagged by p2g_extra_kernel
     auto slot = [&](int ox, int oy, int oz) -> uint32_t {
       int nc[D];
       nc[0] = w.base[0] + ox;
       nc[1] = w.base[1] + oy;
       if constexpr (D == 3) nc[D - 1] = w.base[D - 1] + oz;
-      return node_slot<D>(nc, a.g.GT);
+      return node_slot<T, D>(nc, g);
     };
     auto comp = [](const Node4<T>& nd, int c) -> T { return c == 0 ? nd.x : (c == 1 ? nd.y : nd.z); };
     auto nv = [&](int ox, int oy, int oz, int c) -> T { return comp(a.grid[slot(ox, oy, oz)], c); };
@@ -1142,8 +1158,8 @@ __global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
       else return nv(ox, oy, oz, c);
     };
     T vb[D], vfb[D], G[D * D], Bs[D * D];
-    g2p_gather<T, D, FRICTION>(a.g, a.dt, w, nv, nf, vb, vfb, G, Bs);
-    g2p_store<T, D>(a, p, x, vb, vfb, G, Bs, a.mover_tile[q]);
+    g2p_gather<T, D, FRICTION>(g, a.dt, w, nv, nf, vb, vfb, G, Bs);
+    g2p_store<T, D>(a, g, p, x, vb, vfb, G, Bs, a.mover_tile[q]);
   }
 }
 
@@ -1156,6 +1172,7 @@ __global__ __launch_bounds__(BLOCK) void mark_movers_kernel(Args<T, D> a, uint32
   const uint32_t ntile = a.counts[0];
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
     const uint32_t t = a.ptiles[seg];
+    const Geom<T, D> g = tile_geom(a, t);
     const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
     for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
       const uint32_t p = a.order ? a.order[i] : i;
@@ -1163,11 +1180,11 @@ __global__ __launch_bounds__(BLOCK) void mark_movers_kernel(Args<T, D> a, uint32
       T x[D];
 #pragma unroll
       for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
-      const uint32_t nt = base_tile<T, D>(a.g, x);
+      const uint32_t nt = base_tile<T, D>(g, x);
       if (nt == t) continue;
       if (LIST)
         a.mover_list[atomicAdd(&a.counts[4], 1u)] = p;
-      else if (nt < (uint32_t)a.g.ntiles)
+      else if (nt < a.ntotal)
         a.mstamp[nt] = stamp;
     }
   }
@@ -1232,8 +1249,8 @@ template<typename T> struct AdjArgs {
 // G2P adjoint of particle p (formulas in mpmx_adjoint.h): node velocities
 // come from nv(slot-or-local index, c), node adjoints go to add(index, c, value).
 template<typename T, int D, class Index, class NV, class ADD>
-__device__ __forceinline__ void g2p_adjoint_particle(const Args<T, D>& a, const AdjArgs<T>& q, uint32_t p,
-                                                     const Wts<T, D>& w, Index index, NV nv, ADD add) {
+__device__ __forceinline__ void g2p_adjoint_particle(const Args<T, D>& a, const Geom<T, D>& g, const AdjArgs<T>& q,
+                                                     uint32_t p, const Wts<T, D>& w, Index index, NV nv, ADD add) {
   T Fm[D * D], xb[D], vt[D], Fb[D * D], Bb[D * D], Gb[D * D], G[D * D];
 #pragma unroll
   for (int c = 0; c < D; c++) {
@@ -1260,9 +1277,9 @@ __device__ __forceinline__ void g2p_adjoint_particle(const Args<T, D>& a, const 
   for (int k = 0; k < Tile<D>::K; k++) {
     int o[D];
     T wt, gr[D], H[D][D], dz[D], v[D];
-    stencil_entry<T, D>(a.g, w, k, o, wt, gr, H);
+    stencil_entry<T, D>(g, w, k, o, wt, gr, H);
 #pragma unroll
-    for (int c = 0; c < D; c++) dz[c] = (T)o[c] * a.g.dx[c] - w.xe[c];
+    for (int c = 0; c < D; c++) dz[c] = (T)o[c] * g.dx[c] - w.xe[c];
     const uint32_t ix = index(o);
 #pragma unroll
     for (int c = 0; c < D; c++) v[c] = nv(ix, c);
@@ -1325,14 +1342,15 @@ __global__ __launch_bounds__(BLOCK) void g2p_adjoint_kernel(Args<T, D> a, AdjArg
   const uint32_t ntile = a.counts[0];
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
     const uint32_t t = a.ptiles[seg];
+    const Geom<T, D> g = tile_geom(a, t);
     int org[D];
-    tile_origin<D>(t, a.g.GT, org);
+    tile_origin<T, D>(t, g, org);
     for (int k = threadIdx.x; k < RN; k += BLOCK) {
       int off[D], nc[D];
       region_offsets<D>(k, off);
 #pragma unroll
       for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
-      const Node4<T> nd = a.grid[node_slot<D>(nc, a.g.GT)];
+      const Node4<T> nd = a.grid[node_slot<T, D>(nc, g)];
       s_v[0][k] = nd.x;
       s_v[1][k] = nd.y;
       if constexpr (D == 3) s_v[2][k] = nd.z;
@@ -1348,7 +1366,7 @@ __global__ __launch_bounds__(BLOCK) void g2p_adjoint_kernel(Args<T, D> a, AdjArg
 #pragma unroll
       for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
       Wts<T, D> w;
-      weights<T, D>(a.g, x, w);
+      weights<T, D>(g, x, w);
       bool in = true;
       int lb[D];
 #pragma unroll
@@ -1363,7 +1381,7 @@ __global__ __launch_bounds__(BLOCK) void g2p_adjoint_kernel(Args<T, D> a, AdjArg
       };
       auto nv = [&](uint32_t ix, int c) -> T { return s_v[c][ix]; };
       auto add = [&](uint32_t ix, int c, T val) { atomicAdd(&s_b[c][ix], val); };
-      g2p_adjoint_particle<T, D>(a, q, p, w, index, nv, add);
+      g2p_adjoint_particle<T, D>(a, g, q, p, w, index, nv, add);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < RN; k += BLOCK) {
@@ -1371,7 +1389,7 @@ __global__ __launch_bounds__(BLOCK) void g2p_adjoint_kernel(Args<T, D> a, AdjArg
       region_offsets<D>(k, off);
 #pragma unroll
       for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
-      Node4<T>* an = q.agrid + node_slot<D>(nc, a.g.GT);
+      Node4<T>* an = q.agrid + node_slot<T, D>(nc, g);
       if (s_b[0][k] != (T)0) atomic_add(&an->x, s_b[0][k]);
       if (s_b[1][k] != (T)0) atomic_add(&an->y, s_b[1][k]);
       if constexpr (D == 3)
@@ -1387,20 +1405,21 @@ __global__ __launch_bounds__(BLOCK) void g2p_adjoint_mover_kernel(Args<T, D> a, 
   const uint32_t nm = a.counts[4];
   for (uint32_t m = blockIdx.x * BLOCK + threadIdx.x; m < nm; m += gridDim.x * BLOCK) {
     const uint32_t p = a.mover_list[m];
+    const Geom<T, D> g = particle_geom(a, p);
     T x[D];
 #pragma unroll
     for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
     Wts<T, D> w;
-    weights<T, D>(a.g, x, w);
+    weights<T, D>(g, x, w);
     bool in = true;
 #pragma unroll
-    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < a.g.G[c];
+    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < g.G[c];
     if (!in) continue;
     auto index = [&](const int* o) -> uint32_t {
       int nc[D];
 #pragma unroll
       for (int c = 0; c < D; c++) nc[c] = w.base[c] + o[c];
-      return node_slot<D>(nc, a.g.GT);
+      return node_slot<T, D>(nc, g);
     };
     auto nv = [&](uint32_t ix, int c) -> T {
       const Node4<T>& nd = a.grid[ix];
@@ -1410,7 +1429,7 @@ __global__ __launch_bounds__(BLOCK) void g2p_adjoint_mover_kernel(Args<T, D> a, 
       Node4<T>* an = q.agrid + ix;
       atomic_add(c == 0 ? &an->x : (c == 1 ? &an->y : &an->z), val);
     };
-    g2p_adjoint_particle<T, D>(a, q, p, w, index, nv, add);
+    g2p_adjoint_particle<T, D>(a, g, q, p, w, index, nv, add);
   }
 }
 
@@ -1439,8 +1458,9 @@ __global__ __launch_bounds__(BLOCK) void grid_adjoint_kernel(Args<T, D> a, AdjAr
     vb[1] = vbn.y;
     if constexpr (D == 3) vb[2] = vbn.z;
     if (a.nwalls > 0) {
+      const Geom<T, D> g = tile_geom(a, t);
       int org[D], nc[D];
-      tile_origin<D>(t, a.g.GT, org);
+      tile_origin<T, D>(t, g, org);
       uint32_t r = l;
 #pragma unroll
       for (int c = D - 1; c >= 0; c--) {
@@ -1449,7 +1469,7 @@ __global__ __launch_bounds__(BLOCK) void grid_adjoint_kernel(Args<T, D> a, AdjAr
       }
       T X0[D];
 #pragma unroll
-      for (int c = 0; c < D; c++) X0[c] = a.g.lo[c] + ((T)(nc[c] - 3) + (T)0.5) * a.g.dx[c];
+      for (int c = 0; c < D; c++) X0[c] = g.lo[c] + ((T)(nc[c] - 3) + (T)0.5) * g.dx[c];
       T u0[D], dummy[D];
       int ob;
 #pragma unroll
@@ -1492,12 +1512,13 @@ template<typename T, int D>
 __global__ __launch_bounds__(BLOCK) void p2g_adjoint_kernel(Args<T, D> a, AdjArgs<T> q) {
   for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < a.n; p += gridDim.x * BLOCK) {
     if (!a.valid[p]) continue;
+    const Geom<T, D> g = particle_geom(a, p);
     T x[D], v[D], Cm[D * D], Fm[D * D], tau[D * D], Dinv[D];
 #pragma unroll
     for (int c = 0; c < D; c++) {
       x[c] = a.X[c * a.n + p];
       v[c] = a.V[c * a.n + p];
-      Dinv[c] = (T)4 * a.g.inv_dx[c] * a.g.inv_dx[c];
+      Dinv[c] = (T)4 * g.inv_dx[c] * g.inv_dx[c];
     }
 #pragma unroll
     for (int c = 0; c < D; c++)
@@ -1509,10 +1530,10 @@ __global__ __launch_bounds__(BLOCK) void p2g_adjoint_kernel(Args<T, D> a, AdjArg
     const T m = a.mass[p], vol = a.vol[p], mu = a.mu[p], la = a.la[p];
     kirchhoff_any<T, D>(a.model, mu, la, a.thr, a.root, vol, Fm, tau);
     Wts<T, D> w;
-    weights<T, D>(a.g, x, w);
+    weights<T, D>(g, x, w);
     bool in = true;
 #pragma unroll
-    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < a.g.G[c];
+    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < g.G[c];
     if (!in) continue;
     T vbar[D], Cbar[D * D], tbar[D * D], xbar[D];
 #pragma unroll
@@ -1523,13 +1544,13 @@ __global__ __launch_bounds__(BLOCK) void p2g_adjoint_kernel(Args<T, D> a, AdjArg
     for (int k = 0; k < Tile<D>::K; k++) {
       int o[D], nc[D];
       T wt, gr[D], H[D][D], dz[D];
-      stencil_entry<T, D>(a.g, w, k, o, wt, gr, H);
+      stencil_entry<T, D>(g, w, k, o, wt, gr, H);
 #pragma unroll
       for (int c = 0; c < D; c++) {
         nc[c] = w.base[c] + o[c];
-        dz[c] = (T)o[c] * a.g.dx[c] - w.xe[c];
+        dz[c] = (T)o[c] * g.dx[c] - w.xe[c];
       }
-      const Node4<T> nb = q.agrid[node_slot<D>(nc, a.g.GT)];
+      const Node4<T> nb = q.agrid[node_slot<T, D>(nc, g)];
       const T Pb[3] = {nb.x, nb.y, nb.z};
       T s = nb.m * m, tp[D], ct[D];
 #pragma unroll
@@ -1603,6 +1624,7 @@ template<typename T, int D> struct ParticleBufs {
   T* X; T* V; T* F; T* B; T* mass; T* vol; T* mu; T* la;
   uint8_t* valid;
   uint32_t* pid;
+  uint16_t* pscene;  // scene of each particle (batches only)
 };
 
 template<typename T, int D>
@@ -1627,6 +1649,7 @@ __global__ __launch_bounds__(BLOCK) void reorder_kernel(ParticleBufs<T, D> src, 
   dst.la[i] = src.la[p];
   dst.valid[i] = src.valid[p];
   dst.pid[i] = src.pid[p];
+  if (src.pscene) dst.pscene[i] = src.pscene[p];
 }
 
 // Host layout (per particle contiguous, user order) <-> device SoA (slot order).
@@ -1665,7 +1688,7 @@ __global__ __launch_bounds__(BLOCK) void export_grid_kernel(Args<T, D> a, T* dma
     const uint32_t t = a.gtiles[idx >> 6];
     const uint32_t l = (uint32_t)(idx & 63u);
     int org[D], nc[D];
-    tile_origin<D>(t, a.g.GT, org);
+    tile_origin<T, D>(t, a.g, org);
     uint32_t r = l;
     bool in = true;
 #pragma unroll
@@ -1840,26 +1863,41 @@ static const char* kPhaseNames[PhaseTimer::NPH] = {"bin",  "p2g",  "grid",  "g2p
 template<typename T, int D>
 class Engine final : public EngineBase {
  public:
-  explicit Engine(const mpmx_grid_desc& d) {
-    for (int a = 0; a < D; a++) {
-      if (d.cells[a] < 1 || d.cells[a] > (1 << 14)) throw Error{MPMX_EINVAL, "cells out of range"};
-      if (!(d.hi[a] > d.lo[a])) throw Error{MPMX_EINVAL, "empty domain"};
-      g_.lo[a] = (T)d.lo[a];
-      g_.hi[a] = (T)d.hi[a];
-      g_.dx[a] = (g_.hi[a] - g_.lo[a]) / (T)d.cells[a];  // GRID.cpp:23-24
-      g_.inv_dx[a] = (T)1 / g_.dx[a];
-      g_.G[a] = d.cells[a] + 6;
-      g_.GT[a] = (g_.G[a] + Tile<D>::E - 1) / Tile<D>::E + 1;
-      cells_[a] = d.cells[a];
-      // Reflect_Or_Invalidate_Particle bounds: eps = 4 machine-eps * (B - A)
-      const T eps = std::numeric_limits<T>::epsilon() * (T)4 * (g_.hi[a] - g_.lo[a]);
-      vlo_[a] = g_.lo[a] + eps;
-      vhi_[a] = g_.hi[a] - eps;
+  // One scene, or a batch of independent scenes (each with its own grid) that
+  // share every launch: their tiles are numbered consecutively.
+  Engine(const mpmx_grid_desc* descs, int nscenes) {
+    if (nscenes < 1 || nscenes > 65535) throw Error{MPMX_EINVAL, "scene count must be in [1, 65535]"};
+    uint64_t total = 0;
+    for (int s = 0; s < nscenes; s++) {
+      const mpmx_grid_desc& d = descs[s];
+      if (d.dim != D || d.precision != (int)sizeof(T))
+        throw Error{MPMX_EINVAL, "all scenes of a batch need the same dim and precision"};
+      Geom<T, D> g{};
+      for (int a = 0; a < D; a++) {
+        if (d.cells[a] < 1 || d.cells[a] > (1 << 14)) throw Error{MPMX_EINVAL, "cells out of range"};
+        if (!(d.hi[a] > d.lo[a])) throw Error{MPMX_EINVAL, "empty domain"};
+        g.lo[a] = (T)d.lo[a];
+        g.hi[a] = (T)d.hi[a];
+        g.dx[a] = (g.hi[a] - g.lo[a]) / (T)d.cells[a];  // GRID.cpp:23-24
+        g.inv_dx[a] = (T)1 / g.dx[a];
+        g.G[a] = d.cells[a] + 6;
+        g.GT[a] = (g.G[a] + Tile<D>::E - 1) / Tile<D>::E + 1;
+        // Reflect_Or_Invalidate_Particle bounds: eps = 4 machine-eps * (B - A)
+        const T eps = std::numeric_limits<T>::epsilon() * (T)4 * (g.hi[a] - g.lo[a]);
+        g.vlo[a] = g.lo[a] + eps;
+        g.vhi[a] = g.hi[a] - eps;
+      }
+      uint64_t nt = 1;
+      for (int a = 0; a < D; a++) nt *= (uint64_t)g.GT[a];
+      g.ntiles = (int)std::min<uint64_t>(nt, 0x7fffffff);
+      g.tile0 = (uint32_t)total;
+      total += nt;
+      if (total >= 0xffffffffull / 64) throw Error{MPMX_EINVAL, "grid too large"};
+      scenes_.push_back(g);
     }
-    uint64_t nt = 1;
-    for (int a = 0; a < D; a++) nt *= (uint64_t)g_.GT[a];
-    if (nt >= 0xffffffffull / 64) throw Error{MPMX_EINVAL, "grid too large"};
-    g_.ntiles = (int)nt;
+    g_ = scenes_[0];
+    ntotal_ = (uint32_t)total;
+    const uint64_t nt = total;
     key_bits_ = 1;
     while ((1ull << key_bits_) < nt + 1) key_bits_++;
     for (int a = 0; a < D; a++) grav_[a] = 0;
@@ -1888,6 +1926,16 @@ class Engine final : public EngineBase {
     stats_blocks_ = (int)std::min<uint64_t>(std::max<uint64_t>(cdiv(nt * 64, BLOCK), 1), 2048);
     stats_partial_.alloc((size_t)stats_blocks_ * 5 + 8);
     stats_out_.alloc(8);
+    if (nscenes > 1) {
+      scenes_dev_.alloc(scenes_.size());
+      HIPCHECK(hipMemcpyAsync(scenes_dev_.p, scenes_.data(), scenes_.size() * sizeof(Geom<T, D>),
+                              hipMemcpyHostToDevice, stream_));
+      std::vector<uint16_t> ts(nt);
+      for (size_t s = 0; s < scenes_.size(); s++)
+        std::fill(ts.begin() + scenes_[s].tile0, ts.begin() + scenes_[s].tile0 + scenes_[s].ntiles, (uint16_t)s);
+      tscene_.alloc(nt);
+      HIPCHECK(hipMemcpyAsync(tscene_.p, ts.data(), nt * sizeof(uint16_t), hipMemcpyHostToDevice, stream_));
+    }
     HIPCHECK(hipStreamSynchronize(stream_));
   }
   ~Engine() override {
@@ -1907,6 +1955,28 @@ class Engine final : public EngineBase {
 
   void set_particles(int64_t n, const void* X, const void* V, const void* F, const void* B, const void* mass,
                      const void* volume, const void* mu, const void* lambda, const uint8_t* valid) override {
+    if (scenes_.size() > 1) throw Error{MPMX_ESTATE, "a batch context takes mpmx_set_batch_particles"};
+    upload_particles(n, X, V, F, B, mass, volume, mu, lambda, valid, nullptr);
+  }
+  void set_batch_particles(int nscenes, const int64_t* counts, const void* X, const void* V, const void* F,
+                           const void* B, const void* mass, const void* volume, const void* mu, const void* lambda,
+                           const uint8_t* valid) override {
+    if (nscenes != (int)scenes_.size()) throw Error{MPMX_EINVAL, "scene count differs from the context's"};
+    if (!counts) throw Error{MPMX_EINVAL, "null counts"};
+    std::vector<uint16_t> ids;
+    for (int s = 0; s < nscenes; s++) {
+      if (counts[s] < 0) throw Error{MPMX_EINVAL, "negative particle count"};
+      if ((int64_t)ids.size() + counts[s] > (int64_t)0x7fffffff) throw Error{MPMX_EINVAL, "too many particles"};
+      ids.insert(ids.end(), (size_t)counts[s], (uint16_t)s);
+    }
+    upload_particles((int64_t)ids.size(), X, V, F, B, mass, volume, mu, lambda, valid,
+                     scenes_.size() > 1 ? ids.data() : nullptr);
+  }
+  int scene_count() const override { return (int)scenes_.size(); }
+
+  void upload_particles(int64_t n, const void* X, const void* V, const void* F, const void* B, const void* mass,
+                        const void* volume, const void* mu, const void* lambda, const uint8_t* valid,
+                        const uint16_t* scene_ids) {
     if (n < 0 || n > (int64_t)0x7fffffff) throw Error{MPMX_EINVAL, "particle count out of range"};
     if (n > 0 && (!X || !V || !F || !B || !mass || !volume || !mu || !lambda || !valid))
       throw Error{MPMX_EINVAL, "null particle array"};
@@ -1922,6 +1992,7 @@ class Engine final : public EngineBase {
       upload_soa((const T*)mu, cur_.mu, 1);
       upload_soa((const T*)lambda, cur_.la, 1);
       HIPCHECK(hipMemcpyAsync(cur_.valid, valid, n_, hipMemcpyHostToDevice, stream_));
+      if (scene_ids) HIPCHECK(hipMemcpyAsync(cur_.pscene, scene_ids, n_ * sizeof(uint16_t), hipMemcpyHostToDevice, stream_));
       iota_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_.pid, n_);
       HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
       Args<T, D> a = make_args(0);
@@ -2031,6 +2102,7 @@ class Engine final : public EngineBase {
     return c;
   }
   void get_grid(void* mass, void* velocity) override {
+    if (scenes_.size() > 1) throw Error{MPMX_ESTATE, "mpmx_get_grid takes a single-scene context"};
     if (!has_grid_) throw Error{MPMX_ESTATE, "no grid yet: call mpmx_step first"};
     const int64_t nn = grid_node_count();
     DevBuf<T> dm, dv;
@@ -2105,7 +2177,7 @@ class Engine final : public EngineBase {
     const size_t n = n_;
     aX_.alloc(n * D); aV_.alloc(n * D); aF_.alloc(n * D * D); aB_.alloc(n * D * D);
     amu_.alloc(n); ala_.alloc(n);
-    agrid_.alloc((size_t)g_.ntiles * 64);
+    agrid_.alloc((size_t)ntotal_ * 64);
     upload_adjoint((const T*)dX, aX_.p, D);
     upload_adjoint((const T*)dV, aV_.p, D);
     upload_adjoint((const T*)dF, aF_.p, D * D);
@@ -2124,7 +2196,7 @@ class Engine final : public EngineBase {
     std::vector<StateBuf> seg(kk);
     std::vector<StepGrid> sgrid(kk);
     seg_counts_.alloc(kk);
-    const size_t nodes = (size_t)g_.ntiles * 64;
+    const size_t nodes = (size_t)ntotal_ * 64;
     replaying_ = true;
     hipEvent_t ea = timer_.begin(stream_);
     for (int sg = ns > 0 ? (ns - 1) / k : -1; sg >= 0; sg--) {
@@ -2136,7 +2208,7 @@ class Engine final : public EngineBase {
         save_state(seg[i]);
         sgrid[i].grid.alloc(nodes);
         sgrid[i].gridu.alloc(nodes);
-        sgrid[i].tiles.alloc((size_t)g_.ntiles);
+        sgrid[i].tiles.alloc((size_t)ntotal_);
         grid_override_ = sgrid[i].grid.p;
         gridu_override_ = sgrid[i].gridu.p;
         step_once((T)tape_dt_[(size_t)j]);
@@ -2216,22 +2288,25 @@ class Engine final : public EngineBase {
 
  private:
   void allocate_particles() {
+    const bool batch = scenes_.size() > 1;
     auto alloc_set = [&](ParticleBufs<T, D>& pb, std::vector<DevBuf<T>>& fb, DevBuf<uint8_t>& vb,
-                         DevBuf<uint32_t>& ib) {
+                         DevBuf<uint32_t>& ib, DevBuf<uint16_t>& sb) {
       fb.resize(8);
       const size_t n = std::max<uint32_t>(n_, 1);
       fb[0].alloc(n * D); fb[1].alloc(n * D); fb[2].alloc(n * D * D); fb[3].alloc(n * D * D);
       fb[4].alloc(n); fb[5].alloc(n); fb[6].alloc(n); fb[7].alloc(n);
       vb.alloc(n);
       ib.alloc(n);
-      pb = ParticleBufs<T, D>{fb[0].p, fb[1].p, fb[2].p, fb[3].p, fb[4].p, fb[5].p, fb[6].p, fb[7].p, vb.p, ib.p};
+      if (batch) sb.alloc(n);
+      pb = ParticleBufs<T, D>{fb[0].p, fb[1].p, fb[2].p, fb[3].p, fb[4].p, fb[5].p, fb[6].p, fb[7].p,
+                              vb.p,    ib.p,    batch ? sb.p : nullptr};
     };
     // Buffers are sized exactly to n so that the SoA stride equals n.
     for (auto* fbv : {&fa_, &fb_})
       for (auto& b : *fbv) b.release();
-    va_.release(); vb_.release(); ia_.release(); ib_.release();
-    alloc_set(cur_, fa_, va_, ia_);
-    alloc_set(alt_, fb_, vb_, ib_);
+    va_.release(); vb_.release(); ia_.release(); ib_.release(); sa_.release(); sb_.release();
+    alloc_set(cur_, fa_, va_, ia_, sa_);
+    alloc_set(alt_, fb_, vb_, ib_, sb_);
     const size_t n = std::max<uint32_t>(n_, 1);
     keys_.release(); skeys_.release(); order_.release(); ktmp_.release(); vtmp_.release();
     keys_.alloc(n); skeys_.alloc(n); order_.alloc(n); ktmp_.alloc(n); vtmp_.alloc(n);
@@ -2296,13 +2371,11 @@ class Engine final : public EngineBase {
   // each with its -octant neighbours) -> gtiles, counts[1]; their nodes in
   // z0 (and z1, z2 if set) are cleared.  Also clears counts[3], counts[4].
   void build_grid_tiles(uint32_t ms, Node4<T>* z0, Node4<T>* z1, Node4<T>* z2) {
-    const uint32_t nt = (uint32_t)g_.ntiles, nb = cdiv(nt, BLOCK);
-    gcount_kernel<D, true><<<nb, BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, ms, nt, g_.GT[0],
-                                                      D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gblock_.p,
-                                                      counts_.p);
-    glist_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, ms, nt, g_.GT[0],
-                                                        D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gblock_.p,
-                                                        gtiles_.p, counts_.p + 1, z0, z1, z2);
+    const uint32_t nb = cdiv(ntotal_, BLOCK);
+    const Args<T, D> a = make_args(0);
+    gcount_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms, gblock_.p);
+    glist_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms, gblock_.p, gtiles_.p,
+                                                        counts_.p + 1, z0, z1, z2);
   }
 
   // Reverse of one step from the state it started from (in cur_) and the grid
@@ -2342,11 +2415,12 @@ class Engine final : public EngineBase {
   Args<T, D> make_args(T dt) {
     Args<T, D> a{};
     a.g = g_;
-    for (int c = 0; c < D; c++) {
-      a.vlo[c] = vlo_[c];
-      a.vhi[c] = vhi_[c];
-      a.grav[c] = grav_[c];
-    }
+    const bool batch = scenes_.size() > 1;
+    a.scenes = batch ? scenes_dev_.p : nullptr;
+    a.tscene = batch ? tscene_.p : nullptr;
+    a.pscene = batch ? cur_.pscene : nullptr;
+    a.ntotal = ntotal_;
+    for (int c = 0; c < D; c++) a.grav[c] = grav_[c];
     a.dt = dt;
     a.model = model_;
     a.thr = thr_;
@@ -2381,7 +2455,7 @@ class Engine final : public EngineBase {
       if (walls_[i].friction != (T)0 && walls_[i].type != 0) return true;
     return false;
   }
-  uint32_t tile_blocks() const { return (uint32_t)std::min<int>(std::max(g_.ntiles, 1), 8192); }
+  uint32_t tile_blocks() const { return std::min<uint32_t>(std::max<uint32_t>(ntotal_, 1), 8192); }
   uint32_t grid_blocks() const { return (uint32_t)stats_blocks_; }
 
   // Rebuild the bins when forced, when max_interval steps have passed, or when
@@ -2407,13 +2481,10 @@ class Engine final : public EngineBase {
     bin_stamp_++;
     bounds_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(skeys_.p, n_, tile_begin_.p, tile_end_.p, tile_stamp_.p,
                                                           bin_stamp_);
-    const uint32_t nt = (uint32_t)g_.ntiles, nb = cdiv(nt, BLOCK);
-    gcount_kernel<D, false><<<nb, BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, 0u, nt, g_.GT[0],
-                                                       D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gblock_.p,
-                                                       counts_.p);
-    glist_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(tile_stamp_.p, bin_stamp_, mstamp_.p, 0u, nt, g_.GT[0],
-                                                         D > 1 ? g_.GT[1] : 1, D > 2 ? g_.GT[D - 1] : 1, gblock_.p,
-                                                         ptiles_.p, counts_.p + 0, nullptr, nullptr, nullptr);
+    const uint32_t nb = cdiv(ntotal_, BLOCK);
+    gcount_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, 0u, gblock_.p);
+    glist_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, 0u, gblock_.p, ptiles_.p,
+                                                         counts_.p + 0, nullptr, nullptr, nullptr);
     timer_.end(PH_REBIN, e0, stream_);
     identity_order_ = false;
     if (!taping_ && (force_reorder_ || since_reorder_ >= reorder_interval_)) {
@@ -2423,6 +2494,7 @@ class Engine final : public EngineBase {
       std::swap(fa_, fb_);
       std::swap(va_, vb_);
       std::swap(ia_, ib_);
+      std::swap(sa_, sb_);
       identity_order_ = true;
       since_reorder_ = 0;
       force_reorder_ = false;
@@ -2435,7 +2507,7 @@ class Engine final : public EngineBase {
 
   void step_once(T dt) {
     const bool fric = any_friction();
-    if (fric) gridf_.alloc((size_t)g_.ntiles * 64);
+    if (fric) gridf_.alloc((size_t)ntotal_ * 64);
     hipEvent_t e_step = timer_.begin(stream_);
     if (n_ == 0) {
       time_ += (double)dt;
@@ -2496,8 +2568,10 @@ class Engine final : public EngineBase {
   }
 
   Geom<T, D> g_{};
-  int cells_[D];
-  T vlo_[D], vhi_[D];
+  std::vector<Geom<T, D>> scenes_;
+  DevBuf<Geom<T, D>> scenes_dev_;
+  DevBuf<uint16_t> tscene_;
+  uint32_t ntotal_ = 0;
   T grav_[D];
   int model_ = 0;
   T thr_ = (T)0.25, root_ = (T)(D == 2 ? 0.5 : 0.6299605249474366);
@@ -2543,6 +2617,7 @@ class Engine final : public EngineBase {
   std::vector<DevBuf<T>> fa_, fb_;
   DevBuf<uint8_t> va_, vb_, stage_bytes_;
   DevBuf<uint32_t> ia_, ib_;
+  DevBuf<uint16_t> sa_, sb_;
   DevBuf<uint32_t> keys_, skeys_, order_, ktmp_, vtmp_;
   DevBuf<char> sort_scratch_;
   DevBuf<uint32_t> tile_begin_, tile_end_, tile_stamp_, mstamp_, gblock_, ptiles_, gtiles_, counts_, err_;
@@ -2553,11 +2628,12 @@ class Engine final : public EngineBase {
   PhaseTimer timer_;
 };
 
-EngineBase* make_engine(const mpmx_grid_desc& d) {
-  if (d.dim == 3 && d.precision == MPMX_FP32) return new Engine<float, 3>(d);
-  if (d.dim == 3 && d.precision == MPMX_FP64) return new Engine<double, 3>(d);
-  if (d.dim == 2 && d.precision == MPMX_FP32) return new Engine<float, 2>(d);
-  if (d.dim == 2 && d.precision == MPMX_FP64) return new Engine<double, 2>(d);
+EngineBase* make_engine(const mpmx_grid_desc* d, int nscenes) {
+  if (!d || nscenes < 1) throw Error{MPMX_EINVAL, "no grid description"};
+  if (d[0].dim == 3 && d[0].precision == MPMX_FP32) return new Engine<float, 3>(d, nscenes);
+  if (d[0].dim == 3 && d[0].precision == MPMX_FP64) return new Engine<double, 3>(d, nscenes);
+  if (d[0].dim == 2 && d[0].precision == MPMX_FP32) return new Engine<float, 2>(d, nscenes);
+  if (d[0].dim == 2 && d[0].precision == MPMX_FP64) return new Engine<double, 2>(d, nscenes);
   throw Error{MPMX_EINVAL, "dim must be 2 or 3 and precision MPMX_FP32 or MPMX_FP64"};
 }
 

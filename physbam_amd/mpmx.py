@@ -19,7 +19,8 @@ SHAPE_BOX, SHAPE_SPHERE = 0, 1
 
 # Every exported symbol of the C ABI, checked by the CPU test suite.
 EXPORTS = (
-    "mpmx_last_error", "mpmx_create", "mpmx_destroy", "mpmx_set_particles", "mpmx_get_particles",
+    "mpmx_last_error", "mpmx_create", "mpmx_destroy", "mpmx_create_batch", "mpmx_scene_count",
+    "mpmx_set_batch_particles", "mpmx_set_particles", "mpmx_get_particles",
     "mpmx_particle_count", "mpmx_set_constitutive", "mpmx_set_gravity", "mpmx_add_collision_box",
     "mpmx_clear_collision_objects", "mpmx_step", "mpmx_step_async", "mpmx_synchronize", "mpmx_time",
     "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_grid_node_count",
@@ -64,6 +65,9 @@ def lib():
         L.mpmx_last_error.restype = ctypes.c_char_p
         L.mpmx_create.argtypes = [ctypes.POINTER(GridDesc), ctypes.POINTER(vp)]
         L.mpmx_destroy.argtypes = [vp]
+        L.mpmx_create_batch.argtypes = [ctypes.POINTER(GridDesc), c_int, ctypes.POINTER(vp)]
+        L.mpmx_scene_count.argtypes = [vp]
+        L.mpmx_set_batch_particles.argtypes = [vp, c_int, vp] + [vp] * 9
         L.mpmx_set_particles.argtypes = [vp, i64] + [vp] * 9
         L.mpmx_get_particles.argtypes = [vp] + [vp] * 5
         L.mpmx_particle_count.argtypes = [vp]
@@ -129,6 +133,38 @@ class Context:
         h = ctypes.c_void_p()
         _check(lib().mpmx_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
+
+    @classmethod
+    def batch(cls, dim, grids, precision=FP32):
+        """A batch of independent scenes; grids: list of (cells, lo, hi)."""
+        self = cls.__new__(cls)
+        self.dim, self.precision = dim, precision
+        self.dtype = np.float32 if precision == FP32 else np.float64
+        descs = (GridDesc * len(grids))()
+        for s, (cells, lo, hi) in enumerate(grids):
+            cells = [cells] * dim if np.isscalar(cells) else list(cells)
+            descs[s].dim, descs[s].precision = dim, precision
+            for a in range(3):
+                descs[s].cells[a] = cells[a] if a < dim else 1
+                descs[s].lo[a] = lo[a] if a < dim else 0.0
+                descs[s].hi[a] = hi[a] if a < dim else 1.0
+        h = ctypes.c_void_p()
+        _check(lib().mpmx_create_batch(descs, len(grids), ctypes.byref(h)))
+        self._h = h
+        self.cells = None
+        return self
+
+    def set_batch_particles(self, counts, X, V, F, B, mass, volume, mu, lam, valid=None):
+        """Particles of all scenes concatenated in scene order; counts[s] for scene s."""
+        d = self.dim
+        n = int(np.asarray(X).shape[0])
+        counts = np.ascontiguousarray(counts, np.int64)
+        assert counts.sum() == n
+        arrs = [self._arr(X, (n, d)), self._arr(V, (n, d)), self._arr(F, (n, d * d)), self._arr(B, (n, d * d)),
+                self._arr(mass, (n,)), self._arr(volume, (n,)), self._arr(mu, (n,)), self._arr(lam, (n,))]
+        v = np.ones(n, np.uint8) if valid is None else np.ascontiguousarray(valid, np.uint8)
+        _check(lib().mpmx_set_batch_particles(self._h, len(counts), _p(counts), *[_p(a) for a in arrs], _p(v)))
+        self.n = n
 
     def close(self):
         if getattr(self, "_h", None):
