@@ -156,8 +156,9 @@ int mpmx_get_grid(mpmx_ctx* ctx, void* mass, void* velocity);
    1/128 and 0.  Results do not depend on the policy beyond floating-point
    summation order. */
 int mpmx_set_rebin_policy(mpmx_ctx* ctx, double mover_fraction, int max_interval);
-/* Bin rebuilds so far and the mover count of the last step. */
-int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers);
+/* Bin rebuilds so far, and the movers and FP64-SVD stress particles of the
+   last step (any pointer may be NULL). */
+int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers, int64_t* slow);
 
 /* Tuning: at a bin rebuild, reorder particle storage by tile if at least
    `interval` steps (>=1) have passed since the last reorder. */

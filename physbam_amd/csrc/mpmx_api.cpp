@@ -162,9 +162,9 @@ int mpmx_set_rebin_policy(mpmx_ctx* ctx, double mover_fraction, int max_interval
   return guard([&] { ctx->engine->set_rebin_policy(mover_fraction, max_interval); });
 }
 
-int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers) {
+int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers, int64_t* slow) {
   if (!ctx) return bad("null context");
-  return guard([&] { ctx->engine->bin_stats(rebins, movers); });
+  return guard([&] { ctx->engine->bin_stats(rebins, movers, slow); });
 }
 
 int mpmx_tape_begin(mpmx_ctx* ctx, int checkpoint_every) {

@@ -36,7 +36,7 @@ struct EngineBase {
   virtual void get_grid(void* mass, void* velocity) = 0;
   virtual void set_reorder_interval(int interval) = 0;
   virtual void set_rebin_policy(double mover_fraction, int max_interval) = 0;
-  virtual void bin_stats(int64_t* rebins, int64_t* movers) = 0;
+  virtual void bin_stats(int64_t* rebins, int64_t* movers, int64_t* slow) = 0;
   virtual void tape_begin(int checkpoint_every) = 0;
   virtual void backward(const void* dX, const void* dV, const void* dF, const void* dB, void* gX, void* gV, void* gF,
                         void* gB, double* gmu, double* glambda) = 0;

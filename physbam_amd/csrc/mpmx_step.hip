@@ -46,8 +46,8 @@ constexpr int BLOCK = 256;
 constexpr int EXTRA_BLOCKS = 256;  // grid of the mover / slow-path kernels (grid-stride)
 
 template<typename T, int D> struct Args {
-  Geom<T, D> g;                // the geometry when there is one scene
-  const Geom<T, D>* scenes;    // per-scene geometry of a batch
+  Geom<T, D> g;                // geometry of scene 0 (host-side use and the grid export)
+  const Geom<T, D>* scenes;    // per-scene geometry (one entry for one scene)
   const uint16_t* tscene;      // tile -> scene (null: one scene)
   const uint16_t* pscene;      // particle slot -> scene (null: one scene)
   uint32_t ntotal;             // tiles of all scenes
@@ -81,11 +81,13 @@ template<typename T, int D> struct Args {
   double* stats_partial;
 };
 
+// The geometry always comes from the scene table (one entry for one scene),
+// so only the fields a kernel uses are loaded.
 template<typename T, int D> __device__ __forceinline__ Geom<T, D> tile_geom(const Args<T, D>& a, uint32_t t) {
-  return a.tscene ? a.scenes[a.tscene[t]] : a.g;
+  return a.scenes[a.tscene ? a.tscene[t] : 0];
 }
 template<typename T, int D> __device__ __forceinline__ Geom<T, D> particle_geom(const Args<T, D>& a, uint32_t p) {
-  return a.pscene ? a.scenes[a.pscene[p]] : a.g;
+  return a.scenes[a.pscene ? a.pscene[p] : 0];
 }
 
 // Node origin of tile t (node coordinates of its scene).
@@ -119,6 +121,30 @@ template<int D> __device__ __forceinline__ void region_offsets(int k, int* off) 
 }
 
 template<typename T> __device__ __forceinline__ void atomic_add(T* p, T v) { atomicAdd(p, v); }
+
+// List append with one atomic per wavefront: every lane with `want` gets its
+// own entry index.  Must be reached by all lanes of the wavefront (it votes).
+// One returning atomic per lane on a shared counter would serialise in L2.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return 0;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  return base + (uint32_t)__popcll(m & below);
+}
+
+// Entry of a list of n that work item r takes.  Neighbouring list entries are
+// usually neighbours in space, so consecutive lanes are sent n/64 entries
+// apart: their node atomics then rarely hit the same address.  A bijection on
+// [0, n): the first 64*floor(n/64) entries are transposed, the rest kept.
+__device__ __forceinline__ uint32_t spread_index(uint32_t r, uint32_t n) {
+  const uint32_t m = n >> 6;
+  return r < (m << 6) ? (r & 63u) * m + (r >> 6) : r;
+}
 
 // Tile holding the stencil base of x (node coordinates, ghost offset included).
 template<typename T, int D> __device__ __forceinline__ uint32_t base_tile(const Geom<T, D>& g, const T* x) {
@@ -365,7 +391,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
             if (in) {
               cell = cl;
             } else {
-              const uint32_t q = atomicAdd(&a.counts[4], 1u);
+              const uint32_t q = atomicAdd(&a.counts[4], 1u);  // aggregated per wavefront by the compiler
               a.mover_list[q] = p;
               a.mover_tile[q] = t;
             }
@@ -436,7 +462,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
           if (!kirchhoff_fast<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.vol[p], Fm, tau)) {
 #pragma unroll
             for (int q = 0; q < D * D; q++) tau[q] = 0;
-            a.slow_list[atomicAdd(&a.counts[3], 1u)] = p;
+            a.slow_list[atomicAdd(&a.counts[3], 1u)] = p;  // the compiler aggregates this per wavefront
           }
           s_c[C::M][a_slot] = m;
 #pragma unroll
@@ -602,89 +628,138 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
   }
 }
 
+// Weight, gradient and x_i - x_p of stencil node k (row-major offsets) from
+// the particle's offset xe to its base node (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:
+// 33-56 for degree 2), for kernels that give each node its own lane.
+template<typename T, int D>
+__device__ __forceinline__ void node_weight(const Geom<T, D>& g, const T* xe, int k, int* o, T& wt, T* gr, T* dz) {
+  int r = k;
+#pragma unroll
+  for (int c = D - 1; c >= 0; c--) {
+    o[c] = r % 3;
+    r /= 3;
+  }
+  T w1[D], d1[D];
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    const T inv = g.inv_dx[c];
+    const T x = xe[c] * inv;
+    const T q = x - (o[c] == 0 ? (T)1.5 : (o[c] == 1 ? (T)1 : (T)0.5));  // q0, q1, q2 as in weights()
+    w1[c] = o[c] == 1 ? (T)0.75 - q * q : (T)0.5 * q * q;
+    d1[c] = (o[c] == 1 ? (T)-2 : (T)1) * q * inv;
+    dz[c] = (T)o[c] * g.dx[c] - xe[c];
+  }
+  wt = 1;
+#pragma unroll
+  for (int c = 0; c < D; c++) wt *= w1[c];
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    T gg = d1[c];
+#pragma unroll
+    for (int j = 0; j < D; j++)
+      if (j != c) gg *= w1[j];
+    gr[c] = gg;
+  }
+}
+
 // P2G of what the tile kernel leaves, straight to HBM with atomics:
 //   movers (list [0, counts[4])): the whole APIC transfer and stress impulse;
 //   queued particles (slow list [0, counts[3])): the stress impulse, FP64 SVD path.
+// Pass 1 gives each particle a thread and stores its transfer coefficients in
+// LDS; pass 2 gives each (particle, node) pair a lane, so the 27 atomics of a
+// particle go out in one instruction to 27 different nodes (one lane per
+// particle and node would make neighbouring particles collide in L2).
+template<typename T, int D> struct ScatterCoef {
+  static constexpr int M = 0, MV = 1, MC = 1 + D, TAU = 1 + D + D * D, XE = 1 + D + 2 * D * D, N = XE + D;
+};
+
 template<typename T, int D>
 __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
-  const uint32_t nm = a.counts[4], ns = a.counts[3];
-  for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < nm + ns; q += gridDim.x * BLOCK) {
-    const bool mover = q < nm;
-    const uint32_t p = mover ? a.mover_list[q] : a.slow_list[q - nm];
-    const Geom<T, D> g = particle_geom(a, p);
-    T x[D], Fm[D * D], tau[D * D];
+  using C = ScatterCoef<T, D>;
+  __shared__ T s_c[C::N][BLOCK];
+  __shared__ int s_b[D][BLOCK];
+  __shared__ uint16_t s_sc[BLOCK];
+  const uint32_t nm = a.counts[4], ns = a.counts[3], ntot = nm + ns;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (uint32_t c0 = blockIdx.x * BLOCK; c0 < ntot; c0 += gridDim.x * BLOCK) {
+    // ---- pass 1: one thread per particle
+    const uint32_t q = c0 + threadIdx.x;
+    if (q < ntot) {
+      const bool mover = q < nm;
+      const uint32_t p = mover ? a.mover_list[spread_index(q, nm)] : a.slow_list[spread_index(q - nm, ns)];
+      const Geom<T, D> g = particle_geom(a, p);
+      T x[D], Fm[D * D], tau[D * D];
 #pragma unroll
-    for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+      for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
 #pragma unroll
-    for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
-    Wts<T, D> w;
-    weights<T, D>(g, x, w);
-    bool in = true;
+      for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
+      bool in = true;
+      int base[D];
 #pragma unroll
-    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < g.G[c];
-    if (!in) {
-      atomicOr(a.err, 2u);
-      continue;
-    }
-    if (mover)
-      kirchhoff_any<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
-    else
-      kirchhoff_slow<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
-    T m = 0, mv[D], mC[D][D];
+      for (int c = 0; c < D; c++) {
+        base[c] = stencil_base<T, D>(g, x, c) + 3;
+        in = in && base[c] >= 0 && base[c] + 2 < g.G[c];
+      }
+      if (!in) {
+        atomicOr(a.err, 2u);
+        base[0] = -1;  // skipped in pass 2
+      } else if (mover) {
+        kirchhoff_any<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
+      } else {
+        kirchhoff_slow<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
+      }
+      const T m = mover ? a.mass[p] : (T)0;
+      s_c[C::M][threadIdx.x] = m;
 #pragma unroll
-    for (int c = 0; c < D; c++) mv[c] = 0;
-#pragma unroll
-    for (int c = 0; c < D; c++)
-#pragma unroll
-      for (int j = 0; j < D; j++) mC[c][j] = 0;
-    if (mover) {
-      m = a.mass[p];
-#pragma unroll
-      for (int c = 0; c < D; c++) mv[c] = m * a.V[c * a.n + p];
+      for (int c = 0; c < D; c++) {
+        s_c[C::MV + c][threadIdx.x] = mover ? m * a.V[c * a.n + p] : (T)0;
+        s_c[C::XE + c][threadIdx.x] = x[c] - (g.lo[c] + ((T)(base[c] - 3) + (T)0.5) * g.dx[c]);
+        s_b[c][threadIdx.x] = base[c];
+      }
 #pragma unroll
       for (int j = 0; j < D; j++)
 #pragma unroll
-        for (int c = 0; c < D; c++)
-          mC[c][j] = m * a.B[(c + D * j) * a.n + p] * ((T)4 * g.inv_dx[j] * g.inv_dx[j]);
+        for (int c = 0; c < D; c++) {
+          s_c[C::MC + c + D * j][threadIdx.x] =
+              mover ? m * a.B[(c + D * j) * a.n + p] * ((T)4 * g.inv_dx[j] * g.inv_dx[j]) : (T)0;
+          s_c[C::TAU + c + D * j][threadIdx.x] = a.dt * tau[c + D * j];
+        }
+      s_sc[threadIdx.x] = a.pscene ? a.pscene[p] : 0;
     }
+    __syncthreads();
+    // ---- pass 2: lane = (particle, node); two particles per wavefront instruction
+    const uint32_t cnt = min((uint32_t)BLOCK, ntot - c0);
+    for (uint32_t pp = wv * 2; pp < cnt; pp += 2 * (BLOCK / 64)) {
+      const uint32_t sl = pp + (lane >> 5);
+      const int k = lane & 31;
+      if (sl >= cnt || k >= Tile<D>::K || s_b[0][sl] < 0) continue;
+      const Geom<T, D> g = a.scenes[s_sc[sl]];
+      T xe[D], wt, gr[D], dz[D];
+      int o[D], nc[D];
 #pragma unroll
-    for (int k = 0; k < Tile<D>::K; k++) {
-      int o[D], nc[D], r = k;
-      T gr[D], dz[D];
+      for (int c = 0; c < D; c++) xe[c] = s_c[C::XE + c][sl];
+      node_weight<T, D>(g, xe, k, o, wt, gr, dz);
 #pragma unroll
-      for (int c = D - 1; c >= 0; c--) {
-        o[c] = r % 3;
-        r /= 3;
-      }
-      T wt = 1;
-#pragma unroll
-      for (int c = 0; c < D; c++) {
-        nc[c] = w.base[c] + o[c];
-        dz[c] = (T)o[c] * g.dx[c] - w.xe[c];
-        wt *= w.w[o[c]][c];
-        T gg = w.dw[o[c]][c];
-#pragma unroll
-        for (int j = 0; j < D; j++)
-          if (j != c) gg *= w.w[o[j]][j];
-        gr[c] = gg;
-      }
+      for (int c = 0; c < D; c++) nc[c] = s_b[c][sl] + o[c];
       Node4<T>* nd = a.grid + node_slot<T, D>(nc, g);
       T f[D];
 #pragma unroll
       for (int c = 0; c < D; c++) {
-        T s = 0, ap = mv[c];
+        T st = 0, ap = s_c[C::MV + c][sl];
 #pragma unroll
         for (int j = 0; j < D; j++) {
-          s += tau[c + D * j] * gr[j];
-          ap += mC[c][j] * dz[j];
+          st += s_c[C::TAU + c + D * j][sl] * gr[j];
+          ap += s_c[C::MC + c + D * j][sl] * dz[j];
         }
-        f[c] = wt * ap - a.dt * s;
+        f[c] = wt * ap - st;
       }
-      if (mover) atomic_add(&nd->m, wt * m);
+      const T m = s_c[C::M][sl];
+      if (m != (T)0) atomic_add(&nd->m, wt * m);
       atomic_add(&nd->x, f[0]);
       atomic_add(&nd->y, f[1]);
       if constexpr (D == 3) atomic_add(&nd->z, f[D - 1]);
     }
+    __syncthreads();
   }
 }
 
@@ -1132,7 +1207,8 @@ __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
 template<typename T, int D, bool FRICTION>
 __global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
   const uint32_t nm = a.counts[4];
-  for (uint32_t q = blockIdx.x * BLOCK + threadIdx.x; q < nm; q += gridDim.x * BLOCK) {
+  for (uint32_t r = blockIdx.x * BLOCK + threadIdx.x; r < nm; r += gridDim.x * BLOCK) {
+    const uint32_t q = spread_index(r, nm);
     const uint32_t p = a.mover_list[q];
     const Geom<T, D> g = particle_geom(a, p);
     T x[D];
@@ -1174,18 +1250,26 @@ __global__ __launch_bounds__(BLOCK) void mark_movers_kernel(Args<T, D> a, uint32
     const uint32_t t = a.ptiles[seg];
     const Geom<T, D> g = tile_geom(a, t);
     const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
-    for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
-      const uint32_t p = a.order ? a.order[i] : i;
-      if (!a.valid[p]) continue;
-      T x[D];
+    for (uint32_t i0 = b; i0 < e; i0 += BLOCK) {  // uniform trip count: the append votes
+      const uint32_t i = i0 + threadIdx.x;
+      bool mover = false;
+      uint32_t p = 0, nt = t;
+      if (i < e) {
+        p = a.order ? a.order[i] : i;
+        if (a.valid[p]) {
+          T x[D];
 #pragma unroll
-      for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
-      const uint32_t nt = base_tile<T, D>(g, x);
-      if (nt == t) continue;
-      if (LIST)
-        a.mover_list[atomicAdd(&a.counts[4], 1u)] = p;
-      else if (nt < a.ntotal)
+          for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+          nt = base_tile<T, D>(g, x);
+          mover = nt != t;
+        }
+      }
+      if (LIST) {
+        const uint32_t q = wave_append(&a.counts[4], mover);
+        if (mover) a.mover_list[q] = p;
+      } else if (mover && nt < a.ntotal) {
         a.mstamp[nt] = stamp;
+      }
     }
   }
 }
@@ -1246,11 +1330,18 @@ template<typename T> struct AdjArgs {
   const Node4<T>* gridu;       // (m, u) of the recomputed step
 };
 
-// G2P adjoint of particle p (formulas in mpmx_adjoint.h): node velocities
-// come from nv(slot-or-local index, c), node adjoints go to add(index, c, value).
-template<typename T, int D, class Index, class NV, class ADD>
-__device__ __forceinline__ void g2p_adjoint_particle(const Args<T, D>& a, const Geom<T, D>& g, const AdjArgs<T>& q,
-                                                     uint32_t p, const Wts<T, D>& w, Index index, NV nv, ADD add) {
+// G2P adjoint of particle p, gather part (formulas in mpmx_adjoint.h): node
+// velocities come from nv(index(o), c).  Writes xbar and Fbar of p and returns
+// in coef the node-scatter coefficients: vt = v'bar + dt x'bar, B'bar and
+// Gbar (ScatterCoef layout: MV = vt, MC = B'bar, TAU = -Gbar, M unused).
+template<typename T, int D> struct AdjCoef {
+  static constexpr int VT = 0, BB = D, GB = D + D * D, XE = D + 2 * D * D, N = XE + D;
+};
+
+template<typename T, int D, class Index, class NV>
+__device__ __forceinline__ void g2p_adjoint_gather(const Args<T, D>& a, const Geom<T, D>& g, const AdjArgs<T>& q,
+                                                   uint32_t p, const Wts<T, D>& w, Index index, NV nv, T* coef) {
+  using AC = AdjCoef<T, D>;
   T Fm[D * D], xb[D], vt[D], Fb[D * D], Bb[D * D], Gb[D * D], G[D * D];
 #pragma unroll
   for (int c = 0; c < D; c++) {
@@ -1283,39 +1374,35 @@ __device__ __forceinline__ void g2p_adjoint_particle(const Args<T, D>& a, const 
     const uint32_t ix = index(o);
 #pragma unroll
     for (int c = 0; c < D; c++) v[c] = nv(ix, c);
-    T s1 = 0, hv[D], bv[D], vb[D];
+    T s1 = 0, hv[D], bv[D];
 #pragma unroll
     for (int c = 0; c < D; c++) {
-      T bdz = 0, gbg = 0;
+      T bdz = 0;
 #pragma unroll
       for (int j = 0; j < D; j++) {
         G[c + D * j] += v[c] * gr[j];
         bdz += Bb[c + D * j] * dz[j];
-        gbg += Gb[c + D * j] * gr[j];
       }
       s1 += v[c] * (vt[c] + bdz);
-      vb[c] = wt * (vt[c] + bdz) + gbg;
     }
 #pragma unroll
     for (int j = 0; j < D; j++) {
-      T h = 0, b = 0;
+      T h = 0, bb = 0;
 #pragma unroll
       for (int c = 0; c < D; c++) {
         h += Gb[c + D * j] * v[c];
-        b += Bb[c + D * j] * v[c];
+        bb += Bb[c + D * j] * v[c];
       }
       hv[j] = h;
-      bv[j] = b;
+      bv[j] = bb;
     }
 #pragma unroll
     for (int c = 0; c < D; c++) {
       T hh = 0;
 #pragma unroll
-      for (int b = 0; b < D; b++) hh += H[c][b] * hv[b];
+      for (int bb = 0; bb < D; bb++) hh += H[c][bb] * hv[bb];
       xb[c] += s1 * gr[c] + hh - wt * bv[c];
     }
-#pragma unroll
-    for (int c = 0; c < D; c++) add(ix, c, vb[c]);
   }
 #pragma unroll
   for (int c = 0; c < D; c++) q.aX[c * a.n + p] = xb[c];
@@ -1328,17 +1415,55 @@ __device__ __forceinline__ void g2p_adjoint_particle(const Args<T, D>& a, const 
       for (int r = 0; r < D; r++) s += a.dt * G[r + D * c] * Fb[r + D * j];
       q.aF[(c + D * j) * a.n + p] = s;
     }
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    coef[AC::VT + c] = vt[c];
+    coef[AC::XE + c] = w.xe[c];
+  }
+#pragma unroll
+  for (int k = 0; k < D * D; k++) {
+    coef[AC::BB + k] = Bb[k];
+    coef[AC::GB + k] = Gb[k];
+  }
 }
 
-// One workgroup per binned tile: the region's velocities are staged in LDS
-// and vbar is accumulated there with LDS atomics (one HBM atomic per node and
-// component at the flush, instead of one per particle and node); movers are
+// Scatter part of the G2P adjoint for stencil node k of a particle whose
+// coefficients are column sl of s_c: vbar_i += w vt + Gbar grad w + w B'bar dz.
+template<typename T, int D, int NC>
+__device__ __forceinline__ void g2p_adjoint_node(const Geom<T, D>& g, const T (*s_c)[BLOCK], uint32_t sl, int k,
+                                                 int* o, T* vb) {
+  using AC = AdjCoef<T, D>;
+  T xe[D], wt, gr[D], dz[D];
+#pragma unroll
+  for (int c = 0; c < D; c++) xe[c] = s_c[AC::XE + c][sl];
+  node_weight<T, D>(g, xe, k, o, wt, gr, dz);
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    T s = s_c[AC::VT + c][sl], m = 0;
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      s += s_c[AC::BB + c + D * j][sl] * dz[j];
+      m += s_c[AC::GB + c + D * j][sl] * gr[j];
+    }
+    vb[c] = wt * s + m;
+  }
+}
+
+// One workgroup per binned tile: the region's velocities are staged in LDS;
+// pass 1 gathers each particle (thread per particle) and leaves its scatter
+// coefficients in LDS, pass 2 gives each (particle, node) pair a lane and
+// accumulates vbar in the region with LDS atomics that never collide within a
+// particle; one HBM atomic per node and component at the flush.  Movers are
 // left to g2p_adjoint_mover_kernel.
 template<typename T, int D>
 __global__ __launch_bounds__(BLOCK) void g2p_adjoint_kernel(Args<T, D> a, AdjArgs<T> q) {
+  using AC = AdjCoef<T, D>;
   constexpr int R = Tile<D>::R, RN = Tile<D>::RN, E = Tile<D>::E;
   __shared__ T s_v[D][RN];
   __shared__ T s_b[D][RN];
+  __shared__ T s_c[AC::N][BLOCK];
+  __shared__ int s_li[BLOCK];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t ntile = a.counts[0];
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
     const uint32_t t = a.ptiles[seg];
@@ -1359,31 +1484,55 @@ __global__ __launch_bounds__(BLOCK) void g2p_adjoint_kernel(Args<T, D> a, AdjArg
     }
     __syncthreads();
     const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
-    for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
-      const uint32_t p = a.order ? a.order[i] : i;
-      if (!a.valid[p]) continue;
-      T x[D];
+    for (uint32_t c0 = b; c0 < e; c0 += BLOCK) {
+      // ---- pass 1
+      const uint32_t i = c0 + threadIdx.x;
+      int li0 = -1;
+      if (i < e) {
+        const uint32_t p = a.order ? a.order[i] : i;
+        if (a.valid[p]) {
+          T x[D];
 #pragma unroll
-      for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
-      Wts<T, D> w;
-      weights<T, D>(g, x, w);
-      bool in = true;
-      int lb[D];
+          for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+          Wts<T, D> w;
+          weights<T, D>(g, x, w);
+          bool in = true;
+          int lb[D];
 #pragma unroll
-      for (int c = 0; c < D; c++) {
-        lb[c] = w.base[c] - org[c];
-        in = in && lb[c] >= 0 && lb[c] < E;
+          for (int c = 0; c < D; c++) {
+            lb[c] = w.base[c] - org[c];
+            in = in && lb[c] >= 0 && lb[c] < E;
+          }
+          if (in) {  // else a mover
+            li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
+            auto index = [&](const int* o) -> uint32_t {
+              return (uint32_t)(li0 + (D == 3 ? (o[0] * R + o[1]) * R + o[D - 1] : o[0] * R + o[D - 1]));
+            };
+            auto nv = [&](uint32_t ix, int c) -> T { return s_v[c][ix]; };
+            T coef[AC::N];
+            g2p_adjoint_gather<T, D>(a, g, q, p, w, index, nv, coef);
+#pragma unroll
+            for (int k = 0; k < AC::N; k++) s_c[k][threadIdx.x] = coef[k];
+          }
+        }
       }
-      if (!in) continue;  // mover
-      const int li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
-      auto index = [&](const int* o) -> uint32_t {
-        return (uint32_t)(li0 + (D == 3 ? (o[0] * R + o[1]) * R + o[D - 1] : o[0] * R + o[D - 1]));
-      };
-      auto nv = [&](uint32_t ix, int c) -> T { return s_v[c][ix]; };
-      auto add = [&](uint32_t ix, int c, T val) { atomicAdd(&s_b[c][ix], val); };
-      g2p_adjoint_particle<T, D>(a, g, q, p, w, index, nv, add);
+      s_li[threadIdx.x] = li0;
+      __syncthreads();
+      // ---- pass 2: lane = (particle, node)
+      const uint32_t cnt = min((uint32_t)BLOCK, e - c0);
+      for (uint32_t pp = wv * 2; pp < cnt; pp += 2 * (BLOCK / 64)) {
+        const uint32_t sl = pp + (lane >> 5);
+        const int k = lane & 31;
+        if (sl >= cnt || k >= Tile<D>::K || s_li[sl] < 0) continue;
+        int o[D];
+        T vb[D];
+        g2p_adjoint_node<T, D, AC::N>(g, s_c, sl, k, o, vb);
+        const int li = s_li[sl] + (D == 3 ? (o[0] * R + o[1]) * R + o[D - 1] : o[0] * R + o[D - 1]);
+#pragma unroll
+        for (int c = 0; c < D; c++) atomicAdd(&s_b[c][li], vb[c]);
+      }
+      __syncthreads();
     }
-    __syncthreads();
     for (int k = threadIdx.x; k < RN; k += BLOCK) {
       int off[D], nc[D];
       region_offsets<D>(k, off);
@@ -1399,37 +1548,68 @@ __global__ __launch_bounds__(BLOCK) void g2p_adjoint_kernel(Args<T, D> a, AdjArg
   }
 }
 
-// G2P adjoint of this step's movers, straight from and to HBM.
+// G2P adjoint of this step's movers, straight from and to HBM (same two
+// passes as above, the scatter as HBM atomics).
 template<typename T, int D>
 __global__ __launch_bounds__(BLOCK) void g2p_adjoint_mover_kernel(Args<T, D> a, AdjArgs<T> q) {
+  using AC = AdjCoef<T, D>;
+  __shared__ T s_c[AC::N][BLOCK];
+  __shared__ int s_base[D][BLOCK];
+  __shared__ uint16_t s_sc[BLOCK];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t nm = a.counts[4];
-  for (uint32_t m = blockIdx.x * BLOCK + threadIdx.x; m < nm; m += gridDim.x * BLOCK) {
-    const uint32_t p = a.mover_list[m];
-    const Geom<T, D> g = particle_geom(a, p);
-    T x[D];
+  for (uint32_t c0 = blockIdx.x * BLOCK; c0 < nm; c0 += gridDim.x * BLOCK) {
+    const uint32_t r = c0 + threadIdx.x;
+    s_base[0][threadIdx.x] = -1;
+    if (r < nm) {
+      const uint32_t p = a.mover_list[spread_index(r, nm)];
+      const Geom<T, D> g = particle_geom(a, p);
+      T x[D];
 #pragma unroll
-    for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
-    Wts<T, D> w;
-    weights<T, D>(g, x, w);
-    bool in = true;
+      for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+      Wts<T, D> w;
+      weights<T, D>(g, x, w);
+      bool in = true;
 #pragma unroll
-    for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < g.G[c];
-    if (!in) continue;
-    auto index = [&](const int* o) -> uint32_t {
-      int nc[D];
+      for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < g.G[c];
+      if (in) {
+        auto index = [&](const int* o) -> uint32_t {
+          int nc[D];
 #pragma unroll
-      for (int c = 0; c < D; c++) nc[c] = w.base[c] + o[c];
-      return node_slot<T, D>(nc, g);
-    };
-    auto nv = [&](uint32_t ix, int c) -> T {
-      const Node4<T>& nd = a.grid[ix];
-      return c == 0 ? nd.x : (c == 1 ? nd.y : nd.z);
-    };
-    auto add = [&](uint32_t ix, int c, T val) {
-      Node4<T>* an = q.agrid + ix;
-      atomic_add(c == 0 ? &an->x : (c == 1 ? &an->y : &an->z), val);
-    };
-    g2p_adjoint_particle<T, D>(a, g, q, p, w, index, nv, add);
+          for (int c = 0; c < D; c++) nc[c] = w.base[c] + o[c];
+          return node_slot<T, D>(nc, g);
+        };
+        auto nv = [&](uint32_t ix, int c) -> T {
+          const Node4<T>& nd = a.grid[ix];
+          return c == 0 ? nd.x : (c == 1 ? nd.y : nd.z);
+        };
+        T coef[AC::N];
+        g2p_adjoint_gather<T, D>(a, g, q, p, w, index, nv, coef);
+#pragma unroll
+        for (int k = 0; k < AC::N; k++) s_c[k][threadIdx.x] = coef[k];
+#pragma unroll
+        for (int c = 0; c < D; c++) s_base[c][threadIdx.x] = w.base[c];
+        s_sc[threadIdx.x] = a.pscene ? a.pscene[p] : 0;
+      }
+    }
+    __syncthreads();
+    const uint32_t cnt = min((uint32_t)BLOCK, nm - c0);
+    for (uint32_t pp = wv * 2; pp < cnt; pp += 2 * (BLOCK / 64)) {
+      const uint32_t sl = pp + (lane >> 5);
+      const int k = lane & 31;
+      if (sl >= cnt || k >= Tile<D>::K || s_base[0][sl] < 0) continue;
+      const Geom<T, D> g = a.scenes[s_sc[sl]];
+      int o[D], nc[D];
+      T vb[D];
+      g2p_adjoint_node<T, D, AC::N>(g, s_c, sl, k, o, vb);
+#pragma unroll
+      for (int c = 0; c < D; c++) nc[c] = s_base[c][sl] + o[c];
+      Node4<T>* an = q.agrid + node_slot<T, D>(nc, g);
+      atomic_add(&an->x, vb[0]);
+      atomic_add(&an->y, vb[1]);
+      if constexpr (D == 3) atomic_add(&an->z, vb[D - 1]);
+    }
+    __syncthreads();
   }
 }
 
@@ -1926,10 +2106,10 @@ class Engine final : public EngineBase {
     stats_blocks_ = (int)std::min<uint64_t>(std::max<uint64_t>(cdiv(nt * 64, BLOCK), 1), 2048);
     stats_partial_.alloc((size_t)stats_blocks_ * 5 + 8);
     stats_out_.alloc(8);
+    scenes_dev_.alloc(scenes_.size());
+    HIPCHECK(hipMemcpyAsync(scenes_dev_.p, scenes_.data(), scenes_.size() * sizeof(Geom<T, D>), hipMemcpyHostToDevice,
+                            stream_));
     if (nscenes > 1) {
-      scenes_dev_.alloc(scenes_.size());
-      HIPCHECK(hipMemcpyAsync(scenes_dev_.p, scenes_.data(), scenes_.size() * sizeof(Geom<T, D>),
-                              hipMemcpyHostToDevice, stream_));
       std::vector<uint16_t> ts(nt);
       for (size_t s = 0; s < scenes_.size(); s++)
         std::fill(ts.begin() + scenes_[s].tile0, ts.begin() + scenes_[s].tile0 + scenes_[s].ntiles, (uint16_t)s);
@@ -2127,12 +2307,13 @@ class Engine final : public EngineBase {
     rebin_fraction_ = mover_fraction;
     rebin_max_interval_ = max_interval;
   }
-  void bin_stats(int64_t* rebins, int64_t* movers) override {
+  void bin_stats(int64_t* rebins, int64_t* movers, int64_t* slow) override {
     synchronize();
     uint32_t c[8];
     HIPCHECK(hipMemcpy(c, counts_.p, sizeof(c), hipMemcpyDeviceToHost));
     if (rebins) *rebins = rebins_;
     if (movers) *movers = has_grid_ ? (int64_t)c[4] : 0;
+    if (slow) *slow = has_grid_ ? (int64_t)c[3] : 0;
   }
   void enable_timing(bool on) override {
     synchronize();
@@ -2416,7 +2597,7 @@ class Engine final : public EngineBase {
     Args<T, D> a{};
     a.g = g_;
     const bool batch = scenes_.size() > 1;
-    a.scenes = batch ? scenes_dev_.p : nullptr;
+    a.scenes = scenes_dev_.p;
     a.tscene = batch ? tscene_.p : nullptr;
     a.pscene = batch ? cur_.pscene : nullptr;
     a.ntotal = ntotal_;

@@ -89,7 +89,7 @@ def lib():
         L.mpmx_get_grid.argtypes = [vp, vp, vp]
         L.mpmx_set_reorder_interval.argtypes = [vp, c_int]
         L.mpmx_set_rebin_policy.argtypes = [vp, dbl, c_int]
-        L.mpmx_bin_stats.argtypes = [vp, vp, vp]
+        L.mpmx_bin_stats.argtypes = [vp, vp, vp, vp]
         L.mpmx_tape_begin.argtypes = [vp, c_int]
         L.mpmx_backward.argtypes = [vp] + [vp] * 10
         L.mpmx_enable_timing.argtypes = [vp, c_int]
@@ -262,11 +262,11 @@ class Context:
     def set_rebin_policy(self, mover_fraction=1.0 / 128, max_interval=0):
         _check(lib().mpmx_set_rebin_policy(self._h, float(mover_fraction), int(max_interval)))
 
-    def bin_stats(self):
-        """(bin rebuilds so far, movers in the last step)."""
-        r = np.zeros(2, np.int64)
-        _check(lib().mpmx_bin_stats(self._h, _p(r[:1]), _p(r[1:])))
-        return int(r[0]), int(r[1])
+    def bin_stats(self, with_slow=False):
+        """(bin rebuilds so far, movers in the last step[, FP64-SVD stress particles in the last step])."""
+        r = np.zeros(3, np.int64)
+        _check(lib().mpmx_bin_stats(self._h, _p(r[0:1]), _p(r[1:2]), _p(r[2:3])))
+        return (int(r[0]), int(r[1]), int(r[2])) if with_slow else (int(r[0]), int(r[1]))
 
     def tape_begin(self, checkpoint_every=20):
         _check(lib().mpmx_tape_begin(self._h, int(checkpoint_every)))
