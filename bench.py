@@ -215,6 +215,8 @@ def main():
     ap.add_argument("--rebin-fraction", type=float, default=1.0 / 128,
                     help="rebuild the bins when more than this fraction of particles left their binned tile")
     ap.add_argument("--rebin-interval", type=int, default=0, help="also rebuild the bins every K steps (0: never)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the forward+adjoint / forward measurement and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -230,6 +232,10 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         log(f"copy probe: 3 copies of {args.copy_probe} MiB")
+        return
+
+    if args.adjoint_only:
+        print(json.dumps(adjoint_ratio(log)), flush=True)
         return
 
     if args.gpus != 1:

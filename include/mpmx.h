@@ -164,6 +164,31 @@ int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers, int64_t* slo
    `interval` steps (>=1) have passed since the last reorder. */
 int mpmx_set_reorder_interval(mpmx_ctx* ctx, int interval);
 
+/* Adaptive time step.
+   mpmx_grid_v_upper_bound replaces MPM_DRIVER::Grid_V_Upper_Bound
+   (Hybrid_Methods/Examples_And_Drivers/MPM_DRIVER.cpp:1125-1138, affine CFL
+   on, the default of MPM_EXAMPLE.h:116): per scene, the max over valid
+   particles of |v| + 6 sqrt(dim) min(1/dx) |B|_F, computed in the context's
+   precision with the reference's rounding (bit-identical for the same
+   particle state).  `bound` receives one double per scene.
+   mpmx_compute_dt replaces MPM_DRIVER::Compute_Dt (MPM_DRIVER.cpp:1098-1104):
+   per scene, cfl*min(dx)/bound when the bound exceeds cfl*min(dx)/max_dt,
+   else max_dt.
+   mpmx_advance_frames replaces the substep loop of
+   MPM_DRIVER::Simulate_To_Frame (MPM_DRIVER.cpp:173-197) for `nframes`
+   frames of length frame_dt from the context time: each substep takes
+   Compute_Dt clamped to [min_dt, max_dt], a substep within 1e-4 dt of the
+   frame end lands on it, and the one before the last splits the remainder in
+   half (all in the context's precision).  The substep dts are written to
+   `dts` (up to dts_capacity of them; NULL with capacity 0 skips them) and
+   their number to *nsubsteps.  Single-scene contexts only (MPMX_ESTATE for
+   batches); strong and sound-speed CFL (off by default in the reference) are
+   not provided.  MPMX_ESTATE if a substep dt is not positive. */
+int mpmx_grid_v_upper_bound(mpmx_ctx* ctx, double* bound);
+int mpmx_compute_dt(mpmx_ctx* ctx, double cfl, double max_dt, double* dt);
+int mpmx_advance_frames(mpmx_ctx* ctx, int nframes, double frame_dt, double cfl, double min_dt, double max_dt,
+                        double* dts, int64_t dts_capacity, int64_t* nsubsteps);
+
 /* Differentiation (no counterpart in the reference, whose step has no
    adjoint).  mpmx_tape_begin records the current particle state and then a
    checkpoint every `checkpoint_every` steps of the following mpmx_step calls.

@@ -23,6 +23,8 @@
 //   G2P          MPM_DRIVER.cpp:450-515
 //   invalidation MPM_DRIVER.cpp:1219-1239, simulated list :1142-1157
 //   reductions   Examples_And_Drivers/MPM_EXAMPLE.cpp:209-315
+//   adaptive dt  MPM_DRIVER.cpp:1098-1104 (Compute_Dt), :1125-1138 (Grid_V_Upper_Bound),
+//                :173-197 (substep loop of Simulate_To_Frame), Core/Math_Tools/clamp.h:13-16
 //
 // Each piece is pinned in tests/ against the reference harness outputs
 // (golden vectors under tests/golden/) and against closed forms.
@@ -656,11 +658,98 @@ template<class T,int d> int Run(const int* N,const double* lo,const double* hi,c
                 if(gvfric) gvfric[i*d+a]=G.velocity_friction_save[i][a];}}}
     return 0;
 }
+// MPM_DRIVER::Grid_V_Upper_Bound (MPM_DRIVER.cpp:1125-1138) for use_affine and
+// use_affine_cfl (MPM_EXAMPLE.h:116) with B stored: max over simulated
+// particles of |v| + |B|_F * 6 sqrt(d) / min dx.
+template<class T,int d> T Grid_V_Upper_Bound(const Scene<T,d>& s,const Particles<T,d>& P)
+{
+    T odx=s.one_over_dX[0];
+    for(int a=1;a<d;a++) odx=std::min(odx,s.one_over_dX[a]);
+    T xi=(T)6*std::sqrt((T)d)*odx;
+    T result=0;
+    for(long long p=0;p<P.n;p++){
+        if(!P.valid[p]) continue;
+        Vec<T,d> v;for(int a=0;a<d;a++) v[a]=P.V[p*d+a];
+        Mat<T,d> B;for(int k=0;k<d*d;k++) B.x[k]=P.B[p*d*d+k];
+        T r=magnitude(v)+std::sqrt(Frobenius_Norm_Squared(B))*xi;
+        result=std::max(result,r);}
+    return result;
+}
+
+// MPM_DRIVER::Compute_Dt (MPM_DRIVER.cpp:1098-1104).
+template<class T,int d> T Compute_Dt(const Scene<T,d>& s,const Particles<T,d>& P,T cfl,T max_dt)
+{
+    T dxmin=s.dX[0];
+    for(int a=1;a<d;a++) dxmin=std::min(dxmin,s.dX[a]);
+    T critical_speed=cfl*dxmin/max_dt;
+    T v=Grid_V_Upper_Bound(s,P);
+    return (v>critical_speed)?(cfl*dxmin/v):max_dt;
+}
+
+// The substep loop of MPM_DRIVER::Simulate_To_Frame (MPM_DRIVER.cpp:173-197)
+// for one frame, without Reduce_Dt (strong and sound-speed CFL off, their
+// defaults MPM_EXAMPLE.h:103-104).  Writes the substep dts.
+template<class T,int d> int Advance_Frame(Scene<T,d>& s,Particles<T,d>& P,Grid<T,d>& G,T& time,T frame_dt,T cfl,
+    T min_dt,T max_dt,T* dts,int max_sub)
+{
+    T time_at_frame=time+frame_dt;
+    bool done=false;
+    int n=0;
+    for(int substep=0;!done;substep++){
+        if(n>=max_sub) return -1;
+        T dt=Compute_Dt(s,P,cfl,max_dt);
+        dt=dt<=min_dt?min_dt:(dt>=max_dt?max_dt:dt);
+        T next_time=time+dt;
+        if(next_time+(T).0001*dt>time_at_frame){next_time=time_at_frame;done=true;}
+        else if(next_time+dt>time_at_frame) next_time=(time+time_at_frame)/2;
+        dt=next_time-time;
+        s.dt=dt;
+        Step(s,P,G,time);
+        time=next_time;
+        dts[n++]=dt;}
+    return n;
+}
 }  // namespace oracle
 
 using namespace oracle;
 
+#define DEFINE_FRAME(SUF,T) \
+ORACLE_API_F int oracle_grid_v_bound_##SUF(int dim,const int* N,const double* lo,const double* hi,long long np,const T* V, \
+    const T* B,const unsigned char* valid,double* out) \
+{ \
+    double g[3]={0,0,0}; \
+    Particles<T,2> P2{np,0,(T*)V,0,(T*)B,0,0,0,0,(unsigned char*)valid}; \
+    Particles<T,3> P3{np,0,(T*)V,0,(T*)B,0,0,0,0,(unsigned char*)valid}; \
+    if(dim==2){*out=(double)Grid_V_Upper_Bound(Make_Scene<T,2>(N,lo,hi,g,0,0.25,1,0,0,0,0,0),P2);return 0;} \
+    if(dim==3){*out=(double)Grid_V_Upper_Bound(Make_Scene<T,3>(N,lo,hi,g,0,0.25,1,0,0,0,0,0),P3);return 0;} \
+    return -1; \
+} \
+ORACLE_API_F int oracle_advance_frames_##SUF(int dim,const int* N,const double* lo,const double* hi,const double* g,int model, \
+    double thr,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi, \
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,const T* mu,const T* la,unsigned char* valid, \
+    int nframes,double frame_dt,double cfl,double min_dt,double max_dt,double* dts,int max_sub) \
+{ \
+    int total=0; \
+    if(dim==2){ \
+        Scene<T,2> s=Make_Scene<T,2>(N,lo,hi,g,model,thr,max_dt,nwalls,wtype,wfric,wlo,whi); \
+        Particles<T,2> P{np,X,V,F,B,mass,volume,mu,la,valid};Grid<T,2> G;T time=0;std::vector<T> d(max_sub); \
+        for(int f=0;f<nframes;f++){int n=Advance_Frame(s,P,G,time,(T)frame_dt,(T)cfl,(T)min_dt,(T)max_dt,d.data(),max_sub-total); \
+            if(n<0) return -1; \
+            for(int i=0;i<n;i++) dts[total+i]=(double)d[i]; \
+            total+=n;}} \
+    else if(dim==3){ \
+        Scene<T,3> s=Make_Scene<T,3>(N,lo,hi,g,model,thr,max_dt,nwalls,wtype,wfric,wlo,whi); \
+        Particles<T,3> P{np,X,V,F,B,mass,volume,mu,la,valid};Grid<T,3> G;T time=0;std::vector<T> d(max_sub); \
+        for(int f=0;f<nframes;f++){int n=Advance_Frame(s,P,G,time,(T)frame_dt,(T)cfl,(T)min_dt,(T)max_dt,d.data(),max_sub-total); \
+            if(n<0) return -1; \
+            for(int i=0;i<n;i++) dts[total+i]=(double)d[i]; \
+            total+=n;}} \
+    else return -1; \
+    return total; \
+}
+
 #define ORACLE_API extern "C" __attribute__((visibility("default")))
+#define ORACLE_API_F extern "C" __attribute__((visibility("default")))
 
 #define DEFINE_RUN(SUF,T) \
 ORACLE_API int oracle_run_##SUF(int dim,const int* N,const double* lo,const double* hi,const double* g,int model,double thr,double dt, \
@@ -674,6 +763,8 @@ ORACLE_API int oracle_run_##SUF(int dim,const int* N,const double* lo,const doub
 }
 DEFINE_RUN(f64,double)
 DEFINE_RUN(f32,float)
+DEFINE_FRAME(f64,double)
+DEFINE_FRAME(f32,float)
 
 template<class T,int d> static void Svd_Many(long long n,const T* A,T* U,T* S,T* V,int* iters)
 {

@@ -10,11 +10,13 @@
 //   ref_harness time  <scene.bin> <threads> <warmup> <steps>
 //   ref_harness svd   <prec> <dim> <in.bin> <out.bin>
 //   ref_harness pstrain <prec> <dim> <model> <threshold> <in.bin> <out.bin>
+//   ref_harness frames <scene.bin> <nframes> <frame_dt> <cfl> <min_dt> <max_dt> <out.bin>
 //   ref_harness weights <prec> <dim> <N> <lo> <hi> <in.bin> <out.bin>
 //
 // Scene file layout (little endian) is written by tests/golden/scene_io.py.
 //#####################################################################
 #include <Core/Log/LOG.h>
+#include <Core/Math_Tools/clamp.h>
 #include <Core/Math_Tools/RANGE_ITERATOR.h>
 #include <Core/Matrices/DIAGONAL_MATRIX.h>
 #include <Core/Matrices/MATRIX.h>
@@ -219,6 +221,44 @@ int Run_Step(const SCENE& s,const char* out,STREAM_TYPE st)
     return 0;
 }
 
+// Frames with the reference's adaptive dt: the substep loop of
+// MPM_DRIVER::Simulate_To_Frame (MPM_DRIVER.cpp:173-197) around the
+// reference's Compute_Dt and Advance_One_Time_Step, without callbacks and
+// frame output.  Writes the substep dts and the final state.
+template<class TV>
+int Run_Frames(const SCENE& s,int nframes,double frame_dt,double cfl,double min_dt,double max_dt,const char* out,
+    STREAM_TYPE st)
+{
+    typedef typename TV::SCALAR T;
+#ifdef USE_OPENMP
+    omp_set_num_threads(s.threads>0?s.threads:1);
+#endif
+    RUN<TV> run(s,st);
+    auto& ex=run.ex;
+    ex.frame_dt=(T)frame_dt;ex.cfl=(T)cfl;ex.min_dt=(T)min_dt;ex.max_dt=(T)max_dt;
+    std::vector<double> dts;
+    for(int f=0;f<nframes;f++){
+        T time_at_frame=ex.time+ex.frame_dt;
+        bool done=false;
+        for(int substep=0;!done;substep++){
+            ex.dt=run.driver->Compute_Dt();
+            ex.dt=clamp(ex.dt,ex.min_dt,ex.max_dt);
+            T next_time=ex.time+ex.dt;
+            if(next_time+(T).0001*ex.dt>time_at_frame){next_time=time_at_frame;done=true;}
+            else if(next_time+ex.dt>time_at_frame) next_time=(ex.time+time_at_frame)/2;
+            ex.dt=next_time-ex.time;
+            run.driver->Advance_One_Time_Step();
+            dts.push_back((double)ex.dt);
+            ex.time=next_time;}}
+    WRITER w(out);
+    w.put("MPMFRM1",8);
+    int nd=(int)dts.size(),np=(int)s.np,nn=run.ex.mass.array.m;
+    w.put(nd);w.put(s.dim);w.put(np);w.put(nn);
+    w.put(dts.data(),dts.size());
+    run.Dump(w,nd);
+    return 0;
+}
+
 template<class TV>
 int Run_Time(const SCENE& s,int threads,int warmup,int steps,STREAM_TYPE st)
 {
@@ -326,6 +366,20 @@ int main(int argc,char* argv[])
             STREAM_TYPE st((float()));
             if(s.dim==2) rc=mode=="step"?Run_Step<VECTOR<float,2> >(s,argv[3],st):Run_Time<VECTOR<float,2> >(s,atoi(argv[3]),atoi(argv[4]),atoi(argv[5]),st);
             else rc=mode=="step"?Run_Step<VECTOR<float,3> >(s,argv[3],st):Run_Time<VECTOR<float,3> >(s,atoi(argv[3]),atoi(argv[4]),atoi(argv[5]),st);}
+        LOG::Finish_Logging();
+        return rc;}
+    if(mode=="frames" && argc==9){
+        SCENE s=Read_Scene(argv[2]);
+        int nf=atoi(argv[3]);double fdt=atof(argv[4]),cfl=atof(argv[5]),mind=atof(argv[6]),maxd=atof(argv[7]);
+        int rc;
+        if(s.prec==8){
+            STREAM_TYPE st((double()));
+            rc=s.dim==2?Run_Frames<VECTOR<double,2> >(s,nf,fdt,cfl,mind,maxd,argv[8],st):
+                Run_Frames<VECTOR<double,3> >(s,nf,fdt,cfl,mind,maxd,argv[8],st);}
+        else{
+            STREAM_TYPE st((float()));
+            rc=s.dim==2?Run_Frames<VECTOR<float,2> >(s,nf,fdt,cfl,mind,maxd,argv[8],st):
+                Run_Frames<VECTOR<float,3> >(s,nf,fdt,cfl,mind,maxd,argv[8],st);}
         LOG::Finish_Logging();
         return rc;}
     if(mode=="svd" && argc==6){

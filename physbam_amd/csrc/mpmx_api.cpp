@@ -167,6 +167,25 @@ int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers, int64_t* slo
   return guard([&] { ctx->engine->bin_stats(rebins, movers, slow); });
 }
 
+int mpmx_grid_v_upper_bound(mpmx_ctx* ctx, double* bound) {
+  if (!ctx) return bad("null context");
+  if (!bound) return bad("null array");
+  return guard([&] { ctx->engine->grid_v_upper_bound(bound); });
+}
+
+int mpmx_compute_dt(mpmx_ctx* ctx, double cfl, double max_dt, double* dt) {
+  if (!ctx) return bad("null context");
+  if (!dt) return bad("null array");
+  return guard([&] { ctx->engine->compute_dt(cfl, max_dt, dt); });
+}
+
+int mpmx_advance_frames(mpmx_ctx* ctx, int nframes, double frame_dt, double cfl, double min_dt, double max_dt,
+                        double* dts, int64_t dts_capacity, int64_t* nsubsteps) {
+  if (!ctx) return bad("null context");
+  if (dts_capacity < 0 || (dts_capacity > 0 && !dts)) return bad("bad substep buffer");
+  return guard([&] { ctx->engine->advance_frames(nframes, frame_dt, cfl, min_dt, max_dt, dts, dts_capacity, nsubsteps); });
+}
+
 int mpmx_tape_begin(mpmx_ctx* ctx, int checkpoint_every) {
   if (!ctx) return bad("null context");
   return guard([&] { ctx->engine->tape_begin(checkpoint_every); });

@@ -37,6 +37,10 @@ struct EngineBase {
   virtual void set_reorder_interval(int interval) = 0;
   virtual void set_rebin_policy(double mover_fraction, int max_interval) = 0;
   virtual void bin_stats(int64_t* rebins, int64_t* movers, int64_t* slow) = 0;
+  virtual void grid_v_upper_bound(double* out) = 0;
+  virtual void compute_dt(double cfl, double max_dt, double* dt) = 0;
+  virtual void advance_frames(int nframes, double frame_dt, double cfl, double min_dt, double max_dt, double* dts,
+                              int64_t capacity, int64_t* nsub) = 0;
   virtual void tape_begin(int checkpoint_every) = 0;
   virtual void backward(const void* dX, const void* dV, const void* dF, const void* dB, void* gX, void* gV, void* gF,
                         void* gB, double* gmu, double* glambda) = 0;

@@ -1920,6 +1920,100 @@ __global__ __launch_bounds__(BLOCK) void particle_stats_kernel(const T* V, const
   }
 }
 
+// ------------------------------------------------------- adaptive dt ----
+// Bit pattern of a non-negative float or double: its unsigned ordering is the
+// ordering of the values, so atomicMax on the pattern is a max of the values.
+template<typename T> struct OrderedBits;
+template<> struct OrderedBits<float> {
+  using U = uint32_t;
+  __device__ static U of(float x) { return __float_as_uint(x); }
+};
+template<> struct OrderedBits<double> {
+  using U = unsigned long long;
+  __device__ static U of(double x) { return (U)__double_as_longlong(x); }
+};
+
+// xi = 6 sqrt(d) min_a (1/dx_a) in T, as MPM_DRIVER.cpp:1130 forms it.
+template<typename T, int D> __device__ __forceinline__ T scene_xi(const Geom<T, D>& g) {
+#pragma clang fp contract(off)
+  T odx = g.inv_dx[0];
+#pragma unroll
+  for (int a = 1; a < D; a++) odx = g.inv_dx[a] < odx ? g.inv_dx[a] : odx;
+  return (T)6 * sqrt((T)D) * odx;
+}
+
+// |v| + |B|_F xi of particle p, in the order and rounding of
+// MPM_DRIVER::Grid_V_Upper_Bound (MPM_DRIVER.cpp:1131-1136): squares summed
+// left to right, no fused multiply-add, correctly rounded square roots.
+template<typename T, int D>
+__device__ __noinline__ T particle_speed_bound(const T* V, const T* B, uint32_t n, uint32_t p, T xi) {
+#pragma clang fp contract(off)
+  T v2 = V[p] * V[p];
+#pragma unroll
+  for (int c = 1; c < D; c++) v2 = v2 + V[(size_t)c * n + p] * V[(size_t)c * n + p];
+  T b2 = B[p] * B[p];
+#pragma unroll
+  for (int k = 1; k < D * D; k++) b2 = b2 + B[(size_t)k * n + p] * B[(size_t)k * n + p];
+  return sqrt(v2) + sqrt(b2) * xi;
+}
+
+// Per-scene max over valid particles of |v| + 6 sqrt(d) / max dx |B|_F
+// (Grid_V_Upper_Bound with use_affine_cfl and stored B).  out[s] holds the
+// bit pattern of scene s's bound and starts at zero, the reference's initial
+// result; NaN particles leave it unchanged as std::max does there.
+template<typename T, int D, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void speed_bound_kernel(const T* V, const T* B, const uint8_t* valid,
+                                                            const uint16_t* pscene, const Geom<T, D>* scenes,
+                                                            uint32_t n, typename OrderedBits<T>::U* out) {
+  const int lane = threadIdx.x & 63;
+  T run = 0;
+  T xi0 = 0;
+  if (!BATCH) xi0 = scene_xi(scenes[0]);
+  for (uint32_t p0 = blockIdx.x * BLOCK; p0 < n; p0 += gridDim.x * BLOCK) {
+    const uint32_t p = p0 + threadIdx.x;
+    T r = -1;
+    int s = -1;
+    if (p < n && valid[p]) {
+      s = BATCH ? (int)pscene[p] : 0;
+      const T xi = BATCH ? scene_xi(scenes[s]) : xi0;
+      r = particle_speed_bound<T, D>(V, B, n, p, xi);
+      if (!(r == r)) r = -1;
+    }
+    if (!BATCH) {
+      run = r > run ? r : run;
+      continue;
+    }
+    // batches: one atomic per wave when the wave's valid particles share a scene
+    int s0 = s;
+    for (int o = 32; o > 0; o >>= 1) s0 = max(s0, __shfl_xor(s0, o, 64));
+    const bool uniform = __all(s < 0 || s == s0);
+    if (uniform) {
+      T m = r;
+      for (int o = 32; o > 0; o >>= 1) {
+        const T q = __shfl_xor(m, o, 64);
+        m = q > m ? q : m;
+      }
+      if (lane == 0 && s0 >= 0 && m >= 0) atomicMax(out + s0, OrderedBits<T>::of(m));
+    } else if (s >= 0 && r >= 0) {
+      atomicMax(out + s, OrderedBits<T>::of(r));
+    }
+  }
+  if (!BATCH) {
+    for (int o = 32; o > 0; o >>= 1) {
+      const T q = __shfl_xor(run, o, 64);
+      run = q > run ? q : run;
+    }
+    __shared__ T s_max[BLOCK / 64];
+    if (lane == 0) s_max[threadIdx.x >> 6] = run;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T m = s_max[0];
+      for (int q = 1; q < BLOCK / 64; q++) m = s_max[q] > m ? s_max[q] : m;
+      if (m > 0) atomicMax(out, OrderedBits<T>::of(m));
+    }
+  }
+}
+
 // Stand-alone kernels for component parity tests.
 template<typename T, int D>
 __global__ __launch_bounds__(BLOCK) void stress_kernel(uint32_t n, const T* F, const T* vol, const T* mu, const T* la,
@@ -1990,7 +2084,7 @@ struct PhaseTimer {
   };
   std::vector<hipEvent_t> pool;
   std::vector<Pending> pending;
-  static constexpr int NPH = 8;
+  static constexpr int NPH = 9;
   double total[NPH] = {};
   int64_t count[NPH] = {};
   bool on = false;
@@ -2036,9 +2130,9 @@ struct PhaseTimer {
     for (auto e : pool) (void)hipEventDestroy(e);
   }
 };
-enum { PH_BIN = 0, PH_P2G, PH_GRID, PH_G2P, PH_REORDER, PH_STEP, PH_REBIN, PH_ADJOINT };
-static const char* kPhaseNames[PhaseTimer::NPH] = {"bin",  "p2g",  "grid",  "g2p",
-                                                   "reorder", "step", "rebin", "adjoint"};
+enum { PH_BIN = 0, PH_P2G, PH_GRID, PH_G2P, PH_REORDER, PH_STEP, PH_REBIN, PH_ADJOINT, PH_DT };
+static const char* kPhaseNames[PhaseTimer::NPH] = {"bin",     "p2g",  "grid",  "g2p",    "reorder",
+                                                   "step",    "rebin", "adjoint", "dt"};
 
 template<typename T, int D>
 class Engine final : public EngineBase {
@@ -2126,6 +2220,7 @@ class Engine final : public EngineBase {
     for (int i = 0; i < 4; i++)
       if (rb_ev_[i]) (void)hipEventDestroy(rb_ev_[i]);
     if (rb_host_) (void)hipHostFree(rb_host_);
+    if (vb_host_) (void)hipHostFree(vb_host_);
   }
 
   int dim() const override { return D; }
@@ -2274,6 +2369,92 @@ class Engine final : public EngineBase {
     if (momentum)
       for (int a = 0; a < D; a++) momentum[a] = out[1 + a];
     if (valid_count) *valid_count = (int64_t)out[4];
+  }
+
+  // ---- adaptive dt (MPM_DRIVER.cpp:1098-1138, :173-197) ----
+  // Per-scene Grid_V_Upper_Bound of the current particles; waits for the stream.
+  void speed_bounds(T* out) {
+    const int ns = (int)scenes_.size();
+    if (!vb_host_) {
+      vbound_.alloc(ns);
+      HIPCHECK(hipHostMalloc((void**)&vb_host_, ns * sizeof(Ubits), hipHostMallocDefault));
+    }
+    hipEvent_t e0 = timer_.begin(stream_);
+    const uint32_t words = (uint32_t)(ns * sizeof(Ubits) / sizeof(uint32_t));
+    fill_u32_kernel<<<cdiv(words, BLOCK), BLOCK, 0, stream_>>>((uint32_t*)vbound_.p, words, 0u);
+    if (n_ > 0) {
+      const uint32_t nb = std::min<uint32_t>(cdiv(n_, BLOCK), 2048);
+      if (ns > 1)
+        speed_bound_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(cur_.V, cur_.B, cur_.valid, cur_.pscene,
+                                                                  scenes_dev_.p, n_, vbound_.p);
+      else
+        speed_bound_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(cur_.V, cur_.B, cur_.valid, nullptr,
+                                                                   scenes_dev_.p, n_, vbound_.p);
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(vb_host_, vbound_.p, ns * sizeof(Ubits), hipMemcpyDeviceToHost, stream_));
+    timer_.end(PH_DT, e0, stream_);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    for (int s = 0; s < ns; s++) std::memcpy(out + s, vb_host_ + s, sizeof(T));
+  }
+  // MPM_DRIVER::Compute_Dt for scene s and speed bound v, in T.
+  T scene_dt(int s, T v, T cfl, T max_dt) const {
+    const Geom<T, D>& g = scenes_[s];
+    T dxmin = g.dx[0];
+    for (int a = 1; a < D; a++) dxmin = g.dx[a] < dxmin ? g.dx[a] : dxmin;
+    const T critical_speed = cfl * dxmin / max_dt;
+    return v > critical_speed ? cfl * dxmin / v : max_dt;
+  }
+  void grid_v_upper_bound(double* out) override {
+    std::vector<T> v(scenes_.size());
+    speed_bounds(v.data());
+    for (size_t s = 0; s < v.size(); s++) out[s] = (double)v[s];
+  }
+  void compute_dt(double cfl, double max_dt, double* dt) override {
+    if (!(cfl > 0) || !(max_dt > 0)) throw Error{MPMX_EINVAL, "cfl and max_dt must be positive"};
+    std::vector<T> v(scenes_.size());
+    speed_bounds(v.data());
+    for (size_t s = 0; s < v.size(); s++) dt[s] = (double)scene_dt((int)s, v[s], (T)cfl, (T)max_dt);
+  }
+  // The substep loop of MPM_DRIVER::Simulate_To_Frame in T: dt from
+  // Compute_Dt, clamped to [min_dt, max_dt] (Core/Math_Tools/clamp.h:13-16),
+  // the last substep of a frame lands on the frame time and the one before it
+  // splits the remainder in half.
+  void advance_frames(int nframes, double frame_dt, double cfl, double min_dt, double max_dt, double* dts,
+                      int64_t capacity, int64_t* nsub) override {
+    if (scenes_.size() > 1) throw Error{MPMX_ESTATE, "adaptive frames take a single-scene context"};
+    if (nframes < 0) throw Error{MPMX_EINVAL, "negative frame count"};
+    if (!(frame_dt > 0) || !(cfl > 0) || !(max_dt > 0) || !(min_dt >= 0) || min_dt > max_dt)
+      throw Error{MPMX_EINVAL, "need frame_dt, cfl, max_dt > 0 and 0 <= min_dt <= max_dt"};
+    const T fdt = (T)frame_dt, tcfl = (T)cfl, tmin = (T)min_dt, tmax = (T)max_dt;
+    int64_t count = 0;
+    if (nsub) *nsub = 0;
+    T time = (T)time_;
+    for (int f = 0; f < nframes; f++) {
+      const T time_at_frame = time + fdt;
+      for (bool done = false; !done;) {
+        T v;
+        speed_bounds(&v);
+        T dt = scene_dt(0, v, tcfl, tmax);
+        dt = dt <= tmin ? tmin : (dt >= tmax ? tmax : dt);
+        T next_time = time + dt;
+        if (next_time + (T).0001 * dt > time_at_frame) {
+          next_time = time_at_frame;
+          done = true;
+        } else if (next_time + dt > time_at_frame) {
+          next_time = (time + time_at_frame) / 2;
+        }
+        dt = next_time - time;
+        if (!(dt > 0)) throw Error{MPMX_ESTATE, "substep dt is not positive (raise min_dt)"};
+        step_once(dt);
+        time = next_time;
+        time_ = (double)time;
+        if (dts && count < capacity) dts[count] = (double)dt;
+        count++;
+        if (nsub) *nsub = count;
+      }
+    }
+    synchronize();
   }
 
   int64_t grid_node_count() const override {
@@ -2790,6 +2971,9 @@ class Engine final : public EngineBase {
   Readback rb_meta_[4];
   hipEvent_t rb_ev_[4] = {};
   uint32_t* rb_host_ = nullptr;
+  using Ubits = typename OrderedBits<T>::U;
+  DevBuf<Ubits> vbound_;      // per-scene speed bound (ordered bit patterns)
+  Ubits* vb_host_ = nullptr;  // pinned readback of vbound_
   bool has_grid_ = false;
   bool stats_on_ = false;
   double time_ = 0;

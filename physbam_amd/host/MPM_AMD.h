@@ -113,6 +113,10 @@ template<class TV> class MPM_EXAMPLE {
   GRID<TV> grid;
   MPM_PARTICLES<TV> particles;
   T dt = 0, time = 0;
+  // frame loop and CFL (MPM_EXAMPLE.h:84-95, :116): max_dt defaults to frame_dt
+  T frame_dt = (T)1 / 24, min_dt = 0, max_dt = (T)1 / 24, cfl = 1;
+  bool use_affine_cfl = true;
+  std::vector<std::function<void(int)>> begin_frame, end_frame;
   int ghost = 3;
   bool use_affine = true;
   bool use_symplectic_euler = true;
@@ -211,6 +215,40 @@ template<class TV> class MPM_DRIVER {
   // MPM_DRIVER::Advance_One_Time_Step (MPM_DRIVER.cpp:141-157) with example.dt.
   void Advance_One_Time_Step() { Step([this] { Check(mpmx_step(ctx, (double)example.dt, 1)); }, "time-step"); }
 
+  // MPM_DRIVER::Grid_V_Upper_Bound / Compute_Dt (MPM_DRIVER.cpp:1098-1138) of
+  // the device particles.
+  T Grid_V_Upper_Bound() {
+    double v = 0;
+    Check(mpmx_grid_v_upper_bound(ctx, &v));
+    return (T)v;
+  }
+  T Compute_Dt() {
+    Check_Cfl();
+    double dt = 0;
+    Check(mpmx_compute_dt(ctx, (double)example.cfl, (double)example.max_dt, &dt));
+    return (T)dt;
+  }
+
+  // MPM_DRIVER::Simulate_To_Frame (MPM_DRIVER.cpp:162-201): per frame, the
+  // begin_frame callbacks, the adaptive substeps on the device (Compute_Dt
+  // clamped to [min_dt, max_dt], frame-end rule) and the end_frame callbacks;
+  // example.time follows the device; `substeps` counts the substeps taken.
+  void Simulate_To_Frame(int frame) {
+    Check_Cfl();
+    for (; current_frame < frame; current_frame++) {
+      for (auto& f : example.begin_frame) f(current_frame);
+      int64_t n = 0;
+      Check(mpmx_advance_frames(ctx, 1, (double)example.frame_dt, (double)example.cfl, (double)example.min_dt,
+                                (double)example.max_dt, nullptr, 0, &n));
+      substeps += n;
+      example.time = (T)mpmx_time(ctx);
+      if (example.sync_particles_every_step) Read_Particles();
+      for (auto& f : example.end_frame) f(current_frame);
+    }
+  }
+  int current_frame = 0;
+  int64_t substeps = 0;
+
   // n steps of example.dt enqueued back to back (one host synchronisation).
   void Advance_Steps(int n) {
     Fire("time-step", true);
@@ -237,6 +275,9 @@ template<class TV> class MPM_DRIVER {
 
   static void Check(int rc) {
     if (rc != MPMX_OK) throw std::runtime_error(std::string("mpmx: ") + mpmx_last_error());
+  }
+  void Check_Cfl() const {
+    if (!example.use_affine_cfl) throw std::invalid_argument("the device CFL bound is the affine one (use_affine_cfl)");
   }
   void Fire(const char* name, bool begin) {
     auto it = example.time_step_callbacks.find(name);

@@ -5,8 +5,11 @@
 // dt = 1e-4.  Writes the initial and final particle state to a binary file:
 //   int64 n, int32 dim, then X V F B mass volume mu lambda (double) and valid
 //   (uint8) at the start, then X V F B (double) and valid after the steps.
+// With a frame count, the run is MPM_DRIVER::Simulate_To_Frame instead:
+// adaptive substeps with frame_dt 1/60, cfl 0.5, max_dt 1e-3 (steps ignored),
+// and the substep count (int64) and final time (double) follow the state.
 //
-//   mpm_run <out.bin> [steps=200]
+//   mpm_run <out.bin> [steps=200] [frames=0]
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -25,6 +28,7 @@ int main(int argc, char** argv) {
     return 2;
   }
   const int steps = argc > 2 ? atoi(argv[2]) : 200;
+  const int frames = argc > 3 ? atoi(argv[3]) : 0;
   MPM_EXAMPLE<TV> example;
   TV lo, hi;
   hi(0) = hi(1) = 1;
@@ -67,6 +71,9 @@ int main(int argc, char** argv) {
   whi(1) = 0.05;
   example.Add_Collision_Object(RANGE<TV>(wlo, whi), COLLISION_TYPE::stick, 0);
   example.dt = 1e-4;
+  example.frame_dt = 1.0 / 60;
+  example.cfl = 0.5;
+  example.max_dt = 1e-3;
 
   FILE* f = fopen(argv[1], "wb");
   if (!f) return 1;
@@ -81,14 +88,28 @@ int main(int argc, char** argv) {
   MPM_DRIVER<TV> driver(example);
   try {
     driver.Initialize();
-    for (int s = 0; s < steps; s++) driver.Advance_One_Time_Step();
+    if (frames > 0)
+      driver.Simulate_To_Frame(frames);
+    else
+      for (int s = 0; s < steps; s++) driver.Advance_One_Time_Step();
     driver.Read_Particles();
   } catch (const std::exception& e) {
     fprintf(stderr, "%s\n", e.what());
     return 1;
   }
   put(f, P.X); put(f, P.V); put(f, P.F); put(f, P.B); put(f, P.valid);
+  if (frames > 0) {
+    const int64_t sub = driver.substeps;
+    const double t = example.time;
+    fwrite(&sub, sizeof(sub), 1, f);
+    fwrite(&t, sizeof(t), 1, f);
+  }
   fclose(f);
+  if (frames > 0) {
+    printf("mpm_run: %lld particles, %d frames, %lld substeps, time %.17g\n", (long long)n, frames,
+           (long long)driver.substeps, (double)example.time);
+    return 0;
+  }
   printf("mpm_run: %lld particles, %d steps, %d callbacks\n", (long long)n, steps, callbacks);
   return callbacks == steps ? 0 : 1;
 }

@@ -25,6 +25,7 @@ EXPORTS = (
     "mpmx_clear_collision_objects", "mpmx_step", "mpmx_step_async", "mpmx_synchronize", "mpmx_time",
     "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_grid_node_count",
     "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_set_rebin_policy", "mpmx_bin_stats",
+    "mpmx_grid_v_upper_bound", "mpmx_compute_dt", "mpmx_advance_frames",
     "mpmx_tape_begin", "mpmx_backward",
     "mpmx_enable_timing", "mpmx_reset_timing",
     "mpmx_phase_time", "mpmx_stress", "mpmx_svd", "mpmx_sort_keys", "mpmx_seed_particles",
@@ -90,6 +91,9 @@ def lib():
         L.mpmx_set_reorder_interval.argtypes = [vp, c_int]
         L.mpmx_set_rebin_policy.argtypes = [vp, dbl, c_int]
         L.mpmx_bin_stats.argtypes = [vp, vp, vp, vp]
+        L.mpmx_grid_v_upper_bound.argtypes = [vp, vp]
+        L.mpmx_compute_dt.argtypes = [vp, dbl, dbl, vp]
+        L.mpmx_advance_frames.argtypes = [vp, c_int, dbl, dbl, dbl, dbl, vp, i64, vp]
         L.mpmx_tape_begin.argtypes = [vp, c_int]
         L.mpmx_backward.argtypes = [vp] + [vp] * 10
         L.mpmx_enable_timing.argtypes = [vp, c_int]
@@ -267,6 +271,27 @@ class Context:
         r = np.zeros(3, np.int64)
         _check(lib().mpmx_bin_stats(self._h, _p(r[0:1]), _p(r[1:2]), _p(r[2:3])))
         return (int(r[0]), int(r[1]), int(r[2])) if with_slow else (int(r[0]), int(r[1]))
+
+    def grid_v_upper_bound(self):
+        """Per-scene max over valid particles of |v| + 6 sqrt(dim) min(1/dx) |B|_F (float64 array)."""
+        out = np.zeros(lib().mpmx_scene_count(self._h))
+        _check(lib().mpmx_grid_v_upper_bound(self._h, _p(out)))
+        return out
+
+    def compute_dt(self, cfl=1.0, max_dt=1.0 / 24):
+        """Per-scene CFL time step (float64 array)."""
+        out = np.zeros(lib().mpmx_scene_count(self._h))
+        _check(lib().mpmx_compute_dt(self._h, float(cfl), float(max_dt), _p(out)))
+        return out
+
+    def advance_frames(self, nframes, frame_dt=1.0 / 24, cfl=1.0, min_dt=0.0, max_dt=None, capacity=1 << 20):
+        """Advance ``nframes`` frames with adaptive substeps; returns the substep dts."""
+        max_dt = frame_dt if max_dt is None else max_dt
+        dts = np.zeros(capacity)
+        cnt = np.zeros(1, np.int64)
+        _check(lib().mpmx_advance_frames(self._h, int(nframes), float(frame_dt), float(cfl), float(min_dt),
+                                         float(max_dt), _p(dts), ctypes.c_int64(capacity), _p(cnt)))
+        return dts[:min(int(cnt[0]), capacity)].copy()
 
     def tape_begin(self, checkpoint_every=20):
         _check(lib().mpmx_tape_begin(self._h, int(checkpoint_every)))

@@ -3,7 +3,8 @@
 Runs the reference harness (oracle/_ref/ref_harness, built from the reference
 checkout by oracle/ref/Makefile) on small seeded scenes and component inputs,
 and stores inputs and outputs as compressed .npz fixtures.  Run in the
-authoring container:  python tests/golden/make_golden.py
+authoring container:  python tests/golden/make_golden.py [steps] [components] [frames]
+(no argument: every group).
 """
 import json
 import os
@@ -45,9 +46,58 @@ def step_cases():
     return cases
 
 
+def frame_cases():
+    """Scenes run with the reference's adaptive substep loop (Compute_Dt, clamp, frame-end rule).
+
+    Each entry: scene, (nframes, frame_dt, cfl, min_dt, max_dt)."""
+    sq = lambda X: (np.abs(X[:, 0] - 0.5) <= 0.15) & (np.abs(X[:, 1] - 0.45) <= 0.15)
+    sph = lambda X: ((X - np.array([0.5, 0.5, 0.5])) ** 2).sum(1) < 0.25 ** 2
+    cases = {}
+    # 2D FP64: a square thrown at the sticky floor; the CFL bound (cfl 0.5)
+    # sets dt below max_dt, and the frame end forces shortened substeps.
+    walls = mt.wall_boxes(2, [64, 64], [0, 0], [1, 1], flags=15, inset=0.05, type_=mt.STICK, friction=0.0)
+    cases["throw2d_f64"] = (mt.make_scene(2, 64, prec=8, model=mt.COROTATED, shape=sq, v0=[2.0, -6.0], vrand=0.5,
+                                          walls=walls, dt=1e-3, seed=31), (3, 1.0 / 60, 0.5, 1e-6, 1.0 / 60))
+    # 3D FP32 neo-Hookean ball against slip walls, max_dt binding at first.
+    walls = mt.wall_boxes(3, [16] * 3, [0] * 3, [1] * 3, flags=63, inset=0.12, type_=mt.SLIP, friction=0.0)
+    cases["ball3d_neo_slip_f32"] = (mt.make_scene(3, 16, prec=4, model=mt.NEO_HOOKEAN, shape=sph,
+                                                  v0=[0.5, -4.0, 0.3], vrand=0.5, brand=1e-3, frand=0.1,
+                                                  walls=walls, dt=1e-3, per_particle_E=True, seed=32),
+                                    (2, 1.0 / 48, 0.6, 1e-5, 2e-3))
+    return cases
+
+
+def write_frames():
+    out = {}
+    for name, (sc, (nf, fdt, cfl, mind, maxd)) in frame_cases().items():
+        for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+            out[f"{name}_in_{k}"] = np.asarray(sc[k])
+        meta = {k: sc[k] for k in ("dim", "prec", "model", "N", "lo", "hi", "gravity", "dt", "neo_threshold")}
+        meta.update(nframes=nf, frame_dt=fdt, cfl=cfl, min_dt=mind, max_dt=maxd)
+        out[f"{name}_meta"] = np.array(json.dumps(meta, default=float))
+        out[f"{name}_walls"] = np.array(json.dumps(sc["walls"], default=float))
+        r, dts = scene_io.run_ref_frames(sc, nf, fdt, cfl, mind, maxd)
+        out[f"{name}_dts"] = dts
+        for k in ("X", "V", "F", "B", "valid"):
+            out[f"{name}_out_{k}"] = r[k]
+        print("frames", name, sc["X"].shape[0], "particles,", len(dts), "substeps, dt range",
+              dts.min(), dts.max())
+    np.savez_compressed(os.path.join(HERE, "frames.npz"), **out)
+
+
 def main():
     if not scene_io.ref_available():
         sys.exit("reference harness missing: build it with oracle/ref/Makefile")
+    groups = set(sys.argv[1:]) or {"steps", "components", "frames"}
+    if "frames" in groups:
+        write_frames()
+    if "steps" in groups:
+        write_steps()
+    if "components" in groups:
+        write_components()
+
+
+def write_steps():
     for name, (sc, dumps) in step_cases().items():
         out = {}
         for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
@@ -65,6 +115,8 @@ def main():
         np.savez_compressed(os.path.join(HERE, f"step_{name}.npz"), **out)
         print("wrote", name, sc["X"].shape[0], "particles, dumps", dumps)
 
+
+def write_components():
     rng = np.random.default_rng(2024)
     comp = {}
     for dim in (2, 3):

@@ -95,6 +95,28 @@ def run_ref(sc, nsteps, dump_every=None, threads=1):
         return read_output(op, sc["dim"])
 
 
+def run_ref_frames(sc, nframes, frame_dt, cfl, min_dt, max_dt, threads=1):
+    """Run ``nframes`` frames of the reference's adaptive substep loop; returns (final dump, dts)."""
+    with tempfile.TemporaryDirectory() as td:
+        sp = os.path.join(td, "scene.bin")
+        op = os.path.join(td, "out.bin")
+        write_scene(sp, sc, 0, 1, threads)
+        subprocess.run([REF_HARNESS, "frames", sp, str(nframes), repr(float(frame_dt)), repr(float(cfl)),
+                        repr(float(min_dt)), repr(float(max_dt)), op], check=True)
+        with open(op, "rb") as f:
+            buf = f.read()
+    assert buf[:7] == b"MPMFRM1"
+    nd, d, n, nn = struct.unpack_from("<4i", buf, 8)
+    assert d == sc["dim"]
+    dts = np.frombuffer(buf, np.float64, count=nd, offset=24).copy()
+    rest = b"MPMOUT1\0" + struct.pack("<4i", 1, d, n, nn) + buf[24 + 8 * nd:]
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "dump.bin")
+        with open(p, "wb") as f:
+            f.write(rest)
+        return read_output(p, d)[0], dts
+
+
 def ref_component(mode, args, payload, outspec):
     """Run a component mode (svd, pstrain, weights) of the harness.
 

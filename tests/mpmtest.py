@@ -157,6 +157,52 @@ def oracle_run(sc, nsteps, prec=None, want_grid=True):
     return out
 
 
+def oracle_frames(sc, nframes, frame_dt, cfl, min_dt, max_dt, prec=None, max_sub=100000):
+    """Advance ``nframes`` frames with the adaptive substep loop of the oracle.
+
+    Returns (final state dict, array of substep dts)."""
+    prec = prec or sc["prec"]
+    dt_np = np.float64 if prec == 8 else np.float32
+    d = sc["dim"]
+    n = sc["X"].shape[0]
+    st = {k: np.ascontiguousarray(sc[k], dtype=dt_np).copy() for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda")}
+    valid = np.ascontiguousarray(sc["valid"], np.uint8).copy()
+    N = np.array(list(sc["N"]) + [1] * (3 - d), np.int32)
+    lo = np.array(list(sc["lo"]) + [0] * (3 - d), np.float64)
+    hi = np.array(list(sc["hi"]) + [0] * (3 - d), np.float64)
+    g = np.array(list(sc["gravity"]) + [0] * (3 - d), np.float64)
+    nw, wt, wf, wl, wh = _walls_arrays(sc)
+    dts = np.zeros(max_sub, np.float64)
+    fn = oracle().oracle_advance_frames_f64 if prec == 8 else oracle().oracle_advance_frames_f32
+    cnt = fn(ctypes.c_int(d), _ptr(N), _ptr(lo), _ptr(hi), _ptr(g), ctypes.c_int(sc["model"]),
+             ctypes.c_double(sc.get("neo_threshold", 0.25)), ctypes.c_int(nw), _ptr(wt), _ptr(wf), _ptr(wl), _ptr(wh),
+             ctypes.c_longlong(n), _ptr(st["X"]), _ptr(st["V"]), _ptr(st["F"]), _ptr(st["B"]), _ptr(st["mass"]),
+             _ptr(st["volume"]), _ptr(st["mu"]), _ptr(st["lambda"]), _ptr(valid), ctypes.c_int(nframes),
+             ctypes.c_double(frame_dt), ctypes.c_double(cfl), ctypes.c_double(min_dt), ctypes.c_double(max_dt),
+             _ptr(dts), ctypes.c_int(max_sub))
+    assert cnt >= 0, "substep limit exceeded"
+    return dict(X=st["X"], V=st["V"], F=st["F"], B=st["B"], valid=valid), dts[:cnt].copy()
+
+
+def oracle_grid_v_bound(sc, prec=None):
+    """Grid_V_Upper_Bound of the particles of ``sc`` (max |v| + 6 sqrt(d)/min dx |B|_F over valid particles)."""
+    prec = prec or sc["prec"]
+    dt_np = np.float64 if prec == 8 else np.float32
+    d = sc["dim"]
+    V = np.ascontiguousarray(sc["V"], dtype=dt_np)
+    B = np.ascontiguousarray(sc["B"], dtype=dt_np)
+    valid = np.ascontiguousarray(sc["valid"], np.uint8)
+    N = np.array(list(sc["N"]) + [1] * (3 - d), np.int32)
+    lo = np.array(list(sc["lo"]) + [0] * (3 - d), np.float64)
+    hi = np.array(list(sc["hi"]) + [0] * (3 - d), np.float64)
+    out = np.zeros(1, np.float64)
+    fn = oracle().oracle_grid_v_bound_f64 if prec == 8 else oracle().oracle_grid_v_bound_f32
+    rc = fn(ctypes.c_int(d), _ptr(N), _ptr(lo), _ptr(hi), ctypes.c_longlong(V.shape[0]), _ptr(V), _ptr(B),
+            _ptr(valid), _ptr(out))
+    assert rc == 0
+    return float(out[0])
+
+
 def oracle_svd(A, dim, prec=8):
     dt_np = np.float64 if prec == 8 else np.float32
     A = np.ascontiguousarray(A, dt_np).reshape(-1, dim * dim)

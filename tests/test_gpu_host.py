@@ -63,3 +63,34 @@ def test_host_mirror_runs_square_scene_200_steps(tmp_path):
         err = np.abs(run[k] - orc[k]).max() / scale
         print(k, f"{err:.2e}", flush=True)
         assert err < 1e-8, (k, err)
+
+
+def test_host_mirror_simulate_to_frame_matches_oracle_frames(tmp_path):
+    """MPM_DRIVER::Simulate_To_Frame of the mirror (adaptive substeps on the
+    device) for two frames of 1/60 s, cfl 0.5, max_dt 1e-3, against the
+    oracle's restatement of the same loop from the same initial state."""
+    exe = os.path.join(ROOT, "physbam_amd", "mpm_run")
+    out = tmp_path / "frames.bin"
+    print("mpm_run 2 frames", flush=True)
+    r = subprocess.run([exe, str(out), "0", "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    run = read_run(out)
+    with open(out, "rb") as f:
+        tail = f.read()[-16:]
+    substeps = int(np.frombuffer(tail, np.int64, 1, 0)[0])
+    t_end = float(np.frombuffer(tail, np.float64, 1, 8)[0])
+    walls = mt.wall_boxes(2, [64, 64], [0, 0], [1, 1], flags=8, inset=0.05, type_=mt.STICK, friction=0.0)
+    sc = dict(dim=2, prec=8, model=mt.COROTATED, N=[64, 64], lo=[0.0, 0.0], hi=[1.0, 1.0], gravity=[0.0, -9.8],
+              dt=1e-4, neo_threshold=0.25, walls=walls)
+    for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+        sc[k] = run["in_" + k]
+    orc, dts = mt.oracle_frames(sc, 2, 1.0 / 60, 0.5, 0.0, 1e-3)
+    print("substeps", substeps, "oracle", len(dts), "time", t_end, flush=True)
+    assert substeps == len(dts)
+    assert t_end == pytest.approx(2.0 / 60, rel=1e-12)
+    h = 1.0 / 64
+    for k in ("X", "V", "F", "B"):
+        scale = max(np.abs(orc[k]).max(), np.abs(orc["V"]).max() * h) if k == "B" else np.abs(orc[k]).max()
+        err = np.abs(run[k] - orc[k]).max() / scale
+        print(k, f"{err:.2e}", flush=True)
+        assert err < 1e-8, (k, err)

@@ -245,3 +245,50 @@ def test_oracle_matches_live_reference_harness():
     orc = mt.oracle_run(sc, 8)
     for k in ("X", "V", "F", "B"):
         assert np.array_equal(orc[k].astype(np.float64), ref[k]), k
+
+
+def frame_scene(z, name):
+    meta = json.loads(str(z[f"{name}_meta"]))
+    sc = dict(meta)
+    sc["walls"] = json.loads(str(z[f"{name}_walls"]))
+    for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+        sc[k] = z[f"{name}_in_{k}"]
+    return sc, meta
+
+
+@pytest.mark.parametrize("name", ["throw2d_f64", "ball3d_neo_slip_f32"])
+def test_oracle_adaptive_frames_match_reference(name):
+    """Compute_Dt + clamp + the frame-end rule of Simulate_To_Frame: the
+    substep dts and the final state agree bit for bit with the reference."""
+    z = load("frames.npz")
+    sc, m = frame_scene(z, name)
+    out, dts = mt.oracle_frames(sc, m["nframes"], m["frame_dt"], m["cfl"], m["min_dt"], m["max_dt"])
+    ref = z[f"{name}_dts"]
+    assert len(dts) == len(ref)
+    assert np.array_equal(dts, ref)
+    for k in ("X", "V", "F", "B", "valid"):
+        assert np.array_equal(out[k].astype(np.float64) if k != "valid" else out[k], z[f"{name}_out_{k}"]), k
+    # the fixture exercises the CFL branch, the max_dt clamp and the frame-end split
+    assert dts.min() < 0.999 * m["max_dt"]
+    assert np.isclose(dts.sum(), m["nframes"] * m["frame_dt"], rtol=1e-5)
+
+
+def test_grid_v_upper_bound_closed_form():
+    rng = np.random.default_rng(3)
+    for dim in (2, 3):
+        n = 1000
+        sc = dict(dim=dim, N=[40, 20, 10][:dim], lo=[0.0] * dim, hi=[1.0, 0.5, 0.5][:dim],
+                  V=rng.normal(size=(n, dim)), B=rng.normal(size=(n, dim * dim)) * 1e-3,
+                  valid=(rng.random(n) < 0.9).astype(np.uint8), prec=8)
+        got = mt.oracle_grid_v_bound(sc)
+        ok = sc["valid"] == 1
+        # dx = (1/40, 1/40, 1/20): the bound scales |B| by 6 sqrt(d) times the
+        # smallest 1/dx (MPM_DRIVER.cpp:1130), 20 in 3D and 40 in 2D
+        xi = 6 * np.sqrt(dim) * (20.0 if dim == 3 else 40.0)
+        want = (np.linalg.norm(sc["V"][ok], axis=1) + np.linalg.norm(sc["B"][ok], axis=1) * xi).max()
+        assert abs(got - want) <= 1e-14 * want
+        # the largest particle is excluded once it is invalid
+        j = np.flatnonzero(ok)[np.argmax((np.linalg.norm(sc["V"], axis=1) + np.linalg.norm(sc["B"], axis=1) * xi)[ok])]
+        sc["valid"] = sc["valid"].copy()
+        sc["valid"][j] = 0
+        assert mt.oracle_grid_v_bound(sc) < got
