@@ -680,11 +680,12 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
   __shared__ int s_b[D][BLOCK];
   __shared__ uint16_t s_sc[BLOCK];
   const uint32_t nm = a.counts[4], ns = a.counts[3], ntot = nm + ns;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (uint32_t c0 = blockIdx.x * BLOCK; c0 < ntot; c0 += gridDim.x * BLOCK) {
+  // particles per block and pass: short lists are spread over every block
+  const uint32_t per = min((uint32_t)BLOCK, max(1u, (ntot + gridDim.x - 1) / gridDim.x));
+  for (uint32_t c0 = blockIdx.x * per; c0 < ntot; c0 += gridDim.x * per) {
     // ---- pass 1: one thread per particle
     const uint32_t q = c0 + threadIdx.x;
-    if (q < ntot) {
+    if (threadIdx.x < per && q < ntot) {
       const bool mover = q < nm;
       const uint32_t p = mover ? a.mover_list[spread_index(q, nm)] : a.slow_list[spread_index(q - nm, ns)];
       const Geom<T, D> g = particle_geom(a, p);
@@ -727,12 +728,12 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
       s_sc[threadIdx.x] = a.pscene ? a.pscene[p] : 0;
     }
     __syncthreads();
-    // ---- pass 2: lane = (particle, node); two particles per wavefront instruction
-    const uint32_t cnt = min((uint32_t)BLOCK, ntot - c0);
-    for (uint32_t pp = wv * 2; pp < cnt; pp += 2 * (BLOCK / 64)) {
-      const uint32_t sl = pp + (lane >> 5);
-      const int k = lane & 31;
-      if (sl >= cnt || k >= Tile<D>::K || s_b[0][sl] < 0) continue;
+    // ---- pass 2: one thread per (particle, stencil node)
+    const uint32_t cnt = min(per, ntot - c0);
+    for (uint32_t u = threadIdx.x; u < cnt * Tile<D>::K; u += BLOCK) {
+      const uint32_t sl = u / Tile<D>::K;
+      const int k = (int)(u - sl * Tile<D>::K);
+      if (s_b[0][sl] < 0) continue;
       const Geom<T, D> g = a.scenes[s_sc[sl]];
       T xe[D], wt, gr[D], dz[D];
       int o[D], nc[D];
@@ -1207,7 +1208,9 @@ __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
 template<typename T, int D, bool FRICTION>
 __global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
   const uint32_t nm = a.counts[4];
-  for (uint32_t r = blockIdx.x * BLOCK + threadIdx.x; r < nm; r += gridDim.x * BLOCK) {
+  // consecutive list entries go to different blocks, so a short list still
+  // occupies every compute unit
+  for (uint32_t r = threadIdx.x * gridDim.x + blockIdx.x; r < nm; r += gridDim.x * BLOCK) {
     const uint32_t q = spread_index(r, nm);
     const uint32_t p = a.mover_list[q];
     const Geom<T, D> g = particle_geom(a, p);
