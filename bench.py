@@ -38,8 +38,10 @@ DT = 2e-5
 # V0, mu, lambda (112 B); G2P reads X, F (48 B) and writes X, V, F, B (96 B).
 # Per active grid node: P2G result write 16 B; grid update read 16 + write
 # 12 B; G2P read 12 B.
-BYTES_PER_PARTICLE = {"p2g": 112.0, "g2p": 144.0, "grid": 0.0, "step": 256.0}
-BYTES_PER_NODE = {"p2g": 16.0, "g2p": 12.0, "grid": 28.0, "step": 56.0}
+# g2p2g (the fused G2P of one step and P2G of the next): read X, F, m, V0, mu,
+# lambda (64 B), write X, V, F, B (96 B); per node read v (12 B), write m, p (16 B)
+BYTES_PER_PARTICLE = {"p2g": 112.0, "g2p": 144.0, "g2p2g": 160.0, "grid": 0.0, "step": 256.0}
+BYTES_PER_NODE = {"p2g": 16.0, "g2p": 12.0, "g2p2g": 28.0, "grid": 28.0, "step": 56.0}
 
 
 def headline_boxes():
@@ -270,7 +272,7 @@ def main():
 
     rebins1, movers_last = ctx.bin_stats()
     phases = {}
-    for ph in ("rebin", "bin", "p2g", "grid", "g2p", "reorder", "step"):
+    for ph in ("rebin", "bin", "p2g", "grid", "g2p", "g2p2g", "reorder", "step"):
         tot, cnt = ctx.phase_time(ph)
         phases[ph] = (tot, cnt)
     ctx.enable_timing(False)
@@ -283,7 +285,7 @@ def main():
     ms_per_step = sec * 1e3 / args.steps
     value = n * args.steps / sec
     # dominant kernel by accumulated time
-    kern = max(("p2g", "grid", "g2p"), key=lambda k: phases[k][0])
+    kern = max(("p2g", "grid", "g2p", "g2p2g"), key=lambda k: phases[k][0])
     ktot, kcnt = phases[kern]
     avg_s = ktot / max(kcnt, 1) * 1e-3
     kbytes = BYTES_PER_PARTICLE[kern] * n + BYTES_PER_NODE[kern] * nodes

@@ -77,6 +77,14 @@ template<typename T, int D> struct Args {
   Node4<T>* grid;
   Node4<T>* gridf;
   Node4<T>* gridu;  // if set, the grid update also stores (m, v before the collision objects)
+  // fused G2P -> P2G: P2G target of the next step and the next step's mover
+  // list (movers are appended from the bottom by the tile kernel and from the
+  // top, index mcap-1-k, by the mover kernel; counts[5] and counts[7] count
+  // them; the current list counts are counts[4] and counts[6])
+  Node4<T>* grid_next;
+  uint32_t* mover_list2;
+  uint32_t* mover_tile2;
+  uint32_t mcap;
   uint32_t* err;
   double* stats_partial;
 };
@@ -232,11 +240,14 @@ __device__ __forceinline__ bool grid_tile_flag(const Args<T, D>& a, const uint32
 // GRID: touched grid tiles (octant rule); otherwise binned tiles (stamp == bs).
 template<typename T, int D, bool GRID>
 __global__ __launch_bounds__(BLOCK) void gcount_kernel(Args<T, D> a, const uint32_t* bstamp, uint32_t bs, uint32_t ms,
-                                                       uint32_t* block_cnt) {
+                                                       uint32_t* block_cnt, int reset_lists) {
   const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
-  if (GRID && t == 0) {
+  if (reset_lists && t == 0) {  // slow and mover lists of the step about to run
     a.counts[3] = 0;
     a.counts[4] = 0;
+    a.counts[5] = 0;
+    a.counts[6] = 0;
+    a.counts[7] = 0;
   }
   const bool f = t < a.ntotal && (GRID ? grid_tile_flag<T, D>(a, bstamp, bs, ms, t) : bstamp[t] == bs);
   const int cnt = __syncthreads_count(f ? 1 : 0);
@@ -331,8 +342,156 @@ template<typename T> struct Vec4Type;
 template<> struct Vec4Type<float> { typedef float type __attribute__((ext_vector_type(4))); };
 template<> struct Vec4Type<double> { typedef double type __attribute__((ext_vector_type(4))); };
 
+template<typename T, int D, bool FRICTION>
+__device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>& g, const int* org,
+                                          T (*sv)[Tile<D>::RN]);
+template<typename T, int D, bool FRICTION, bool FUSED>
+__device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, const Geom<T, D>& g, const int* org,
+                                              const T (*sv)[Tile<D>::RN]);
+
+// Phase B of the tile P2G: the particle in coefficient slot sl, whose base
+// node is region node base_li, is added into this wavefront's copy of the
+// tile's stencil region.  Lanes of a wavefront hold distinct cells, so the nodes of
+// one read-modify-write step are distinct; the compiler barrier after each
+// write keeps the steps in order (a later step of one lane may touch the node
+// an earlier step of another lane wrote).
 template<typename T, int D>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void p2g_kernel(Args<T, D> a) {
+__device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*s_c)[BLOCK], int sl, int base_li,
+                                                 typename Vec4Type<T>::type* reg) {
+  constexpr int R = Tile<D>::R;
+  using C = P2GCoef<T, D>;
+  using T4 = typename Vec4Type<T>::type;
+  const T m = s_c[C::M][sl];
+  T mv[D], mC[D][D], Q[D][D], w[3][D], dw[3][D], dz[3][D];
+#pragma unroll
+  for (int c = 0; c < D; c++) mv[c] = s_c[C::MV + c][sl];
+#pragma unroll
+  for (int j = 0; j < D; j++)
+#pragma unroll
+    for (int c = 0; c < D; c++) mC[c][j] = s_c[C::MC + c + D * j][sl];
+  {
+    int qq = 0;
+#pragma unroll
+    for (int c = 0; c < D; c++)
+#pragma unroll
+      for (int j = c; j < D; j++) {
+        const T qv = s_c[C::Q + qq++][sl];
+        Q[c][j] = qv;
+        Q[j][c] = qv;
+      }
+  }
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    const T xe = s_c[C::XE + c][sl];
+    const T inv = g.inv_dx[c];
+    const T xx = xe * inv;
+    const T q0 = xx - (T)1.5, q1 = xx - (T)1, q2 = xx - (T)0.5;
+    w[0][c] = (T)0.5 * q0 * q0;
+    w[1][c] = (T)0.75 - q1 * q1;
+    w[2][c] = (T)0.5 * q2 * q2;
+    dw[0][c] = q0 * inv;
+    dw[1][c] = (T)(-2) * q1 * inv;
+    dw[2][c] = q2 * inv;
+#pragma unroll
+    for (int o = 0; o < 3; o++) dz[o][c] = (T)o * g.dx[c] - xe;
+  }
+  // Separable form: with per-axis factors w, dw and dz = o dx - xe,
+  //   m_i   += w_x w_y w_z m
+  //   p_i,c += w_x w_y w_z (mv_c + sum_j mC_cj dz_j) - sum_j Q_cj (grad w)_j
+  // is built axis by axis so that each node costs one FMA per value.
+  if constexpr (D == 3) {
+    T wzd[3], wydz[3];
+#pragma unroll
+    for (int o = 0; o < 3; o++) {
+      wzd[o] = w[o][2] * dz[o][2];
+      wydz[o] = w[o][1] * dz[o][1];
+    }
+#pragma unroll
+    for (int ox = 0; ox < 3; ox++) {
+      const T wx = w[ox][0], dwx = dw[ox][0], dzx = dz[ox][0];
+      T ax[3], bx[3], dx_[3], ex[3], fx[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        ax[c] = wx * (mv[c] + mC[c][0] * dzx) - Q[c][0] * dwx;  // coefficient of w_y w_z
+        bx[c] = wx * mC[c][1];                                  // of w_y dz_y w_z
+        dx_[c] = -wx * Q[c][1];                                 // of dw_y w_z
+        ex[c] = wx * mC[c][2];                                  // of w_y w_z dz_z
+        fx[c] = -wx * Q[c][2];                                  // of w_y dw_z
+      }
+      const T mx = wx * m;
+#pragma unroll
+      for (int oy = 0; oy < 3; oy++) {
+        const T wy = w[oy][1], dwy = dw[oy][1], wyz = wydz[oy];
+        T P[3], S[3], Tz[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          P[c] = wy * ax[c] + wyz * bx[c] + dwy * dx_[c];  // coefficient of w_z
+          S[c] = wy * ex[c];                                // of w_z dz_z
+          Tz[c] = wy * fx[c];                               // of dw_z
+        }
+        const T mxy = wy * mx;
+#pragma unroll
+        for (int oz = 0; oz < 3; oz++) {
+          const T wz = w[oz][2], dwz = dw[oz][2], wzz = wzd[oz];
+          const int li = base_li + (ox * R + oy) * R + oz;
+          T4 cur = reg[li];
+          cur.x += wz * mxy;
+          cur.y += wz * P[0];
+          cur.z += wz * P[1];
+          cur.w += wz * P[2];
+          cur.y += wzz * S[0];
+          cur.z += wzz * S[1];
+          cur.w += wzz * S[2];
+          cur.y += dwz * Tz[0];
+          cur.z += dwz * Tz[1];
+          cur.w += dwz * Tz[2];
+          reg[li] = cur;
+          __asm__ __volatile__("" ::: "memory");
+        }
+      }
+    }
+  } else {
+    T wyd[3];
+#pragma unroll
+    for (int o = 0; o < 3; o++) wyd[o] = w[o][D - 1] * dz[o][D - 1];
+#pragma unroll
+    for (int ox = 0; ox < 3; ox++) {
+      const T wx = w[ox][0], dwx = dw[ox][0], dzx = dz[ox][0];
+      T ax[2], ex[2], fx[2];
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        ax[c] = wx * (mv[c] + mC[c][0] * dzx) - Q[c][0] * dwx;
+        ex[c] = wx * mC[c][D - 1];
+        fx[c] = -wx * Q[c][D - 1];
+      }
+      const T mx = wx * m;
+#pragma unroll
+      for (int oy = 0; oy < 3; oy++) {
+        const T wy = w[oy][D - 1], dwy = dw[oy][D - 1], wyz = wyd[oy];
+        const int li = base_li + ox * R + oy;
+        T4 cur = reg[li];
+        cur.x += wy * mx;
+        cur.y += wy * ax[0];
+        cur.z += wy * ax[1];
+        cur.y += wyz * ex[0];
+        cur.z += wyz * ex[1];
+        cur.y += dwy * fx[0];
+        cur.z += dwy * fx[1];
+        reg[li] = cur;
+        __asm__ __volatile__("" ::: "memory");
+      }
+    }
+  }
+}
+
+// FUSED (g2p2g): each tile first runs the G2P of the current step
+// (g2p_tile_pass on a.grid), then the P2G of the next step of the same
+// particles into a.grid_next.  The P2G half reads the state the G2P half just
+// wrote (L2-resident) and skips particles whose base is outside the tile:
+// movers of the current list (still at the old position, left to
+// g2p_mover_kernel<FUSED>) and new movers (appended by the G2P half).
+template<typename T, int D, bool FUSED, bool FRICTION>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (FUSED ? 3 : 4) : 1))) void p2g_kernel(Args<T, D> a) {
   constexpr int R = Tile<D>::R, RN = Tile<D>::RN, NW = BLOCK / 64, E = Tile<D>::E;
   constexpr int JT = 2, CH = JT * BLOCK;  // particles per chunk
   using C = P2GCoef<T, D>;
@@ -341,6 +500,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
   __shared__ union {
     T c[C::N][BLOCK];            // phase A -> phase B coefficients
     uint32_t wc[JT][NW][64];     // grouping: per (pass, wave) cell counts, then offsets
+    T v[FUSED ? (FRICTION ? 2 * D : D) : 1][FUSED ? RN : 1];  // FUSED: staged grid of the G2P half
   } s_u;
   __shared__ uint32_t s_idx[CH];  // chunk's particle slots grouped by cell
   __shared__ uint32_t s_cb[64], s_cn[64], s_max;
@@ -350,8 +510,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
   const int a_cell = threadIdx.x >> 2, a_j = threadIdx.x & 3;  // phase A assignment
   const int a_slot = a_j * 64 + a_cell;                         // = phase B slot (wave a_j, lane a_cell)
   const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
-  int lcell[D];
-  cell_offsets<D>((uint32_t)lane, lcell);
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
     const uint32_t t = a.ptiles[seg];
     const Geom<T, D> g = tile_geom(a, t);
@@ -360,6 +518,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
     for (int c = 0; c < D; c++) dinv[c] = (T)4 * g.inv_dx[c] * g.inv_dx[c];  // D_p^-1 (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:152-158)
     int org[D];
     tile_origin<T, D>(t, g, org);
+    if constexpr (FUSED) {
+      g2p_stage<T, D, FRICTION>(a, g, org, s_u.v);
+      __syncthreads();
+      g2p_tile_pass<T, D, FRICTION, true>(a, t, g, org, s_u.v);
+      __syncthreads();
+    }
     for (int k = threadIdx.x; k < NW * RN; k += BLOCK) {
       T4 z = {0, 0, 0, 0};
       s_reg[k / RN][k % RN] = z;
@@ -390,7 +554,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
             }
             if (in) {
               cell = cl;
-            } else {
+            } else if (!FUSED) {
               const uint32_t q = atomicAdd(&a.counts[4], 1u);  // aggregated per wavefront by the compiler
               a.mover_list[q] = p;
               a.mover_tile[q] = t;
@@ -480,130 +644,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
         __syncthreads();
         // ---- phase B
         if (k * NW + wv < b_cnt) {
-          const int sl = threadIdx.x;  // = wv * 64 + lane
-          const T m = s_c[C::M][sl];
-          T mv[D], mC[D][D], Q[D][D], w[3][D], dw[3][D], dz[3][D];
-#pragma unroll
-          for (int c = 0; c < D; c++) mv[c] = s_c[C::MV + c][sl];
-#pragma unroll
-          for (int j = 0; j < D; j++)
-#pragma unroll
-            for (int c = 0; c < D; c++) mC[c][j] = s_c[C::MC + c + D * j][sl];
-          {
-            int qq = 0;
-#pragma unroll
-            for (int c = 0; c < D; c++)
-#pragma unroll
-              for (int j = c; j < D; j++) {
-                const T qv = s_c[C::Q + qq++][sl];
-                Q[c][j] = qv;
-                Q[j][c] = qv;
-              }
-          }
-#pragma unroll
-          for (int c = 0; c < D; c++) {
-            const T xe = s_c[C::XE + c][sl];
-            const T inv = g.inv_dx[c];
-            const T xx = xe * inv;
-            const T q0 = xx - (T)1.5, q1 = xx - (T)1, q2 = xx - (T)0.5;
-            w[0][c] = (T)0.5 * q0 * q0;
-            w[1][c] = (T)0.75 - q1 * q1;
-            w[2][c] = (T)0.5 * q2 * q2;
-            dw[0][c] = q0 * inv;
-            dw[1][c] = (T)(-2) * q1 * inv;
-            dw[2][c] = q2 * inv;
-#pragma unroll
-            for (int o = 0; o < 3; o++) dz[o][c] = (T)o * g.dx[c] - xe;
-          }
-          // Separable form: with per-axis factors w, dw and dz = o dx - xe,
-          //   m_i   += w_x w_y w_z m
-          //   p_i,c += w_x w_y w_z (mv_c + sum_j mC_cj dz_j) - sum_j Q_cj (grad w)_j
-          // is built axis by axis so that each node costs one FMA per value.
-          if constexpr (D == 3) {
-            const int base_li = (lcell[0] * R + lcell[1]) * R + lcell[2];
-            T wzd[3], wydz[3];
-#pragma unroll
-            for (int o = 0; o < 3; o++) {
-              wzd[o] = w[o][2] * dz[o][2];
-              wydz[o] = w[o][1] * dz[o][1];
-            }
-#pragma unroll
-            for (int ox = 0; ox < 3; ox++) {
-              const T wx = w[ox][0], dwx = dw[ox][0], dzx = dz[ox][0];
-              T ax[3], bx[3], dx_[3], ex[3], fx[3];
-#pragma unroll
-              for (int c = 0; c < 3; c++) {
-                ax[c] = wx * (mv[c] + mC[c][0] * dzx) - Q[c][0] * dwx;  // coefficient of w_y w_z
-                bx[c] = wx * mC[c][1];                                  // of w_y dz_y w_z
-                dx_[c] = -wx * Q[c][1];                                 // of dw_y w_z
-                ex[c] = wx * mC[c][2];                                  // of w_y w_z dz_z
-                fx[c] = -wx * Q[c][2];                                  // of w_y dw_z
-              }
-              const T mx = wx * m;
-#pragma unroll
-              for (int oy = 0; oy < 3; oy++) {
-                const T wy = w[oy][1], dwy = dw[oy][1], wyz = wydz[oy];
-                T P[3], S[3], Tz[3];
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                  P[c] = wy * ax[c] + wyz * bx[c] + dwy * dx_[c];  // coefficient of w_z
-                  S[c] = wy * ex[c];                                // of w_z dz_z
-                  Tz[c] = wy * fx[c];                               // of dw_z
-                }
-                const T mxy = wy * mx;
-#pragma unroll
-                for (int oz = 0; oz < 3; oz++) {
-                  const T wz = w[oz][2], dwz = dw[oz][2], wzz = wzd[oz];
-                  const int li = base_li + (ox * R + oy) * R + oz;
-                  T4 cur = s_reg[wv][li];
-                  cur.x += wz * mxy;
-                  cur.y += wz * P[0];
-                  cur.z += wz * P[1];
-                  cur.w += wz * P[2];
-                  cur.y += wzz * S[0];
-                  cur.z += wzz * S[1];
-                  cur.w += wzz * S[2];
-                  cur.y += dwz * Tz[0];
-                  cur.z += dwz * Tz[1];
-                  cur.w += dwz * Tz[2];
-                  s_reg[wv][li] = cur;
-                  __asm__ __volatile__("" ::: "memory");
-                }
-              }
-            }
-          } else {
-            const int base_li = lcell[0] * R + lcell[D - 1];
-            T wyd[3];
-#pragma unroll
-            for (int o = 0; o < 3; o++) wyd[o] = w[o][D - 1] * dz[o][D - 1];
-#pragma unroll
-            for (int ox = 0; ox < 3; ox++) {
-              const T wx = w[ox][0], dwx = dw[ox][0], dzx = dz[ox][0];
-              T ax[2], ex[2], fx[2];
-#pragma unroll
-              for (int c = 0; c < 2; c++) {
-                ax[c] = wx * (mv[c] + mC[c][0] * dzx) - Q[c][0] * dwx;
-                ex[c] = wx * mC[c][D - 1];
-                fx[c] = -wx * Q[c][D - 1];
-              }
-              const T mx = wx * m;
-#pragma unroll
-              for (int oy = 0; oy < 3; oy++) {
-                const T wy = w[oy][D - 1], dwy = dw[oy][D - 1], wyz = wyd[oy];
-                const int li = base_li + ox * R + oy;
-                T4 cur = s_reg[wv][li];
-                cur.x += wy * mx;
-                cur.y += wy * ax[0];
-                cur.z += wy * ax[1];
-                cur.y += wyz * ex[0];
-                cur.z += wyz * ex[1];
-                cur.y += dwy * fx[0];
-                cur.z += dwy * fx[1];
-                s_reg[wv][li] = cur;
-                __asm__ __volatile__("" ::: "memory");
-              }
-            }
-          }
+          int lcell[D];
+          cell_offsets<D>((uint32_t)lane, lcell);
+          const int lane_li = D == 3 ? (lcell[0] * R + lcell[1]) * R + lcell[D - 1] : lcell[0] * R + lcell[D - 1];
+          p2g_scatter_slot<T, D>(g, s_c, threadIdx.x, lane_li, s_reg[wv]);
         }
         __syncthreads();
       }
@@ -617,7 +661,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
         region_offsets<D>(k, off);
 #pragma unroll
         for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
-        Node4<T>* nd = a.grid + node_slot<T, D>(nc, g);
+        Node4<T>* nd = (FUSED ? a.grid_next : a.grid) + node_slot<T, D>(nc, g);
         atomic_add(&nd->m, sum.x);
         atomic_add(&nd->x, sum.y);
         atomic_add(&nd->y, sum.z);
@@ -673,13 +717,18 @@ template<typename T, int D> struct ScatterCoef {
   static constexpr int M = 0, MV = 1, MC = 1 + D, TAU = 1 + D + D * D, XE = 1 + D + 2 * D * D, N = XE + D;
 };
 
-template<typename T, int D>
+template<typename T, int D, bool FUSED>
 __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
   using C = ScatterCoef<T, D>;
   __shared__ T s_c[C::N][BLOCK];
   __shared__ int s_b[D][BLOCK];
   __shared__ uint16_t s_sc[BLOCK];
-  const uint32_t nm = a.counts[4], ns = a.counts[3], ntot = nm + ns;
+  // FUSED: the movers are the fused step's current list (bottom counts[4],
+  // top counts[6]; their new state is already stored) followed by the bottom
+  // of the next list (counts[5]); the target is grid_next
+  const uint32_t n4 = a.counts[4], n6 = FUSED ? a.counts[6] : 0u, n5 = FUSED ? a.counts[5] : 0u;
+  const uint32_t nm = n4 + n6 + n5, ns = a.counts[3], ntot = nm + ns;
+  Node4<T>* const target = FUSED ? a.grid_next : a.grid;
   // particles per block and pass: short lists are spread over every block
   const uint32_t per = min((uint32_t)BLOCK, max(1u, (ntot + gridDim.x - 1) / gridDim.x));
   for (uint32_t c0 = blockIdx.x * per; c0 < ntot; c0 += gridDim.x * per) {
@@ -687,7 +736,14 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
     const uint32_t q = c0 + threadIdx.x;
     if (threadIdx.x < per && q < ntot) {
       const bool mover = q < nm;
-      const uint32_t p = mover ? a.mover_list[spread_index(q, nm)] : a.slow_list[spread_index(q - nm, ns)];
+      uint32_t p;
+      if (mover) {
+        const uint32_t r = spread_index(q, nm);
+        p = r < n4 ? a.mover_list[r]
+                   : (r < n4 + n6 ? a.mover_list[a.mcap - 1u - (r - n4)] : a.mover_list2[r - n4 - n6]);
+      } else {
+        p = a.slow_list[spread_index(q - nm, ns)];
+      }
       const Geom<T, D> g = particle_geom(a, p);
       T x[D], Fm[D * D], tau[D * D];
 #pragma unroll
@@ -701,7 +757,9 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
         base[c] = stencil_base<T, D>(g, x, c) + 3;
         in = in && base[c] >= 0 && base[c] + 2 < g.G[c];
       }
-      if (!in) {
+      if (FUSED && !a.valid[p]) {
+        base[0] = -1;  // left the domain in this step's G2P
+      } else if (!in) {
         atomicOr(a.err, 2u);
         base[0] = -1;  // skipped in pass 2
       } else if (mover) {
@@ -742,7 +800,7 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
       node_weight<T, D>(g, xe, k, o, wt, gr, dz);
 #pragma unroll
       for (int c = 0; c < D; c++) nc[c] = s_b[c][sl] + o[c];
-      Node4<T>* nd = a.grid + node_slot<T, D>(nc, g);
+      Node4<T>* nd = target + node_slot<T, D>(nc, g);
       T f[D];
 #pragma unroll
       for (int c = 0; c < D; c++) {
@@ -1107,7 +1165,7 @@ __device__ __forceinline__ void g2p_gather(const Geom<T, D>& gm, T dt, const Wts
 // stencil base leaves its binned tile marks the tile it moves to, so that the
 // next step clears and updates the grid tiles around it.
 template<typename T, int D>
-__device__ __forceinline__ void g2p_store(const Args<T, D>& a, const Geom<T, D>& g, uint32_t p, const T* x,
+__device__ __forceinline__ bool g2p_store(const Args<T, D>& a, const Geom<T, D>& g, uint32_t p, const T* x,
                                           const T* vb, const T* vfb, const T* G, const T* Bs, uint32_t bin_tile) {
   T Fm[D * D], Fn[D * D];
 #pragma unroll
@@ -1134,9 +1192,78 @@ __device__ __forceinline__ void g2p_store(const Args<T, D>& a, const Geom<T, D>&
   for (int k = 0; k < D * D; k++) { a.F[k * a.n + p] = Fn[k]; a.B[k * a.n + p] = Bs[k]; }
   if (!ok) {
     a.valid[p] = 0;
-  } else {
-    const uint32_t nt = base_tile<T, D>(g, xn);
-    if (nt != bin_tile && nt < a.ntotal) a.mstamp[nt] = a.next_stamp;
+    return false;
+  }
+  const uint32_t nt = base_tile<T, D>(g, xn);
+  if (nt == bin_tile) return false;
+  if (nt < a.ntotal) a.mstamp[nt] = a.next_stamp;
+  return true;
+}
+
+// The tile's stencil region of v (rows 0..D-1) and, with FRICTION, of the
+// friction velocity (rows D..2D-1) of a.grid / a.gridf, staged in LDS.
+template<typename T, int D, bool FRICTION>
+__device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>& g, const int* org,
+                                          T (*sv)[Tile<D>::RN]) {
+  for (int k = threadIdx.x; k < Tile<D>::RN; k += BLOCK) {
+    int off[D], nc[D];
+    region_offsets<D>(k, off);
+#pragma unroll
+    for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
+    const uint32_t slot = node_slot<T, D>(nc, g);
+    const Node4<T> nd = a.grid[slot];
+    sv[0][k] = nd.x;
+    sv[1][k] = nd.y;
+    if constexpr (D == 3) sv[2][k] = nd.z;
+    if constexpr (FRICTION) {
+      const Node4<T> nf = a.gridf[slot];
+      sv[D][k] = nf.x;
+      sv[D + 1][k] = nf.y;
+      if constexpr (D == 3) sv[D + 2][k] = nf.z;
+    }
+  }
+}
+
+// G2P of the particles binned to tile t from the staged region; movers
+// (base outside the tile) are left to g2p_mover_kernel.  FUSED: particles
+// whose new base leaves the tile are appended to the bottom of the next
+// mover list (counts[5]).
+template<typename T, int D, bool FRICTION, bool FUSED>
+__device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, const Geom<T, D>& g, const int* org,
+                                              const T (*sv)[Tile<D>::RN]) {
+  constexpr int R = Tile<D>::R, E = Tile<D>::E;
+  const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
+  for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
+    const uint32_t p = a.order ? a.order[i] : i;
+    if (!a.valid[p]) continue;
+    T x[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+    Wts<T, D> w;
+    weights<T, D>(g, x, w);
+    bool in = true;
+    int lb[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      lb[c] = w.base[c] - org[c];
+      in = in && lb[c] >= 0 && lb[c] < E;
+    }
+    if (!in) continue;  // mover
+    const int li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
+    auto nv = [&](int ox, int oy, int oz, int c) -> T {
+      return sv[c][li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)];
+    };
+    auto nf = [&](int ox, int oy, int oz, int c) -> T {
+      if constexpr (FRICTION) return sv[D + c][li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)];
+      else return nv(ox, oy, oz, c);
+    };
+    T vb[D], vfb[D], G[D * D], Bs[D * D];
+    g2p_gather<T, D, FRICTION>(g, a.dt, w, nv, nf, vb, vfb, G, Bs);
+    if (g2p_store<T, D>(a, g, p, x, vb, vfb, G, Bs, t) && FUSED) {
+      const uint32_t k = atomicAdd(&a.counts[5], 1u);
+      a.mover_list2[k] = p;
+      a.mover_tile2[k] = t;
+    }
   }
 }
 
@@ -1144,74 +1271,71 @@ __device__ __forceinline__ void g2p_store(const Args<T, D>& a, const Geom<T, D>&
 // friction velocity) is staged in LDS; movers are left to g2p_mover_kernel.
 template<typename T, int D, bool FRICTION>
 __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
-  constexpr int R = Tile<D>::R, RN = Tile<D>::RN, E = Tile<D>::E;
-  __shared__ T s_v[D][RN];
-  __shared__ T s_f[FRICTION ? D : 1][FRICTION ? RN : 1];
+  __shared__ T s_v[FRICTION ? 2 * D : D][Tile<D>::RN];
   const uint32_t ntile = a.counts[0];
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
     const uint32_t t = a.ptiles[seg];
     const Geom<T, D> g = tile_geom(a, t);
     int org[D];
     tile_origin<T, D>(t, g, org);
-    for (int k = threadIdx.x; k < RN; k += BLOCK) {
-      int off[D], nc[D];
-      region_offsets<D>(k, off);
-#pragma unroll
-      for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
-      const uint32_t slot = node_slot<T, D>(nc, g);
-      const Node4<T> nd = a.grid[slot];
-      s_v[0][k] = nd.x;
-      s_v[1][k] = nd.y;
-      if constexpr (D == 3) s_v[2][k] = nd.z;
-      if constexpr (FRICTION) {
-        const Node4<T> nf = a.gridf[slot];
-        s_f[0][k] = nf.x;
-        s_f[1][k] = nf.y;
-        if constexpr (D == 3) s_f[D - 1][k] = nf.z;
-      }
-    }
+    g2p_stage<T, D, FRICTION>(a, g, org, s_v);
     __syncthreads();
-    const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
-    for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
-      const uint32_t p = a.order ? a.order[i] : i;
-      if (!a.valid[p]) continue;
-      T x[D];
-#pragma unroll
-      for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
-      Wts<T, D> w;
-      weights<T, D>(g, x, w);
-      bool in = true;
-      int lb[D];
-#pragma unroll
-      for (int c = 0; c < D; c++) {
-        lb[c] = w.base[c] - org[c];
-        in = in && lb[c] >= 0 && lb[c] < E;
-      }
-      if (!in) continue;  // mover
-      const int li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
-      auto nv = [&](int ox, int oy, int oz, int c) -> T {
-        return s_v[c][li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)];
-      };
-      auto nf = [&](int ox, int oy, int oz, int c) -> T {
-        if constexpr (FRICTION) return s_f[c][li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)];
-        else return nv(ox, oy, oz, c);
-      };
-      T vb[D], vfb[D], G[D * D], Bs[D * D];
-      g2p_gather<T, D, FRICTION>(g, a.dt, w, nv, nf, vb, vfb, G, Bs);
-      g2p_store<T, D>(a, g, p, x, vb, vfb, G, Bs, t);
-    }
+    g2p_tile_pass<T, D, FRICTION, false>(a, t, g, org, s_v);
     __syncthreads();
   }
 }
 
+// Zeroes the grid tiles of a tile list (count at *count).
+template<typename T>
+__global__ __launch_bounds__(BLOCK) void clear_tiles_kernel(Node4<T>* grid, const uint32_t* tiles, const uint32_t* count) {
+  const uint64_t total = (uint64_t)(*count) * 64u;
+  Node4<T> z;
+  z.m = z.x = z.y = z.z = 0;
+  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total; idx += (uint64_t)gridDim.x * BLOCK)
+    grid[(uint64_t)tiles[idx >> 6] * 64u + (idx & 63u)] = z;
+}
+
+// Counter updates of the fused step.  mode 0 (bins rebuilt): the current
+// mover list is empty.  mode 1 (end of step): the next lists become current
+// ([4] <- [5], [6] <- [7]), the pending grid's tile count becomes the last
+// grid's (last[1] <- [1]), [9] <- movers for the host's rebin policy and
+// [10] <- movers of the step just taken.  mode 2: end of the priming P2G.
+// Modes 1 and 2 keep the P2G's FP64-stress count in [11].
+__global__ void fused_counts_kernel(uint32_t* counts, uint32_t* last, int mode) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (mode == 0) {
+    counts[4] = 0;
+    counts[6] = 0;
+    counts[9] = 0;
+  } else if (mode == 1) {
+    counts[10] = counts[4] + counts[6];  // movers of the step just taken
+    counts[4] = counts[5];
+    counts[6] = counts[7];
+    counts[9] = counts[4] + counts[6];
+    last[1] = counts[1];
+  }
+  // modes 1 and 2 (end of a fused step, end of the priming P2G): the slow
+  // list is consumed and the next mover list starts empty
+  if (mode != 0) {
+    counts[11] = counts[3];  // FP64-stress particles of the P2G just done
+    counts[3] = 0;
+    counts[5] = 0;
+    counts[7] = 0;
+  }
+}
+
 // G2P of the movers listed by P2G this step, reading the grid from HBM.
-template<typename T, int D, bool FRICTION>
+// FUSED: the current list of the fused step (counts[4] entries from the
+// bottom, counts[6] from the top); particles still outside their binned tile
+// are appended to the top of the next list.
+template<typename T, int D, bool FRICTION, bool FUSED>
 __global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
-  const uint32_t nm = a.counts[4];
+  const uint32_t nb = a.counts[4], nm = nb + (FUSED ? a.counts[6] : 0u);
   // consecutive list entries go to different blocks, so a short list still
   // occupies every compute unit
   for (uint32_t r = threadIdx.x * gridDim.x + blockIdx.x; r < nm; r += gridDim.x * BLOCK) {
-    const uint32_t q = spread_index(r, nm);
+    const uint32_t s_ = spread_index(r, nm);
+    const uint32_t q = s_ < nb ? s_ : a.mcap - 1u - (s_ - nb);
     const uint32_t p = a.mover_list[q];
     const Geom<T, D> g = particle_geom(a, p);
     T x[D];
@@ -1238,7 +1362,12 @@ __global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
     };
     T vb[D], vfb[D], G[D * D], Bs[D * D];
     g2p_gather<T, D, FRICTION>(g, a.dt, w, nv, nf, vb, vfb, G, Bs);
-    g2p_store<T, D>(a, g, p, x, vb, vfb, G, Bs, a.mover_tile[q]);
+    const uint32_t bt = a.mover_tile[q];
+    if (g2p_store<T, D>(a, g, p, x, vb, vfb, G, Bs, bt) && FUSED) {
+      const uint32_t k = a.mcap - 1u - atomicAdd(&a.counts[7], 1u);
+      a.mover_list2[k] = p;
+      a.mover_tile2[k] = bt;
+    }
   }
 }
 
@@ -2087,7 +2216,7 @@ struct PhaseTimer {
   };
   std::vector<hipEvent_t> pool;
   std::vector<Pending> pending;
-  static constexpr int NPH = 9;
+  static constexpr int NPH = 10;
   double total[NPH] = {};
   int64_t count[NPH] = {};
   bool on = false;
@@ -2133,9 +2262,9 @@ struct PhaseTimer {
     for (auto e : pool) (void)hipEventDestroy(e);
   }
 };
-enum { PH_BIN = 0, PH_P2G, PH_GRID, PH_G2P, PH_REORDER, PH_STEP, PH_REBIN, PH_ADJOINT, PH_DT };
-static const char* kPhaseNames[PhaseTimer::NPH] = {"bin",     "p2g",  "grid",  "g2p",    "reorder",
-                                                   "step",    "rebin", "adjoint", "dt"};
+enum { PH_BIN = 0, PH_P2G, PH_GRID, PH_G2P, PH_REORDER, PH_STEP, PH_REBIN, PH_ADJOINT, PH_DT, PH_G2P2G };
+static const char* kPhaseNames[PhaseTimer::NPH] = {"bin",  "p2g",   "grid",    "g2p", "reorder",
+                                                   "step", "rebin", "adjoint", "dt",  "g2p2g"};
 
 template<typename T, int D>
 class Engine final : public EngineBase {
@@ -2186,7 +2315,9 @@ class Engine final : public EngineBase {
     gblock_.alloc(cdiv((uint32_t)nt, BLOCK));
     ptiles_.alloc(nt);
     gtiles_.alloc(nt);
-    counts_.alloc(8);
+    gtiles2_.alloc(nt);
+    counts_.alloc(16);
+    lastcnt_.alloc(8);
     err_.alloc(4);
     grid_.alloc(nt * 64);
     HIPCHECK(hipMemsetAsync(tile_stamp_.p, 0, nt * sizeof(uint32_t), stream_));
@@ -2198,7 +2329,14 @@ class Engine final : public EngineBase {
       rb_host_[i] = 0;
     }
     HIPCHECK(hipMemsetAsync(grid_.p, 0, nt * 64 * sizeof(Node4<T>), stream_));
-    HIPCHECK(hipMemsetAsync(counts_.p, 0, 8 * sizeof(uint32_t), stream_));
+    grid2_.alloc(nt * 64);
+    HIPCHECK(hipMemsetAsync(grid2_.p, 0, nt * 64 * sizeof(Node4<T>), stream_));
+    HIPCHECK(hipMemsetAsync(lastcnt_.p, 0, 8 * sizeof(uint32_t), stream_));
+    gcur_ = grid_.p;
+    gother_ = grid2_.p;
+    gt_cur_ = gtiles_.p;
+    gt_other_ = gtiles2_.p;
+    HIPCHECK(hipMemsetAsync(counts_.p, 0, 16 * sizeof(uint32_t), stream_));
     HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
     stats_blocks_ = (int)std::min<uint64_t>(std::max<uint64_t>(cdiv(nt * 64, BLOCK), 1), 2048);
     stats_partial_.alloc((size_t)stats_blocks_ * 5 + 8);
@@ -2286,6 +2424,8 @@ class Engine final : public EngineBase {
     force_rebin_ = true;
     force_reorder_ = true;
     has_grid_ = false;
+    fused_ = false;
+    fused_last_ = false;
   }
 
   void get_particles(void* X, void* V, void* F, void* B, uint8_t* valid) override {
@@ -2307,6 +2447,7 @@ class Engine final : public EngineBase {
     if (model == 1 && !(thr > 0)) throw Error{MPMX_EINVAL, "neo-Hookean threshold must be positive"};
     model_ = model;
     thr_ = (T)thr;
+    fused_ = false;  // the pending P2G used the previous model
     root_ = D == 2 ? (T)std::sqrt(thr) : (T)std::cbrt(thr);  // pow<1,d>(failure_threshold)
   }
   void set_gravity(const double* g) override {
@@ -2333,7 +2474,13 @@ class Engine final : public EngineBase {
   void step(double dt, int nsteps, bool wait) override {
     if (!(dt > 0)) throw Error{MPMX_EINVAL, "dt must be positive"};
     if (nsteps < 0) throw Error{MPMX_EINVAL, "negative step count"};
-    for (int s = 0; s < nsteps; s++) step_once((T)dt);
+    const bool fuse = !taping_ && n_ > 0;
+    for (int s = 0; s < nsteps; s++) {
+      if (fuse)
+        step_fused((T)dt);
+      else
+        step_once((T)dt);
+    }
     if (wait) synchronize();
   }
 
@@ -2475,6 +2622,11 @@ class Engine final : public EngineBase {
     HIPCHECK(hipMemsetAsync(dm.p, 0, nn * sizeof(T), stream_));
     HIPCHECK(hipMemsetAsync(dv.p, 0, nn * D * sizeof(T), stream_));
     Args<T, D> a = make_args(0);
+    if (fused_last_) {  // the last step's grid is the second buffer of the fused pipeline
+      a.grid = gother_;
+      a.gtiles = gt_other_;
+      a.counts = lastcnt_.p;
+    }
     export_grid_kernel<T, D><<<grid_blocks(), BLOCK, 0, stream_>>>(a, dm.p, dv.p);
     if (mass) HIPCHECK(hipMemcpyAsync(mass, dm.p, nn * sizeof(T), hipMemcpyDeviceToHost, stream_));
     if (velocity) HIPCHECK(hipMemcpyAsync(velocity, dv.p, nn * D * sizeof(T), hipMemcpyDeviceToHost, stream_));
@@ -2493,11 +2645,11 @@ class Engine final : public EngineBase {
   }
   void bin_stats(int64_t* rebins, int64_t* movers, int64_t* slow) override {
     synchronize();
-    uint32_t c[8];
+    uint32_t c[16];
     HIPCHECK(hipMemcpy(c, counts_.p, sizeof(c), hipMemcpyDeviceToHost));
     if (rebins) *rebins = rebins_;
-    if (movers) *movers = has_grid_ ? (int64_t)c[4] : 0;
-    if (slow) *slow = has_grid_ ? (int64_t)c[3] : 0;
+    if (movers) *movers = has_grid_ ? (int64_t)(fused_last_ ? c[10] : c[4]) : 0;
+    if (slow) *slow = has_grid_ ? (int64_t)(fused_last_ ? c[11] : c[3]) : 0;
   }
   void enable_timing(bool on) override {
     synchronize();
@@ -2528,6 +2680,7 @@ class Engine final : public EngineBase {
     if (any_friction()) throw Error{MPMX_EINVAL, "the adjoint step has no friction: remove wall friction"};
     if (force_reorder_ || force_rebin_) rebin();  // settle the storage order before recording
     particle_stats(nullptr, nullptr, &tape_valid_);
+    fused_ = false;
     taping_ = true;
     tape_k_ = k;
     tape_dt_.clear();
@@ -2683,8 +2836,16 @@ class Engine final : public EngineBase {
     slow_list_.alloc(n);
     mover_list_.release();
     mover_list_.alloc(n);
+    mover_list2_.release();
+    mover_list2_.alloc(n);
     mover_tile_.release();
     mover_tile_.alloc(n);
+    mover_tile2_.release();
+    mover_tile2_.alloc(n);
+    ml_cur_ = mover_list_.p;
+    mt_cur_ = mover_tile_.p;
+    ml_next_ = mover_list2_.p;
+    mt_next_ = mover_tile2_.p;
   }
 
   struct StateBuf {
@@ -2714,6 +2875,8 @@ class Engine final : public EngineBase {
                                                                  b.valid.p, cur_.X, cur_.V, cur_.F, cur_.B,
                                                                  cur_.valid, n_);
     if (rebin_next) force_rebin_ = true;
+    fused_ = false;
+    fused_last_ = false;
   }
   uint32_t copy_blocks() const { return std::max<uint32_t>(1, std::min<uint32_t>(cdiv(n_, BLOCK), 2048)); }
   // Host adjoint (user order, NULL = zero) -> device SoA in slot order.
@@ -2738,8 +2901,8 @@ class Engine final : public EngineBase {
   void build_grid_tiles(uint32_t ms, Node4<T>* z0, Node4<T>* z1, Node4<T>* z2) {
     const uint32_t nb = cdiv(ntotal_, BLOCK);
     const Args<T, D> a = make_args(0);
-    gcount_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms, gblock_.p);
-    glist_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms, gblock_.p, gtiles_.p,
+    gcount_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms, gblock_.p, 1);
+    glist_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms, gblock_.p, gt_cur_,
                                                         counts_.p + 1, z0, z1, z2);
   }
 
@@ -2801,13 +2964,17 @@ class Engine final : public EngineBase {
     a.tile_begin = tile_begin_.p;
     a.tile_end = tile_end_.p;
     a.ptiles = ptiles_.p;
-    a.gtiles = gtiles_.p;
+    a.gtiles = gt_cur_;
     a.counts = counts_.p;
     a.slow_list = slow_list_.p;
-    a.mover_list = mover_list_.p;
-    a.mover_tile = mover_tile_.p;
+    a.mover_list = ml_cur_;
+    a.mover_tile = mt_cur_;
+    a.mover_list2 = ml_next_;
+    a.mover_tile2 = mt_next_;
+    a.mcap = n_;
+    a.grid_next = gother_;
     a.mstamp = mstamp_.p;
-    a.grid = grid_override_ ? grid_override_ : grid_.p;
+    a.grid = grid_override_ ? grid_override_ : gcur_;
     a.gridf = gridf_.p;
     a.gridu = gridu_override_;
     a.err = err_.p;
@@ -2847,7 +3014,7 @@ class Engine final : public EngineBase {
     bounds_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(skeys_.p, n_, tile_begin_.p, tile_end_.p, tile_stamp_.p,
                                                           bin_stamp_);
     const uint32_t nb = cdiv(ntotal_, BLOCK);
-    gcount_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, 0u, gblock_.p);
+    gcount_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, 0u, gblock_.p, 0);
     glist_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, 0u, gblock_.p, ptiles_.p,
                                                          counts_.p + 0, nullptr, nullptr, nullptr);
     timer_.end(PH_REBIN, e0, stream_);
@@ -2871,6 +3038,8 @@ class Engine final : public EngineBase {
   }
 
   void step_once(T dt) {
+    fused_ = false;
+    fused_last_ = false;
     const bool fric = any_friction();
     if (fric) gridf_.alloc((size_t)ntotal_ * 64);
     hipEvent_t e_step = timer_.begin(stream_);
@@ -2882,15 +3051,15 @@ class Engine final : public EngineBase {
     if (rebin_due()) rebin();
     hipEvent_t e0 = timer_.begin(stream_);
     const uint32_t ms = (uint32_t)(step_count_ + 1);  // mover stamp of this step
-    build_grid_tiles(ms, grid_override_ ? grid_override_ : grid_.p, fric ? gridf_.p : nullptr, nullptr);
+    build_grid_tiles(ms, grid_override_ ? grid_override_ : gcur_, fric ? gridf_.p : nullptr, nullptr);
     timer_.end(PH_BIN, e0, stream_);
 
     Args<T, D> a = make_args(dt);
     a.order = identity_order_ ? nullptr : order_.p;
     a.next_stamp = ms + 1;
     hipEvent_t e1 = timer_.begin(stream_);
-    p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-    p2g_extra_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+    p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+    p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
     timer_.end(PH_P2G, e1, stream_);
     const int slot = (int)(step_count_ & 3);
     HIPCHECK(hipMemcpyAsync(rb_host_ + slot, counts_.p + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
@@ -2910,10 +3079,10 @@ class Engine final : public EngineBase {
     hipEvent_t e3 = timer_.begin(stream_);
     if (fric) {
       g2p_kernel<T, D, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-      g2p_mover_kernel<T, D, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+      g2p_mover_kernel<T, D, true, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
     } else {
       g2p_kernel<T, D, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-      g2p_mover_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+      g2p_mover_kernel<T, D, false, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
     }
     timer_.end(PH_G2P, e3, stream_);
     step_count_++;
@@ -2928,6 +3097,98 @@ class Engine final : public EngineBase {
     }
     HIPCHECK(hipGetLastError());
     has_grid_ = true;
+    time_ += (double)dt;
+    timer_.end(PH_STEP, e_step, stream_);
+  }
+
+  // P2G of the current state into gcur_, starting the fused pipeline: both
+  // grids are cleared in full (the fused step clears only the tiles it used).
+  void prime_fused(T dt, bool fric) {
+    const size_t bytes = (size_t)ntotal_ * 64 * sizeof(Node4<T>);
+    HIPCHECK(hipMemsetAsync(gcur_, 0, bytes, stream_));
+    HIPCHECK(hipMemsetAsync(gother_, 0, bytes, stream_));
+    HIPCHECK(hipMemsetAsync(lastcnt_.p, 0, 8 * sizeof(uint32_t), stream_));
+    if (rebin_due()) rebin();
+    hipEvent_t e0 = timer_.begin(stream_);
+    const uint32_t ms = (uint32_t)(step_count_ + 1);
+    build_grid_tiles(ms, gcur_, fric ? gridf_.p : nullptr, nullptr);
+    timer_.end(PH_BIN, e0, stream_);
+    Args<T, D> a = make_args(dt);
+    a.order = identity_order_ ? nullptr : order_.p;
+    a.next_stamp = ms + 1;
+    hipEvent_t e1 = timer_.begin(stream_);
+    p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+    p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+    fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 2);
+    timer_.end(PH_P2G, e1, stream_);
+    HIPCHECK(hipGetLastError());
+    fused_ = true;
+    fused_dt_ = dt;
+  }
+
+  // One step with the G2P of this step and the P2G of the next one fused
+  // (p2g_kernel<FUSED>).  The P2G uses this step's dt; a later step with another
+  // dt, or any step through step_once, starts the pipeline again.
+  void step_fused(T dt) {
+    const bool fric = any_friction();
+    if (fric) gridf_.alloc((size_t)ntotal_ * 64);
+    hipEvent_t e_step = timer_.begin(stream_);
+    if (!fused_ || fused_dt_ != dt) prime_fused(dt, fric);
+    if (rebin_due()) {
+      rebin();
+      fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 0);
+    }
+    Args<T, D> a = make_args(dt);
+    a.order = identity_order_ ? nullptr : order_.p;
+    const uint32_t ms_next = (uint32_t)(step_count_ + 2);  // mover stamp of the next step's grid
+    a.next_stamp = ms_next;
+
+    hipEvent_t e2 = timer_.begin(stream_);
+    a.stats_partial = stats_on_ ? stats_partial_.p : nullptr;
+    if (fric)
+      grid_kernel<T, D, true><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
+    else
+      grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
+    if (stats_on_) reduce_partials_kernel<<<1, 64, 0, stream_>>>(stats_partial_.p, grid_blocks(), stats_out_.p);
+    a.stats_partial = nullptr;
+    clear_tiles_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(gother_, gt_other_, lastcnt_.p + 1);
+    timer_.end(PH_GRID, e2, stream_);
+
+    hipEvent_t e3 = timer_.begin(stream_);
+    if (fric) {
+      p2g_kernel<T, D, true, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+      g2p_mover_kernel<T, D, true, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+    } else {
+      p2g_kernel<T, D, true, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+      g2p_mover_kernel<T, D, false, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+    }
+    p2g_extra_kernel<T, D, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+    fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 1);
+    timer_.end(PH_G2P2G, e3, stream_);
+    const int slot = (int)(step_count_ & 3);
+    HIPCHECK(hipMemcpyAsync(rb_host_ + slot, counts_.p + 9, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipEventRecord(rb_ev_[slot], stream_));
+    rb_meta_[slot] = Readback{step_count_, bin_stamp_};
+
+    // tile list of the next step's grid (now in gother_): binned tiles and
+    // the tiles marked by this step's movers; the friction grid is cleared
+    // over it
+    hipEvent_t e0 = timer_.begin(stream_);
+    const uint32_t nb = cdiv(ntotal_, BLOCK);
+    gcount_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms_next, gblock_.p, 0);
+    glist_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms_next, gblock_.p, gt_other_,
+                                                        counts_.p + 1, fric ? gridf_.p : nullptr, nullptr, nullptr);
+    timer_.end(PH_BIN, e0, stream_);
+    std::swap(gcur_, gother_);
+    std::swap(gt_cur_, gt_other_);
+    std::swap(ml_cur_, ml_next_);
+    std::swap(mt_cur_, mt_next_);
+    step_count_++;
+    steps_since_rebin_++;
+    since_reorder_++;
+    HIPCHECK(hipGetLastError());
+    has_grid_ = true;
+    fused_last_ = true;
     time_ += (double)dt;
     timer_.end(PH_STEP, e_step, stream_);
   }
@@ -2989,7 +3250,19 @@ class Engine final : public EngineBase {
   DevBuf<uint32_t> keys_, skeys_, order_, ktmp_, vtmp_;
   DevBuf<char> sort_scratch_;
   DevBuf<uint32_t> tile_begin_, tile_end_, tile_stamp_, mstamp_, gblock_, ptiles_, gtiles_, counts_, err_;
-  DevBuf<Node4<T>> grid_, gridf_;
+  DevBuf<Node4<T>> grid_, grid2_, gridf_;
+  DevBuf<uint32_t> gtiles2_, lastcnt_, mover_list2_, mover_tile2_;
+  // Fused steps alternate two grids: gcur_ holds the P2G of the current state
+  // (tile list gt_cur_, count counts[1]) and gother_ the last step's grid
+  // (gt_other_, count lastcnt_[1]), cleared before it takes the next P2G.
+  Node4<T>* gcur_ = nullptr;
+  Node4<T>* gother_ = nullptr;
+  uint32_t* gt_cur_ = nullptr;
+  uint32_t* gt_other_ = nullptr;
+  uint32_t *ml_cur_ = nullptr, *mt_cur_ = nullptr, *ml_next_ = nullptr, *mt_next_ = nullptr;
+  bool fused_ = false;       // gcur_ holds the P2G of the current state with dt fused_dt_
+  bool fused_last_ = false;  // gother_ holds the last step's grid (for mpmx_get_grid)
+  T fused_dt_ = 0;
   DevBuf<double> stats_partial_, stats_out_;
   DevBuf<T> stage_;
   DevBuf<uint32_t> slow_list_, mover_list_, mover_tile_;

@@ -7,7 +7,7 @@ and F, and for B to the tolerance the reference's own FP32 B meets against
 FP64 (its B update cancels large terms); B's error is measured against
 max(|B|, |V| h), since B of a body starting at rest is round-off at first.  The compressed neo-Hookean scene
 sends a third of the particles through the clamp branch, i.e. through the
-FP64 SVD stress path (p2g_slow_stress_kernel).
+FP64 SVD stress path (the slow list of p2g_extra_kernel).
 """
 import json
 import os
