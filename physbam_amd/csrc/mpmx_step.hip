@@ -344,10 +344,10 @@ template<> struct Vec4Type<double> { typedef double type __attribute__((ext_vect
 
 template<typename T, int D, bool FRICTION>
 __device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>& g, const int* org,
-                                          T (*sv)[Tile<D>::RN]);
+                                          typename Vec4Type<T>::type* sv);
 template<typename T, int D, bool FRICTION, bool FUSED>
 __device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, const Geom<T, D>& g, const int* org,
-                                              const T (*sv)[Tile<D>::RN]);
+                                              const typename Vec4Type<T>::type* sv);
 
 // Phase B of the tile P2G: the particle in coefficient slot sl, whose base
 // node is region node base_li, is added into this wavefront's copy of the
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
   __shared__ union {
     T c[C::N][BLOCK];            // phase A -> phase B coefficients
     uint32_t wc[JT][NW][64];     // grouping: per (pass, wave) cell counts, then offsets
-    T v[FUSED ? (FRICTION ? 2 * D : D) : 1][FUSED ? RN : 1];  // FUSED: staged grid of the G2P half
+    T4 v[FUSED ? (FRICTION ? 2 : 1) : 1][FUSED ? RN : 1];  // FUSED: staged grid of the G2P half
   } s_u;
   __shared__ uint32_t s_idx[CH];  // chunk's particle slots grouped by cell
   __shared__ uint32_t s_cb[64], s_cn[64], s_max;
@@ -519,9 +519,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
     int org[D];
     tile_origin<T, D>(t, g, org);
     if constexpr (FUSED) {
-      g2p_stage<T, D, FRICTION>(a, g, org, s_u.v);
+      g2p_stage<T, D, FRICTION>(a, g, org, &s_u.v[0][0]);
       __syncthreads();
-      g2p_tile_pass<T, D, FRICTION, true>(a, t, g, org, s_u.v);
+      g2p_tile_pass<T, D, FRICTION, true>(a, t, g, org, &s_u.v[0][0]);
       __syncthreads();
     }
     for (int k = threadIdx.x; k < NW * RN; k += BLOCK) {
@@ -1200,11 +1200,13 @@ __device__ __forceinline__ bool g2p_store(const Args<T, D>& a, const Geom<T, D>&
   return true;
 }
 
-// The tile's stencil region of v (rows 0..D-1) and, with FRICTION, of the
-// friction velocity (rows D..2D-1) of a.grid / a.gridf, staged in LDS.
+// The tile's stencil region of the grid nodes (m, v) of a.grid and, with
+// FRICTION, of a.gridf (second half), staged in LDS as 4-vectors so that a
+// node is one 16-byte (FP32) LDS read.
 template<typename T, int D, bool FRICTION>
 __device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>& g, const int* org,
-                                          T (*sv)[Tile<D>::RN]) {
+                                          typename Vec4Type<T>::type* sv) {
+  using T4 = typename Vec4Type<T>::type;
   for (int k = threadIdx.x; k < Tile<D>::RN; k += BLOCK) {
     int off[D], nc[D];
     region_offsets<D>(k, off);
@@ -1212,14 +1214,10 @@ __device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>&
     for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
     const uint32_t slot = node_slot<T, D>(nc, g);
     const Node4<T> nd = a.grid[slot];
-    sv[0][k] = nd.x;
-    sv[1][k] = nd.y;
-    if constexpr (D == 3) sv[2][k] = nd.z;
+    sv[k] = T4{nd.m, nd.x, nd.y, nd.z};
     if constexpr (FRICTION) {
       const Node4<T> nf = a.gridf[slot];
-      sv[D][k] = nf.x;
-      sv[D + 1][k] = nf.y;
-      if constexpr (D == 3) sv[D + 2][k] = nf.z;
+      sv[Tile<D>::RN + k] = T4{nf.m, nf.x, nf.y, nf.z};
     }
   }
 }
@@ -1230,8 +1228,8 @@ __device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>&
 // mover list (counts[5]).
 template<typename T, int D, bool FRICTION, bool FUSED>
 __device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, const Geom<T, D>& g, const int* org,
-                                              const T (*sv)[Tile<D>::RN]) {
-  constexpr int R = Tile<D>::R, E = Tile<D>::E;
+                                              const typename Vec4Type<T>::type* sv) {
+  constexpr int R = Tile<D>::R, E = Tile<D>::E, RN = Tile<D>::RN;
   const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
   for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) {
     const uint32_t p = a.order ? a.order[i] : i;
@@ -1250,11 +1248,12 @@ __device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, c
     }
     if (!in) continue;  // mover
     const int li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
+    auto comp = [](const typename Vec4Type<T>::type& q, int c) -> T { return c == 0 ? q.y : (c == 1 ? q.z : q.w); };
     auto nv = [&](int ox, int oy, int oz, int c) -> T {
-      return sv[c][li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)];
+      return comp(sv[li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)], c);
     };
     auto nf = [&](int ox, int oy, int oz, int c) -> T {
-      if constexpr (FRICTION) return sv[D + c][li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)];
+      if constexpr (FRICTION) return comp(sv[RN + li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)], c);
       else return nv(ox, oy, oz, c);
     };
     T vb[D], vfb[D], G[D * D], Bs[D * D];
@@ -1271,16 +1270,16 @@ __device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, c
 // friction velocity) is staged in LDS; movers are left to g2p_mover_kernel.
 template<typename T, int D, bool FRICTION>
 __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
-  __shared__ T s_v[FRICTION ? 2 * D : D][Tile<D>::RN];
+  __shared__ typename Vec4Type<T>::type s_v[FRICTION ? 2 : 1][Tile<D>::RN];
   const uint32_t ntile = a.counts[0];
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
     const uint32_t t = a.ptiles[seg];
     const Geom<T, D> g = tile_geom(a, t);
     int org[D];
     tile_origin<T, D>(t, g, org);
-    g2p_stage<T, D, FRICTION>(a, g, org, s_v);
+    g2p_stage<T, D, FRICTION>(a, g, org, &s_v[0][0]);
     __syncthreads();
-    g2p_tile_pass<T, D, FRICTION, false>(a, t, g, org, s_v);
+    g2p_tile_pass<T, D, FRICTION, false>(a, t, g, org, &s_v[0][0]);
     __syncthreads();
   }
 }
