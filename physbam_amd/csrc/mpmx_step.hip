@@ -333,8 +333,17 @@ __global__ __launch_bounds__(BLOCK) void glist_kernel(Args<T, D> a, const uint32
 // impulse afterwards.
 template<typename T, int D> struct P2GCoef {
   // m, mv[D], mC[D*D], Q (symmetric, D(D+1)/2), xe[D]
+  static constexpr bool MASS = true, SYM = true;
   static constexpr int NQ = D * (D + 1) / 2;
   static constexpr int M = 0, MV = 1, MC = 1 + D, Q = 1 + D + D * D, XE = 1 + D + D * D + NQ;
+  static constexpr int N = XE + D;
+};
+// The scatter of the G2P adjoint in the same form (vbar_i += w (vt + Bbar dz)
+// - Q grad w with Q = -Gbar, no mass): vt[D], Bbar[D*D], Q[D*D], xe[D] -- the
+// order of AdjCoef, with Gbar negated.
+template<typename T, int D> struct AdjScatterCoef {
+  static constexpr bool MASS = false, SYM = false;
+  static constexpr int M = 0, MV = 0, MC = D, Q = D + D * D, XE = D + 2 * D * D;
   static constexpr int N = XE + D;
 };
 
@@ -355,13 +364,12 @@ __device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, c
 // one read-modify-write step are distinct; the compiler barrier after each
 // write keeps the steps in order (a later step of one lane may touch the node
 // an earlier step of another lane wrote).
-template<typename T, int D>
+template<typename T, int D, class C = P2GCoef<T, D>>
 __device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*s_c)[BLOCK], int sl, int base_li,
                                                  typename Vec4Type<T>::type* reg) {
   constexpr int R = Tile<D>::R;
-  using C = P2GCoef<T, D>;
   using T4 = typename Vec4Type<T>::type;
-  const T m = s_c[C::M][sl];
+  const T m = C::MASS ? s_c[C::M][sl] : (T)0;
   T mv[D], mC[D][D], Q[D][D], w[3][D], dw[3][D], dz[3][D];
 #pragma unroll
   for (int c = 0; c < D; c++) mv[c] = s_c[C::MV + c][sl];
@@ -369,7 +377,7 @@ __device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*
   for (int j = 0; j < D; j++)
 #pragma unroll
     for (int c = 0; c < D; c++) mC[c][j] = s_c[C::MC + c + D * j][sl];
-  {
+  if constexpr (C::SYM) {
     int qq = 0;
 #pragma unroll
     for (int c = 0; c < D; c++)
@@ -379,6 +387,11 @@ __device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*
         Q[c][j] = qv;
         Q[j][c] = qv;
       }
+  } else {
+#pragma unroll
+    for (int j = 0; j < D; j++)
+#pragma unroll
+      for (int c = 0; c < D; c++) Q[c][j] = s_c[C::Q + c + D * j][sl];
   }
 #pragma unroll
   for (int c = 0; c < D; c++) {
@@ -435,7 +448,7 @@ __device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*
           const T wz = w[oz][2], dwz = dw[oz][2], wzz = wzd[oz];
           const int li = base_li + (ox * R + oy) * R + oz;
           T4 cur = reg[li];
-          cur.x += wz * mxy;
+          if constexpr (C::MASS) cur.x += wz * mxy;
           cur.y += wz * P[0];
           cur.z += wz * P[1];
           cur.w += wz * P[2];
@@ -470,7 +483,7 @@ __device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*
         const T wy = w[oy][D - 1], dwy = dw[oy][D - 1], wyz = wyd[oy];
         const int li = base_li + ox * R + oy;
         T4 cur = reg[li];
-        cur.x += wy * mx;
+        if constexpr (C::MASS) cur.x += wy * mx;
         cur.y += wy * ax[0];
         cur.z += wy * ax[1];
         cur.y += wyz * ex[0];
@@ -1581,21 +1594,34 @@ __device__ __forceinline__ void g2p_adjoint_node(const Geom<T, D>& g, const T (*
 }
 
 // One workgroup per binned tile: the region's velocities are staged in LDS;
-// pass 1 gathers each particle (thread per particle) and leaves its scatter
-// coefficients in LDS, pass 2 gives each (particle, node) pair a lane and
-// accumulates vbar in the region with LDS atomics that never collide within a
-// particle; one HBM atomic per node and component at the flush.  Movers are
+// the tile's particles are grouped by base cell in chunks of 512 as in
+// p2g_kernel, phase A gathers each particle (x-bar, F-bar, ... written to HBM)
+// and leaves its vbar scatter coefficients in LDS, phase B scatters them with
+// the P2G scatter (lanes = cells, per-wavefront copies of the region, no LDS
+// atomics); one HBM atomic per node and component at the flush.  Movers are
 // left to g2p_adjoint_mover_kernel.
 template<typename T, int D>
-__global__ __launch_bounds__(BLOCK) void g2p_adjoint_kernel(Args<T, D> a, AdjArgs<T> q) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void g2p_adjoint_kernel(Args<T, D> a, AdjArgs<T> q) {
   using AC = AdjCoef<T, D>;
-  constexpr int R = Tile<D>::R, RN = Tile<D>::RN, E = Tile<D>::E;
+  using SC = AdjScatterCoef<T, D>;
+  using T4 = typename Vec4Type<T>::type;
+  static_assert(SC::N == AC::N, "adjoint coefficient layouts");
+  constexpr int R = Tile<D>::R, RN = Tile<D>::RN, NW = BLOCK / 64, E = Tile<D>::E;
+  constexpr int JT = 2, CH = JT * BLOCK;
   __shared__ T s_v[D][RN];
-  __shared__ T s_b[D][RN];
-  __shared__ T s_c[AC::N][BLOCK];
-  __shared__ int s_li[BLOCK];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ T4 s_reg[NW][RN];
+  __shared__ union {
+    T c[SC::N][BLOCK];
+    uint32_t wc[JT][NW][64];
+  } s_u;
+  __shared__ uint32_t s_idx[CH];
+  __shared__ uint32_t s_cb[64], s_cn[64], s_max;
+  auto& s_c = s_u.c;
   const uint32_t ntile = a.counts[0];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int a_cell = threadIdx.x >> 2, a_j = threadIdx.x & 3;
+  const int a_slot = a_j * 64 + a_cell;
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
   for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
     const uint32_t t = a.ptiles[seg];
     const Geom<T, D> g = tile_geom(a, t);
@@ -1610,70 +1636,130 @@ __global__ __launch_bounds__(BLOCK) void g2p_adjoint_kernel(Args<T, D> a, AdjArg
       s_v[0][k] = nd.x;
       s_v[1][k] = nd.y;
       if constexpr (D == 3) s_v[2][k] = nd.z;
-#pragma unroll
-      for (int c = 0; c < D; c++) s_b[c][k] = 0;
     }
-    __syncthreads();
-    const uint32_t b = a.tile_begin[t], e = a.tile_end[t];
-    for (uint32_t c0 = b; c0 < e; c0 += BLOCK) {
-      // ---- pass 1
-      const uint32_t i = c0 + threadIdx.x;
-      int li0 = -1;
-      if (i < e) {
-        const uint32_t p = a.order ? a.order[i] : i;
-        if (a.valid[p]) {
+    for (int k = threadIdx.x; k < NW * RN; k += BLOCK) {
+      T4 z = {0, 0, 0, 0};
+      s_reg[k / RN][k % RN] = z;
+    }
+    const uint32_t tb = a.tile_begin[t], te = a.tile_end[t];
+    for (uint32_t c0 = tb; c0 < te; c0 += CH) {
+      for (int k = threadIdx.x; k < JT * NW * 64; k += BLOCK) (&s_u.wc[0][0][0])[k] = 0;
+      __syncthreads();
+      // ---- group the chunk by base cell (movers are skipped)
+      uint32_t cell_j[JT], rank_j[JT], slot_j[JT];
+#pragma unroll
+      for (int j = 0; j < JT; j++) {
+        const uint32_t i = c0 + j * BLOCK + threadIdx.x;
+        uint32_t cell = 64, p = 0;
+        if (i < te) {
+          p = a.order ? a.order[i] : i;
+          if (a.valid[p]) {
+            T x[D];
+#pragma unroll
+            for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+            bool in = true;
+            uint32_t cl = 0;
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+              const int lb = stencil_base<T, D>(g, x, c) + 3 - org[c];
+              in = in && lb >= 0 && lb < E;
+              cl = cl * E + (uint32_t)(lb & (E - 1));
+            }
+            if (in) cell = cl;
+          }
+        }
+        const bool act = cell < 64;
+        uint64_t m = __ballot(act);
+#pragma unroll
+        for (int bb = 0; bb < 6; bb++) {
+          const bool bit = (cell >> bb) & 1u;
+          const uint64_t bal = __ballot(bit);
+          m &= bit ? bal : ~bal;
+        }
+        rank_j[j] = (uint32_t)__popcll(m & lt_mask);
+        if (act && lane == __ffsll((unsigned long long)m) - 1) s_u.wc[j][wv][cell] = (uint32_t)__popcll(m);
+        cell_j[j] = cell;
+        slot_j[j] = p;
+      }
+      __syncthreads();
+      if (threadIdx.x < 64) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int j = 0; j < JT; j++)
+#pragma unroll
+          for (int w = 0; w < NW; w++) {
+            const uint32_t v = s_u.wc[j][w][threadIdx.x];
+            s_u.wc[j][w][threadIdx.x] = run;
+            run += v;
+          }
+        uint32_t x = run, mx = run;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = __shfl_up(x, o, 64);
+          if (lane >= o) x += y;
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o, 64));
+        s_cb[threadIdx.x] = x - run;
+        s_cn[threadIdx.x] = run;
+        if (threadIdx.x == 0) s_max = mx;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < JT; j++)
+        if (cell_j[j] < 64) s_idx[s_cb[cell_j[j]] + s_u.wc[j][wv][cell_j[j]] + rank_j[j]] = slot_j[j];
+      __syncthreads();
+      const uint32_t a_cb = s_cb[a_cell], a_cnt = s_cn[a_cell];
+      const uint32_t b_cnt = s_cn[lane];
+      const uint32_t rounds = (s_max + NW - 1) / NW;
+      for (uint32_t k = 0; k < rounds; k++) {
+        // ---- phase A: gather of particle 4k + a_j of cell a_cell
+        const uint32_t ja = k * NW + a_j;
+        if (ja < a_cnt) {
+          const uint32_t p = s_idx[a_cb + ja];
           T x[D];
 #pragma unroll
           for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
           Wts<T, D> w;
           weights<T, D>(g, x, w);
-          bool in = true;
           int lb[D];
 #pragma unroll
-          for (int c = 0; c < D; c++) {
-            lb[c] = w.base[c] - org[c];
-            in = in && lb[c] >= 0 && lb[c] < E;
-          }
-          if (in) {  // else a mover
-            li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
-            auto index = [&](const int* o) -> uint32_t {
-              return (uint32_t)(li0 + (D == 3 ? (o[0] * R + o[1]) * R + o[D - 1] : o[0] * R + o[D - 1]));
-            };
-            auto nv = [&](uint32_t ix, int c) -> T { return s_v[c][ix]; };
-            T coef[AC::N];
-            g2p_adjoint_gather<T, D>(a, g, q, p, w, index, nv, coef);
+          for (int c = 0; c < D; c++) lb[c] = w.base[c] - org[c];
+          const int li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
+          auto index = [&](const int* o) -> uint32_t {
+            return (uint32_t)(li0 + (D == 3 ? (o[0] * R + o[1]) * R + o[D - 1] : o[0] * R + o[D - 1]));
+          };
+          auto nv = [&](uint32_t ix, int c) -> T { return s_v[c][ix]; };
+          T coef[AC::N];
+          g2p_adjoint_gather<T, D>(a, g, q, p, w, index, nv, coef);
 #pragma unroll
-            for (int k = 0; k < AC::N; k++) s_c[k][threadIdx.x] = coef[k];
-          }
+          for (int kk = 0; kk < AC::N; kk++)
+            s_c[kk][a_slot] = kk >= AC::GB && kk < AC::GB + D * D ? -coef[kk] : coef[kk];
         }
+        __syncthreads();
+        // ---- phase B
+        if (k * NW + wv < b_cnt) {
+          int lcell[D];
+          cell_offsets<D>((uint32_t)lane, lcell);
+          const int lane_li = D == 3 ? (lcell[0] * R + lcell[1]) * R + lcell[D - 1] : lcell[0] * R + lcell[D - 1];
+          p2g_scatter_slot<T, D, SC>(g, s_c, threadIdx.x, lane_li, s_reg[wv]);
+        }
+        __syncthreads();
       }
-      s_li[threadIdx.x] = li0;
-      __syncthreads();
-      // ---- pass 2: lane = (particle, node)
-      const uint32_t cnt = min((uint32_t)BLOCK, e - c0);
-      for (uint32_t pp = wv * 2; pp < cnt; pp += 2 * (BLOCK / 64)) {
-        const uint32_t sl = pp + (lane >> 5);
-        const int k = lane & 31;
-        if (sl >= cnt || k >= Tile<D>::K || s_li[sl] < 0) continue;
-        int o[D];
-        T vb[D];
-        g2p_adjoint_node<T, D, AC::N>(g, s_c, sl, k, o, vb);
-        const int li = s_li[sl] + (D == 3 ? (o[0] * R + o[1]) * R + o[D - 1] : o[0] * R + o[D - 1]);
-#pragma unroll
-        for (int c = 0; c < D; c++) atomicAdd(&s_b[c][li], vb[c]);
-      }
-      __syncthreads();
     }
     for (int k = threadIdx.x; k < RN; k += BLOCK) {
+      T4 sum = s_reg[0][k];
+#pragma unroll
+      for (int w = 1; w < NW; w++) sum += s_reg[w][k];
       int off[D], nc[D];
       region_offsets<D>(k, off);
 #pragma unroll
       for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
       Node4<T>* an = q.agrid + node_slot<T, D>(nc, g);
-      if (s_b[0][k] != (T)0) atomic_add(&an->x, s_b[0][k]);
-      if (s_b[1][k] != (T)0) atomic_add(&an->y, s_b[1][k]);
+      if (sum.y != (T)0) atomic_add(&an->x, sum.y);
+      if (sum.z != (T)0) atomic_add(&an->y, sum.z);
       if constexpr (D == 3)
-        if (s_b[2][k] != (T)0) atomic_add(&an->z, s_b[2][k]);
+        if (sum.w != (T)0) atomic_add(&an->z, sum.w);
     }
     __syncthreads();
   }
@@ -2473,7 +2559,7 @@ class Engine final : public EngineBase {
   void step(double dt, int nsteps, bool wait) override {
     if (!(dt > 0)) throw Error{MPMX_EINVAL, "dt must be positive"};
     if (nsteps < 0) throw Error{MPMX_EINVAL, "negative step count"};
-    const bool fuse = !taping_ && n_ > 0;
+    const bool fuse = n_ > 0;  // recorded steps too: the reverse pass recomputes its own
     for (int s = 0; s < nsteps; s++) {
       if (fuse)
         step_fused((T)dt);
@@ -2729,7 +2815,7 @@ class Engine final : public EngineBase {
         grid_override_ = sgrid[i].grid.p;
         gridu_override_ = sgrid[i].gridu.p;
         step_once((T)tape_dt_[(size_t)j]);
-        save_tiles_kernel<<<copy_blocks(), BLOCK, 0, stream_>>>(gtiles_.p, counts_.p, sgrid[i].tiles.p,
+        save_tiles_kernel<<<copy_blocks(), BLOCK, 0, stream_>>>(gt_cur_, counts_.p, sgrid[i].tiles.p,
                                                                 seg_counts_.p + i);
         grid_override_ = nullptr;
         gridu_override_ = nullptr;
@@ -3185,6 +3271,13 @@ class Engine final : public EngineBase {
     step_count_++;
     steps_since_rebin_++;
     since_reorder_++;
+    if (taping_) {
+      tape_dt_.push_back((double)dt);
+      if (tape_dt_.size() % (size_t)tape_k_ == 0) {
+        ckpts_.emplace_back();
+        save_state(ckpts_.back());
+      }
+    }
     HIPCHECK(hipGetLastError());
     has_grid_ = true;
     fused_last_ = true;
