@@ -189,6 +189,23 @@ int mpmx_compute_dt(mpmx_ctx* ctx, double cfl, double max_dt, double* dt);
 int mpmx_advance_frames(mpmx_ctx* ctx, int nframes, double frame_dt, double cfl, double min_dt, double max_dt,
                         double* dts, int64_t dts_capacity, int64_t* nsubsteps);
 
+/* Plasticity: MPM_PLASTICITY_CLAMP with hardening
+   (Hybrid_Methods/Forces/MPM_PLASTICITY_CLAMP.cpp:31-45 and
+   MPM_PLASTICITY_MODEL.cpp:34-48, as Projects/mpm/STANDARD_TESTS_BASE.cpp:
+   465-486 Add_Clamped_Plasticity adds it to the explicit step).  After every
+   G2P the singular values of each simulated particle's F are clamped to
+   [1 - theta_c, 1 + theta_s]; when any changes, mu and lambda become mu0 and
+   lambda0 times max(1e-8, exp(min(max_hardening, hardening_factor (1 - det
+   Fp)))), F = U clamp(S) V^T and Fp = V clamp(S)^-1 U^T F Fp.  Enabling sets
+   Fp = I and mu0, lambda0 = the current mu, lambda of every particle (and so
+   does each later mpmx_set_particles); enable = 0 turns it off.  Steps with
+   plasticity take the unfused G2P/P2G kernels.  mpmx_get_plastic_state
+   returns Fp (dim*dim per particle, column-major), mu and lambda (NULL
+   skips).  Not differentiable: mpmx_tape_begin rejects it. */
+int mpmx_set_plasticity(mpmx_ctx* ctx, int enable, double theta_c, double theta_s, double max_hardening,
+                        double hardening_factor);
+int mpmx_get_plastic_state(mpmx_ctx* ctx, void* Fp, void* mu, void* lambda);
+
 /* Differentiation (no counterpart in the reference, whose step has no
    adjoint).  mpmx_tape_begin records the current particle state and then a
    checkpoint every `checkpoint_every` steps of the following mpmx_step calls.

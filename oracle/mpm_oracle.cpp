@@ -23,6 +23,9 @@
 //   G2P          MPM_DRIVER.cpp:450-515
 //   invalidation MPM_DRIVER.cpp:1219-1239, simulated list :1142-1157
 //   reductions   Examples_And_Drivers/MPM_EXAMPLE.cpp:209-315
+//   plasticity   Forces/MPM_PLASTICITY_CLAMP.cpp:31-45 (Compute with hardening),
+//                Forces/MPM_PLASTICITY_MODEL.cpp:34-48 (Update_Particles),
+//                MPM_DRIVER.cpp:156, :920-926 (after G2P, over the step's simulated particles)
 //   adaptive dt  MPM_DRIVER.cpp:1098-1104 (Compute_Dt), :1125-1138 (Grid_V_Upper_Bound),
 //                :173-197 (substep loop of Simulate_To_Frame), Core/Math_Tools/clamp.h:13-16
 //
@@ -658,6 +661,69 @@ template<class T,int d> int Run(const int* N,const double* lo,const double* hi,c
                 if(gvfric) gvfric[i*d+a]=G.velocity_friction_save[i][a];}}}
     return 0;
 }
+// Clamped plasticity with hardening (MPM_PLASTICITY_CLAMP, the snow model):
+// the singular values of F are clamped to [1 - theta_c, 1 + theta_s]; when
+// any changes, mu and lambda become mu0, lambda0 times
+// exp(min(max_hardening, hardening_factor (1 - det Fp_new))) (at least 1e-8),
+// F = U clamp(S) V^T and Fp = V clamp(S)^-1 U^T F Fp.
+template<class T,int d> struct Plastic
+{
+    T theta_c,theta_s,max_hardening,hardening_factor;
+    T* Fp;const T* mu0;const T* lambda0;T* mu;T* lambda;
+};
+template<class T> T Determinant(const Mat<T,3>& A)
+{const T* x=A.x;return x[0]*(x[4]*x[8]-x[7]*x[5])+x[3]*(x[7]*x[2]-x[1]*x[8])+x[6]*(x[1]*x[5]-x[4]*x[2]);}
+template<class T> T Determinant(const Mat<T,2>& A){const T* x=A.x;return x[0]*x[3]-x[1]*x[2];}
+
+template<class T,int d> void Plasticity_Update(const Plastic<T,d>& pl,Particles<T,d>& P,const std::vector<unsigned char>& simulated)
+{
+    const T lo=1-pl.theta_c,hi=1+pl.theta_s;
+    for(long long p=0;p<P.n;p++){
+        if(!simulated[p]) continue;
+        Mat<T,d> Fe=P.f(p),U,V;
+        Vec<T,d> sv;
+        Svd(Fe,U,sv,V);
+        Vec<T,d> strain;
+        bool same=true;
+        for(int a=0;a<d;a++){
+            strain[a]=sv[a]<=lo?lo:(sv[a]>=hi?hi:sv[a]);
+            if(!(strain[a]==sv[a])) same=false;}
+        if(same) continue;
+        Mat<T,d> Fp;for(int k=0;k<d*d;k++) Fp.x[k]=pl.Fp[p*d*d+k];
+        T det=Product(sv)/Product(strain)*Determinant(Fp);
+        T hc=std::exp(std::min(pl.max_hardening,pl.hardening_factor*(1-det)));
+        if(hc<(T)1e-8) hc=(T)1e-8;
+        pl.mu[p]=pl.mu0[p]*hc;
+        pl.lambda[p]=pl.lambda0[p]*hc;
+        Mat<T,d> US;   // MATRIX * DIAGONAL_MATRIX: column j scaled
+        for(int j=0;j<d;j++) for(int i=0;i<d;i++) US(i,j)=U(i,j)*strain[j];
+        Mat<T,d> Fnew=Times_Transpose(US,V);
+        Mat<T,d> DU;   // DIAGONAL_MATRIX::Times_Transpose: r(i,j)=1/s_i U(j,i)
+        for(int i=0;i<d;i++) for(int j=0;j<d;j++) DU(i,j)=((T)1/strain[i])*U(j,i);
+        Mat<T,d> Fpn=((V*DU)*Fe)*Fp;
+        for(int k=0;k<d*d;k++){P.F[p*d*d+k]=Fnew.x[k];pl.Fp[p*d*d+k]=Fpn.x[k];}}
+}
+
+template<class T,int d> int Run_Plastic(const int* N,const double* lo,const double* hi,const double* g,int model,double thr,
+    double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi,
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,T* mu,T* la,unsigned char* valid,int nsteps,
+    const double* plastic,T* Fp)
+{
+    Scene<T,d> s=Make_Scene<T,d>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi);
+    std::vector<T> mu0(mu,mu+np),la0(la,la+np);
+    Plastic<T,d> pl{(T)plastic[0],(T)plastic[1],(T)plastic[2],(T)plastic[3],Fp,mu0.data(),la0.data(),mu,la};
+    Particles<T,d> P{np,X,V,F,B,mass,volume,mu,la,valid};
+    Grid<T,d> G;
+    T time=0;
+    std::vector<unsigned char> simulated(np);
+    for(int k=0;k<nsteps;k++){
+        for(long long p=0;p<np;p++) simulated[p]=valid[p];
+        Step(s,P,G,time);
+        Plasticity_Update(pl,P,simulated);
+        time+=s.dt;}
+    return 0;
+}
+
 // MPM_DRIVER::Grid_V_Upper_Bound (MPM_DRIVER.cpp:1125-1138) for use_affine and
 // use_affine_cfl (MPM_EXAMPLE.h:116) with B stored: max over simulated
 // particles of |v| + |B|_F * 6 sqrt(d) / min dx.
@@ -765,6 +831,23 @@ DEFINE_RUN(f64,double)
 DEFINE_RUN(f32,float)
 DEFINE_FRAME(f64,double)
 DEFINE_FRAME(f32,float)
+
+// plastic[4] = theta_c, theta_s, max_hardening, hardening_factor; mu0 and
+// lambda0 are the mu and lambda passed in; Fp, mu, lambda are updated in place.
+#define DEFINE_PLASTIC(SUF,T) \
+ORACLE_API int oracle_run_plastic_##SUF(int dim,const int* N,const double* lo,const double* hi,const double* g,int model, \
+    double thr,double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi, \
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,T* mu,T* la,unsigned char* valid,int nsteps, \
+    const double* plastic,T* Fp) \
+{ \
+    if(dim==2) return Run_Plastic<T,2>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume,mu,la,valid, \
+        nsteps,plastic,Fp); \
+    if(dim==3) return Run_Plastic<T,3>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume,mu,la,valid, \
+        nsteps,plastic,Fp); \
+    return -1; \
+}
+DEFINE_PLASTIC(f64,double)
+DEFINE_PLASTIC(f32,float)
 
 template<class T,int d> static void Svd_Many(long long n,const T* A,T* U,T* S,T* V,int* iters)
 {

@@ -11,6 +11,8 @@
 //   ref_harness svd   <prec> <dim> <in.bin> <out.bin>
 //   ref_harness pstrain <prec> <dim> <model> <threshold> <in.bin> <out.bin>
 //   ref_harness frames <scene.bin> <nframes> <frame_dt> <cfl> <min_dt> <max_dt> <out.bin>
+// A scene with the plasticity flag set adds MPM_PLASTICITY_CLAMP (Fp, mu0,
+// lambda0 stored) and its dumps end with Fp, mu and lambda.
 //   ref_harness weights <prec> <dim> <N> <lo> <hi> <in.bin> <out.bin>
 //
 // Scene file layout (little endian) is written by tests/golden/scene_io.py.
@@ -28,6 +30,7 @@
 #include <Hybrid_Methods/Examples_And_Drivers/MPM_PARTICLES.h>
 #include <Hybrid_Methods/Forces/MPM_FINITE_ELEMENTS.h>
 #include <Hybrid_Methods/Forces/MPM_GRAVITY.h>
+#include <Hybrid_Methods/Forces/MPM_PLASTICITY_CLAMP.h>
 #include <Hybrid_Methods/Forces/MPM_FORCE_HELPER.h>
 #include <Hybrid_Methods/Iterators/GATHER_SCATTER.h>
 #include <Hybrid_Methods/Iterators/PARTICLE_GRID_WEIGHTS_SPLINE.h>
@@ -64,9 +67,10 @@ struct WRITER
 struct WALL {int type;double friction;double lo[3],hi[3];};
 struct SCENE
 {
-    int dim,prec,model,nsteps,dump_every,nwalls,threads;
+    int dim,prec,model,nsteps,dump_every,nwalls,threads,plastic;
     int N[3];
     double lo[3],hi[3],gravity[3],dt,neo_threshold;
+    double theta_c,theta_s,max_hardening,hardening_factor;  // clamped plasticity (plastic != 0)
     long long np;
     std::vector<WALL> walls;
     std::vector<double> X,V,F,B,mass,volume,mu,lambda;
@@ -80,9 +84,12 @@ SCENE Read_Scene(const char* path)
     if(memcmp(magic,"MPMSCN1",7)){fprintf(stderr,"bad scene magic\n");exit(2);}
     SCENE s;
     s.dim=r.get<int>();s.prec=r.get<int>();s.model=r.get<int>();s.nsteps=r.get<int>();
-    s.dump_every=r.get<int>();s.nwalls=r.get<int>();s.threads=r.get<int>();r.get<int>();
+    s.dump_every=r.get<int>();s.nwalls=r.get<int>();s.threads=r.get<int>();s.plastic=r.get<int>();
     r.get(s.N,3);r.get(s.lo,3);r.get(s.hi,3);r.get(s.gravity,3);
     s.dt=r.get<double>();s.neo_threshold=r.get<double>();
+    s.theta_c=s.theta_s=s.max_hardening=s.hardening_factor=0;
+    if(s.plastic){s.theta_c=r.get<double>();s.theta_s=r.get<double>();s.max_hardening=r.get<double>();
+        s.hardening_factor=r.get<double>();}
     s.np=r.get<long long>();
     for(int i=0;i<s.nwalls;i++){
         WALL w;w.type=r.get<int>();r.get<int>();w.friction=r.get<double>();
@@ -137,7 +144,17 @@ struct RUN
         ISOTROPIC_CONSTITUTIVE_MODEL<T,d>* cm=0;
         if(s.model==0) cm=new COROTATED_FIXED<T,d>((T)1e4,(T).3);
         else cm=new NEO_HOOKEAN<T,d>((T)1e4,(T).3,(T).05,(T)s.neo_threshold);
-        ex.Add_Force(*new MPM_FINITE_ELEMENTS<TV>(ex.force_helper,*cm,ex.gather_scatter,0));
+        MPM_FINITE_ELEMENTS<TV>* fe=new MPM_FINITE_ELEMENTS<TV>(ex.force_helper,*cm,ex.gather_scatter,0);
+        ex.Add_Force(*fe);
+        if(s.plastic){
+            // STANDARD_TESTS_BASE::Add_Clamped_Plasticity (symplectic, explicit plasticity)
+            P.Store_Fp(true);
+            for(int p=0;p<(int)s.np;p++){
+                P.Fp(p)=MATRIX<T,d>::Identity_Matrix();
+                P.mu0(p)=P.mu(p);P.lambda0(p)=P.lambda(p);}
+            MPM_PLASTICITY_CLAMP<TV>* pl=new MPM_PLASTICITY_CLAMP<TV>(P,&fe->gather_scatter,(T)s.theta_c,(T)s.theta_s,
+                (T)s.max_hardening,(T)s.hardening_factor);
+            ex.plasticity_models.Append(pl);}
         ex.Add_Force(*new MPM_GRAVITY<TV>(ex.force_helper,g,ex.gather_scatter,0));
         for(const WALL& w:s.walls){
             TV a,b;for(int k=0;k<d;k++){a(k)=(T)w.lo[k];b(k)=(T)w.hi[k];}
@@ -198,6 +215,15 @@ struct RUN
         buf.resize((size_t)nn*d);
         for(int k=0;k<3;k++){
             for(int i=0;i<nn;i++) for(int a=0;a<d;a++) buf[i*d+a]=arrs[k]->array(i)(a);
+            w.put(buf.data(),buf.size());}
+        if(s.plastic){  // plastic deformation and hardened Lame parameters
+            buf.resize((size_t)n*d*d);
+            for(int p=0;p<n;p++) for(int k=0;k<d*d;k++) buf[p*d*d+k]=P.Fp(p).x[k];
+            w.put(buf.data(),buf.size());
+            buf.resize(n);
+            for(int p=0;p<n;p++) buf[p]=P.mu(p);
+            w.put(buf.data(),buf.size());
+            for(int p=0;p<n;p++) buf[p]=P.lambda(p);
             w.put(buf.data(),buf.size());}
     }
 };

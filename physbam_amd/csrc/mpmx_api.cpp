@@ -186,6 +186,17 @@ int mpmx_advance_frames(mpmx_ctx* ctx, int nframes, double frame_dt, double cfl,
   return guard([&] { ctx->engine->advance_frames(nframes, frame_dt, cfl, min_dt, max_dt, dts, dts_capacity, nsubsteps); });
 }
 
+int mpmx_set_plasticity(mpmx_ctx* ctx, int enable, double theta_c, double theta_s, double max_hardening,
+                        double hardening_factor) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->set_plasticity(enable != 0, theta_c, theta_s, max_hardening, hardening_factor); });
+}
+
+int mpmx_get_plastic_state(mpmx_ctx* ctx, void* Fp, void* mu, void* lambda) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->get_plastic_state(Fp, mu, lambda); });
+}
+
 int mpmx_tape_begin(mpmx_ctx* ctx, int checkpoint_every) {
   if (!ctx) return bad("null context");
   return guard([&] { ctx->engine->tape_begin(checkpoint_every); });

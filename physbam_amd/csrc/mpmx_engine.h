@@ -41,6 +41,8 @@ struct EngineBase {
   virtual void compute_dt(double cfl, double max_dt, double* dt) = 0;
   virtual void advance_frames(int nframes, double frame_dt, double cfl, double min_dt, double max_dt, double* dts,
                               int64_t capacity, int64_t* nsub) = 0;
+  virtual void set_plasticity(bool enable, double theta_c, double theta_s, double max_hardening, double xi) = 0;
+  virtual void get_plastic_state(void* Fp, void* mu, void* lambda) = 0;
   virtual void tape_begin(int checkpoint_every) = 0;
   virtual void backward(const void* dX, const void* dV, const void* dF, const void* dB, void* gX, void* gV, void* gF,
                         void* gB, double* gmu, double* glambda) = 0;

@@ -2022,6 +2022,7 @@ template<typename T, int D> struct ParticleBufs {
   uint8_t* valid;
   uint32_t* pid;
   uint16_t* pscene;  // scene of each particle (batches only)
+  T* Fp; T* mu0; T* la0;  // plastic deformation, initial Lame parameters (plasticity only)
 };
 
 template<typename T, int D>
@@ -2047,6 +2048,101 @@ __global__ __launch_bounds__(BLOCK) void reorder_kernel(ParticleBufs<T, D> src, 
   dst.valid[i] = src.valid[p];
   dst.pid[i] = src.pid[p];
   if (src.pscene) dst.pscene[i] = src.pscene[p];
+  if (src.Fp) {
+#pragma unroll
+    for (int k = 0; k < D * D; k++) dst.Fp[k * n + i] = src.Fp[k * n + p];
+    dst.mu0[i] = src.mu0[p];
+    dst.la0[i] = src.la0[p];
+  }
+}
+
+// ---------------------------------------------------------- plasticity ----
+// MPM_PLASTICITY_CLAMP with hardening (Forces/MPM_PLASTICITY_CLAMP.cpp:31-45,
+// MPM_PLASTICITY_MODEL.cpp:34-48), after G2P, for the particles simulated in
+// the step (valid at its start: `simulated`).  In FP64: F = U S V^T; the
+// singular values clamped to [1 - theta_c, 1 + theta_s]; if any changed,
+//   h = max(1e-8, exp(min(max_hardening, xi (1 - prod(S)/prod(Sc) det Fp)))),
+//   mu = mu0 h, lambda = lambda0 h,  F = U Sc V^T,  Fp = V Sc^-1 U^T F Fp.
+template<typename T> struct PlasticDev {
+  double lo, hi, max_hardening, xi;
+};
+
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void plasticity_kernel(ParticleBufs<T, D> b, uint32_t n, const uint8_t* simulated,
+                                                           PlasticDev<T> pl) {
+  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < n; p += gridDim.x * BLOCK) {
+    if (!simulated[p]) continue;
+    double Fe[D * D], U[D * D], S[D], V[D * D];
+#pragma unroll
+    for (int k = 0; k < D * D; k++) Fe[k] = (double)b.F[(size_t)k * n + p];
+    svd<double, D>(Fe, U, S, V);
+    double Sc[D];
+    bool same = true;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      Sc[a] = S[a] <= pl.lo ? pl.lo : (S[a] >= pl.hi ? pl.hi : S[a]);
+      same = same && Sc[a] == S[a];
+    }
+    if (same) continue;
+    double Fp[D * D];
+#pragma unroll
+    for (int k = 0; k < D * D; k++) Fp[k] = (double)b.Fp[(size_t)k * n + p];
+    double ps = 1, pc = 1;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      ps *= S[a];
+      pc *= Sc[a];
+    }
+    double detp;
+    if constexpr (D == 3)
+      detp = Fp[0] * (Fp[4] * Fp[8] - Fp[7] * Fp[5]) + Fp[3] * (Fp[7] * Fp[2] - Fp[1] * Fp[8]) +
+             Fp[6] * (Fp[1] * Fp[5] - Fp[4] * Fp[2]);
+    else
+      detp = Fp[0] * Fp[3] - Fp[1] * Fp[2];
+    const double det = ps / pc * detp;
+    double h = exp(fmin(pl.max_hardening, pl.xi * (1 - det)));
+    if (h < 1e-8) h = 1e-8;
+    b.mu[p] = (T)((double)b.mu0[p] * h);
+    b.la[p] = (T)((double)b.la0[p] * h);
+    // F = U Sc V^T;  M = V Sc^-1 U^T F_e;  Fp = M Fp  (column-major)
+    double M[D * D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        double f = 0, m = 0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+          f += U[i + D * k] * Sc[k] * V[j + D * k];
+          double ut = 0;  // (U^T Fe)_kj
+#pragma unroll
+          for (int l = 0; l < D; l++) ut += U[l + D * k] * Fe[l + D * j];
+          m += V[i + D * k] / Sc[k] * ut;
+        }
+        b.F[(size_t)(i + D * j) * n + p] = (T)f;
+        M[i + D * j] = m;
+      }
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        double f = 0;
+#pragma unroll
+        for (int k = 0; k < D; k++) f += M[i + D * k] * Fp[k + D * j];
+        b.Fp[(size_t)(i + D * j) * n + p] = (T)f;
+      }
+  }
+}
+
+// Fp = I, mu0 = mu, lambda0 = lambda.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void plastic_init_kernel(ParticleBufs<T, D> b, uint32_t n) {
+  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < n; p += gridDim.x * BLOCK) {
+#pragma unroll
+    for (int k = 0; k < D * D; k++) b.Fp[(size_t)k * n + p] = k % (D + 1) == 0 ? (T)1 : (T)0;
+    b.mu0[p] = b.mu[p];
+    b.la0[p] = b.la[p];
+  }
 }
 
 // Host layout (per particle contiguous, user order) <-> device SoA (slot order).
@@ -2495,6 +2591,7 @@ class Engine final : public EngineBase {
       HIPCHECK(hipMemcpyAsync(cur_.valid, valid, n_, hipMemcpyHostToDevice, stream_));
       if (scene_ids) HIPCHECK(hipMemcpyAsync(cur_.pscene, scene_ids, n_ * sizeof(uint16_t), hipMemcpyHostToDevice, stream_));
       iota_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_.pid, n_);
+      if (plastic_) plastic_init_kernel<T, D><<<std::min<uint32_t>(cdiv(n_, BLOCK), 4096), BLOCK, 0, stream_>>>(cur_, n_);
       HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
       Args<T, D> a = make_args(0);
       key_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(a, err_.p + 1);
@@ -2559,7 +2656,9 @@ class Engine final : public EngineBase {
   void step(double dt, int nsteps, bool wait) override {
     if (!(dt > 0)) throw Error{MPMX_EINVAL, "dt must be positive"};
     if (nsteps < 0) throw Error{MPMX_EINVAL, "negative step count"};
-    const bool fuse = n_ > 0;  // recorded steps too: the reverse pass recomputes its own
+    // recorded steps too (the reverse pass recomputes its own); plasticity
+    // projects F between G2P and the next P2G, which the fused kernel does not
+    const bool fuse = n_ > 0 && !plastic_;
     for (int s = 0; s < nsteps; s++) {
       if (fuse)
         step_fused((T)dt);
@@ -2728,6 +2827,44 @@ class Engine final : public EngineBase {
     rebin_fraction_ = mover_fraction;
     rebin_max_interval_ = max_interval;
   }
+  void set_plasticity(bool enable, double theta_c, double theta_s, double max_hardening, double xi) override {
+    if (enable) {
+      if (!(theta_c >= 0 && theta_c < 1) || !(theta_s >= 0) || !std::isfinite(max_hardening) || !std::isfinite(xi))
+        throw Error{MPMX_EINVAL, "need 0 <= theta_c < 1, theta_s >= 0 and finite hardening parameters"};
+      if (taping_) throw Error{MPMX_ESTATE, "plasticity cannot change while a tape is recorded"};
+      // the clamp bounds as the reference forms them in T
+      pl_ = PlasticDev<T>{(double)((T)1 - (T)theta_c), (double)((T)1 + (T)theta_s), (double)(T)max_hardening,
+                          (double)(T)xi};
+    }
+    const bool was = plastic_;
+    plastic_ = enable;
+    fused_ = false;
+    if (enable && !was && n_ > 0) {
+      // add the plastic arrays to both particle buffer sets, keeping the state
+      auto add = [&](ParticleBufs<T, D>& pb, std::vector<DevBuf<T>>& fb) {
+        fb.resize(11);
+        fb[8].alloc((size_t)n_ * D * D); fb[9].alloc(n_); fb[10].alloc(n_);
+        pb.Fp = fb[8].p;
+        pb.mu0 = fb[9].p;
+        pb.la0 = fb[10].p;
+      };
+      add(cur_, fa_.size() && fa_[0].p == cur_.X ? fa_ : fb_);
+      add(alt_, fa_.size() && fa_[0].p == alt_.X ? fa_ : fb_);
+      plastic_init_kernel<T, D><<<std::min<uint32_t>(cdiv(n_, BLOCK), 4096), BLOCK, 0, stream_>>>(cur_, n_);
+      HIPCHECK(hipGetLastError());
+    } else if (!enable) {
+      cur_.Fp = cur_.mu0 = cur_.la0 = nullptr;
+      alt_.Fp = alt_.mu0 = alt_.la0 = nullptr;
+    }
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+  void get_plastic_state(void* Fp, void* mu, void* lambda) override {
+    if (!plastic_) throw Error{MPMX_ESTATE, "plasticity is not enabled"};
+    if (n_ == 0) return;
+    download_aos(cur_.Fp, (T*)Fp, D * D);
+    download_aos(cur_.mu, (T*)mu, 1);
+    download_aos(cur_.la, (T*)lambda, 1);
+  }
   void bin_stats(int64_t* rebins, int64_t* movers, int64_t* slow) override {
     synchronize();
     uint32_t c[16];
@@ -2763,6 +2900,7 @@ class Engine final : public EngineBase {
     if (k < 1) throw Error{MPMX_EINVAL, "checkpoint interval must be >= 1"};
     if (n_ == 0) throw Error{MPMX_ESTATE, "no particles"};
     if (any_friction()) throw Error{MPMX_EINVAL, "the adjoint step has no friction: remove wall friction"};
+    if (plastic_) throw Error{MPMX_EINVAL, "the adjoint step has no plasticity"};
     if (force_reorder_ || force_rebin_) rebin();  // settle the storage order before recording
     particle_stats(nullptr, nullptr, &tape_valid_);
     fused_ = false;
@@ -2902,7 +3040,14 @@ class Engine final : public EngineBase {
       ib.alloc(n);
       if (batch) sb.alloc(n);
       pb = ParticleBufs<T, D>{fb[0].p, fb[1].p, fb[2].p, fb[3].p, fb[4].p, fb[5].p, fb[6].p, fb[7].p,
-                              vb.p,    ib.p,    batch ? sb.p : nullptr};
+                              vb.p,    ib.p,    batch ? sb.p : nullptr, nullptr, nullptr, nullptr};
+      if (plastic_) {
+        fb.resize(11);
+        fb[8].alloc(n * D * D); fb[9].alloc(n); fb[10].alloc(n);
+        pb.Fp = fb[8].p;
+        pb.mu0 = fb[9].p;
+        pb.la0 = fb[10].p;
+      }
     };
     // Buffers are sized exactly to n so that the SoA stride equals n.
     for (auto* fbv : {&fa_, &fb_})
@@ -3162,6 +3307,10 @@ class Engine final : public EngineBase {
     timer_.end(PH_GRID, e2, stream_);
 
     hipEvent_t e3 = timer_.begin(stream_);
+    if (plastic_) {  // the particles simulated in this step, before G2P invalidates any
+      simmask_.alloc(n_);
+      HIPCHECK(hipMemcpyAsync(simmask_.p, cur_.valid, n_, hipMemcpyDeviceToDevice, stream_));
+    }
     if (fric) {
       g2p_kernel<T, D, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
       g2p_mover_kernel<T, D, true, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
@@ -3169,6 +3318,9 @@ class Engine final : public EngineBase {
       g2p_kernel<T, D, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
       g2p_mover_kernel<T, D, false, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
     }
+    if (plastic_)
+      plasticity_kernel<T, D><<<std::min<uint32_t>(cdiv(n_, BLOCK), 4096), BLOCK, 0, stream_>>>(cur_, n_, simmask_.p,
+                                                                                              pl_);
     timer_.end(PH_G2P, e3, stream_);
     step_count_++;
     steps_since_rebin_++;
@@ -3352,6 +3504,9 @@ class Engine final : public EngineBase {
   uint32_t* gt_cur_ = nullptr;
   uint32_t* gt_other_ = nullptr;
   uint32_t *ml_cur_ = nullptr, *mt_cur_ = nullptr, *ml_next_ = nullptr, *mt_next_ = nullptr;
+  bool plastic_ = false;  // clamped plasticity after every G2P (unfused steps)
+  PlasticDev<T> pl_{};
+  DevBuf<uint8_t> simmask_;
   bool fused_ = false;       // gcur_ holds the P2G of the current state with dt fused_dt_
   bool fused_last_ = false;  // gother_ holds the last step's grid (for mpmx_get_grid)
   T fused_dt_ = 0;

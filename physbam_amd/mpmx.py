@@ -26,6 +26,7 @@ EXPORTS = (
     "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_grid_node_count",
     "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_set_rebin_policy", "mpmx_bin_stats",
     "mpmx_grid_v_upper_bound", "mpmx_compute_dt", "mpmx_advance_frames",
+    "mpmx_set_plasticity", "mpmx_get_plastic_state",
     "mpmx_tape_begin", "mpmx_backward",
     "mpmx_enable_timing", "mpmx_reset_timing",
     "mpmx_phase_time", "mpmx_stress", "mpmx_svd", "mpmx_sort_keys", "mpmx_seed_particles",
@@ -94,6 +95,8 @@ def lib():
         L.mpmx_grid_v_upper_bound.argtypes = [vp, vp]
         L.mpmx_compute_dt.argtypes = [vp, dbl, dbl, vp]
         L.mpmx_advance_frames.argtypes = [vp, c_int, dbl, dbl, dbl, dbl, vp, i64, vp]
+        L.mpmx_set_plasticity.argtypes = [vp, c_int, dbl, dbl, dbl, dbl]
+        L.mpmx_get_plastic_state.argtypes = [vp, vp, vp, vp]
         L.mpmx_tape_begin.argtypes = [vp, c_int]
         L.mpmx_backward.argtypes = [vp] + [vp] * 10
         L.mpmx_enable_timing.argtypes = [vp, c_int]
@@ -292,6 +295,20 @@ class Context:
         _check(lib().mpmx_advance_frames(self._h, int(nframes), float(frame_dt), float(cfl), float(min_dt),
                                          float(max_dt), _p(dts), ctypes.c_int64(capacity), _p(cnt)))
         return dts[:min(int(cnt[0]), capacity)].copy()
+
+    def set_plasticity(self, theta_c, theta_s, max_hardening, hardening_factor, enable=True):
+        """Clamped plasticity with hardening after every G2P (MPM_PLASTICITY_CLAMP)."""
+        _check(lib().mpmx_set_plasticity(self._h, 1 if enable else 0, float(theta_c), float(theta_s),
+                                         float(max_hardening), float(hardening_factor)))
+
+    def get_plastic_state(self):
+        """dict Fp (n, dim*dim), mu, lambda of the particles (user order)."""
+        d, n = self.dim, self.n
+        Fp = np.zeros((n, d * d), self.dtype)
+        mu = np.zeros(n, self.dtype)
+        la = np.zeros(n, self.dtype)
+        _check(lib().mpmx_get_plastic_state(self._h, _p(Fp), _p(mu), _p(la)))
+        return {"Fp": Fp, "mu": mu, "lambda": la}
 
     def tape_begin(self, checkpoint_every=20):
         _check(lib().mpmx_tape_begin(self._h, int(checkpoint_every)))

@@ -3,7 +3,7 @@
 Runs the reference harness (oracle/_ref/ref_harness, built from the reference
 checkout by oracle/ref/Makefile) on small seeded scenes and component inputs,
 and stores inputs and outputs as compressed .npz fixtures.  Run in the
-authoring container:  python tests/golden/make_golden.py [steps] [components] [frames]
+authoring container:  python tests/golden/make_golden.py [steps] [components] [frames] [plastic]
 (no argument: every group).
 """
 import json
@@ -85,12 +85,51 @@ def write_frames():
     np.savez_compressed(os.path.join(HERE, "frames.npz"), **out)
 
 
+def plastic_cases():
+    """Scenes with MPM_PLASTICITY_CLAMP (Add_Clamped_Plasticity); F is perturbed
+    so that singular values leave [1 - theta_c, 1 + theta_s] from the first step."""
+    sq = lambda X: (np.abs(X[:, 0] - 0.5) <= 0.2) & (np.abs(X[:, 1] - 0.45) <= 0.2)
+    sph = lambda X: ((X - np.array([0.5, 0.45, 0.5])) ** 2).sum(1) < 0.3 ** 2
+    cases = {}
+    walls = mt.wall_boxes(2, [32, 32], [0, 0], [1, 1], flags=15, inset=0.1, type_=mt.STICK, friction=0.0)
+    sc = mt.make_scene(2, 32, prec=8, model=mt.COROTATED, shape=sq, v0=[0.5, -3.0], vrand=0.5, frand=0.08,
+                       walls=walls, dt=2e-4, seed=41)
+    sc["plasticity"] = dict(theta_c=0.025, theta_s=0.0075, max_hardening=5.0, hardening_factor=10.0)
+    cases["snow2d_f64"] = (sc, (1, 10))
+    walls = mt.wall_boxes(3, [16] * 3, [0] * 3, [1] * 3, flags=63, inset=0.12, type_=mt.SLIP, friction=0.0)
+    sc = mt.make_scene(3, 16, prec=4, model=mt.COROTATED, shape=sph, v0=[0.3, -3.0, 0.2], vrand=1.0, brand=1e-3,
+                       frand=0.08, walls=walls, dt=5e-4, per_particle_E=True, seed=42)
+    sc["plasticity"] = dict(theta_c=0.02, theta_s=0.01, max_hardening=3.0, hardening_factor=10.0)
+    cases["snow3d_f32"] = (sc, (1, 8))
+    return cases
+
+
+def write_plastic():
+    out = {}
+    for name, (sc, dumps) in plastic_cases().items():
+        for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+            out[f"{name}_in_{k}"] = np.asarray(sc[k])
+        meta = {k: sc[k] for k in ("dim", "prec", "model", "N", "lo", "hi", "gravity", "dt", "neo_threshold",
+                                   "plasticity")}
+        out[f"{name}_meta"] = np.array(json.dumps(meta, default=float))
+        out[f"{name}_walls"] = np.array(json.dumps(sc["walls"], default=float))
+        for n in dumps:
+            r = scene_io.run_ref(sc, n)[-1]
+            for k in ("X", "V", "F", "B", "valid", "Fp", "mu", "lambda"):
+                out[f"{name}_s{n}_{k}"] = r[k]
+            moved = (np.abs(r["Fp"] - np.tile(np.eye(sc["dim"]).reshape(-1), (len(r["Fp"]), 1))).max(1) > 0).sum()
+            print("plastic", name, "steps", n, "particles with Fp != I:", moved, "of", len(r["Fp"]))
+    np.savez_compressed(os.path.join(HERE, "plastic.npz"), **out)
+
+
 def main():
     if not scene_io.ref_available():
         sys.exit("reference harness missing: build it with oracle/ref/Makefile")
-    groups = set(sys.argv[1:]) or {"steps", "components", "frames"}
+    groups = set(sys.argv[1:]) or {"steps", "components", "frames", "plastic"}
     if "frames" in groups:
         write_frames()
+    if "plastic" in groups:
+        write_plastic()
     if "steps" in groups:
         write_steps()
     if "components" in groups:

@@ -29,15 +29,18 @@ def write_scene(path, sc, nsteps, dump_every, threads=1):
     hi = list(sc["hi"]) + [0.0] * (3 - d)
     g = list(sc["gravity"]) + [0.0] * (3 - d)
     walls = sc.get("walls", [])
+    pl = sc.get("plasticity")
     with open(path, "wb") as f:
         f.write(b"MPMSCN1\0")
         f.write(struct.pack("<8i", d, sc["prec"], sc["model"], nsteps, dump_every,
-                            len(walls), threads, 0))
+                            len(walls), threads, 1 if pl else 0))
         f.write(struct.pack("<3i", *N))
         f.write(struct.pack("<3d", *lo))
         f.write(struct.pack("<3d", *hi))
         f.write(struct.pack("<3d", *g))
         f.write(struct.pack("<2d", sc["dt"], sc.get("neo_threshold", 0.25)))
+        if pl:
+            f.write(struct.pack("<4d", pl["theta_c"], pl["theta_s"], pl["max_hardening"], pl["hardening_factor"]))
         f.write(struct.pack("<q", n))
         for w in walls:
             wl = list(w["lo"]) + [0.0] * (3 - d)
@@ -51,7 +54,7 @@ def write_scene(path, sc, nsteps, dump_every, threads=1):
         f.write(np.ascontiguousarray(sc["valid"], dtype=np.uint8).tobytes())
 
 
-def read_output(path, dim):
+def read_output(path, dim, plastic=False):
     with open(path, "rb") as f:
         buf = f.read()
     assert buf[:7] == b"MPMOUT1"
@@ -79,6 +82,10 @@ def read_output(path, dim):
         r["grid_velocity"] = take(nn * d, np.float64).reshape(nn, d)
         r["grid_velocity_save"] = take(nn * d, np.float64).reshape(nn, d)
         r["grid_velocity_friction_save"] = take(nn * d, np.float64).reshape(nn, d)
+        if plastic:
+            r["Fp"] = take(n * d * d, np.float64).reshape(n, d * d)
+            r["mu"] = take(n, np.float64)
+            r["lambda"] = take(n, np.float64)
         dumps.append(r)
     return dumps
 
@@ -92,7 +99,7 @@ def run_ref(sc, nsteps, dump_every=None, threads=1):
         op = os.path.join(td, "out.bin")
         write_scene(sp, sc, nsteps, dump_every, threads)
         subprocess.run([REF_HARNESS, "step", sp, op], check=True)
-        return read_output(op, sc["dim"])
+        return read_output(op, sc["dim"], plastic=bool(sc.get("plasticity")))
 
 
 def run_ref_frames(sc, nframes, frame_dt, cfl, min_dt, max_dt, threads=1):

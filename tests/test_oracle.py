@@ -292,3 +292,57 @@ def test_grid_v_upper_bound_closed_form():
         sc["valid"] = sc["valid"].copy()
         sc["valid"][j] = 0
         assert mt.oracle_grid_v_bound(sc) < got
+
+
+@pytest.mark.parametrize("name", ["snow2d_f64", "snow3d_f32"])
+def test_oracle_plasticity_matches_reference(name):
+    """MPM_PLASTICITY_CLAMP after every step (Fp, hardened mu and lambda, the
+    projected F): bit-exact with the reference's own run."""
+    z = load("plastic.npz")
+    meta = json.loads(str(z[f"{name}_meta"]))
+    sc = dict(meta)
+    sc["walls"] = json.loads(str(z[f"{name}_walls"]))
+    for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+        sc[k] = z[f"{name}_in_{k}"]
+    steps = sorted({int(k.split("_s")[1].split("_")[0]) for k in z.files if k.startswith(name + "_s")})
+    assert steps
+    for n in steps:
+        out = mt.oracle_run_plastic(sc, n)
+        for k in ("X", "V", "F", "B", "valid", "Fp", "mu", "lambda"):
+            got = out[k].astype(np.float64) if k != "valid" else out[k]
+            assert np.array_equal(got, z[f"{name}_s{n}_{k}"]), (name, n, k)
+    # the fixture clamps: Fp moved away from I and the Lame parameters hardened or softened
+    assert (np.abs(out["mu"] - sc["mu"]) > 0).mean() > 0.5
+
+
+def test_plasticity_leaves_elastic_range_alone():
+    """Singular values inside [1 - theta_c, 1 + theta_s]: no change to F, Fp,
+    mu or lambda (the clamp is the identity there)."""
+    sc = mt.make_scene(2, 16, prec=8, shape=lambda X: np.abs(X[:, 0] - 0.5) < 0.2, gravity=[0.0, 0.0], dt=1e-5,
+                       frand=1e-4)
+    sc["plasticity"] = dict(theta_c=0.1, theta_s=0.1, max_hardening=5.0, hardening_factor=10.0)
+    out = mt.oracle_run_plastic(sc, 1)
+    ref = mt.oracle_run(sc, 1, want_grid=False)
+    assert np.array_equal(out["F"], ref["F"])
+    assert np.array_equal(out["Fp"], np.tile(np.eye(2).reshape(-1), (sc["X"].shape[0], 1)))
+    assert np.array_equal(out["mu"], sc["mu"])
+
+
+def test_plasticity_projection_closed_form():
+    """After the update, the singular values of F lie in the clamp range,
+    F Fp is the elastic-plastic total (F_before Fp_before), and the hardening
+    factor is exp(xi (1 - det Fp)) clipped at exp(max_hardening)."""
+    sc = mt.make_scene(3, 8, prec=8, shape=lambda X: np.abs(X[:, 0] - 0.5) < 0.3, gravity=[0.0, 0.0, 0.0],
+                       dt=1e-6, frand=0.3, seed=5)
+    pl = dict(theta_c=0.05, theta_s=0.02, max_hardening=0.5, hardening_factor=10.0)
+    sc["plasticity"] = pl
+    before = mt.oracle_run(sc, 1, want_grid=False)   # the elastic step alone
+    out = mt.oracle_run_plastic(sc, 1)
+    F = out["F"].reshape(-1, 3, 3).transpose(0, 2, 1)
+    Fp = out["Fp"].reshape(-1, 3, 3).transpose(0, 2, 1)
+    Fe0 = before["F"].reshape(-1, 3, 3).transpose(0, 2, 1)
+    s = np.linalg.svd(F, compute_uv=False)
+    assert (s >= 1 - pl["theta_c"] - 1e-12).all() and (s <= 1 + pl["theta_s"] + 1e-12).all()
+    assert np.abs(F @ Fp - Fe0).max() < 1e-12
+    h = np.exp(np.minimum(pl["max_hardening"], pl["hardening_factor"] * (1 - np.linalg.det(Fp))))
+    assert np.allclose(out["mu"], sc["mu"] * h, rtol=1e-12)
