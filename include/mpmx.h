@@ -44,6 +44,7 @@ extern "C" {
 /* Constitutive models (Deformables/Constitutive_Models). */
 #define MPMX_FIXED_COROTATED 0 /* COROTATED_FIXED.cpp:59-65 */
 #define MPMX_NEO_HOOKEAN 1     /* NEO_HOOKEAN.cpp:44-81 */
+#define MPMX_HENCKY 2          /* ST_VENANT_KIRCHHOFF_HENCKY_STRAIN.cpp:60-70 */
 
 /* Collision types, MPM_COLLISION_OBJECT<TV>::COLLISION_TYPE
    (Hybrid_Methods/Collisions/MPM_COLLISION_OBJECT.h:19). */
@@ -105,7 +106,8 @@ int64_t mpmx_particle_count(const mpmx_ctx* ctx);
 
 /* Elastic force: one MPM_FINITE_ELEMENTS with the given model over all
    particles (STANDARD_TESTS_BASE::Add_Fixed_Corotated / Add_Neo_Hookean,
-   STANDARD_TESTS_BASE.cpp:372-392).  neo_threshold is NEO_HOOKEAN's
+   STANDARD_TESTS_BASE.cpp:372-392; MPMX_HENCKY is the St. Venant-Kirchhoff
+   Hencky-strain model that Add_Drucker_Prager uses, :409-432).  neo_threshold is NEO_HOOKEAN's
    failure_threshold (default 0.25, NEO_HOOKEAN.h:33). */
 int mpmx_set_constitutive(mpmx_ctx* ctx, int model, double neo_threshold);
 
@@ -205,6 +207,25 @@ int mpmx_advance_frames(mpmx_ctx* ctx, int nframes, double frame_dt, double cfl,
 int mpmx_set_plasticity(mpmx_ctx* ctx, int enable, double theta_c, double theta_s, double max_hardening,
                         double hardening_factor);
 int mpmx_get_plastic_state(mpmx_ctx* ctx, void* Fp, void* mu, void* lambda);
+
+/* Plasticity: MPM_DRUCKER_PRAGER (sand; Hybrid_Methods/Forces/
+   MPM_DRUCKER_PRAGER.cpp:40-131, as STANDARD_TESTS_BASE.cpp:407-431
+   Add_Drucker_Prager adds it).  After every G2P the log singular values of
+   each simulated particle's F are returned onto the Drucker-Prager cone whose
+   friction angle a0 + (a1 q - a4) exp(-a3 q) degrees hardens with the
+   accumulated plastic deformation q; `cohesion` is sigma_Y, and
+   volume_correction keeps the volume lost in tension (the reference's vc,
+   on unless -no_vc).  Used with MPMX_HENCKY (St. Venant-Kirchhoff with Hencky
+   strain), the model the reference pairs it with.  Enabling sets Fp = I,
+   q = 0, vc = 0 and sigma_Y = cohesion for every particle (and so does each
+   later mpmx_set_particles); it replaces clamped plasticity, and enable = 0
+   turns it off.  mpmx_get_plastic_state returns Fp; mpmx_get_drucker_prager_
+   state returns q, rho_F (the cone's slope), sigma_Y and vc per particle
+   (NULL skips).  Steps with it take the unfused kernels; not
+   differentiable. */
+int mpmx_set_drucker_prager(mpmx_ctx* ctx, int enable, double a0, double a1, double a3, double a4, double cohesion,
+                            int volume_correction);
+int mpmx_get_drucker_prager_state(mpmx_ctx* ctx, void* plastic_def, void* rho_F, void* cohesion, void* vc);
 
 /* Differentiation (no counterpart in the reference, whose step has no
    adjoint).  mpmx_tape_begin records the current particle state and then a

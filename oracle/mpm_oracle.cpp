@@ -24,6 +24,8 @@
 //   invalidation MPM_DRIVER.cpp:1219-1239, simulated list :1142-1157
 //   reductions   Examples_And_Drivers/MPM_EXAMPLE.cpp:209-315
 //   plasticity   Forces/MPM_PLASTICITY_CLAMP.cpp:31-45 (Compute with hardening),
+//                Forces/MPM_DRUCKER_PRAGER.cpp:64-94 (Compute), :123-131 (Update_Hardening),
+//                :40-58 (Initialize_Particles); Hencky stress ST_VENANT_KIRCHHOFF_HENCKY_STRAIN.cpp:60-70
 //                Forces/MPM_PLASTICITY_MODEL.cpp:34-48 (Update_Particles),
 //                MPM_DRIVER.cpp:156, :920-926 (after G2P, over the step's simulated particles)
 //   adaptive dt  MPM_DRIVER.cpp:1098-1104 (Compute_Dt), :1125-1138 (Grid_V_Upper_Bound),
@@ -319,6 +321,19 @@ template<class T> Vec<T,3> Clamp_To_Hyperbola(const Vec<T,3>& F,T thr,T root)
 // pow<1,d>(t) of the constructor: the d-th root of the threshold.
 template<class T> T Dth_Root(T t,int d){return d==2?std::sqrt(t):std::cbrt(t);}
 // NEO_HOOKEAN.cpp:73-81
+// ST_VENANT_KIRCHHOFF_HENCKY_STRAIN::P_From_Strain (ST_VENANT_KIRCHHOFF_HENCKY_STRAIN.cpp:60-70):
+// ((2 mu) ln s + lambda tr(ln s)) (1/s), zero if the last singular value is not positive.
+template<class T,int d> Vec<T,d> P_Hencky(const Vec<T,d>& s,T mu,T la)
+{
+    Vec<T,d> P;
+    if(s[d-1]<=0){for(int i=0;i<d;i++) P[i]=0;return P;}
+    Vec<T,d> l;for(int i=0;i<d;i++) l[i]=std::log(s[i]);
+    T tr=l[0];for(int i=1;i<d;i++) tr+=l[i];
+    T c=la*tr,m2=(T)2*mu;
+    for(int i=0;i<d;i++) P[i]=(m2*l[i]+c)*((T)1/s[i]);
+    return P;
+}
+
 template<class T,int d> Vec<T,d> P_Neo_Hookean(const Vec<T,d>& s,T mu,T la,T thr)
 {
     T J=Product(s);
@@ -469,7 +484,7 @@ template<class T,int d> Mat<T,d> Stress_PFT(const Scene<T,d>& s,const Mat<T,d>& 
     Mat<T,d> U,V;Vec<T,d> sig;
     Svd(F,U,sig,V);
     Mat<T,d> FV=Fn*V;
-    Vec<T,d> P=s.model==0?P_Corotated(sig,mu,la):P_Neo_Hookean(sig,mu,la,s.neo_threshold);
+    Vec<T,d> P=s.model==0?P_Corotated(sig,mu,la):(s.model==2?P_Hencky(sig,mu,la):P_Neo_Hookean(sig,mu,la,s.neo_threshold));
     Mat<T,d> DT;  // DIAGONAL_MATRIX::Times_Transpose: r(i,j)=P(i)*FV(j,i)
     for(int i=0;i<d;i++) for(int j=0;j<d;j++) DT(i,j)=P[i]*FV(j,i);
     return U*DT;
@@ -704,6 +719,96 @@ template<class T,int d> void Plasticity_Update(const Plastic<T,d>& pl,Particles<
         for(int k=0;k<d*d;k++){P.F[p*d*d+k]=Fnew.x[k];pl.Fp[p*d*d+k]=Fpn.x[k];}}
 }
 
+// Drucker-Prager (MPM_DRUCKER_PRAGER, sand): per particle plastic_def, rho_F
+// (friction), sigma_Y (cohesion) and vc (volume correction).
+template<class T,int d> struct DruckerPrager
+{
+    T a0,a1,a3,a4;bool use_vc;
+    T* Fp;T* plastic_def;T* rho_F;T* sigma_Y;T* vc;
+    void Update_Hardening(long long id,T inc) const
+    {
+        plastic_def[id]+=inc;
+        T phi_F=M_PI/180*(a0+(a1*plastic_def[id]-a4)*std::exp(-a3*plastic_def[id]));
+        T sin_phi_F=std::sin(phi_F);
+        rho_F[id]=std::sqrt(2.0/3.0)*2*sin_phi_F/(3-sin_phi_F);
+    }
+    // MPM_DRUCKER_PRAGER::Compute with store_hardening
+    bool Compute(Vec<T,d>& strain,const Vec<T,d>& Fe,long long id,T mu,T lambda) const
+    {
+        T g=1/(d*lambda+2*mu);
+        T b=1/(2*mu);
+        T beta=-sigma_Y[id]*g/(rho_F[id]*d);
+        Vec<T,d> log_Fe;for(int a=0;a<d;a++) log_Fe[a]=std::log(std::abs(Fe[a]));
+        Vec<T,d> strain_trial;
+        T vcd=vc[id]/d;
+        for(int a=0;a<d;a++) strain_trial[a]=(log_Fe[a]+beta)+vcd;
+        T k=strain_trial[0];for(int a=1;a<d;a++) k+=strain_trial[a];
+        Vec<T,d> sh;T kd=k/d;
+        for(int a=0;a<d;a++) sh[a]=strain_trial[a]-kd;
+        T q=magnitude(sh);
+        T r=rho_F[id]*k/g;
+        T dg=q+b*r;
+        if(dg<=0) return false;
+        if(q==0 || k>0){
+            for(int a=0;a<d;a++) strain[a]=sigma_Y[id]?std::exp(-beta)*(T)1:(T)1;
+            if(use_vc){T ls=log_Fe[0];for(int a=1;a<d;a++) ls+=log_Fe[a];vc[id]+=ls+beta*d;}
+            Update_Hardening(id,magnitude(log_Fe));
+            return true;}
+        T p=r*b/q;
+        for(int a=0;a<d;a++) strain[a]=std::exp((kd-p*sh[a])-beta);
+        vc[id]=0;
+        Update_Hardening(id,dg);
+        return true;
+    }
+};
+
+// MPM_PLASTICITY_MODEL::Update_Particles with the Drucker-Prager Compute.
+template<class T,int d> void Drucker_Prager_Update(const DruckerPrager<T,d>& dp,Particles<T,d>& P,
+    const std::vector<unsigned char>& simulated)
+{
+    for(long long p=0;p<P.n;p++){
+        if(!simulated[p]) continue;
+        Mat<T,d> Fe=P.f(p),U,V;
+        Vec<T,d> sv;
+        Svd(Fe,U,sv,V);
+        Vec<T,d> strain;
+        if(!dp.Compute(strain,sv,p,P.mu[p],P.lambda[p])) continue;
+        Mat<T,d> Fp;for(int k=0;k<d*d;k++) Fp.x[k]=dp.Fp[p*d*d+k];
+        Mat<T,d> US;
+        for(int j=0;j<d;j++) for(int i=0;i<d;i++) US(i,j)=U(i,j)*strain[j];
+        Mat<T,d> Fnew=Times_Transpose(US,V);
+        Mat<T,d> DU;
+        for(int i=0;i<d;i++) for(int j=0;j<d;j++) DU(i,j)=((T)1/strain[i])*U(j,i);
+        Mat<T,d> Fpn=((V*DU)*Fe)*Fp;
+        for(int k=0;k<d*d;k++){P.F[p*d*d+k]=Fnew.x[k];dp.Fp[p*d*d+k]=Fpn.x[k];}}
+}
+
+// state[4*np]: plastic_def, rho_F, sigma_Y, vc (initialized here from params
+// a0, a1, a3, a4, sigma_Y0, use_vc as MPM_DRUCKER_PRAGER::Initialize_Particles).
+template<class T,int d> int Run_Drucker_Prager(const int* N,const double* lo,const double* hi,const double* g,int model,
+    double thr,double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi,
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,T* mu,T* la,unsigned char* valid,int nsteps,
+    const double* params,T* Fp,T* state)
+{
+    Scene<T,d> s=Make_Scene<T,d>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi);
+    DruckerPrager<T,d> dp{(T)params[0],(T)params[1],(T)params[2],(T)params[3],params[5]!=0,Fp,state,state+np,
+        state+2*np,state+3*np};
+    for(long long p=0;p<np;p++){
+        dp.plastic_def[p]=0;dp.vc[p]=0;
+        dp.sigma_Y[p]=(T)params[4];
+        dp.Update_Hardening(p,0);}
+    Particles<T,d> P{np,X,V,F,B,mass,volume,mu,la,valid};
+    Grid<T,d> G;
+    T time=0;
+    std::vector<unsigned char> simulated(np);
+    for(int k=0;k<nsteps;k++){
+        for(long long p=0;p<np;p++) simulated[p]=valid[p];
+        Step(s,P,G,time);
+        Drucker_Prager_Update(dp,P,simulated);
+        time+=s.dt;}
+    return 0;
+}
+
 template<class T,int d> int Run_Plastic(const int* N,const double* lo,const double* hi,const double* g,int model,double thr,
     double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi,
     long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,T* mu,T* la,unsigned char* valid,int nsteps,
@@ -849,6 +954,22 @@ ORACLE_API int oracle_run_plastic_##SUF(int dim,const int* N,const double* lo,co
 DEFINE_PLASTIC(f64,double)
 DEFINE_PLASTIC(f32,float)
 
+// params[6] = a0, a1, a3, a4, sigma_Y0, use_vc; state[4*np] out.
+#define DEFINE_DP(SUF,T) \
+ORACLE_API int oracle_run_drucker_prager_##SUF(int dim,const int* N,const double* lo,const double* hi,const double* g, \
+    int model,double thr,double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi, \
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,T* mu,T* la,unsigned char* valid,int nsteps, \
+    const double* params,T* Fp,T* state) \
+{ \
+    if(dim==2) return Run_Drucker_Prager<T,2>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume, \
+        mu,la,valid,nsteps,params,Fp,state); \
+    if(dim==3) return Run_Drucker_Prager<T,3>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume, \
+        mu,la,valid,nsteps,params,Fp,state); \
+    return -1; \
+}
+DEFINE_DP(f64,double)
+DEFINE_DP(f32,float)
+
 template<class T,int d> static void Svd_Many(long long n,const T* A,T* U,T* S,T* V,int* iters)
 {
     for(long long i=0;i<n;i++){
@@ -873,7 +994,7 @@ template<class T,int d> static void Pstrain_Many(int model,T thr,long long n,con
 {
     for(long long i=0;i<n;i++){
         Vec<T,d> s;for(int k=0;k<d;k++) s[k]=S[i*d+k];
-        Vec<T,d> r=model==0?P_Corotated(s,mu[i],la[i]):P_Neo_Hookean(s,mu[i],la[i],thr);
+        Vec<T,d> r=model==0?P_Corotated(s,mu[i],la[i]):(model==2?P_Hencky(s,mu[i],la[i]):P_Neo_Hookean(s,mu[i],la[i],thr));
         for(int k=0;k<d;k++) P[i*d+k]=r[k];}
 }
 #define DEFINE_PSTRAIN(SUF,T) \

@@ -11,8 +11,10 @@
 //   ref_harness svd   <prec> <dim> <in.bin> <out.bin>
 //   ref_harness pstrain <prec> <dim> <model> <threshold> <in.bin> <out.bin>
 //   ref_harness frames <scene.bin> <nframes> <frame_dt> <cfl> <min_dt> <max_dt> <out.bin>
-// A scene with the plasticity flag set adds MPM_PLASTICITY_CLAMP (Fp, mu0,
-// lambda0 stored) and its dumps end with Fp, mu and lambda.
+// A scene with plasticity flag 1 adds MPM_PLASTICITY_CLAMP (Fp, mu0, lambda0
+// stored), flag 2 MPM_DRUCKER_PRAGER; their dumps end with Fp, mu and lambda
+// (and for flag 2 plastic_def, rho_F, sigma_Y, vc).  Model 2 is the Hencky
+// strain model.
 //   ref_harness weights <prec> <dim> <N> <lo> <hi> <in.bin> <out.bin>
 //
 // Scene file layout (little endian) is written by tests/golden/scene_io.py.
@@ -31,6 +33,8 @@
 #include <Hybrid_Methods/Forces/MPM_FINITE_ELEMENTS.h>
 #include <Hybrid_Methods/Forces/MPM_GRAVITY.h>
 #include <Hybrid_Methods/Forces/MPM_PLASTICITY_CLAMP.h>
+#include <Hybrid_Methods/Forces/MPM_DRUCKER_PRAGER.h>
+#include <Deformables/Constitutive_Models/ST_VENANT_KIRCHHOFF_HENCKY_STRAIN.h>
 #include <Hybrid_Methods/Forces/MPM_FORCE_HELPER.h>
 #include <Hybrid_Methods/Iterators/GATHER_SCATTER.h>
 #include <Hybrid_Methods/Iterators/PARTICLE_GRID_WEIGHTS_SPLINE.h>
@@ -70,7 +74,8 @@ struct SCENE
     int dim,prec,model,nsteps,dump_every,nwalls,threads,plastic;
     int N[3];
     double lo[3],hi[3],gravity[3],dt,neo_threshold;
-    double theta_c,theta_s,max_hardening,hardening_factor;  // clamped plasticity (plastic != 0)
+    double theta_c,theta_s,max_hardening,hardening_factor;  // clamped plasticity (plastic == 1)
+    double dp[6];  // Drucker-Prager (plastic == 2): a0, a1, a3, a4, sigma_Y0, use_vc
     long long np;
     std::vector<WALL> walls;
     std::vector<double> X,V,F,B,mass,volume,mu,lambda;
@@ -88,8 +93,9 @@ SCENE Read_Scene(const char* path)
     r.get(s.N,3);r.get(s.lo,3);r.get(s.hi,3);r.get(s.gravity,3);
     s.dt=r.get<double>();s.neo_threshold=r.get<double>();
     s.theta_c=s.theta_s=s.max_hardening=s.hardening_factor=0;
-    if(s.plastic){s.theta_c=r.get<double>();s.theta_s=r.get<double>();s.max_hardening=r.get<double>();
+    if(s.plastic==1){s.theta_c=r.get<double>();s.theta_s=r.get<double>();s.max_hardening=r.get<double>();
         s.hardening_factor=r.get<double>();}
+    if(s.plastic==2) r.get(s.dp,6);
     s.np=r.get<long long>();
     for(int i=0;i<s.nwalls;i++){
         WALL w;w.type=r.get<int>();r.get<int>();w.friction=r.get<double>();
@@ -119,6 +125,7 @@ struct RUN
     typedef VECTOR<int,d> TV_INT;
     HARNESS_EXAMPLE<TV> ex;
     MPM_DRIVER<TV>* driver=0;
+    MPM_DRUCKER_PRAGER<TV>* dp=0;
     const SCENE& s;
 
     RUN(const SCENE& s,STREAM_TYPE st):ex(st),s(s)
@@ -143,10 +150,19 @@ struct RUN
             P.mu(p)=(T)s.mu[p];P.lambda(p)=(T)s.lambda[p];P.valid(p)=s.valid[p]!=0;}
         ISOTROPIC_CONSTITUTIVE_MODEL<T,d>* cm=0;
         if(s.model==0) cm=new COROTATED_FIXED<T,d>((T)1e4,(T).3);
+        else if(s.model==2) cm=new ST_VENANT_KIRCHHOFF_HENCKY_STRAIN<T,d>((T)1e4,(T).3);
         else cm=new NEO_HOOKEAN<T,d>((T)1e4,(T).3,(T).05,(T)s.neo_threshold);
         MPM_FINITE_ELEMENTS<TV>* fe=new MPM_FINITE_ELEMENTS<TV>(ex.force_helper,*cm,ex.gather_scatter,0);
         ex.Add_Force(*fe);
-        if(s.plastic){
+        if(s.plastic==2){
+            // STANDARD_TESTS_BASE::Add_Drucker_Prager (explicit plasticity), Fp stored
+            P.Store_Fp(true);
+            for(int p=0;p<(int)s.np;p++) P.Fp(p)=MATRIX<T,d>::Identity_Matrix();
+            dp=new MPM_DRUCKER_PRAGER<TV>(P,&fe->gather_scatter,(T)s.dp[0],(T)s.dp[1],(T)s.dp[2],(T)s.dp[3]);
+            dp->use_vc=s.dp[5]!=0;
+            dp->Initialize_Particles(0,(T)s.dp[4]);
+            ex.plasticity_models.Append(dp);}
+        if(s.plastic==1){
             // STANDARD_TESTS_BASE::Add_Clamped_Plasticity (symplectic, explicit plasticity)
             P.Store_Fp(true);
             for(int p=0;p<(int)s.np;p++){
@@ -225,6 +241,11 @@ struct RUN
             w.put(buf.data(),buf.size());
             for(int p=0;p<n;p++) buf[p]=P.lambda(p);
             w.put(buf.data(),buf.size());}
+        if(s.plastic==2){  // plastic_def, rho_F, sigma_Y, vc
+            const ARRAY_VIEW<T>* a[4]={&dp->plastic_def,&dp->rho_F,&dp->sigma_Y,&dp->vc};
+            for(int k=0;k<4;k++){
+                for(int p=0;p<n;p++) buf[p]=(*a[k])(p);
+                w.put(buf.data(),n);}}
     }
 };
 

@@ -197,6 +197,17 @@ int mpmx_get_plastic_state(mpmx_ctx* ctx, void* Fp, void* mu, void* lambda) {
   return guard([&] { ctx->engine->get_plastic_state(Fp, mu, lambda); });
 }
 
+int mpmx_set_drucker_prager(mpmx_ctx* ctx, int enable, double a0, double a1, double a3, double a4, double cohesion,
+                            int volume_correction) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->set_drucker_prager(enable != 0, a0, a1, a3, a4, cohesion, volume_correction != 0); });
+}
+
+int mpmx_get_drucker_prager_state(mpmx_ctx* ctx, void* plastic_def, void* rho_F, void* cohesion, void* vc) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->get_drucker_prager_state(plastic_def, rho_F, cohesion, vc); });
+}
+
 int mpmx_tape_begin(mpmx_ctx* ctx, int checkpoint_every) {
   if (!ctx) return bad("null context");
   return guard([&] { ctx->engine->tape_begin(checkpoint_every); });

@@ -277,11 +277,30 @@ template<> MPMX_HD void svd<double, 2>(const double* F, double* U, double* s, do
 //     (Deformables/Constitutive_Models/COROTATED_FIXED.cpp:59-65)
 //   neo-Hookean:     mu s - (mu - lambda ln J) / s           for J >= t,
 //     otherwise the hyperbola-clamped extrapolation of NEO_HOOKEAN.cpp:44-81.
+//   St. Venant-Kirchhoff with Hencky strain: (2 mu ln s + lambda tr ln s) / s,
+//     zero when the smallest singular value is not positive
+//     (ST_VENANT_KIRCHHOFF_HENCKY_STRAIN.cpp:60-70).
 template<typename T, int D>
 MPMX_HD void p_hat(int model, T mu, T la, T thr, T root, const T* s, T* P) {
   T J = s[0];
 #pragma unroll
   for (int i = 1; i < D; i++) J *= s[i];
+  if (model == 2) {
+    if (!(s[D - 1] > (T)0)) {
+#pragma unroll
+      for (int i = 0; i < D; i++) P[i] = 0;
+      return;
+    }
+    T l[D], tr = 0;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      l[i] = tlog<T>(s[i]);
+      tr += l[i];
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++) P[i] = ((T)2 * mu * l[i] + la * tr) * ((T)1 / s[i]);
+    return;
+  }
   if (model == 0) {
     T cof[D];
     if constexpr (D == 3) { cof[0] = s[1] * s[2]; cof[1] = s[0] * s[2]; cof[2] = s[0] * s[1]; }
@@ -438,9 +457,16 @@ template<> struct SeriesRadius<double> { static constexpr double r2 = 1.0 / 6553
 // Every partial product is a polynomial in E, so it commutes with E and is
 // symmetric: only the upper triangle is formed.
 template<typename T, int D>
-__device__ __forceinline__ void corot_series(const T (&E)[D][D], T (&f)[D][D]) {
-  const T c[8] = {(T)0, (T)0.5, (T)0.125, (T)-0.0625, (T)(5.0 / 128), (T)(-7.0 / 256), (T)(21.0 / 1024),
-                  (T)(-33.0 / 2048)};
+__device__ __forceinline__ void corot_series(const T (&E)[D][D], T (&f)[D][D], bool hencky = false) {
+  // hencky: ln(I + E) = E - E^2/2 + E^3/3 - ... + E^7/7 instead
+  const T c[8] = {(T)0,
+                  hencky ? (T)1 : (T)0.5,
+                  hencky ? (T)-0.5 : (T)0.125,
+                  hencky ? (T)(1.0 / 3) : (T)-0.0625,
+                  hencky ? (T)-0.25 : (T)(5.0 / 128),
+                  hencky ? (T)0.2 : (T)(-7.0 / 256),
+                  hencky ? (T)(-1.0 / 6) : (T)(21.0 / 1024),
+                  hencky ? (T)(1.0 / 7) : (T)(-33.0 / 2048)};
   T P[D][D];
 #pragma unroll
   for (int i = 0; i < D; i++)
@@ -513,16 +539,21 @@ __device__ __forceinline__ bool kirchhoff_fast(int model, T mu, T la, T thr, T v
     }
     return false;
   }
+  // fixed corotated (model 0) and Hencky (model 2) share the two branches:
+  //   corotated tau = vol (2 mu ((I+E) - sqrt(I+E)) + lambda (J-1) J I),
+  //   Hencky    tau = vol (mu ln(I+E) + lambda ln J I)  (= U diag(2 mu ln s + lambda tr ln s) U^T)
+  const bool hencky = model == 2;
   if (!slow) {
     T r2 = 0;
 #pragma unroll
     for (int i = 0; i < D; i++)
 #pragma unroll
       for (int j = 0; j < D; j++) r2 += E[i][j] * E[i][j];
-    if (r2 <= SeriesRadius<T>::r2) {
+    // the log series converges more slowly: a quarter of the radius^2
+    if (r2 <= (hencky ? SeriesRadius<T>::r2 * (T)0.25 : SeriesRadius<T>::r2)) {
       T f[D][D];
-      corot_series<T, D>(E, f);
-      const T pj = la * Jm1 * J, m2 = (T)2 * mu;
+      corot_series<T, D>(E, f, hencky);
+      const T pj = hencky ? la * tlog1p<T>(Jm1) : la * Jm1 * J, m2 = hencky ? mu : (T)2 * mu;
 #pragma unroll
       for (int i = 0; i < D; i++)
 #pragma unroll
@@ -540,14 +571,18 @@ __device__ __forceinline__ bool kirchhoff_fast(int model, T mu, T la, T thr, T v
     if ((T)1 + emin < (T)1e-4 * ((T)1 + emax)) {
       slow = true;
     } else {
-      const T pj = la * Jm1 * J;
+      const T pj = hencky ? la * tlog1p<T>(Jm1) : la * Jm1 * J;
       T k[D];
 #pragma unroll
       for (int i = 0; i < D; i++) {
         const T e = E[i][i];
-        const T s = tsqrt<T>((T)1 + e);
-        const T sm1 = e / ((T)1 + s);
-        k[i] = vol * ((T)2 * mu * s * sm1 + pj);
+        if (hencky) {
+          k[i] = vol * (mu * tlog1p<T>(e) + pj);
+        } else {
+          const T s = tsqrt<T>((T)1 + e);
+          const T sm1 = e / ((T)1 + s);
+          k[i] = vol * ((T)2 * mu * s * sm1 + pj);
+        }
       }
 #pragma unroll
       for (int i = 0; i < D; i++)

@@ -2023,6 +2023,7 @@ template<typename T, int D> struct ParticleBufs {
   uint32_t* pid;
   uint16_t* pscene;  // scene of each particle (batches only)
   T* Fp; T* mu0; T* la0;  // plastic deformation, initial Lame parameters (plasticity only)
+  T* pdef; T* rhoF; T* coh; T* vc;  // Drucker-Prager: plastic_def, rho_F, cohesion, volume correction
 };
 
 template<typename T, int D>
@@ -2053,19 +2054,38 @@ __global__ __launch_bounds__(BLOCK) void reorder_kernel(ParticleBufs<T, D> src, 
     for (int k = 0; k < D * D; k++) dst.Fp[k * n + i] = src.Fp[k * n + p];
     dst.mu0[i] = src.mu0[p];
     dst.la0[i] = src.la0[p];
+    dst.pdef[i] = src.pdef[p];
+    dst.rhoF[i] = src.rhoF[p];
+    dst.coh[i] = src.coh[p];
+    dst.vc[i] = src.vc[p];
   }
 }
 
 // ---------------------------------------------------------- plasticity ----
-// MPM_PLASTICITY_CLAMP with hardening (Forces/MPM_PLASTICITY_CLAMP.cpp:31-45,
-// MPM_PLASTICITY_MODEL.cpp:34-48), after G2P, for the particles simulated in
-// the step (valid at its start: `simulated`).  In FP64: F = U S V^T; the
-// singular values clamped to [1 - theta_c, 1 + theta_s]; if any changed,
-//   h = max(1e-8, exp(min(max_hardening, xi (1 - prod(S)/prod(Sc) det Fp)))),
-//   mu = mu0 h, lambda = lambda0 h,  F = U Sc V^T,  Fp = V Sc^-1 U^T F Fp.
+// Plasticity after G2P (MPM_PLASTICITY_MODEL::Update_Particles,
+// MPM_PLASTICITY_MODEL.cpp:34-48) for the particles simulated in the step
+// (valid at its start: `simulated`), in FP64: F = U S V^T, the model maps S
+// to Sc, and if it changed, F = U Sc V^T and Fp = V Sc^-1 U^T F Fp.
+//   clamp (MPM_PLASTICITY_CLAMP.cpp:31-45): Sc = clamp(S, 1 - theta_c,
+//     1 + theta_s); h = max(1e-8, exp(min(max_hardening, xi (1 - prod(S) /
+//     prod(Sc) det Fp)))), mu = mu0 h, lambda = lambda0 h;
+//   Drucker-Prager (MPM_DRUCKER_PRAGER.cpp:64-94): return mapping of ln S
+//     onto the cone, with hardening of the friction angle.
 template<typename T> struct PlasticDev {
-  double lo, hi, max_hardening, xi;
+  int kind;                           // 1 clamp, 2 Drucker-Prager
+  double lo, hi, max_hardening, xi;   // clamp
+  double a0, a1, a3, a4, cohesion;    // Drucker-Prager (friction angle in degrees)
+  int use_vc;
 };
+
+// MPM_DRUCKER_PRAGER::Update_Hardening (MPM_DRUCKER_PRAGER.cpp:123-131): the
+// friction angle phi = a0 + (a1 q - a4) exp(-a3 q) degrees at plastic
+// deformation q, rho_F = sqrt(2/3) 2 sin(phi) / (3 - sin(phi)).
+template<typename T> __device__ __forceinline__ double dp_rho_f(const PlasticDev<T>& pl, double q) {
+  const double phi = 3.14159265358979323846 / 180 * (pl.a0 + (pl.a1 * q - pl.a4) * exp(-pl.a3 * q));
+  const double sp = sin(phi);
+  return sqrt(2.0 / 3.0) * 2 * sp / (3 - sp);
+}
 
 template<typename T, int D>
 __global__ __launch_bounds__(BLOCK) void plasticity_kernel(ParticleBufs<T, D> b, uint32_t n, const uint8_t* simulated,
@@ -2077,33 +2097,81 @@ __global__ __launch_bounds__(BLOCK) void plasticity_kernel(ParticleBufs<T, D> b,
     for (int k = 0; k < D * D; k++) Fe[k] = (double)b.F[(size_t)k * n + p];
     svd<double, D>(Fe, U, S, V);
     double Sc[D];
-    bool same = true;
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-      Sc[a] = S[a] <= pl.lo ? pl.lo : (S[a] >= pl.hi ? pl.hi : S[a]);
-      same = same && Sc[a] == S[a];
-    }
-    if (same) continue;
     double Fp[D * D];
 #pragma unroll
     for (int k = 0; k < D * D; k++) Fp[k] = (double)b.Fp[(size_t)k * n + p];
-    double ps = 1, pc = 1;
+    if (pl.kind == 1) {
+      bool same = true;
 #pragma unroll
-    for (int a = 0; a < D; a++) {
-      ps *= S[a];
-      pc *= Sc[a];
+      for (int a = 0; a < D; a++) {
+        Sc[a] = S[a] <= pl.lo ? pl.lo : (S[a] >= pl.hi ? pl.hi : S[a]);
+        same = same && Sc[a] == S[a];
+      }
+      if (same) continue;
+      double ps = 1, pc = 1;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        ps *= S[a];
+        pc *= Sc[a];
+      }
+      double detp;
+      if constexpr (D == 3)
+        detp = Fp[0] * (Fp[4] * Fp[8] - Fp[7] * Fp[5]) + Fp[3] * (Fp[7] * Fp[2] - Fp[1] * Fp[8]) +
+               Fp[6] * (Fp[1] * Fp[5] - Fp[4] * Fp[2]);
+      else
+        detp = Fp[0] * Fp[3] - Fp[1] * Fp[2];
+      const double det = ps / pc * detp;
+      double h = exp(fmin(pl.max_hardening, pl.xi * (1 - det)));
+      if (h < 1e-8) h = 1e-8;
+      b.mu[p] = (T)((double)b.mu0[p] * h);
+      b.la[p] = (T)((double)b.la0[p] * h);
+    } else {
+      // MPM_DRUCKER_PRAGER::Compute (MPM_DRUCKER_PRAGER.cpp:64-94): return
+      // mapping of the log strain onto the Drucker-Prager cone
+      const double mu = (double)b.mu[p], la = (double)b.la[p];
+      const double g = 1 / (D * la + 2 * mu), bb = 1 / (2 * mu);
+      const double rho = (double)b.rhoF[p], coh = (double)b.coh[p];
+      double vcp = (double)b.vc[p], pdef = (double)b.pdef[p];
+      const double beta = -coh * g / (rho * D);
+      double lf[D], sh[D], k = 0;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        lf[a] = log(fabs(S[a]));
+        sh[a] = lf[a] + beta + vcp / D;
+        k += sh[a];
+      }
+      double q2 = 0;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        sh[a] -= k / D;
+        q2 += sh[a] * sh[a];
+      }
+      const double q = sqrt(q2), r = rho * k / g, dg = q + bb * r;
+      if (dg <= 0) continue;
+      double inc;
+      if (q == 0 || k > 0) {  // tension: to the tip of the cone
+#pragma unroll
+        for (int a = 0; a < D; a++) Sc[a] = coh != 0 ? exp(-beta) : 1.0;
+        double ls = 0, l2 = 0;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+          ls += lf[a];
+          l2 += lf[a] * lf[a];
+        }
+        if (pl.use_vc) vcp += ls + beta * D;
+        inc = sqrt(l2);
+      } else {
+        const double pp = r * bb / q;
+#pragma unroll
+        for (int a = 0; a < D; a++) Sc[a] = exp(k / D - pp * sh[a] - beta);
+        vcp = 0;
+        inc = dg;
+      }
+      pdef += inc;
+      b.vc[p] = (T)vcp;
+      b.pdef[p] = (T)pdef;
+      b.rhoF[p] = (T)dp_rho_f<T>(pl, pdef);
     }
-    double detp;
-    if constexpr (D == 3)
-      detp = Fp[0] * (Fp[4] * Fp[8] - Fp[7] * Fp[5]) + Fp[3] * (Fp[7] * Fp[2] - Fp[1] * Fp[8]) +
-             Fp[6] * (Fp[1] * Fp[5] - Fp[4] * Fp[2]);
-    else
-      detp = Fp[0] * Fp[3] - Fp[1] * Fp[2];
-    const double det = ps / pc * detp;
-    double h = exp(fmin(pl.max_hardening, pl.xi * (1 - det)));
-    if (h < 1e-8) h = 1e-8;
-    b.mu[p] = (T)((double)b.mu0[p] * h);
-    b.la[p] = (T)((double)b.la0[p] * h);
     // F = U Sc V^T;  M = V Sc^-1 U^T F_e;  Fp = M Fp  (column-major)
     double M[D * D];
 #pragma unroll
@@ -2134,14 +2202,21 @@ __global__ __launch_bounds__(BLOCK) void plasticity_kernel(ParticleBufs<T, D> b,
   }
 }
 
-// Fp = I, mu0 = mu, lambda0 = lambda.
+// Fp = I, mu0 = mu, lambda0 = lambda; Drucker-Prager state as
+// MPM_DRUCKER_PRAGER::Initialize_Particles (MPM_DRUCKER_PRAGER.cpp:40-58):
+// plastic_def = 0, vc = 0, cohesion = sigma_Y0, rho_F at zero deformation.
 template<typename T, int D>
-__global__ __launch_bounds__(BLOCK) void plastic_init_kernel(ParticleBufs<T, D> b, uint32_t n) {
+__global__ __launch_bounds__(BLOCK) void plastic_init_kernel(ParticleBufs<T, D> b, uint32_t n, PlasticDev<T> pl) {
+  const T rho0 = (T)dp_rho_f<T>(pl, 0.0);
   for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < n; p += gridDim.x * BLOCK) {
 #pragma unroll
     for (int k = 0; k < D * D; k++) b.Fp[(size_t)k * n + p] = k % (D + 1) == 0 ? (T)1 : (T)0;
     b.mu0[p] = b.mu[p];
     b.la0[p] = b.la[p];
+    b.pdef[p] = 0;
+    b.vc[p] = 0;
+    b.coh[p] = (T)pl.cohesion;
+    b.rhoF[p] = rho0;
   }
 }
 
@@ -2591,7 +2666,8 @@ class Engine final : public EngineBase {
       HIPCHECK(hipMemcpyAsync(cur_.valid, valid, n_, hipMemcpyHostToDevice, stream_));
       if (scene_ids) HIPCHECK(hipMemcpyAsync(cur_.pscene, scene_ids, n_ * sizeof(uint16_t), hipMemcpyHostToDevice, stream_));
       iota_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_.pid, n_);
-      if (plastic_) plastic_init_kernel<T, D><<<std::min<uint32_t>(cdiv(n_, BLOCK), 4096), BLOCK, 0, stream_>>>(cur_, n_);
+      if (plastic_)
+        plastic_init_kernel<T, D><<<std::min<uint32_t>(cdiv(n_, BLOCK), 4096), BLOCK, 0, stream_>>>(cur_, n_, pl_);
       HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
       Args<T, D> a = make_args(0);
       key_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(a, err_.p + 1);
@@ -2625,7 +2701,7 @@ class Engine final : public EngineBase {
   }
 
   void set_constitutive(int model, double thr) override {
-    if (model != 0 && model != 1) throw Error{MPMX_EINVAL, "unknown constitutive model"};
+    if (model != 0 && model != 1 && model != 2) throw Error{MPMX_EINVAL, "unknown constitutive model"};
     if (model == 1 && !(thr > 0)) throw Error{MPMX_EINVAL, "neo-Hookean threshold must be positive"};
     model_ = model;
     thr_ = (T)thr;
@@ -2827,36 +2903,89 @@ class Engine final : public EngineBase {
     rebin_fraction_ = mover_fraction;
     rebin_max_interval_ = max_interval;
   }
-  void set_plasticity(bool enable, double theta_c, double theta_s, double max_hardening, double xi) override {
-    if (enable) {
-      if (!(theta_c >= 0 && theta_c < 1) || !(theta_s >= 0) || !std::isfinite(max_hardening) || !std::isfinite(xi))
-        throw Error{MPMX_EINVAL, "need 0 <= theta_c < 1, theta_s >= 0 and finite hardening parameters"};
-      if (taping_) throw Error{MPMX_ESTATE, "plasticity cannot change while a tape is recorded"};
-      // the clamp bounds as the reference forms them in T
-      pl_ = PlasticDev<T>{(double)((T)1 - (T)theta_c), (double)((T)1 + (T)theta_s), (double)(T)max_hardening,
-                          (double)(T)xi};
-    }
-    const bool was = plastic_;
-    plastic_ = enable;
+  // Plastic arrays of one particle buffer set: Fp, mu0, lambda0 and the
+  // Drucker-Prager state.
+  static void add_plastic_arrays(ParticleBufs<T, D>& pb, std::vector<DevBuf<T>>& fb, size_t n) {
+    fb.resize(15);
+    fb[8].alloc(n * D * D);
+    for (int k = 9; k < 15; k++) fb[k].alloc(n);
+    pb.Fp = fb[8].p;
+    pb.mu0 = fb[9].p;
+    pb.la0 = fb[10].p;
+    pb.pdef = fb[11].p;
+    pb.rhoF = fb[12].p;
+    pb.coh = fb[13].p;
+    pb.vc = fb[14].p;
+  }
+  // Switches plasticity model `kind` (0 off, 1 clamp, 2 Drucker-Prager) on,
+  // starting from Fp = I, mu0 = mu, lambda0 = lambda and the initial
+  // Drucker-Prager state.
+  void enable_plastic(int kind, const PlasticDev<T>& pl) {
+    if (taping_) throw Error{MPMX_ESTATE, "plasticity cannot change while a tape is recorded"};
     fused_ = false;
-    if (enable && !was && n_ > 0) {
-      // add the plastic arrays to both particle buffer sets, keeping the state
-      auto add = [&](ParticleBufs<T, D>& pb, std::vector<DevBuf<T>>& fb) {
-        fb.resize(11);
-        fb[8].alloc((size_t)n_ * D * D); fb[9].alloc(n_); fb[10].alloc(n_);
-        pb.Fp = fb[8].p;
-        pb.mu0 = fb[9].p;
-        pb.la0 = fb[10].p;
-      };
-      add(cur_, fa_.size() && fa_[0].p == cur_.X ? fa_ : fb_);
-      add(alt_, fa_.size() && fa_[0].p == alt_.X ? fa_ : fb_);
-      plastic_init_kernel<T, D><<<std::min<uint32_t>(cdiv(n_, BLOCK), 4096), BLOCK, 0, stream_>>>(cur_, n_);
+    if (kind == 0) {
+      plastic_ = 0;
+      cur_.Fp = cur_.mu0 = cur_.la0 = cur_.pdef = cur_.rhoF = cur_.coh = cur_.vc = nullptr;
+      alt_.Fp = alt_.mu0 = alt_.la0 = alt_.pdef = alt_.rhoF = alt_.coh = alt_.vc = nullptr;
+      return;
+    }
+    pl_ = pl;
+    pl_.kind = kind;
+    plastic_ = kind;
+    if (n_ > 0) {
+      // cur_ always uses fa_ and alt_ fb_ (a reorder swaps both pairs)
+      add_plastic_arrays(cur_, fa_, n_);
+      add_plastic_arrays(alt_, fb_, n_);
+      plastic_init_kernel<T, D><<<std::min<uint32_t>(cdiv(n_, BLOCK), 4096), BLOCK, 0, stream_>>>(cur_, n_, pl_);
       HIPCHECK(hipGetLastError());
-    } else if (!enable) {
-      cur_.Fp = cur_.mu0 = cur_.la0 = nullptr;
-      alt_.Fp = alt_.mu0 = alt_.la0 = nullptr;
     }
     HIPCHECK(hipStreamSynchronize(stream_));
+  }
+  void set_plasticity(bool enable, double theta_c, double theta_s, double max_hardening, double xi) override {
+    if (!enable) {
+      if (plastic_ == 1) enable_plastic(0, pl_);
+      return;
+    }
+    if (!(theta_c >= 0 && theta_c < 1) || !(theta_s >= 0) || !std::isfinite(max_hardening) || !std::isfinite(xi))
+      throw Error{MPMX_EINVAL, "need 0 <= theta_c < 1, theta_s >= 0 and finite hardening parameters"};
+    PlasticDev<T> pl{};
+    // the clamp bounds as the reference forms them in T
+    pl.lo = (double)((T)1 - (T)theta_c);
+    pl.hi = (double)((T)1 + (T)theta_s);
+    pl.max_hardening = (double)(T)max_hardening;
+    pl.xi = (double)(T)xi;
+    enable_plastic(1, pl);
+  }
+  void set_drucker_prager(bool enable, double a0, double a1, double a3, double a4, double cohesion,
+                          bool volume_correction) override {
+    if (!enable) {
+      if (plastic_ == 2) enable_plastic(0, pl_);
+      return;
+    }
+    if (!std::isfinite(a0) || !std::isfinite(a1) || !std::isfinite(a3) || !std::isfinite(a4) || !(cohesion >= 0))
+      throw Error{MPMX_EINVAL, "need finite friction-angle parameters and cohesion >= 0"};
+    PlasticDev<T> pl{};
+    pl.a0 = (double)(T)a0;
+    pl.a1 = (double)(T)a1;
+    pl.a3 = (double)(T)a3;
+    pl.a4 = (double)(T)a4;
+    pl.cohesion = (double)(T)cohesion;
+    pl.use_vc = volume_correction ? 1 : 0;
+    if (!(dp_rho_host(pl, 0.0) > 0)) throw Error{MPMX_EINVAL, "the initial friction angle must be positive"};
+    enable_plastic(2, pl);
+  }
+  static double dp_rho_host(const PlasticDev<T>& pl, double q) {
+    const double phi = 3.14159265358979323846 / 180 * (pl.a0 + (pl.a1 * q - pl.a4) * std::exp(-pl.a3 * q));
+    const double sp = std::sin(phi);
+    return std::sqrt(2.0 / 3.0) * 2 * sp / (3 - sp);
+  }
+  void get_drucker_prager_state(void* plastic_def, void* rho_F, void* cohesion, void* vc) override {
+    if (plastic_ != 2) throw Error{MPMX_ESTATE, "Drucker-Prager plasticity is not enabled"};
+    if (n_ == 0) return;
+    download_aos(cur_.pdef, (T*)plastic_def, 1);
+    download_aos(cur_.rhoF, (T*)rho_F, 1);
+    download_aos(cur_.coh, (T*)cohesion, 1);
+    download_aos(cur_.vc, (T*)vc, 1);
   }
   void get_plastic_state(void* Fp, void* mu, void* lambda) override {
     if (!plastic_) throw Error{MPMX_ESTATE, "plasticity is not enabled"};
@@ -2901,6 +3030,7 @@ class Engine final : public EngineBase {
     if (n_ == 0) throw Error{MPMX_ESTATE, "no particles"};
     if (any_friction()) throw Error{MPMX_EINVAL, "the adjoint step has no friction: remove wall friction"};
     if (plastic_) throw Error{MPMX_EINVAL, "the adjoint step has no plasticity"};
+    if (model_ == 2) throw Error{MPMX_EINVAL, "the adjoint step has no Hencky-strain model"};
     if (force_reorder_ || force_rebin_) rebin();  // settle the storage order before recording
     particle_stats(nullptr, nullptr, &tape_valid_);
     fused_ = false;
@@ -3040,14 +3170,8 @@ class Engine final : public EngineBase {
       ib.alloc(n);
       if (batch) sb.alloc(n);
       pb = ParticleBufs<T, D>{fb[0].p, fb[1].p, fb[2].p, fb[3].p, fb[4].p, fb[5].p, fb[6].p, fb[7].p,
-                              vb.p,    ib.p,    batch ? sb.p : nullptr, nullptr, nullptr, nullptr};
-      if (plastic_) {
-        fb.resize(11);
-        fb[8].alloc(n * D * D); fb[9].alloc(n); fb[10].alloc(n);
-        pb.Fp = fb[8].p;
-        pb.mu0 = fb[9].p;
-        pb.la0 = fb[10].p;
-      }
+                              vb.p,    ib.p,    batch ? sb.p : nullptr};
+      if (plastic_) add_plastic_arrays(pb, fb, n);
     };
     // Buffers are sized exactly to n so that the SoA stride equals n.
     for (auto* fbv : {&fa_, &fb_})
@@ -3504,7 +3628,7 @@ class Engine final : public EngineBase {
   uint32_t* gt_cur_ = nullptr;
   uint32_t* gt_other_ = nullptr;
   uint32_t *ml_cur_ = nullptr, *mt_cur_ = nullptr, *ml_next_ = nullptr, *mt_next_ = nullptr;
-  bool plastic_ = false;  // clamped plasticity after every G2P (unfused steps)
+  int plastic_ = 0;  // plasticity after every G2P (unfused steps): 1 clamp, 2 Drucker-Prager
   PlasticDev<T> pl_{};
   DevBuf<uint8_t> simmask_;
   bool fused_ = false;       // gcur_ holds the P2G of the current state with dt fused_dt_

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmpmx.so")
 
 FP32, FP64 = 4, 8
-FIXED_COROTATED, NEO_HOOKEAN = 0, 1
+FIXED_COROTATED, NEO_HOOKEAN, HENCKY = 0, 1, 2
 STICK, SLIP, SEPARATE = 0, 1, 2
 SHAPE_BOX, SHAPE_SPHERE = 0, 1
 
@@ -26,7 +26,7 @@ EXPORTS = (
     "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_grid_node_count",
     "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_set_rebin_policy", "mpmx_bin_stats",
     "mpmx_grid_v_upper_bound", "mpmx_compute_dt", "mpmx_advance_frames",
-    "mpmx_set_plasticity", "mpmx_get_plastic_state",
+    "mpmx_set_plasticity", "mpmx_get_plastic_state", "mpmx_set_drucker_prager", "mpmx_get_drucker_prager_state",
     "mpmx_tape_begin", "mpmx_backward",
     "mpmx_enable_timing", "mpmx_reset_timing",
     "mpmx_phase_time", "mpmx_stress", "mpmx_svd", "mpmx_sort_keys", "mpmx_seed_particles",
@@ -97,6 +97,8 @@ def lib():
         L.mpmx_advance_frames.argtypes = [vp, c_int, dbl, dbl, dbl, dbl, vp, i64, vp]
         L.mpmx_set_plasticity.argtypes = [vp, c_int, dbl, dbl, dbl, dbl]
         L.mpmx_get_plastic_state.argtypes = [vp, vp, vp, vp]
+        L.mpmx_set_drucker_prager.argtypes = [vp, c_int, dbl, dbl, dbl, dbl, dbl, c_int]
+        L.mpmx_get_drucker_prager_state.argtypes = [vp, vp, vp, vp, vp]
         L.mpmx_tape_begin.argtypes = [vp, c_int]
         L.mpmx_backward.argtypes = [vp] + [vp] * 10
         L.mpmx_enable_timing.argtypes = [vp, c_int]
@@ -309,6 +311,18 @@ class Context:
         la = np.zeros(n, self.dtype)
         _check(lib().mpmx_get_plastic_state(self._h, _p(Fp), _p(mu), _p(la)))
         return {"Fp": Fp, "mu": mu, "lambda": la}
+
+    def set_drucker_prager(self, a0, a1=0.0, a3=0.0, a4=0.0, cohesion=0.0, volume_correction=True, enable=True):
+        """Drucker-Prager plasticity after every G2P (MPM_DRUCKER_PRAGER); friction angle in degrees."""
+        _check(lib().mpmx_set_drucker_prager(self._h, 1 if enable else 0, float(a0), float(a1), float(a3), float(a4),
+                                             float(cohesion), 1 if volume_correction else 0))
+
+    def get_drucker_prager_state(self):
+        """dict plastic_def, rho_F, cohesion, vc of the particles (user order)."""
+        out = {k: np.zeros(self.n, self.dtype) for k in ("plastic_def", "rho_F", "cohesion", "vc")}
+        _check(lib().mpmx_get_drucker_prager_state(self._h, *(_p(out[k]) for k in ("plastic_def", "rho_F", "cohesion",
+                                                                                  "vc"))))
+        return out
 
     def tape_begin(self, checkpoint_every=20):
         _check(lib().mpmx_tape_begin(self._h, int(checkpoint_every)))

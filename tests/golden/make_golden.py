@@ -101,6 +101,21 @@ def plastic_cases():
                        frand=0.08, walls=walls, dt=5e-4, per_particle_E=True, seed=42)
     sc["plasticity"] = dict(theta_c=0.02, theta_s=0.01, max_hardening=3.0, hardening_factor=10.0)
     cases["snow3d_f32"] = (sc, (1, 8))
+    # sand: Drucker-Prager on the Hencky-strain model (Add_Drucker_Prager), a
+    # block dropped on a sticky floor; friction angle hardening a0..a4 of
+    # Klar et al. (the reference's default case constants) and no cohesion
+    walls = mt.wall_boxes(2, [32, 32], [0, 0], [1, 1], flags=15, inset=0.1, type_=mt.STICK, friction=0.0)
+    sc = mt.make_scene(2, 32, prec=8, model=mt.HENCKY, shape=sq, v0=[0.5, -3.0], vrand=1.0, frand=0.06,
+                       walls=walls, dt=2e-4, seed=43)
+    sc["plasticity"] = dict(kind="drucker_prager", a0=35.0, a1=0.0, a3=0.2, a4=10.0, cohesion=0.0,
+                            volume_correction=True)
+    cases["sand2d_f64"] = (sc, (1, 10))
+    walls = mt.wall_boxes(3, [16] * 3, [0] * 3, [1] * 3, flags=63, inset=0.12, type_=mt.STICK, friction=0.0)
+    sc = mt.make_scene(3, 16, prec=4, model=mt.HENCKY, shape=sph, v0=[0.3, -3.0, 0.2], vrand=1.0, frand=0.06,
+                       walls=walls, dt=5e-4, per_particle_E=True, seed=44)
+    sc["plasticity"] = dict(kind="drucker_prager", a0=30.0, a1=9.0, a3=0.2, a4=10.0, cohesion=100.0,
+                            volume_correction=True)
+    cases["sand3d_f32"] = (sc, (1, 8))
     return cases
 
 
@@ -111,11 +126,12 @@ def write_plastic():
             out[f"{name}_in_{k}"] = np.asarray(sc[k])
         meta = {k: sc[k] for k in ("dim", "prec", "model", "N", "lo", "hi", "gravity", "dt", "neo_threshold",
                                    "plasticity")}
+        extra = ("plastic_def", "rho_F", "cohesion", "vc") if scene_io.plastic_kind(sc) == 2 else ()
         out[f"{name}_meta"] = np.array(json.dumps(meta, default=float))
         out[f"{name}_walls"] = np.array(json.dumps(sc["walls"], default=float))
         for n in dumps:
             r = scene_io.run_ref(sc, n)[-1]
-            for k in ("X", "V", "F", "B", "valid", "Fp", "mu", "lambda"):
+            for k in ("X", "V", "F", "B", "valid", "Fp", "mu", "lambda") + extra:
                 out[f"{name}_s{n}_{k}"] = r[k]
             moved = (np.abs(r["Fp"] - np.tile(np.eye(sc["dim"]).reshape(-1), (len(r["Fp"]), 1))).max(1) > 0).sum()
             print("plastic", name, "steps", n, "particles with Fp != I:", moved, "of", len(r["Fp"]))

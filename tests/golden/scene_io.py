@@ -20,6 +20,14 @@ def ref_available():
     return os.path.isfile(REF_HARNESS) and os.access(REF_HARNESS, os.X_OK)
 
 
+def plastic_kind(sc):
+    """0 none, 1 MPM_PLASTICITY_CLAMP, 2 MPM_DRUCKER_PRAGER (sc["plasticity"]["kind"])."""
+    pl = sc.get("plasticity")
+    if not pl:
+        return 0
+    return 2 if pl.get("kind") == "drucker_prager" else 1
+
+
 def write_scene(path, sc, nsteps, dump_every, threads=1):
     """Write scene dict ``sc`` (see tests/mpmtest.py make_* helpers)."""
     d = sc["dim"]
@@ -33,14 +41,17 @@ def write_scene(path, sc, nsteps, dump_every, threads=1):
     with open(path, "wb") as f:
         f.write(b"MPMSCN1\0")
         f.write(struct.pack("<8i", d, sc["prec"], sc["model"], nsteps, dump_every,
-                            len(walls), threads, 1 if pl else 0))
+                            len(walls), threads, plastic_kind(sc)))
         f.write(struct.pack("<3i", *N))
         f.write(struct.pack("<3d", *lo))
         f.write(struct.pack("<3d", *hi))
         f.write(struct.pack("<3d", *g))
         f.write(struct.pack("<2d", sc["dt"], sc.get("neo_threshold", 0.25)))
-        if pl:
+        if plastic_kind(sc) == 1:
             f.write(struct.pack("<4d", pl["theta_c"], pl["theta_s"], pl["max_hardening"], pl["hardening_factor"]))
+        elif plastic_kind(sc) == 2:
+            f.write(struct.pack("<6d", pl["a0"], pl["a1"], pl["a3"], pl["a4"], pl["cohesion"],
+                                1.0 if pl.get("volume_correction", True) else 0.0))
         f.write(struct.pack("<q", n))
         for w in walls:
             wl = list(w["lo"]) + [0.0] * (3 - d)
@@ -54,7 +65,7 @@ def write_scene(path, sc, nsteps, dump_every, threads=1):
         f.write(np.ascontiguousarray(sc["valid"], dtype=np.uint8).tobytes())
 
 
-def read_output(path, dim, plastic=False):
+def read_output(path, dim, plastic=0):
     with open(path, "rb") as f:
         buf = f.read()
     assert buf[:7] == b"MPMOUT1"
@@ -86,6 +97,9 @@ def read_output(path, dim, plastic=False):
             r["Fp"] = take(n * d * d, np.float64).reshape(n, d * d)
             r["mu"] = take(n, np.float64)
             r["lambda"] = take(n, np.float64)
+        if plastic == 2:
+            for k in ("plastic_def", "rho_F", "cohesion", "vc"):
+                r[k] = take(n, np.float64)
         dumps.append(r)
     return dumps
 
@@ -99,7 +113,7 @@ def run_ref(sc, nsteps, dump_every=None, threads=1):
         op = os.path.join(td, "out.bin")
         write_scene(sp, sc, nsteps, dump_every, threads)
         subprocess.run([REF_HARNESS, "step", sp, op], check=True)
-        return read_output(op, sc["dim"], plastic=bool(sc.get("plasticity")))
+        return read_output(op, sc["dim"], plastic=plastic_kind(sc))
 
 
 def run_ref_frames(sc, nframes, frame_dt, cfl, min_dt, max_dt, threads=1):

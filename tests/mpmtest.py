@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle_mpm.so")
 
 STICK, SLIP, SEPARATE = 0, 1, 2
-COROTATED, NEO_HOOKEAN = 0, 1
+COROTATED, NEO_HOOKEAN, HENCKY = 0, 1, 2
 
 
 def lame(E, nu):
@@ -182,6 +182,38 @@ def oracle_run_plastic(sc, nsteps, prec=None):
             _ptr(st["mu"]), _ptr(st["lambda"]), _ptr(valid), ctypes.c_int(nsteps), _ptr(par), _ptr(Fp))
     assert rc == 0
     return dict(X=st["X"], V=st["V"], F=st["F"], B=st["B"], valid=valid, Fp=Fp, mu=st["mu"], **{"lambda": st["lambda"]})
+
+
+def oracle_run_drucker_prager(sc, nsteps, prec=None):
+    """Advance ``nsteps`` with MPM_DRUCKER_PRAGER after every step (sc["plasticity"]: a0, a1, a3, a4,
+    cohesion, volume_correction).  Returns the final state with Fp, plastic_def, rho_F, cohesion, vc."""
+    prec = prec or sc["prec"]
+    dt_np = np.float64 if prec == 8 else np.float32
+    d = sc["dim"]
+    n = sc["X"].shape[0]
+    st = {k: np.ascontiguousarray(sc[k], dtype=dt_np).copy() for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda")}
+    valid = np.ascontiguousarray(sc["valid"], np.uint8).copy()
+    Fp = np.ascontiguousarray(np.tile(np.eye(d).reshape(-1), (n, 1)), dtype=dt_np).copy()
+    state = np.zeros(4 * n, dt_np)
+    N = np.array(list(sc["N"]) + [1] * (3 - d), np.int32)
+    lo = np.array(list(sc["lo"]) + [0] * (3 - d), np.float64)
+    hi = np.array(list(sc["hi"]) + [0] * (3 - d), np.float64)
+    g = np.array(list(sc["gravity"]) + [0] * (3 - d), np.float64)
+    pl = sc["plasticity"]
+    par = np.array([pl["a0"], pl["a1"], pl["a3"], pl["a4"], pl["cohesion"],
+                    1.0 if pl.get("volume_correction", True) else 0.0], np.float64)
+    nw, wt, wf, wl, wh = _walls_arrays(sc)
+    fn = oracle().oracle_run_drucker_prager_f64 if prec == 8 else oracle().oracle_run_drucker_prager_f32
+    rc = fn(ctypes.c_int(d), _ptr(N), _ptr(lo), _ptr(hi), _ptr(g), ctypes.c_int(sc["model"]),
+            ctypes.c_double(sc.get("neo_threshold", 0.25)), ctypes.c_double(sc["dt"]),
+            ctypes.c_int(nw), _ptr(wt), _ptr(wf), _ptr(wl), _ptr(wh), ctypes.c_longlong(n),
+            _ptr(st["X"]), _ptr(st["V"]), _ptr(st["F"]), _ptr(st["B"]), _ptr(st["mass"]), _ptr(st["volume"]),
+            _ptr(st["mu"]), _ptr(st["lambda"]), _ptr(valid), ctypes.c_int(nsteps), _ptr(par), _ptr(Fp), _ptr(state))
+    assert rc == 0
+    out = dict(X=st["X"], V=st["V"], F=st["F"], B=st["B"], valid=valid, Fp=Fp, mu=st["mu"], **{"lambda": st["lambda"]})
+    for i, k in enumerate(("plastic_def", "rho_F", "cohesion", "vc")):
+        out[k] = state[i * n:(i + 1) * n]
+    return out
 
 
 def oracle_frames(sc, nframes, frame_dt, cfl, min_dt, max_dt, prec=None, max_sub=100000):

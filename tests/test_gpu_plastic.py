@@ -1,12 +1,16 @@
-"""Clamped plasticity with hardening on the GPU (mpmx_set_plasticity) against
-the reference's own runs (tests/golden/plastic.npz, MPM_PLASTICITY_CLAMP after
-every step) and the oracle.
+"""Plasticity on the GPU against the reference's own runs
+(tests/golden/plastic.npz: MPM_PLASTICITY_CLAMP after every step of the snow
+scenes, MPM_DRUCKER_PRAGER on the Hencky-strain model for the sand scenes) and
+the oracle.
 
-FP64: X, V, F, B, Fp, mu, lambda to 1e-9 relative after one step and 1e-8
-after ten.  FP32: against the FP64 oracle of the same float inputs, X, V, F,
-Fp, mu, lambda to 1e-5 and B to the bound the golden step test uses (B is
-measured against max(|B|, |V| h)).  The clamp is continuous in F, so the
-plastic update keeps these tolerances.
+FP64: X, V, F, B, Fp, mu, lambda (and plastic_def, rho_F, cohesion, vc for
+sand) to 1e-9 relative after one step and 1e-8 after ten.  FP32: against the
+FP64 oracle of the same float inputs, to 1e-5 and B to the bound the golden
+step test uses (B is measured against max(|B|, |V| h)).  Both return mappings
+are continuous in F (the Drucker-Prager tension branch meets the cone branch
+at zero trace), so the plastic update keeps these tolerances.  plastic_def
+and vc are sums of log strains and get the same bounds as absolute errors
+(each FP32 step adds the log of an F rounded to FP32).
 """
 import json
 import os
@@ -32,17 +36,29 @@ def fixture(name):
     return z, sc, steps
 
 
+DP_KEYS = ("plastic_def", "rho_F", "cohesion", "vc")
+
+
+def is_sand(sc):
+    return sc["plasticity"].get("kind") == "drucker_prager"
+
+
 def plastic_context(sc):
     ctx = gpu_context(sc)
     pl = sc["plasticity"]
-    ctx.set_plasticity(pl["theta_c"], pl["theta_s"], pl["max_hardening"], pl["hardening_factor"])
+    if is_sand(sc):
+        ctx.set_drucker_prager(pl["a0"], pl["a1"], pl["a3"], pl["a4"], pl["cohesion"], pl["volume_correction"])
+    else:
+        ctx.set_plasticity(pl["theta_c"], pl["theta_s"], pl["max_hardening"], pl["hardening_factor"])
     return ctx
 
 
-@pytest.mark.parametrize("name", ["snow2d_f64", "snow3d_f32"])
+@pytest.mark.parametrize("name", ["snow2d_f64", "snow3d_f32", "sand2d_f64", "sand3d_f32"])
 def test_plasticity_matches_reference(name):
     z, sc, steps = fixture(name)
     fp32 = sc["prec"] == 4
+    sand = is_sand(sc)
+    keys = ("X", "V", "F", "B", "Fp", "mu", "lambda") + (DP_KEYS if sand else ())
     h = (np.asarray(sc["hi"]) - np.asarray(sc["lo"])).max() / max(sc["N"])
     with plastic_context(sc) as ctx:
         done = 0
@@ -52,14 +68,24 @@ def test_plasticity_matches_reference(name):
             done = n
             got = dict(ctx.get_particles())
             got.update(ctx.get_plastic_state())
+            if sand:
+                got.update(ctx.get_drucker_prager_state())
             if fp32:  # the exact answer for the float inputs: the FP64 oracle
-                ref = mt.oracle_run_plastic(sc, n, prec=8)
+                run = mt.oracle_run_drucker_prager if sand else mt.oracle_run_plastic
+                ref = run(sc, n, prec=8)
             else:
-                ref = {k: z[f"{name}_s{n}_{k}"] for k in ("X", "V", "F", "B", "valid", "Fp", "mu", "lambda")}
+                ref = {k: z[f"{name}_s{n}_{k}"] for k in keys + ("valid",)}
             assert np.array_equal(got["valid"], ref["valid"]), (name, n)
-            for k in ("X", "V", "F", "B", "Fp", "mu", "lambda"):
+            for k in keys:
                 r = np.asarray(ref[k], np.float64)
-                scale = max(np.abs(r).max(), np.abs(np.asarray(ref["V"])).max() * h) if k == "B" else np.abs(r).max()
+                if k == "B":
+                    scale = max(np.abs(r).max(), np.abs(np.asarray(ref["V"])).max() * h)
+                elif k in ("plastic_def", "vc"):  # sums of log strains: absolute
+                    scale = 1.0
+                elif k == "cohesion":
+                    scale = max(np.abs(r).max(), 1.0)
+                else:
+                    scale = np.abs(r).max()
                 err = float(np.abs(got[k] - r).max() / scale)
                 tol = (1e-5 if k != "B" else 3e-4 * max(1, n / 2)) if fp32 else (1e-9 if n <= 1 else 1e-8)
                 print(f"  {k} {err:.2e} (tol {tol:.0e})", flush=True)
@@ -75,6 +101,49 @@ def test_plasticity_off_again_is_the_elastic_step():
         with pytest.raises(mpmx.MpmxError):
             ctx.get_plastic_state()
     orc = mt.oracle_run(sc, 3, want_grid=False)
+    assert rel(got["F"], orc["F"]) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["sand2d_f64", "sand3d_f32"])
+def test_hencky_elastic_step_matches_oracle(name):
+    """The Hencky-strain model alone (fused FP32 kernels, FP64 unfused) against
+    the oracle, whose Hencky stress the sand fixtures pin."""
+    _, sc, _ = fixture(name)
+    fp32 = sc["prec"] == 4
+    with gpu_context(sc) as ctx:
+        ctx.step(sc["dt"], 4)
+        got = ctx.get_particles()
+    orc = mt.oracle_run(sc, 4, want_grid=False, prec=8)
+    for k in ("X", "V", "F"):
+        err = rel(got[k], orc[k])
+        print(name, k, f"{err:.2e}", flush=True)
+        assert err <= (1e-5 if fp32 else 1e-9), (k, err)
+
+
+def test_drucker_prager_switching_and_bad_parameters():
+    _, sc, _ = fixture("sand2d_f64")
+    pl = sc["plasticity"]
+    with gpu_context(sc) as ctx:
+        with pytest.raises(mpmx.MpmxError):
+            ctx.get_drucker_prager_state()
+        for args in ((float("nan"),), (35.0, 0.0, 0.2, 10.0, -1.0), (0.0,)):
+            with pytest.raises(mpmx.MpmxError):
+                ctx.set_drucker_prager(*args)
+        ctx.set_drucker_prager(pl["a0"], pl["a1"], pl["a3"], pl["a4"], 7.0)
+        st = ctx.get_drucker_prager_state()
+        assert np.all(st["cohesion"] == 7.0) and np.all(st["plastic_def"] == 0) and np.all(st["vc"] == 0)
+        phi = np.radians(pl["a0"] - pl["a4"])
+        assert st["rho_F"] == pytest.approx(np.sqrt(2 / 3) * 2 * np.sin(phi) / (3 - np.sin(phi)), rel=1e-12)
+        with pytest.raises(mpmx.MpmxError):
+            ctx.tape_begin(2)
+        # the clamp replaces it; turning the clamp off leaves the elastic step
+        ctx.set_plasticity(0.02, 0.01, 5.0, 10.0)
+        with pytest.raises(mpmx.MpmxError):
+            ctx.get_drucker_prager_state()
+        ctx.set_plasticity(0.02, 0.01, 5.0, 10.0, enable=False)
+        ctx.step(sc["dt"], 2)
+        got = ctx.get_particles()
+    orc = mt.oracle_run(sc, 2, want_grid=False)
     assert rel(got["F"], orc["F"]) < 1e-9
 
 

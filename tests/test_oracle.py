@@ -110,17 +110,21 @@ def test_oracle_p_hat_matches_reference(dim, model, prec):
 
 def energy(S, mu, la, model, thr=0.25):
     """Textbook energy densities: fixed corotated mu sum (s-1)^2 + la/2 (J-1)^2;
-    neo-Hookean mu/2 (sum s^2 - d) - mu ln J + la/2 (ln J)^2 (J >= threshold)."""
+    neo-Hookean mu/2 (sum s^2 - d) - mu ln J + la/2 (ln J)^2 (J >= threshold);
+    St. Venant-Kirchhoff with Hencky strain mu sum (ln s)^2 + la/2 (sum ln s)^2."""
     J = np.prod(S, axis=1)
     if model == 0:
         return mu * ((S - 1) ** 2).sum(1) + la / 2 * (J - 1) ** 2
+    if model == 2:
+        L = np.log(S)
+        return mu * (L ** 2).sum(1) + la / 2 * L.sum(1) ** 2
     d = S.shape[1]
     lj = np.log(J)
     return mu / 2 * ((S ** 2).sum(1) - d) - mu * lj + la / 2 * lj ** 2
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("model", [0, 1])
+@pytest.mark.parametrize("model", [0, 1, 2])
 def test_p_hat_is_energy_gradient(dim, model):
     rng = np.random.default_rng(5)
     n = 300
@@ -294,10 +298,12 @@ def test_grid_v_upper_bound_closed_form():
         assert mt.oracle_grid_v_bound(sc) < got
 
 
-@pytest.mark.parametrize("name", ["snow2d_f64", "snow3d_f32"])
+@pytest.mark.parametrize("name", ["snow2d_f64", "snow3d_f32", "sand2d_f64", "sand3d_f32"])
 def test_oracle_plasticity_matches_reference(name):
-    """MPM_PLASTICITY_CLAMP after every step (Fp, hardened mu and lambda, the
-    projected F): bit-exact with the reference's own run."""
+    """MPM_PLASTICITY_CLAMP (snow) and MPM_DRUCKER_PRAGER on the Hencky-strain
+    model (sand) after every step: Fp, the hardened mu and lambda or the
+    Drucker-Prager state, and the projected F are bit-exact with the
+    reference's own run."""
     z = load("plastic.npz")
     meta = json.loads(str(z[f"{name}_meta"]))
     sc = dict(meta)
@@ -306,13 +312,20 @@ def test_oracle_plasticity_matches_reference(name):
         sc[k] = z[f"{name}_in_{k}"]
     steps = sorted({int(k.split("_s")[1].split("_")[0]) for k in z.files if k.startswith(name + "_s")})
     assert steps
+    sand = sc["plasticity"].get("kind") == "drucker_prager"
+    keys = ("X", "V", "F", "B", "valid", "Fp", "mu", "lambda")
+    if sand:
+        keys += ("plastic_def", "rho_F", "cohesion", "vc")
     for n in steps:
-        out = mt.oracle_run_plastic(sc, n)
-        for k in ("X", "V", "F", "B", "valid", "Fp", "mu", "lambda"):
+        out = mt.oracle_run_drucker_prager(sc, n) if sand else mt.oracle_run_plastic(sc, n)
+        for k in keys:
             got = out[k].astype(np.float64) if k != "valid" else out[k]
             assert np.array_equal(got, z[f"{name}_s{n}_{k}"]), (name, n, k)
-    # the fixture clamps: Fp moved away from I and the Lame parameters hardened or softened
-    assert (np.abs(out["mu"] - sc["mu"]) > 0).mean() > 0.5
+    if sand:  # projected particles: plastic deformation accumulated, friction hardened
+        assert (out["plastic_def"] > 0).mean() > 0.3
+        assert out["rho_F"].max() > out["rho_F"].min()
+    else:     # the fixture clamps: the Lame parameters hardened or softened
+        assert (np.abs(out["mu"] - sc["mu"]) > 0).mean() > 0.5
 
 
 def test_plasticity_leaves_elastic_range_alone():
