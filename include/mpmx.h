@@ -191,6 +191,28 @@ int mpmx_compute_dt(mpmx_ctx* ctx, double cfl, double max_dt, double* dt);
 int mpmx_advance_frames(mpmx_ctx* ctx, int nframes, double frame_dt, double cfl, double min_dt, double max_dt,
                         double* dts, int64_t dts_capacity, int64_t* nsubsteps);
 
+/* Strong and sound-speed CFL: MPM_DRIVER::Reduce_Dt (MPM_DRIVER.cpp:519-545)
+   between the grid update and G2P of every step, as -strong_cfl and
+   -sound_cfl turn it on (Projects/mpm/STANDARD_TESTS_BASE.cpp:128-129).
+   strong_cfl: Grid_To_Particle_Limit_Dt (:572-620) lowers dt so that no entry
+   of dt grad v changes by more than cfl_F and no particle moves more than cfl
+   (in the domain's length units, as example.cfl) along an axis; sound_cfl:
+   Limit_Dt_Sound_Speed (:624-637) caps dt at cfl_sound min dx / c_max with
+   the largest particle sound speed (ISOTROPIC_CONSTITUTIVE_MODEL::
+   Sound_Speed).  The result is raised to min_dt; when it is below 0.99 dt
+   the step's grid velocity changes are scaled to it and G2P takes it, and
+   mpmx_step advances the time by it (mpmx_last_dt reads it back);
+   mpmx_advance_frames then continues the frame as Simulate_To_Frame does
+   (:190-194), using its own cfl and min_dt.  Defaults cfl 1, cfl_F 0.5,
+   cfl_sound 0.9, min_dt 0 (MPM_EXAMPLE.h:95-97); both off.  Steps with a
+   limit on read two values back from the GPU and take the unfused kernels;
+   not for batches or tapes.  mpmx_max_sound_speed is Compute_Max_Sound_Speed
+   (:893-918) of the current particles. */
+int mpmx_set_dt_limits(mpmx_ctx* ctx, int strong_cfl, int sound_cfl, double cfl, double cfl_F, double cfl_sound,
+                       double min_dt);
+int mpmx_max_sound_speed(mpmx_ctx* ctx, double* out);
+int mpmx_last_dt(mpmx_ctx* ctx, double* out);
+
 /* Plasticity: MPM_PLASTICITY_CLAMP with hardening
    (Hybrid_Methods/Forces/MPM_PLASTICITY_CLAMP.cpp:31-45 and
    MPM_PLASTICITY_MODEL.cpp:34-48, as Projects/mpm/STANDARD_TESTS_BASE.cpp:

@@ -354,6 +354,99 @@ template<class T,int d> Vec<T,d> P_Neo_Hookean(const Vec<T,d>& s,T mu,T la,T thr
     return r;
 }
 
+// The parts of Isotropic_Stress_Derivative that Sound_Speed reads: the
+// diagonal of H and the B (spin) entries, for the three models.
+//   fixed corotated COROTATED_FIXED.cpp:127-140 (2D), :144-167 (3D), panic_threshold 1e-6 (:22)
+//   neo-Hookean     NEO_HOOKEAN.cpp:143-156 (2D), :161-180 (3D)
+//   Hencky          ST_VENANT_KIRCHHOFF_HENCKY_STRAIN.cpp:85-102 (2D), :106-137 (3D), failure_threshold 0 (:28)
+// log1p_x_over_x / diff_log_over_diff: Core/Math_Tools/Robust_Functions.h:33-39.
+template<class T> T Log1p_X_Over_X(T x){if(std::fabs(x)<1e-18) return 1;return std::log1p(x)/x;}
+template<class T> T Diff_Log_Over_Diff(T x,T y){return Log1p_X_Over_X((y-x)/x)/x;}
+template<class T> T Panic(T d){const T pt=(T)1e-6;if(std::fabs(d)<pt) d=d<0?-pt:pt;return d;}
+template<class T> void Stress_Derivative_HB(int model,const Vec<T,2>& F,T mu,T la,T thr,Vec<T,2>& H,T* B)
+{
+    if(model==0){
+        T mu2=2*mu,d01=Panic(F[0]+F[1]);
+        H[0]=mu2+la*(F[1]*F[1]);H[1]=mu2+la*(F[0]*F[0]);
+        B[0]=mu2*(1-1/d01);}
+    else if(model==1){
+        Vec<T,2> Fc=(F[0]*F[1]>=thr)?F:Clamp_To_Hyperbola(F,thr,Dth_Root(thr,2));
+        Vec<T,2> Fi;for(int a=0;a<2;a++) Fi[a]=1/Fc[a];
+        T mlj=mu+la*std::log(Fi[0]*Fi[1]);
+        for(int a=0;a<2;a++) H[a]=mu+(la+mlj)*(Fi[a]*Fi[a]);
+        B[0]=mu;}
+    else{
+        if(F[1]<=0){H[0]=H[1]=B[0]=0;return;}
+        T s0=std::max(F[0],(T)0),s1=std::max(F[1],(T)0),lp2m=la+2*mu,two_mu=2*mu;
+        T l0=std::log(s0),l1=std::log(s1),sum_log=l0+l1;
+        T x00=s0*s0,x11=s1*s1,x10=s1*s0;
+        T m01=(two_mu/s0)*(Diff_Log_Over_Diff(s0,s1)-l1/s1)-la*sum_log/x10;
+        T p01=((la*s0+lp2m*s1)*l0+(la*s1+lp2m*s0)*l1)/(x10*(s0+s1));
+        H[0]=(lp2m*(1-l0)-la*l1)/x00;H[1]=(lp2m*(1-l1)-la*l0)/x11;
+        B[0]=(T)0.5*(m01+p01);}
+}
+template<class T> void Stress_Derivative_HB(int model,const Vec<T,3>& F,T mu,T la,T thr,Vec<T,3>& H,T* B)
+{
+    if(model==0){
+        T mu2=2*mu;
+        Vec<T,3> cof=Cofactor(F);
+        T d01=Panic(F[0]+F[1]),d02=Panic(F[0]+F[2]),d12=Panic(F[1]+F[2]);
+        for(int a=0;a<3;a++) H[a]=mu2+la*(cof[a]*cof[a]);
+        B[2]=mu2*(1-1/d01);B[1]=mu2*(1-1/d02);B[0]=mu2*(1-1/d12);}
+    else if(model==1){
+        Vec<T,3> Fc=(Product(F)>=thr)?F:Clamp_To_Hyperbola(F,thr,Dth_Root(thr,3));
+        Vec<T,3> Fi;for(int a=0;a<3;a++) Fi[a]=1/Fc[a];
+        T mlj=mu+la*std::log(Product(Fi));
+        for(int a=0;a<3;a++) H[a]=mu+(la+mlj)*(Fi[a]*Fi[a]);
+        B[0]=B[1]=B[2]=mu;}
+    else{
+        if(F[2]<=0){H[0]=H[1]=H[2]=B[0]=B[1]=B[2]=0;return;}
+        T s[3],l[3];
+        for(int a=0;a<3;a++){s[a]=std::max(F[a],(T)0);l[a]=std::log(s[a]);}
+        T lp2m=la+2*mu,two_mu=2*mu,sum_log=l[0]+l[1]+l[2];
+        T row[3];for(int i=0;i<3;i++) row[i]=s[i]*l[0]+s[i]*l[1]+s[i]*l[2];
+        for(int a=0;a<3;a++) H[a]=(lp2m-two_mu*l[a]-la*sum_log)/(s[a]*s[a]);
+        // B(2) from the pair (0,1), B(1) from (0,2), B(0) from (1,2)
+        const int pi[3]={1,0,0},pj[3]={2,2,1};
+        for(int k=0;k<3;k++){
+            int i=pi[k],j=pj[k];
+            T xij=s[j]*s[i];
+            T mij=(two_mu/s[i])*(Diff_Log_Over_Diff(s[i],s[j])-l[j]/s[j])-la*sum_log/xij;
+            T pij=(two_mu*(s[j]*l[i]+s[i]*l[j])+la*(row[i]+row[j]))/(xij*(s[i]+s[j]));
+            B[k]=(T)0.5*(mij+pij);}}
+}
+// ISOTROPIC_CONSTITUTIVE_MODEL::Sound_Speed (ISOTROPIC_CONSTITUTIVE_MODEL.cpp:151-162):
+// sqrt(max(H_ii F_i^2, B_a F_{(a+1)%m}^2, B_a F_{(a+2)%m}^2) / density), m the spin size.
+template<class T,int d> T Sound_Speed(int model,const Vec<T,d>& F,T mu,T la,T thr,T density)
+{
+    Vec<T,d> H;T B[3];
+    Stress_Derivative_HB(model,F,mu,la,thr,H,B);
+    Vec<T,d> F2;for(int a=0;a<d;a++) F2[a]=F[a]*F[a];
+    T cm=H[0]*F2[0];for(int a=1;a<d;a++) cm=std::max(cm,H[a]*F2[a]);
+    const int m=d==2?1:3;
+    for(int a=0;a<m;a++) for(int i=0;i<2;i++) cm=std::max(cm,B[a]*F2[(a+i+1)%m]);
+    return std::sqrt(cm/density);
+}
+// Core/Math_Tools/Robust_Arithmetic.h:22-26 (FLT_MAX in T for both precisions).
+template<class T> T Robust_Divide(T a,T b)
+{
+    const T fm=(T)std::numeric_limits<float>::max();
+    T aa=std::fabs(a),ab=std::fabs(b);
+    if(ab==fm) return T();
+    if(ab>1 || aa<fm*ab) return a/b;
+    return (a>0)==(b>=0)?fm:-fm;
+}
+// Enforce_Limit_Max (MPM_DRIVER.cpp:549-559): lower s so that
+// |t (a + t b)| <= bound for 0 <= t <= s.
+template<class T> void Enforce_Limit_Max(T& s,T bound,T a,T b)
+{
+    if(a<0){a=-a;b=-b;}
+    if(a*a+4*b*bound>=0){
+        if(s*(a+s*b)>bound) s=(2*bound)/(a+std::sqrt(a*a+4*b*bound));}
+    else{
+        if(s*(a+s*b)<-bound) s=-(a+std::sqrt(a*a-4*b*bound))/(2*b);}
+}
+
 //#####################################################################
 // Scene and state
 //#####################################################################
@@ -365,6 +458,10 @@ template<class T,int d> struct Scene
     Vec<T,d> lo,hi,dX,one_over_dX,gravity;
     int model;T neo_threshold;T dt;
     std::vector<Wall> walls;
+    // Reduce_Dt settings (MPM_EXAMPLE.h:95-104): example.cfl, cfl_F,
+    // cfl_sound, min_dt, use_strong_cfl, use_sound_speed_cfl
+    int strong_cfl=0,sound_cfl=0;
+    T cfl=1,cfl_F=(T).5,cfl_sound=(T).9,min_dt=0;
     int gn[3];        // nodes per axis including 3 ghost layers each side
     long long nn;     // total nodes
 };
@@ -490,7 +587,76 @@ template<class T,int d> Mat<T,d> Stress_PFT(const Scene<T,d>& s,const Mat<T,d>& 
     return U*DT;
 }
 
-template<class T,int d> void Step(const Scene<T,d>& s,Particles<T,d>& P,Grid<T,d>& G,T time)
+// MPM_DRIVER::Grid_To_Particle_Limit_Dt (MPM_DRIVER.cpp:572-620) without the
+// midpoint rule: s = min over the simulated particles of the largest t <= 100
+// with |t (dt grad v_old + t dt grad(v_new - v_old))| <= cfl_F per entry and
+// |t (dt v_old + t dt (v_new - v_old))| <= cfl per axis (gathered with the
+// step's weights); returns dt s.
+template<class T,int d> T Grid_To_Particle_Limit_Dt(const Scene<T,d>& s,const Particles<T,d>& P,const Grid<T,d>& G,
+    const std::vector<Weights<T,d> >& W,const std::vector<long long>& sim)
+{
+    const T dt=s.dt;
+    const int K=d==3?27:9;
+    T smin=100;
+    for(long long p:sim){
+        Vec<T,d> V_pic,V_pic_s;
+        Mat<T,d> grad,grad_s;
+        for(int k=0;k<K;k++){
+            int idx[d];T w;Vec<T,d> g;
+            Stencil(W[p],k,idx,w,g);
+            long long f=Flat(s,idx);
+            Vec<T,d> V_new=G.velocity[f],V_old=G.velocity_save[f];
+            V_pic=V_pic+w*V_old;
+            V_pic_s=V_pic_s+w*(V_new-V_old);
+            Vec<T,d> V_grid_s=(T)1*(V_new-V_old);
+            grad=grad+Outer(V_old,g);
+            grad_s=grad_s+Outer(V_grid_s,g);}
+        T sp=100;
+        Mat<T,d> a=dt*grad,b=dt*grad_s;
+        for(int i=0;i<d;i++) for(int j=0;j<d;j++) Enforce_Limit_Max(sp,s.cfl_F,a(i,j),b(i,j));
+        Vec<T,d> xa=dt*V_pic,xb=dt*V_pic_s;
+        for(int i=0;i<d;i++) Enforce_Limit_Max(sp,s.cfl,xa[i],xb[i]);
+        smin=std::min(sp,smin);}
+    return s.dt*smin;
+}
+
+// MPM_DRIVER::Reduce_Dt (MPM_DRIVER.cpp:519-545) with Limit_Dt_Sound_Speed
+// (:624-637) and Compute_Max_Sound_Speed (:893-918): the sound speed of each
+// simulated particle from the singular values of its F at the start of the
+// step (MPM_FINITE_ELEMENTS::Precompute), density m / V0.  When the new dt is
+// below 0.99 dt, the grid velocities on valid nodes are scaled toward
+// velocity_save so the step's changes shrink with it.  Returns the dt G2P uses.
+template<class T,int d> T Reduce_Dt(const Scene<T,d>& s,const Particles<T,d>& P,Grid<T,d>& G,
+    const std::vector<Weights<T,d> >& W,const std::vector<long long>& sim)
+{
+    T dt=s.dt;
+    if(s.strong_cfl){
+        T new_dt=Grid_To_Particle_Limit_Dt(s,P,G,W,sim);
+        if(new_dt<dt) dt=new_dt;}
+    T dts=s.dt;
+    if(s.sound_cfl){
+        T max_speed=0;
+        for(long long p:sim){
+            Mat<T,d> U,V;Vec<T,d> sig;
+            Svd(P.f(p),U,sig,V);
+            T density=P.mass[p]/P.volume[p];
+            max_speed=std::max(max_speed,Sound_Speed(s.model,sig,P.mu[p],P.lambda[p],s.neo_threshold,density));}
+        T dxmin=s.dX[0];for(int a=1;a<d;a++) dxmin=std::min(dxmin,s.dX[a]);
+        T new_dt=Robust_Divide(dxmin,max_speed)*s.cfl_sound;
+        dts=std::min(dts,new_dt);}
+    dt=std::min(dt,dts);
+    dt=std::max(dt,s.min_dt);
+    if(dt>=s.dt*(T)0.99) return s.dt;
+    T sc=dt/s.dt;
+    for(long long j:G.valid){
+        Vec<T,d> vs=G.velocity_save[j];
+        G.velocity[j]=(G.velocity[j]-vs)*sc+vs;
+        G.velocity_friction_save[j]=(G.velocity_friction_save[j]-vs)*sc+vs;}
+    return dt;
+}
+
+// One Advance_One_Time_Step; returns the dt it took (s.dt unless Reduce_Dt lowered it).
+template<class T,int d> T Step(const Scene<T,d>& s,Particles<T,d>& P,Grid<T,d>& G,T time)
 {
     const T dt=s.dt;
     // Update_Simulated_Particles (MPM_DRIVER.cpp:1142-1157)
@@ -603,6 +769,8 @@ template<class T,int d> void Step(const Scene<T,d>& s,Particles<T,d>& P,Grid<T,d
         G.velocity_friction_save[j]=G.velocity_friction_save[j]+G.v0[j];}
     for(long long j:stuck) G.velocity_friction_save[j]=Vec<T,d>();
 
+    const T dtg=(s.strong_cfl || s.sound_cfl)?Reduce_Dt(s,P,G,W,sim):dt;
+
     // Grid_To_Particle (MPM_DRIVER.cpp:450-515) + Reflect_Or_Invalidate_Particle (:1219-1239)
     Vec<T,d> A=s.lo,Bd=s.hi;
     Vec<T,d> eps=(std::numeric_limits<T>::epsilon()*(T)4)*(Bd-A);
@@ -620,13 +788,13 @@ template<class T,int d> void Step(const Scene<T,d>& s,Particles<T,d>& P,Grid<T,d
             V_pic_fric=V_pic_fric+w*V_fric;
             grad=grad+Outer(V_new,g);
             Vec<T,d> Z=Node_Center(s,f);
-            Vec<T,d> xi_new=Z+dt*V_new;
+            Vec<T,d> xi_new=Z+dtg*V_new;
             Bh=Bh+(w/2)*(Outer(V_fric,Z+xi_new)+Outer(Z-xi_new,V_fric));}
-        Mat<T,d> Am=dt*grad;
+        Mat<T,d> Am=dtg*grad;
         for(int i=0;i<d;i++) Am(i,i)+=1;
         Mat<T,d> Fn=Am*P.f(p);
         Vec<T,d> Xp=P.x(p);
-        Vec<T,d> xp_new=Xp+dt*V_pic;
+        Vec<T,d> xp_new=Xp+dtg*V_pic;
         Mat<T,d> corr=(T).5*(Outer(V_pic_fric,Xp+xp_new)+Outer(Xp-xp_new,V_pic_fric));
         for(int q=0;q<d*d;q++) Bh.x[q]-=corr.x[q];
         Vec<T,d> V_flip=(P.v(p)+V_pic_fric)-V_weight_old;
@@ -636,6 +804,7 @@ template<class T,int d> void Step(const Scene<T,d>& s,Particles<T,d>& P,Grid<T,d
         for(int a=0;a<d;a++){
             if(P.X[p*d+a]<A[a]) P.valid[p]=0;
             else if(P.X[p*d+a]>Bd[a]) P.valid[p]=0;}}
+    return dtg;
 }
 
 template<class T,int d> Scene<T,d> Make_Scene(const int* N,const double* lo,const double* hi,const double* g,int model,double thr,double dt,
@@ -666,7 +835,7 @@ template<class T,int d> int Run(const int* N,const double* lo,const double* hi,c
     Particles<T,d> P{np,X,V,F,B,mass,volume,mu,la,valid};
     Grid<T,d> G;
     T time=0;
-    for(int k=0;k<nsteps;k++){Step(s,P,G,time);time+=s.dt;}
+    for(int k=0;k<nsteps;k++) time+=Step(s,P,G,time);
     if(nsteps>0){
         for(long long i=0;i<s.nn;i++){
             if(gmass) gmass[i]=G.mass[i];
@@ -858,8 +1027,8 @@ template<class T,int d> T Compute_Dt(const Scene<T,d>& s,const Particles<T,d>& P
 }
 
 // The substep loop of MPM_DRIVER::Simulate_To_Frame (MPM_DRIVER.cpp:173-197)
-// for one frame, without Reduce_Dt (strong and sound-speed CFL off, their
-// defaults MPM_EXAMPLE.h:103-104).  Writes the substep dts.
+// for one frame.  When Reduce_Dt lowers a substep's dt, time advances by the
+// dt taken and the frame goes on (:190-194).  Writes the substep dts taken.
 template<class T,int d> int Advance_Frame(Scene<T,d>& s,Particles<T,d>& P,Grid<T,d>& G,T& time,T frame_dt,T cfl,
     T min_dt,T max_dt,T* dts,int max_sub)
 {
@@ -875,10 +1044,33 @@ template<class T,int d> int Advance_Frame(Scene<T,d>& s,Particles<T,d>& P,Grid<T
         else if(next_time+dt>time_at_frame) next_time=(time+time_at_frame)/2;
         dt=next_time-time;
         s.dt=dt;
-        Step(s,P,G,time);
+        T taken=Step(s,P,G,time);
+        if(taken<dt){next_time=time+taken;done=false;}
         time=next_time;
-        dts[n++]=dt;}
+        dts[n++]=taken;}
     return n;
+}
+// lim = {use_strong_cfl, use_sound_speed_cfl, cfl_F, cfl_sound} (null: both off).
+template<class T,int d> void Set_Limits(Scene<T,d>& s,const double* lim,double cfl,double min_dt)
+{
+    s.cfl=(T)cfl;s.min_dt=(T)min_dt;
+    if(!lim) return;
+    s.strong_cfl=lim[0]!=0;s.sound_cfl=lim[1]!=0;s.cfl_F=(T)lim[2];s.cfl_sound=(T)lim[3];
+}
+
+// Fixed-dt steps with Reduce_Dt: dts[k] is the dt step k took.
+template<class T,int d> int Run_Limited(const int* N,const double* lo,const double* hi,const double* g,int model,double thr,
+    double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi,
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,const T* mu,const T* la,unsigned char* valid,int nsteps,
+    const double* lim,double cfl,double min_dt,double* dts)
+{
+    Scene<T,d> s=Make_Scene<T,d>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi);
+    Set_Limits(s,lim,cfl,min_dt);
+    Particles<T,d> P{np,X,V,F,B,mass,volume,mu,la,valid};
+    Grid<T,d> G;
+    T time=0;
+    for(int k=0;k<nsteps;k++){T taken=Step(s,P,G,time);time+=taken;dts[k]=(double)taken;}
+    return 0;
 }
 }  // namespace oracle
 
@@ -898,11 +1090,12 @@ ORACLE_API_F int oracle_grid_v_bound_##SUF(int dim,const int* N,const double* lo
 ORACLE_API_F int oracle_advance_frames_##SUF(int dim,const int* N,const double* lo,const double* hi,const double* g,int model, \
     double thr,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi, \
     long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,const T* mu,const T* la,unsigned char* valid, \
-    int nframes,double frame_dt,double cfl,double min_dt,double max_dt,double* dts,int max_sub) \
+    int nframes,double frame_dt,double cfl,double min_dt,double max_dt,double* dts,int max_sub,const double* lim) \
 { \
     int total=0; \
     if(dim==2){ \
         Scene<T,2> s=Make_Scene<T,2>(N,lo,hi,g,model,thr,max_dt,nwalls,wtype,wfric,wlo,whi); \
+        Set_Limits(s,lim,cfl,min_dt); \
         Particles<T,2> P{np,X,V,F,B,mass,volume,mu,la,valid};Grid<T,2> G;T time=0;std::vector<T> d(max_sub); \
         for(int f=0;f<nframes;f++){int n=Advance_Frame(s,P,G,time,(T)frame_dt,(T)cfl,(T)min_dt,(T)max_dt,d.data(),max_sub-total); \
             if(n<0) return -1; \
@@ -910,6 +1103,7 @@ ORACLE_API_F int oracle_advance_frames_##SUF(int dim,const int* N,const double* 
             total+=n;}} \
     else if(dim==3){ \
         Scene<T,3> s=Make_Scene<T,3>(N,lo,hi,g,model,thr,max_dt,nwalls,wtype,wfric,wlo,whi); \
+        Set_Limits(s,lim,cfl,min_dt); \
         Particles<T,3> P{np,X,V,F,B,mass,volume,mu,la,valid};Grid<T,3> G;T time=0;std::vector<T> d(max_sub); \
         for(int f=0;f<nframes;f++){int n=Advance_Frame(s,P,G,time,(T)frame_dt,(T)cfl,(T)min_dt,(T)max_dt,d.data(),max_sub-total); \
             if(n<0) return -1; \
@@ -969,6 +1163,38 @@ ORACLE_API int oracle_run_drucker_prager_##SUF(int dim,const int* N,const double
 }
 DEFINE_DP(f64,double)
 DEFINE_DP(f32,float)
+
+// lim[4] = use_strong_cfl, use_sound_speed_cfl, cfl_F, cfl_sound.
+#define DEFINE_LIMITED(SUF,T) \
+ORACLE_API int oracle_run_limited_##SUF(int dim,const int* N,const double* lo,const double* hi,const double* g, \
+    int model,double thr,double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi, \
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,const T* mu,const T* la,unsigned char* valid,int nsteps, \
+    const double* lim,double cfl,double min_dt,double* dts) \
+{ \
+    if(dim==2) return Run_Limited<T,2>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume,mu,la, \
+        valid,nsteps,lim,cfl,min_dt,dts); \
+    if(dim==3) return Run_Limited<T,3>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume,mu,la, \
+        valid,nsteps,lim,cfl,min_dt,dts); \
+    return -1; \
+}
+DEFINE_LIMITED(f64,double)
+DEFINE_LIMITED(f32,float)
+
+// Sound_Speed of each particle from the singular values of its F (n x d*d).
+#define DEFINE_SOUND(SUF,T) \
+ORACLE_API int oracle_sound_speeds_##SUF(int dim,int model,double thr,long long n,const T* F,const T* mass, \
+    const T* volume,const T* mu,const T* la,T* out) \
+{ \
+    for(long long p=0;p<n;p++){ \
+        if(dim==2){Mat<T,2> A,U,V;Vec<T,2> s;for(int k=0;k<4;k++) A.x[k]=F[p*4+k];Svd(A,U,s,V); \
+            out[p]=Sound_Speed<T,2>(model,s,mu[p],la[p],(T)thr,mass[p]/volume[p]);} \
+        else if(dim==3){Mat<T,3> A,U,V;Vec<T,3> s;for(int k=0;k<9;k++) A.x[k]=F[p*9+k];Svd(A,U,s,V); \
+            out[p]=Sound_Speed<T,3>(model,s,mu[p],la[p],(T)thr,mass[p]/volume[p]);} \
+        else return -1;} \
+    return 0; \
+}
+DEFINE_SOUND(f64,double)
+DEFINE_SOUND(f32,float)
 
 template<class T,int d> static void Svd_Many(long long n,const T* A,T* U,T* S,T* V,int* iters)
 {

@@ -11,6 +11,10 @@
 //   ref_harness svd   <prec> <dim> <in.bin> <out.bin>
 //   ref_harness pstrain <prec> <dim> <model> <threshold> <in.bin> <out.bin>
 //   ref_harness frames <scene.bin> <nframes> <frame_dt> <cfl> <min_dt> <max_dt> <out.bin>
+//               [<strong_cfl> <sound_cfl> <cfl_F> <cfl_sound>]
+//   ref_harness limited <scene.bin> <strong_cfl> <sound_cfl> <cfl> <cfl_F> <cfl_sound> <min_dt> <out.bin>
+//     (the scene's nsteps fixed-dt steps with MPM_DRIVER::Reduce_Dt: the dt
+//     each step took and the final state, in the frames output layout)
 // A scene with plasticity flag 1 adds MPM_PLASTICITY_CLAMP (Fp, mu0, lambda0
 // stored), flag 2 MPM_DRUCKER_PRAGER; their dumps end with Fp, mu and lambda
 // (and for flag 2 plastic_def, rho_F, sigma_Y, vc).  Model 2 is the Hencky
@@ -274,7 +278,7 @@ int Run_Step(const SCENE& s,const char* out,STREAM_TYPE st)
 // frame output.  Writes the substep dts and the final state.
 template<class TV>
 int Run_Frames(const SCENE& s,int nframes,double frame_dt,double cfl,double min_dt,double max_dt,const char* out,
-    STREAM_TYPE st)
+    STREAM_TYPE st,const double* lim)
 {
     typedef typename TV::SCALAR T;
 #ifdef USE_OPENMP
@@ -283,6 +287,7 @@ int Run_Frames(const SCENE& s,int nframes,double frame_dt,double cfl,double min_
     RUN<TV> run(s,st);
     auto& ex=run.ex;
     ex.frame_dt=(T)frame_dt;ex.cfl=(T)cfl;ex.min_dt=(T)min_dt;ex.max_dt=(T)max_dt;
+    if(lim){ex.use_strong_cfl=lim[0]!=0;ex.use_sound_speed_cfl=lim[1]!=0;ex.cfl_F=(T)lim[2];ex.cfl_sound=(T)lim[3];}
     std::vector<double> dts;
     for(int f=0;f<nframes;f++){
         T time_at_frame=ex.time+ex.frame_dt;
@@ -294,9 +299,34 @@ int Run_Frames(const SCENE& s,int nframes,double frame_dt,double cfl,double min_
             if(next_time+(T).0001*ex.dt>time_at_frame){next_time=time_at_frame;done=true;}
             else if(next_time+ex.dt>time_at_frame) next_time=(ex.time+time_at_frame)/2;
             ex.dt=next_time-ex.time;
+            T original_dt=ex.dt;
             run.driver->Advance_One_Time_Step();
+            if(ex.dt<original_dt){next_time=ex.time+ex.dt;done=false;}  // Reduce_Dt lowered it
             dts.push_back((double)ex.dt);
             ex.time=next_time;}}
+    WRITER w(out);
+    w.put("MPMFRM1",8);
+    int nd=(int)dts.size(),np=(int)s.np,nn=run.ex.mass.array.m;
+    w.put(nd);w.put(s.dim);w.put(np);w.put(nn);
+    w.put(dts.data(),dts.size());
+    run.Dump(w,nd);
+    return 0;
+}
+
+// Fixed-dt steps with Reduce_Dt (use_strong_cfl, use_sound_speed_cfl).
+template<class TV>
+int Run_Limited(const SCENE& s,const double* lim,double cfl,double min_dt,const char* out,STREAM_TYPE st)
+{
+    typedef typename TV::SCALAR T;
+#ifdef USE_OPENMP
+    omp_set_num_threads(s.threads>0?s.threads:1);
+#endif
+    RUN<TV> run(s,st);
+    auto& ex=run.ex;
+    ex.use_strong_cfl=lim[0]!=0;ex.use_sound_speed_cfl=lim[1]!=0;ex.cfl_F=(T)lim[2];ex.cfl_sound=(T)lim[3];
+    ex.cfl=(T)cfl;ex.min_dt=(T)min_dt;
+    std::vector<double> dts;
+    for(int k=0;k<s.nsteps;k++){run.Step();dts.push_back((double)ex.dt);}
     WRITER w(out);
     w.put("MPMFRM1",8);
     int nd=(int)dts.size(),np=(int)s.np,nn=run.ex.mass.array.m;
@@ -415,18 +445,36 @@ int main(int argc,char* argv[])
             else rc=mode=="step"?Run_Step<VECTOR<float,3> >(s,argv[3],st):Run_Time<VECTOR<float,3> >(s,atoi(argv[3]),atoi(argv[4]),atoi(argv[5]),st);}
         LOG::Finish_Logging();
         return rc;}
-    if(mode=="frames" && argc==9){
+    if(mode=="frames" && (argc==9 || argc==13)){
         SCENE s=Read_Scene(argv[2]);
         int nf=atoi(argv[3]);double fdt=atof(argv[4]),cfl=atof(argv[5]),mind=atof(argv[6]),maxd=atof(argv[7]);
+        double limv[4];
+        const double* lim=0;
+        if(argc==13){for(int i=0;i<4;i++) limv[i]=atof(argv[9+i]);lim=limv;}
         int rc;
         if(s.prec==8){
             STREAM_TYPE st((double()));
-            rc=s.dim==2?Run_Frames<VECTOR<double,2> >(s,nf,fdt,cfl,mind,maxd,argv[8],st):
-                Run_Frames<VECTOR<double,3> >(s,nf,fdt,cfl,mind,maxd,argv[8],st);}
+            rc=s.dim==2?Run_Frames<VECTOR<double,2> >(s,nf,fdt,cfl,mind,maxd,argv[8],st,lim):
+                Run_Frames<VECTOR<double,3> >(s,nf,fdt,cfl,mind,maxd,argv[8],st,lim);}
         else{
             STREAM_TYPE st((float()));
-            rc=s.dim==2?Run_Frames<VECTOR<float,2> >(s,nf,fdt,cfl,mind,maxd,argv[8],st):
-                Run_Frames<VECTOR<float,3> >(s,nf,fdt,cfl,mind,maxd,argv[8],st);}
+            rc=s.dim==2?Run_Frames<VECTOR<float,2> >(s,nf,fdt,cfl,mind,maxd,argv[8],st,lim):
+                Run_Frames<VECTOR<float,3> >(s,nf,fdt,cfl,mind,maxd,argv[8],st,lim);}
+        LOG::Finish_Logging();
+        return rc;}
+    if(mode=="limited" && argc==10){
+        SCENE s=Read_Scene(argv[2]);
+        double lim[4]={atof(argv[3]),atof(argv[4]),atof(argv[6]),atof(argv[7])};
+        double cfl=atof(argv[5]),mind=atof(argv[8]);
+        int rc;
+        if(s.prec==8){
+            STREAM_TYPE st((double()));
+            rc=s.dim==2?Run_Limited<VECTOR<double,2> >(s,lim,cfl,mind,argv[9],st):
+                Run_Limited<VECTOR<double,3> >(s,lim,cfl,mind,argv[9],st);}
+        else{
+            STREAM_TYPE st((float()));
+            rc=s.dim==2?Run_Limited<VECTOR<float,2> >(s,lim,cfl,mind,argv[9],st):
+                Run_Limited<VECTOR<float,3> >(s,lim,cfl,mind,argv[9],st);}
         LOG::Finish_Logging();
         return rc;}
     if(mode=="svd" && argc==6){

@@ -197,6 +197,22 @@ int mpmx_get_plastic_state(mpmx_ctx* ctx, void* Fp, void* mu, void* lambda) {
   return guard([&] { ctx->engine->get_plastic_state(Fp, mu, lambda); });
 }
 
+int mpmx_set_dt_limits(mpmx_ctx* ctx, int strong_cfl, int sound_cfl, double cfl, double cfl_F, double cfl_sound,
+                       double min_dt) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->set_dt_limits(strong_cfl != 0, sound_cfl != 0, cfl, cfl_F, cfl_sound, min_dt); });
+}
+
+int mpmx_max_sound_speed(mpmx_ctx* ctx, double* out) {
+  if (!ctx || !out) return bad("null argument");
+  return guard([&] { ctx->engine->max_sound_speed(out); });
+}
+
+int mpmx_last_dt(mpmx_ctx* ctx, double* out) {
+  if (!ctx || !out) return bad("null argument");
+  return guard([&] { *out = ctx->engine->last_dt(); });
+}
+
 int mpmx_set_drucker_prager(mpmx_ctx* ctx, int enable, double a0, double a1, double a3, double a4, double cohesion,
                             int volume_correction) {
   if (!ctx) return bad("null context");

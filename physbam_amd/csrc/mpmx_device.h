@@ -333,6 +333,119 @@ MPMX_HD void p_hat(int model, T mu, T la, T thr, T root, const T* s, T* P) {
   for (int i = 0; i < D; i++) P[i] = mu * s[i] + c * (Fi[i] * Fi[i] * dF[i] - Fi[i]) + la * ip * Fi[i];
 }
 
+// Sound speed of a particle (ISOTROPIC_CONSTITUTIVE_MODEL::Sound_Speed,
+// ISOTROPIC_CONSTITUTIVE_MODEL.cpp:151-162) from its singular values s:
+// sqrt(max(H_ii s_i^2, B_a s_{(a+1)%m}^2, B_a s_{(a+2)%m}^2) / density) with
+// the diagonal of H and the spin entries B of Isotropic_Stress_Derivative
+// (m = 1 in 2D, 3 in 3D):
+//   fixed corotated  COROTATED_FIXED.cpp:127-140, :144-167 (panic threshold 1e-6, :22)
+//   neo-Hookean      NEO_HOOKEAN.cpp:143-156, :161-180 (F clamped to the hyperbola below thr)
+//   Hencky           ST_VENANT_KIRCHHOFF_HENCKY_STRAIN.cpp:85-102, :106-137 (failure threshold 0)
+template<typename T> MPMX_HD T sound_panic(T d) {
+  const T pt = (T)1e-6;
+  return tabs(d) < pt ? (d < 0 ? -pt : pt) : d;
+}
+// (log x - log y) / (x - y) as Core/Math_Tools/Robust_Functions.h:33-39 forms it.
+template<typename T> MPMX_HD T diff_log_over_diff(T x, T y) {
+  const T z = (y - x) / x;
+  const T r = tabs(z) < (T)1e-18 ? (T)1 : (T)log1p(z) / z;
+  return r / x;
+}
+template<typename T, int D>
+MPMX_HD T sound_speed(int model, T mu, T la, T thr, T root, const T* F, T density) {
+  constexpr int M = D == 3 ? 3 : 1;
+  T H[D], B[M];
+  if (model == 0) {
+    const T mu2 = 2 * mu;
+    if constexpr (D == 3) {
+      const T cof[3] = {F[1] * F[2], F[2] * F[0], F[1] * F[0]};
+#pragma unroll
+      for (int a = 0; a < 3; a++) H[a] = mu2 + la * (cof[a] * cof[a]);
+      B[2] = mu2 * (1 - 1 / sound_panic(F[0] + F[1]));
+      B[1] = mu2 * (1 - 1 / sound_panic(F[0] + F[2]));
+      B[0] = mu2 * (1 - 1 / sound_panic(F[1] + F[2]));
+    } else {
+      H[0] = mu2 + la * (F[1] * F[1]);
+      H[1] = mu2 + la * (F[0] * F[0]);
+      B[0] = mu2 * (1 - 1 / sound_panic(F[0] + F[1]));
+    }
+  } else if (model == 1) {
+    T J = F[0];
+#pragma unroll
+    for (int a = 1; a < D; a++) J *= F[a];
+    T Fc[D];
+    if (J >= thr) {
+#pragma unroll
+      for (int a = 0; a < D; a++) Fc[a] = F[a];
+    } else if constexpr (D == 3) {
+      if (F[0] * F[1] * F[1] > thr) { Fc[0] = F[0]; Fc[1] = F[1]; Fc[2] = thr / (F[0] * F[1]); }
+      else if (F[0] * F[0] * F[0] > thr) { const T q = tsqrt(thr / F[0]); Fc[0] = F[0]; Fc[1] = q; Fc[2] = q; }
+      else { Fc[0] = root; Fc[1] = root; Fc[2] = root; }
+    } else {
+      if (F[0] * F[0] > thr) { Fc[0] = F[0]; Fc[1] = thr / F[0]; }
+      else { Fc[0] = root; Fc[1] = root; }
+    }
+    T Fi[D], ji = 1;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      Fi[a] = 1 / Fc[a];
+      ji = a == 0 ? Fi[0] : ji * Fi[a];
+    }
+    const T mlj = mu + la * tlog<T>(ji);
+#pragma unroll
+    for (int a = 0; a < D; a++) H[a] = mu + (la + mlj) * (Fi[a] * Fi[a]);
+#pragma unroll
+    for (int a = 0; a < M; a++) B[a] = mu;
+  } else {
+    if (!(F[D - 1] > 0)) return 0;  // every entry of the derivative is zero
+    T sv[D], l[D], sum_log = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      sv[a] = F[a] > 0 ? F[a] : (T)0;
+      l[a] = tlog<T>(sv[a]);
+      sum_log += l[a];
+    }
+    const T lp2m = la + 2 * mu, two_mu = 2 * mu;
+    if constexpr (D == 3) {
+      T row[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) row[i] = sv[i] * l[0] + sv[i] * l[1] + sv[i] * l[2];
+#pragma unroll
+      for (int a = 0; a < 3; a++) H[a] = (lp2m - two_mu * l[a] - la * sum_log) / (sv[a] * sv[a]);
+      const int pi[3] = {1, 0, 0}, pj[3] = {2, 2, 1};  // B(2): (0,1), B(1): (0,2), B(0): (1,2)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const int i = pi[k], j = pj[k];
+        const T xij = sv[j] * sv[i];
+        const T mij = (two_mu / sv[i]) * (diff_log_over_diff(sv[i], sv[j]) - l[j] / sv[j]) - la * sum_log / xij;
+        const T pij = (two_mu * (sv[j] * l[i] + sv[i] * l[j]) + la * (row[i] + row[j])) / (xij * (sv[i] + sv[j]));
+        B[k] = (T)0.5 * (mij + pij);
+      }
+    } else {
+      const T s0 = sv[0], s1 = sv[1], x10 = s1 * s0;
+      const T m01 = (two_mu / s0) * (diff_log_over_diff(s0, s1) - l[1] / s1) - la * sum_log / x10;
+      const T p01 = ((la * s0 + lp2m * s1) * l[0] + (la * s1 + lp2m * s0) * l[1]) / (x10 * (s0 + s1));
+      H[0] = (lp2m * (1 - l[0]) - la * l[1]) / (s0 * s0);
+      H[1] = (lp2m * (1 - l[1]) - la * l[0]) / (s1 * s1);
+      B[0] = (T)0.5 * (m01 + p01);
+    }
+  }
+  T F2[D];
+#pragma unroll
+  for (int a = 0; a < D; a++) F2[a] = F[a] * F[a];
+  T cm = H[0] * F2[0];
+#pragma unroll
+  for (int a = 1; a < D; a++) cm = H[a] * F2[a] > cm ? H[a] * F2[a] : cm;
+#pragma unroll
+  for (int a = 0; a < M; a++)
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const T c = B[a] * F2[(a + i + 1) % M];
+      cm = c > cm ? c : cm;
+    }
+  return tsqrt(cm / density);
+}
+
 // tau = vol * P F^T = vol * U diag(P_hat_i * s_i) U^T, symmetric; returned as a
 // full column-major D x D matrix.  The FP64 path is used for near-degenerate F
 // (singular values far apart in magnitude or the neo-Hookean clamp branch).

@@ -43,6 +43,9 @@ struct EngineBase {
                               int64_t capacity, int64_t* nsub) = 0;
   virtual void set_plasticity(bool enable, double theta_c, double theta_s, double max_hardening, double xi) = 0;
   virtual void get_plastic_state(void* Fp, void* mu, void* lambda) = 0;
+  virtual void set_dt_limits(bool strong, bool sound, double cfl, double cfl_F, double cfl_sound, double min_dt) = 0;
+  virtual void max_sound_speed(double* out) = 0;
+  virtual double last_dt() const = 0;
   virtual void set_drucker_prager(bool enable, double a0, double a1, double a3, double a4, double cohesion,
                                   bool volume_correction) = 0;
   virtual void get_drucker_prager_state(void* plastic_def, void* rho_F, void* cohesion, void* vc) = 0;

@@ -2433,6 +2433,186 @@ __global__ __launch_bounds__(BLOCK) void svd_kernel(uint32_t n, const T* F, T* U
 // ------------------------------------------------------------------ host ---
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// ---------------------------------------------------------- Reduce_Dt ----
+// MPM_DRIVER::Reduce_Dt (MPM_DRIVER.cpp:519-545) between the grid update and
+// G2P: the strong CFL (Grid_To_Particle_Limit_Dt, :572-620, without the
+// midpoint rule) and the sound-speed CFL (Limit_Dt_Sound_Speed, :624-637,
+// Compute_Max_Sound_Speed, :893-918).  velocity_save, the P2G velocity before
+// forces, comes from a second P2G of the step with dt = 0 (no stress impulse)
+// into its own grid, normalised by the node mass.
+
+// Enforce_Limit_Max (MPM_DRIVER.cpp:549-559): lower s so that
+// |t (a + t b)| <= bound for every 0 <= t <= s.
+template<typename T> __device__ __forceinline__ void enforce_limit_max(T& s, T bound, T a, T b) {
+  if (a < 0) {
+    a = -a;
+    b = -b;
+  }
+  if (a * a + 4 * b * bound >= 0) {
+    if (s * (a + s * b) > bound) s = (2 * bound) / (a + sqrt(a * a + 4 * b * bound));
+  } else {
+    if (s * (a + s * b) < -bound) s = -(a + sqrt(a * a - 4 * b * bound)) / (2 * b);
+  }
+}
+
+// v <- m v / m on the nodes of the listed tiles that hold mass.
+template<typename T>
+__global__ __launch_bounds__(BLOCK) void normalize_nodes_kernel(Node4<T>* g, const uint32_t* tiles,
+                                                                const uint32_t* count) {
+  const uint64_t total = (uint64_t)count[0] * 64u;
+  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total; idx += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t slot = tiles[idx >> 6] * 64u + (uint32_t)(idx & 63u);
+    Node4<T> nd = g[slot];
+    if (nd.m == (T)0) continue;
+    const T im = (T)1 / nd.m;
+    nd.x *= im;
+    nd.y *= im;
+    nd.z *= im;
+    g[slot] = nd;
+  }
+}
+
+// Reduce_Dt's rescale on the valid nodes: v = (v - vs) s + vs, and the same
+// for the friction velocity.
+template<typename T>
+__global__ __launch_bounds__(BLOCK) void rescale_nodes_kernel(Node4<T>* grid, Node4<T>* gridf, const Node4<T>* vsave,
+                                                              const uint32_t* tiles, const uint32_t* count, T sc) {
+  const uint64_t total = (uint64_t)count[0] * 64u;
+  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total; idx += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t slot = tiles[idx >> 6] * 64u + (uint32_t)(idx & 63u);
+    const Node4<T> vs = vsave[slot];
+    if (vs.m == (T)0) continue;
+    Node4<T> v = grid[slot];
+    v.x = (v.x - vs.x) * sc + vs.x;
+    v.y = (v.y - vs.y) * sc + vs.y;
+    v.z = (v.z - vs.z) * sc + vs.z;
+    grid[slot] = v;
+    if (gridf) {
+      Node4<T> f = gridf[slot];
+      f.x = (f.x - vs.x) * sc + vs.x;
+      f.y = (f.y - vs.y) * sc + vs.y;
+      f.z = (f.z - vs.z) * sc + vs.z;
+      gridf[slot] = f;
+    }
+  }
+}
+
+template<typename T> __global__ void dt_limit_init_kernel(typename OrderedBits<T>::U* out) {
+  out[0] = OrderedBits<T>::of((T)0);
+  out[1] = OrderedBits<T>::of((T)100);
+}
+
+// Over the valid particles: out[0] = max sound speed (from the singular values
+// of F at the start of the step, density m / V0), out[1] = min over particles
+// of the strong-CFL factor s (starting at 100).  Both hold bit patterns of
+// non-negative values, so integer max/min order them.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void dt_limit_kernel(Args<T, D> a, const Node4<T>* vsave, int strong, int sound,
+                                                         T cfl, T cfl_F, typename OrderedBits<T>::U* out) {
+  T cmax = 0, smin = 100;
+  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < a.n; p += gridDim.x * BLOCK) {
+    if (!a.valid[p]) continue;
+    if (sound) {
+      T Fm[D * D], U[D * D], S[D], V[D * D];
+#pragma unroll
+      for (int k = 0; k < D * D; k++) Fm[k] = a.F[(size_t)k * a.n + p];
+      svd<T, D>(Fm, U, S, V);
+      const T c = sound_speed<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, S, a.mass[p] / a.vol[p]);
+      if (c > cmax) cmax = c;
+    }
+    if (strong) {
+      const Geom<T, D> g = particle_geom(a, p);
+      T x[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) x[c] = a.X[(size_t)c * a.n + p];
+      Wts<T, D> w;
+      weights<T, D>(g, x, w);
+      bool in = true;
+#pragma unroll
+      for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < g.G[c];
+      if (!in) continue;  // flagged by the P2G side path
+      T vp[D], vd[D], G[D * D], Gd[D * D];
+#pragma unroll
+      for (int c = 0; c < D; c++) vp[c] = vd[c] = 0;
+#pragma unroll
+      for (int k = 0; k < D * D; k++) G[k] = Gd[k] = 0;
+      constexpr int K = D == 3 ? 27 : 9;
+#pragma unroll 1
+      for (int k = 0; k < K; k++) {
+        int o[D], r = k;
+#pragma unroll
+        for (int c = D - 1; c >= 0; c--) {
+          o[c] = r % 3;
+          r /= 3;
+        }
+        T wt = w.w[o[0]][0], gr[D];
+#pragma unroll
+        for (int c = 1; c < D; c++) wt *= w.w[o[c]][c];
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          T q = 1;
+#pragma unroll
+          for (int j = 0; j < D; j++) q *= i == j ? w.dw[o[j]][j] : w.w[o[j]][j];
+          gr[i] = q;
+        }
+        int nc[D];
+#pragma unroll
+        for (int c = 0; c < D; c++) nc[c] = w.base[c] + o[c];
+        const uint32_t slot = node_slot<T, D>(nc, g);
+        const Node4<T> nn = a.grid[slot], no = vsave[slot];
+        const T vn[3] = {nn.x, nn.y, nn.z}, vo[3] = {no.x, no.y, no.z};
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          const T dv = vn[i] - vo[i];
+          vp[i] += wt * vo[i];
+          vd[i] += wt * dv;
+#pragma unroll
+          for (int j = 0; j < D; j++) {
+            G[i + D * j] += vo[i] * gr[j];
+            Gd[i + D * j] += dv * gr[j];
+          }
+        }
+      }
+      T sp = 100;
+#pragma unroll
+      for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) enforce_limit_max<T>(sp, cfl_F, a.dt * G[i + D * j], a.dt * Gd[i + D * j]);
+#pragma unroll
+      for (int i = 0; i < D; i++) enforce_limit_max<T>(sp, cfl, a.dt * vp[i], a.dt * vd[i]);
+      smin = sp < smin ? sp : smin;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const T c = __shfl_xor(cmax, o, 64), q = __shfl_xor(smin, o, 64);
+    cmax = c > cmax ? c : cmax;
+    smin = q < smin ? q : smin;
+  }
+  __shared__ T s_c[BLOCK / 64], s_s[BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) {
+    s_c[threadIdx.x >> 6] = cmax;
+    s_s[threadIdx.x >> 6] = smin;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < BLOCK / 64; q++) {
+      cmax = s_c[q] > cmax ? s_c[q] : cmax;
+      smin = s_s[q] < smin ? s_s[q] : smin;
+    }
+    if (cmax > 0) atomicMax(out, OrderedBits<T>::of(cmax));
+    if (smin < 100) atomicMin(out + 1, OrderedBits<T>::of(smin));
+  }
+}
+
+// Core/Math_Tools/Robust_Arithmetic.h:22-26 (FLT_MAX in T for both precisions).
+template<typename T> T robust_divide(T a, T b) {
+  const T fm = (T)std::numeric_limits<float>::max();
+  const T aa = std::fabs(a), ab = std::fabs(b);
+  if (ab == fm) return T();
+  if (ab > 1 || aa < fm * ab) return a / b;
+  return (a > 0) == (b >= 0) ? fm : -fm;
+}
+
 template<typename X> struct DevBuf {
   X* p = nullptr;
   size_t count = 0;
@@ -2618,6 +2798,7 @@ class Engine final : public EngineBase {
       if (rb_ev_[i]) (void)hipEventDestroy(rb_ev_[i]);
     if (rb_host_) (void)hipHostFree(rb_host_);
     if (vb_host_) (void)hipHostFree(vb_host_);
+    if (lim_host_) (void)hipHostFree(lim_host_);
   }
 
   int dim() const override { return D; }
@@ -2734,7 +2915,7 @@ class Engine final : public EngineBase {
     if (nsteps < 0) throw Error{MPMX_EINVAL, "negative step count"};
     // recorded steps too (the reverse pass recomputes its own); plasticity
     // projects F between G2P and the next P2G, which the fused kernel does not
-    const bool fuse = n_ > 0 && !plastic_;
+    const bool fuse = n_ > 0 && !plastic_ && !limits_on();
     for (int s = 0; s < nsteps; s++) {
       if (fuse)
         step_fused((T)dt);
@@ -2829,7 +3010,8 @@ class Engine final : public EngineBase {
   // The substep loop of MPM_DRIVER::Simulate_To_Frame in T: dt from
   // Compute_Dt, clamped to [min_dt, max_dt] (Core/Math_Tools/clamp.h:13-16),
   // the last substep of a frame lands on the frame time and the one before it
-  // splits the remainder in half.
+  // splits the remainder in half.  With the strong or sound-speed CFL on,
+  // Reduce_Dt uses this cfl and min_dt.
   void advance_frames(int nframes, double frame_dt, double cfl, double min_dt, double max_dt, double* dts,
                       int64_t capacity, int64_t* nsub) override {
     if (scenes_.size() > 1) throw Error{MPMX_ESTATE, "adaptive frames take a single-scene context"};
@@ -2856,10 +3038,14 @@ class Engine final : public EngineBase {
         }
         dt = next_time - time;
         if (!(dt > 0)) throw Error{MPMX_ESTATE, "substep dt is not positive (raise min_dt)"};
-        step_once(dt);
+        const T taken = step_once(dt, tcfl, tmin);
+        if (taken < dt) {  // Reduce_Dt lowered it: the frame goes on (MPM_DRIVER.cpp:190-194)
+          next_time = time + taken;
+          done = false;
+        }
         time = next_time;
         time_ = (double)time;
-        if (dts && count < capacity) dts[count] = (double)dt;
+        if (dts && count < capacity) dts[count] = (double)taken;
         count++;
         if (nsub) *nsub = count;
       }
@@ -3031,6 +3217,7 @@ class Engine final : public EngineBase {
     if (any_friction()) throw Error{MPMX_EINVAL, "the adjoint step has no friction: remove wall friction"};
     if (plastic_) throw Error{MPMX_EINVAL, "the adjoint step has no plasticity"};
     if (model_ == 2) throw Error{MPMX_EINVAL, "the adjoint step has no Hencky-strain model"};
+    if (limits_on()) throw Error{MPMX_EINVAL, "the adjoint step has a fixed dt: turn the strong and sound CFL off"};
     if (force_reorder_ || force_rebin_) rebin();  // settle the storage order before recording
     particle_stats(nullptr, nullptr, &tape_valid_);
     fused_ = false;
@@ -3391,7 +3578,10 @@ class Engine final : public EngineBase {
     rebins_++;
   }
 
-  void step_once(T dt) {
+  // One unfused step; returns the dt it took (dt unless Reduce_Dt lowered it,
+  // with cfl and min_dt as example.cfl and example.min_dt).
+  T step_once(T dt) { return step_once(dt, lim_cfl_, lim_min_dt_); }
+  T step_once(T dt, T cfl, T min_dt) {
     fused_ = false;
     fused_last_ = false;
     const bool fric = any_friction();
@@ -3399,8 +3589,9 @@ class Engine final : public EngineBase {
     hipEvent_t e_step = timer_.begin(stream_);
     if (n_ == 0) {
       time_ += (double)dt;
+      last_dt_ = (double)dt;
       timer_.end(PH_STEP, e_step, stream_);
-      return;
+      return dt;
     }
     if (rebin_due()) rebin();
     hipEvent_t e0 = timer_.begin(stream_);
@@ -3429,6 +3620,11 @@ class Engine final : public EngineBase {
     if (stats_on_) reduce_partials_kernel<<<1, 64, 0, stream_>>>(stats_partial_.p, grid_blocks(), stats_out_.p);
     a.stats_partial = nullptr;
     timer_.end(PH_GRID, e2, stream_);
+    if (limits_on()) {
+      hipEvent_t e4 = timer_.begin(stream_);
+      a.dt = reduce_dt(a, fric, cfl, min_dt);
+      timer_.end(PH_DT, e4, stream_);
+    }
 
     hipEvent_t e3 = timer_.begin(stream_);
     if (plastic_) {  // the particles simulated in this step, before G2P invalidates any
@@ -3450,7 +3646,7 @@ class Engine final : public EngineBase {
     steps_since_rebin_++;
     since_reorder_++;
     if (taping_ && !replaying_) {
-      tape_dt_.push_back((double)dt);
+      tape_dt_.push_back((double)a.dt);
       if (tape_dt_.size() % (size_t)tape_k_ == 0) {
         ckpts_.emplace_back();
         save_state(ckpts_.back());
@@ -3458,9 +3654,91 @@ class Engine final : public EngineBase {
     }
     HIPCHECK(hipGetLastError());
     has_grid_ = true;
-    time_ += (double)dt;
+    time_ += (double)a.dt;
+    last_dt_ = (double)a.dt;
     timer_.end(PH_STEP, e_step, stream_);
+    return a.dt;
   }
+
+  bool limits_on() const { return lim_strong_ || lim_sound_; }
+  // Max sound speed (out[0]) and strong-CFL factor (out[1]) of the current
+  // particles against the grid of a.grid and vsave_; waits for the stream.
+  void dt_limit_values(const Args<T, D>& a, int strong, int sound, T cfl, T cfl_F, T* out) {
+    lim_.alloc(2);
+    if (!lim_host_) HIPCHECK(hipHostMalloc((void**)&lim_host_, 2 * sizeof(Ubits), hipHostMallocDefault));
+    dt_limit_init_kernel<T><<<1, 1, 0, stream_>>>(lim_.p);
+    const uint32_t nb = std::min<uint32_t>(cdiv(n_, BLOCK), 4096);
+    dt_limit_kernel<T, D><<<nb, BLOCK, 0, stream_>>>(a, vsave_.p, strong, sound, cfl, cfl_F, lim_.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(lim_host_, lim_.p, 2 * sizeof(Ubits), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+    std::memcpy(out, lim_host_, sizeof(T));
+    std::memcpy(out + 1, lim_host_ + 1, sizeof(T));
+  }
+  // MPM_DRIVER::Reduce_Dt (MPM_DRIVER.cpp:519-545) in T after the grid
+  // update of a step with a.dt; returns the dt G2P takes and, when it is
+  // lowered below 0.99 dt, rescales the valid nodes' velocities.
+  T reduce_dt(const Args<T, D>& a, bool fric, T cfl, T min_dt) {
+    vsave_.alloc((size_t)ntotal_ * 64);
+    // velocity_save: the step's P2G again, with dt = 0 (no stress impulse); the
+    // slow and mover lists are rebuilt by it, so they start empty
+    clear_tiles_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(vsave_.p, a.gtiles, counts_.p + 1);
+    fill_u32_kernel<<<1, BLOCK, 0, stream_>>>(counts_.p + 3, 2u, 0u);
+    Args<T, D> b = a;
+    b.dt = 0;
+    b.grid = vsave_.p;
+    b.stats_partial = nullptr;
+    p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(b);
+    p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(b);
+    normalize_nodes_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(vsave_.p, a.gtiles, counts_.p + 1);
+    T v[2];
+    dt_limit_values(a, lim_strong_, lim_sound_, cfl, lim_cfl_F_, v);
+    const T dt0 = a.dt;
+    T dt = dt0;
+    if (lim_strong_) {
+      const T new_dt = dt0 * v[1];
+      if (new_dt < dt) dt = new_dt;
+    }
+    T dts = dt0;
+    if (lim_sound_) {
+      T dxmin = a.g.dx[0];
+      for (int c = 1; c < D; c++) dxmin = std::min(dxmin, a.g.dx[c]);
+      dts = std::min(dts, robust_divide(dxmin, v[0]) * lim_cfl_sound_);
+    }
+    dt = std::min(dt, dts);
+    dt = std::max(dt, min_dt);
+    if (dt >= dt0 * (T)0.99) return dt0;
+    const T sc = dt / dt0;
+    rescale_nodes_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(a.grid, fric ? a.gridf : nullptr, vsave_.p, a.gtiles,
+                                                                 counts_.p + 1, sc);
+    HIPCHECK(hipGetLastError());
+    return dt;
+  }
+  void set_dt_limits(bool strong, bool sound, double cfl, double cfl_F, double cfl_sound, double min_dt) override {
+    if (!(cfl > 0) || !(cfl_F > 0) || !(cfl_sound > 0) || !(min_dt >= 0))
+      throw Error{MPMX_EINVAL, "need cfl, cfl_F, cfl_sound > 0 and min_dt >= 0"};
+    if ((strong || sound) && scenes_.size() > 1)
+      throw Error{MPMX_ESTATE, "the strong and sound-speed CFL take a single-scene context"};
+    if (taping_) throw Error{MPMX_ESTATE, "dt limits cannot change while a tape is recorded"};
+    lim_strong_ = strong;
+    lim_sound_ = sound;
+    lim_cfl_ = (T)cfl;
+    lim_cfl_F_ = (T)cfl_F;
+    lim_cfl_sound_ = (T)cfl_sound;
+    lim_min_dt_ = (T)min_dt;
+    fused_ = false;
+  }
+  // MPM_DRIVER::Compute_Max_Sound_Speed (MPM_DRIVER.cpp:893-918) of the current particles.
+  void max_sound_speed(double* out) override {
+    if (scenes_.size() > 1) throw Error{MPMX_ESTATE, "the sound speed takes a single-scene context"};
+    T v[2] = {0, 100};
+    if (n_ > 0) {
+      Args<T, D> a = make_args(0);
+      dt_limit_values(a, 0, 1, 1, 1, v);
+    }
+    *out = (double)v[0];
+  }
+  double last_dt() const override { return last_dt_; }
 
   // P2G of the current state into gcur_, starting the fused pipeline: both
   // grids are cleared in full (the fused step clears only the tiles it used).
@@ -3558,6 +3836,7 @@ class Engine final : public EngineBase {
     has_grid_ = true;
     fused_last_ = true;
     time_ += (double)dt;
+    last_dt_ = (double)dt;
     timer_.end(PH_STEP, e_step, stream_);
   }
 
@@ -3606,6 +3885,14 @@ class Engine final : public EngineBase {
   using Ubits = typename OrderedBits<T>::U;
   DevBuf<Ubits> vbound_;      // per-scene speed bound (ordered bit patterns)
   Ubits* vb_host_ = nullptr;  // pinned readback of vbound_
+  // Reduce_Dt settings (example.use_strong_cfl, use_sound_speed_cfl, cfl,
+  // cfl_F, cfl_sound, min_dt; MPM_EXAMPLE.h:95-104) and its buffers
+  int lim_strong_ = 0, lim_sound_ = 0;
+  T lim_cfl_ = 1, lim_cfl_F_ = (T)0.5, lim_cfl_sound_ = (T)0.9, lim_min_dt_ = 0;
+  DevBuf<Node4<T>> vsave_;
+  DevBuf<Ubits> lim_;
+  Ubits* lim_host_ = nullptr;
+  double last_dt_ = 0;
   bool has_grid_ = false;
   bool stats_on_ = false;
   double time_ = 0;

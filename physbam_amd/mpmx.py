@@ -26,6 +26,7 @@ EXPORTS = (
     "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_grid_node_count",
     "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_set_rebin_policy", "mpmx_bin_stats",
     "mpmx_grid_v_upper_bound", "mpmx_compute_dt", "mpmx_advance_frames",
+    "mpmx_set_dt_limits", "mpmx_max_sound_speed", "mpmx_last_dt",
     "mpmx_set_plasticity", "mpmx_get_plastic_state", "mpmx_set_drucker_prager", "mpmx_get_drucker_prager_state",
     "mpmx_tape_begin", "mpmx_backward",
     "mpmx_enable_timing", "mpmx_reset_timing",
@@ -97,6 +98,9 @@ def lib():
         L.mpmx_advance_frames.argtypes = [vp, c_int, dbl, dbl, dbl, dbl, vp, i64, vp]
         L.mpmx_set_plasticity.argtypes = [vp, c_int, dbl, dbl, dbl, dbl]
         L.mpmx_get_plastic_state.argtypes = [vp, vp, vp, vp]
+        L.mpmx_set_dt_limits.argtypes = [vp, c_int, c_int, dbl, dbl, dbl, dbl]
+        L.mpmx_max_sound_speed.argtypes = [vp, vp]
+        L.mpmx_last_dt.argtypes = [vp, vp]
         L.mpmx_set_drucker_prager.argtypes = [vp, c_int, dbl, dbl, dbl, dbl, dbl, c_int]
         L.mpmx_get_drucker_prager_state.argtypes = [vp, vp, vp, vp, vp]
         L.mpmx_tape_begin.argtypes = [vp, c_int]
@@ -297,6 +301,22 @@ class Context:
         _check(lib().mpmx_advance_frames(self._h, int(nframes), float(frame_dt), float(cfl), float(min_dt),
                                          float(max_dt), _p(dts), ctypes.c_int64(capacity), _p(cnt)))
         return dts[:min(int(cnt[0]), capacity)].copy()
+
+    def set_dt_limits(self, strong_cfl=False, sound_cfl=False, cfl=1.0, cfl_F=0.5, cfl_sound=0.9, min_dt=0.0):
+        """Strong and sound-speed CFL (MPM_DRIVER::Reduce_Dt) inside every step."""
+        _check(lib().mpmx_set_dt_limits(self._h, 1 if strong_cfl else 0, 1 if sound_cfl else 0, float(cfl),
+                                        float(cfl_F), float(cfl_sound), float(min_dt)))
+
+    def max_sound_speed(self):
+        out = np.zeros(1)
+        _check(lib().mpmx_max_sound_speed(self._h, _p(out)))
+        return float(out[0])
+
+    @property
+    def last_dt(self):
+        out = np.zeros(1)
+        _check(lib().mpmx_last_dt(self._h, _p(out)))
+        return float(out[0])
 
     def set_plasticity(self, theta_c, theta_s, max_hardening, hardening_factor, enable=True):
         """Clamped plasticity with hardening after every G2P (MPM_PLASTICITY_CLAMP)."""

@@ -85,6 +85,66 @@ def write_frames():
     np.savez_compressed(os.path.join(HERE, "frames.npz"), **out)
 
 
+def limit_cases():
+    """Scenes stepped with Reduce_Dt's strong and sound-speed CFL
+    (use_strong_cfl, use_sound_speed_cfl): fixed-dt steps whose dt is lowered
+    in some steps and kept in others, one per constitutive model, and one
+    frame run whose substeps are lowered inside the frame.
+
+    Each entry: scene, limits, ("steps", n) or ("frames", nframes, frame_dt, cfl, min_dt, max_dt)."""
+    sq = lambda X: (np.abs(X[:, 0] - 0.5) <= 0.15) & (np.abs(X[:, 1] - 0.45) <= 0.15)
+    sph = lambda X: ((X - 0.5) ** 2).sum(1) < 0.25 ** 2
+    cases = {}
+    walls = mt.wall_boxes(2, [32, 32], [0, 0], [1, 1], flags=15, inset=0.1, type_=mt.STICK, friction=0.0)
+    sc = mt.make_scene(2, 32, prec=8, model=mt.COROTATED, shape=sq, v0=[2.0, -6.0], vrand=3.0, frand=0.05,
+                       walls=walls, dt=3.15e-3, seed=51)
+    cases["strong2d_f64"] = (sc, dict(strong_cfl=True, cfl=0.02, cfl_F=0.1), ("steps", 6))
+    walls = mt.wall_boxes(2, [32, 32], [0, 0], [1, 1], flags=15, inset=0.1, type_=mt.STICK, friction=0.3)
+    sc = mt.make_scene(2, 32, prec=8, model=mt.HENCKY, shape=sq, v0=[1.0, -3.0], vrand=4.0, frand=0.1, walls=walls,
+                       dt=1e-3, E=3e5, seed=53)
+    cases["both2d_hencky_fric_f64"] = (sc, dict(strong_cfl=True, sound_cfl=True, cfl=0.05, cfl_F=0.05, cfl_sound=0.9,
+                                                min_dt=1e-5), ("steps", 6))
+    walls = mt.wall_boxes(3, [16] * 3, [0] * 3, [1] * 3, flags=63, inset=0.12, type_=mt.STICK, friction=0.0)
+    sc = mt.make_scene(3, 16, prec=8, model=mt.COROTATED, shape=sph, v0=[0.5, -2.0, 0.3], vrand=0.5, frand=0.1,
+                       walls=walls, dt=1e-3, E=1e6, per_particle_E=True, seed=54)
+    cases["sound3d_coro_f64"] = (sc, dict(sound_cfl=True, cfl_sound=0.9), ("steps", 3))
+    walls = mt.wall_boxes(3, [16] * 3, [0] * 3, [1] * 3, flags=63, inset=0.12, type_=mt.SLIP, friction=0.0)
+    sc = mt.make_scene(3, 16, prec=4, model=mt.NEO_HOOKEAN, shape=sph, v0=[0.5, -2.0, 0.3], vrand=0.5, frand=0.1,
+                       walls=walls, dt=2e-3, E=1e6, per_particle_E=True, seed=52)
+    cases["sound3d_neo_f32"] = (sc, dict(sound_cfl=True, cfl_sound=0.9), ("steps", 4))
+    walls = mt.wall_boxes(3, [16] * 3, [0] * 3, [1] * 3, flags=63, inset=0.12, type_=mt.STICK, friction=0.0)
+    sc = mt.make_scene(3, 16, prec=4, model=mt.HENCKY, shape=sph, v0=[0.5, -2.0, 0.3], vrand=2.0, frand=0.1,
+                       walls=walls, dt=1e-3, E=1e5, per_particle_E=True, seed=55)
+    cases["strong3d_hencky_f32"] = (sc, dict(strong_cfl=True, sound_cfl=True, cfl=0.002, cfl_F=0.05, cfl_sound=0.9),
+                                    ("steps", 4))
+    walls = mt.wall_boxes(2, [32, 32], [0, 0], [1, 1], flags=15, inset=0.1, type_=mt.STICK, friction=0.0)
+    sc = mt.make_scene(2, 32, prec=8, model=mt.COROTATED, shape=sq, v0=[1.0, -4.0], vrand=1.0, walls=walls, dt=1e-3,
+                       E=2e5, seed=56)
+    cases["frames2d_f64"] = (sc, dict(strong_cfl=True, sound_cfl=True, cfl_F=0.1, cfl_sound=0.9),
+                             ("frames", 2, 1.0 / 240, 0.5, 1e-6, 1.0 / 240))
+    return cases
+
+
+def write_limits():
+    out = {}
+    for name, (sc, lim, run) in limit_cases().items():
+        for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+            out[f"{name}_in_{k}"] = np.asarray(sc[k])
+        meta = {k: sc[k] for k in ("dim", "prec", "model", "N", "lo", "hi", "gravity", "dt", "neo_threshold")}
+        meta.update(limits=lim, run=list(run))
+        out[f"{name}_meta"] = np.array(json.dumps(meta, default=float))
+        out[f"{name}_walls"] = np.array(json.dumps(sc["walls"], default=float))
+        if run[0] == "steps":
+            r, dts = scene_io.run_ref_limited(sc, run[1], lim)
+        else:
+            r, dts = scene_io.run_ref_frames(sc, *run[1:], lim=lim)
+        out[f"{name}_dts"] = dts
+        for k in ("X", "V", "F", "B", "valid"):
+            out[f"{name}_out_{k}"] = r[k]
+        print("limits", name, sc["X"].shape[0], "particles, dt / requested", np.round(dts / sc["dt"], 4))
+    np.savez_compressed(os.path.join(HERE, "limits.npz"), **out)
+
+
 def plastic_cases():
     """Scenes with MPM_PLASTICITY_CLAMP (Add_Clamped_Plasticity); F is perturbed
     so that singular values leave [1 - theta_c, 1 + theta_s] from the first step."""
@@ -141,11 +201,13 @@ def write_plastic():
 def main():
     if not scene_io.ref_available():
         sys.exit("reference harness missing: build it with oracle/ref/Makefile")
-    groups = set(sys.argv[1:]) or {"steps", "components", "frames", "plastic"}
+    groups = set(sys.argv[1:]) or {"steps", "components", "frames", "plastic", "limits"}
     if "frames" in groups:
         write_frames()
     if "plastic" in groups:
         write_plastic()
+    if "limits" in groups:
+        write_limits()
     if "steps" in groups:
         write_steps()
     if "components" in groups:

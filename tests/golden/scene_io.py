@@ -116,16 +116,42 @@ def run_ref(sc, nsteps, dump_every=None, threads=1):
         return read_output(op, sc["dim"], plastic=plastic_kind(sc))
 
 
-def run_ref_frames(sc, nframes, frame_dt, cfl, min_dt, max_dt, threads=1):
-    """Run ``nframes`` frames of the reference's adaptive substep loop; returns (final dump, dts)."""
+def _limit_args(lim):
+    return [str(1 if lim.get("strong_cfl") else 0), str(1 if lim.get("sound_cfl") else 0),
+            repr(float(lim.get("cfl_F", 0.5))), repr(float(lim.get("cfl_sound", 0.9)))]
+
+
+def run_ref_frames(sc, nframes, frame_dt, cfl, min_dt, max_dt, threads=1, lim=None):
+    """Run ``nframes`` frames of the reference's adaptive substep loop (with
+    Reduce_Dt's strong / sound-speed CFL per ``lim``); returns (final dump, dts)."""
     with tempfile.TemporaryDirectory() as td:
         sp = os.path.join(td, "scene.bin")
         op = os.path.join(td, "out.bin")
         write_scene(sp, sc, 0, 1, threads)
         subprocess.run([REF_HARNESS, "frames", sp, str(nframes), repr(float(frame_dt)), repr(float(cfl)),
-                        repr(float(min_dt)), repr(float(max_dt)), op], check=True)
+                        repr(float(min_dt)), repr(float(max_dt)), op] + (_limit_args(lim) if lim else []),
+                       check=True, stdout=subprocess.DEVNULL)
         with open(op, "rb") as f:
             buf = f.read()
+    return _frames_output(sc, buf)
+
+
+def run_ref_limited(sc, nsteps, lim, threads=1):
+    """``nsteps`` fixed-dt steps of the reference with Reduce_Dt (lim: strong_cfl,
+    sound_cfl, cfl, cfl_F, cfl_sound, min_dt); returns (final dump, dts taken)."""
+    with tempfile.TemporaryDirectory() as td:
+        sp = os.path.join(td, "scene.bin")
+        op = os.path.join(td, "out.bin")
+        write_scene(sp, sc, nsteps, nsteps, threads)
+        a = _limit_args(lim)
+        subprocess.run([REF_HARNESS, "limited", sp, a[0], a[1], repr(float(lim.get("cfl", 1.0))), a[2], a[3],
+                        repr(float(lim.get("min_dt", 0.0))), op], check=True, stdout=subprocess.DEVNULL)
+        with open(op, "rb") as f:
+            buf = f.read()
+    return _frames_output(sc, buf)
+
+
+def _frames_output(sc, buf):
     assert buf[:7] == b"MPMFRM1"
     nd, d, n, nn = struct.unpack_from("<4i", buf, 8)
     assert d == sc["dim"]

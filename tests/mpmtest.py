@@ -216,8 +216,61 @@ def oracle_run_drucker_prager(sc, nsteps, prec=None):
     return out
 
 
-def oracle_frames(sc, nframes, frame_dt, cfl, min_dt, max_dt, prec=None, max_sub=100000):
-    """Advance ``nframes`` frames with the adaptive substep loop of the oracle.
+def _limits_array(lim):
+    """(strong_cfl, sound_cfl, cfl_F, cfl_sound) from a dict, or None."""
+    if not lim:
+        return None
+    return np.array([1.0 if lim.get("strong_cfl") else 0.0, 1.0 if lim.get("sound_cfl") else 0.0,
+                     lim.get("cfl_F", 0.5), lim.get("cfl_sound", 0.9)], np.float64)
+
+
+def oracle_run_limited(sc, nsteps, lim, prec=None):
+    """``nsteps`` fixed-dt steps with Reduce_Dt (lim: strong_cfl, sound_cfl, cfl, cfl_F, cfl_sound, min_dt).
+
+    Returns (final state dict, dts taken)."""
+    prec = prec or sc["prec"]
+    dt_np = np.float64 if prec == 8 else np.float32
+    d = sc["dim"]
+    n = sc["X"].shape[0]
+    st = {k: np.ascontiguousarray(sc[k], dtype=dt_np).copy() for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda")}
+    valid = np.ascontiguousarray(sc["valid"], np.uint8).copy()
+    N = np.array(list(sc["N"]) + [1] * (3 - d), np.int32)
+    lo = np.array(list(sc["lo"]) + [0] * (3 - d), np.float64)
+    hi = np.array(list(sc["hi"]) + [0] * (3 - d), np.float64)
+    g = np.array(list(sc["gravity"]) + [0] * (3 - d), np.float64)
+    nw, wt, wf, wl, wh = _walls_arrays(sc)
+    la_ = _limits_array(lim)
+    dts = np.zeros(max(nsteps, 1), np.float64)
+    fn = oracle().oracle_run_limited_f64 if prec == 8 else oracle().oracle_run_limited_f32
+    rc = fn(ctypes.c_int(d), _ptr(N), _ptr(lo), _ptr(hi), _ptr(g), ctypes.c_int(sc["model"]),
+            ctypes.c_double(sc.get("neo_threshold", 0.25)), ctypes.c_double(sc["dt"]),
+            ctypes.c_int(nw), _ptr(wt), _ptr(wf), _ptr(wl), _ptr(wh), ctypes.c_longlong(n),
+            _ptr(st["X"]), _ptr(st["V"]), _ptr(st["F"]), _ptr(st["B"]), _ptr(st["mass"]), _ptr(st["volume"]),
+            _ptr(st["mu"]), _ptr(st["lambda"]), _ptr(valid), ctypes.c_int(nsteps), _ptr(la_),
+            ctypes.c_double(lim.get("cfl", 1.0)), ctypes.c_double(lim.get("min_dt", 0.0)), _ptr(dts))
+    assert rc == 0
+    return dict(X=st["X"], V=st["V"], F=st["F"], B=st["B"], valid=valid), dts[:nsteps].copy()
+
+
+def oracle_sound_speeds(sc, prec=None, F=None):
+    """Sound_Speed of every particle of ``sc`` (or of the given F) from the singular values of F."""
+    prec = prec or sc["prec"]
+    dt_np = np.float64 if prec == 8 else np.float32
+    n = sc["X"].shape[0]
+    arr = {k: np.ascontiguousarray(sc[k] if k != "F" or F is None else F, dtype=dt_np)
+           for k in ("F", "mass", "volume", "mu", "lambda")}
+    out = np.zeros(n, dt_np)
+    fn = oracle().oracle_sound_speeds_f64 if prec == 8 else oracle().oracle_sound_speeds_f32
+    rc = fn(ctypes.c_int(sc["dim"]), ctypes.c_int(sc["model"]), ctypes.c_double(sc.get("neo_threshold", 0.25)),
+            ctypes.c_longlong(n), _ptr(arr["F"]), _ptr(arr["mass"]), _ptr(arr["volume"]), _ptr(arr["mu"]),
+            _ptr(arr["lambda"]), _ptr(out))
+    assert rc == 0
+    return out
+
+
+def oracle_frames(sc, nframes, frame_dt, cfl, min_dt, max_dt, prec=None, max_sub=100000, lim=None):
+    """Advance ``nframes`` frames with the adaptive substep loop of the oracle
+    (with Reduce_Dt when ``lim`` turns the strong or sound-speed CFL on).
 
     Returns (final state dict, array of substep dts)."""
     prec = prec or sc["prec"]
@@ -238,7 +291,7 @@ def oracle_frames(sc, nframes, frame_dt, cfl, min_dt, max_dt, prec=None, max_sub
              ctypes.c_longlong(n), _ptr(st["X"]), _ptr(st["V"]), _ptr(st["F"]), _ptr(st["B"]), _ptr(st["mass"]),
              _ptr(st["volume"]), _ptr(st["mu"]), _ptr(st["lambda"]), _ptr(valid), ctypes.c_int(nframes),
              ctypes.c_double(frame_dt), ctypes.c_double(cfl), ctypes.c_double(min_dt), ctypes.c_double(max_dt),
-             _ptr(dts), ctypes.c_int(max_sub))
+             _ptr(dts), ctypes.c_int(max_sub), _ptr(_limits_array(lim)))
     assert cnt >= 0, "substep limit exceeded"
     return dict(X=st["X"], V=st["V"], F=st["F"], B=st["B"], valid=valid), dts[:cnt].copy()
 

@@ -359,3 +359,99 @@ def test_plasticity_projection_closed_form():
     assert np.abs(F @ Fp - Fe0).max() < 1e-12
     h = np.exp(np.minimum(pl["max_hardening"], pl["hardening_factor"] * (1 - np.linalg.det(Fp))))
     assert np.allclose(out["mu"], sc["mu"] * h, rtol=1e-12)
+
+
+LIMIT_CASES = ["strong2d_f64", "both2d_hencky_fric_f64", "sound3d_coro_f64", "sound3d_neo_f32", "strong3d_hencky_f32",
+               "frames2d_f64"]
+
+
+def limit_scene(z, name):
+    meta = json.loads(str(z[f"{name}_meta"]))
+    sc = dict(meta)
+    sc["walls"] = json.loads(str(z[f"{name}_walls"]))
+    for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+        sc[k] = z[f"{name}_in_{k}"]
+    return sc, meta
+
+
+@pytest.mark.parametrize("name", LIMIT_CASES)
+def test_oracle_dt_limits_match_reference(name):
+    """Reduce_Dt (strong CFL, sound-speed CFL, min_dt, the 0.99 rule and the
+    grid rescale) inside fixed-dt steps and inside the frame loop: the dts
+    taken and the final state agree bit for bit with the reference's own runs
+    (tests/golden/limits.npz)."""
+    z = load("limits.npz")
+    sc, m = limit_scene(z, name)
+    run, lim = m["run"], m["limits"]
+    if run[0] == "steps":
+        out, dts = mt.oracle_run_limited(sc, run[1], lim)
+    else:
+        out, dts = mt.oracle_frames(sc, *run[1:], lim=lim)
+    ref = z[f"{name}_dts"]
+    assert np.array_equal(dts, ref), (dts, ref)
+    for k in ("X", "V", "F", "B", "valid"):
+        assert np.array_equal(out[k].astype(np.float64) if k != "valid" else out[k], z[f"{name}_out_{k}"]), k
+
+
+def test_dt_limit_fixtures_lower_some_steps_and_keep_others():
+    """The fixtures cover both outcomes of Reduce_Dt: steps whose dt is lowered
+    (below 0.99 of the request) and steps that keep it."""
+    z = load("limits.npz")
+    lowered = kept = 0
+    for name in LIMIT_CASES[:-1]:
+        sc, m = limit_scene(z, name)
+        r = z[f"{name}_dts"] / np.float64(np.float32(sc["dt"]) if sc["prec"] == 4 else sc["dt"])
+        lowered += int((r < 0.99).sum())
+        kept += int((r == 1).sum())
+    assert lowered >= 10 and kept >= 5
+
+
+@pytest.mark.parametrize("name", ["sound3d_coro_f64", "sound3d_neo_f32"])
+def test_sound_speed_sets_the_first_dt_of_the_reference(name):
+    """With only the sound-speed CFL on, the reference's first dt is
+    cfl_sound * min dx / max_p c_p over the step's particles: the oracle's
+    per-particle Sound_Speed reproduces it bit for bit."""
+    z = load("limits.npz")
+    sc, m = limit_scene(z, name)
+    T = np.float32 if sc["prec"] == 4 else np.float64
+    c = mt.oracle_sound_speeds(sc)[sc["valid"] == 1]
+    dx = min(T(T(h) - T(l)) / T(n) for l, h, n in zip(sc["lo"], sc["hi"], sc["N"]))
+    want = T(dx / c.max()) * T(m["limits"]["cfl_sound"])
+    assert float(want) == z[f"{name}_dts"][0]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("model", [mt.COROTATED, mt.NEO_HOOKEAN, mt.HENCKY])
+def test_sound_speed_at_rest_is_the_p_wave_speed(dim, model):
+    """At F = I every model's stiffness is lambda + 2 mu along the axes (and mu
+    in shear), so the sound speed is the P-wave speed sqrt((lambda + 2 mu) / rho);
+    rotations leave it unchanged (it depends on the singular values only)."""
+    rng = np.random.default_rng(dim * 10 + model)
+    n = 50
+    E = rng.uniform(1e3, 1e6, n)
+    nu = rng.uniform(0.1, 0.45, n)
+    mu, la = mt.lame(E, nu)
+    rho = rng.uniform(500, 3000, n)
+    vol = np.full(n, 1e-4)
+    sc = dict(dim=dim, model=model, prec=8, X=np.zeros((n, dim)), mass=rho * vol, volume=vol, mu=mu, **{"lambda": la},
+              F=np.tile(np.eye(dim).reshape(-1), (n, 1)))
+    want = np.sqrt((la + 2 * mu) / rho)
+    assert np.allclose(mt.oracle_sound_speeds(sc), want, rtol=1e-14)
+    Q = np.linalg.qr(rng.normal(size=(n, dim, dim)))[0]
+    Q[np.linalg.det(Q) < 0, :, 0] *= -1
+    Fr = Q.transpose(0, 2, 1).reshape(n, -1)  # column-major storage of Q
+    assert np.allclose(mt.oracle_sound_speeds(sc, F=Fr), want, rtol=1e-12)
+
+
+def test_sound_speed_grows_under_compression_for_neo_hookean():
+    """Neo-Hookean stiffens under compression: H_ii = mu + (lambda + mu - lambda
+    ln J) / s_i^2 exceeds its rest value for J < 1, so the speed rises."""
+    n = 3
+    mu, la = mt.lame(np.full(n, 1e5), np.full(n, 0.3))
+    sc = dict(dim=3, model=mt.NEO_HOOKEAN, prec=8, X=np.zeros((n, 3)), mass=np.full(n, 1.0), volume=np.full(n, 1e-3),
+              mu=mu, **{"lambda": la}, F=np.array([np.diag([s] * 3).reshape(-1) for s in (1.0, 0.9, 0.8)]))
+    c = mt.oracle_sound_speeds(sc)
+    assert c[0] < c[1] < c[2]
+    s = 0.8
+    H = mu[2] + (la[2] + mu[2] - la[2] * np.log(s ** 3)) / s ** 2
+    assert np.isclose(c[2], np.sqrt(H * s ** 2 / 1000.0), rtol=1e-13)
