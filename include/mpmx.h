@@ -103,6 +103,10 @@ int mpmx_set_particles(mpmx_ctx* ctx, int64_t n, const void* X, const void* V, c
    mpmx_set_particles. */
 int mpmx_get_particles(mpmx_ctx* ctx, void* X, void* V, void* F, void* B, uint8_t* valid);
 int64_t mpmx_particle_count(const mpmx_ctx* ctx);
+/* mass, volume and the Lame parameters mu, lambda of the particles (any
+   pointer may be NULL), in the same order; with clamped plasticity mu and
+   lambda are the hardened values. */
+int mpmx_get_material(mpmx_ctx* ctx, void* mass, void* volume, void* mu, void* lambda);
 
 /* Elastic force: one MPM_FINITE_ELEMENTS with the given model over all
    particles (STANDARD_TESTS_BASE::Add_Fixed_Corotated / Add_Neo_Hookean,
@@ -248,6 +252,55 @@ int mpmx_get_plastic_state(mpmx_ctx* ctx, void* Fp, void* mu, void* lambda);
 int mpmx_set_drucker_prager(mpmx_ctx* ctx, int enable, double a0, double a1, double a3, double a4, double cohesion,
                             int volume_correction);
 int mpmx_get_drucker_prager_state(mpmx_ctx* ctx, void* plastic_def, void* rho_F, void* cohesion, void* vc);
+
+/* Particle and restart files in PhysBAM's own format, for restarts and for
+   the tools that read PhysBAM output (the viewer, mpm2partio).
+   mpmx_write_particles_file is PARTICLES<TV>::Write (Tools/Particles/
+   PARTICLES.cpp:362-373) behind a FILE_OSTREAM (Core/Read_Write/
+   TYPED_STREAM.cpp:30-36): a byte saying whether the stream holds doubles,
+   version 2, the particle count and each named array (stored in name order)
+   with the coded type id of ATTRIBUTE_INFO::Encode (:15-25).  It writes the
+   `count` arrays given, each n elements in the precision given; a path ending
+   in ".gz" is gzip-compressed (Safe_Open_Output, FILE_UTILITIES.cpp:237-261).
+   mpmx_read_particles_file is PARTICLES<TV>::Read for version 2 (:323-358):
+   it returns the particle count in *n (arrays = NULL, count = 0 reads only
+   the header) and fills the named arrays found in the file, converting float
+   and double; found[i] = 1 for each one present (found = NULL makes a missing
+   array an error); other arrays are skipped. */
+#define MPMX_ARRAY_SCALAR 0  /* one value per particle (ARRAY<T>) */
+#define MPMX_ARRAY_VECTOR 1  /* dim values (ARRAY<TV>) */
+#define MPMX_ARRAY_VECTOR3 2 /* 3 values (ARRAY<VECTOR<T,3> >, e.g. color) */
+#define MPMX_ARRAY_MATRIX 3  /* dim*dim values, column-major (ARRAY<MATRIX<T,d> >) */
+#define MPMX_ARRAY_BOOL 4    /* one byte (ARRAY<bool>) */
+typedef struct mpmx_particle_array {
+  const char* name; /* attribute name, e.g. "X", "mass", "valid" */
+  int kind;         /* MPMX_ARRAY_* */
+  void* data;       /* n elements in the call's precision */
+} mpmx_particle_array;
+int mpmx_write_particles_file(const char* path, int dim, int precision, int64_t n, const mpmx_particle_array* arrays,
+                              int count);
+int mpmx_read_particles_file(const char* path, int dim, int precision, int64_t* n, mpmx_particle_array* arrays,
+                             int count, int* found);
+
+/* restart_data: the time behind the same one-byte stream header, in the
+   given precision (Write_To_File of example.time, MPM_EXAMPLE.cpp:66). */
+int mpmx_write_restart_file(const char* path, int precision, double time);
+int mpmx_read_restart_file(const char* path, double* time);
+
+/* MPM_EXAMPLE::Write_Output_Files' particle and restart files for one frame
+   directory (MPM_EXAMPLE.cpp:62-74, DEFORMABLE_BODY_COLLECTION.cpp:112):
+   <frame_dir>/deformable_object_particles.gz with the arrays an MPM scene of
+   the reference stores (X, V, F, B, color, mass, volume, valid, mu, lambda;
+   with plasticity also Fp, mu0, lambda0 and, for Drucker-Prager,
+   plastic_deformation, dp_rho_f, dp_cohesion, vcp), the link mpm_particles.gz
+   to it, and <frame_dir>/restart_data.gz with the time.  color is written as
+   zeros (rendering data this library does not keep).  mpmx_read_output_files
+   is MPM_EXAMPLE::Read_Output_Files (:95-102): mpm_particles (or
+   deformable_object_particles) and restart_data, gzipped or not, replace the
+   particles (as mpmx_set_particles) and the time; with plasticity enabled the
+   plastic arrays in the file are restored too. */
+int mpmx_write_output_files(mpmx_ctx* ctx, const char* frame_dir);
+int mpmx_read_output_files(mpmx_ctx* ctx, const char* frame_dir);
 
 /* Differentiation (no counterpart in the reference, whose step has no
    adjoint).  mpmx_tape_begin records the current particle state and then a

@@ -12,6 +12,14 @@
 //   ref_harness pstrain <prec> <dim> <model> <threshold> <in.bin> <out.bin>
 //   ref_harness frames <scene.bin> <nframes> <frame_dt> <cfl> <min_dt> <max_dt> <out.bin>
 //               [<strong_cfl> <sound_cfl> <cfl_F> <cfl_sound>]
+//   ref_harness write <scene.bin> <dir>
+//     (the scene's nsteps steps, then MPM_EXAMPLE::Write_Output_Files's
+//     particle and restart files: <dir>/deformable_object_particles.gz, the
+//     PARTICLES<TV>::Write stream, and <dir>/restart_data.gz, the time)
+//   ref_harness read <prec> <dim> <dir> <out.bin>
+//     (Read_From_File of <dir>/deformable_object_particles into MPM_PARTICLES
+//     and of <dir>/restart_data: a dump of X, V, F, B, valid, mass, volume,
+//     mu, lambda, Fp, mu0, lambda0 when stored, and the time)
 //   ref_harness limited <scene.bin> <strong_cfl> <sound_cfl> <cfl> <cfl_F> <cfl_sound> <min_dt> <out.bin>
 //     (the scene's nsteps fixed-dt steps with MPM_DRIVER::Reduce_Dt: the dt
 //     each step took and the final state, in the frames output layout)
@@ -32,6 +40,7 @@
 #include <Deformables/Constitutive_Models/COROTATED_FIXED.h>
 #include <Deformables/Constitutive_Models/NEO_HOOKEAN.h>
 #include <Hybrid_Methods/Examples_And_Drivers/MPM_DRIVER.h>
+#include <Core/Read_Write/FILE_UTILITIES.h>
 #include <Hybrid_Methods/Examples_And_Drivers/MPM_EXAMPLE.h>
 #include <Hybrid_Methods/Examples_And_Drivers/MPM_PARTICLES.h>
 #include <Hybrid_Methods/Forces/MPM_FINITE_ELEMENTS.h>
@@ -313,6 +322,57 @@ int Run_Frames(const SCENE& s,int nframes,double frame_dt,double cfl,double min_
     return 0;
 }
 
+// Steps, then the particle and restart files of Write_Output_Files
+// (MPM_EXAMPLE.cpp:64-74; DEFORMABLE_BODY_COLLECTION.cpp:112).
+template<class TV>
+int Run_Write(const SCENE& s,const char* dir,STREAM_TYPE st)
+{
+#ifdef USE_OPENMP
+    omp_set_num_threads(s.threads>0?s.threads:1);
+#endif
+    RUN<TV> run(s,st);
+    for(int k=0;k<s.nsteps;k++) run.Step();
+    Write_To_File(st,std::string(dir)+"/deformable_object_particles",run.ex.particles);
+    Write_To_File(st,std::string(dir)+"/restart_data",run.ex.time);
+    return 0;
+}
+
+// MPM_EXAMPLE::Read_Output_Files (MPM_EXAMPLE.cpp:95-102) into fresh particles.
+template<class TV>
+int Run_Read(const char* dir,const char* out)
+{
+    typedef typename TV::SCALAR T;
+    enum {d=TV::m};
+    MPM_PARTICLES<TV> P;
+    T time=0;
+    Read_From_File(std::string(dir)+"/deformable_object_particles",P);
+    Read_From_File(std::string(dir)+"/restart_data",time);
+    WRITER w(out);
+    int n=P.X.m;
+    w.put(n);
+    std::vector<double> buf;
+    auto vec=[&](const ARRAY_VIEW<TV>& a){buf.assign((size_t)n*d,0);
+        for(int p=0;p<n;p++) for(int k=0;k<d;k++) buf[p*d+k]=a(p)(k);w.put(buf.data(),buf.size());};
+    auto mat=[&](const ARRAY_VIEW<MATRIX<T,d> >& a){buf.assign((size_t)n*d*d,0);
+        for(int p=0;p<n;p++) for(int k=0;k<d*d;k++) buf[p*d*d+k]=a(p).x[k];w.put(buf.data(),buf.size());};
+    auto sca=[&](const ARRAY_VIEW<T>& a){buf.assign(n,0);for(int p=0;p<n;p++) buf[p]=a(p);w.put(buf.data(),buf.size());};
+    vec(P.X);vec(P.V);mat(P.F);
+    // the flags store_B / store_Fp stay false after a read: look the arrays up by name
+    const ARRAY_VIEW<MATRIX<T,d> >* Bp=P.template Get_Array<MATRIX<T,d> >("B");
+    if(Bp) mat(*Bp); else {buf.assign((size_t)n*d*d,0);w.put(buf.data(),buf.size());}
+    std::vector<unsigned char> v(n);for(int p=0;p<n;p++) v[p]=P.valid(p);w.put(v.data(),v.size());
+    sca(P.mass);sca(P.volume);
+    const ARRAY_VIEW<T>* mu=P.template Get_Array<T>("mu");
+    const ARRAY_VIEW<T>* la=P.template Get_Array<T>("lambda");
+    if(mu) sca(*mu); else {buf.assign(n,0);w.put(buf.data(),n);}
+    if(la) sca(*la); else {buf.assign(n,0);w.put(buf.data(),n);}
+    const ARRAY_VIEW<MATRIX<T,d> >* Fpp=P.template Get_Array<MATRIX<T,d> >("Fp");
+    int has_fp=Fpp?1:0;w.put(has_fp);
+    if(has_fp){mat(*Fpp);sca(*P.template Get_Array<T>("mu0"));sca(*P.template Get_Array<T>("lambda0"));}
+    double t=(double)time;w.put(t);
+    return 0;
+}
+
 // Fixed-dt steps with Reduce_Dt (use_strong_cfl, use_sound_speed_cfl).
 template<class TV>
 int Run_Limited(const SCENE& s,const double* lim,double cfl,double min_dt,const char* out,STREAM_TYPE st)
@@ -460,6 +520,24 @@ int main(int argc,char* argv[])
             STREAM_TYPE st((float()));
             rc=s.dim==2?Run_Frames<VECTOR<float,2> >(s,nf,fdt,cfl,mind,maxd,argv[8],st,lim):
                 Run_Frames<VECTOR<float,3> >(s,nf,fdt,cfl,mind,maxd,argv[8],st,lim);}
+        LOG::Finish_Logging();
+        return rc;}
+    if(mode=="write" && argc==4){
+        SCENE s=Read_Scene(argv[2]);
+        int rc;
+        if(s.prec==8){
+            STREAM_TYPE st((double()));
+            rc=s.dim==2?Run_Write<VECTOR<double,2> >(s,argv[3],st):Run_Write<VECTOR<double,3> >(s,argv[3],st);}
+        else{
+            STREAM_TYPE st((float()));
+            rc=s.dim==2?Run_Write<VECTOR<float,2> >(s,argv[3],st):Run_Write<VECTOR<float,3> >(s,argv[3],st);}
+        LOG::Finish_Logging();
+        return rc;}
+    if(mode=="read" && argc==6){
+        int prec=atoi(argv[2]),dim=atoi(argv[3]);
+        int rc;
+        if(prec==8) rc=dim==2?Run_Read<VECTOR<double,2> >(argv[4],argv[5]):Run_Read<VECTOR<double,3> >(argv[4],argv[5]);
+        else rc=dim==2?Run_Read<VECTOR<float,2> >(argv[4],argv[5]):Run_Read<VECTOR<float,3> >(argv[4],argv[5]);
         LOG::Finish_Logging();
         return rc;}
     if(mode=="limited" && argc==10){

@@ -7,6 +7,12 @@
 #include <string>
 
 #include "mpmx_engine.h"
+#include "mpmx_io.h"
+
+#include <unistd.h>
+
+#include <cstdio>
+#include <vector>
 
 struct mpmx_ctx {
   mpmx::EngineBase* engine;
@@ -89,6 +95,11 @@ int mpmx_set_particles(mpmx_ctx* ctx, int64_t n, const void* X, const void* V, c
 int mpmx_get_particles(mpmx_ctx* ctx, void* X, void* V, void* F, void* B, uint8_t* valid) {
   if (!ctx) return bad("null context");
   return guard([&] { ctx->engine->get_particles(X, V, F, B, valid); });
+}
+
+int mpmx_get_material(mpmx_ctx* ctx, void* mass, void* volume, void* mu, void* lambda) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->get_material(mass, volume, mu, lambda); });
 }
 
 int64_t mpmx_particle_count(const mpmx_ctx* ctx) { return ctx ? ctx->engine->particle_count() : -1; }
@@ -211,6 +222,129 @@ int mpmx_max_sound_speed(mpmx_ctx* ctx, double* out) {
 int mpmx_last_dt(mpmx_ctx* ctx, double* out) {
   if (!ctx || !out) return bad("null argument");
   return guard([&] { *out = ctx->engine->last_dt(); });
+}
+
+int mpmx_write_particles_file(const char* path, int dim, int precision, int64_t n, const mpmx_particle_array* arrays,
+                              int count) {
+  return guard([&] { mpmx::io::write_particles(path, dim, precision, n, arrays, count); });
+}
+
+int mpmx_read_particles_file(const char* path, int dim, int precision, int64_t* n, mpmx_particle_array* arrays,
+                             int count, int* found) {
+  if (!n) return bad("null count");
+  return guard([&] { *n = mpmx::io::read_particles(path, dim, precision, arrays, count, found); });
+}
+
+int mpmx_write_restart_file(const char* path, int precision, double time) {
+  if (!path) return bad("null path");
+  if (precision != MPMX_FP32 && precision != MPMX_FP64) return bad("precision must be FP32 or FP64");
+  return guard([&] { mpmx::io::write_restart(path, precision, time); });
+}
+
+int mpmx_read_restart_file(const char* path, double* time) {
+  if (!path || !time) return bad("null argument");
+  return guard([&] { *time = mpmx::io::read_restart(path); });
+}
+
+int mpmx_write_output_files(mpmx_ctx* ctx, const char* frame_dir) {
+  if (!ctx || !frame_dir) return bad("null argument");
+  return guard([&] {
+    mpmx::EngineBase& e = *ctx->engine;
+    if (e.scene_count() > 1) throw mpmx::Error{MPMX_ESTATE, "output files take a single-scene context"};
+    const int d = e.dim(), prec = e.precision(), ts = prec == MPMX_FP64 ? 8 : 4;
+    const int64_t n = e.particle_count();
+    const size_t nn = (size_t)n;
+    std::vector<char> X(nn * d * ts), V(nn * d * ts), F(nn * d * d * ts), B(nn * d * d * ts), color(nn * 3 * ts, 0);
+    std::vector<char> mass(nn * ts), vol(nn * ts), mu(nn * ts), la(nn * ts);
+    std::vector<uint8_t> valid(nn);
+    e.get_particles(X.data(), V.data(), F.data(), B.data(), valid.data());
+    e.get_material(mass.data(), vol.data(), mu.data(), la.data());
+    std::vector<mpmx_particle_array> a = {
+        {"X", MPMX_ARRAY_VECTOR, X.data()},        {"V", MPMX_ARRAY_VECTOR, V.data()},
+        {"F", MPMX_ARRAY_MATRIX, F.data()},        {"B", MPMX_ARRAY_MATRIX, B.data()},
+        {"color", MPMX_ARRAY_VECTOR3, color.data()}, {"mass", MPMX_ARRAY_SCALAR, mass.data()},
+        {"volume", MPMX_ARRAY_SCALAR, vol.data()}, {"valid", MPMX_ARRAY_BOOL, valid.data()},
+        {"mu", MPMX_ARRAY_SCALAR, mu.data()},      {"lambda", MPMX_ARRAY_SCALAR, la.data()}};
+    std::vector<char> Fp, mu0, la0, pdef, rho, coh, vc;
+    const int kind = e.plastic_kind();
+    if (kind) {  // Store_Fp with Store_Lame stores mu0 and lambda0 too (MPM_PARTICLES.cpp:35-43, :95-106)
+      Fp.resize(nn * d * d * ts);
+      mu0.resize(nn * ts);
+      la0.resize(nn * ts);
+      e.get_plastic_state(Fp.data(), nullptr, nullptr);
+      e.get_lame0(mu0.data(), la0.data());
+      a.push_back({"Fp", MPMX_ARRAY_MATRIX, Fp.data()});
+      a.push_back({"mu0", MPMX_ARRAY_SCALAR, mu0.data()});
+      a.push_back({"lambda0", MPMX_ARRAY_SCALAR, la0.data()});
+    }
+    if (kind == 2) {  // MPM_DRUCKER_PRAGER's arrays (MPM_DRUCKER_PRAGER.cpp:20-23)
+      pdef.resize(nn * ts);
+      rho.resize(nn * ts);
+      coh.resize(nn * ts);
+      vc.resize(nn * ts);
+      e.get_drucker_prager_state(pdef.data(), rho.data(), coh.data(), vc.data());
+      a.push_back({"plastic_deformation", MPMX_ARRAY_SCALAR, pdef.data()});
+      a.push_back({"dp_rho_f", MPMX_ARRAY_SCALAR, rho.data()});
+      a.push_back({"dp_cohesion", MPMX_ARRAY_SCALAR, coh.data()});
+      a.push_back({"vcp", MPMX_ARRAY_SCALAR, vc.data()});
+    }
+    const std::string dir = frame_dir;
+    mpmx::io::write_particles((dir + "/deformable_object_particles.gz").c_str(), d, prec, n, a.data(), (int)a.size());
+    // MPM_EXAMPLE.cpp:64: mpm_particles.gz -> ./deformable_object_particles.gz
+    const std::string link = dir + "/mpm_particles.gz";
+    std::remove(link.c_str());
+    if (symlink("./deformable_object_particles.gz", link.c_str()) != 0)
+      throw mpmx::Error{MPMX_EINTERNAL, "cannot create " + link};
+    mpmx::io::write_restart((dir + "/restart_data.gz").c_str(), prec, e.time());
+  });
+}
+
+int mpmx_read_output_files(mpmx_ctx* ctx, const char* frame_dir) {
+  if (!ctx || !frame_dir) return bad("null argument");
+  return guard([&] {
+    mpmx::EngineBase& e = *ctx->engine;
+    if (e.scene_count() > 1) throw mpmx::Error{MPMX_ESTATE, "output files take a single-scene context"};
+    const int d = e.dim(), prec = e.precision(), ts = prec == MPMX_FP64 ? 8 : 4;
+    const std::string dir = frame_dir;
+    auto exists = [](const std::string& p) { return access(p.c_str(), R_OK) == 0; };
+    // Read_From_File tries the name with ".gz" first (FILE_UTILITIES.cpp:219-235)
+    std::string path;
+    for (const char* base : {"/mpm_particles", "/deformable_object_particles"})
+      for (const char* suf : {".gz", ""})
+        if (path.empty() && exists(dir + base + suf)) path = dir + base + suf;
+    if (path.empty()) throw mpmx::Error{MPMX_EINVAL, "no particle file in " + dir};
+    std::string rpath = exists(dir + "/restart_data.gz") ? dir + "/restart_data.gz" : dir + "/restart_data";
+    if (!exists(rpath)) throw mpmx::Error{MPMX_EINVAL, "no restart_data in " + dir};
+    const int64_t n = mpmx::io::read_particles(path.c_str(), d, prec, nullptr, 0, nullptr);
+    const size_t nn = (size_t)n;
+    std::vector<char> X(nn * d * ts), V(nn * d * ts), F(nn * d * d * ts), B(nn * d * d * ts, 0);
+    std::vector<char> mass(nn * ts), vol(nn * ts), mu(nn * ts), la(nn * ts);
+    std::vector<uint8_t> valid(nn);
+    std::vector<char> Fp(nn * d * d * ts), mu0(nn * ts), la0(nn * ts), pdef(nn * ts), rho(nn * ts), coh(nn * ts),
+        vc(nn * ts);
+    std::vector<mpmx_particle_array> a = {
+        {"X", MPMX_ARRAY_VECTOR, X.data()},          {"V", MPMX_ARRAY_VECTOR, V.data()},
+        {"F", MPMX_ARRAY_MATRIX, F.data()},          {"mass", MPMX_ARRAY_SCALAR, mass.data()},
+        {"volume", MPMX_ARRAY_SCALAR, vol.data()},   {"valid", MPMX_ARRAY_BOOL, valid.data()},
+        {"mu", MPMX_ARRAY_SCALAR, mu.data()},        {"lambda", MPMX_ARRAY_SCALAR, la.data()},
+        {"B", MPMX_ARRAY_MATRIX, B.data()},          {"Fp", MPMX_ARRAY_MATRIX, Fp.data()},
+        {"mu0", MPMX_ARRAY_SCALAR, mu0.data()},      {"lambda0", MPMX_ARRAY_SCALAR, la0.data()},
+        {"plastic_deformation", MPMX_ARRAY_SCALAR, pdef.data()}, {"dp_rho_f", MPMX_ARRAY_SCALAR, rho.data()},
+        {"dp_cohesion", MPMX_ARRAY_SCALAR, coh.data()}, {"vcp", MPMX_ARRAY_SCALAR, vc.data()}};
+    std::vector<int> found(a.size());
+    mpmx::io::read_particles(path.c_str(), d, prec, a.data(), (int)a.size(), found.data());
+    for (int i = 0; i < 8; i++)
+      if (!found[i]) throw mpmx::Error{MPMX_EINVAL, std::string("particle file lacks ") + a[i].name};
+    const double t = mpmx::io::read_restart(rpath.c_str());
+    e.set_particles(n, X.data(), V.data(), F.data(), B.data(), mass.data(), vol.data(), mu.data(), la.data(),
+                    valid.data());
+    if (e.plastic_kind())
+      e.restore_plastic(found[9] ? Fp.data() : nullptr, found[10] ? mu0.data() : nullptr,
+                        found[11] ? la0.data() : nullptr, found[12] ? pdef.data() : nullptr,
+                        found[13] ? rho.data() : nullptr, found[14] ? coh.data() : nullptr,
+                        found[15] ? vc.data() : nullptr);
+    e.set_time(t);
+  });
 }
 
 int mpmx_set_drucker_prager(mpmx_ctx* ctx, int enable, double a0, double a1, double a3, double a4, double cohesion,

@@ -22,6 +22,12 @@ struct EngineBase {
   virtual int scene_count() const = 0;
   virtual void get_particles(void* X, void* V, void* F, void* B, uint8_t* valid) = 0;
   virtual int64_t particle_count() const = 0;
+  virtual void get_material(void* mass, void* volume, void* mu, void* lambda) = 0;
+  virtual void set_time(double t) = 0;
+  virtual int plastic_kind() const = 0;
+  virtual void get_lame0(void* mu0, void* lambda0) = 0;
+  virtual void restore_plastic(const void* Fp, const void* mu0, const void* lambda0, const void* plastic_def,
+                               const void* rho_F, const void* cohesion, const void* vc) = 0;
   virtual void set_constitutive(int model, double neo_threshold) = 0;
   virtual void set_gravity(const double* g) = 0;
   virtual void add_collision_box(int type, double friction, const double* lo, const double* hi) = 0;

@@ -2847,6 +2847,7 @@ class Engine final : public EngineBase {
       HIPCHECK(hipMemcpyAsync(cur_.valid, valid, n_, hipMemcpyHostToDevice, stream_));
       if (scene_ids) HIPCHECK(hipMemcpyAsync(cur_.pscene, scene_ids, n_ * sizeof(uint16_t), hipMemcpyHostToDevice, stream_));
       iota_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_.pid, n_);
+      user_order_ = true;
       if (plastic_)
         plastic_init_kernel<T, D><<<std::min<uint32_t>(cdiv(n_, BLOCK), 4096), BLOCK, 0, stream_>>>(cur_, n_, pl_);
       HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
@@ -2879,6 +2880,39 @@ class Engine final : public EngineBase {
       HIPCHECK(hipMemcpyAsync(valid, stage_bytes_.p, n_, hipMemcpyDeviceToHost, stream_));
     }
     HIPCHECK(hipStreamSynchronize(stream_));
+  }
+
+  void get_material(void* mass, void* volume, void* mu, void* lambda) override {
+    if (n_ == 0) return;
+    download_aos(cur_.mass, (T*)mass, 1);
+    download_aos(cur_.vol, (T*)volume, 1);
+    download_aos(cur_.mu, (T*)mu, 1);
+    download_aos(cur_.la, (T*)lambda, 1);
+  }
+  void set_time(double t) override { time_ = t; }
+  int plastic_kind() const override { return plastic_; }
+  void get_lame0(void* mu0, void* lambda0) override {
+    if (!plastic_) throw Error{MPMX_ESTATE, "plasticity is not enabled"};
+    if (n_ == 0) return;
+    download_aos(cur_.mu0, (T*)mu0, 1);
+    download_aos(cur_.la0, (T*)lambda0, 1);
+  }
+  // Plastic state of the particles just given to set_particles (user order),
+  // as a restart reads it; null arrays keep the values set_particles gave.
+  void restore_plastic(const void* Fp, const void* mu0, const void* lambda0, const void* plastic_def,
+                       const void* rho_F, const void* cohesion, const void* vc) override {
+    if (!plastic_) throw Error{MPMX_ESTATE, "plasticity is not enabled"};
+    if (!user_order_) throw Error{MPMX_ESTATE, "plastic state is restored right after mpmx_set_particles"};
+    if (n_ == 0) return;
+    if (Fp) upload_soa((const T*)Fp, cur_.Fp, D * D);
+    if (mu0) upload_soa((const T*)mu0, cur_.mu0, 1);
+    if (lambda0) upload_soa((const T*)lambda0, cur_.la0, 1);
+    if (plastic_ == 2) {
+      if (plastic_def) upload_soa((const T*)plastic_def, cur_.pdef, 1);
+      if (rho_F) upload_soa((const T*)rho_F, cur_.rhoF, 1);
+      if (cohesion) upload_soa((const T*)cohesion, cur_.coh, 1);
+      if (vc) upload_soa((const T*)vc, cur_.vc, 1);
+    }
   }
 
   void set_constitutive(int model, double thr) override {
@@ -3563,6 +3597,7 @@ class Engine final : public EngineBase {
     if (!taping_ && (force_reorder_ || since_reorder_ >= reorder_interval_)) {
       hipEvent_t er = timer_.begin(stream_);
       reorder_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_, alt_, order_.p, n_);
+      user_order_ = false;
       std::swap(cur_, alt_);
       std::swap(fa_, fb_);
       std::swap(va_, vb_);
@@ -3916,6 +3951,7 @@ class Engine final : public EngineBase {
   uint32_t* gt_other_ = nullptr;
   uint32_t *ml_cur_ = nullptr, *mt_cur_ = nullptr, *ml_next_ = nullptr, *mt_next_ = nullptr;
   int plastic_ = 0;  // plasticity after every G2P (unfused steps): 1 clamp, 2 Drucker-Prager
+  bool user_order_ = true;  // particle slots are in user order (no reorder since set_particles)
   PlasticDev<T> pl_{};
   DevBuf<uint8_t> simmask_;
   bool fused_ = false;       // gcur_ holds the P2G of the current state with dt fused_dt_

@@ -20,13 +20,15 @@ SHAPE_BOX, SHAPE_SPHERE = 0, 1
 # Every exported symbol of the C ABI, checked by the CPU test suite.
 EXPORTS = (
     "mpmx_last_error", "mpmx_create", "mpmx_destroy", "mpmx_create_batch", "mpmx_scene_count",
-    "mpmx_set_batch_particles", "mpmx_set_particles", "mpmx_get_particles",
+    "mpmx_set_batch_particles", "mpmx_set_particles", "mpmx_get_particles", "mpmx_get_material",
     "mpmx_particle_count", "mpmx_set_constitutive", "mpmx_set_gravity", "mpmx_add_collision_box",
     "mpmx_clear_collision_objects", "mpmx_step", "mpmx_step_async", "mpmx_synchronize", "mpmx_time",
     "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_grid_node_count",
     "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_set_rebin_policy", "mpmx_bin_stats",
     "mpmx_grid_v_upper_bound", "mpmx_compute_dt", "mpmx_advance_frames",
     "mpmx_set_dt_limits", "mpmx_max_sound_speed", "mpmx_last_dt",
+    "mpmx_write_particles_file", "mpmx_read_particles_file", "mpmx_write_output_files", "mpmx_read_output_files",
+    "mpmx_write_restart_file", "mpmx_read_restart_file",
     "mpmx_set_plasticity", "mpmx_get_plastic_state", "mpmx_set_drucker_prager", "mpmx_get_drucker_prager_state",
     "mpmx_tape_begin", "mpmx_backward",
     "mpmx_enable_timing", "mpmx_reset_timing",
@@ -44,6 +46,13 @@ class MpmxError(RuntimeError):
 class GridDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("precision", ctypes.c_int), ("cells", ctypes.c_int * 3),
                 ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3)]
+
+
+ARRAY_SCALAR, ARRAY_VECTOR, ARRAY_VECTOR3, ARRAY_MATRIX, ARRAY_BOOL = 0, 1, 2, 3, 4
+
+
+class ParticleArray(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("kind", ctypes.c_int), ("data", ctypes.c_void_p)]
 
 
 class SeedDesc(ctypes.Structure):
@@ -98,6 +107,14 @@ def lib():
         L.mpmx_advance_frames.argtypes = [vp, c_int, dbl, dbl, dbl, dbl, vp, i64, vp]
         L.mpmx_set_plasticity.argtypes = [vp, c_int, dbl, dbl, dbl, dbl]
         L.mpmx_get_plastic_state.argtypes = [vp, vp, vp, vp]
+        L.mpmx_get_material.argtypes = [vp, vp, vp, vp, vp]
+        L.mpmx_write_particles_file.argtypes = [ctypes.c_char_p, c_int, c_int, i64, ctypes.POINTER(ParticleArray), c_int]
+        L.mpmx_read_particles_file.argtypes = [ctypes.c_char_p, c_int, c_int, ctypes.POINTER(ctypes.c_int64),
+                                               ctypes.POINTER(ParticleArray), c_int, vp]
+        L.mpmx_write_output_files.argtypes = [vp, ctypes.c_char_p]
+        L.mpmx_write_restart_file.argtypes = [ctypes.c_char_p, c_int, dbl]
+        L.mpmx_read_restart_file.argtypes = [ctypes.c_char_p, vp]
+        L.mpmx_read_output_files.argtypes = [vp, ctypes.c_char_p]
         L.mpmx_set_dt_limits.argtypes = [vp, c_int, c_int, dbl, dbl, dbl, dbl]
         L.mpmx_max_sound_speed.argtypes = [vp, vp]
         L.mpmx_last_dt.argtypes = [vp, vp]
@@ -126,6 +143,53 @@ def _p(a):
 def _check(rc):
     if rc != 0:
         raise MpmxError(rc, lib().mpmx_last_error().decode())
+
+
+def _array_shape(kind, dim):
+    return {ARRAY_SCALAR: (), ARRAY_VECTOR: (dim,), ARRAY_VECTOR3: (3,), ARRAY_MATRIX: (dim * dim,),
+            ARRAY_BOOL: ()}[kind]
+
+
+def write_particles_file(path, dim, precision, arrays):
+    """PARTICLES<TV>::Write of ``arrays`` = {name: (kind, array)} to ``path`` (gzip if it ends in .gz)."""
+    dt = np.float64 if precision == FP64 else np.float32
+    keep, descs, n = [], [], None
+    for name, (kind, a) in arrays.items():
+        a = np.ascontiguousarray(a, np.uint8 if kind == ARRAY_BOOL else dt)
+        m = a.shape[0]
+        n = m if n is None else n
+        width = int(np.prod(_array_shape(kind, dim)))
+        if m != n or a.size != m * width:
+            raise MpmxError(-1, f"array {name} has the wrong shape")
+        keep.append(a)
+        descs.append(ParticleArray(name.encode(), kind, a.ctypes.data))
+    arr = (ParticleArray * len(descs))(*descs)
+    _check(lib().mpmx_write_particles_file(os.fsencode(path), dim, precision, n or 0, arr, len(descs)))
+
+
+def read_particles_file(path, dim, precision, kinds):
+    """PARTICLES<TV>::Read of the arrays named in ``kinds`` = {name: kind}; missing ones are left out."""
+    dt = np.float64 if precision == FP64 else np.float32
+    n = ctypes.c_int64(0)
+    _check(lib().mpmx_read_particles_file(os.fsencode(path), dim, precision, ctypes.byref(n), None, 0, None))
+    out = {k: np.zeros((n.value,) + _array_shape(kind, dim), np.uint8 if kind == ARRAY_BOOL else dt)
+           for k, kind in kinds.items()}
+    descs = [ParticleArray(k.encode(), kinds[k], out[k].ctypes.data) for k in kinds]
+    arr = (ParticleArray * len(descs))(*descs)
+    found = np.zeros(len(descs), np.int32)
+    _check(lib().mpmx_read_particles_file(os.fsencode(path), dim, precision, ctypes.byref(n), arr, len(descs),
+                                          found.ctypes.data))
+    return {k: out[k] for i, k in enumerate(kinds) if found[i]}
+
+
+def write_restart_file(path, precision, time):
+    _check(lib().mpmx_write_restart_file(os.fsencode(path), precision, float(time)))
+
+
+def read_restart_file(path):
+    out = np.zeros(1)
+    _check(lib().mpmx_read_restart_file(os.fsencode(path), out.ctypes.data))
+    return float(out[0])
 
 
 class Context:
@@ -301,6 +365,20 @@ class Context:
         _check(lib().mpmx_advance_frames(self._h, int(nframes), float(frame_dt), float(cfl), float(min_dt),
                                          float(max_dt), _p(dts), ctypes.c_int64(capacity), _p(cnt)))
         return dts[:min(int(cnt[0]), capacity)].copy()
+
+    def get_material(self):
+        """dict mass, volume, mu, lambda of the particles (user order)."""
+        out = {k: np.zeros(self.n, self.dtype) for k in ("mass", "volume", "mu", "lambda")}
+        _check(lib().mpmx_get_material(self._h, *(_p(out[k]) for k in ("mass", "volume", "mu", "lambda"))))
+        return out
+
+    def write_output_files(self, frame_dir):
+        """MPM_EXAMPLE::Write_Output_Files' particle and restart files into ``frame_dir``."""
+        _check(lib().mpmx_write_output_files(self._h, os.fsencode(frame_dir)))
+
+    def read_output_files(self, frame_dir):
+        """MPM_EXAMPLE::Read_Output_Files: particles and time from ``frame_dir``."""
+        _check(lib().mpmx_read_output_files(self._h, os.fsencode(frame_dir)))
 
     def set_dt_limits(self, strong_cfl=False, sound_cfl=False, cfl=1.0, cfl_F=0.5, cfl_sound=0.9, min_dt=0.0):
         """Strong and sound-speed CFL (MPM_DRIVER::Reduce_Dt) inside every step."""

@@ -145,6 +145,49 @@ def write_limits():
     np.savez_compressed(os.path.join(HERE, "limits.npz"), **out)
 
 
+def io_cases():
+    """Scenes whose particle and restart files the reference writes after a
+    few steps (MPM_EXAMPLE::Write_Output_Files' particle part).
+
+    Each entry: scene, steps."""
+    sq = lambda X: (np.abs(X[:, 0] - 0.5) <= 0.15) & (np.abs(X[:, 1] - 0.45) <= 0.15)
+    sph = lambda X: ((X - 0.5) ** 2).sum(1) < 0.25 ** 2
+    cases = {}
+    walls = mt.wall_boxes(2, [32, 32], [0, 0], [1, 1], flags=15, inset=0.1, type_=mt.STICK, friction=0.0)
+    cases["io2d_f64"] = (mt.make_scene(2, 32, prec=8, model=mt.COROTATED, shape=sq, v0=[1.0, -3.0], vrand=0.5,
+                                       frand=0.05, walls=walls, dt=5e-4, seed=61), 3)
+    walls = mt.wall_boxes(3, [12] * 3, [0] * 3, [1] * 3, flags=63, inset=0.12, type_=mt.SLIP, friction=0.0)
+    cases["io3d_f32"] = (mt.make_scene(3, 12, prec=4, model=mt.NEO_HOOKEAN, shape=sph, v0=[0.3, -2.0, 0.1],
+                                       vrand=0.5, frand=0.05, walls=walls, dt=5e-4, per_particle_E=True, seed=62), 2)
+    walls = mt.wall_boxes(2, [32, 32], [0, 0], [1, 1], flags=15, inset=0.1, type_=mt.STICK, friction=0.0)
+    sc = mt.make_scene(2, 32, prec=8, model=mt.COROTATED, shape=sq, v0=[0.5, -3.0], vrand=0.5, frand=0.08,
+                       walls=walls, dt=2e-4, seed=63)
+    sc["plasticity"] = dict(theta_c=0.025, theta_s=0.0075, max_hardening=5.0, hardening_factor=10.0)
+    cases["io2d_snow_f64"] = (sc, 2)
+    return cases
+
+
+def write_io():
+    out = {}
+    for name, (sc, nsteps) in io_cases().items():
+        for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+            out[f"{name}_in_{k}"] = np.asarray(sc[k])
+        meta = {k: sc[k] for k in ("dim", "prec", "model", "N", "lo", "hi", "gravity", "dt", "neo_threshold")}
+        meta.update(nsteps=nsteps)
+        if "plasticity" in sc:
+            meta["plasticity"] = sc["plasticity"]
+        out[f"{name}_meta"] = np.array(json.dumps(meta, default=float))
+        out[f"{name}_walls"] = np.array(json.dumps(sc["walls"], default=float))
+        parts, restart = scene_io.run_ref_write(sc, nsteps)
+        out[f"{name}_particles_gz"] = np.frombuffer(parts, np.uint8)
+        out[f"{name}_restart_gz"] = np.frombuffer(restart, np.uint8)
+        r = scene_io.run_ref_read(sc["prec"], sc["dim"], parts, restart)
+        for k, v in r.items():
+            out[f"{name}_read_{k}"] = np.asarray(v)
+        print("io", name, sc["X"].shape[0], "particles,", len(parts), "bytes gz, time", r["time"])
+    np.savez_compressed(os.path.join(HERE, "io.npz"), **out)
+
+
 def plastic_cases():
     """Scenes with MPM_PLASTICITY_CLAMP (Add_Clamped_Plasticity); F is perturbed
     so that singular values leave [1 - theta_c, 1 + theta_s] from the first step."""
@@ -201,13 +244,15 @@ def write_plastic():
 def main():
     if not scene_io.ref_available():
         sys.exit("reference harness missing: build it with oracle/ref/Makefile")
-    groups = set(sys.argv[1:]) or {"steps", "components", "frames", "plastic", "limits"}
+    groups = set(sys.argv[1:]) or {"steps", "components", "frames", "plastic", "limits", "io"}
     if "frames" in groups:
         write_frames()
     if "plastic" in groups:
         write_plastic()
     if "limits" in groups:
         write_limits()
+    if "io" in groups:
+        write_io()
     if "steps" in groups:
         write_steps()
     if "components" in groups:

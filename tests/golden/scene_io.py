@@ -151,6 +151,60 @@ def run_ref_limited(sc, nsteps, lim, threads=1):
     return _frames_output(sc, buf)
 
 
+def run_ref_write(sc, nsteps, threads=1):
+    """The reference's particle and restart files after ``nsteps`` steps:
+    (deformable_object_particles.gz bytes, restart_data.gz bytes)."""
+    with tempfile.TemporaryDirectory() as td:
+        sp = os.path.join(td, "scene.bin")
+        write_scene(sp, sc, nsteps, nsteps, threads)
+        subprocess.run([REF_HARNESS, "write", sp, td], check=True, stdout=subprocess.DEVNULL)
+        with open(os.path.join(td, "deformable_object_particles.gz"), "rb") as f:
+            parts = f.read()
+        with open(os.path.join(td, "restart_data.gz"), "rb") as f:
+            restart = f.read()
+    return parts, restart
+
+
+def run_ref_read(prec, dim, particles_gz, restart_gz):
+    """The reference's own Read_From_File of a frame directory holding the two
+    files: dict X, V, F, B, valid, mass, volume, mu, lambda (Fp, mu0, lambda0
+    when stored) and time."""
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "deformable_object_particles.gz"), "wb") as f:
+            f.write(particles_gz)
+        with open(os.path.join(td, "restart_data.gz"), "wb") as f:
+            f.write(restart_gz)
+        op = os.path.join(td, "out.bin")
+        subprocess.run([REF_HARNESS, "read", str(prec), str(dim), td, op], check=True, stdout=subprocess.DEVNULL)
+        with open(op, "rb") as f:
+            buf = f.read()
+    n, = struct.unpack_from("<i", buf, 0)
+    off = 4
+    out = {}
+
+    def take(k, count, dtype=np.float64):
+        nonlocal off
+        a = np.frombuffer(buf, dtype, count=count, offset=off).copy()
+        off += a.nbytes
+        out[k] = a
+
+    d = dim
+    take("X", n * d); take("V", n * d); take("F", n * d * d); take("B", n * d * d)
+    take("valid", n, np.uint8)
+    take("mass", n); take("volume", n); take("mu", n); take("lambda", n)
+    has_fp, = struct.unpack_from("<i", buf, off)
+    off += 4
+    if has_fp:
+        take("Fp", n * d * d); take("mu0", n); take("lambda0", n)
+    take("time", 1)
+    for k, w in (("X", d), ("V", d), ("F", d * d), ("B", d * d), ("Fp", d * d)):
+        if k in out:
+            out[k] = out[k].reshape(n, w)
+    out["time"] = float(out["time"][0])
+    assert off == len(buf)
+    return out
+
+
 def _frames_output(sc, buf):
     assert buf[:7] == b"MPMFRM1"
     nd, d, n, nn = struct.unpack_from("<4i", buf, 8)
