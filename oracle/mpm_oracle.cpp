@@ -462,6 +462,12 @@ template<class T,int d> struct Scene
     // cfl_sound, min_dt, use_strong_cfl, use_sound_speed_cfl
     int strong_cfl=0,sound_cfl=0;
     T cfl=1,cfl_F=(T).5,cfl_sound=(T).9,min_dt=0;
+    // reflection boundary conditions (MPM_EXAMPLE.h:106-109, :134):
+    // reflection_bc, use_full_reflection, reflection_bc_friction and
+    // side_bc_type per side -x, +x, -y, +y, -z, +z (0 free, 1 slip, 2 noslip, 3 separate)
+    int reflect=0,full_reflect=0;
+    T reflect_friction=0;
+    int side_type[6]={0,0,0,0,0,0};
     int gn[3];        // nodes per axis including 3 ghost layers each side
     long long nn;     // total nodes
 };
@@ -587,6 +593,106 @@ template<class T,int d> Mat<T,d> Stress_PFT(const Scene<T,d>& s,const Mat<T,d>& 
     return U*DT;
 }
 
+//#####################################################################
+// Reflection boundary conditions (MPM_DRIVER.cpp:310-446).  Each visits the
+// ghost nodes of the domain side by side (-x, +x, -y, +y, -z, +z), each side
+// over its full slab, corners included (RANGE_ITERATOR with ghost |
+// duplicate_corners, Core/Math_Tools/RANGE_ITERATOR.cpp:125-150), pairing a
+// ghost node with its mirror image across the side: index 2 r - i - 1 along
+// the side's axis, r = 0 or N.
+//#####################################################################
+enum{BC_FREE=0,BC_SLIP=1,BC_NOSLIP=2,BC_SEP=3};
+template<class T,int d,class FN> void For_Ghost_Pairs(const Scene<T,d>& s,int side,FN fn)
+{
+    const int axis=side/2,hi=side%2,N=s.N[axis];
+    int lo[d],up[d];
+    for(int a=0;a<d;a++){lo[a]=-3;up[a]=s.N[a]+3;}
+    lo[axis]=hi?N:-3;up[axis]=hi?N+3:0;
+    int idx[d];for(int a=0;a<d;a++) idx[a]=lo[a];
+    while(true){
+        int img[d];for(int a=0;a<d;a++) img[a]=idx[a];
+        img[axis]=2*(hi?N:0)-idx[axis]-1;
+        fn(Flat(s,img),Flat(s,idx),axis,hi);
+        int a=d-1;
+        while(a>=0 && ++idx[a]==up[a]){idx[a]=lo[a];a--;}
+        if(a<0) break;}
+}
+// Reflect_Boundary_Mass_Momentum (:310-342) without a boundary velocity; the
+// same map on a force field is Reflect_Boundary_Force (:422-446).
+template<class T,int d> void Reflect_Pair_Vectors(int type,int axis,Vec<T,d>& vi,Vec<T,d>& vo)
+{
+    Vec<T,d> id=vi-vo,od=(T)-1*id,in=vi+vo,on=in;
+    if(type==BC_SLIP){in[axis]=id[axis];on[axis]=od[axis];}
+    if(type==BC_NOSLIP){vi=id;vo=od;}
+    else{vi=in;vo=on;}
+}
+template<class T,int d> void Reflect_Boundary_Mass_Momentum(const Scene<T,d>& s,Grid<T,d>& G)
+{
+    for(int side=0;side<2*d;side++)
+        For_Ghost_Pairs(s,side,[&](long long i,long long o,int axis,int hi){
+            T mt=G.mass[i]+G.mass[o];
+            Reflect_Pair_Vectors(s.side_type[side],axis,G.velocity[i],G.velocity[o]);
+            G.mass[o]=G.mass[i]=mt;});
+}
+template<class T,int d> void Reflect_Boundary_Force(const Scene<T,d>& s,std::vector<Vec<T,d> >& f)
+{
+    for(int side=0;side<2*d;side++)
+        For_Ghost_Pairs(s,side,[&](long long i,long long o,int axis,int hi){
+            Reflect_Pair_Vectors(s.side_type[side],axis,f[i],f[o]);});
+}
+// Reflect_Boundary_Velocity (:395-420): the ghost node takes the mirrored
+// velocity of its image (normal part negated for slip, all of it for noslip).
+template<class T,int d> void Reflect_Boundary_Velocity(const Scene<T,d>& s,const Grid<T,d>& G,std::vector<Vec<T,d> >& u)
+{
+    for(int side=0;side<2*d;side++)
+        For_Ghost_Pairs(s,side,[&](long long i,long long o,int axis,int hi){
+            if(!G.mass[o]) return;
+            Vec<T,d> vi=u[i],od=(T)-1*vi,on=vi;
+            int type=s.side_type[side];
+            if(type==BC_SLIP) on[axis]=od[axis];
+            if(type==BC_NOSLIP) u[o]=od;
+            else u[o]=on;});
+}
+// VECTOR::Normalize: divides by the magnitude (the x axis if zero) and returns it.
+template<class T,int d> T Normalize(Vec<T,d>& v)
+{
+    T m=magnitude(v);
+    if(m){T r=1/m;for(int a=0;a<d;a++) v[a]*=r;}
+    else{v=Vec<T,d>();v[0]=1;}
+    return m;
+}
+// Reflect_Helper (:343-369) without a boundary velocity.
+template<class T,int d> void Reflect_Helper(T m_i,T m_o,Vec<T,d>& v_i,Vec<T,d>& v_o,int type,T mu,int a,int s)
+{
+    T m_t=m_i+m_o,a_i=m_i/m_t,a_o=1-a_i;
+    Vec<T,d> v_in=a_i*v_i+a_o*v_o,v_on=v_in;
+    if(type==BC_FREE){v_i=v_in;v_o=v_on;return;}
+    Vec<T,d> bc;
+    Vec<T,d> v_id=a_i*v_i+a_o*((T)2*bc-v_o),v_od=a_o*v_o+a_i*((T)2*bc-v_i);
+    if(type==BC_NOSLIP){v_i=v_id;v_o=v_od;return;}
+    if(type==BC_SLIP){v_in[a]=v_id[a];v_on[a]=v_od[a];v_i=v_in;v_o=v_on;return;}
+    T d_v=2*a_o*a_i*(2*bc[a]-v_i[a]-v_o[a]);
+    T d_v_n=s?-d_v:d_v;
+    if(d_v_n<0) return;
+    Vec<T,d> v_id_n,v_od_n;
+    v_id_n[a]=v_id[a];
+    v_od_n[a]=v_od[a];
+    v_in[a]=0;
+    T d_v_t_mag=Normalize(v_in);
+    T r=d_v_t_mag-mu*d_v_n;
+    if(r<0){v_i=v_id_n;v_o=v_od_n;return;}
+    v_i=v_id_n+r*v_in;v_o=v_od_n+r*v_in;
+}
+// Reflect_Boundary_Friction (:371-393): velocity pairs where both nodes hold mass.
+template<class T,int d> void Reflect_Boundary_Friction(const Scene<T,d>& s,Grid<T,d>& G)
+{
+    for(int side=0;side<2*d;side++)
+        For_Ghost_Pairs(s,side,[&](long long i,long long o,int axis,int hi){
+            T m_i=G.mass[i],m_o=G.mass[o];
+            if(!m_i || !m_o) return;
+            Reflect_Helper(m_i,m_o,G.velocity[i],G.velocity[o],s.side_type[side],s.reflect_friction,axis,hi);});
+}
+
 // MPM_DRIVER::Grid_To_Particle_Limit_Dt (MPM_DRIVER.cpp:572-620) without the
 // midpoint rule: s = min over the simulated particles of the largest t <= 100
 // with |t (dt grad v_old + t dt grad(v_new - v_old))| <= cfl_F per entry and
@@ -688,6 +794,7 @@ template<class T,int d> T Step(const Scene<T,d>& s,Particles<T,d>& P,Grid<T,d>& 
             for(int i=0;i<d;i++){T acc=(i==0?dpinv[0]:(T)0)*Y[0];for(int j=1;j<d;j++) acc+=(i==j?dpinv[i]:(T)0)*Y[j];DY[i]=acc;}
             Vv=Vv+(Bp*DY)*w;
             G.velocity[f]=G.velocity[f]+mp*Vv;}}
+    if(s.reflect && !s.full_reflect) Reflect_Boundary_Mass_Momentum(s,G);
     G.valid.clear();
     for(long long i=0;i<nn;i++){
         if(G.mass[i]!=0){
@@ -716,6 +823,7 @@ template<class T,int d> T Step(const Scene<T,d>& s,Particles<T,d>& P,Grid<T,d>& 
             Stencil(W[p],k,idx,w,g);
             long long f=Flat(s,idx);
             G.tmp2[f]=G.tmp2[f]+(w*P.mass[p])*s.gravity;}}
+    if(s.reflect && !s.full_reflect) Reflect_Boundary_Force(s,G.tmp2);
     for(long long i:G.valid) G.dv[i]=(dt/G.mass[i])*G.tmp2[i];
     for(long long i:G.valid) G.tmp1[i]=G.dv[i];
 
@@ -746,6 +854,10 @@ template<class T,int d> T Step(const Scene<T,d>& s,Particles<T,d>& P,Grid<T,d>& 
             if(type==0){Vn=Vec<T,d>();stuck.push_back(i);}
             G.dv[i]=(Vn-G.v0[i])*((T)1/(T)1);}}
     for(long long i:G.valid) G.tmp0[i]=G.dv[i];
+    if(s.reflect && s.full_reflect){  // MPM_DRIVER.cpp:999-1019
+        for(long long j:G.valid) G.velocity[j]=G.dv[j]+G.v0[j];
+        Reflect_Boundary_Friction(s,G);
+        for(long long j:G.valid) G.dv[j]=G.velocity[j]-G.v0[j];}
 
     // Apply_Friction (MPM_DRIVER.cpp:1064-1094)
     for(long long j:G.valid) G.velocity_friction_save[j]=G.dv[j];
@@ -768,6 +880,9 @@ template<class T,int d> T Step(const Scene<T,d>& s,Particles<T,d>& P,Grid<T,d>& 
         G.velocity[j]=G.dv[j]+G.v0[j];
         G.velocity_friction_save[j]=G.velocity_friction_save[j]+G.v0[j];}
     for(long long j:stuck) G.velocity_friction_save[j]=Vec<T,d>();
+    if(s.reflect && !s.full_reflect){  // MPM_DRIVER.cpp:1057-1059
+        Reflect_Boundary_Velocity(s,G,G.velocity);
+        Reflect_Boundary_Velocity(s,G,G.velocity_friction_save);}
 
     const T dtg=(s.strong_cfl || s.sound_cfl)?Reduce_Dt(s,P,G,W,sim):dt;
 
@@ -801,9 +916,12 @@ template<class T,int d> T Step(const Scene<T,d>& s,Particles<T,d>& P,Grid<T,d>& 
         Vec<T,d> Vnew=V_flip*(T)0+V_pic_fric*(T)1;
         for(int q=0;q<d*d;q++){P.F[p*d*d+q]=Fn.x[q];P.B[p*d*d+q]=Bh.x[q];}
         for(int a=0;a<d;a++){P.X[p*d+a]=xp_new[a];P.V[p*d+a]=Vnew[a];}
-        for(int a=0;a<d;a++){
-            if(P.X[p*d+a]<A[a]) P.valid[p]=0;
-            else if(P.X[p*d+a]>Bd[a]) P.valid[p]=0;}}
+        for(int a=0;a<d;a++){  // sides with a boundary condition reflect the particle
+            const bool rl=s.side_type[2*a]!=BC_FREE,rh=s.side_type[2*a+1]!=BC_FREE;
+            T& x=P.X[p*d+a];
+            T& v=P.V[p*d+a];
+            if(x<A[a]){if(rl){x=2*A[a]-x;v=-v;}else P.valid[p]=0;}
+            else if(x>Bd[a]){if(rh){x=2*Bd[a]-x;v=-v;}else P.valid[p]=0;}}}
     return dtg;
 }
 
@@ -1058,6 +1176,22 @@ template<class T,int d> void Set_Limits(Scene<T,d>& s,const double* lim,double c
     s.strong_cfl=lim[0]!=0;s.sound_cfl=lim[1]!=0;s.cfl_F=(T)lim[2];s.cfl_sound=(T)lim[3];
 }
 
+// Fixed-dt steps with reflection boundary conditions.
+template<class T,int d> int Run_Reflect(const int* N,const double* lo,const double* hi,const double* g,int model,double thr,
+    double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi,
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,const T* mu,const T* la,unsigned char* valid,int nsteps,
+    const double* refl)
+{
+    Scene<T,d> s=Make_Scene<T,d>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi);
+    s.reflect=1;s.full_reflect=refl[0]!=0;s.reflect_friction=(T)refl[1];
+    for(int i=0;i<2*d;i++) s.side_type[i]=(int)refl[2+i];
+    Particles<T,d> P{np,X,V,F,B,mass,volume,mu,la,valid};
+    Grid<T,d> G;
+    T time=0;
+    for(int k=0;k<nsteps;k++) time+=Step(s,P,G,time);
+    return 0;
+}
+
 // Fixed-dt steps with Reduce_Dt: dts[k] is the dt step k took.
 template<class T,int d> int Run_Limited(const int* N,const double* lo,const double* hi,const double* g,int model,double thr,
     double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi,
@@ -1179,6 +1313,22 @@ ORACLE_API int oracle_run_limited_##SUF(int dim,const int* N,const double* lo,co
 }
 DEFINE_LIMITED(f64,double)
 DEFINE_LIMITED(f32,float)
+
+// refl[2+2d] = use_full_reflection, reflection_bc_friction, side_bc_type per side.
+#define DEFINE_REFLECT(SUF,T) \
+ORACLE_API int oracle_run_reflect_##SUF(int dim,const int* N,const double* lo,const double* hi,const double* g, \
+    int model,double thr,double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi, \
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,const T* mu,const T* la,unsigned char* valid,int nsteps, \
+    const double* refl) \
+{ \
+    if(dim==2) return Run_Reflect<T,2>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume,mu,la, \
+        valid,nsteps,refl); \
+    if(dim==3) return Run_Reflect<T,3>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume,mu,la, \
+        valid,nsteps,refl); \
+    return -1; \
+}
+DEFINE_REFLECT(f64,double)
+DEFINE_REFLECT(f32,float)
 
 // Sound_Speed of each particle from the singular values of its F (n x d*d).
 #define DEFINE_SOUND(SUF,T) \

@@ -6,7 +6,9 @@
 // vectors and, as a binary under oracle/_ref/, to time the reference CPU step.
 //
 // Modes
-//   ref_harness step  <scene.bin> <out.bin>
+//   ref_harness step  <scene.bin> <out.bin> [reflect <full> <friction> <side types...>]
+//     (with "reflect": reflection_bc on, use_full_reflection = full,
+//     reflection_bc_friction, side_bc_type of -x, +x, -y, +y[, -z, +z])
 //   ref_harness time  <scene.bin> <threads> <warmup> <steps>
 //   ref_harness svd   <prec> <dim> <in.bin> <out.bin>
 //   ref_harness pstrain <prec> <dim> <model> <threshold> <in.bin> <out.bin>
@@ -89,6 +91,8 @@ struct SCENE
     double lo[3],hi[3],gravity[3],dt,neo_threshold;
     double theta_c,theta_s,max_hardening,hardening_factor;  // clamped plasticity (plastic == 1)
     double dp[6];  // Drucker-Prager (plastic == 2): a0, a1, a3, a4, sigma_Y0, use_vc
+    int reflect=0,full_reflect=0,side_type[6]={0,0,0,0,0,0};  // from the command line
+    double reflect_friction=0;
     long long np;
     std::vector<WALL> walls;
     std::vector<double> X,V,F,B,mass,volume,mu,lambda;
@@ -185,6 +189,11 @@ struct RUN
                 (T)s.max_hardening,(T)s.hardening_factor);
             ex.plasticity_models.Append(pl);}
         ex.Add_Force(*new MPM_GRAVITY<TV>(ex.force_helper,g,ex.gather_scatter,0));
+        if(s.reflect){  // as -use_reflect [-use_reflect_friction] with Add_Walls (STANDARD_TESTS_BASE.cpp:487-511)
+            ex.reflection_bc=true;
+            ex.use_full_reflection=s.full_reflect!=0;
+            ex.reflection_bc_friction=(T)s.reflect_friction;
+            for(int i=0;i<2*d;i++) ex.side_bc_type(i)=(typename MPM_EXAMPLE<TV>::BC_TYPE)s.side_type[i];}
         for(const WALL& w:s.walls){
             TV a,b;for(int k=0;k<d;k++){a(k)=(T)w.lo[k];b(k)=(T)w.hi[k];}
             ex.Add_Collision_Object(RANGE<TV>(a,b),(typename MPM_EXAMPLE<TV>::COLLISION_TYPE)w.type,(T)w.friction);}
@@ -492,8 +501,13 @@ int main(int argc,char* argv[])
     if(argc<2){fprintf(stderr,"usage: ref_harness step|time|svd|pstrain|weights ...\n");return 2;}
     std::string mode=argv[1];
     if(mode=="step" || mode=="time"){
-        if((mode=="step" && argc!=4) || (mode=="time" && argc!=6)){fprintf(stderr,"bad arguments\n");return 2;}
+        bool refl=mode=="step" && argc>4 && std::string(argv[4])=="reflect";
+        if((mode=="step" && argc!=4 && !refl) || (mode=="time" && argc!=6)){fprintf(stderr,"bad arguments\n");return 2;}
         SCENE s=Read_Scene(argv[2]);
+        if(refl){
+            if(argc!=7+2*s.dim){fprintf(stderr,"bad reflect arguments\n");return 2;}
+            s.reflect=1;s.full_reflect=atoi(argv[5]);s.reflect_friction=atof(argv[6]);
+            for(int i=0;i<2*s.dim;i++) s.side_type[i]=atoi(argv[7+i]);}
         int rc=2;
         if(s.prec==8){
             STREAM_TYPE st((double()));

@@ -87,6 +87,11 @@ template<typename T, int D> struct Args {
   uint32_t mcap;
   uint32_t* err;
   double* stats_partial;
+  // reflection boundary conditions: bit 2a (2a+1) set when particles leaving
+  // through the low (high) side of axis a are reflected instead of
+  // invalidated; band: the tiles along the domain sides are always active
+  uint32_t reflect_sides;
+  int band;
 };
 
 // The geometry always comes from the scene table (one entry for one scene),
@@ -221,6 +226,13 @@ __device__ __forceinline__ bool grid_tile_flag(const Args<T, D>& a, const uint32
     r /= (uint32_t)g.GT[k];
   }
   bool f = false;
+  if (a.band) {  // tiles holding ghost nodes or their mirror images along some axis
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      constexpr int E = Tile<D>::E;  // the tile covers ghosted nodes [E c, E c + E)
+      f = f || E * c[k] < 6 || E * c[k] + E > g.G[k] - 6;
+    }
+  }
 #pragma unroll
   for (int m = 0; m < (1 << D); m++) {
     bool ok = true;
@@ -1192,15 +1204,32 @@ __device__ __forceinline__ bool g2p_store(const Args<T, D>& a, const Geom<T, D>&
       for (int q = 0; q < D; q++) s += a.dt * G[c + D * q] * Fm[q + D * j];
       Fn[c + D * j] = s;
     }
-  T xn[D];
+  T xn[D], vn[D];
   bool ok = true;
 #pragma unroll
   for (int c = 0; c < D; c++) {
+    // Reflect_Or_Invalidate_Particle (MPM_DRIVER.cpp:1219-1239): a side with
+    // a reflection condition mirrors the particle back, others invalidate it
     xn[c] = x[c] + a.dt * vb[c];
-    if (xn[c] < g.vlo[c] || xn[c] > g.vhi[c]) ok = false;
+    vn[c] = vfb[c];
+    if (xn[c] < g.vlo[c]) {
+      if ((a.reflect_sides >> (2 * c)) & 1u) {
+        xn[c] = 2 * g.vlo[c] - xn[c];
+        vn[c] = -vn[c];
+      } else {
+        ok = false;
+      }
+    } else if (xn[c] > g.vhi[c]) {
+      if ((a.reflect_sides >> (2 * c + 1)) & 1u) {
+        xn[c] = 2 * g.vhi[c] - xn[c];
+        vn[c] = -vn[c];
+      } else {
+        ok = false;
+      }
+    }
   }
 #pragma unroll
-  for (int c = 0; c < D; c++) { a.X[c * a.n + p] = xn[c]; a.V[c * a.n + p] = vfb[c]; }
+  for (int c = 0; c < D; c++) { a.X[c * a.n + p] = xn[c]; a.V[c * a.n + p] = vn[c]; }
 #pragma unroll
   for (int k = 0; k < D * D; k++) { a.F[k * a.n + p] = Fn[k]; a.B[k * a.n + p] = Bs[k]; }
   if (!ok) {
@@ -2601,6 +2630,196 @@ __global__ __launch_bounds__(BLOCK) void dt_limit_kernel(Args<T, D> a, const Nod
     }
     if (cmax > 0) atomicMax(out, OrderedBits<T>::of(cmax));
     if (smin < 100) atomicMin(out + 1, OrderedBits<T>::of(smin));
+  }
+}
+
+// ------------------------------------------------------- reflection BC ----
+// Reflection boundary conditions (MPM_DRIVER.cpp:310-446, example.reflection_bc).
+// The reference visits the ghost nodes side by side (-x, +x, -y, +y, -z, +z),
+// each side over its whole slab with the corners (RANGE_ITERATOR with ghost |
+// duplicate_corners), pairing ghost node o with its mirror image i across
+// the side (index 2 r - o - 1 along the axis, r = 0 or N).  The pairs of one
+// axis are disjoint (N >= 6), so one launch handles both sides of an axis and
+// the axes run in order.  Side types: 0 free, 1 slip, 2 noslip, 3 separate.
+enum { RBC_FREE = 0, RBC_SLIP = 1, RBC_NOSLIP = 2, RBC_SEP = 3 };
+struct ReflectAxis {
+  int axis;
+  int type[2];  // low, high side
+};
+
+// Pair q of axis r.axis: ghost node o and image i slots, the side (0 low, 1 high).
+template<typename T, int D>
+__device__ __forceinline__ bool reflect_pair(const Geom<T, D>& g, const ReflectAxis& r, uint32_t q, uint32_t* io,
+                                             uint32_t* oo, int* hi) {
+  const int k = r.axis, N = g.G[k] - 6;
+  int nc[D], other = 1;
+#pragma unroll
+  for (int c = 0; c < D; c++)
+    if (c != k) other *= g.G[c];
+  if (q >= (uint32_t)(6 * other)) return false;
+  uint32_t rest = q % (uint32_t)other;
+  const int lay = (int)(q / (uint32_t)other);  // 0..5: side * 3 + layer
+  *hi = lay / 3;
+#pragma unroll
+  for (int c = D - 1; c >= 0; c--) {
+    if (c == k) continue;
+    nc[c] = (int)(rest % (uint32_t)g.G[c]);
+    rest /= (uint32_t)g.G[c];
+  }
+  const int o = *hi ? N + (lay % 3) : (lay % 3) - 3;  // node index (ghost excluded)
+  const int i = 2 * (*hi ? N : 0) - o - 1;
+  nc[k] = o + 3;
+  *oo = node_slot<T, D>(nc, g);
+  nc[k] = i + 3;
+  *io = node_slot<T, D>(nc, g);
+  return true;
+}
+template<typename T> __device__ __forceinline__ void reflect_vectors(int type, int axis, T* vi, T* vo) {
+  T id[3], od[3], in[3], on[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    id[c] = vi[c] - vo[c];
+    od[c] = -id[c];
+    in[c] = vi[c] + vo[c];
+    on[c] = in[c];
+  }
+  if (type == RBC_SLIP) {
+    in[axis] = id[axis];
+    on[axis] = od[axis];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    vi[c] = type == RBC_NOSLIP ? id[c] : in[c];
+    vo[c] = type == RBC_NOSLIP ? od[c] : on[c];
+  }
+}
+// Reflect_Boundary_Mass_Momentum (:310-342) on the P2G sums (m, m v + dt f):
+// the force reflection of Reflect_Boundary_Force (:422-446) is the same map,
+// so it is applied once to the sum (gravity included beforehand).
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void reflect_mass_momentum_kernel(Node4<T>* grid, const Geom<T, D>* scene,
+                                                                      ReflectAxis r) {
+  const Geom<T, D> g = scene[0];
+  uint32_t i, o;
+  int hi;
+  if (!reflect_pair<T, D>(g, r, blockIdx.x * BLOCK + threadIdx.x, &i, &o, &hi)) return;
+  Node4<T> a = grid[i], b = grid[o];
+  T vi[3] = {a.x, a.y, a.z}, vo[3] = {b.x, b.y, b.z};
+  reflect_vectors<T>(r.type[hi], r.axis, vi, vo);
+  const T mt = a.m + b.m;
+  grid[i] = Node4<T>{mt, vi[0], vi[1], vi[2]};
+  grid[o] = Node4<T>{mt, vo[0], vo[1], vo[2]};
+}
+// Reflect_Boundary_Velocity (:395-420): a ghost node with mass takes its
+// image's velocity, normal part negated for slip and all of it for noslip.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void reflect_velocity_kernel(Node4<T>* grid, Node4<T>* gridf,
+                                                                 const Geom<T, D>* scene, ReflectAxis r) {
+  const Geom<T, D> g = scene[0];
+  uint32_t i, o;
+  int hi;
+  if (!reflect_pair<T, D>(g, r, blockIdx.x * BLOCK + threadIdx.x, &i, &o, &hi)) return;
+  const int type = r.type[hi];
+  for (int f = 0; f < (gridf ? 2 : 1); f++) {
+    Node4<T>* G = f ? gridf : grid;
+    const Node4<T> a = G[i];
+    Node4<T> b = G[o];
+    if (grid[o].m == (T)0) return;
+    T vi[3] = {a.x, a.y, a.z}, v[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[c] = type == RBC_NOSLIP ? -vi[c] : vi[c];
+    if (type == RBC_SLIP) v[r.axis] = -vi[r.axis];
+    b.x = v[0];
+    b.y = v[1];
+    b.z = D == 3 ? v[2] : b.z;
+    G[o] = b;
+  }
+}
+// Reflect_Helper (:343-369) without a boundary velocity: mass-weighted mixing
+// of the pair's velocities; separate sides keep a separating pair and apply
+// Coulomb friction mu to an approaching one.
+template<typename T, int D>
+__device__ __forceinline__ void reflect_helper(T m_i, T m_o, T* v_i, T* v_o, int type, T mu, int a, int s) {
+  const T m_t = m_i + m_o, a_i = m_i / m_t, a_o = 1 - a_i;
+  T v_in[D], v_on[D], v_id[D], v_od[D];
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    v_in[c] = a_i * v_i[c] + a_o * v_o[c];
+    v_on[c] = v_in[c];
+    v_id[c] = a_i * v_i[c] + a_o * ((T)0 - v_o[c]);
+    v_od[c] = a_o * v_o[c] + a_i * ((T)0 - v_i[c]);
+  }
+  if (type == RBC_FREE) {
+#pragma unroll
+    for (int c = 0; c < D; c++) { v_i[c] = v_in[c]; v_o[c] = v_on[c]; }
+    return;
+  }
+  if (type == RBC_NOSLIP) {
+#pragma unroll
+    for (int c = 0; c < D; c++) { v_i[c] = v_id[c]; v_o[c] = v_od[c]; }
+    return;
+  }
+  if (type == RBC_SLIP) {
+    v_in[a] = v_id[a];
+    v_on[a] = v_od[a];
+#pragma unroll
+    for (int c = 0; c < D; c++) { v_i[c] = v_in[c]; v_o[c] = v_on[c]; }
+    return;
+  }
+  const T d_v = 2 * a_o * a_i * ((T)0 - v_i[a] - v_o[a]);
+  const T d_v_n = s ? -d_v : d_v;
+  if (d_v_n < 0) return;
+  v_in[a] = 0;
+  T m2 = v_in[0] * v_in[0];
+#pragma unroll
+  for (int c = 1; c < D; c++) m2 += v_in[c] * v_in[c];
+  const T mag = sqrt(m2);  // VECTOR::Normalize: x axis when zero
+  T t[D];
+#pragma unroll
+  for (int c = 0; c < D; c++) t[c] = mag ? v_in[c] * ((T)1 / mag) : (c == 0 ? (T)1 : (T)0);
+  const T rr = mag - mu * d_v_n;
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    const T ti = c == a ? v_id[a] : (T)0, to = c == a ? v_od[a] : (T)0;
+    v_i[c] = rr < 0 ? ti : ti + rr * t[c];
+    v_o[c] = rr < 0 ? to : to + rr * t[c];
+  }
+}
+// Reflect_Boundary_Friction (:371-393) on the pairs where both nodes hold mass.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void reflect_friction_kernel(Node4<T>* grid, const Geom<T, D>* scene,
+                                                                 ReflectAxis r, T mu) {
+  const Geom<T, D> g = scene[0];
+  uint32_t i, o;
+  int hi;
+  if (!reflect_pair<T, D>(g, r, blockIdx.x * BLOCK + threadIdx.x, &i, &o, &hi)) return;
+  Node4<T> a = grid[i], b = grid[o];
+  if (a.m == (T)0 || b.m == (T)0) return;
+  T vi[D], vo[D];
+  vi[0] = a.x; vi[1] = a.y;
+  vo[0] = b.x; vo[1] = b.y;
+  if constexpr (D == 3) { vi[2] = a.z; vo[2] = b.z; }
+  reflect_helper<T, D>(a.m, b.m, vi, vo, r.type[hi], mu, r.axis, hi);
+  a.x = vi[0]; a.y = vi[1];
+  b.x = vo[0]; b.y = vo[1];
+  if constexpr (D == 3) { a.z = vi[2]; b.z = vo[2]; }
+  grid[i] = a;
+  grid[o] = b;
+}
+// m v += dt m g on the listed tiles (the gravity impulse of MPM_GRAVITY::
+// Add_Forces taken into the P2G sums before they are reflected).
+template<typename T>
+__global__ __launch_bounds__(BLOCK) void add_gravity_kernel(Node4<T>* g, const uint32_t* tiles, const uint32_t* count,
+                                                            T gx, T gy, T gz) {
+  const uint64_t total = (uint64_t)count[0] * 64u;
+  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total; idx += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t slot = tiles[idx >> 6] * 64u + (uint32_t)(idx & 63u);
+    Node4<T> nd = g[slot];
+    if (nd.m == (T)0) continue;
+    nd.x += gx * nd.m;
+    nd.y += gy * nd.m;
+    nd.z += gz * nd.m;
+    g[slot] = nd;
   }
 }
 

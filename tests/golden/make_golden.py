@@ -145,6 +145,56 @@ def write_limits():
     np.savez_compressed(os.path.join(HERE, "limits.npz"), **out)
 
 
+def reflect_cases():
+    """Scenes with reflection boundary conditions (-use_reflect, with
+    -use_reflect_friction for the full variant) instead of wall objects:
+    particles fill the domain up to the sides, fast enough that some leave it
+    and are reflected back, one side left free.
+
+    Each entry: scene, steps."""
+    fill = lambda X: np.all((X > 0.003) & (X < 0.997), axis=1)
+    shell = lambda X: fill(X) & ~np.all((X > 0.2) & (X < 0.8), axis=1)
+    cases = {}
+    sc = mt.make_scene(2, 16, prec=8, model=mt.COROTATED, shape=fill, v0=[-1.0, -2.0], vrand=8.0, frand=0.05, dt=1e-3,
+                       seed=71)
+    sc["reflection"] = dict(full=False, friction=0.0, sides=[mt.SLIP_BC, mt.NOSLIP_BC, mt.NOSLIP_BC, mt.FREE])
+    cases["reflect2d_f64"] = (sc, 6)
+    sc = mt.make_scene(2, 16, prec=8, model=mt.NEO_HOOKEAN, shape=fill, v0=[1.0, -2.0], vrand=8.0, frand=0.05,
+                       dt=1e-3, seed=72)
+    sc["reflection"] = dict(full=True, friction=0.4,
+                            sides=[mt.SEPARATE_BC, mt.SLIP_BC, mt.SEPARATE_BC, mt.NOSLIP_BC])
+    cases["reflect2d_full_f64"] = (sc, 6)
+    sc = mt.make_scene(3, 8, prec=4, model=mt.COROTATED, shape=shell, v0=[0.5, -2.0, 1.0], vrand=8.0, frand=0.05,
+                       dt=1e-3, seed=73, per_particle_E=True)
+    sc["reflection"] = dict(full=False, friction=0.0, sides=[mt.SLIP_BC, mt.NOSLIP_BC, mt.SEPARATE_BC, mt.FREE,
+                                                             mt.NOSLIP_BC, mt.SLIP_BC])
+    cases["reflect3d_f32"] = (sc, 4)
+    sc = mt.make_scene(3, 8, prec=8, model=mt.COROTATED, shape=shell, v0=[0.5, -2.0, 1.0], vrand=8.0, frand=0.05,
+                       dt=1e-3, seed=74)
+    sc["reflection"] = dict(full=True, friction=0.3, sides=[mt.SEPARATE_BC] * 4 + [mt.NOSLIP_BC, mt.SLIP_BC])
+    cases["reflect3d_full_f64"] = (sc, 4)
+    return cases
+
+
+def write_reflect():
+    out = {}
+    for name, (sc, nsteps) in reflect_cases().items():
+        for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+            out[f"{name}_in_{k}"] = np.asarray(sc[k])
+        meta = {k: sc[k] for k in ("dim", "prec", "model", "N", "lo", "hi", "gravity", "dt", "neo_threshold")}
+        meta.update(nsteps=nsteps, reflection=sc["reflection"])
+        out[f"{name}_meta"] = np.array(json.dumps(meta, default=float))
+        out[f"{name}_walls"] = np.array(json.dumps(sc["walls"], default=float))
+        for n in (1, nsteps):
+            r = scene_io.run_ref(sc, n)[-1]
+            for k in ("X", "V", "F", "B", "valid"):
+                out[f"{name}_s{n}_{k}"] = r[k]
+        free = dict(sc, reflection=dict(sc["reflection"], sides=[0] * (2 * sc["dim"])))
+        kept = int(r["valid"].sum()) - int(scene_io.run_ref(free, nsteps)[-1]["valid"].sum())
+        print("reflect", name, sc["X"].shape[0], "particles,", kept, "kept by reflection")
+    np.savez_compressed(os.path.join(HERE, "reflect.npz"), **out)
+
+
 def io_cases():
     """Scenes whose particle and restart files the reference writes after a
     few steps (MPM_EXAMPLE::Write_Output_Files' particle part).
@@ -244,7 +294,7 @@ def write_plastic():
 def main():
     if not scene_io.ref_available():
         sys.exit("reference harness missing: build it with oracle/ref/Makefile")
-    groups = set(sys.argv[1:]) or {"steps", "components", "frames", "plastic", "limits", "io"}
+    groups = set(sys.argv[1:]) or {"steps", "components", "frames", "plastic", "limits", "io", "reflect"}
     if "frames" in groups:
         write_frames()
     if "plastic" in groups:
@@ -253,6 +303,8 @@ def main():
         write_limits()
     if "io" in groups:
         write_io()
+    if "reflect" in groups:
+        write_reflect()
     if "steps" in groups:
         write_steps()
     if "components" in groups:

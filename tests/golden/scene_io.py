@@ -104,6 +104,15 @@ def read_output(path, dim, plastic=0):
     return dumps
 
 
+def reflect_args(sc):
+    """Command-line tail of the harness for sc["reflection"] (full, friction, sides)."""
+    r = sc.get("reflection")
+    if not r:
+        return []
+    return ["reflect", str(1 if r.get("full") else 0), repr(float(r.get("friction", 0.0)))] + \
+        [str(int(t)) for t in r["sides"]]
+
+
 def run_ref(sc, nsteps, dump_every=None, threads=1):
     """Run the reference MPM_DRIVER on scene ``sc``; returns the list of dumps."""
     if dump_every is None:
@@ -112,7 +121,7 @@ def run_ref(sc, nsteps, dump_every=None, threads=1):
         sp = os.path.join(td, "scene.bin")
         op = os.path.join(td, "out.bin")
         write_scene(sp, sc, nsteps, dump_every, threads)
-        subprocess.run([REF_HARNESS, "step", sp, op], check=True)
+        subprocess.run([REF_HARNESS, "step", sp, op] + reflect_args(sc), check=True, stdout=subprocess.DEVNULL)
         return read_output(op, sc["dim"], plastic=plastic_kind(sc))
 
 

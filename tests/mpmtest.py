@@ -252,6 +252,34 @@ def oracle_run_limited(sc, nsteps, lim, prec=None):
     return dict(X=st["X"], V=st["V"], F=st["F"], B=st["B"], valid=valid), dts[:nsteps].copy()
 
 
+FREE, SLIP_BC, NOSLIP_BC, SEPARATE_BC = 0, 1, 2, 3
+
+
+def oracle_run_reflect(sc, nsteps, prec=None):
+    """``nsteps`` steps with reflection boundary conditions (sc["reflection"]: full, friction, sides)."""
+    prec = prec or sc["prec"]
+    dt_np = np.float64 if prec == 8 else np.float32
+    d = sc["dim"]
+    n = sc["X"].shape[0]
+    st = {k: np.ascontiguousarray(sc[k], dtype=dt_np).copy() for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda")}
+    valid = np.ascontiguousarray(sc["valid"], np.uint8).copy()
+    N = np.array(list(sc["N"]) + [1] * (3 - d), np.int32)
+    lo = np.array(list(sc["lo"]) + [0] * (3 - d), np.float64)
+    hi = np.array(list(sc["hi"]) + [0] * (3 - d), np.float64)
+    g = np.array(list(sc["gravity"]) + [0] * (3 - d), np.float64)
+    nw, wt, wf, wl, wh = _walls_arrays(sc)
+    r = sc["reflection"]
+    refl = np.array([1.0 if r.get("full") else 0.0, r.get("friction", 0.0)] + [float(t) for t in r["sides"]], np.float64)
+    fn = oracle().oracle_run_reflect_f64 if prec == 8 else oracle().oracle_run_reflect_f32
+    rc = fn(ctypes.c_int(d), _ptr(N), _ptr(lo), _ptr(hi), _ptr(g), ctypes.c_int(sc["model"]),
+            ctypes.c_double(sc.get("neo_threshold", 0.25)), ctypes.c_double(sc["dt"]),
+            ctypes.c_int(nw), _ptr(wt), _ptr(wf), _ptr(wl), _ptr(wh), ctypes.c_longlong(n),
+            _ptr(st["X"]), _ptr(st["V"]), _ptr(st["F"]), _ptr(st["B"]), _ptr(st["mass"]), _ptr(st["volume"]),
+            _ptr(st["mu"]), _ptr(st["lambda"]), _ptr(valid), ctypes.c_int(nsteps), _ptr(refl))
+    assert rc == 0
+    return dict(X=st["X"], V=st["V"], F=st["F"], B=st["B"], valid=valid)
+
+
 def oracle_sound_speeds(sc, prec=None, F=None):
     """Sound_Speed of every particle of ``sc`` (or of the given F) from the singular values of F."""
     prec = prec or sc["prec"]

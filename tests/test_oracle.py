@@ -455,3 +455,36 @@ def test_sound_speed_grows_under_compression_for_neo_hookean():
     s = 0.8
     H = mu[2] + (la[2] + mu[2] - la[2] * np.log(s ** 3)) / s ** 2
     assert np.isclose(c[2], np.sqrt(H * s ** 2 / 1000.0), rtol=1e-13)
+
+
+REFLECT_CASES = ["reflect2d_f64", "reflect2d_full_f64", "reflect3d_f32", "reflect3d_full_f64"]
+
+
+@pytest.mark.parametrize("name", REFLECT_CASES)
+def test_oracle_reflection_bc_matches_reference(name):
+    """Reflection boundary conditions (mass-momentum, force and velocity
+    reflection, or the frictional full variant; particles mirrored back at
+    sides with a condition, invalidated at free ones): bit for bit with the
+    reference's own runs after one step and after several
+    (tests/golden/reflect.npz)."""
+    z = load("reflect.npz")
+    sc, m = limit_scene(z, name)
+    sc["reflection"] = m["reflection"]
+    for n in (1, m["nsteps"]):
+        out = mt.oracle_run_reflect(sc, n)
+        for k in ("X", "V", "F", "B", "valid"):
+            assert np.array_equal(out[k].astype(np.float64) if k != "valid" else out[k], z[f"{name}_s{n}_{k}"]), (n, k)
+
+
+def test_reflection_keeps_particles_that_free_sides_lose():
+    """Particles leaving through a side with a condition come back mirrored
+    (X -> 2 A - X, V_a -> -V_a); with every side free the same run loses them."""
+    z = load("reflect.npz")
+    sc, m = limit_scene(z, "reflect3d_f32")
+    sc["reflection"] = m["reflection"]
+    kept = mt.oracle_run_reflect(sc, m["nsteps"])
+    free = dict(sc, reflection=dict(m["reflection"], sides=[mt.FREE] * 6))
+    lost = mt.oracle_run_reflect(free, m["nsteps"])
+    assert kept["valid"].sum() > lost["valid"].sum()
+    X = kept["X"].astype(np.float64)
+    assert np.all(X[kept["valid"] == 1] >= 0) and np.all(X[kept["valid"] == 1] <= 1)
