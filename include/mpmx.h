@@ -195,6 +195,28 @@ int mpmx_compute_dt(mpmx_ctx* ctx, double cfl, double max_dt, double* dt);
 int mpmx_advance_frames(mpmx_ctx* ctx, int nframes, double frame_dt, double cfl, double min_dt, double max_dt,
                         double* dts, int64_t dts_capacity, int64_t* nsubsteps);
 
+/* Reflection boundary conditions on the domain sides instead of wall
+   objects: example.reflection_bc with side_bc_type (MPM_EXAMPLE.h:106-109,
+   :134), as -use_reflect makes Add_Walls set them (Projects/mpm/
+   STANDARD_TESTS_BASE.cpp:487-505).  side_types holds 2*dim entries for
+   -x, +x, -y, +y[, -z, +z]: MPMX_BC_FREE, MPMX_BC_SLIP, MPMX_BC_NOSLIP or
+   MPMX_BC_SEPARATE.  full = 0: the P2G mass and momentum, the forces and the
+   final velocities are reflected across the sides (Reflect_Boundary_Mass_
+   Momentum, _Force, _Velocity, MPM_DRIVER.cpp:310-342, :395-446); full = 1
+   (-use_reflect_friction): the velocities of node pairs across a side are
+   mixed by mass after the collision objects, separating sides applying
+   Coulomb friction `friction` (Reflect_Boundary_Friction, :343-393).
+   Particles leaving through a side whose type is not free are mirrored back
+   (Reflect_Or_Invalidate_Particle, :1219-1239).  Needs at least 6 cells per
+   axis; not for batches, tapes or the dt limits, and the full variant not
+   with collision objects that have friction.  Steps with it take the unfused
+   kernels; the tiles along the sides stay in the grid every step. */
+#define MPMX_BC_FREE 0
+#define MPMX_BC_SLIP 1
+#define MPMX_BC_NOSLIP 2
+#define MPMX_BC_SEPARATE 3
+int mpmx_set_reflection(mpmx_ctx* ctx, int enable, int full, double friction, const int* side_types);
+
 /* Strong and sound-speed CFL: MPM_DRIVER::Reduce_Dt (MPM_DRIVER.cpp:519-545)
    between the grid update and G2P of every step, as -strong_cfl and
    -sound_cfl turn it on (Projects/mpm/STANDARD_TESTS_BASE.cpp:128-129).

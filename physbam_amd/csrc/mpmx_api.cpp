@@ -208,6 +208,11 @@ int mpmx_get_plastic_state(mpmx_ctx* ctx, void* Fp, void* mu, void* lambda) {
   return guard([&] { ctx->engine->get_plastic_state(Fp, mu, lambda); });
 }
 
+int mpmx_set_reflection(mpmx_ctx* ctx, int enable, int full, double friction, const int* side_types) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->set_reflection(enable != 0, full != 0, friction, side_types); });
+}
+
 int mpmx_set_dt_limits(mpmx_ctx* ctx, int strong_cfl, int sound_cfl, double cfl, double cfl_F, double cfl_sound,
                        double min_dt) {
   if (!ctx) return bad("null context");

@@ -49,6 +49,7 @@ struct EngineBase {
                               int64_t capacity, int64_t* nsub) = 0;
   virtual void set_plasticity(bool enable, double theta_c, double theta_s, double max_hardening, double xi) = 0;
   virtual void get_plastic_state(void* Fp, void* mu, void* lambda) = 0;
+  virtual void set_reflection(bool enable, bool full, double friction, const int* sides) = 0;
   virtual void set_dt_limits(bool strong, bool sound, double cfl, double cfl_F, double cfl_sound, double min_dt) = 0;
   virtual void max_sound_speed(double* out) = 0;
   virtual double last_dt() const = 0;

@@ -3168,7 +3168,7 @@ class Engine final : public EngineBase {
     if (nsteps < 0) throw Error{MPMX_EINVAL, "negative step count"};
     // recorded steps too (the reverse pass recomputes its own); plasticity
     // projects F between G2P and the next P2G, which the fused kernel does not
-    const bool fuse = n_ > 0 && !plastic_ && !limits_on();
+    const bool fuse = n_ > 0 && !plastic_ && !limits_on() && !refl_on_;
     for (int s = 0; s < nsteps; s++) {
       if (fuse)
         step_fused((T)dt);
@@ -3471,6 +3471,7 @@ class Engine final : public EngineBase {
     if (plastic_) throw Error{MPMX_EINVAL, "the adjoint step has no plasticity"};
     if (model_ == 2) throw Error{MPMX_EINVAL, "the adjoint step has no Hencky-strain model"};
     if (limits_on()) throw Error{MPMX_EINVAL, "the adjoint step has a fixed dt: turn the strong and sound CFL off"};
+    if (refl_on_) throw Error{MPMX_EINVAL, "the adjoint step has no reflection boundary conditions"};
     if (force_reorder_ || force_rebin_) rebin();  // settle the storage order before recording
     particle_stats(nullptr, nullptr, &tape_valid_);
     fused_ = false;
@@ -3773,6 +3774,11 @@ class Engine final : public EngineBase {
     a.gridu = gridu_override_;
     a.err = err_.p;
     a.stats_partial = nullptr;
+    a.band = refl_on_;
+    a.reflect_sides = 0;
+    if (refl_on_)
+      for (int i = 0; i < 2 * D; i++)
+        if (refl_side_[i] != RBC_FREE) a.reflect_sides |= 1u << i;
     return a;
   }
 
@@ -3839,6 +3845,8 @@ class Engine final : public EngineBase {
     fused_ = false;
     fused_last_ = false;
     const bool fric = any_friction();
+    if (refl_on_ && refl_full_ && fric)
+      throw Error{MPMX_ESTATE, "frictional reflection does not combine with collision objects that have friction"};
     if (fric) gridf_.alloc((size_t)ntotal_ * 64);
     hipEvent_t e_step = timer_.begin(stream_);
     if (n_ == 0) {
@@ -3866,13 +3874,32 @@ class Engine final : public EngineBase {
     rb_meta_[slot] = Readback{step_count_, bin_stamp_};
 
     hipEvent_t e2 = timer_.begin(stream_);
-    a.stats_partial = stats_on_ ? stats_partial_.p : nullptr;
+    Args<T, D> ag = a;  // the grid update's arguments
+    if (refl_on_ && !refl_full_) {
+      // gravity into the P2G sums, then Reflect_Boundary_Mass_Momentum with
+      // the force reflection folded in (the same linear map on m v and dt f)
+      add_gravity_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(a.grid, a.gtiles, counts_.p + 1, dt * a.grav[0],
+                                                                  dt * a.grav[1], D == 3 ? dt * a.grav[D - 1] : (T)0);
+      for (int k = 0; k < D; k++)
+        reflect_mass_momentum_kernel<T, D><<<reflect_blocks(k), BLOCK, 0, stream_>>>(a.grid, scenes_dev_.p,
+                                                                                       reflect_axis(k));
+      for (int c = 0; c < D; c++) ag.grav[c] = 0;
+    }
+    ag.stats_partial = stats_on_ ? stats_partial_.p : nullptr;
     if (fric)
-      grid_kernel<T, D, true><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
+      grid_kernel<T, D, true><<<grid_blocks(), BLOCK, 0, stream_>>>(ag);
     else
-      grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
+      grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(ag);
     if (stats_on_) reduce_partials_kernel<<<1, 64, 0, stream_>>>(stats_partial_.p, grid_blocks(), stats_out_.p);
-    a.stats_partial = nullptr;
+    if (refl_on_ && refl_full_) {  // Reflect_Boundary_Friction after the collision objects (MPM_DRIVER.cpp:999-1019)
+      for (int k = 0; k < D; k++)
+        reflect_friction_kernel<T, D><<<reflect_blocks(k), BLOCK, 0, stream_>>>(a.grid, scenes_dev_.p,
+                                                                                  reflect_axis(k), refl_fric_);
+    } else if (refl_on_) {  // Reflect_Boundary_Velocity of velocity and velocity_friction_save (:1057-1059)
+      for (int k = 0; k < D; k++)
+        reflect_velocity_kernel<T, D><<<reflect_blocks(k), BLOCK, 0, stream_>>>(a.grid, fric ? a.gridf : nullptr,
+                                                                                  scenes_dev_.p, reflect_axis(k));
+    }
     timer_.end(PH_GRID, e2, stream_);
     if (limits_on()) {
       hipEvent_t e4 = timer_.begin(stream_);
@@ -3915,6 +3942,43 @@ class Engine final : public EngineBase {
   }
 
   bool limits_on() const { return lim_strong_ || lim_sound_; }
+  ReflectAxis reflect_axis(int k) const {
+    ReflectAxis r;
+    r.axis = k;
+    r.type[0] = refl_side_[2 * k];
+    r.type[1] = refl_side_[2 * k + 1];
+    return r;
+  }
+  // one thread per ghost node of axis k: 2 sides x 3 layers x the other axes' nodes
+  uint32_t reflect_blocks(int k) const {
+    uint64_t other = 1;
+    for (int c = 0; c < D; c++)
+      if (c != k) other *= (uint64_t)g_.G[c];
+    return (uint32_t)cdiv(6 * other, (uint64_t)BLOCK);
+  }
+  // Reflection boundary conditions on the domain sides (-use_reflect,
+  // -use_reflect_friction: MPM_EXAMPLE.h:106-109, :134; STANDARD_TESTS_BASE.cpp:487-505).
+  void set_reflection(bool enable, bool full, double friction, const int* sides) override {
+    if (taping_) throw Error{MPMX_ESTATE, "boundary conditions cannot change while a tape is recorded"};
+    if (!enable) {
+      refl_on_ = 0;
+      fused_ = false;
+      return;
+    }
+    if (scenes_.size() > 1) throw Error{MPMX_ESTATE, "reflection boundary conditions take a single-scene context"};
+    if (!sides) throw Error{MPMX_EINVAL, "null side types"};
+    if (!(friction >= 0)) throw Error{MPMX_EINVAL, "friction must be >= 0"};
+    if (limits_on()) throw Error{MPMX_ESTATE, "reflection boundary conditions do not combine with the dt limits"};
+    for (int c = 0; c < D; c++)
+      if (g_.G[c] - 6 < 6) throw Error{MPMX_EINVAL, "reflection boundary conditions need at least 6 cells per axis"};
+    for (int i = 0; i < 2 * D; i++)
+      if (sides[i] < RBC_FREE || sides[i] > RBC_SEP) throw Error{MPMX_EINVAL, "side type must be 0..3"};
+    for (int i = 0; i < 2 * D; i++) refl_side_[i] = sides[i];
+    refl_full_ = full;
+    refl_fric_ = (T)friction;
+    refl_on_ = 1;
+    fused_ = false;
+  }
   // Max sound speed (out[0]) and strong-CFL factor (out[1]) of the current
   // particles against the grid of a.grid and vsave_; waits for the stream.
   void dt_limit_values(const Args<T, D>& a, int strong, int sound, T cfl, T cfl_F, T* out) {
@@ -3973,6 +4037,8 @@ class Engine final : public EngineBase {
       throw Error{MPMX_EINVAL, "need cfl, cfl_F, cfl_sound > 0 and min_dt >= 0"};
     if ((strong || sound) && scenes_.size() > 1)
       throw Error{MPMX_ESTATE, "the strong and sound-speed CFL take a single-scene context"};
+    if ((strong || sound) && refl_on_)
+      throw Error{MPMX_ESTATE, "the dt limits do not combine with reflection boundary conditions"};
     if (taping_) throw Error{MPMX_ESTATE, "dt limits cannot change while a tape is recorded"};
     lim_strong_ = strong;
     lim_sound_ = sound;
@@ -4171,6 +4237,11 @@ class Engine final : public EngineBase {
   uint32_t *ml_cur_ = nullptr, *mt_cur_ = nullptr, *ml_next_ = nullptr, *mt_next_ = nullptr;
   int plastic_ = 0;  // plasticity after every G2P (unfused steps): 1 clamp, 2 Drucker-Prager
   bool user_order_ = true;  // particle slots are in user order (no reorder since set_particles)
+  // reflection boundary conditions (example.reflection_bc, use_full_reflection,
+  // reflection_bc_friction, side_bc_type)
+  int refl_on_ = 0, refl_full_ = 0;
+  T refl_fric_ = 0;
+  int refl_side_[6] = {0, 0, 0, 0, 0, 0};
   PlasticDev<T> pl_{};
   DevBuf<uint8_t> simmask_;
   bool fused_ = false;       // gcur_ holds the P2G of the current state with dt fused_dt_

@@ -26,7 +26,7 @@ EXPORTS = (
     "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_grid_node_count",
     "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_set_rebin_policy", "mpmx_bin_stats",
     "mpmx_grid_v_upper_bound", "mpmx_compute_dt", "mpmx_advance_frames",
-    "mpmx_set_dt_limits", "mpmx_max_sound_speed", "mpmx_last_dt",
+    "mpmx_set_dt_limits", "mpmx_max_sound_speed", "mpmx_last_dt", "mpmx_set_reflection",
     "mpmx_write_particles_file", "mpmx_read_particles_file", "mpmx_write_output_files", "mpmx_read_output_files",
     "mpmx_write_restart_file", "mpmx_read_restart_file",
     "mpmx_set_plasticity", "mpmx_get_plastic_state", "mpmx_set_drucker_prager", "mpmx_get_drucker_prager_state",
@@ -49,6 +49,7 @@ class GridDesc(ctypes.Structure):
 
 
 ARRAY_SCALAR, ARRAY_VECTOR, ARRAY_VECTOR3, ARRAY_MATRIX, ARRAY_BOOL = 0, 1, 2, 3, 4
+BC_FREE, BC_SLIP, BC_NOSLIP, BC_SEPARATE = 0, 1, 2, 3
 
 
 class ParticleArray(ctypes.Structure):
@@ -115,6 +116,7 @@ def lib():
         L.mpmx_write_restart_file.argtypes = [ctypes.c_char_p, c_int, dbl]
         L.mpmx_read_restart_file.argtypes = [ctypes.c_char_p, vp]
         L.mpmx_read_output_files.argtypes = [vp, ctypes.c_char_p]
+        L.mpmx_set_reflection.argtypes = [vp, c_int, c_int, dbl, vp]
         L.mpmx_set_dt_limits.argtypes = [vp, c_int, c_int, dbl, dbl, dbl, dbl]
         L.mpmx_max_sound_speed.argtypes = [vp, vp]
         L.mpmx_last_dt.argtypes = [vp, vp]
@@ -379,6 +381,11 @@ class Context:
     def read_output_files(self, frame_dir):
         """MPM_EXAMPLE::Read_Output_Files: particles and time from ``frame_dir``."""
         _check(lib().mpmx_read_output_files(self._h, os.fsencode(frame_dir)))
+
+    def set_reflection(self, sides, full=False, friction=0.0, enable=True):
+        """Reflection boundary conditions; sides: 2*dim types (BC_FREE, BC_SLIP, BC_NOSLIP, BC_SEPARATE)."""
+        t = np.ascontiguousarray(sides, np.int32)
+        _check(lib().mpmx_set_reflection(self._h, 1 if enable else 0, 1 if full else 0, float(friction), _p(t)))
 
     def set_dt_limits(self, strong_cfl=False, sound_cfl=False, cfl=1.0, cfl_F=0.5, cfl_sound=0.9, min_dt=0.0):
         """Strong and sound-speed CFL (MPM_DRIVER::Reduce_Dt) inside every step."""
