@@ -48,6 +48,24 @@ typedef struct mpmx_seed_desc {
 int64_t mpmx_seed_particles(const mpmx_seed_desc* desc, int precision, int64_t capacity, void* X, void* V,
                             void* F, void* B, void* mass, void* volume, void* mu, void* lambda, uint8_t* valid);
 
+/* Poisson-disk seeding, STANDARD_TESTS_BASE::Seed_Particles_Poisson
+   (Projects/mpm/STANDARD_TESTS_BASE.cpp:223-234), the reference's default
+   seeding: one RANDOM_NUMBERS stream seeded with desc->seed, a POISSON_DISK
+   whose minimum distance comes from Set_Distance_By_Volume(dX.Product() /
+   particles_per_cell) of the MAC grid (Geometry/Seeding/POISSON_DISK.cpp:
+   130-137; sqrt(2/3 v) in 2D, cbrt(13/18 v) in 3D), and one Sample per shape
+   in order over the shape's box, points outside a sphere then dropped (the
+   last point swapped into the hole, as Remove_Index_Lazy).  All of it in the
+   given precision, so that a seed gives the reference's points bit for bit.
+   volume = dX.Product() / particles_per_cell and mass = density * volume in
+   that precision; V, F = I, B = 0, mu, lambda as mpmx_seed_particles.
+   particles_per_cell <= 0 takes ppc_axis^dim.  desc->jitter is not used.
+   Returns the particle count (or -1); arrays are filled only if X is
+   non-null and capacity >= count. */
+int64_t mpmx_seed_particles_poisson(const mpmx_seed_desc* desc, int precision, double particles_per_cell,
+                                    int64_t capacity, void* X, void* V, void* F, void* B, void* mass, void* volume,
+                                    void* mu, void* lambda, uint8_t* valid);
+
 /* Random non-overlapping axis-aligned boxes in [region_lo, region_hi]^3 with
    side lengths uniform in [side_min, side_max] (one draw per box if cubes is
    nonzero, else one per axis), placed by rejection (at most max_tries per

@@ -22,6 +22,13 @@
 //     (Read_From_File of <dir>/deformable_object_particles into MPM_PARTICLES
 //     and of <dir>/restart_data: a dump of X, V, F, B, valid, mass, volume,
 //     mu, lambda, Fp, mu0, lambda0 when stored, and the time)
+//   ref_harness poisson <in.bin> <out.bin>
+//     (STANDARD_TESTS_BASE::Seed_Particles_Poisson's sampling: RANDOM_NUMBERS
+//     seeded once, POISSON_DISK with Set_Distance_By_Volume(dX.Product() /
+//     particles_per_cell) of the MAC grid, Sample per shape in order; in.bin:
+//     int prec, dim, seed, nshapes, cells[3]; double ppc, lo[3], hi[3]; per
+//     shape int type (0 box, 1 sphere), double a[3], b[3]; out.bin: int count
+//     per shape, then the points as doubles)
 //   ref_harness limited <scene.bin> <strong_cfl> <sound_cfl> <cfl> <cfl_F> <cfl_sound> <min_dt> <out.bin>
 //     (the scene's nsteps fixed-dt steps with MPM_DRIVER::Reduce_Dt: the dt
 //     each step took and the final state, in the frames output layout)
@@ -43,6 +50,10 @@
 #include <Deformables/Constitutive_Models/NEO_HOOKEAN.h>
 #include <Hybrid_Methods/Examples_And_Drivers/MPM_DRIVER.h>
 #include <Core/Read_Write/FILE_UTILITIES.h>
+#include <Core/Random_Numbers/RANDOM_NUMBERS.h>
+#include <Geometry/Basic_Geometry/SPHERE.h>
+#include <Geometry/Implicit_Objects/ANALYTIC_IMPLICIT_OBJECT.h>
+#include <Geometry/Seeding/POISSON_DISK.h>
 #include <Hybrid_Methods/Examples_And_Drivers/MPM_EXAMPLE.h>
 #include <Hybrid_Methods/Examples_And_Drivers/MPM_PARTICLES.h>
 #include <Hybrid_Methods/Forces/MPM_FINITE_ELEMENTS.h>
@@ -382,6 +393,35 @@ int Run_Read(const char* dir,const char* out)
     return 0;
 }
 
+template<class TV>
+int Run_Poisson(READER& r,int seed,int nshapes,const int* cells,double ppc,const double* lo,const double* hi,
+    const char* out)
+{
+    typedef typename TV::SCALAR T;
+    enum {d=TV::m};
+    VECTOR<int,d> N;TV L,H;
+    for(int a=0;a<d;a++){N(a)=cells[a];L(a)=(T)lo[a];H(a)=(T)hi[a];}
+    GRID<TV> grid(N,RANGE<TV>(L,H),true);
+    RANDOM_NUMBERS<T> random(seed);
+    POISSON_DISK<TV> pd(1);
+    pd.Set_Distance_By_Volume(grid.dX.Product()/(T)ppc);
+    std::vector<int> counts;
+    std::vector<double> pts;
+    for(int k=0;k<nshapes;k++){
+        int type=r.get<int>();
+        double a[3],b[3];r.get(a,3);r.get(b,3);
+        TV A,B;for(int c=0;c<d;c++){A(c)=(T)a[c];B(c)=(T)b[c];}
+        ARRAY<TV> X;
+        if(type==0){ANALYTIC_IMPLICIT_OBJECT<RANGE<TV> > io(RANGE<TV>(A,B));pd.Sample(random,io,X);}
+        else{ANALYTIC_IMPLICIT_OBJECT<SPHERE<TV> > io(SPHERE<TV>(A,(T)b[0]));pd.Sample(random,io,X);}
+        counts.push_back(X.m);
+        for(int i=0;i<X.m;i++) for(int c=0;c<d;c++) pts.push_back((double)X(i)(c));}
+    WRITER w(out);
+    w.put(counts.data(),counts.size());
+    w.put(pts.data(),pts.size());
+    return 0;
+}
+
 // Fixed-dt steps with Reduce_Dt (use_strong_cfl, use_sound_speed_cfl).
 template<class TV>
 int Run_Limited(const SCENE& s,const double* lim,double cfl,double min_dt,const char* out,STREAM_TYPE st)
@@ -552,6 +592,18 @@ int main(int argc,char* argv[])
         int rc;
         if(prec==8) rc=dim==2?Run_Read<VECTOR<double,2> >(argv[4],argv[5]):Run_Read<VECTOR<double,3> >(argv[4],argv[5]);
         else rc=dim==2?Run_Read<VECTOR<float,2> >(argv[4],argv[5]):Run_Read<VECTOR<float,3> >(argv[4],argv[5]);
+        LOG::Finish_Logging();
+        return rc;}
+    if(mode=="poisson" && argc==4){
+        READER r(argv[2]);
+        int prec=r.get<int>(),dim=r.get<int>(),seed=r.get<int>(),ns=r.get<int>();
+        int cells[3];r.get(cells,3);
+        double ppc=r.get<double>(),lo[3],hi[3];r.get(lo,3);r.get(hi,3);
+        int rc;
+        if(prec==8) rc=dim==2?Run_Poisson<VECTOR<double,2> >(r,seed,ns,cells,ppc,lo,hi,argv[3]):
+            Run_Poisson<VECTOR<double,3> >(r,seed,ns,cells,ppc,lo,hi,argv[3]);
+        else rc=dim==2?Run_Poisson<VECTOR<float,2> >(r,seed,ns,cells,ppc,lo,hi,argv[3]):
+            Run_Poisson<VECTOR<float,3> >(r,seed,ns,cells,ppc,lo,hi,argv[3]);
         LOG::Finish_Logging();
         return rc;}
     if(mode=="limited" && argc==10){

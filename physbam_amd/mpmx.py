@@ -33,7 +33,7 @@ EXPORTS = (
     "mpmx_tape_begin", "mpmx_backward",
     "mpmx_enable_timing", "mpmx_reset_timing",
     "mpmx_phase_time", "mpmx_stress", "mpmx_svd", "mpmx_sort_keys", "mpmx_seed_particles",
-    "mpmx_random_boxes",
+    "mpmx_random_boxes", "mpmx_seed_particles_poisson",
 )
 
 
@@ -132,6 +132,8 @@ def lib():
         L.mpmx_sort_keys.argtypes = [vp, i64, vp, c_int, vp, vp]
         L.mpmx_seed_particles.argtypes = [ctypes.POINTER(SeedDesc), c_int, i64] + [vp] * 9
         L.mpmx_seed_particles.restype = i64
+        L.mpmx_seed_particles_poisson.argtypes = [ctypes.POINTER(SeedDesc), c_int, dbl, i64] + [vp] * 9
+        L.mpmx_seed_particles_poisson.restype = i64
         L.mpmx_random_boxes.argtypes = [ctypes.c_uint32, c_int, dbl, dbl, dbl, dbl, dbl, dbl, dbl, dbl, c_int,
                                         c_int, vp, vp, vp, vp]
         _lib = L
@@ -484,8 +486,9 @@ class Context:
         return ko, oo
 
 
-def seed_particles(dim, cells, lo, hi, shapes, ppc_axis=2, seed=1234, precision=FP32, jitter=True):
-    """Jittered seeding of shapes, see include/mpmx_scene.h.
+def seed_particles(dim, cells, lo, hi, shapes, ppc_axis=2, seed=1234, precision=FP32, jitter=True, poisson=False,
+                   particles_per_cell=0.0):
+    """Jittered seeding of shapes (Poisson-disk seeding with ``poisson``), see include/mpmx_scene.h.
 
     ``shapes``: list of dicts {type: SHAPE_BOX|SHAPE_SPHERE, a: 3 floats,
     b: 3 floats (box max or [radius]), density, E, nu, v: velocity}.
@@ -519,12 +522,20 @@ def seed_particles(dim, cells, lo, hi, shapes, ppc_axis=2, seed=1234, precision=
         a = np.asarray(s["a"][:dim], float)
         ext = np.asarray(s["b"][:dim], float) - a if s["type"] == SHAPE_BOX else np.full(dim, 2 * s["b"][0])
         cap += int(np.prod(np.ceil(ext / h) + 2)) * ppc_axis ** dim
+    if poisson:  # Seed_Particles_Poisson: count first, then fill
+        cap = lib().mpmx_seed_particles_poisson(ctypes.byref(d), precision, float(particles_per_cell), 0,
+                                                *([None] * 9))
+        if cap < 0:
+            raise MpmxError(-1, "bad seeding description")
     dt = np.float32 if precision == FP32 else np.float64
     out = dict(X=np.zeros((cap, dim), dt), V=np.zeros((cap, dim), dt), F=np.zeros((cap, dim * dim), dt),
                B=np.zeros((cap, dim * dim), dt), mass=np.zeros(cap, dt), volume=np.zeros(cap, dt),
                mu=np.zeros(cap, dt), lam=np.zeros(cap, dt), valid=np.zeros(cap, np.uint8))
-    n = lib().mpmx_seed_particles(ctypes.byref(d), precision, cap, *[_p(out[k]) for k in
-                                  ("X", "V", "F", "B", "mass", "volume", "mu", "lam", "valid")])
+    ptrs = [_p(out[k]) for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lam", "valid")]
+    if poisson:
+        n = lib().mpmx_seed_particles_poisson(ctypes.byref(d), precision, float(particles_per_cell), cap, *ptrs)
+    else:
+        n = lib().mpmx_seed_particles(ctypes.byref(d), precision, cap, *ptrs)
     del keep
     if n < 0:
         raise MpmxError(-1, "bad seeding description")

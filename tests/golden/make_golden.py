@@ -145,6 +145,29 @@ def write_limits():
     np.savez_compressed(os.path.join(HERE, "limits.npz"), **out)
 
 
+POISSON_CASES = {
+    # name: (prec, dim, cells, particles per cell, shapes (type 0 box | 1 sphere, a, b))
+    "poisson2d_f64": (8, 2, [32, 32], 4.0, [(1, [0.5, 0.5, 0.0], [0.2, 0.0, 0.0]),
+                                            (0, [0.1, 0.1, 0.0], [0.3, 0.25, 0.0])]),
+    "poisson2d_f32_ppc6": (4, 2, [40, 24], 6.0, [(1, [0.4, 0.6, 0.0], [0.25, 0.0, 0.0])]),
+    "poisson3d_f32": (4, 3, [16, 16, 16], 8.0, [(1, [0.5, 0.5, 0.5], [0.25, 0.0, 0.0]),
+                                                (0, [0.1, 0.1, 0.1], [0.3, 0.2, 0.35])]),
+    "poisson3d_f64": (8, 3, [16, 12, 16], 8.0, [(0, [0.2, 0.3, 0.1], [0.6, 0.5, 0.4])]),
+}
+
+
+def write_poisson():
+    out = {}
+    for name, (prec, dim, cells, ppc, shapes) in POISSON_CASES.items():
+        pts = scene_io.run_ref_poisson(prec, dim, 1234, cells, [0.0] * dim, [1.0] * dim, ppc, shapes)
+        out[f"{name}_meta"] = np.array(json.dumps(dict(prec=prec, dim=dim, cells=cells, ppc=ppc, seed=1234,
+                                                       shapes=[[t, list(a), list(b)] for t, a, b in shapes])))
+        for i, p in enumerate(pts):
+            out[f"{name}_shape{i}_X"] = p
+        print("poisson", name, [len(p) for p in pts], "points")
+    np.savez_compressed(os.path.join(HERE, "poisson.npz"), **out)
+
+
 def reflect_cases():
     """Scenes with reflection boundary conditions (-use_reflect, with
     -use_reflect_friction for the full variant) instead of wall objects:
@@ -294,7 +317,7 @@ def write_plastic():
 def main():
     if not scene_io.ref_available():
         sys.exit("reference harness missing: build it with oracle/ref/Makefile")
-    groups = set(sys.argv[1:]) or {"steps", "components", "frames", "plastic", "limits", "io", "reflect"}
+    groups = set(sys.argv[1:]) or {"steps", "components", "frames", "plastic", "limits", "io", "reflect", "poisson"}
     if "frames" in groups:
         write_frames()
     if "plastic" in groups:
@@ -305,6 +328,8 @@ def main():
         write_io()
     if "reflect" in groups:
         write_reflect()
+    if "poisson" in groups:
+        write_poisson()
     if "steps" in groups:
         write_steps()
     if "components" in groups:

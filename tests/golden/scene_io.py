@@ -214,6 +214,36 @@ def run_ref_read(prec, dim, particles_gz, restart_gz):
     return out
 
 
+def run_ref_poisson(prec, dim, seed, cells, lo, hi, ppc, shapes):
+    """The reference's Seed_Particles_Poisson sampling (harness mode
+    "poisson"); shapes: list of (type 0 box | 1 sphere, a[3], b[3]).
+    Returns a list of point arrays, one per shape."""
+    with tempfile.TemporaryDirectory() as td:
+        ip = os.path.join(td, "in.bin")
+        op = os.path.join(td, "out.bin")
+        c = list(cells) + [1] * (3 - len(cells))
+        with open(ip, "wb") as f:
+            f.write(struct.pack("<4i", prec, dim, seed, len(shapes)))
+            f.write(struct.pack("<3i", *c))
+            f.write(struct.pack("<d", ppc))
+            f.write(struct.pack("<3d", *(list(lo) + [0.0] * (3 - len(lo)))))
+            f.write(struct.pack("<3d", *(list(hi) + [1.0] * (3 - len(hi)))))
+            for t, a, b in shapes:
+                f.write(struct.pack("<i", t))
+                f.write(struct.pack("<3d", *(list(a) + [0.0] * (3 - len(a)))))
+                f.write(struct.pack("<3d", *(list(b) + [0.0] * (3 - len(b)))))
+        subprocess.run([REF_HARNESS, "poisson", ip, op], check=True, stdout=subprocess.DEVNULL)
+        with open(op, "rb") as f:
+            buf = f.read()
+    counts = struct.unpack_from(f"<{len(shapes)}i", buf, 0)
+    pts = np.frombuffer(buf, np.float64, offset=4 * len(shapes)).reshape(-1, dim)
+    out, o = [], 0
+    for c in counts:
+        out.append(pts[o:o + c].copy())
+        o += c
+    return out
+
+
 def _frames_output(sc, buf):
     assert buf[:7] == b"MPMFRM1"
     nd, d, n, nn = struct.unpack_from("<4i", buf, 8)
