@@ -1189,7 +1189,9 @@ __device__ __forceinline__ void g2p_gather(const Geom<T, D>& gm, T dt, const Wts
 // the domain (MPM_DRIVER.cpp:450-515, :1219-1239).  A particle whose new
 // stencil base leaves its binned tile marks the tile it moves to, so that the
 // next step clears and updates the grid tiles around it.
-template<typename T, int D>
+// REFLECT: sides with a reflection condition mirror particles back (only the
+// unfused kernels, which reflection boundary conditions take).
+template<typename T, int D, bool REFLECT>
 __device__ __forceinline__ bool g2p_store(const Args<T, D>& a, const Geom<T, D>& g, uint32_t p, const T* x,
                                           const T* vb, const T* vfb, const T* G, const T* Bs, uint32_t bin_tile) {
   T Fm[D * D], Fn[D * D];
@@ -1204,32 +1206,43 @@ __device__ __forceinline__ bool g2p_store(const Args<T, D>& a, const Geom<T, D>&
       for (int q = 0; q < D; q++) s += a.dt * G[c + D * q] * Fm[q + D * j];
       Fn[c + D * j] = s;
     }
-  T xn[D], vn[D];
+  T xn[D];
   bool ok = true;
 #pragma unroll
   for (int c = 0; c < D; c++) {
+    xn[c] = x[c] + a.dt * vb[c];
+    if (xn[c] < g.vlo[c] || xn[c] > g.vhi[c]) ok = false;
+  }
+  if constexpr (REFLECT) {
     // Reflect_Or_Invalidate_Particle (MPM_DRIVER.cpp:1219-1239): a side with
     // a reflection condition mirrors the particle back, others invalidate it
-    xn[c] = x[c] + a.dt * vb[c];
-    vn[c] = vfb[c];
-    if (xn[c] < g.vlo[c]) {
-      if ((a.reflect_sides >> (2 * c)) & 1u) {
-        xn[c] = 2 * g.vlo[c] - xn[c];
-        vn[c] = -vn[c];
-      } else {
-        ok = false;
-      }
-    } else if (xn[c] > g.vhi[c]) {
-      if ((a.reflect_sides >> (2 * c + 1)) & 1u) {
-        xn[c] = 2 * g.vhi[c] - xn[c];
-        vn[c] = -vn[c];
-      } else {
-        ok = false;
+    T vn[D];
+    ok = true;
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      vn[c] = vfb[c];
+      if (xn[c] < g.vlo[c]) {
+        if ((a.reflect_sides >> (2 * c)) & 1u) {
+          xn[c] = 2 * g.vlo[c] - xn[c];
+          vn[c] = -vn[c];
+        } else {
+          ok = false;
+        }
+      } else if (xn[c] > g.vhi[c]) {
+        if ((a.reflect_sides >> (2 * c + 1)) & 1u) {
+          xn[c] = 2 * g.vhi[c] - xn[c];
+          vn[c] = -vn[c];
+        } else {
+          ok = false;
+        }
       }
     }
-  }
 #pragma unroll
-  for (int c = 0; c < D; c++) { a.X[c * a.n + p] = xn[c]; a.V[c * a.n + p] = vn[c]; }
+    for (int c = 0; c < D; c++) { a.X[c * a.n + p] = xn[c]; a.V[c * a.n + p] = vn[c]; }
+  } else {
+#pragma unroll
+    for (int c = 0; c < D; c++) { a.X[c * a.n + p] = xn[c]; a.V[c * a.n + p] = vfb[c]; }
+  }
 #pragma unroll
   for (int k = 0; k < D * D; k++) { a.F[k * a.n + p] = Fn[k]; a.B[k * a.n + p] = Bs[k]; }
   if (!ok) {
@@ -1300,7 +1313,7 @@ __device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, c
     };
     T vb[D], vfb[D], G[D * D], Bs[D * D];
     g2p_gather<T, D, FRICTION>(g, a.dt, w, nv, nf, vb, vfb, G, Bs);
-    if (g2p_store<T, D>(a, g, p, x, vb, vfb, G, Bs, t) && FUSED) {
+    if (g2p_store<T, D, !FUSED>(a, g, p, x, vb, vfb, G, Bs, t) && FUSED) {
       const uint32_t k = atomicAdd(&a.counts[5], 1u);
       a.mover_list2[k] = p;
       a.mover_tile2[k] = t;
@@ -1404,7 +1417,7 @@ __global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
     T vb[D], vfb[D], G[D * D], Bs[D * D];
     g2p_gather<T, D, FRICTION>(g, a.dt, w, nv, nf, vb, vfb, G, Bs);
     const uint32_t bt = a.mover_tile[q];
-    if (g2p_store<T, D>(a, g, p, x, vb, vfb, G, Bs, bt) && FUSED) {
+    if (g2p_store<T, D, !FUSED>(a, g, p, x, vb, vfb, G, Bs, bt) && FUSED) {
       const uint32_t k = a.mcap - 1u - atomicAdd(&a.counts[7], 1u);
       a.mover_list2[k] = p;
       a.mover_tile2[k] = bt;
