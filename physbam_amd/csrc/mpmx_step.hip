@@ -2664,27 +2664,34 @@ struct ReflectAxis {
 template<typename T, int D>
 __device__ __forceinline__ bool reflect_pair(const Geom<T, D>& g, const ReflectAxis& r, uint32_t q, uint32_t* io,
                                              uint32_t* oo, int* hi) {
-  const int k = r.axis, N = g.G[k] - 6;
-  int nc[D], other = 1;
+  const int k = r.axis;
+  // runtime axis: selected per component (a dynamic index would put the
+  // arrays in scratch memory)
+  int N = 0, other = 1;
 #pragma unroll
-  for (int c = 0; c < D; c++)
-    if (c != k) other *= g.G[c];
+  for (int c = 0; c < D; c++) {
+    if (c == k) N = g.G[c] - 6;
+    else other *= g.G[c];
+  }
   if (q >= (uint32_t)(6 * other)) return false;
   uint32_t rest = q % (uint32_t)other;
   const int lay = (int)(q / (uint32_t)other);  // 0..5: side * 3 + layer
   *hi = lay / 3;
-#pragma unroll
-  for (int c = D - 1; c >= 0; c--) {
-    if (c == k) continue;
-    nc[c] = (int)(rest % (uint32_t)g.G[c]);
-    rest /= (uint32_t)g.G[c];
-  }
   const int o = *hi ? N + (lay % 3) : (lay % 3) - 3;  // node index (ghost excluded)
   const int i = 2 * (*hi ? N : 0) - o - 1;
-  nc[k] = o + 3;
-  *oo = node_slot<T, D>(nc, g);
-  nc[k] = i + 3;
-  *io = node_slot<T, D>(nc, g);
+  int no[D], ni[D];
+#pragma unroll
+  for (int c = D - 1; c >= 0; c--) {
+    if (c == k) {
+      no[c] = o + 3;
+      ni[c] = i + 3;
+    } else {
+      no[c] = ni[c] = (int)(rest % (uint32_t)g.G[c]);
+      rest /= (uint32_t)g.G[c];
+    }
+  }
+  *oo = node_slot<T, D>(no, g);
+  *io = node_slot<T, D>(ni, g);
   return true;
 }
 template<typename T> __device__ __forceinline__ void reflect_vectors(int type, int axis, T* vi, T* vo) {
@@ -2696,14 +2703,11 @@ template<typename T> __device__ __forceinline__ void reflect_vectors(int type, i
     in[c] = vi[c] + vo[c];
     on[c] = in[c];
   }
-  if (type == RBC_SLIP) {
-    in[axis] = id[axis];
-    on[axis] = od[axis];
-  }
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    vi[c] = type == RBC_NOSLIP ? id[c] : in[c];
-    vo[c] = type == RBC_NOSLIP ? od[c] : on[c];
+  for (int c = 0; c < 3; c++) {  // selects, not a dynamic index (which would live in scratch)
+    const bool normal = type == RBC_SLIP && c == axis;
+    vi[c] = type == RBC_NOSLIP || normal ? id[c] : in[c];
+    vo[c] = type == RBC_NOSLIP || normal ? od[c] : on[c];
   }
 }
 // Reflect_Boundary_Mass_Momentum (:310-342) on the P2G sums (m, m v + dt f):
@@ -2740,8 +2744,7 @@ __global__ __launch_bounds__(BLOCK) void reflect_velocity_kernel(Node4<T>* grid,
     if (grid[o].m == (T)0) return;
     T vi[3] = {a.x, a.y, a.z}, v[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) v[c] = type == RBC_NOSLIP ? -vi[c] : vi[c];
-    if (type == RBC_SLIP) v[r.axis] = -vi[r.axis];
+    for (int c = 0; c < 3; c++) v[c] = type == RBC_NOSLIP || (type == RBC_SLIP && c == r.axis) ? -vi[c] : vi[c];
     b.x = v[0];
     b.y = v[1];
     b.z = D == 3 ? v[2] : b.z;
@@ -2773,16 +2776,27 @@ __device__ __forceinline__ void reflect_helper(T m_i, T m_o, T* v_i, T* v_o, int
     return;
   }
   if (type == RBC_SLIP) {
-    v_in[a] = v_id[a];
-    v_on[a] = v_od[a];
 #pragma unroll
-    for (int c = 0; c < D; c++) { v_i[c] = v_in[c]; v_o[c] = v_on[c]; }
+    for (int c = 0; c < D; c++) {
+      v_i[c] = c == a ? v_id[c] : v_in[c];
+      v_o[c] = c == a ? v_od[c] : v_on[c];
+    }
     return;
   }
-  const T d_v = 2 * a_o * a_i * ((T)0 - v_i[a] - v_o[a]);
+  T via = 0, voa = 0, vida = 0, voda = 0;  // the components along axis a
+#pragma unroll
+  for (int c = 0; c < D; c++)
+    if (c == a) {
+      via = v_i[c];
+      voa = v_o[c];
+      vida = v_id[c];
+      voda = v_od[c];
+    }
+  const T d_v = 2 * a_o * a_i * ((T)0 - via - voa);
   const T d_v_n = s ? -d_v : d_v;
   if (d_v_n < 0) return;
-  v_in[a] = 0;
+#pragma unroll
+  for (int c = 0; c < D; c++) v_in[c] = c == a ? (T)0 : v_in[c];
   T m2 = v_in[0] * v_in[0];
 #pragma unroll
   for (int c = 1; c < D; c++) m2 += v_in[c] * v_in[c];
@@ -2793,7 +2807,7 @@ __device__ __forceinline__ void reflect_helper(T m_i, T m_o, T* v_i, T* v_o, int
   const T rr = mag - mu * d_v_n;
 #pragma unroll
   for (int c = 0; c < D; c++) {
-    const T ti = c == a ? v_id[a] : (T)0, to = c == a ? v_od[a] : (T)0;
+    const T ti = c == a ? vida : (T)0, to = c == a ? voda : (T)0;
     v_i[c] = rr < 0 ? ti : ti + rr * t[c];
     v_o[c] = rr < 0 ? to : to + rr * t[c];
   }
