@@ -164,7 +164,8 @@ def adjoint_ratio(log, steps=200, checkpoint=20):
     (~56.6k particles), fixed corotated E 1e4, nu 0.3, gravity, dt 1e-4,
     200 steps, checkpoint every 20, FP32; loss |mean x(T) - (0.7, 0.3, 0.5)|^2.
     Returns the time of a taped forward + reverse pass over the time of a
-    plain forward pass of the same steps."""
+    plain forward pass of the same steps, both after an untimed warm-up of
+    each (as in a training loop past its first iteration)."""
     cells, dt = 64, 1e-4
     box = [dict(type=mpmx.SHAPE_BOX, a=[0.35] * 3, b=[0.65] * 3, density=1000.0, E=1e4, nu=0.3, v=[0.0] * 3)]
     P = mpmx.seed_particles(3, cells, [0.0] * 3, [1.0] * 3, box, ppc_axis=2, seed=SEED, precision=mpmx.FP32)
@@ -179,6 +180,12 @@ def adjoint_ratio(log, steps=200, checkpoint=20):
 
     load()
     ctx.step(dt, 20)  # warm-up
+    # one untimed taped run as a training loop's first iteration: it sizes the
+    # tape storage (checkpoints, segment states and grids), kept for later tapes
+    load()
+    ctx.tape_begin(checkpoint)
+    ctx.step(dt, steps)
+    ctx.backward(dX=np.zeros((n, 3)))
     load()
     ctx.synchronize()
     t0 = time.perf_counter()

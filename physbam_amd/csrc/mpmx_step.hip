@@ -3505,9 +3505,8 @@ class Engine final : public EngineBase {
     taping_ = true;
     tape_k_ = k;
     tape_dt_.clear();
-    ckpts_.clear();
-    ckpts_.emplace_back();
-    save_state(ckpts_.back());
+    nckpt_ = 0;
+    save_state(next_checkpoint());
   }
 
   void backward(const void* dX, const void* dV, const void* dF, const void* dB, void* gX, void* gV, void* gF, void* gB,
@@ -3532,8 +3531,10 @@ class Engine final : public EngineBase {
     const bool invalidated = valid_now != tape_valid_;
     const int ns = (int)tape_dt_.size(), k = tape_k_;
     const size_t kk = (size_t)std::min(k, std::max(ns, 1));
-    std::vector<StateBuf> seg(kk);
-    std::vector<StepGrid> sgrid(kk);
+    if (seg_.size() < kk) seg_.resize(kk);
+    if (sgrid_.size() < kk) sgrid_.resize(kk);
+    std::vector<StateBuf>& seg = seg_;
+    std::vector<StepGrid>& sgrid = sgrid_;
     seg_counts_.alloc(kk);
     const size_t nodes = (size_t)ntotal_ * 64;
     replaying_ = true;
@@ -3541,7 +3542,7 @@ class Engine final : public EngineBase {
     for (int sg = ns > 0 ? (ns - 1) / k : -1; sg >= 0; sg--) {
       const int s0 = sg * k, s1 = std::min(ns, s0 + k);
       // forward from the checkpoint, keeping each step's state, grid and tiles
-      load_state(ckpts_[(size_t)sg]);
+      load_state(ckpts_[(size_t)sg]);  // sg < nckpt_
       for (int j = s0; j < s1; j++) {
         const size_t i = (size_t)(j - s0);
         save_state(seg[i]);
@@ -3577,7 +3578,7 @@ class Engine final : public EngineBase {
     load_state(fin);
     time_ = time0;
     taping_ = false;
-    ckpts_.clear();
+    nckpt_ = 0;
     HIPCHECK(hipStreamSynchronize(stream_));
     if (errs[2]) throw Error{MPMX_EINVAL, "adjoint: a neo-Hookean particle is in the clamp branch (J < threshold)"};
     if (errs[0]) throw Error{MPMX_EINTERNAL, "particle stencil outside the grid"};
@@ -3956,8 +3957,7 @@ class Engine final : public EngineBase {
     if (taping_ && !replaying_) {
       tape_dt_.push_back((double)a.dt);
       if (tape_dt_.size() % (size_t)tape_k_ == 0) {
-        ckpts_.emplace_back();
-        save_state(ckpts_.back());
+        save_state(next_checkpoint());
       }
     }
     HIPCHECK(hipGetLastError());
@@ -4175,8 +4175,7 @@ class Engine final : public EngineBase {
     if (taping_) {
       tape_dt_.push_back((double)dt);
       if (tape_dt_.size() % (size_t)tape_k_ == 0) {
-        ckpts_.emplace_back();
-        save_state(ckpts_.back());
+        save_state(next_checkpoint());
       }
     }
     HIPCHECK(hipGetLastError());
@@ -4210,7 +4209,17 @@ class Engine final : public EngineBase {
   int tape_k_ = 1;
   int64_t tape_valid_ = 0;
   std::vector<double> tape_dt_;
+  // Tape storage, kept across tapes so that a training loop allocates once:
+  // checkpoints (the first nckpt_ are live) and, for the reverse pass, the
+  // states and grids of one segment.
   std::vector<StateBuf> ckpts_;
+  size_t nckpt_ = 0;
+  std::vector<StateBuf> seg_;
+  std::vector<StepGrid> sgrid_;
+  StateBuf& next_checkpoint() {
+    if (nckpt_ == ckpts_.size()) ckpts_.emplace_back();
+    return ckpts_[nckpt_++];
+  }
   DevBuf<T> aX_, aV_, aF_, aB_;
   DevBuf<double> amu_, ala_, paramstage_;
   DevBuf<Node4<T>> agrid_;
