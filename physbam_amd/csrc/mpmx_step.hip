@@ -3896,10 +3896,12 @@ class Engine final : public EngineBase {
     p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
     p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
     timer_.end(PH_P2G, e1, stream_);
-    const int slot = (int)(step_count_ & 3);
-    HIPCHECK(hipMemcpyAsync(rb_host_ + slot, counts_.p + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
-    HIPCHECK(hipEventRecord(rb_ev_[slot], stream_));
-    rb_meta_[slot] = Readback{step_count_, bin_stamp_};
+    if (!replaying_) {  // the mover count for the rebin policy; a reverse pass's replay keeps the host free
+      const int slot = (int)(step_count_ & 3);
+      HIPCHECK(hipMemcpyAsync(rb_host_ + slot, counts_.p + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+      HIPCHECK(hipEventRecord(rb_ev_[slot], stream_));
+      rb_meta_[slot] = Readback{step_count_, bin_stamp_};
+    }
 
     hipEvent_t e2 = timer_.begin(stream_);
     Args<T, D> ag = a;  // the grid update's arguments
