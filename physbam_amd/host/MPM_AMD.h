@@ -127,6 +127,20 @@ template<class TV> class MPM_EXAMPLE {
   std::vector<COLLISION_BOX> collision_objects;
   int reorder_interval = 16;
   bool sync_particles_every_step = false;
+  // Reduce_Dt (MPM_EXAMPLE.h:95-104): -strong_cfl, -sound_cfl, -cfl_F, -cfl_c
+  bool use_strong_cfl = false, use_sound_speed_cfl = false;
+  T cfl_F = (T)0.5, cfl_sound = (T)0.9;
+  // reflection boundary conditions (MPM_EXAMPLE.h:38, :106-109, :134):
+  // -use_reflect, -use_reflect_friction
+  enum BC_TYPE { BC_FREE = MPMX_BC_FREE, BC_SLIP = MPMX_BC_SLIP, BC_NOSLIP = MPMX_BC_NOSLIP, BC_SEP = MPMX_BC_SEPARATE };
+  bool reflection_bc = false, use_full_reflection = false;
+  T reflection_bc_friction = 0;
+  std::array<BC_TYPE, 2 * TV::m> side_bc_type{};  // -x, +x, -y, +y, -z, +z
+  // plasticity after G2P: 0 none, 1 MPM_PLASTICITY_CLAMP, 2 MPM_DRUCKER_PRAGER
+  int plasticity = 0;
+  T theta_c = 0, theta_s = 0, max_hardening = 0, hardening_factor = 0;  // clamp
+  T dp_a0 = 0, dp_a1 = 0, dp_a3 = 0, dp_a4 = 0, dp_sigma_Y = 0;          // Drucker-Prager
+  bool use_vc = true;
   std::map<std::string, std::array<std::vector<std::function<void()>>, 2>> time_step_callbacks;  // end, begin
 
   // STANDARD_TESTS_BASE::Set_Lame_On_Particles (Projects/mpm/STANDARD_TESTS_BASE.cpp:567-586)
@@ -150,6 +164,30 @@ template<class TV> class MPM_EXAMPLE {
   void Add_Neo_Hookean(T E, T nu, const std::vector<int>* affected = nullptr) {
     Set_Model(MPMX_NEO_HOOKEAN);
     Set_Lame_On_Particles(E, nu, affected);
+  }
+  // STANDARD_TESTS_BASE::Add_St_Venant_Kirchhoff_Hencky_Strain (STANDARD_TESTS_BASE.cpp:394-405)
+  void Add_St_Venant_Kirchhoff_Hencky_Strain(T E, T nu, const std::vector<int>* affected = nullptr) {
+    Set_Model(MPMX_HENCKY);
+    Set_Lame_On_Particles(E, nu, affected);
+  }
+  // STANDARD_TESTS_BASE::Add_Clamped_Plasticity (STANDARD_TESTS_BASE.cpp:465-486), all particles
+  void Add_Clamped_Plasticity(T tc, T ts, T max_h, T xi) {
+    plasticity = 1;
+    theta_c = tc;
+    theta_s = ts;
+    max_hardening = max_h;
+    hardening_factor = xi;
+  }
+  // STANDARD_TESTS_BASE::Add_Drucker_Prager (STANDARD_TESTS_BASE.cpp:407-431): the Hencky-strain
+  // model with Drucker-Prager plasticity on all particles
+  void Add_Drucker_Prager(T E, T nu, T a0, T a1, T a3, T a4, T sigma_Y = 0) {
+    Add_St_Venant_Kirchhoff_Hencky_Strain(E, nu);
+    plasticity = 2;
+    dp_a0 = a0;
+    dp_a1 = a1;
+    dp_a3 = a3;
+    dp_a4 = a4;
+    dp_sigma_Y = sigma_Y;
   }
   void Add_Gravity(const TV& g) { gravity = g; }
   // MPM_EXAMPLE::Add_Collision_Object for an ANALYTIC_IMPLICIT_OBJECT<RANGE<TV>>
@@ -209,11 +247,43 @@ template<class TV> class MPM_DRIVER {
       Check(mpmx_add_collision_box(ctx, (int)c.type, (double)c.friction, lo, hi));
     }
     Check(mpmx_set_reorder_interval(ctx, example.reorder_interval));
+    if (example.reflection_bc) {
+      int sides[6] = {0, 0, 0, 0, 0, 0};
+      for (int i = 0; i < 2 * d; i++) sides[i] = (int)example.side_bc_type[i];
+      Check(mpmx_set_reflection(ctx, 1, example.use_full_reflection ? 1 : 0, (double)example.reflection_bc_friction,
+                                sides));
+    }
+    if (example.use_strong_cfl || example.use_sound_speed_cfl)
+      Check(mpmx_set_dt_limits(ctx, example.use_strong_cfl, example.use_sound_speed_cfl, (double)example.cfl,
+                               (double)example.cfl_F, (double)example.cfl_sound, (double)example.min_dt));
+    if (example.plasticity == 1)
+      Check(mpmx_set_plasticity(ctx, 1, (double)example.theta_c, (double)example.theta_s,
+                                (double)example.max_hardening, (double)example.hardening_factor));
+    if (example.plasticity == 2)
+      Check(mpmx_set_drucker_prager(ctx, 1, (double)example.dp_a0, (double)example.dp_a1, (double)example.dp_a3,
+                                    (double)example.dp_a4, (double)example.dp_sigma_Y, example.use_vc ? 1 : 0));
     Write_Particles();
   }
 
-  // MPM_DRIVER::Advance_One_Time_Step (MPM_DRIVER.cpp:141-157) with example.dt.
-  void Advance_One_Time_Step() { Step([this] { Check(mpmx_step(ctx, (double)example.dt, 1)); }, "time-step"); }
+  // MPM_EXAMPLE::Write_Output_Files / Read_Output_Files (MPM_EXAMPLE.cpp:52-102):
+  // particles and restart data of one frame directory.
+  void Write_Output_Files(const std::string& frame_dir) { Check(mpmx_write_output_files(ctx, frame_dir.c_str())); }
+  void Read_Output_Files(const std::string& frame_dir) {
+    Check(mpmx_read_output_files(ctx, frame_dir.c_str()));
+    example.time = (T)mpmx_time(ctx);
+    Read_Particles();
+  }
+
+  // MPM_DRIVER::Advance_One_Time_Step (MPM_DRIVER.cpp:141-157) with example.dt;
+  // Reduce_Dt may lower example.dt, as in the reference.
+  void Advance_One_Time_Step() {
+    Step([this] {
+      Check(mpmx_step(ctx, (double)example.dt, 1));
+      double taken = 0;
+      Check(mpmx_last_dt(ctx, &taken));
+      example.dt = (T)taken;
+    }, "time-step");
+  }
 
   // MPM_DRIVER::Grid_V_Upper_Bound / Compute_Dt (MPM_DRIVER.cpp:1098-1138) of
   // the device particles.

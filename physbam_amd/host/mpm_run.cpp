@@ -8,10 +8,16 @@
 // With a frame count, the run is MPM_DRIVER::Simulate_To_Frame instead:
 // adaptive substeps with frame_dt 1/60, cfl 0.5, max_dt 1e-3 (steps ignored),
 // and the substep count (int64) and final time (double) follow the state.
+// With "reflect", the ground is a reflection boundary condition instead of a
+// wall object (-use_reflect: noslip at -y, slip on the other sides), the
+// square starts at 30 m/s downward so that it reaches the ground, and the
+// run writes its output files (particles and restart data) into <dir> at the
+// end (MPM_EXAMPLE::Write_Output_Files).
 //
-//   mpm_run <out.bin> [steps=200] [frames=0]
+//   mpm_run <out.bin> [steps=200] [frames=0] [reflect <dir>]
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "MPM_AMD.h"
@@ -29,6 +35,7 @@ int main(int argc, char** argv) {
   }
   const int steps = argc > 2 ? atoi(argv[2]) : 200;
   const int frames = argc > 3 ? atoi(argv[3]) : 0;
+  const bool reflect = argc > 5 && std::string(argv[4]) == "reflect";
   MPM_EXAMPLE<TV> example;
   TV lo, hi;
   hi(0) = hi(1) = 1;
@@ -36,7 +43,9 @@ int main(int argc, char** argv) {
 
   // seeding (jittered, 2x2 samples per cell)
   int type[1] = {MPMX_SHAPE_BOX};
-  double a[3] = {0.3, 0.4, 0}, b[3] = {0.7, 0.8, 0}, den[1] = {1000}, E[1] = {1e4}, nu[1] = {0.3}, vel[3] = {0, 0, 0};
+  // with reflection the square is thrown at the ground (30 m/s) so that it reaches the side in the run
+  double a[3] = {0.3, 0.4, 0}, b[3] = {0.7, 0.8, 0}, den[1] = {1000}, E[1] = {1e4}, nu[1] = {0.3},
+         vel[3] = {0, reflect ? -30.0 : 0.0, 0};
   mpmx_seed_desc sd{};
   sd.dim = 2;
   sd.cells[0] = sd.cells[1] = 64;
@@ -69,7 +78,13 @@ int main(int argc, char** argv) {
   wlo(0) = wlo(1) = -7 * h;
   whi(0) = 1 + 7 * h;
   whi(1) = 0.05;
-  example.Add_Collision_Object(RANGE<TV>(wlo, whi), COLLISION_TYPE::stick, 0);
+  if (reflect) {  // Add_Walls under -use_reflect sets side conditions instead of objects
+    example.reflection_bc = true;
+    example.side_bc_type = {MPM_EXAMPLE<TV>::BC_SLIP, MPM_EXAMPLE<TV>::BC_SLIP, MPM_EXAMPLE<TV>::BC_NOSLIP,
+                            MPM_EXAMPLE<TV>::BC_SLIP};
+  } else {
+    example.Add_Collision_Object(RANGE<TV>(wlo, whi), COLLISION_TYPE::stick, 0);
+  }
   example.dt = 1e-4;
   example.frame_dt = 1.0 / 60;
   example.cfl = 0.5;
@@ -93,6 +108,7 @@ int main(int argc, char** argv) {
     else
       for (int s = 0; s < steps; s++) driver.Advance_One_Time_Step();
     driver.Read_Particles();
+    if (reflect) driver.Write_Output_Files(argv[5]);
   } catch (const std::exception& e) {
     fprintf(stderr, "%s\n", e.what());
     return 1;

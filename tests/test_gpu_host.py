@@ -94,3 +94,38 @@ def test_host_mirror_simulate_to_frame_matches_oracle_frames(tmp_path):
         err = np.abs(run[k] - orc[k]).max() / scale
         print(k, f"{err:.2e}", flush=True)
         assert err < 1e-8, (k, err)
+
+
+def test_host_mirror_reflection_and_output_files(tmp_path):
+    """The mirror with example.reflection_bc (noslip ground, slip sides) on the
+    square thrown at the ground, against the oracle's reflection step; the
+    frame directory it writes (MPM_EXAMPLE::Write_Output_Files) holds the
+    final particles and time."""
+    exe = os.path.join(ROOT, "physbam_amd", "mpm_run")
+    out = tmp_path / "reflect.bin"
+    frame = tmp_path / "frame"
+    frame.mkdir()
+    print("mpm_run 200 steps with reflection", flush=True)
+    r = subprocess.run([exe, str(out), "200", "0", "reflect", str(frame)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    run = read_run(out)
+    sc = dict(dim=2, prec=8, model=mt.COROTATED, N=[64, 64], lo=[0.0, 0.0], hi=[1.0, 1.0], gravity=[0.0, -9.8],
+              dt=1e-4, neo_threshold=0.25, walls=[],
+              reflection=dict(full=False, friction=0.0, sides=[mt.SLIP_BC, mt.SLIP_BC, mt.NOSLIP_BC, mt.SLIP_BC]))
+    for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+        sc[k] = run["in_" + k]
+    orc = mt.oracle_run_reflect(sc, 200)
+    assert np.asarray(orc["X"])[:, 1].min() < 0.05  # the square reached the ground
+    assert np.array_equal(run["valid"], orc["valid"])
+    h = 1.0 / 64
+    for k in ("X", "V", "F", "B"):
+        scale = max(np.abs(orc[k]).max(), np.abs(orc["V"]).max() * h) if k == "B" else np.abs(orc[k]).max()
+        err = np.abs(run[k] - orc[k]).max() / scale
+        print(k, f"{err:.2e}", flush=True)
+        assert err < 1e-8, (k, err)
+    from physbam_amd import mpmx
+    got = mpmx.read_particles_file(str(frame / "mpm_particles.gz"), 2, mpmx.FP64,
+                                   {"X": mpmx.ARRAY_VECTOR, "valid": mpmx.ARRAY_BOOL})
+    assert np.array_equal(got["X"], run["X"]) and np.array_equal(got["valid"], run["valid"])
+    assert mpmx.read_restart_file(str(frame / "restart_data.gz")) == pytest.approx(200 * 1e-4, rel=1e-12)
