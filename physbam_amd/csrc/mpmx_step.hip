@@ -3557,13 +3557,33 @@ class Engine final : public EngineBase {
         grid_override_ = nullptr;
         gridu_override_ = nullptr;
       }
+      const ParticleBufs<T, D> live = cur_;
       for (int j = s1 - 1; j >= s0; j--) {
         // The bins of a later state cover an earlier one (movers are listed)
         // unless a particle was invalidated in between: then re-bin.
         const size_t i = (size_t)(j - s0);
-        load_state(seg[i], invalidated);
+        if (invalidated) {
+          load_state(seg[i], true);
+        } else {
+          // the reverse kernels only read X, V, F, B and valid: they read the
+          // saved state where it lies (no copy); a re-bin while taping sorts
+          // keys but moves no particle data
+          T* q = seg[i].s.p;
+          cur_.X = q;
+          cur_.V = q + n * D;
+          cur_.F = q + 2 * n * D;
+          cur_.B = q + n * (2 * D + D * D);
+          cur_.valid = seg[i].valid.p;
+          fused_ = false;
+          fused_last_ = false;
+        }
         backward_step((T)tape_dt_[(size_t)j], sgrid[i], seg_counts_.p + i);
       }
+      cur_.X = live.X;
+      cur_.V = live.V;
+      cur_.F = live.F;
+      cur_.B = live.B;
+      cur_.valid = live.valid;
     }
     timer_.end(PH_ADJOINT, ea, stream_);
     replaying_ = false;
