@@ -165,7 +165,8 @@ def adjoint_ratio(log, steps=200, checkpoint=20):
     200 steps, checkpoint every 20, FP32; loss |mean x(T) - (0.7, 0.3, 0.5)|^2.
     Returns the time of a taped forward + reverse pass over the time of a
     plain forward pass of the same steps, both after an untimed warm-up of
-    each (as in a training loop past its first iteration)."""
+    each (as in a training loop past its first iteration), each the median of
+    three runs."""
     cells, dt = 64, 1e-4
     box = [dict(type=mpmx.SHAPE_BOX, a=[0.35] * 3, b=[0.65] * 3, density=1000.0, E=1e4, nu=0.3, v=[0.0] * 3)]
     P = mpmx.seed_particles(3, cells, [0.0] * 3, [1.0] * 3, box, ppc_axis=2, seed=SEED, precision=mpmx.FP32)
@@ -186,22 +187,32 @@ def adjoint_ratio(log, steps=200, checkpoint=20):
     ctx.tape_begin(checkpoint)
     ctx.step(dt, steps)
     ctx.backward(dX=np.zeros((n, 3)))
-    load()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.step(dt, steps)
-    t_fwd = time.perf_counter() - t0
-    load()
-    ctx.step(dt, 1)
-    load()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.tape_begin(checkpoint)
-    ctx.step(dt, steps)
-    X = ctx.get_particles()["X"].astype(np.float64)
-    dX = np.broadcast_to(2 * (X.mean(0) - target) / n, X.shape)
-    g = ctx.backward(dX=dX)
-    t_fa = time.perf_counter() - t0
+    def forward():
+        load()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.step(dt, steps)
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
+    def forward_adjoint():
+        load()
+        ctx.step(dt, 1)
+        load()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.tape_begin(checkpoint)
+        ctx.step(dt, steps)
+        X = ctx.get_particles()["X"].astype(np.float64)
+        dX = np.broadcast_to(2 * (X.mean(0) - target) / n, X.shape)
+        g = ctx.backward(dX=dX)
+        return time.perf_counter() - t0, g
+
+    # the median of three runs of each (the runs last tens of milliseconds)
+    t_fwd = float(np.median([forward() for _ in range(3)]))
+    runs = [forward_adjoint() for _ in range(3)]
+    t_fa = float(np.median([t for t, _ in runs]))
+    g = runs[-1][1]
     ctx.close()
     assert np.isfinite(g["V"]).all() and np.isfinite(g["mu"]).all()
     log(f"adjoint: {n} particles, forward {t_fwd * 1e3:.1f} ms, forward+adjoint {t_fa * 1e3:.1f} ms")
