@@ -1514,6 +1514,7 @@ template<typename T> struct AdjArgs {
   double* amu; double* ala;    // per-particle Lame adjoints, accumulated over steps
   Node4<T>* agrid;             // vbar (G2P adjoint), then (mbar, Pbar)
   const Node4<T>* gridu;       // (m, u) of the recomputed step
+  uint32_t split;              // g2p_adjoint_kernel: workgroups sharing one tile's particle chunks
 };
 
 // G2P adjoint of particle p, gather part (formulas in mpmx_adjoint.h): node
@@ -1635,21 +1636,23 @@ __device__ __forceinline__ void g2p_adjoint_node(const Geom<T, D>& g, const T (*
   }
 }
 
-// One workgroup per binned tile: the region's velocities are staged in LDS;
-// the tile's particles are grouped by base cell in chunks of 512 as in
-// p2g_kernel, phase A gathers each particle (x-bar, F-bar, ... written to HBM)
-// and leaves its vbar scatter coefficients in LDS, phase B scatters them with
-// the P2G scatter (lanes = cells, per-wavefront copies of the region, no LDS
-// atomics); one HBM atomic per node and component at the flush.  Movers are
-// left to g2p_adjoint_mover_kernel.
-template<typename T, int D>
+// q.split workgroups per binned tile (chunk k of the tile goes to workgroup
+// k mod split, so that scenes with few tiles still fill the device): the
+// region's velocities are staged in LDS; the particles are grouped by base
+// cell in chunks of JT * BLOCK as in p2g_kernel, phase A gathers each particle
+// (x-bar, F-bar, ... written to HBM) and leaves its vbar scatter coefficients
+// in LDS, phase B scatters them with the P2G scatter (lanes = cells,
+// per-wavefront copies of the region, no LDS atomics); one HBM atomic per
+// node and component at each workgroup's flush.  Movers are left to
+// g2p_adjoint_mover_kernel.
+template<typename T, int D, int JT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void g2p_adjoint_kernel(Args<T, D> a, AdjArgs<T> q) {
   using AC = AdjCoef<T, D>;
   using SC = AdjScatterCoef<T, D>;
   using T4 = typename Vec4Type<T>::type;
   static_assert(SC::N == AC::N, "adjoint coefficient layouts");
   constexpr int R = Tile<D>::R, RN = Tile<D>::RN, NW = BLOCK / 64, E = Tile<D>::E;
-  constexpr int JT = 2, CH = JT * BLOCK;
+  constexpr int CH = JT * BLOCK;
   __shared__ T s_v[D][RN];
   __shared__ T4 s_reg[NW][RN];
   __shared__ union {
@@ -1664,8 +1667,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
   const int a_cell = threadIdx.x >> 2, a_j = threadIdx.x & 3;
   const int a_slot = a_j * 64 + a_cell;
   const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
-  for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
+  const uint32_t split = q.split;
+  for (uint32_t item = blockIdx.x; item < ntile * split; item += gridDim.x) {
+    const uint32_t seg = item / split, sub = item - seg * split;
     const uint32_t t = a.ptiles[seg];
+    const uint32_t tb = a.tile_begin[t], te = a.tile_end[t];
+    if (tb + sub * CH >= te) continue;  // uniform over the workgroup: no chunk here
     const Geom<T, D> g = tile_geom(a, t);
     int org[D];
     tile_origin<T, D>(t, g, org);
@@ -1683,8 +1690,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
       T4 z = {0, 0, 0, 0};
       s_reg[k / RN][k % RN] = z;
     }
-    const uint32_t tb = a.tile_begin[t], te = a.tile_end[t];
-    for (uint32_t c0 = tb; c0 < te; c0 += CH) {
+    for (uint32_t c0 = tb + sub * CH; c0 < te; c0 += split * CH) {
       for (int k = threadIdx.x; k < JT * NW * 64; k += BLOCK) (&s_u.wc[0][0][0])[k] = 0;
       __syncthreads();
       // ---- group the chunk by base cell (movers are skipped)
@@ -3759,9 +3765,16 @@ class Engine final : public EngineBase {
     a.gtiles = sgr.tiles.p;
     restore_tiles_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(sgr.tiles.p, seg_count, counts_.p, agrid_.p);
     mark_movers_kernel<T, D, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a, 0u);
-    AdjArgs<T> q{aX_.p, aV_.p, aF_.p, aB_.p, amu_.p, ala_.p, agrid_.p, sgr.gridu.p};
+    // Below ~512 tiles of 512 particles two workgroups of 256-particle chunks
+    // share each tile: twice the workgroups in flight at the same total work.
+    const uint32_t split = n_ < (size_t)512 * 512 ? 2u : 1u;
+    AdjArgs<T> q{aX_.p, aV_.p, aF_.p, aB_.p, amu_.p, ala_.p, agrid_.p, sgr.gridu.p, split};
     const uint32_t pb = std::min<uint32_t>(cdiv(n_, BLOCK), 4096);
-    g2p_adjoint_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a, q);
+    const uint32_t tb = std::min<uint32_t>(tile_blocks() * split, 8192);
+    if (split == 2)
+      g2p_adjoint_kernel<T, D, 1><<<tb, BLOCK, 0, stream_>>>(a, q);
+    else
+      g2p_adjoint_kernel<T, D, 2><<<tb, BLOCK, 0, stream_>>>(a, q);
     g2p_adjoint_mover_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a, q);
     grid_adjoint_kernel<T, D><<<grid_blocks(), BLOCK, 0, stream_>>>(a, q);
     p2g_adjoint_kernel<T, D><<<pb, BLOCK, 0, stream_>>>(a, q);
