@@ -25,6 +25,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -2961,6 +2962,7 @@ class Engine final : public EngineBase {
   // share every launch: their tiles are numbered consecutively.
   Engine(const mpmx_grid_desc* descs, int nscenes) {
     if (nscenes < 1 || nscenes > 65535) throw Error{MPMX_EINVAL, "scene count must be in [1, 65535]"};
+    descs_.assign(descs, descs + nscenes);
     uint64_t total = 0;
     for (int s = 0; s < nscenes; s++) {
       const mpmx_grid_desc& d = descs[s];
@@ -3048,6 +3050,8 @@ class Engine final : public EngineBase {
     }
     for (int i = 0; i < 4; i++)
       if (rb_ev_[i]) (void)hipEventDestroy(rb_ev_[i]);
+    for (hipEvent_t e : {seg_done_[0], seg_done_[1], seg_free_[0], seg_free_[1], rep_go_})
+      if (e) (void)hipEventDestroy(e);
     if (rb_host_) (void)hipHostFree(rb_host_);
     if (vb_host_) (void)hipHostFree(vb_host_);
     if (lim_host_) (void)hipHostFree(lim_host_);
@@ -3497,6 +3501,16 @@ class Engine final : public EngineBase {
   }
 
   // ---- differentiation: checkpointed tape and reverse pass
+  // One saved particle state; one recomputed step's grids and touched tiles.
+  struct StateBuf {
+    DevBuf<T> s;
+    DevBuf<uint8_t> valid;
+  };
+  struct StepGrid {
+    DevBuf<Node4<T>> grid, gridu;
+    DevBuf<uint32_t> tiles;
+  };
+
   void tape_begin(int k) override {
     if (k < 1) throw Error{MPMX_EINVAL, "checkpoint interval must be >= 1"};
     if (n_ == 0) throw Error{MPMX_ESTATE, "no particles"};
@@ -3545,7 +3559,15 @@ class Engine final : public EngineBase {
     const size_t nodes = (size_t)ntotal_ * 64;
     replaying_ = true;
     hipEvent_t ea = timer_.begin(stream_);
-    for (int sg = ns > 0 ? (ns - 1) / k : -1; sg >= 0; sg--) {
+    const int nseg = ns > 0 ? (ns - 1) / k + 1 : 0;
+    if (!invalidated && nseg >= 2) {
+      reverse_overlapped(nseg, kk);
+      timer_.end(PH_ADJOINT, ea, stream_);
+      replaying_ = false;
+      finish_backward(fin, time0, gX, gV, gF, gB, gmu, gla);
+      return;
+    }
+    for (int sg = nseg - 1; sg >= 0; sg--) {
       const int s0 = sg * k, s1 = std::min(ns, s0 + k);
       // forward from the checkpoint, keeping each step's state, grid and tiles
       load_state(ckpts_[(size_t)sg]);  // sg < nckpt_
@@ -3593,6 +3615,12 @@ class Engine final : public EngineBase {
     }
     timer_.end(PH_ADJOINT, ea, stream_);
     replaying_ = false;
+    finish_backward(fin, time0, gX, gV, gF, gB, gmu, gla);
+  }
+
+  // The reverse pass's results to the host and the live state back.
+  void finish_backward(const StateBuf& fin, double time0, void* gX, void* gV, void* gF, void* gB, double* gmu,
+                       double* gla) {
     download_aos(aX_.p, (T*)gX, D);
     download_aos(aV_.p, (T*)gV, D);
     download_aos(aF_.p, (T*)gF, D * D);
@@ -3608,6 +3636,116 @@ class Engine final : public EngineBase {
     HIPCHECK(hipStreamSynchronize(stream_));
     if (errs[2]) throw Error{MPMX_EINVAL, "adjoint: a neo-Hookean particle is in the clamp branch (J < threshold)"};
     if (errs[0]) throw Error{MPMX_EINTERNAL, "particle stencil outside the grid"};
+  }
+
+  // Reverse pass over nseg >= 2 segments with no particle invalidated on the
+  // tape.  A second engine on its own stream recomputes segment s-1 from its
+  // checkpoint (states, grids, touched tiles into storage set (s-1) mod 2)
+  // while this stream runs the reverse kernels of segment s on set s mod 2;
+  // events order each set's writer after its last reader.  The reverse
+  // kernels read the saved states in place and bin them on this engine.
+  void reverse_overlapped(int nseg, size_t kk) {
+    const int k = tape_k_, ns = (int)tape_dt_.size();
+    const size_t n = n_;
+    for (hipEvent_t* e : {&seg_done_[0], &seg_done_[1], &seg_free_[0], &seg_free_[1], &rep_go_})
+      if (!*e) HIPCHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    std::vector<StateBuf>* segs[2] = {&seg_, &segb_};
+    std::vector<StepGrid>* grids[2] = {&sgrid_, &sgridb_};
+    DevBuf<uint32_t>* cnts[2] = {&seg_counts_, &seg_countsb_};
+    for (int x = 0; x < 2; x++) {
+      if (segs[x]->size() < kk) segs[x]->resize(kk);
+      if (grids[x]->size() < kk) grids[x]->resize(kk);
+      cnts[x]->alloc(kk);
+    }
+    Engine& r = replayer();
+    auto replay = [&](int sg) {
+      const int x = sg & 1, s0 = sg * k, s1 = std::min(ns, s0 + k);
+      r.replay_segment(ckpts_[(size_t)sg], tape_dt_.data() + s0, s1 - s0, segs[x]->data(), grids[x]->data(),
+                       cnts[x]->p, seg_free_[x], seg_done_[x]);
+    };
+    replay(nseg - 1);
+    const ParticleBufs<T, D> live = cur_;
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+      const int x = sg & 1, s0 = sg * k, s1 = std::min(ns, s0 + k);
+      HIPCHECK(hipStreamWaitEvent(stream_, seg_done_[x], 0));
+      force_rebin_ = true;  // bins of this engine, from the segment's last saved state
+      for (int j = s1 - 1; j >= s0; j--) {
+        const size_t i = (size_t)(j - s0);
+        T* q = (*segs[x])[i].s.p;
+        cur_.X = q;
+        cur_.V = q + n * D;
+        cur_.F = q + 2 * n * D;
+        cur_.B = q + n * (2 * D + D * D);
+        cur_.valid = (*segs[x])[i].valid.p;
+        fused_ = false;
+        fused_last_ = false;
+        backward_step((T)tape_dt_[(size_t)j], (*grids[x])[i], cnts[x]->p + i);
+      }
+      HIPCHECK(hipEventRecord(seg_free_[x], stream_));
+      if (sg > 0) replay(sg - 1);  // runs beside the reverse kernels just queued
+    }
+    cur_.X = live.X;
+    cur_.V = live.V;
+    cur_.F = live.F;
+    cur_.B = live.B;
+    cur_.valid = live.valid;
+  }
+
+  // The replay engine, set up like this one (gravity, model, walls, rebin
+  // policy, materials in this engine's slot order) in taping mode, so that a
+  // re-bin sorts keys but moves no particle data.
+  Engine& replayer() {
+    if (!rep_) rep_.reset(new Engine(descs_.data(), (int)descs_.size()));
+    Engine& r = *rep_;
+    for (int c = 0; c < D; c++) r.grav_[c] = grav_[c];
+    r.model_ = model_;
+    r.thr_ = thr_;
+    r.root_ = root_;
+    r.nwalls_ = nwalls_;
+    for (int w = 0; w < MAX_WALLS; w++) r.walls_[w] = walls_[w];
+    r.rebin_fraction_ = rebin_fraction_;
+    r.rebin_max_interval_ = rebin_max_interval_;
+    if (r.n_ != n_ || !r.cur_.X) {
+      r.n_ = n_;
+      r.allocate_particles();
+    }
+    r.taping_ = true;
+    r.replaying_ = true;
+    r.fused_ = false;
+    r.fused_last_ = false;
+    HIPCHECK(hipEventRecord(rep_go_, stream_));  // checkpoints and materials are written on this stream
+    HIPCHECK(hipStreamWaitEvent(r.stream_, rep_go_, 0));
+    const size_t n = n_;
+    HIPCHECK(hipMemcpyAsync(r.cur_.mass, cur_.mass, n * sizeof(T), hipMemcpyDeviceToDevice, r.stream_));
+    HIPCHECK(hipMemcpyAsync(r.cur_.vol, cur_.vol, n * sizeof(T), hipMemcpyDeviceToDevice, r.stream_));
+    HIPCHECK(hipMemcpyAsync(r.cur_.mu, cur_.mu, n * sizeof(T), hipMemcpyDeviceToDevice, r.stream_));
+    HIPCHECK(hipMemcpyAsync(r.cur_.la, cur_.la, n * sizeof(T), hipMemcpyDeviceToDevice, r.stream_));
+    if (cur_.pscene && r.cur_.pscene)
+      HIPCHECK(hipMemcpyAsync(r.cur_.pscene, cur_.pscene, n * sizeof(uint16_t), hipMemcpyDeviceToDevice, r.stream_));
+    return r;
+  }
+
+  // On the replay engine: after free_ev, the cnt steps from checkpoint ck,
+  // each step's starting state, grids and touched tiles into seg, sg and
+  // counts_out; done_ev marks the end.
+  void replay_segment(const StateBuf& ck, const double* dts, int cnt, StateBuf* seg, StepGrid* sg, uint32_t* counts_out,
+                      hipEvent_t free_ev, hipEvent_t done_ev) {
+    HIPCHECK(hipStreamWaitEvent(stream_, free_ev, 0));
+    load_state(ck);
+    const size_t nodes = (size_t)ntotal_ * 64;
+    for (int i = 0; i < cnt; i++) {
+      save_state(seg[i]);
+      sg[i].grid.alloc(nodes);
+      sg[i].gridu.alloc(nodes);
+      sg[i].tiles.alloc((size_t)ntotal_);
+      grid_override_ = sg[i].grid.p;
+      gridu_override_ = sg[i].gridu.p;
+      step_once((T)dts[i]);
+      save_tiles_kernel<<<copy_blocks(), BLOCK, 0, stream_>>>(gt_cur_, counts_.p, sg[i].tiles.p, counts_out + i);
+      grid_override_ = nullptr;
+      gridu_override_ = nullptr;
+    }
+    HIPCHECK(hipEventRecord(done_ev, stream_));
   }
 
   void stress(int64_t n, const void* F, const void* volume, const void* mu, const void* lambda, void* tau) override {
@@ -3697,14 +3835,6 @@ class Engine final : public EngineBase {
     mt_next_ = mover_tile2_.p;
   }
 
-  struct StateBuf {
-    DevBuf<T> s;
-    DevBuf<uint8_t> valid;
-  };
-  struct StepGrid {
-    DevBuf<Node4<T>> grid, gridu;
-    DevBuf<uint32_t> tiles;
-  };
   // X, V, F, B and valid of the current state, in slot order.
   void save_state(StateBuf& b) {
     const size_t n = n_;
@@ -4259,6 +4389,13 @@ class Engine final : public EngineBase {
   DevBuf<double> amu_, ala_, paramstage_;
   DevBuf<Node4<T>> agrid_;
   DevBuf<uint32_t> seg_counts_;
+  // Second storage set and the replay engine of the overlapped reverse pass.
+  std::vector<StateBuf> segb_;
+  std::vector<StepGrid> sgridb_;
+  DevBuf<uint32_t> seg_countsb_;
+  std::vector<mpmx_grid_desc> descs_;
+  std::unique_ptr<Engine> rep_;
+  hipEvent_t seg_done_[2] = {}, seg_free_[2] = {}, rep_go_ = nullptr;
   Node4<T>* grid_override_ = nullptr;  // the grid arrays of a recomputed step
   Node4<T>* gridu_override_ = nullptr;
   uint32_t bin_stamp_ = 0;
