@@ -168,3 +168,31 @@ def test_rollout_gradients_match_finite_differences():
             assert abs(fd - an) <= 1e-3 * abs(fd)
     finally:
         sim.close()
+
+
+def test_reverse_passes_in_a_row_on_one_context():
+    """Tapes of different scenes on one context (other particle counts and Lame
+    parameters, as in a training loop): the replay engine of the overlapped
+    reverse pass is re-sized and re-loaded for each, so every reverse pass
+    matches a fresh context's to round-off."""
+    scs = [scene(3, 8, mt.COROTATED, mt.SLIP, seed=s) for s in (1234, 1234, 99)]
+    scs[1]["mu"] = scs[1]["mu"] * 1.7  # the same count with other Lame parameters, then another count
+    adjs = [random_adjoint(sc, 5 + i) for i, sc in enumerate(scs)]
+    sc0 = scs[0]
+    with mpmx.Context(3, sc0["N"], sc0["lo"], sc0["hi"], precision=8) as ctx:
+        ctx.set_constitutive(sc0["model"], 0.25)
+        ctx.set_gravity(sc0["gravity"])
+        for w in sc0["walls"]:
+            ctx.add_collision_box(w["type"], w["friction"], w["lo"], w["hi"])
+        for i, (sc, adj) in enumerate(zip(scs, adjs)):
+            print(f"reverse pass {i}: n={sc['X'].shape[0]}", flush=True)
+            ctx.set_particles(sc["X"], sc["V"], sc["F"], sc["B"], sc["mass"], sc["volume"], sc["mu"], sc["lambda"],
+                              sc["valid"])
+            ctx.tape_begin(3)
+            ctx.step(sc["dt"], 7)
+            got = ctx.backward(*adj)
+            ref = gpu_backward(sc, 7, 3, adj)
+            errs = {q: rel(got[q], ref[q]) for q in ("X", "V", "F", "B", "mu", "lambda")}
+            print(f"  {', '.join(f'{q} {e:.1e}' for q, e in errs.items())}", flush=True)
+            for q, e in errs.items():
+                assert e <= 1e-9, (i, q, e)
