@@ -329,8 +329,9 @@ int mpmx_read_output_files(mpmx_ctx* ctx, const char* frame_dir);
    checkpoint every `checkpoint_every` steps of the following mpmx_step calls.
    mpmx_backward takes the adjoints of the final state (dL/dX, dL/dV, dL/dF,
    dL/dB in the mpmx_set_particles layout; NULL = zero), runs the reverse pass
-   through every recorded step (recomputing each checkpoint segment forward),
-   and returns the adjoints of the state at mpmx_tape_begin (NULL outputs are
+   through every recorded step (recomputing each checkpoint segment forward;
+   with two or more segments, on a second internal stream beside the reverse
+   kernels of the segment after it), and returns the adjoints of the state at mpmx_tape_begin (NULL outputs are
    skipped) and dL/dmu, dL/dlambda per particle (double, accumulated in FP64).
    The particle state afterwards is the final state again; the tape is
    consumed.  Not supported: wall friction (rejected by mpmx_tape_begin) and
