@@ -344,7 +344,9 @@ int mpmx_backward(mpmx_ctx* ctx, const void* dX, const void* dV, const void* dF,
 /* Per-phase GPU timing with HIP events on the context stream.  Phases:
    "rebin" (keys, sort, tile ranges; only on rebuild steps), "bin" (per-step
    grid-tile list and clearing), "p2g", "grid", "g2p", "reorder", "step",
-   "adjoint" (a whole mpmx_backward pass).
+   "adjoint" (a whole mpmx_backward pass), "dt" (the dt limits of a step),
+   "g2p2g" (a fused step's G2P and next P2G with their mover side paths) and
+   "g2p2g_kernel" (the fused kernel of that phase alone).
    mpmx_phase_time returns the accumulated milliseconds and launch count since
    the last reset. */
 int mpmx_enable_timing(mpmx_ctx* ctx, int enable);

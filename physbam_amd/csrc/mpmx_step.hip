@@ -2905,7 +2905,7 @@ struct PhaseTimer {
   };
   std::vector<hipEvent_t> pool;
   std::vector<Pending> pending;
-  static constexpr int NPH = 10;
+  static constexpr int NPH = 11;
   double total[NPH] = {};
   int64_t count[NPH] = {};
   bool on = false;
@@ -2951,9 +2951,9 @@ struct PhaseTimer {
     for (auto e : pool) (void)hipEventDestroy(e);
   }
 };
-enum { PH_BIN = 0, PH_P2G, PH_GRID, PH_G2P, PH_REORDER, PH_STEP, PH_REBIN, PH_ADJOINT, PH_DT, PH_G2P2G };
-static const char* kPhaseNames[PhaseTimer::NPH] = {"bin",  "p2g",   "grid",    "g2p", "reorder",
-                                                   "step", "rebin", "adjoint", "dt",  "g2p2g"};
+enum { PH_BIN = 0, PH_P2G, PH_GRID, PH_G2P, PH_REORDER, PH_STEP, PH_REBIN, PH_ADJOINT, PH_DT, PH_G2P2G, PH_FUSED };
+static const char* kPhaseNames[PhaseTimer::NPH] = {"bin",  "p2g",   "grid",    "g2p", "reorder",     "step",
+                                                   "rebin", "adjoint", "dt",  "g2p2g", "g2p2g_kernel"};
 
 template<typename T, int D>
 class Engine final : public EngineBase {
@@ -4306,13 +4306,16 @@ class Engine final : public EngineBase {
     timer_.end(PH_GRID, e2, stream_);
 
     hipEvent_t e3 = timer_.begin(stream_);
-    if (fric) {
+    hipEvent_t ek = timer_.begin(stream_);  // the fused kernel alone
+    if (fric)
       p2g_kernel<T, D, true, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-      g2p_mover_kernel<T, D, true, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
-    } else {
+    else
       p2g_kernel<T, D, true, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+    timer_.end(PH_FUSED, ek, stream_);
+    if (fric)
+      g2p_mover_kernel<T, D, true, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+    else
       g2p_mover_kernel<T, D, false, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
-    }
     p2g_extra_kernel<T, D, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
     fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 1);
     timer_.end(PH_G2P2G, e3, stream_);
