@@ -145,6 +145,40 @@ int mpmx_grid_stats(mpmx_ctx* ctx, double* mass, double* momentum);
    (MPM_EXAMPLE::Total_Particle_Linear_Momentum, MPM_EXAMPLE.cpp:209-225). */
 int mpmx_particle_stats(mpmx_ctx* ctx, double* mass, double* momentum, int64_t* valid_count);
 
+/* Conservation totals in FP64, the quantities -print_stats reports around
+   each step (MPM_DRIVER::Print_Particle_Stats, Print_Grid_Stats and
+   Print_Energy_Stats, Hybrid_Methods/Examples_And_Drivers/MPM_DRIVER.cpp:
+   1159-1213).  Twelve doubles, in this order in memory: */
+typedef struct mpmx_totals {
+  double mass;       /* sum of particle (node) masses */
+  double linear[3];  /* linear momentum (dim entries) */
+  double angular[3]; /* angular momentum about the origin (2D: angular[0]) */
+  double kinetic;    /* kinetic energy */
+  double potential;  /* particles: elastic + gravity potential energy; grid: 0 */
+  double elastic;    /* particles: sum volume * Energy_Density(singular values of F) */
+  double gravity;    /* particles: -sum m x . g */
+  double count;      /* valid particles, or nodes with nonzero mass */
+} mpmx_totals;
+
+/* Particle totals of the current state over valid particles:
+   Total_Particle_Linear_Momentum, Total_Particle_Angular_Momentum (sum
+   m x cross v - m B:eps, eps the permutation tensor) and
+   Total_Particle_Kinetic_Energy (sum m/2 |v|^2 + m/2 (B D^-1):B with
+   D^-1 = 4/dx^2 of the quadratic spline) (MPM_EXAMPLE.cpp:209-225, :248-263,
+   :302-315), and MPM_EXAMPLE::Potential_Energy (MPM_EXAMPLE.cpp:123-133) of
+   the elastic force (MPM_FINITE_ELEMENTS.cpp:72-80) and gravity
+   (MPM_GRAVITY.cpp:47-55).  Deterministic for a given particle storage order. */
+int mpmx_particle_totals(mpmx_ctx* ctx, mpmx_totals* out);
+
+/* Grid totals of the last step, recorded by every step while grid stats are
+   enabled (mpmx_enable_grid_stats): which = 0 after Particle_To_Grid (node
+   velocities p/m; with reflection boundary conditions before the mass and
+   momentum reflection), which = 1 after the grid update (the velocity G2P
+   reads): Total_Grid_Linear_Momentum, Total_Grid_Angular_Momentum and
+   Total_Grid_Kinetic_Energy (MPM_EXAMPLE.cpp:226-241, :268-297) over the
+   nodes with nonzero mass, node locations the MAC cell centres. */
+int mpmx_grid_totals(mpmx_ctx* ctx, int which, mpmx_totals* out);
+
 /* Dense grid arrays of the last step, indexed like MPM_EXAMPLE::mass and
    ::velocity over Domain_Indices(ghost=3): (cells+6)^dim nodes, row-major
    with the last axis fastest (Core/Arrays_Nd/ARRAYS_ND_BASE.h:88-93).

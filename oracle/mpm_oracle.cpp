@@ -22,7 +22,9 @@
 //   friction     MPM_DRIVER.cpp:1064-1094
 //   G2P          MPM_DRIVER.cpp:450-515
 //   invalidation MPM_DRIVER.cpp:1219-1239, simulated list :1142-1157
-//   reductions   Examples_And_Drivers/MPM_EXAMPLE.cpp:209-315
+//   totals       Examples_And_Drivers/MPM_EXAMPLE.cpp:209-315 (linear and angular momentum, kinetic
+//                energy of particles and grid), :123-133 potential energy with MPM_FINITE_ELEMENTS.cpp:72-80,
+//                MPM_GRAVITY.cpp:47-55 and the models' Energy_Density; MPM_DRIVER.cpp:1159-1213 (-print_stats)
 //   plasticity   Forces/MPM_PLASTICITY_CLAMP.cpp:31-45 (Compute with hardening),
 //                Forces/MPM_DRUCKER_PRAGER.cpp:64-94 (Compute), :123-131 (Update_Hardening),
 //                :40-58 (Initialize_Particles); Hencky stress ST_VENANT_KIRCHHOFF_HENCKY_STRAIN.cpp:60-70
@@ -426,6 +428,29 @@ template<class T,int d> T Sound_Speed(int model,const Vec<T,d>& F,T mu,T la,T th
     const int m=d==2?1:3;
     for(int a=0;a<m;a++) for(int i=0;i<2;i++) cm=std::max(cm,B[a]*F2[(a+i+1)%m]);
     return std::sqrt(cm/density);
+}
+
+// Energy_Density of the singular values F (what MPM_FINITE_ELEMENTS::Potential_Energy
+// sums, MPM_FINITE_ELEMENTS.cpp:72-80):
+//   fixed corotated COROTATED_FIXED.cpp:179-185: mu tr((F-1)^2) + lambda/2 (J-1)^2
+//   neo-Hookean     NEO_HOOKEAN.cpp:191-199:     mu (tr(F F^T)/2 - d/2 - ln J) + lambda/2 ln^2 J, 1e16 for J <= 0
+//   Hencky          ST_VENANT_KIRCHHOFF_HENCKY_STRAIN.cpp:49-59: mu |ln F|^2 + lambda/2 tr(ln F)^2, FLT_MAX if F_d <= 0
+template<class T,int d> T Energy_Density(int model,const Vec<T,d>& F,T mu,T la)
+{
+    if(model==0){
+        T s=0;for(int i=0;i<d;i++){T f=F[i]-1;s+=f*f;}
+        T j=Product(F)-1;
+        return mu*s+(T).5*la*(j*j);}
+    if(model==1){
+        T I1=0;for(int i=0;i<d;i++) I1+=F[i]*F[i];
+        T J=Product(F);
+        if(J<=0) return (T)1e16;
+        T lj=std::log(J);
+        return mu*((T).5*(I1-d)-lj)+(T).5*la*(lj*lj);}
+    if(F[d-1]<=0) return std::numeric_limits<float>::max();
+    Vec<T,d> l;for(int i=0;i<d;i++) l[i]=std::log(F[i]);
+    T tr=0,fn=0;for(int i=0;i<d;i++){tr+=l[i];fn+=l[i]*l[i];}
+    return mu*fn+(T)0.5*la*(tr*tr);
 }
 // Core/Math_Tools/Robust_Arithmetic.h:22-26 (FLT_MAX in T for both precisions).
 template<class T> T Robust_Divide(T a,T b)
@@ -944,6 +969,77 @@ template<class T,int d> Scene<T,d> Make_Scene(const int* N,const double* lo,cons
     return s;
 }
 
+//#####################################################################
+// Conservation totals and energies, as -print_stats reports them
+// (MPM_DRIVER.cpp:1159-1213).  Output layout (11 values): mass, linear
+// momentum (d), angular momentum (SPIN: 1 value in 2D, 3 in 3D) at [4..6],
+// kinetic energy [7], potential energy [8] = elastic [9] + gravity [10]
+// (particles only).  Serial sums in the reference's order (threads = 1).
+//#####################################################################
+// TV::Cross_Product (VECTOR_2D.h:331-332, VECTOR_3D.h:321-322)
+template<class T> void Cross(const Vec<T,2>& a,const Vec<T,2>& b,T* c){c[0]=a[0]*b[1]-a[1]*b[0];}
+template<class T> void Cross(const Vec<T,3>& a,const Vec<T,3>& b,T* c)
+{c[0]=a[1]*b[2]-a[2]*b[1];c[1]=a[2]*b[0]-a[0]*b[2];c[2]=a[0]*b[1]-a[1]*b[0];}
+// MATRIX::Contract_Permutation_Tensor (MATRIX_2X2.h:339-340, MATRIX_3X3.h:392-393)
+template<class T> void Contract_Permutation(const Mat<T,2>& B,T* c){c[0]=B.x[2]-B.x[1];}
+template<class T> void Contract_Permutation(const Mat<T,3>& B,T* c){c[0]=B.x[7]-B.x[5];c[1]=B.x[2]-B.x[6];c[2]=B.x[3]-B.x[1];}
+
+// Total_Particle_Linear_Momentum, Total_Particle_Angular_Momentum (with the
+// -m B:eps term of store_B), Total_Particle_Kinetic_Energy (with the
+// m/2 (B D^-1):B term, D^-1 = 4/dx^2 of the quadratic spline)
+// (MPM_EXAMPLE.cpp:209-225, :248-263, :302-315) and Potential_Energy
+// (MPM_EXAMPLE.cpp:123-133: MPM_FINITE_ELEMENTS then MPM_GRAVITY,
+// MPM_FINITE_ELEMENTS.cpp:72-80 over the singular values of F, MPM_GRAVITY.cpp:47-55)
+// over the valid particles.
+template<class T,int d> void Particle_Totals(const Scene<T,d>& s,const Particles<T,d>& P,T* out)
+{
+    const int m=d==2?1:3;
+    for(int i=0;i<11;i++) out[i]=0;
+    T dpinv[d];
+    for(int a=0;a<d;a++) dpinv[a]=((T)4*s.one_over_dX[a])*s.one_over_dX[a];
+    T pe_fe=0,pe_g=0;
+    for(long long p=0;p<P.n;p++){
+        if(!P.valid[p]) continue;
+        const T mp=P.mass[p];
+        Vec<T,d> X=P.x(p),V=P.v(p);
+        Mat<T,d> B=P.b(p);
+        out[0]+=mp;
+        for(int a=0;a<d;a++) out[1+a]+=mp*V[a];
+        T c[3],e[3];
+        Cross(X,V,c);Contract_Permutation(B,e);
+        for(int a=0;a<m;a++){out[4+a]+=mp*c[a];out[4+a]-=mp*e[a];}
+        T ke=mp/2*dot(V,V);
+        // (B*D).Double_Contract(B): column by column, rows inside (MATRIX_BASE.h:446-449)
+        T bb=0;
+        for(int j=0;j<d;j++) for(int i=0;i<d;i++) bb+=(B(i,j)*dpinv[j])*B(i,j);
+        ke+=mp/2*bb;
+        out[7]+=ke;
+        Mat<T,d> U,W;Vec<T,d> sig;
+        Svd(P.f(p),U,sig,W);
+        pe_fe+=P.volume[p]*Energy_Density(s.model,sig,P.mu[p],P.lambda[p]);
+        pe_g-=mp*dot(X,s.gravity);}
+    out[9]=pe_fe;out[10]=pe_g;
+    T pe=0;pe+=pe_fe;pe+=pe_g;
+    out[8]=pe;
+}
+// Total_Grid_Linear_Momentum, Total_Grid_Angular_Momentum (no midpoint),
+// Total_Grid_Kinetic_Energy of the node velocities u over valid_grid_indices
+// (MPM_EXAMPLE.cpp:226-241, :268-284, :289-297), node locations the cell
+// centres of the MAC grid (MPM_DRIVER.cpp:115-126).
+template<class T,int d> void Grid_Totals(const Scene<T,d>& s,const Grid<T,d>& G,const std::vector<Vec<T,d> >& u,T* out)
+{
+    const int m=d==2?1:3;
+    for(int i=0;i<11;i++) out[i]=0;
+    for(long long i:G.valid){
+        const T mi=G.mass[i];
+        out[0]+=mi;
+        for(int a=0;a<d;a++) out[1+a]+=mi*u[i][a];
+        T c[3];
+        Cross(Node_Center(s,i),u[i],c);
+        for(int a=0;a<m;a++) out[4+a]+=mi*c[a];
+        out[7]+=(T).5*mi*dot(u[i],u[i]);}
+}
+
 template<class T,int d> int Run(const int* N,const double* lo,const double* hi,const double* g,int model,double thr,double dt,
     int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi,
     long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,const T* mu,const T* la,unsigned char* valid,int nsteps,
@@ -1262,6 +1358,42 @@ ORACLE_API int oracle_run_##SUF(int dim,const int* N,const double* lo,const doub
 }
 DEFINE_RUN(f64,double)
 DEFINE_RUN(f32,float)
+
+// Totals of a run, as -print_stats reports them each step: the particles
+// after Update_Simulated_Particles, the grid after Particle_To_Grid (the
+// velocity_save copy) and after Apply_Forces; then the particles at the end.
+// out holds (3 nsteps + 1) records of 11 values (layout of Particle_Totals).
+template<class T,int d> int Run_Totals(const int* N,const double* lo,const double* hi,const double* g,int model,double thr,
+    double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi,
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,const T* mu,const T* la,unsigned char* valid,int nsteps,
+    T* out)
+{
+    Scene<T,d> s=Make_Scene<T,d>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi);
+    Particles<T,d> P{np,X,V,F,B,mass,volume,mu,la,valid};
+    Grid<T,d> G;
+    T time=0;
+    for(int k=0;k<nsteps;k++){
+        Particle_Totals(s,P,out+(3*k)*11);
+        time+=Step(s,P,G,time);
+        Grid_Totals(s,G,G.velocity_save,out+(3*k+1)*11);
+        Grid_Totals(s,G,G.velocity,out+(3*k+2)*11);}
+    Particle_Totals(s,P,out+(3*nsteps)*11);
+    return 0;
+}
+#define DEFINE_TOTALS(SUF,T) \
+ORACLE_API int oracle_run_totals_##SUF(int dim,const int* N,const double* lo,const double* hi,const double* g,int model, \
+    double thr,double dt,int nwalls,const int* wtype,const double* wfric,const double* wlo,const double* whi, \
+    long long np,T* X,T* V,T* F,T* B,const T* mass,const T* volume,const T* mu,const T* la,unsigned char* valid,int nsteps, \
+    T* out) \
+{ \
+    if(dim==2) return Run_Totals<T,2>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume,mu,la,valid, \
+        nsteps,out); \
+    if(dim==3) return Run_Totals<T,3>(N,lo,hi,g,model,thr,dt,nwalls,wtype,wfric,wlo,whi,np,X,V,F,B,mass,volume,mu,la,valid, \
+        nsteps,out); \
+    return -1; \
+}
+DEFINE_TOTALS(f64,double)
+DEFINE_TOTALS(f32,float)
 DEFINE_FRAME(f64,double)
 DEFINE_FRAME(f32,float)
 

@@ -37,6 +37,14 @@
 // (and for flag 2 plastic_def, rho_F, sigma_Y, vc).  Model 2 is the Hencky
 // strain model.
 //   ref_harness weights <prec> <dim> <N> <lo> <hi> <in.bin> <out.bin>
+//   ref_harness totals <scene.bin> <out.bin>
+//     (the scene's nsteps steps with the reductions that -print_stats reports,
+//     taken through the reference's own named-step callbacks: after
+//     "update-weights" the particle totals and MPM_EXAMPLE::Potential_Energy,
+//     after "p2g" and after "forces" the grid totals of example.velocity; then
+//     the particle totals of the final state.  Records of 11 doubles: mass,
+//     linear momentum (3), angular momentum (3; 2D in the first), kinetic
+//     energy, potential energy, its MPM_FINITE_ELEMENTS and MPM_GRAVITY parts)
 //
 // Scene file layout (little endian) is written by tests/golden/scene_io.py.
 //#####################################################################
@@ -301,6 +309,55 @@ int Run_Step(const SCENE& s,const char* out,STREAM_TYPE st)
     return 0;
 }
 
+// Totals of the reference's MPM_EXAMPLE (MPM_EXAMPLE.cpp:123-133, :209-315),
+// recorded through Add_Callbacks around the named steps of Advance_One_Time_Step.
+template<class TV>
+int Run_Totals(const SCENE& s,const char* out,STREAM_TYPE st)
+{
+    typedef typename TV::SCALAR T;
+    enum {d=TV::m};
+#ifdef USE_OPENMP
+    omp_set_num_threads(s.threads>0?s.threads:1);
+#endif
+    RUN<TV> run(s,st);
+    auto& ex=run.ex;
+    std::vector<double> rec;
+    auto put_spin=[&](const typename TV::SPIN& a){for(int i=0;i<3;i++) rec.push_back(i<a.m?(double)a(i):0.0);};
+    auto put_tv=[&](const TV& a){for(int i=0;i<3;i++) rec.push_back(i<d?(double)a(i):0.0);};
+    auto particles=[&](){
+        T m=0;
+        for(int k=0;k<ex.simulated_particles.m;k++) m+=ex.particles.mass(ex.simulated_particles(k));
+        rec.push_back(m);
+        put_tv(ex.Total_Particle_Linear_Momentum());
+        put_spin(ex.Total_Particle_Angular_Momentum());
+        rec.push_back(ex.Total_Particle_Kinetic_Energy());
+        ex.Capture_Stress();
+        ex.Precompute_Forces(ex.time,ex.dt,false);
+        rec.push_back(ex.Potential_Energy(ex.time));
+        rec.push_back(ex.forces(0)->Potential_Energy(ex.time));
+        rec.push_back(ex.forces(1)->Potential_Energy(ex.time));};
+    auto grid=[&](){
+        T m=0;
+        for(int k=0;k<ex.valid_grid_indices.m;k++) m+=ex.mass.array(ex.valid_grid_indices(k));
+        rec.push_back(m);
+        put_tv(ex.Total_Grid_Linear_Momentum(ex.velocity));
+        put_spin(ex.Total_Grid_Angular_Momentum(ex.dt,ex.velocity,0));
+        rec.push_back(ex.Total_Grid_Kinetic_Energy(ex.velocity));
+        for(int i=0;i<3;i++) rec.push_back(0);};
+    ex.Add_Callbacks(false,"update-weights",particles);
+    ex.Add_Callbacks(false,"p2g",grid);
+    ex.Add_Callbacks(false,"forces",grid);
+    for(int k=0;k<s.nsteps;k++) run.Step();
+    run.driver->Update_Simulated_Particles();
+    run.driver->Update_Particle_Weights();
+    particles();
+    WRITER w(out);
+    int nrec=(int)(rec.size()/11);
+    w.put(nrec);
+    w.put(rec.data(),rec.size());
+    return 0;
+}
+
 // Frames with the reference's adaptive dt: the substep loop of
 // MPM_DRIVER::Simulate_To_Frame (MPM_DRIVER.cpp:173-197) around the
 // reference's Compute_Dt and Advance_One_Time_Step, without callbacks and
@@ -557,6 +614,17 @@ int main(int argc,char* argv[])
             STREAM_TYPE st((float()));
             if(s.dim==2) rc=mode=="step"?Run_Step<VECTOR<float,2> >(s,argv[3],st):Run_Time<VECTOR<float,2> >(s,atoi(argv[3]),atoi(argv[4]),atoi(argv[5]),st);
             else rc=mode=="step"?Run_Step<VECTOR<float,3> >(s,argv[3],st):Run_Time<VECTOR<float,3> >(s,atoi(argv[3]),atoi(argv[4]),atoi(argv[5]),st);}
+        LOG::Finish_Logging();
+        return rc;}
+    if(mode=="totals" && argc==4){
+        SCENE s=Read_Scene(argv[2]);
+        int rc;
+        if(s.prec==8){
+            STREAM_TYPE st((double()));
+            rc=s.dim==2?Run_Totals<VECTOR<double,2> >(s,argv[3],st):Run_Totals<VECTOR<double,3> >(s,argv[3],st);}
+        else{
+            STREAM_TYPE st((float()));
+            rc=s.dim==2?Run_Totals<VECTOR<float,2> >(s,argv[3],st):Run_Totals<VECTOR<float,3> >(s,argv[3],st);}
         LOG::Finish_Logging();
         return rc;}
     if(mode=="frames" && (argc==9 || argc==13)){

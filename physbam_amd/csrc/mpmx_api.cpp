@@ -9,6 +9,8 @@
 #include "mpmx_engine.h"
 #include "mpmx_io.h"
 
+static_assert(sizeof(mpmx_totals) == 12 * sizeof(double), "mpmx_totals is twelve packed doubles");
+
 #include <unistd.h>
 
 #include <cstdio>
@@ -154,6 +156,18 @@ int mpmx_grid_stats(mpmx_ctx* ctx, double* mass, double* momentum) {
 int mpmx_particle_stats(mpmx_ctx* ctx, double* mass, double* momentum, int64_t* valid_count) {
   if (!ctx) return bad("null context");
   return guard([&] { ctx->engine->particle_stats(mass, momentum, valid_count); });
+}
+
+int mpmx_particle_totals(mpmx_ctx* ctx, mpmx_totals* out) {
+  if (!ctx) return bad("null context");
+  if (!out) return bad("null output");
+  return guard([&] { ctx->engine->particle_totals(&out->mass); });
+}
+
+int mpmx_grid_totals(mpmx_ctx* ctx, int which, mpmx_totals* out) {
+  if (!ctx) return bad("null context");
+  if (!out) return bad("null output");
+  return guard([&] { ctx->engine->grid_totals(which, &out->mass); });
 }
 
 int64_t mpmx_grid_node_count(const mpmx_ctx* ctx) { return ctx ? ctx->engine->grid_node_count() : -1; }

@@ -333,6 +333,43 @@ MPMX_HD void p_hat(int model, T mu, T la, T thr, T root, const T* s, T* P) {
   for (int i = 0; i < D; i++) P[i] = mu * s[i] + c * (Fi[i] * Fi[i] * dF[i] - Fi[i]) + la * ip * Fi[i];
 }
 
+// Energy density Psi of the singular values s (the term MPM_FINITE_ELEMENTS::
+// Potential_Energy sums, Forces/MPM_FINITE_ELEMENTS.cpp:72-80):
+//   fixed corotated: mu sum (s_i - 1)^2 + lambda/2 (J - 1)^2       (COROTATED_FIXED.cpp:179-185)
+//   neo-Hookean:     mu (sum s_i^2 / 2 - d/2 - ln J) + lambda/2 ln^2 J, 1e16 for J <= 0
+//                    (NEO_HOOKEAN.cpp:191-199; no clamp: the energy reads s itself)
+//   Hencky:          mu sum ln^2 s_i + lambda/2 (sum ln s_i)^2, FLT_MAX if s_d <= 0
+//                    (ST_VENANT_KIRCHHOFF_HENCKY_STRAIN.cpp:49-59)
+template<typename T, int D>
+MPMX_HD T energy_density(int model, T mu, T la, const T* s) {
+  T J = s[0];
+#pragma unroll
+  for (int i = 1; i < D; i++) J *= s[i];
+  if (model == 0) {
+    T q = 0;
+#pragma unroll
+    for (int i = 0; i < D; i++) q += (s[i] - (T)1) * (s[i] - (T)1);
+    return mu * q + (T)0.5 * la * (J - (T)1) * (J - (T)1);
+  }
+  if (model == 1) {
+    if (!(J > (T)0)) return (T)1e16;
+    T i1 = 0;
+#pragma unroll
+    for (int i = 0; i < D; i++) i1 += s[i] * s[i];
+    const T lj = tlog<T>(J);
+    return mu * ((T)0.5 * (i1 - (T)D) - lj) + (T)0.5 * la * lj * lj;
+  }
+  if (!(s[D - 1] > (T)0)) return (T)3.402823466e38;
+  T tr = 0, fn = 0;
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    const T l = tlog<T>(s[i]);
+    tr += l;
+    fn += l * l;
+  }
+  return mu * fn + (T)0.5 * la * tr * tr;
+}
+
 // Sound speed of a particle (ISOTROPIC_CONSTITUTIVE_MODEL::Sound_Speed,
 // ISOTROPIC_CONSTITUTIVE_MODEL.cpp:151-162) from its singular values s:
 // sqrt(max(H_ii s_i^2, B_a s_{(a+1)%m}^2, B_a s_{(a+2)%m}^2) / density) with

@@ -38,6 +38,8 @@ struct EngineBase {
   virtual void enable_grid_stats(bool on) = 0;
   virtual void grid_stats(double* mass, double* momentum) = 0;
   virtual void particle_stats(double* mass, double* momentum, int64_t* valid_count) = 0;
+  virtual void grid_totals(int which, double* out) = 0;
+  virtual void particle_totals(double* out) = 0;
   virtual int64_t grid_node_count() const = 0;
   virtual void get_grid(void* mass, void* velocity) = 0;
   virtual void set_reorder_interval(int interval) = 0;

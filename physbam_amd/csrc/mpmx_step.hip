@@ -87,7 +87,6 @@ template<typename T, int D> struct Args {
   uint32_t* mover_tile2;
   uint32_t mcap;
   uint32_t* err;
-  double* stats_partial;
   // reflection boundary conditions: bit 2a (2a+1) set when particles leaving
   // through the low (high) side of axis a are reflected instead of
   // invalidated; band: the tiles along the domain sides are always active
@@ -896,7 +895,6 @@ __device__ __forceinline__ int walls_apply(const Args<T, D>& a, const T* X0, T* 
 template<typename T, int D, bool FRICTION>
 __global__ __launch_bounds__(BLOCK) void grid_kernel(Args<T, D> a) {
   const uint64_t total = (uint64_t)a.counts[1] * 64u;
-  double acc[4] = {0, 0, 0, 0};
   for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total; idx += (uint64_t)gridDim.x * BLOCK) {
     const uint32_t t = a.gtiles[idx >> 6];
     const uint32_t l = (uint32_t)(idx & 63u);
@@ -1003,37 +1001,7 @@ __global__ __launch_bounds__(BLOCK) void grid_kernel(Args<T, D> a) {
       of.z = D == 3 ? vf[D - 1] : (T)0;
       a.gridf[slot] = of;
     }
-    if (a.stats_partial) {
-      acc[0] += (double)nd.m;
-#pragma unroll
-      for (int c = 0; c < D; c++) acc[1 + c] += (double)nd.m * (double)v[c];
-    }
   }
-  if (a.stats_partial) {
-    __shared__ double s_acc[BLOCK / 64][4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      double x = acc[c];
-      for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-      if (lane == 0) s_acc[wv][c] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-      double s = 0;
-      for (int q = 0; q < BLOCK / 64; q++) s += s_acc[q][threadIdx.x];
-      a.stats_partial[blockIdx.x * 4 + threadIdx.x] = s;
-    }
-  }
-}
-
-__global__ void reduce_partials_kernel(const double* partial, int nblocks, double* out) {
-  // one wave; fixed order per lane then a fixed shuffle tree
-  const int c = threadIdx.x & 3, lane = threadIdx.x;
-  double s = 0;
-  for (int b = lane >> 2; b < nblocks; b += 16) s += partial[b * 4 + c];
-  for (int o = 32; o >= 4; o >>= 1) s += __shfl_down(s, o, 64);
-  if (lane < 4) out[lane] = s;
 }
 
 // ------------------------------------------------------------------ G2P ----
@@ -2327,34 +2295,184 @@ __global__ __launch_bounds__(BLOCK) void export_grid_kernel(Args<T, D> a, T* dma
   }
 }
 
-template<typename T, int D>
-__global__ __launch_bounds__(BLOCK) void particle_stats_kernel(const T* V, const T* mass, const uint8_t* valid,
-                                                               uint32_t n, double* partial) {
-  double acc[4] = {0, 0, 0, 0};
-  double cnt = 0;
-  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < n; p += gridDim.x * BLOCK) {
-    if (!valid[p]) continue;
-    const double m = (double)mass[p];
-    acc[0] += m;
-#pragma unroll
-    for (int c = 0; c < D; c++) acc[1 + c] += m * (double)V[(size_t)c * n + p];
-    cnt += 1;
+// ---------------------------------------------------- conservation totals --
+// FP64 totals as -print_stats reports them (MPM_DRIVER.cpp:1159-1213), in the
+// layout of mpmx_totals: mass [0], linear momentum [1..3], angular momentum
+// [4..6] (2D: [4]), kinetic energy [7], potential energy [8] = elastic [9] +
+// gravity [10], count [11].  Deterministic: a fixed number of blocks strides
+// over the items, each block reduces in a fixed tree, and one wavefront sums
+// the block partials in block order.
+constexpr int NTOT = 12;
+constexpr int TOTALS_BLOCKS = 1024;  // block partials of one totals pass
+
+template<int D> __device__ __forceinline__ void cross_d(const double* x, const double* v, double* c) {
+  if constexpr (D == 3) {
+    c[0] = x[1] * v[2] - x[2] * v[1];
+    c[1] = x[2] * v[0] - x[0] * v[2];
+    c[2] = x[0] * v[1] - x[1] * v[0];
+  } else {
+    c[0] = x[0] * v[1] - x[1] * v[0];
+    c[1] = c[2] = 0;
   }
-  __shared__ double s[BLOCK / 64][5];
+}
+
+// Block partial of NTOT accumulators into partial[blockIdx.x * NTOT + c].
+__device__ __forceinline__ void totals_block_partial(const double* acc, double* partial) {
+  __shared__ double s[BLOCK / 64][NTOT];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  double vals[5] = {acc[0], acc[1], acc[2], acc[3], cnt};
 #pragma unroll
-  for (int c = 0; c < 5; c++) {
-    double x = vals[c];
+  for (int c = 0; c < NTOT; c++) {
+    double x = acc[c];
     for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
     if (lane == 0) s[wv][c] = x;
   }
   __syncthreads();
-  if (threadIdx.x < 5) {
+  if (threadIdx.x < NTOT) {
     double t = 0;
     for (int q = 0; q < BLOCK / 64; q++) t += s[q][threadIdx.x];
-    partial[blockIdx.x * 5 + threadIdx.x] = t;
+    partial[blockIdx.x * NTOT + threadIdx.x] = t;
   }
+}
+
+// Particle totals over the valid particles: Total_Particle_Linear_Momentum,
+// Total_Particle_Angular_Momentum with the -m B:eps term, Total_Particle_
+// Kinetic_Energy with the m/2 (B D^-1):B term (D^-1 = 4/dx^2 per axis)
+// (MPM_EXAMPLE.cpp:209-225, :248-263, :302-315) and Potential_Energy of
+// MPM_FINITE_ELEMENTS (vol Psi over the singular values of F, FP64 SVD) and
+// MPM_GRAVITY (-m x.g) (MPM_EXAMPLE.cpp:123-133, MPM_FINITE_ELEMENTS.cpp:72-80,
+// MPM_GRAVITY.cpp:47-55).  X, V, F, B in the particles' scene coordinates.
+template<typename T, int D> struct TotalsIn {
+  const T* X; const T* V; const T* F; const T* B; const T* mass; const T* vol; const T* mu; const T* la;
+  const uint8_t* valid;
+  const uint16_t* pscene;
+  const Geom<T, D>* scenes;
+  uint32_t n;
+  int model;
+  double grav[3];
+};
+
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) void particle_totals_kernel(TotalsIn<T, D> in, double* partial) {
+  double acc[NTOT];
+#pragma unroll
+  for (int c = 0; c < NTOT; c++) acc[c] = 0;
+  const uint32_t n = in.n;
+  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < n; p += gridDim.x * BLOCK) {
+    if (!in.valid[p]) continue;
+    const Geom<T, D> g = in.scenes[in.pscene ? in.pscene[p] : 0];
+    const double m = (double)in.mass[p];
+    double x[D], v[D], Fd[D * D], Bd[D * D];
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      x[c] = (double)in.X[(size_t)c * n + p];
+      v[c] = (double)in.V[(size_t)c * n + p];
+    }
+#pragma unroll
+    for (int k = 0; k < D * D; k++) {
+      Fd[k] = (double)in.F[(size_t)k * n + p];
+      Bd[k] = (double)in.B[(size_t)k * n + p];
+    }
+    acc[0] += m;
+    double vv = 0;
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      acc[1 + c] += m * v[c];
+      vv += v[c] * v[c];
+    }
+    double cr[3], e[3];
+    cross_d<D>(x, v, cr);
+    // MATRIX::Contract_Permutation_Tensor (MATRIX_3X3.h:392-393, MATRIX_2X2.h:339-340), column-major
+    if constexpr (D == 3) {
+      e[0] = Bd[7] - Bd[5];
+      e[1] = Bd[2] - Bd[6];
+      e[2] = Bd[3] - Bd[1];
+    } else {
+      e[0] = Bd[2] - Bd[1];
+    }
+    constexpr int NS = D == 3 ? 3 : 1;
+#pragma unroll
+    for (int c = 0; c < NS; c++) acc[4 + c] += m * cr[c] - m * e[c];
+    double bb = 0;
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      const double dinv = 4.0 * (double)g.inv_dx[j] * (double)g.inv_dx[j];
+#pragma unroll
+      for (int i = 0; i < D; i++) bb += Bd[i + D * j] * Bd[i + D * j] * dinv;
+    }
+    acc[7] += 0.5 * m * vv + 0.5 * m * bb;
+    double U[D * D], sg[D], W[D * D];
+    svd<double, D>(Fd, U, sg, W);
+    acc[9] += (double)in.vol[p] * energy_density<double, D>(in.model, (double)in.mu[p], (double)in.la[p], sg);
+    double xg = 0;
+#pragma unroll
+    for (int c = 0; c < D; c++) xg += x[c] * in.grav[c];
+    acc[10] -= m * xg;
+    acc[11] += 1;
+  }
+  totals_block_partial(acc, partial);
+}
+
+// Grid totals over the nodes of the grid-tile list with nonzero mass, node
+// locations the MAC cell centres lo + (i + 1/2) dx (MPM_DRIVER.cpp:115-126):
+// Total_Grid_Linear_Momentum, Total_Grid_Angular_Momentum (no midpoint) and
+// Total_Grid_Kinetic_Energy (MPM_EXAMPLE.cpp:226-241, :268-284, :289-297).
+// MOMENTUM: the nodes hold P2G's (m, m v) (the velocity after Particle_To_Grid
+// is v = p / m); otherwise (m, v) after the grid update.
+template<typename T, int D, bool MOMENTUM>
+__global__ __launch_bounds__(BLOCK) void grid_totals_kernel(Args<T, D> a, const Node4<T>* grid, double* partial) {
+  double acc[NTOT];
+#pragma unroll
+  for (int c = 0; c < NTOT; c++) acc[c] = 0;
+  const uint64_t total = (uint64_t)a.counts[1] * 64u;
+  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total; idx += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t t = a.gtiles[idx >> 6];
+    const uint32_t l = (uint32_t)(idx & 63u);
+    const Node4<T> nd = grid[(size_t)t * 64u + l];
+    if (nd.m == (T)0) continue;
+    const Geom<T, D> g = tile_geom(a, t);
+    int org[D];
+    tile_origin<T, D>(t, g, org);
+    uint32_t r = l;
+    double x[D];
+#pragma unroll
+    for (int c = D - 1; c >= 0; c--) {
+      const int nc = org[c] + (int)(r % Tile<D>::E);
+      r /= Tile<D>::E;
+      x[c] = (double)(g.lo[c] + ((T)(nc - 3) + (T)0.5) * g.dx[c]);  // GRID::Center in T
+    }
+    const double m = (double)nd.m;
+    const T comp[3] = {nd.x, nd.y, nd.z};
+    double p[D], vv = 0;
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      p[c] = MOMENTUM ? (double)comp[c] : m * (double)comp[c];
+      vv += MOMENTUM ? p[c] * p[c] / m : m * (double)comp[c] * (double)comp[c];
+    }
+    double cr[3];
+    cross_d<D>(x, p, cr);
+    acc[0] += m;
+#pragma unroll
+    for (int c = 0; c < D; c++) acc[1 + c] += p[c];
+#pragma unroll
+    for (int c = 0; c < (D == 3 ? 3 : 1); c++) acc[4 + c] += cr[c];
+    acc[7] += 0.5 * vv;
+    acc[11] += 1;
+  }
+  totals_block_partial(acc, partial);
+}
+
+// Sums the block partials in block order (one thread per component) into
+// out[0..NTOT), potential = elastic + gravity.
+__global__ void totals_finish_kernel(const double* partial, int nblocks, double* out) {
+  __shared__ double s[NTOT];
+  const int c = threadIdx.x;
+  if (c < NTOT) {
+    double t = 0;
+    for (int b = 0; b < nblocks; b++) t += partial[b * NTOT + c];
+    s[c] = t;
+  }
+  __syncthreads();
+  if (c < NTOT) out[c] = c == 8 ? s[9] + s[10] : s[c];
 }
 
 // ------------------------------------------------------- adaptive dt ----
@@ -3029,8 +3147,9 @@ class Engine final : public EngineBase {
     HIPCHECK(hipMemsetAsync(counts_.p, 0, 16 * sizeof(uint32_t), stream_));
     HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
     stats_blocks_ = (int)std::min<uint64_t>(std::max<uint64_t>(cdiv(nt * 64, BLOCK), 1), 2048);
-    stats_partial_.alloc((size_t)stats_blocks_ * 5 + 8);
-    stats_out_.alloc(8);
+    stats_partial_.alloc((size_t)TOTALS_BLOCKS * NTOT);
+    stats_out_.alloc(2 * NTOT);
+    HIPCHECK(hipMemsetAsync(stats_out_.p, 0, 2 * NTOT * sizeof(double), stream_));
     scenes_dev_.alloc(scenes_.size());
     HIPCHECK(hipMemcpyAsync(scenes_dev_.p, scenes_.data(), scenes_.size() * sizeof(Geom<T, D>), hipMemcpyHostToDevice,
                             stream_));
@@ -3225,31 +3344,65 @@ class Engine final : public EngineBase {
 
   void enable_grid_stats(bool on) override { stats_on_ = on; }
   void grid_stats(double* mass, double* momentum) override {
-    if (!stats_on_ || !has_grid_) throw Error{MPMX_ESTATE, "grid stats need enable_grid_stats and a step"};
-    double out[4];
-    HIPCHECK(hipStreamSynchronize(stream_));
-    HIPCHECK(hipMemcpy(out, stats_out_.p, sizeof(out), hipMemcpyDeviceToHost));
+    double out[NTOT];
+    grid_totals(1, out);
     if (mass) *mass = out[0];
     if (momentum)
       for (int a = 0; a < D; a++) momentum[a] = out[1 + a];
   }
-  void particle_stats(double* mass, double* momentum, int64_t* valid_count) override {
-    double out[5] = {0, 0, 0, 0, 0};
-    if (n_ > 0) {
-      const int nb = (int)std::min<uint32_t>(cdiv(n_, BLOCK), 1024);
-      DevBuf<double> part;
-      part.alloc((size_t)nb * 5);
-      particle_stats_kernel<T, D><<<nb, BLOCK, 0, stream_>>>(cur_.V, cur_.mass, cur_.valid, n_, part.p);
-      std::vector<double> h((size_t)nb * 5);
-      HIPCHECK(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
-      HIPCHECK(hipStreamSynchronize(stream_));
-      for (int b = 0; b < nb; b++)
-        for (int c = 0; c < 5; c++) out[c] += h[(size_t)b * 5 + c];
+  // Grid totals of the last step (after P2G: which = 0; after the grid
+  // update: which = 1), recorded when grid stats are enabled.
+  void grid_totals(int which, double* out) override {
+    if (!stats_on_ || !has_grid_) throw Error{MPMX_ESTATE, "grid totals need enable_grid_stats and a step"};
+    if (which != 0 && which != 1) throw Error{MPMX_EINVAL, "which must be 0 (after P2G) or 1 (after the grid update)"};
+    HIPCHECK(hipStreamSynchronize(stream_));
+    HIPCHECK(hipMemcpy(out, stats_out_.p + which * NTOT, NTOT * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  // Grid totals pass over the current tile list into stats_out_[which].
+  void record_grid_totals(const Args<T, D>& a, const Node4<T>* grid, int which) {
+    if (which == 0)
+      grid_totals_kernel<T, D, true><<<TOTALS_BLOCKS, BLOCK, 0, stream_>>>(a, grid, stats_partial_.p);
+    else
+      grid_totals_kernel<T, D, false><<<TOTALS_BLOCKS, BLOCK, 0, stream_>>>(a, grid, stats_partial_.p);
+    totals_finish_kernel<<<1, 64, 0, stream_>>>(stats_partial_.p, TOTALS_BLOCKS, stats_out_.p + which * NTOT);
+  }
+  TotalsIn<T, D> totals_in() const {
+    TotalsIn<T, D> in;
+    in.X = cur_.X; in.V = cur_.V; in.F = cur_.F; in.B = cur_.B;
+    in.mass = cur_.mass; in.vol = cur_.vol; in.mu = cur_.mu; in.la = cur_.la;
+    in.valid = cur_.valid;
+    in.pscene = scenes_.size() > 1 ? cur_.pscene : nullptr;
+    in.scenes = scenes_dev_.p;
+    in.n = n_;
+    in.model = model_;
+    for (int c = 0; c < 3; c++) in.grav[c] = c < D ? (double)grav_[c] : 0.0;
+    return in;
+  }
+  // Particle totals of the current state into the device array out[NTOT]
+  // (enqueued on the context stream), block partials in `partial`.
+  void enqueue_particle_totals(double* partial, double* out) {
+    if (n_ == 0) {
+      fill_u32_kernel<<<1, BLOCK, 0, stream_>>>((uint32_t*)out, 2 * NTOT, 0u);
+    } else {
+      particle_totals_kernel<T, D><<<TOTALS_BLOCKS, BLOCK, 0, stream_>>>(totals_in(), partial);
+      totals_finish_kernel<<<1, 64, 0, stream_>>>(partial, TOTALS_BLOCKS, out);
     }
+    HIPCHECK(hipGetLastError());
+  }
+  void particle_totals(double* out) override {
+    ptot_partial_.alloc((size_t)TOTALS_BLOCKS * NTOT + NTOT);
+    enqueue_particle_totals(ptot_partial_.p, ptot_partial_.p + (size_t)TOTALS_BLOCKS * NTOT);
+    HIPCHECK(hipMemcpyAsync(out, ptot_partial_.p + (size_t)TOTALS_BLOCKS * NTOT, NTOT * sizeof(double),
+                            hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+  void particle_stats(double* mass, double* momentum, int64_t* valid_count) override {
+    double out[NTOT];
+    particle_totals(out);
     if (mass) *mass = out[0];
     if (momentum)
       for (int a = 0; a < D; a++) momentum[a] = out[1 + a];
-    if (valid_count) *valid_count = (int64_t)out[4];
+    if (valid_count) *valid_count = (int64_t)out[11];
   }
 
   // ---- adaptive dt (MPM_DRIVER.cpp:1098-1138, :173-197) ----
@@ -3964,7 +4117,6 @@ class Engine final : public EngineBase {
     a.gridf = gridf_.p;
     a.gridu = gridu_override_;
     a.err = err_.p;
-    a.stats_partial = nullptr;
     a.band = refl_on_;
     a.reflect_sides = 0;
     if (refl_on_)
@@ -4067,6 +4219,7 @@ class Engine final : public EngineBase {
     }
 
     hipEvent_t e2 = timer_.begin(stream_);
+    if (stats_on_) record_grid_totals(a, a.grid, 0);
     Args<T, D> ag = a;  // the grid update's arguments
     if (refl_on_ && !refl_full_) {
       // gravity into the P2G sums, then Reflect_Boundary_Mass_Momentum with
@@ -4078,12 +4231,10 @@ class Engine final : public EngineBase {
                                                                                        reflect_axis(k));
       for (int c = 0; c < D; c++) ag.grav[c] = 0;
     }
-    ag.stats_partial = stats_on_ ? stats_partial_.p : nullptr;
     if (fric)
       grid_kernel<T, D, true><<<grid_blocks(), BLOCK, 0, stream_>>>(ag);
     else
       grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(ag);
-    if (stats_on_) reduce_partials_kernel<<<1, 64, 0, stream_>>>(stats_partial_.p, grid_blocks(), stats_out_.p);
     if (refl_on_ && refl_full_) {  // Reflect_Boundary_Friction after the collision objects (MPM_DRIVER.cpp:999-1019)
       for (int k = 0; k < D; k++)
         reflect_friction_kernel<T, D><<<reflect_blocks(k), BLOCK, 0, stream_>>>(a.grid, scenes_dev_.p,
@@ -4093,6 +4244,7 @@ class Engine final : public EngineBase {
         reflect_velocity_kernel<T, D><<<reflect_blocks(k), BLOCK, 0, stream_>>>(a.grid, fric ? a.gridf : nullptr,
                                                                                   scenes_dev_.p, reflect_axis(k));
     }
+    if (stats_on_) record_grid_totals(a, a.grid, 1);
     timer_.end(PH_GRID, e2, stream_);
     if (limits_on()) {
       hipEvent_t e4 = timer_.begin(stream_);
@@ -4197,7 +4349,6 @@ class Engine final : public EngineBase {
     Args<T, D> b = a;
     b.dt = 0;
     b.grid = vsave_.p;
-    b.stats_partial = nullptr;
     p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(b);
     p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(b);
     normalize_nodes_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(vsave_.p, a.gtiles, counts_.p + 1);
@@ -4295,13 +4446,12 @@ class Engine final : public EngineBase {
     a.next_stamp = ms_next;
 
     hipEvent_t e2 = timer_.begin(stream_);
-    a.stats_partial = stats_on_ ? stats_partial_.p : nullptr;
+    if (stats_on_) record_grid_totals(a, a.grid, 0);
     if (fric)
       grid_kernel<T, D, true><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
     else
       grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
-    if (stats_on_) reduce_partials_kernel<<<1, 64, 0, stream_>>>(stats_partial_.p, grid_blocks(), stats_out_.p);
-    a.stats_partial = nullptr;
+    if (stats_on_) record_grid_totals(a, a.grid, 1);
     clear_tiles_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(gother_, gt_other_, lastcnt_.p + 1);
     timer_.end(PH_GRID, e2, stream_);
 
@@ -4458,7 +4608,8 @@ class Engine final : public EngineBase {
   bool fused_ = false;       // gcur_ holds the P2G of the current state with dt fused_dt_
   bool fused_last_ = false;  // gother_ holds the last step's grid (for mpmx_get_grid)
   T fused_dt_ = 0;
-  DevBuf<double> stats_partial_, stats_out_;
+  DevBuf<double> stats_partial_, stats_out_;  // grid totals: block partials, [after P2G, after update]
+  DevBuf<double> ptot_partial_;                // particle totals: block partials, then the result
   DevBuf<T> stage_;
   DevBuf<uint32_t> slow_list_, mover_list_, mover_tile_;
   PhaseTimer timer_;

@@ -16,6 +16,8 @@ FP32, FP64 = 4, 8
 FIXED_COROTATED, NEO_HOOKEAN, HENCKY = 0, 1, 2
 STICK, SLIP, SEPARATE = 0, 1, 2
 SHAPE_BOX, SHAPE_SPHERE = 0, 1
+# mpmx_totals: mass, linear[3], angular[3], kinetic, potential, elastic, gravity, count
+TOTALS_LEN = 12
 
 # Every exported symbol of the C ABI, checked by the CPU test suite.
 EXPORTS = (
@@ -23,7 +25,8 @@ EXPORTS = (
     "mpmx_set_batch_particles", "mpmx_set_particles", "mpmx_get_particles", "mpmx_get_material",
     "mpmx_particle_count", "mpmx_set_constitutive", "mpmx_set_gravity", "mpmx_add_collision_box",
     "mpmx_clear_collision_objects", "mpmx_step", "mpmx_step_async", "mpmx_synchronize", "mpmx_time",
-    "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_grid_node_count",
+    "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_particle_totals",
+    "mpmx_grid_totals", "mpmx_grid_node_count",
     "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_set_rebin_policy", "mpmx_bin_stats",
     "mpmx_grid_v_upper_bound", "mpmx_compute_dt", "mpmx_advance_frames",
     "mpmx_set_dt_limits", "mpmx_max_sound_speed", "mpmx_last_dt", "mpmx_set_reflection",
@@ -97,6 +100,8 @@ def lib():
         L.mpmx_enable_grid_stats.argtypes = [vp, c_int]
         L.mpmx_grid_stats.argtypes = [vp, vp, vp]
         L.mpmx_particle_stats.argtypes = [vp, vp, vp, vp]
+        L.mpmx_particle_totals.argtypes = [vp, vp]
+        L.mpmx_grid_totals.argtypes = [vp, c_int, vp]
         L.mpmx_grid_node_count.argtypes = [vp]
         L.mpmx_grid_node_count.restype = i64
         L.mpmx_get_grid.argtypes = [vp, vp, vp]
@@ -329,6 +334,20 @@ class Context:
         c = np.zeros(1, np.int64)
         _check(lib().mpmx_particle_stats(self._h, _p(m), _p(p), _p(c)))
         return float(m[0]), p[: self.dim].copy(), int(c[0])
+
+    def particle_totals(self):
+        """FP64 particle totals (mpmx_totals as 12 doubles: mass, linear[3],
+        angular[3], kinetic, potential, elastic, gravity, count)."""
+        out = np.zeros(TOTALS_LEN)
+        _check(lib().mpmx_particle_totals(self._h, _p(out)))
+        return out
+
+    def grid_totals(self, which):
+        """FP64 grid totals of the last step: which 0 after P2G, 1 after the grid
+        update (needs enable_grid_stats)."""
+        out = np.zeros(TOTALS_LEN)
+        _check(lib().mpmx_grid_totals(self._h, int(which), _p(out)))
+        return out
 
     def get_grid(self):
         nn = lib().mpmx_grid_node_count(self._h)

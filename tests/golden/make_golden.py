@@ -3,7 +3,7 @@
 Runs the reference harness (oracle/_ref/ref_harness, built from the reference
 checkout by oracle/ref/Makefile) on small seeded scenes and component inputs,
 and stores inputs and outputs as compressed .npz fixtures.  Run in the
-authoring container:  python tests/golden/make_golden.py [steps] [components] [frames] [plastic]
+authoring container:  python tests/golden/make_golden.py [steps] [components] [frames] [plastic] [totals] ...
 (no argument: every group).
 """
 import json
@@ -314,10 +314,52 @@ def write_plastic():
     np.savez_compressed(os.path.join(HERE, "plastic.npz"), **out)
 
 
+def totals_cases():
+    """Scenes for the conservation totals (angular momentum and energies).
+
+    Each entry: scene, steps."""
+    cases = {}
+    # the reference's rotating sphere (FP64, fixed corotated) and rotating
+    # circle (neo-Hookean), no gravity: angular momentum is conserved
+    cases["rot3d_coro_f64"] = (mt.make_rotating(3, 16, prec=8, model=mt.COROTATED, omega=[0.4, 0.3, -0.2],
+                                                dt=2e-3, seed=41), 6)
+    cases["rot2d_neo_f64"] = (mt.make_rotating(2, 32, prec=8, model=mt.NEO_HOOKEAN, omega=[3.0], dt=1e-3,
+                                               seed=42), 6)
+    # FP32 with gravity, a sticky floor, perturbed F and the Hencky model on
+    # an FP64 scene: the potential energy terms of every model
+    walls = mt.wall_boxes(3, [16] * 3, [0] * 3, [1] * 3, flags=4, inset=0.12, type_=mt.STICK, friction=0.0)
+    sc = mt.make_rotating(3, 16, prec=4, model=mt.NEO_HOOKEAN, omega=[0.0, 2.0, 1.0], dt=1e-3, seed=43,
+                          gravity=[0.0, -9.8, 0.0], walls=walls)
+    sc["F"] = (sc["F"] + 0.1 * (np.random.default_rng(43).random(sc["F"].shape) - 0.5)).astype(np.float32).astype(
+        np.float64)
+    cases["rot3d_neo_floor_f32"] = (sc, 4)
+    sc = mt.make_rotating(2, 32, prec=8, model=2, omega=[1.0], dt=1e-3, seed=44, gravity=[0.0, -9.8])
+    sc["F"] = sc["F"] + 0.05 * (np.random.default_rng(44).random(sc["F"].shape) - 0.5)
+    cases["rot2d_hencky_f64"] = (sc, 3)
+    return cases
+
+
+def write_totals():
+    out = {}
+    for name, (sc, steps) in totals_cases().items():
+        for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+            out[f"{name}_in_{k}"] = np.asarray(sc[k])
+        meta = {k: sc[k] for k in ("dim", "prec", "model", "N", "lo", "hi", "gravity", "dt", "neo_threshold")}
+        meta["steps"] = steps
+        out[f"{name}_meta"] = np.array(json.dumps(meta, default=float))
+        out[f"{name}_walls"] = np.array(json.dumps(sc["walls"], default=float))
+        out[f"{name}_totals"] = scene_io.run_ref_totals(sc, steps)
+        print("totals", name, sc["X"].shape[0], "particles,", steps, "steps")
+    np.savez_compressed(os.path.join(HERE, "totals.npz"), **out)
+
+
 def main():
     if not scene_io.ref_available():
         sys.exit("reference harness missing: build it with oracle/ref/Makefile")
-    groups = set(sys.argv[1:]) or {"steps", "components", "frames", "plastic", "limits", "io", "reflect", "poisson"}
+    groups = set(sys.argv[1:]) or {"steps", "components", "frames", "plastic", "limits", "io", "reflect", "poisson",
+                                   "totals"}
+    if "totals" in groups:
+        write_totals()
     if "frames" in groups:
         write_frames()
     if "plastic" in groups:

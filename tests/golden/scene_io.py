@@ -125,6 +125,21 @@ def run_ref(sc, nsteps, dump_every=None, threads=1):
         return read_output(op, sc["dim"], plastic=plastic_kind(sc))
 
 
+def run_ref_totals(sc, nsteps, threads=1):
+    """Conservation totals of the reference over ``nsteps`` steps of scene
+    ``sc`` (harness mode "totals"): an array of (3 nsteps + 1) records of 11
+    values -- per step the particles after "update-weights", the grid after
+    "p2g" and after "forces"; then the particles at the end."""
+    with tempfile.TemporaryDirectory() as td:
+        sp = os.path.join(td, "scene.bin")
+        op = os.path.join(td, "out.bin")
+        write_scene(sp, sc, nsteps, 0, threads)
+        subprocess.run([REF_HARNESS, "totals", sp, op], check=True, stdout=subprocess.DEVNULL)
+        with open(op, "rb") as f:
+            (nrec,) = struct.unpack("<i", f.read(4))
+            return np.frombuffer(f.read(8 * 11 * nrec), dtype=np.float64).reshape(nrec, 11).copy()
+
+
 def _limit_args(lim):
     return [str(1 if lim.get("strong_cfl") else 0), str(1 if lim.get("sound_cfl") else 0),
             repr(float(lim.get("cfl_F", 0.5))), repr(float(lim.get("cfl_sound", 0.9)))]

@@ -98,6 +98,43 @@ def make_scene(dim, N, prec=8, model=COROTATED, shape=None, ppc_axis=2, seed=123
     return sc
 
 
+def make_rotating(dim, N, prec=8, model=COROTATED, omega=None, center=None, radius=0.3, E=1e3, nu=0.3,
+                  rho=2000.0, seed=1234, dt=1e-3, gravity=None, walls=None):
+    """A rotating ball, as the reference's rotating-sphere and rotating-circle
+    tests seed it (Projects/mpm/STANDARD_TESTS_3D.cpp:78-91, STANDARD_TESTS_2D.cpp:
+    84-99): v = omega x (x - c) with the mean momentum removed, B = (omega x) D_p
+    with D_p = dx^2/4 I (STANDARD_TESTS_BASE.cpp:300-303), jittered seeding, no
+    gravity unless given."""
+    if center is None:
+        center = [0.5] * dim
+    if omega is None:
+        omega = [0.4] if dim == 2 else [0.4, 0.0, 0.0]
+    c = np.asarray(center, float)
+    ball = lambda X: ((X - c) ** 2).sum(1) < radius ** 2
+    sc = make_scene(dim, N, prec=8, model=model, shape=ball, E=E, nu=nu, rho=rho, seed=seed, dt=dt,
+                    gravity=gravity if gravity is not None else [0.0] * dim, walls=walls)
+    X = sc["X"]
+    if dim == 2:
+        w = float(omega[0])
+        V = np.stack([-w * (X[:, 1] - c[1]), w * (X[:, 0] - c[0])], 1)
+        W = np.array([[0.0, -w], [w, 0.0]])
+    else:
+        w = np.asarray(omega, float)
+        V = np.cross(w, X - c)
+        W = np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+    m = sc["mass"]
+    V -= (m[:, None] * V).sum(0) / m.sum()
+    dx = (np.asarray(sc["hi"]) - np.asarray(sc["lo"])) / np.asarray(sc["N"], float)
+    Bm = W * (dx ** 2 / 4.0)[None, :]  # (omega x) D_p, column j scaled by dx_j^2/4
+    sc["V"] = V
+    sc["B"] = np.tile(Bm.T.reshape(-1), (X.shape[0], 1))  # column-major per particle
+    sc["prec"] = prec
+    if prec == 4:
+        for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda"):
+            sc[k] = sc[k].astype(np.float32).astype(np.float64)
+    return sc
+
+
 # ---------------------------------------------------------------- oracle ----
 _lib = None
 
@@ -155,6 +192,35 @@ def oracle_run(sc, nsteps, prec=None, want_grid=True):
     if want_grid:
         out.update(grid_mass=gm, grid_velocity=gv, grid_velocity_save=gs, grid_velocity_friction_save=gf)
     return out
+
+
+def oracle_totals(sc, nsteps, prec=None):
+    """Conservation totals of ``nsteps`` oracle steps (oracle_run_totals_*): an
+    array of (3 nsteps + 1) records of 11 values -- per step the particles at
+    its start, the grid after P2G and after the forces; then the particles at
+    the end.  Record layout: mass, linear momentum [1:4], angular momentum
+    [4:7] (2D: [4]), kinetic energy [7], potential energy [8] = elastic [9] +
+    gravity [10]."""
+    prec = prec or sc["prec"]
+    dt_np = np.float64 if prec == 8 else np.float32
+    d = sc["dim"]
+    n = sc["X"].shape[0]
+    st = {k: np.ascontiguousarray(sc[k], dtype=dt_np).copy() for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda")}
+    valid = np.ascontiguousarray(sc["valid"], np.uint8).copy()
+    N = np.array(list(sc["N"]) + [1] * (3 - d), np.int32)
+    lo = np.array(list(sc["lo"]) + [0] * (3 - d), np.float64)
+    hi = np.array(list(sc["hi"]) + [0] * (3 - d), np.float64)
+    g = np.array(list(sc["gravity"]) + [0] * (3 - d), np.float64)
+    out = np.zeros((3 * nsteps + 1, 11), dt_np)
+    nw, wt, wf, wl, wh = _walls_arrays(sc)
+    fn = oracle().oracle_run_totals_f64 if prec == 8 else oracle().oracle_run_totals_f32
+    rc = fn(ctypes.c_int(d), _ptr(N), _ptr(lo), _ptr(hi), _ptr(g), ctypes.c_int(sc["model"]),
+            ctypes.c_double(sc.get("neo_threshold", 0.25)), ctypes.c_double(sc["dt"]),
+            ctypes.c_int(nw), _ptr(wt), _ptr(wf), _ptr(wl), _ptr(wh), ctypes.c_longlong(n),
+            _ptr(st["X"]), _ptr(st["V"]), _ptr(st["F"]), _ptr(st["B"]), _ptr(st["mass"]), _ptr(st["volume"]),
+            _ptr(st["mu"]), _ptr(st["lambda"]), _ptr(valid), ctypes.c_int(nsteps), _ptr(out))
+    assert rc == 0
+    return out.astype(np.float64)
 
 
 def oracle_run_plastic(sc, nsteps, prec=None):

@@ -488,3 +488,75 @@ def test_reflection_keeps_particles_that_free_sides_lose():
     assert kept["valid"].sum() > lost["valid"].sum()
     X = kept["X"].astype(np.float64)
     assert np.all(X[kept["valid"] == 1] >= 0) and np.all(X[kept["valid"] == 1] <= 1)
+
+
+# ------------------------------------------------------ conservation totals ----
+TOTALS = ["rot3d_coro_f64", "rot2d_neo_f64", "rot3d_neo_floor_f32", "rot2d_hencky_f64"]
+
+
+def totals_scene(z, name):
+    meta = json.loads(str(z[name + "_meta"]))
+    sc = dict(meta)
+    sc["walls"] = json.loads(str(z[name + "_walls"]))
+    for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+        sc[k] = z[f"{name}_in_{k}"]
+    return sc, meta["steps"]
+
+
+@pytest.mark.parametrize("name", TOTALS)
+def test_oracle_totals_match_reference(name):
+    """Linear and angular momentum, kinetic and potential energy of particles
+    and grid, as the reference's own MPM_EXAMPLE::Total_* and Potential_Energy
+    report them around each step (recorded through its named-step callbacks),
+    bit for bit."""
+    z = load("totals.npz")
+    sc, steps = totals_scene(z, name)
+    ref = z[name + "_totals"]
+    got = mt.oracle_totals(sc, steps)
+    assert ref.shape == got.shape == (3 * steps + 1, 11)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_totals_fixtures_exercise_every_term():
+    """The fixtures hold nonzero angular momentum, a B term, elastic energy of
+    three models and gravity energy, so a dropped term changes a value."""
+    z = load("totals.npz")
+    for name in TOTALS:
+        sc, _ = totals_scene(z, name)
+        ref = z[name + "_totals"]
+        assert np.abs(sc["B"]).max() > 0
+        assert np.abs(ref[:, 4:7]).max() > 0
+        assert ref[-1, 9] > 0, name  # elastic energy after the body deformed
+    assert {totals_scene(z, n)[0]["model"] for n in TOTALS} == {0, 1, 2}
+    assert abs(z["rot3d_neo_floor_f32_totals"][0, 10]) > 0 and abs(z["rot2d_hencky_f64_totals"][0, 10]) > 0
+
+
+@pytest.mark.parametrize("dim,model,omega,N", [(3, mt.COROTATED, [0.4, 0.3, -0.2], 16), (2, mt.NEO_HOOKEAN, [3.0], 32),
+                                              (3, mt.NEO_HOOKEAN, [2.0, -1.0, 5.0], 16)])
+def test_apic_conserves_angular_momentum(dim, model, omega, N):
+    """APIC's defining property: with no gravity and no walls, the angular
+    momentum of the particles (with the -m B:eps term), of the grid after P2G
+    and of the grid after the forces stays the same through every step (FP64
+    rounding), and so do mass and linear momentum.  Dropping the B term, or
+    using another node location or B update, breaks it at the 1e-4 level."""
+    sc = mt.make_rotating(dim, N, prec=8, model=model, omega=omega, dt=2e-3, seed=41)
+    o = mt.oracle_totals(sc, 20)
+    L = np.abs(o[0, 4:7]).max()
+    assert L > 0
+    assert np.abs(o[:, 4:7] - o[0, 4:7]).max() <= 1e-13 * L
+    assert np.abs(o[:, 0] - o[0, 0]).max() <= 1e-13 * o[0, 0]
+    assert np.abs(o[:, 1:4]).max() <= 1e-13 * o[0, 0] * np.abs(sc["V"]).max()
+    # without the -m B:eps term the particle total is off by far more
+    m = sc["mass"]
+    X, V = sc["X"], sc["V"]
+    naive = (m[:, None] * (np.cross(X, V) if dim == 3 else (X[:, 0] * V[:, 1] - X[:, 1] * V[:, 0])[:, None])).sum(0)
+    assert np.abs(naive[: (3 if dim == 3 else 1)] - o[0, 4:4 + (3 if dim == 3 else 1)]).max() > 1e-4 * L
+
+
+def test_rigid_rotation_kinetic_energy_is_transferred_exactly():
+    """A rigid rotation is an affine velocity field, which APIC transfers
+    exactly: the particle kinetic energy with its B term equals the grid's
+    after P2G."""
+    sc = mt.make_rotating(3, 16, prec=8, model=mt.COROTATED, omega=[0.4, 0.3, -0.2], dt=2e-3, seed=41)
+    o = mt.oracle_totals(sc, 1)
+    assert abs(o[0, 7] - o[1, 7]) <= 1e-13 * o[0, 7]
