@@ -51,13 +51,76 @@ def headline_boxes():
             for i in range(len(lo))]
 
 
-def floor_wall():
-    # STANDARD_TESTS_BASE::Add_Walls geometry for the -y side: the domain
-    # thickened by 7 cells, cut at y = 0.05.
-    h = 1.0 / CELLS
-    lo = [-7 * h, -7 * h, -7 * h]
-    hi = [1 + 7 * h, 0.05, 1 + 7 * h]
+def wall_box(cells, axis, low_side, inset):
+    """STANDARD_TESTS_BASE::Add_Walls geometry on [0,1]^3 (Projects/mpm/
+    STANDARD_TESTS_BASE.cpp:487-511): the domain thickened by 7 cells, cut at
+    `inset` inside the low or high side of `axis`."""
+    h = 1.0 / cells
+    lo, hi = [-7 * h] * 3, [1 + 7 * h] * 3
+    if low_side:
+        hi[axis] = inset
+    else:
+        lo[axis] = 1.0 - inset
     return lo, hi
+
+
+def floor_wall():
+    # the -y side cut at y = 0.05
+    return wall_box(CELLS, 1, True, 0.05)
+
+
+# The 3D FP32 neo-Hookean sphere: 128^3 cells, a sphere of radius 0.25 at
+# (0.5, 0.6, 0.5), 8 jittered particles per cell (seed 1234), E 5e4, nu 0.3,
+# failure threshold 0.25, density 1000, one velocity for the whole sphere
+# drawn uniform in [-0.5, 0.5]^3 (numpy MT19937, seed 1234), gravity, slip
+# walls on all six sides inset 3 cells, dt 5e-5, 500 steps.
+SPHERE_CELLS = 128
+SPHERE_DT = 5e-5
+SPHERE_STEPS = 500
+
+
+def sphere_scene(precision=mpmx.FP32):
+    """Particles (dict of arrays, material per particle) and wall boxes of the
+    neo-Hookean sphere workload."""
+    v = np.random.RandomState(SEED).uniform(-0.5, 0.5, 3)
+    sph = [dict(type=mpmx.SHAPE_SPHERE, a=[0.5, 0.6, 0.5], b=[0.25, 0.0, 0.0], density=1000.0, E=5e4, nu=0.3,
+                v=list(v))]
+    P = mpmx.seed_particles(3, SPHERE_CELLS, [0.0] * 3, [1.0] * 3, sph, ppc_axis=2, seed=SEED, precision=precision)
+    inset = 3.0 / SPHERE_CELLS
+    walls = [wall_box(SPHERE_CELLS, a, s, inset) for a in range(3) for s in (True, False)]
+    return P, walls
+
+
+def sphere_context(P, walls, precision=mpmx.FP32):
+    ctx = mpmx.Context(3, SPHERE_CELLS, [0.0] * 3, [1.0] * 3, precision=precision)
+    ctx.set_constitutive(mpmx.NEO_HOOKEAN, 0.25)
+    ctx.set_gravity([0.0, -9.8, 0.0])
+    for lo, hi in walls:
+        ctx.add_collision_box(mpmx.SLIP, 0.0, lo, hi)
+    ctx.set_particles(P["X"], P["V"], P["F"], P["B"], P["mass"], P["volume"], P["mu"], P["lam"], P["valid"])
+    return ctx
+
+
+def sphere_throughput(log, steps=SPHERE_STEPS, warmup=10):
+    """The neo-Hookean sphere at full size: time of `steps` steps after
+    `warmup` untimed ones, inputs resident in HBM."""
+    P, walls = sphere_scene()
+    n = int(P["X"].shape[0])
+    ctx = sphere_context(P, walls)
+    del P
+    ctx.step_async(SPHERE_DT, warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step_async(SPHERE_DT, steps)
+    ctx.synchronize()
+    sec = time.perf_counter() - t0
+    _, _, nvalid = ctx.particle_stats()
+    ctx.close()
+    log(f"sphere: {n} particles, {steps} steps in {sec * 1e3:.1f} ms")
+    return {"value": n * steps / sec, "unit": "particle-steps/s", "ms_per_step": sec * 1e3 / steps, "particles": n,
+            "valid_particles_after": nvalid, "steps": steps,
+            "workload": "neo-Hookean sphere r 0.25 at (0.5, 0.6, 0.5), 128^3 grid, 8 ppc, E 5e4, nu 0.3, "
+                        "one random velocity in [-0.5, 0.5]^3, gravity, slip walls inset 3 cells, dt 5e-5, FP32"}
 
 
 def make_context(P):
@@ -70,61 +133,132 @@ def make_context(P):
     return ctx
 
 
-def cpu_baseline(shapes, log):
-    """Reference CPU step (the PhysBAM MPM_DRIVER built under oracle/_ref) on a
-    bounded sample of the same workload: the first box of the scene alone at
-    the same resolution, particles per cell, materials, floor and dt."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    threads = max(1, min(threads, 16))
-    P = mpmx.seed_particles(3, CELLS, [0.0] * 3, [1.0] * 3, shapes[:1], ppc_axis=2, seed=SEED,
-                            precision=mpmx.FP32)
-    n = int(P["X"].shape[0])
-    lo, hi = floor_wall()
-    sc = dict(dim=3, prec=4, model=0, N=[CELLS] * 3, lo=[0.0] * 3, hi=[1.0] * 3, gravity=[0.0, -9.8, 0.0], dt=DT,
-              neo_threshold=0.25, walls=[dict(type=mpmx.STICK, friction=0.0, lo=lo, hi=hi)], X=P["X"], V=P["V"],
-              F=P["F"], B=P["B"], mass=P["mass"], volume=P["volume"], mu=P["mu"], valid=P["valid"],
-              **{"lambda": P["lam"]})
-    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
-    warm, steps = 1, 3
-    if os.access(harness, os.X_OK):
-        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
-        import scene_io
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 
-        with tempfile.TemporaryDirectory() as td:
-            path = os.path.join(td, "sample.bin")
-            scene_io.write_scene(path, sc, 0, 0, threads)
-            log(f"cpu baseline: reference MPM_DRIVER, {n} particles, {threads} threads")
-            r = subprocess.run([harness, "time", path, str(threads), str(warm), str(steps)], capture_output=True,
-                               text=True, timeout=600)
-        if r.returncode != 0:
-            raise RuntimeError(f"reference harness failed: {r.stderr[-400:]}")
-        res = json.loads(r.stdout.strip().splitlines()[-1])
+
+def cpu_threads():
+    """OpenMP threads for the reference CPU step: OMP_NUM_THREADS if set, else
+    every core the process sees."""
+    return max(1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1))
+
+
+def reference_time(sc, threads, warm, steps, log=None):
+    """Wall time of `steps` MPM_DRIVER::Advance_One_Time_Step calls of the
+    reference built under oracle/_ref after `warm` untimed ones (harness mode
+    "time": output and statistics off, fixed dt); returns its JSON record."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import scene_io
+
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "scene.bin")
+        scene_io.write_scene(path, sc, 0, 0, threads)
+        r = subprocess.run([HARNESS, "time", path, str(threads), str(warm), str(steps)], capture_output=True,
+                           text=True, timeout=1800)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference harness failed: {r.stderr[-400:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def scene_dict(P, cells, model, walls, dt, gravity=(0.0, -9.8, 0.0), wall_type=mpmx.STICK, dim=3):
+    """Scene record of the oracle and the harness from seeded particles."""
+    return dict(dim=dim, prec=4 if P["X"].dtype == np.float32 else 8, model=model, N=[cells] * dim,
+                lo=[0.0] * dim, hi=[1.0] * dim, gravity=list(gravity)[:dim], dt=dt, neo_threshold=0.25,
+                walls=[dict(type=wall_type, friction=0.0, lo=lo, hi=hi) for lo, hi in walls], X=P["X"], V=P["V"],
+                F=P["F"], B=P["B"], mass=P["mass"], volume=P["volume"], mu=P["mu"], valid=P["valid"],
+                **{"lambda": P["lam"]})
+
+
+def cpu_baseline(shapes, log):
+    """Reference CPU step (the PhysBAM MPM_DRIVER built under oracle/_ref) on
+    the whole headline scene: 1 warm-up + 3 timed steps (~10-30 s of CPU work
+    at OpenMP threads = OMP_NUM_THREADS, else all cores).  Without the harness
+    (a box where the reference was not built), the oracle restatement on one
+    cube of the scene, serially."""
+    threads = cpu_threads()
+    if os.access(HARNESS, os.X_OK):
+        P = mpmx.seed_particles(3, CELLS, [0.0] * 3, [1.0] * 3, shapes, ppc_axis=2, seed=SEED,
+                                precision=mpmx.FP32)
+        n = int(P["X"].shape[0])
+        sc = scene_dict(P, CELLS, mpmx.FIXED_COROTATED, [floor_wall()], DT)
+        del P
+        warm, steps = 1, 3
+        log(f"cpu baseline: reference MPM_DRIVER, {n} particles, {threads} threads")
+        res = reference_time(sc, threads, warm, steps, log)
         return {"value": res["particle_steps_per_s"], "unit": "particle-steps/s", "cores": threads,
-                "kind": "reference",
-                "sample": f"one cube of the 12-box scene ({n} particles, 256^3 grid, fixed-corotated, sticky floor, "
+                "nproc": os.cpu_count(), "kind": "reference",
+                "sample": f"the whole 12-box scene ({n} particles, 256^3 grid, fixed-corotated, sticky floor, "
                           f"dt 2e-5), {warm} warm-up + {steps} timed MPM_DRIVER::Advance_One_Time_Step calls, "
                           f"FP32, OpenMP threads={threads}"}
-    # Fallback: the oracle restatement, serial.
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import mpmtest
 
+    P = mpmx.seed_particles(3, CELLS, [0.0] * 3, [1.0] * 3, shapes[:1], ppc_axis=2, seed=SEED,
+                            precision=mpmx.FP32)
+    n = int(P["X"].shape[0])
+    sc = scene_dict(P, CELLS, mpmx.FIXED_COROTATED, [floor_wall()], DT)
     log(f"cpu baseline: oracle restatement, {n} particles, 1 thread")
     mpmtest.oracle_run(sc, 1, prec=4, want_grid=False)
     t0 = time.perf_counter()
     mpmtest.oracle_run(sc, 1, prec=4, want_grid=False)
     sec = time.perf_counter() - t0
-    return {"value": n / sec, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+    return {"value": n / sec, "unit": "particle-steps/s", "cores": 1, "nproc": os.cpu_count(), "kind": "port",
             "sample": f"one cube of the 12-box scene ({n} particles), 1 step of the serial oracle, FP32"}
 
 
-def batched(log, nscenes=256, steps=100, seed=BOX_SEED):
-    """Batched configuration: independent 3D scenes in one context, grid N
-    uniform in {32, 48, 64, 96}, one sphere of radius U[0.1, 0.3] at the centre,
-    8 ppc, E log-U[1e3, 1e5], nu U[0.2, 0.4], fixed corotated, gravity, FP32,
-    dt 1e-4; every step advances all scenes with the same launches."""
+def cpu_table(log, threads_list):
+    """Reference CPU particle-steps/s of every configuration (harness mode
+    "time"), for each thread count in threads_list: the 2D square (FP64, 200
+    steps), the neo-Hookean sphere (2 warm-up + 20), the 12-box scene (2 + 10),
+    the 256 batched scenes (each scene 1 warm-up + 5, summed) and the
+    differentiable box's forward step (200)."""
+    rows = []
+
+    def row(name, sc, warm, steps, threads):
+        r = reference_time(sc, threads, warm, steps)
+        log(f"{name}: {r['particles']} particles, {threads} threads: {r['particle_steps_per_s']:.3e} particle-steps/s")
+        return r
+
+    for th in threads_list:
+        # 2D FP64 square: 64^2, [0.3,0.7]x[0.4,0.8], 4 ppc, fixed corotated E 1e4 nu 0.3, sticky ground y < 0.05
+        sq = [dict(type=mpmx.SHAPE_BOX, a=[0.3, 0.4, 0.0], b=[0.7, 0.8, 0.0], density=1000.0, E=1e4, nu=0.3,
+                   v=[0.0] * 3)]
+        P = mpmx.seed_particles(2, 64, [0.0] * 2, [1.0] * 2, sq, ppc_axis=2, seed=SEED, precision=mpmx.FP64)
+        h = 1.0 / 64
+        r = row("square2d", scene_dict(P, 64, mpmx.FIXED_COROTATED, [([-7 * h, -7 * h], [1 + 7 * h, 0.05])], 1e-4,
+                                       dim=2), 0, 200, th)
+        rows.append(dict(r, config="2D square, 64^2, FP64"))
+        P, walls = sphere_scene()
+        r = row("sphere", scene_dict(P, SPHERE_CELLS, mpmx.NEO_HOOKEAN, walls, SPHERE_DT, wall_type=mpmx.SLIP), 2,
+                20, th)
+        rows.append(dict(r, config="neo-Hookean sphere, 128^3, FP32"))
+        P = mpmx.seed_particles(3, CELLS, [0.0] * 3, [1.0] * 3, headline_boxes(), ppc_axis=2, seed=SEED,
+                                precision=mpmx.FP32)
+        r = row("12-box", scene_dict(P, CELLS, mpmx.FIXED_COROTATED, [floor_wall()], DT), 2, 10, th)
+        rows.append(dict(r, config="12 boxes, 256^3, FP32"))
+        del P
+        sizes, parts = batched_scenes()
+        tot_n, tot_s = 0, 0.0
+        for N, P in zip(sizes, parts):
+            rr = reference_time(scene_dict(P, N, mpmx.FIXED_COROTATED, [], 1e-4), th, 1, 5)
+            tot_n += rr["particles"] * rr["steps"]
+            tot_s += rr["seconds"]
+        log(f"batched: {len(parts)} scenes, {th} threads: {tot_n / tot_s:.3e} particle-steps/s")
+        rows.append(dict(config="256 batched scenes, 32^3..96^3, FP32", threads=th, particles=sum(
+            int(P["X"].shape[0]) for P in parts), steps=5, seconds=tot_s, particle_steps_per_s=tot_n / tot_s))
+        box = [dict(type=mpmx.SHAPE_BOX, a=[0.35] * 3, b=[0.65] * 3, density=1000.0, E=1e4, nu=0.3, v=[0.0] * 3)]
+        P = mpmx.seed_particles(3, 64, [0.0] * 3, [1.0] * 3, box, ppc_axis=2, seed=SEED, precision=mpmx.FP32)
+        r = row("diff box", scene_dict(P, 64, mpmx.FIXED_COROTATED, [], 1e-4), 0, 200, th)
+        rows.append(dict(r, config="differentiable box forward, 64^3, FP32"))
+    return rows
+
+
+def batched_scenes(nscenes=256, seed=BOX_SEED):
+    """The batched workload's scenes: grid N uniform in {32, 48, 64, 96}, one
+    sphere of radius U[0.1, 0.3] at the centre, 8 ppc (seed 1234 + scene),
+    E log-U[1e3, 1e5], nu U[0.2, 0.4]; returns the grid sizes and the
+    per-scene particle dicts."""
     rng = np.random.default_rng(seed)
-    grids, parts = [], []
-    t0 = time.time()
+    sizes, parts = [], []
     for s in range(nscenes):
         N = int(rng.choice([32, 48, 64, 96]))
         r = float(rng.uniform(0.1, 0.3))
@@ -132,9 +266,19 @@ def batched(log, nscenes=256, steps=100, seed=BOX_SEED):
         nu = float(rng.uniform(0.2, 0.4))
         sph = [dict(type=mpmx.SHAPE_SPHERE, a=[0.5, 0.5, 0.5], b=[r, 0.0, 0.0], density=1000.0, E=E, nu=nu,
                     v=[0.0] * 3)]
-        grids.append((N, [0.0] * 3, [1.0] * 3))
+        sizes.append(N)
         parts.append(mpmx.seed_particles(3, N, [0.0] * 3, [1.0] * 3, sph, ppc_axis=2, seed=SEED + s,
                                          precision=mpmx.FP32))
+    return sizes, parts
+
+
+def batched(log, nscenes=256, steps=100, seed=BOX_SEED):
+    """Batched configuration: independent 3D scenes in one context (see
+    batched_scenes), fixed corotated, gravity, FP32, dt 1e-4; every step
+    advances all scenes with the same launches."""
+    t0 = time.time()
+    sizes, parts = batched_scenes(nscenes, seed)
+    grids = [(N, [0.0] * 3, [1.0] * 3) for N in sizes]
     counts = [int(P["X"].shape[0]) for P in parts]
     n = sum(counts)
     cat = lambda k: np.concatenate([P[k] for P in parts])
@@ -231,12 +375,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the reference CPU timing")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint / forward ratio")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes throughput")
+    ap.add_argument("--no-sphere", action="store_true", help="skip the neo-Hookean sphere (128^3) throughput")
     ap.add_argument("--reorder-interval", type=int, default=16)
     ap.add_argument("--rebin-fraction", type=float, default=1.0 / 128,
                     help="rebuild the bins when more than this fraction of particles left their binned tile")
     ap.add_argument("--rebin-interval", type=int, default=0, help="also rebuild the bins every K steps (0: never)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the forward+adjoint / forward measurement and print its JSON")
+    ap.add_argument("--cpu-table", type=str, default="", metavar="THREADS",
+                    help="only time the reference CPU step of every configuration at these comma-separated "
+                         "OpenMP thread counts and print the rows as JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -252,6 +400,10 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         log(f"copy probe: 3 copies of {args.copy_probe} MiB")
+        return
+
+    if args.cpu_table:
+        print(json.dumps(cpu_table(log, [int(t) for t in args.cpu_table.split(",")])), flush=True)
         return
 
     if args.adjoint_only:
@@ -348,6 +500,11 @@ def main():
         "adjoint_ratio": None,
         "cpu_baseline": None,
     }
+    if not args.no_sphere:
+        try:
+            out["sphere"] = sphere_throughput(log)
+        except Exception as e:  # report, never hide the GPU number
+            log(f"sphere run failed: {e}")
     if not args.no_batched:
         try:
             out["batched"] = batched(log)

@@ -131,6 +131,28 @@ int mpmx_step(mpmx_ctx* ctx, double dt, int nsteps);
 int mpmx_step_async(mpmx_ctx* ctx, double dt, int nsteps);
 int mpmx_synchronize(mpmx_ctx* ctx);
 
+/* The step one named step at a time, for callers that fire callbacks around
+   each (MPM_EXAMPLE::Add_Callbacks, MPM_EXAMPLE.cpp:350-356, fired by
+   MPM_DRIVER::Step, MPM_DRIVER.cpp:1380-1393).  The stages are the named
+   steps of Advance_One_Time_Step in order (MPM_DRIVER.cpp:141-157):
+     0 "simulated-particles"  (nothing to do: invalid particles are skipped)
+     1 "update-weights"       (nothing to do: the kernels weight from X)
+     2 "prepare-scatter"      (the bins, rebuilt when due, and the grid tiles)
+     3 "p2g"
+     4 "forces"               (grid update: gravity, collision objects, friction)
+     5 "reduce-dt"            (the strong and sound-speed CFL when set)
+     6 "g2p"
+     7 "plasticity-hardening" (then the step ends and the time advances)
+   Stage 0 opens a step with example.dt = dt (dt is ignored by the others);
+   each call must take the next stage (MPMX_ESTATE otherwise, and mpmx_step
+   fails while a step is open) and returns after its GPU work.  Such a step
+   runs the unfused kernels.  Between stages 3 and 4, mpmx_get_grid returns
+   P2G's velocities (momentum / mass), as example.velocity holds them there.
+   mpmx_stage_name returns the stage's name, or NULL. */
+#define MPMX_STAGES 8
+int mpmx_step_stage(mpmx_ctx* ctx, double dt, int stage);
+const char* mpmx_stage_name(int stage);
+
 /* Simulation time (example.time). */
 double mpmx_time(const mpmx_ctx* ctx);
 

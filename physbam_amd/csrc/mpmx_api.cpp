@@ -158,6 +158,17 @@ int mpmx_particle_stats(mpmx_ctx* ctx, double* mass, double* momentum, int64_t* 
   return guard([&] { ctx->engine->particle_stats(mass, momentum, valid_count); });
 }
 
+int mpmx_step_stage(mpmx_ctx* ctx, double dt, int stage) {
+  if (!ctx) return bad("null context");
+  return guard([&] { ctx->engine->step_stage(dt, stage); });
+}
+
+const char* mpmx_stage_name(int stage) {
+  static const char* const names[MPMX_STAGES] = {"simulated-particles", "update-weights", "prepare-scatter", "p2g",
+                                                 "forces", "reduce-dt", "g2p", "plasticity-hardening"};
+  return stage >= 0 && stage < MPMX_STAGES ? names[stage] : nullptr;
+}
+
 int mpmx_particle_totals(mpmx_ctx* ctx, mpmx_totals* out) {
   if (!ctx) return bad("null context");
   if (!out) return bad("null output");

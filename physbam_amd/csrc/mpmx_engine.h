@@ -33,6 +33,7 @@ struct EngineBase {
   virtual void add_collision_box(int type, double friction, const double* lo, const double* hi) = 0;
   virtual void clear_collision_objects() = 0;
   virtual void step(double dt, int nsteps, bool wait) = 0;
+  virtual void step_stage(double dt, int stage) = 0;
   virtual void synchronize() = 0;
   virtual double time() const = 0;
   virtual void enable_grid_stats(bool on) = 0;

@@ -2267,7 +2267,7 @@ __global__ __launch_bounds__(BLOCK) void iota_kernel(uint32_t* p, uint32_t n) {
 
 // Dense export of the active tiles, MPM_EXAMPLE::mass / ::velocity layout.
 template<typename T, int D>
-__global__ __launch_bounds__(BLOCK) void export_grid_kernel(Args<T, D> a, T* dmass, T* dvel) {
+__global__ __launch_bounds__(BLOCK) void export_grid_kernel(Args<T, D> a, T* dmass, T* dvel, int momentum) {
   const uint64_t total = (uint64_t)a.counts[1] * 64u;
   for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total; idx += (uint64_t)gridDim.x * BLOCK) {
     const uint32_t t = a.gtiles[idx >> 6];
@@ -2289,9 +2289,10 @@ __global__ __launch_bounds__(BLOCK) void export_grid_kernel(Args<T, D> a, T* dma
 #pragma unroll
     for (int c = 0; c < D; c++) f = f * (uint64_t)a.g.G[c] + (uint64_t)nc[c];
     dmass[f] = nd.m;
-    dvel[f * D + 0] = nd.x;
-    dvel[f * D + 1] = nd.y;
-    if constexpr (D == 3) dvel[f * D + 2] = nd.z;
+    // between P2G and the grid update the nodes hold momentum: v = p / m
+    dvel[f * D + 0] = momentum ? nd.x / nd.m : nd.x;
+    dvel[f * D + 1] = momentum ? nd.y / nd.m : nd.y;
+    if constexpr (D == 3) dvel[f * D + 2] = momentum ? nd.z / nd.m : nd.z;
   }
 }
 
@@ -3202,9 +3203,16 @@ class Engine final : public EngineBase {
   }
   int scene_count() const override { return (int)scenes_.size(); }
 
+  // The recorded steps are replayed from the scene as it is when the reverse
+  // pass runs, so nothing it depends on may change while a tape is open.
+  void check_no_tape(const char* what) const {
+    if (taping_ && !replaying_) throw Error{MPMX_ESTATE, std::string(what) + " cannot change while a tape is recorded"};
+  }
+
   void upload_particles(int64_t n, const void* X, const void* V, const void* F, const void* B, const void* mass,
                         const void* volume, const void* mu, const void* lambda, const uint8_t* valid,
                         const uint16_t* scene_ids) {
+    check_no_tape("particles");
     if (n < 0 || n > (int64_t)0x7fffffff) throw Error{MPMX_EINVAL, "particle count out of range"};
     if (n > 0 && (!X || !V || !F || !B || !mass || !volume || !mu || !lambda || !valid))
       throw Error{MPMX_EINVAL, "null particle array"};
@@ -3291,6 +3299,7 @@ class Engine final : public EngineBase {
   }
 
   void set_constitutive(int model, double thr) override {
+    check_no_tape("the constitutive model");
     if (model != 0 && model != 1 && model != 2) throw Error{MPMX_EINVAL, "unknown constitutive model"};
     if (model == 1 && !(thr > 0)) throw Error{MPMX_EINVAL, "neo-Hookean threshold must be positive"};
     model_ = model;
@@ -3299,10 +3308,12 @@ class Engine final : public EngineBase {
     root_ = D == 2 ? (T)std::sqrt(thr) : (T)std::cbrt(thr);  // pow<1,d>(failure_threshold)
   }
   void set_gravity(const double* g) override {
+    check_no_tape("gravity");
     if (!g) throw Error{MPMX_EINVAL, "null gravity"};
     for (int a = 0; a < D; a++) grav_[a] = (T)g[a];
   }
   void add_collision_box(int type, double friction, const double* lo, const double* hi) override {
+    check_no_tape("collision objects");
     if (type < 0 || type > 2) throw Error{MPMX_EINVAL, "unknown collision type"};
     if (!lo || !hi) throw Error{MPMX_EINVAL, "null box corner"};
     if (nwalls_ >= MAX_WALLS) throw Error{MPMX_EINVAL, "too many collision objects (max 16)"};
@@ -3317,10 +3328,14 @@ class Engine final : public EngineBase {
       if (!(w.hi[a] > w.lo[a])) throw Error{MPMX_EINVAL, "degenerate collision box"};
     walls_[nwalls_++] = w;
   }
-  void clear_collision_objects() override { nwalls_ = 0; }
+  void clear_collision_objects() override {
+    check_no_tape("collision objects");
+    nwalls_ = 0;
+  }
 
   void step(double dt, int nsteps, bool wait) override {
     if (!(dt > 0)) throw Error{MPMX_EINVAL, "dt must be positive"};
+    if (stage_next_ != 0) throw Error{MPMX_ESTATE, "a step run stage by stage is open: finish its stages first"};
     if (nsteps < 0) throw Error{MPMX_EINVAL, "negative step count"};
     // recorded steps too (the reverse pass recomputes its own); plasticity
     // projects F between G2P and the next P2G, which the fused kernel does not
@@ -3503,7 +3518,8 @@ class Engine final : public EngineBase {
   }
   void get_grid(void* mass, void* velocity) override {
     if (scenes_.size() > 1) throw Error{MPMX_ESTATE, "mpmx_get_grid takes a single-scene context"};
-    if (!has_grid_) throw Error{MPMX_ESTATE, "no grid yet: call mpmx_step first"};
+    const bool momentum = grid_holds_momentum();
+    if (!has_grid_ && !momentum && stage_next_ <= ST_P2G) throw Error{MPMX_ESTATE, "no grid yet: call mpmx_step first"};
     const int64_t nn = grid_node_count();
     DevBuf<T> dm, dv;
     dm.alloc(nn);
@@ -3516,7 +3532,7 @@ class Engine final : public EngineBase {
       a.gtiles = gt_other_;
       a.counts = lastcnt_.p;
     }
-    export_grid_kernel<T, D><<<grid_blocks(), BLOCK, 0, stream_>>>(a, dm.p, dv.p);
+    export_grid_kernel<T, D><<<grid_blocks(), BLOCK, 0, stream_>>>(a, dm.p, dv.p, momentum ? 1 : 0);
     if (mass) HIPCHECK(hipMemcpyAsync(mass, dm.p, nn * sizeof(T), hipMemcpyDeviceToHost, stream_));
     if (velocity) HIPCHECK(hipMemcpyAsync(velocity, dv.p, nn * D * sizeof(T), hipMemcpyDeviceToHost, stream_));
     HIPCHECK(hipStreamSynchronize(stream_));
@@ -3698,6 +3714,35 @@ class Engine final : public EngineBase {
     HIPCHECK(hipMemsetAsync(err_.p, 0, 4 * sizeof(uint32_t), stream_));
     StateBuf fin;
     save_state(fin);
+    // On any exception the live state pointers come back, the flags clear and
+    // the final state is copied back (best effort); the tape is dropped.
+    struct Restore {
+      Engine& e;
+      const StateBuf& fin;
+      ParticleBufs<T, D> live;
+      bool armed = true;
+      ~Restore() {
+        if (!armed) return;
+        e.cur_.X = live.X;
+        e.cur_.V = live.V;
+        e.cur_.F = live.F;
+        e.cur_.B = live.B;
+        e.cur_.valid = live.valid;
+        e.grid_override_ = nullptr;
+        e.gridu_override_ = nullptr;
+        e.replaying_ = false;
+        e.taping_ = false;
+        e.nckpt_ = 0;
+        e.fused_ = false;
+        e.fused_last_ = false;
+        e.force_rebin_ = true;
+        try {
+          e.load_state(fin);
+          (void)hipStreamSynchronize(e.stream_);
+        } catch (...) {
+        }
+      }
+    } restore{*this, fin, cur_};
     const double time0 = time_;
     int64_t valid_now = 0;
     particle_stats(nullptr, nullptr, &valid_now);
@@ -3717,6 +3762,7 @@ class Engine final : public EngineBase {
       reverse_overlapped(nseg, kk);
       timer_.end(PH_ADJOINT, ea, stream_);
       replaying_ = false;
+      restore.armed = false;
       finish_backward(fin, time0, gX, gV, gF, gB, gmu, gla);
       return;
     }
@@ -3768,6 +3814,7 @@ class Engine final : public EngineBase {
     }
     timer_.end(PH_ADJOINT, ea, stream_);
     replaying_ = false;
+    restore.armed = false;
     finish_backward(fin, time0, gX, gV, gF, gB, gmu, gla);
   }
 
@@ -4181,109 +4228,200 @@ class Engine final : public EngineBase {
     rebins_++;
   }
 
+  // The unfused step as the named steps of MPM_DRIVER::Advance_One_Time_Step
+  // (MPM_DRIVER.cpp:141-157), in order; step_once() runs them all, and
+  // step_stage() one at a time so that a caller can fire its callbacks
+  // around each (MPM_DRIVER::Step, :1380-1393).
+  enum {
+    ST_SIMULATED = 0,  // "simulated-particles": invalid particles carry INVALID_KEY
+    ST_WEIGHTS,        // "update-weights": the kernels recompute the weights from X
+    ST_SCATTER,        // "prepare-scatter": bins (rebuilt when due) and the grid-tile list
+    ST_P2G,            // "p2g"
+    ST_FORCES,         // "forces": grid update with gravity, collision objects, friction
+    ST_REDUCE_DT,      // "reduce-dt": the strong and sound-speed CFL, if on
+    ST_G2P,            // "g2p"
+    ST_PLASTICITY,     // "plasticity-hardening", then the end of the step
+    NSTAGES
+  };
+  struct OpenStep {
+    Args<T, D> a;
+    bool fric = false, empty = false;
+    T dt = 0, cfl = 0, min_dt = 0;
+    hipEvent_t e_step = nullptr;
+    uint32_t ms = 0;
+  };
+  OpenStep os_;
+  int stage_next_ = 0;  // next stage of an open step run stage by stage (0: none open)
+
+  void run_stage(int k) {
+    OpenStep& s = os_;
+    switch (k) {
+      case ST_SIMULATED: {
+        fused_ = false;
+        fused_last_ = false;
+        s.fric = any_friction();
+        if (refl_on_ && refl_full_ && s.fric)
+          throw Error{MPMX_ESTATE, "frictional reflection does not combine with collision objects that have friction"};
+        if (s.fric) gridf_.alloc((size_t)ntotal_ * 64);
+        s.e_step = timer_.begin(stream_);
+        s.empty = n_ == 0;
+        s.a.dt = s.dt;
+        break;
+      }
+      case ST_WEIGHTS:
+        break;
+      case ST_SCATTER: {
+        if (s.empty) break;
+        if (rebin_due()) rebin();
+        hipEvent_t e0 = timer_.begin(stream_);
+        s.ms = (uint32_t)(step_count_ + 1);  // mover stamp of this step
+        build_grid_tiles(s.ms, grid_override_ ? grid_override_ : gcur_, s.fric ? gridf_.p : nullptr, nullptr);
+        timer_.end(PH_BIN, e0, stream_);
+        s.a = make_args(s.dt);
+        s.a.order = identity_order_ ? nullptr : order_.p;
+        s.a.next_stamp = s.ms + 1;
+        break;
+      }
+      case ST_P2G: {
+        if (s.empty) break;
+        const Args<T, D>& a = s.a;
+        hipEvent_t e1 = timer_.begin(stream_);
+        p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+        p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+        timer_.end(PH_P2G, e1, stream_);
+        if (!replaying_) {  // the mover count for the rebin policy; a reverse pass's replay keeps the host free
+          const int slot = (int)(step_count_ & 3);
+          HIPCHECK(hipMemcpyAsync(rb_host_ + slot, counts_.p + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+          HIPCHECK(hipEventRecord(rb_ev_[slot], stream_));
+          rb_meta_[slot] = Readback{step_count_, bin_stamp_};
+        }
+        break;
+      }
+      case ST_FORCES: {
+        if (s.empty) break;
+        const Args<T, D>& a = s.a;
+        const T dt = s.dt;
+        hipEvent_t e2 = timer_.begin(stream_);
+        if (stats_on_) record_grid_totals(a, a.grid, 0);
+        Args<T, D> ag = a;  // the grid update's arguments
+        if (refl_on_ && !refl_full_) {
+          // gravity into the P2G sums, then Reflect_Boundary_Mass_Momentum with
+          // the force reflection folded in (the same linear map on m v and dt f)
+          add_gravity_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(a.grid, a.gtiles, counts_.p + 1, dt * a.grav[0],
+                                                                      dt * a.grav[1], D == 3 ? dt * a.grav[D - 1] : (T)0);
+          for (int k2 = 0; k2 < D; k2++)
+            reflect_mass_momentum_kernel<T, D><<<reflect_blocks(k2), BLOCK, 0, stream_>>>(a.grid, scenes_dev_.p,
+                                                                                            reflect_axis(k2));
+          for (int c = 0; c < D; c++) ag.grav[c] = 0;
+        }
+        if (s.fric)
+          grid_kernel<T, D, true><<<grid_blocks(), BLOCK, 0, stream_>>>(ag);
+        else
+          grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(ag);
+        if (refl_on_ && refl_full_) {  // Reflect_Boundary_Friction after the collision objects (MPM_DRIVER.cpp:999-1019)
+          for (int k2 = 0; k2 < D; k2++)
+            reflect_friction_kernel<T, D><<<reflect_blocks(k2), BLOCK, 0, stream_>>>(a.grid, scenes_dev_.p,
+                                                                                       reflect_axis(k2), refl_fric_);
+        } else if (refl_on_) {  // Reflect_Boundary_Velocity of velocity and velocity_friction_save (:1057-1059)
+          for (int k2 = 0; k2 < D; k2++)
+            reflect_velocity_kernel<T, D><<<reflect_blocks(k2), BLOCK, 0, stream_>>>(
+                a.grid, s.fric ? a.gridf : nullptr, scenes_dev_.p, reflect_axis(k2));
+        }
+        if (stats_on_) record_grid_totals(a, a.grid, 1);
+        timer_.end(PH_GRID, e2, stream_);
+        break;
+      }
+      case ST_REDUCE_DT: {
+        if (s.empty || !limits_on()) break;
+        hipEvent_t e4 = timer_.begin(stream_);
+        s.a.dt = reduce_dt(s.a, s.fric, s.cfl, s.min_dt);
+        timer_.end(PH_DT, e4, stream_);
+        break;
+      }
+      case ST_G2P: {
+        if (s.empty) break;
+        const Args<T, D>& a = s.a;
+        hipEvent_t e3 = timer_.begin(stream_);
+        if (plastic_) {  // the particles simulated in this step, before G2P invalidates any
+          simmask_.alloc(n_);
+          HIPCHECK(hipMemcpyAsync(simmask_.p, cur_.valid, n_, hipMemcpyDeviceToDevice, stream_));
+        }
+        if (s.fric) {
+          g2p_kernel<T, D, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+          g2p_mover_kernel<T, D, true, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+        } else {
+          g2p_kernel<T, D, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+          g2p_mover_kernel<T, D, false, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+        }
+        timer_.end(PH_G2P, e3, stream_);
+        break;
+      }
+      case ST_PLASTICITY: {
+        if (s.empty) {
+          time_ += (double)s.dt;
+          last_dt_ = (double)s.dt;
+          timer_.end(PH_STEP, s.e_step, stream_);
+          break;
+        }
+        if (plastic_) {
+          hipEvent_t e5 = timer_.begin(stream_);
+          plasticity_kernel<T, D><<<std::min<uint32_t>(cdiv(n_, BLOCK), 4096), BLOCK, 0, stream_>>>(cur_, n_,
+                                                                                                  simmask_.p, pl_);
+          timer_.end(PH_G2P, e5, stream_);
+        }
+        step_count_++;
+        steps_since_rebin_++;
+        since_reorder_++;
+        if (taping_ && !replaying_) {
+          tape_dt_.push_back((double)s.a.dt);
+          if (tape_dt_.size() % (size_t)tape_k_ == 0) save_state(next_checkpoint());
+        }
+        HIPCHECK(hipGetLastError());
+        has_grid_ = true;
+        time_ += (double)s.a.dt;
+        last_dt_ = (double)s.a.dt;
+        timer_.end(PH_STEP, s.e_step, stream_);
+        break;
+      }
+    }
+  }
+
   // One unfused step; returns the dt it took (dt unless Reduce_Dt lowered it,
   // with cfl and min_dt as example.cfl and example.min_dt).
   T step_once(T dt) { return step_once(dt, lim_cfl_, lim_min_dt_); }
   T step_once(T dt, T cfl, T min_dt) {
-    fused_ = false;
-    fused_last_ = false;
-    const bool fric = any_friction();
-    if (refl_on_ && refl_full_ && fric)
-      throw Error{MPMX_ESTATE, "frictional reflection does not combine with collision objects that have friction"};
-    if (fric) gridf_.alloc((size_t)ntotal_ * 64);
-    hipEvent_t e_step = timer_.begin(stream_);
-    if (n_ == 0) {
-      time_ += (double)dt;
-      last_dt_ = (double)dt;
-      timer_.end(PH_STEP, e_step, stream_);
-      return dt;
-    }
-    if (rebin_due()) rebin();
-    hipEvent_t e0 = timer_.begin(stream_);
-    const uint32_t ms = (uint32_t)(step_count_ + 1);  // mover stamp of this step
-    build_grid_tiles(ms, grid_override_ ? grid_override_ : gcur_, fric ? gridf_.p : nullptr, nullptr);
-    timer_.end(PH_BIN, e0, stream_);
-
-    Args<T, D> a = make_args(dt);
-    a.order = identity_order_ ? nullptr : order_.p;
-    a.next_stamp = ms + 1;
-    hipEvent_t e1 = timer_.begin(stream_);
-    p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-    p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
-    timer_.end(PH_P2G, e1, stream_);
-    if (!replaying_) {  // the mover count for the rebin policy; a reverse pass's replay keeps the host free
-      const int slot = (int)(step_count_ & 3);
-      HIPCHECK(hipMemcpyAsync(rb_host_ + slot, counts_.p + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
-      HIPCHECK(hipEventRecord(rb_ev_[slot], stream_));
-      rb_meta_[slot] = Readback{step_count_, bin_stamp_};
-    }
-
-    hipEvent_t e2 = timer_.begin(stream_);
-    if (stats_on_) record_grid_totals(a, a.grid, 0);
-    Args<T, D> ag = a;  // the grid update's arguments
-    if (refl_on_ && !refl_full_) {
-      // gravity into the P2G sums, then Reflect_Boundary_Mass_Momentum with
-      // the force reflection folded in (the same linear map on m v and dt f)
-      add_gravity_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(a.grid, a.gtiles, counts_.p + 1, dt * a.grav[0],
-                                                                  dt * a.grav[1], D == 3 ? dt * a.grav[D - 1] : (T)0);
-      for (int k = 0; k < D; k++)
-        reflect_mass_momentum_kernel<T, D><<<reflect_blocks(k), BLOCK, 0, stream_>>>(a.grid, scenes_dev_.p,
-                                                                                       reflect_axis(k));
-      for (int c = 0; c < D; c++) ag.grav[c] = 0;
-    }
-    if (fric)
-      grid_kernel<T, D, true><<<grid_blocks(), BLOCK, 0, stream_>>>(ag);
-    else
-      grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(ag);
-    if (refl_on_ && refl_full_) {  // Reflect_Boundary_Friction after the collision objects (MPM_DRIVER.cpp:999-1019)
-      for (int k = 0; k < D; k++)
-        reflect_friction_kernel<T, D><<<reflect_blocks(k), BLOCK, 0, stream_>>>(a.grid, scenes_dev_.p,
-                                                                                  reflect_axis(k), refl_fric_);
-    } else if (refl_on_) {  // Reflect_Boundary_Velocity of velocity and velocity_friction_save (:1057-1059)
-      for (int k = 0; k < D; k++)
-        reflect_velocity_kernel<T, D><<<reflect_blocks(k), BLOCK, 0, stream_>>>(a.grid, fric ? a.gridf : nullptr,
-                                                                                  scenes_dev_.p, reflect_axis(k));
-    }
-    if (stats_on_) record_grid_totals(a, a.grid, 1);
-    timer_.end(PH_GRID, e2, stream_);
-    if (limits_on()) {
-      hipEvent_t e4 = timer_.begin(stream_);
-      a.dt = reduce_dt(a, fric, cfl, min_dt);
-      timer_.end(PH_DT, e4, stream_);
-    }
-
-    hipEvent_t e3 = timer_.begin(stream_);
-    if (plastic_) {  // the particles simulated in this step, before G2P invalidates any
-      simmask_.alloc(n_);
-      HIPCHECK(hipMemcpyAsync(simmask_.p, cur_.valid, n_, hipMemcpyDeviceToDevice, stream_));
-    }
-    if (fric) {
-      g2p_kernel<T, D, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-      g2p_mover_kernel<T, D, true, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
-    } else {
-      g2p_kernel<T, D, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-      g2p_mover_kernel<T, D, false, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
-    }
-    if (plastic_)
-      plasticity_kernel<T, D><<<std::min<uint32_t>(cdiv(n_, BLOCK), 4096), BLOCK, 0, stream_>>>(cur_, n_, simmask_.p,
-                                                                                              pl_);
-    timer_.end(PH_G2P, e3, stream_);
-    step_count_++;
-    steps_since_rebin_++;
-    since_reorder_++;
-    if (taping_ && !replaying_) {
-      tape_dt_.push_back((double)a.dt);
-      if (tape_dt_.size() % (size_t)tape_k_ == 0) {
-        save_state(next_checkpoint());
-      }
-    }
-    HIPCHECK(hipGetLastError());
-    has_grid_ = true;
-    time_ += (double)a.dt;
-    last_dt_ = (double)a.dt;
-    timer_.end(PH_STEP, e_step, stream_);
-    return a.dt;
+    if (stage_next_ != 0) throw Error{MPMX_ESTATE, "a step run stage by stage is open: finish its stages first"};
+    os_.dt = dt;
+    os_.cfl = cfl;
+    os_.min_dt = min_dt;
+    for (int k = 0; k < NSTAGES; k++) run_stage(k);
+    return os_.empty ? dt : os_.a.dt;
   }
+  // One named step of an unfused step: stage 0 opens a step with dt (cfl and
+  // min_dt of the dt limits), the stages follow in order, and the last one
+  // closes it.  Returns after the stage's GPU work.
+  void step_stage(double dt, int k) override {
+    if (k < 0 || k >= NSTAGES) throw Error{MPMX_EINVAL, "stage must be in [0, 8)"};
+    if (k != stage_next_)
+      throw Error{MPMX_ESTATE, "stage " + std::to_string(k) + " out of order: the next stage is " +
+                                   std::to_string(stage_next_)};
+    if (k == 0) {
+      if (!(dt > 0)) throw Error{MPMX_EINVAL, "dt must be positive"};
+      os_.dt = (T)dt;
+      os_.cfl = lim_cfl_;
+      os_.min_dt = lim_min_dt_;
+    }
+    stage_next_ = (k + 1) % NSTAGES;
+    try {
+      run_stage(k);
+      synchronize();
+    } catch (...) {
+      stage_next_ = 0;  // the step is abandoned
+      throw;
+    }
+  }
+  // The grid holds P2G's momentum between the "p2g" and "forces" stages.
+  bool grid_holds_momentum() const { return stage_next_ == ST_FORCES; }
 
   bool limits_on() const { return lim_strong_ || lim_sound_; }
   ReflectAxis reflect_axis(int k) const {

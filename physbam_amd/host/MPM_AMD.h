@@ -13,8 +13,13 @@
 //     them back (or set example.sync_particles_every_step);
 //   * one constitutive model per example (fixed corotated or neo-Hookean) with
 //     per-particle mu/lambda, plus gravity; collision objects are boxes;
-//   * named-step callbacks fire for "time-step" only, around the whole GPU
-//     step (MPM_DRIVER.cpp:1380-1393 fires them per sub-step as well);
+//   * named-step callbacks (Add_Callbacks) fire as MPM_DRIVER::Step fires
+//     them (MPM_DRIVER.cpp:1380-1393): around each named step of
+//     Advance_One_Time_Step ("simulated-particles" ... "plasticity-hardening",
+//     :141-157) and around "time-step" in Simulate_To_Frame (:186).  A step
+//     with any such callback registered runs stage by stage on the unfused
+//     kernels (mpmx_step_stage); without callbacks the step, and a whole
+//     frame's substep loop, run in one call on the fused kernels;
 //   * as in the reference, Advance_One_Time_Step leaves example.time alone
 //     (Simulate_To_Frame advances it, MPM_DRIVER.cpp:197).
 #pragma once
@@ -275,14 +280,24 @@ template<class TV> class MPM_DRIVER {
   }
 
   // MPM_DRIVER::Advance_One_Time_Step (MPM_DRIVER.cpp:141-157) with example.dt;
-  // Reduce_Dt may lower example.dt, as in the reference.
+  // Reduce_Dt may lower example.dt, as in the reference.  With callbacks on
+  // any of its named steps, each step runs between its begin and end
+  // callbacks.
   void Advance_One_Time_Step() {
-    Step([this] {
+    if (Has_Stage_Callbacks()) {
+      for (int k = 0; k < MPMX_STAGES; k++) {
+        const char* name = mpmx_stage_name(k);
+        Fire(name, true);
+        Check(mpmx_step_stage(ctx, (double)example.dt, k));
+        Fire(name, false);
+      }
+    } else {
       Check(mpmx_step(ctx, (double)example.dt, 1));
-      double taken = 0;
-      Check(mpmx_last_dt(ctx, &taken));
-      example.dt = (T)taken;
-    }, "time-step");
+    }
+    double taken = 0;
+    Check(mpmx_last_dt(ctx, &taken));
+    example.dt = (T)taken;
+    if (example.sync_particles_every_step) Read_Particles();
   }
 
   // MPM_DRIVER::Grid_V_Upper_Bound / Compute_Dt (MPM_DRIVER.cpp:1098-1138) of
@@ -303,27 +318,60 @@ template<class TV> class MPM_DRIVER {
   // begin_frame callbacks, the adaptive substeps on the device (Compute_Dt
   // clamped to [min_dt, max_dt], frame-end rule) and the end_frame callbacks;
   // example.time follows the device; `substeps` counts the substeps taken.
+  // With callbacks registered the substep loop runs here, as the reference
+  // writes it, each substep between the "time-step" callbacks.
   void Simulate_To_Frame(int frame) {
     Check_Cfl();
     for (; current_frame < frame; current_frame++) {
       for (auto& f : example.begin_frame) f(current_frame);
-      int64_t n = 0;
-      Check(mpmx_advance_frames(ctx, 1, (double)example.frame_dt, (double)example.cfl, (double)example.min_dt,
-                                (double)example.max_dt, nullptr, 0, &n));
-      substeps += n;
-      example.time = (T)mpmx_time(ctx);
-      if (example.sync_particles_every_step) Read_Particles();
+      if (example.time_step_callbacks.empty()) {
+        int64_t n = 0;
+        Check(mpmx_advance_frames(ctx, 1, (double)example.frame_dt, (double)example.cfl, (double)example.min_dt,
+                                  (double)example.max_dt, nullptr, 0, &n));
+        substeps += n;
+        example.time = (T)mpmx_time(ctx);
+        if (example.sync_particles_every_step) Read_Particles();
+      } else {
+        const T time_at_frame = example.time + example.frame_dt;
+        bool done = false;
+        while (!done) {
+          example.dt = Compute_Dt();
+          example.dt = std::min(std::max(example.dt, example.min_dt), example.max_dt);  // clamp.h:13-16
+          T next_time = example.time + example.dt;
+          if (next_time + (T).0001 * example.dt > time_at_frame) {
+            next_time = time_at_frame;
+            done = true;
+          } else if (next_time + example.dt > time_at_frame) {
+            next_time = (example.time + time_at_frame) / 2;
+          }
+          example.dt = next_time - example.time;
+          const T original_dt = example.dt;
+          if (!(example.dt > 0)) throw std::runtime_error("mpmx: substep dt is not positive");
+          Step([this] { Advance_One_Time_Step(); }, "time-step");
+          if (example.dt < original_dt) {  // Reduce_Dt lowered it: the frame goes on
+            next_time = example.time + example.dt;
+            done = false;
+          }
+          example.time = next_time;
+          substeps++;
+        }
+      }
       for (auto& f : example.end_frame) f(current_frame);
     }
   }
   int current_frame = 0;
   int64_t substeps = 0;
 
-  // n steps of example.dt enqueued back to back (one host synchronisation).
+  // n steps of example.dt: enqueued back to back (one host synchronisation)
+  // when no callback is registered, else one Advance_One_Time_Step each
+  // between the "time-step" callbacks.
   void Advance_Steps(int n) {
-    Fire("time-step", true);
-    Check(mpmx_step(ctx, (double)example.dt, n));
-    Finish_Step("time-step");
+    if (example.time_step_callbacks.empty()) {
+      Check(mpmx_step(ctx, (double)example.dt, n));
+      if (example.sync_particles_every_step) Read_Particles();
+      return;
+    }
+    for (int i = 0; i < n; i++) Step([this] { Advance_One_Time_Step(); }, "time-step");
   }
 
   // Device -> example.particles (X, V, F, B, valid; mass, volume, mu, lambda do not change).
@@ -349,20 +397,21 @@ template<class TV> class MPM_DRIVER {
   void Check_Cfl() const {
     if (!example.use_affine_cfl) throw std::invalid_argument("the device CFL bound is the affine one (use_affine_cfl)");
   }
+  bool Has_Stage_Callbacks() const {
+    for (const auto& c : example.time_step_callbacks)
+      if (c.first != "time-step" && (!c.second[0].empty() || !c.second[1].empty())) return true;
+    return false;
+  }
   void Fire(const char* name, bool begin) {
     auto it = example.time_step_callbacks.find(name);
     if (it == example.time_step_callbacks.end()) return;
     for (auto& f : it->second[begin ? 1 : 0]) f();
   }
-  void Finish_Step(const char* name) {
-    if (example.sync_particles_every_step) Read_Particles();
-    Fire(name, false);
-  }
   // MPM_DRIVER::Step (MPM_DRIVER.cpp:1380-1393)
   template<class F> void Step(F&& f, const char* name) {
     Fire(name, true);
     f();
-    Finish_Step(name);
+    Fire(name, false);
   }
 };
 

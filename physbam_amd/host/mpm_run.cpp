@@ -13,8 +13,12 @@
 // square starts at 30 m/s downward so that it reaches the ground, and the
 // run writes its output files (particles and restart data) into <dir> at the
 // end (MPM_EXAMPLE::Write_Output_Files).
+// With "stages", begin and end callbacks are registered on every named step
+// of Advance_One_Time_Step and on "time-step" (MPM_EXAMPLE::Add_Callbacks),
+// so each step runs stage by stage; the program checks that they fire in the
+// reference's order (MPM_DRIVER.cpp:141-157, :1380-1393) and once per step.
 //
-//   mpm_run <out.bin> [steps=200] [frames=0] [reflect <dir>]
+//   mpm_run <out.bin> [steps=200] [frames=0] [reflect <dir> | stages]
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -36,6 +40,7 @@ int main(int argc, char** argv) {
   const int steps = argc > 2 ? atoi(argv[2]) : 200;
   const int frames = argc > 3 ? atoi(argv[3]) : 0;
   const bool reflect = argc > 5 && std::string(argv[4]) == "reflect";
+  const bool stages = argc > 4 && std::string(argv[4]) == "stages";
   MPM_EXAMPLE<TV> example;
   TV lo, hi;
   hi(0) = hi(1) = 1;
@@ -100,6 +105,14 @@ int main(int argc, char** argv) {
 
   int callbacks = 0;
   example.Add_Callbacks(false, "time-step", [&] { callbacks++; });
+  // with "stages": every begin/end callback appends its name and side
+  std::vector<std::string> fired;
+  if (stages)
+    for (int k = 0; k < MPMX_STAGES; k++) {
+      const std::string nm = mpmx_stage_name(k);
+      example.Add_Callbacks(true, nm.c_str(), [&fired, nm] { fired.push_back(nm + "+"); });
+      example.Add_Callbacks(false, nm.c_str(), [&fired, nm] { fired.push_back(nm + "-"); });
+    }
   MPM_DRIVER<TV> driver(example);
   try {
     driver.Initialize();
@@ -122,10 +135,23 @@ int main(int argc, char** argv) {
   }
   fclose(f);
   if (frames > 0) {
-    printf("mpm_run: %lld particles, %d frames, %lld substeps, time %.17g\n", (long long)n, frames,
-           (long long)driver.substeps, (double)example.time);
-    return 0;
+    printf("mpm_run: %lld particles, %d frames, %lld substeps, %d time-step callbacks, time %.17g\n", (long long)n,
+           frames, (long long)driver.substeps, callbacks, (double)example.time);
+    return callbacks == driver.substeps ? 0 : 1;
   }
-  printf("mpm_run: %lld particles, %d steps, %d callbacks\n", (long long)n, steps, callbacks);
-  return callbacks == steps ? 0 : 1;
+  if (stages) {  // per step: each named step's begin then end callback, in the reference's order
+    size_t i = 0;
+    bool ok = fired.size() == (size_t)steps * 2 * MPMX_STAGES;
+    for (int st = 0; ok && st < steps; st++)
+      for (int k = 0; ok && k < MPMX_STAGES; k++) {
+        ok = fired[i++] == std::string(mpmx_stage_name(k)) + "+";
+        ok = ok && fired[i++] == std::string(mpmx_stage_name(k)) + "-";
+      }
+    printf("mpm_run: %lld particles, %d steps, %zu stage callbacks in order: %s\n", (long long)n, steps,
+           fired.size(), ok ? "yes" : "no");
+    return ok && callbacks == 0 ? 0 : 1;  // "time-step" fires in Simulate_To_Frame only
+  }
+  // "time-step" fires around each substep of Simulate_To_Frame, not around Advance_One_Time_Step
+  printf("mpm_run: %lld particles, %d steps, %d time-step callbacks\n", (long long)n, steps, callbacks);
+  return callbacks == 0 ? 0 : 1;
 }

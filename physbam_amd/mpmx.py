@@ -25,6 +25,7 @@ EXPORTS = (
     "mpmx_set_batch_particles", "mpmx_set_particles", "mpmx_get_particles", "mpmx_get_material",
     "mpmx_particle_count", "mpmx_set_constitutive", "mpmx_set_gravity", "mpmx_add_collision_box",
     "mpmx_clear_collision_objects", "mpmx_step", "mpmx_step_async", "mpmx_synchronize", "mpmx_time",
+    "mpmx_step_stage", "mpmx_stage_name",
     "mpmx_enable_grid_stats", "mpmx_grid_stats", "mpmx_particle_stats", "mpmx_particle_totals",
     "mpmx_grid_totals", "mpmx_grid_node_count",
     "mpmx_get_grid", "mpmx_set_reorder_interval", "mpmx_set_rebin_policy", "mpmx_bin_stats",
@@ -38,6 +39,15 @@ EXPORTS = (
     "mpmx_phase_time", "mpmx_stress", "mpmx_svd", "mpmx_sort_keys", "mpmx_seed_particles",
     "mpmx_random_boxes", "mpmx_seed_particles_poisson",
 )
+
+
+STAGES = 8
+
+
+def stage_name(stage):
+    """Name of a stage of mpmx_step_stage ("simulated-particles" ... "plasticity-hardening"), or None."""
+    r = lib().mpmx_stage_name(int(stage))
+    return None if r is None else r.decode()
 
 
 class MpmxError(RuntimeError):
@@ -94,6 +104,9 @@ def lib():
         L.mpmx_clear_collision_objects.argtypes = [vp]
         L.mpmx_step.argtypes = [vp, dbl, c_int]
         L.mpmx_step_async.argtypes = [vp, dbl, c_int]
+        L.mpmx_step_stage.argtypes = [vp, dbl, c_int]
+        L.mpmx_stage_name.argtypes = [c_int]
+        L.mpmx_stage_name.restype = ctypes.c_char_p
         L.mpmx_synchronize.argtypes = [vp]
         L.mpmx_time.argtypes = [vp]
         L.mpmx_time.restype = dbl
@@ -311,6 +324,10 @@ class Context:
 
     def step_async(self, dt, nsteps=1):
         _check(lib().mpmx_step_async(self._h, dt, nsteps))
+
+    def step_stage(self, dt, stage):
+        """One named step of MPM_DRIVER::Advance_One_Time_Step (stage 0..7, see stage_name)."""
+        _check(lib().mpmx_step_stage(self._h, dt, int(stage)))
 
     def synchronize(self):
         _check(lib().mpmx_synchronize(self._h))

@@ -196,3 +196,43 @@ def test_reverse_passes_in_a_row_on_one_context():
             print(f"  {', '.join(f'{q} {e:.1e}' for q, e in errs.items())}", flush=True)
             for q, e in errs.items():
                 assert e <= 1e-9, (i, q, e)
+
+
+def test_scene_cannot_change_while_taping():
+    """The reverse pass replays the recorded steps from the scene as it then is,
+    so while a tape is open every call that would change the particles, the
+    model, gravity or the collision objects fails with MPMX_ESTATE (on the
+    host, before any kernel runs), and the tape's gradients are those of an
+    undisturbed run."""
+    sc = scene(2, 8, mt.COROTATED, mt.SLIP, seed=7)
+    d = sc["dim"]
+    n = sc["X"].shape[0]
+    rng = np.random.default_rng(3)
+    adj = (rng.normal(size=(n, d)), None, None, None)
+    want = gpu_backward(sc, 4, 2, adj)
+    with mpmx.Context(d, sc["N"], sc["lo"], sc["hi"], precision=8) as ctx:
+        ctx.set_constitutive(sc["model"], 0.25)
+        ctx.set_gravity(sc["gravity"])
+        for w in sc["walls"]:
+            ctx.add_collision_box(w["type"], w["friction"], w["lo"][:d], w["hi"][:d])
+        args = (sc["X"], sc["V"], sc["F"], sc["B"], sc["mass"], sc["volume"], sc["mu"], sc["lambda"], sc["valid"])
+        ctx.set_particles(*args)
+        ctx.tape_begin(2)
+        ctx.step(sc["dt"], 2)
+        w0 = sc["walls"][0]
+        calls = [lambda: ctx.set_particles(*args),
+                 lambda: ctx.set_particles(*(a[: n // 2] for a in args)),
+                 lambda: ctx.set_gravity([0.0, -1.0]),
+                 lambda: ctx.set_constitutive(mt.NEO_HOOKEAN, 0.25),
+                 lambda: ctx.add_collision_box(w0["type"], 0.0, w0["lo"][:d], w0["hi"][:d]),
+                 lambda: ctx.clear_collision_objects()]
+        for call in calls:
+            with pytest.raises(mpmx.MpmxError) as e:
+                call()
+            assert e.value.code == -3  # MPMX_ESTATE
+        ctx.step(sc["dt"], 2)
+        got = ctx.backward(*adj)
+        # the tape is consumed: the scene can change again
+        ctx.set_gravity(sc["gravity"])
+    for k in ("X", "V", "mu", "lambda"):
+        np.testing.assert_allclose(got[k], want[k], rtol=0, atol=1e-12 * max(np.abs(want[k]).max(), 1e-30))

@@ -61,3 +61,72 @@ def test_batch_matches_each_scene_alone(dim, prec, walls):
             print("scene", s, k, f"{err:.2e}", flush=True)
             assert err <= (1e-8 if prec == 8 else 1e-4), (s, k, err)
         off += n
+
+
+@pytest.fixture(scope="module")
+def batched_workload():
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+
+    print("seeding the 256 batched scenes", flush=True)
+    sizes, parts = bench.batched_scenes()
+    return sizes, parts
+
+
+@pytest.mark.timeout(900)
+def test_batched_workload_at_full_shape(batched_workload):
+    """The batched benchmark workload itself: 256 scenes of 32^3..96^3 cells,
+    one sphere each (the generator the benchmark times), fixed corotated,
+    gravity, FP32, dt 1e-4, in one context.  After 3 steps the smallest and
+    the largest scene of each grid size match their own FP64 oracle runs to
+    1e-4 (X, V, F, B; valid flags equal); after 10 steps no particle is lost
+    and each scene's momentum is M g t (no walls: the internal forces and the
+    APIC transfers add no momentum), to 1e-5 of M |g| t."""
+    sizes, parts = batched_workload
+    counts = [int(P["X"].shape[0]) for P in parts]
+    n = sum(counts)
+    print(f"{len(parts)} scenes, {n} particles ({min(counts)}..{max(counts)})", flush=True)
+    cat = lambda k: np.concatenate([P[k] for P in parts])
+    dt, g = 1e-4, np.array([0.0, -9.8, 0.0])
+    ctx = mpmx.Context.batch(3, [(N, [0.0] * 3, [1.0] * 3) for N in sizes], precision=mpmx.FP32)
+    with ctx:
+        ctx.set_constitutive(mpmx.FIXED_COROTATED)
+        ctx.set_gravity(list(g))
+        ctx.set_batch_particles(counts, cat("X"), cat("V"), cat("F"), cat("B"), cat("mass"), cat("volume"),
+                                cat("mu"), cat("lam"), cat("valid"))
+        ctx.step(dt, 3)
+        at3 = ctx.get_particles()
+        ctx.step(dt, 7)
+        at10 = ctx.get_particles()
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    picks = set()
+    for N in (32, 48, 64, 96):
+        ids = [s for s in range(len(sizes)) if sizes[s] == N]
+        assert ids, N
+        picks.add(min(ids, key=lambda s: counts[s]))
+        picks.add(max(ids, key=lambda s: counts[s]))
+    assert len(picks) == 8
+    for s in sorted(picks):
+        P = parts[s]
+        sc = dict(dim=3, prec=4, model=mpmx.FIXED_COROTATED, N=[sizes[s]] * 3, lo=[0.0] * 3, hi=[1.0] * 3,
+                  gravity=list(g), dt=dt, neo_threshold=0.25, walls=[], X=P["X"], V=P["V"], F=P["F"], B=P["B"],
+                  mass=P["mass"], volume=P["volume"], mu=P["mu"], valid=P["valid"], **{"lambda": P["lam"]})
+        orc = mt.oracle_run(sc, 3, prec=8, want_grid=False)
+        part = {k: at3[k][offs[s]:offs[s + 1]] for k in ("X", "V", "F", "B", "valid")}
+        assert np.array_equal(part["valid"], orc["valid"]), s
+        errs = {k: float(np.abs(part[k] - orc[k]).max() / max(np.abs(orc[k]).max(), 1e-30))
+                for k in ("X", "V", "F", "B")}
+        print(f"scene {s} (N {sizes[s]}, {counts[s]} particles):", {k: f"{v:.1e}" for k, v in errs.items()},
+              flush=True)
+        for k, v in errs.items():
+            assert v <= 1e-4, (s, k, v)
+    assert np.all(at10["valid"] == 1)
+    t = 10 * dt
+    for s in range(len(parts)):
+        m = parts[s]["mass"].astype(np.float64)
+        V = at10["V"][offs[s]:offs[s + 1]].astype(np.float64)
+        M = m.sum()
+        err = np.abs((m[:, None] * V).sum(0) - M * g * t).max() / (M * 9.8 * t)
+        assert err <= 1e-5, (s, err)

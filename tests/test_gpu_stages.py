@@ -1,0 +1,61 @@
+"""The step run one named step at a time (mpmx_step_stage), as a caller that
+fires callbacks around each named step of MPM_DRIVER::Advance_One_Time_Step
+drives it (MPM_DRIVER.cpp:141-157, :1380-1393): the stages must come in order,
+between "p2g" and "forces" the grid holds P2G's velocities (checked against
+the oracle's velocity_save of the same step), and the result equals the
+oracle's step."""
+import numpy as np
+import pytest
+
+import mpmtest as mt
+from physbam_amd import mpmx
+from test_gpu_step import compare, gpu_context, rel
+
+pytestmark = pytest.mark.gpu
+
+sph3 = lambda X: ((X - np.array([0.5, 0.45, 0.5])) ** 2).sum(1) < 0.3 ** 2
+
+
+def scene():
+    walls = mt.wall_boxes(3, [16] * 3, [0] * 3, [1] * 3, flags=63, inset=0.12, type_=mt.SEPARATE, friction=0.3)
+    return mt.make_scene(3, 16, prec=8, model=mt.COROTATED, shape=sph3, v0=[0.3, -3.0, 0.2], vrand=1.0,
+                         brand=1e-3, frand=0.2, walls=walls, dt=1e-3, per_particle_E=True, seed=6)
+
+
+def test_stage_names():
+    names = [mpmx.stage_name(k) for k in range(mpmx.STAGES)]
+    assert names == ["simulated-particles", "update-weights", "prepare-scatter", "p2g", "forces", "reduce-dt", "g2p",
+                     "plasticity-hardening"]
+    assert mpmx.stage_name(8) is None and mpmx.stage_name(-1) is None
+
+
+def test_stages_in_order_match_the_oracle():
+    sc = scene()
+    ctx = gpu_context(sc)
+    with ctx:
+        with pytest.raises(mpmx.MpmxError) as e:
+            ctx.step_stage(sc["dt"], 2)  # a step starts at stage 0
+        assert e.value.code == -3
+        for k in range(4):
+            ctx.step_stage(sc["dt"], k)
+        m, v = ctx.get_grid()  # after "p2g": P2G's velocity
+        with pytest.raises(mpmx.MpmxError) as e:
+            ctx.step(sc["dt"], 1)  # not while a staged step is open
+        assert e.value.code == -3
+        with pytest.raises(mpmx.MpmxError):
+            ctx.step_stage(sc["dt"], 3)  # no stage twice
+        for k in range(4, mpmx.STAGES):
+            ctx.step_stage(sc["dt"], k)
+        m1, v1 = ctx.get_grid()  # after the step: the velocity G2P read
+        for k in range(mpmx.STAGES):
+            ctx.step_stage(sc["dt"], k)
+        got = ctx.get_particles()
+        assert ctx.time() == pytest.approx(2 * sc["dt"], rel=1e-15)
+        ctx.step(sc["dt"], 1)  # a plain step again once the staged step is closed
+    o1 = mt.oracle_run(sc, 1, prec=8)
+    assert rel(m, o1["grid_mass"]) <= 1e-12
+    assert rel(v, o1["grid_velocity_save"]) <= 1e-10
+    assert rel(m1, o1["grid_mass"]) <= 1e-12
+    assert rel(v1, o1["grid_velocity"]) <= 1e-10
+    o2 = mt.oracle_run(sc, 2, prec=8, want_grid=False)
+    compare(got, o2, 1e-9, "stages 2 steps")
