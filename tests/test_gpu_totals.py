@@ -33,6 +33,8 @@ def fixture_scene(name):
     sc["walls"] = json.loads(str(z[name + "_walls"]))
     for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
         sc[k] = z[f"{name}_in_{k}"]
+    if sc["prec"] == 4:  # an FP32 context holds gravity as a float: the FP64 oracle takes the same value
+        sc["gravity"] = [float(np.float32(g)) for g in sc["gravity"]]
     return sc, meta["steps"]
 
 
