@@ -606,17 +606,18 @@ template<> struct SeriesRadius<double> { static constexpr double r2 = 1.0 / 6553
 // the Taylor series of (1+x) - sqrt(1+x) (binomial series of sqrt(1+x)).
 // Every partial product is a polynomial in E, so it commutes with E and is
 // symmetric: only the upper triangle is formed.
-template<typename T, int D>
-__device__ __forceinline__ void corot_series(const T (&E)[D][D], T (&f)[D][D], bool hencky = false) {
-  // hencky: ln(I + E) = E - E^2/2 + E^3/3 - ... + E^7/7 instead
+// HENCKY: ln(I + E) = E - E^2/2 + E^3/3 - ... + E^7/7 instead (a template
+// flag, so that the coefficients are immediates).
+template<typename T, int D, bool HENCKY>
+__device__ __forceinline__ void corot_series(const T (&E)[D][D], T (&f)[D][D]) {
   const T c[8] = {(T)0,
-                  hencky ? (T)1 : (T)0.5,
-                  hencky ? (T)-0.5 : (T)0.125,
-                  hencky ? (T)(1.0 / 3) : (T)-0.0625,
-                  hencky ? (T)-0.25 : (T)(5.0 / 128),
-                  hencky ? (T)0.2 : (T)(-7.0 / 256),
-                  hencky ? (T)(-1.0 / 6) : (T)(21.0 / 1024),
-                  hencky ? (T)(1.0 / 7) : (T)(-33.0 / 2048)};
+                  HENCKY ? (T)1 : (T)0.5,
+                  HENCKY ? (T)-0.5 : (T)0.125,
+                  HENCKY ? (T)(1.0 / 3) : (T)-0.0625,
+                  HENCKY ? (T)-0.25 : (T)(5.0 / 128),
+                  HENCKY ? (T)0.2 : (T)(-7.0 / 256),
+                  HENCKY ? (T)(-1.0 / 6) : (T)(21.0 / 1024),
+                  HENCKY ? (T)(1.0 / 7) : (T)(-33.0 / 2048)};
   T P[D][D];
 #pragma unroll
   for (int i = 0; i < D; i++)
@@ -702,7 +703,10 @@ __device__ __forceinline__ bool kirchhoff_fast(int model, T mu, T la, T thr, T v
     // the log series converges more slowly: a quarter of the radius^2
     if (r2 <= (hencky ? SeriesRadius<T>::r2 * (T)0.25 : SeriesRadius<T>::r2)) {
       T f[D][D];
-      corot_series<T, D>(E, f, hencky);
+      if (hencky)
+        corot_series<T, D, true>(E, f);
+      else
+        corot_series<T, D, false>(E, f);
       const T pj = hencky ? la * tlog1p<T>(Jm1) : la * Jm1 * J, m2 = hencky ? mu : (T)2 * mu;
 #pragma unroll
       for (int i = 0; i < D; i++)

@@ -174,6 +174,9 @@ template<typename T, int D> __device__ __forceinline__ uint32_t base_tile(const 
 // by P2G each step and transferred through HBM by p2g_extra_kernel and
 // g2p_mover_kernel, and the tiles holding them are marked in mstamp so that the
 // grid tiles they touch are cleared and updated.
+// Binning key: the tile holding the particle's stencil base times 64 plus the
+// base's cell inside that tile, so that the sort groups particles by tile
+// and, inside a tile, by cell.
 template<typename T, int D>
 __global__ __launch_bounds__(BLOCK) void key_kernel(Args<T, D> a, uint32_t* range_err) {
   const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
@@ -183,10 +186,40 @@ __global__ __launch_bounds__(BLOCK) void key_kernel(Args<T, D> a, uint32_t* rang
     T x[D];
 #pragma unroll
     for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
-    key = tile_key<T, D>(particle_geom(a, p), x);
-    if (key == INVALID_KEY) atomicAdd(range_err, 1u);
+    const Geom<T, D> g = particle_geom(a, p);
+    key = tile_key<T, D>(g, x);
+    if (key == INVALID_KEY) {
+      atomicAdd(range_err, 1u);
+    } else {
+      uint32_t cell = 0;
+#pragma unroll
+      for (int c = 0; c < D; c++) cell = cell * Tile<D>::E + (uint32_t)((stencil_base<T, D>(g, x, c) + 3) % Tile<D>::E);
+      key = key * 64u + cell;
+    }
   }
   a.keys[p] = key;
+}
+
+// Stratified order inside each tile.  The sort leaves a tile's particles
+// grouped by cell; with K = round(n_t / 64) (the tile's particles per cell),
+// position j of the tile goes to stratum j % K, and the strata are laid out
+// one after another: a run of 64 consecutive positions then holds about one
+// particle of every cell, so the fused kernel's 256-particle chunks hold about
+// four of each (one round of its P2G).  A bijection on the tile's positions.
+__global__ __launch_bounds__(BLOCK) void stratify_kernel(const uint32_t* sk, const uint32_t* order_in,
+                                                         uint32_t* order_out, const uint32_t* tb,
+                                                         const uint32_t* te, uint32_t n) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t key = sk[i];
+  if (key == INVALID_KEY) {
+    order_out[i] = order_in[i];
+    return;
+  }
+  const uint32_t t = key >> 6, b = tb[t], nt = te[t] - b, j = i - b;
+  const uint32_t K = max(1u, (nt + 32u) >> 6);
+  const uint32_t Q = nt / K, Rm = nt % K, sidx = j % K, q = j / K;
+  order_out[b + sidx * Q + min(sidx, Rm) + q] = order_in[i];
 }
 
 __global__ __launch_bounds__(BLOCK) void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
@@ -200,13 +233,13 @@ __global__ __launch_bounds__(BLOCK) void bounds_kernel(const uint32_t* sk, uint3
                                                        uint32_t* stamp_arr, uint32_t stamp) {
   const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
-  const uint32_t t = sk[i];
-  if (t == INVALID_KEY) return;
-  if (i == 0 || sk[i - 1] != t) {
+  if (sk[i] == INVALID_KEY) return;
+  const uint32_t t = sk[i] >> 6;  // keys are tile * 64 + cell
+  if (i == 0 || (sk[i - 1] >> 6) != t) {
     tb[t] = i;
     stamp_arr[t] = stamp;
   }
-  if (i == n - 1 || sk[i + 1] != t) te[t] = i + 1;
+  if (i == n - 1 || sk[i + 1] == INVALID_KEY || (sk[i + 1] >> 6) != t) te[t] = i + 1;
 }
 
 // Grid-tile list in two launches, no device-wide scan: gcount_kernel counts
@@ -1304,6 +1337,237 @@ __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
     g2p_stage<T, D, FRICTION>(a, g, org, &s_v[0][0]);
     __syncthreads();
     g2p_tile_pass<T, D, FRICTION, false>(a, t, g, org, &s_v[0][0]);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------- fused step --
+// One workgroup per binned tile (grid-stride over the tile list): this step's
+// G2P and the next step's P2G of the tile's particles, whose state stays on
+// chip in between.  The particles are taken 256 at a time, one per thread, in
+// the stratified bin order (a chunk holds about four particles of each cell):
+//   G2P:      thread t gathers its particle from the tile's staged grid
+//             region, stores X, V, F, B and, if the new stencil base stays in
+//             the tile, computes the particle's P2G coefficients (stress from
+//             the new F, m v, m B D^-1, dt tau, x - x_base) in registers;
+//   grouping: wave-ballot ranks of the new base cells (stable), per-wave
+//             counts and offsets in LDS;
+//   rounds:   in round k the threads whose rank r in their cell has r / 4 = k
+//             write their coefficients to slot (r % 4) * 64 + cell, then lane
+//             l of wave w adds slot w * 64 + l into the wave's copy of the
+//             tile's region (p2g_scatter_slot: the 64 lanes of an instruction
+//             hold 64 distinct cells, so no two touch the same node);
+//   flush:    the copies are summed and added to grid_next, one atomic per
+//             touched node and component.
+// Particles of the current mover list (base outside the tile) are left to
+// g2p_mover_kernel<FUSED>; particles whose new base leaves the tile are
+// appended to the next mover list, and their P2G is left to
+// p2g_extra_kernel<FUSED>, which also adds the stress impulse of particles
+// whose stress needs the FP64 SVD (queued here with tau = 0).
+template<typename T, int D, bool FRICTION>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (FRICTION ? 2 : 3) : 1))) void g2p2g_kernel(
+    Args<T, D> a) {
+  constexpr int R = Tile<D>::R, RN = Tile<D>::RN, NW = BLOCK / 64, E = Tile<D>::E;
+  using C = P2GCoef<T, D>;
+  using T4 = typename Vec4Type<T>::type;
+  __shared__ T4 s_reg[NW][RN];              // per-wave copies of the region: (m, p_x, p_y, p_z)
+  __shared__ T4 s_v[FRICTION ? 2 : 1][RN];  // staged grid of the G2P half (and friction velocities)
+  __shared__ T s_c[C::N][BLOCK];            // coefficients of one round: slot (r % 4) * 64 + cell
+  __shared__ uint32_t s_wc[NW][64];         // per-wave cell counts, then offsets
+  __shared__ uint32_t s_cn[64], s_max;
+  const uint32_t ntile = a.counts[0];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (uint32_t seg = blockIdx.x; seg < ntile; seg += gridDim.x) {
+    const uint32_t t = a.ptiles[seg];
+    const Geom<T, D> g = tile_geom(a, t);
+    int org[D];
+    tile_origin<T, D>(t, g, org);
+    g2p_stage<T, D, FRICTION>(a, g, org, &s_v[0][0]);
+    for (int k = threadIdx.x; k < NW * RN; k += BLOCK) {
+      T4 z = {0, 0, 0, 0};
+      s_reg[k / RN][k % RN] = z;
+    }
+    __syncthreads();
+    const uint32_t tb = a.tile_begin[t], te = a.tile_end[t];
+    for (uint32_t c0 = tb; c0 < te; c0 += BLOCK) {
+      // ---- G2P of one particle per thread; P2G coefficients kept in registers
+      uint32_t cell = 64;
+      T cf[C::N];
+      const uint32_t i = c0 + threadIdx.x;
+      if (i < te) {
+        const uint32_t p = a.order ? a.order[i] : i;
+        if (a.valid[p]) {
+          T x[D];
+#pragma unroll
+          for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+          Wts<T, D> w;
+          weights<T, D>(g, x, w);
+          bool in = true;
+          int lb[D];
+#pragma unroll
+          for (int c = 0; c < D; c++) {
+            lb[c] = w.base[c] - org[c];
+            in = in && lb[c] >= 0 && lb[c] < E;
+          }
+          if (in) {  // otherwise a mover of the current list
+            const int li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
+            const T4* sv = &s_v[0][0];
+            auto comp = [](const T4& q, int c) -> T { return c == 0 ? q.y : (c == 1 ? q.z : q.w); };
+            auto nv = [&](int ox, int oy, int oz, int c) -> T {
+              return comp(sv[li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)], c);
+            };
+            auto nf = [&](int ox, int oy, int oz, int c) -> T {
+              if constexpr (FRICTION) return comp(sv[RN + li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)], c);
+              else return nv(ox, oy, oz, c);
+            };
+            T vb[D], vfb[D], G[D * D], Bs[D * D];
+            g2p_gather<T, D, FRICTION>(g, a.dt, w, nv, nf, vb, vfb, G, Bs);
+            // F <- (I + dt grad v) F, x <- x + dt v, v <- vf, B; invalidation (MPM_DRIVER.cpp:450-515, :1219-1239)
+            T Fn[D * D];
+            {
+              T Fm[D * D];
+#pragma unroll
+              for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
+#pragma unroll
+              for (int j = 0; j < D; j++)
+#pragma unroll
+                for (int c = 0; c < D; c++) {
+                  T sacc = Fm[c + D * j];
+#pragma unroll
+                  for (int q = 0; q < D; q++) sacc += a.dt * G[c + D * q] * Fm[q + D * j];
+                  Fn[c + D * j] = sacc;
+                }
+            }
+            T xn[D];
+            bool ok = true;
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+              xn[c] = x[c] + a.dt * vb[c];
+              if (xn[c] < g.vlo[c] || xn[c] > g.vhi[c]) ok = false;
+              a.X[c * a.n + p] = xn[c];
+              a.V[c * a.n + p] = vfb[c];
+            }
+#pragma unroll
+            for (int k = 0; k < D * D; k++) {
+              a.F[k * a.n + p] = Fn[k];
+              a.B[k * a.n + p] = Bs[k];
+            }
+            if (!ok) {
+              a.valid[p] = 0;
+            } else {
+              // the new base: in this tile (P2G here) or a mover of the next step
+              bool stay = true;
+              uint32_t cl = 0, nt = 0;
+#pragma unroll
+              for (int c = 0; c < D; c++) {
+                const int nb = stencil_base<T, D>(g, xn, c) + 3;
+                nt = nt * (uint32_t)g.GT[c] + (uint32_t)(nb / E);
+                const int q = nb - org[c];
+                stay = stay && q >= 0 && q < E;
+                cl = cl * E + (uint32_t)(q & (E - 1));
+              }
+              if (!stay) {
+                nt += g.tile0;
+                if (nt < a.ntotal) a.mstamp[nt] = a.next_stamp;
+                const uint32_t k = atomicAdd(&a.counts[5], 1u);  // aggregated per wavefront by the compiler
+                a.mover_list2[k] = p;
+                a.mover_tile2[k] = t;
+              } else {
+                cell = cl;
+                const T m = a.mass[p];
+                T tau[D * D];
+                if (!kirchhoff_fast<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.vol[p], Fn, tau)) {
+#pragma unroll
+                  for (int q = 0; q < D * D; q++) tau[q] = 0;
+                  a.slow_list[atomicAdd(&a.counts[3], 1u)] = p;  // the compiler aggregates this per wavefront
+                }
+                cf[C::M] = m;
+#pragma unroll
+                for (int c = 0; c < D; c++) {
+                  cf[C::MV + c] = m * vfb[c];
+                  const int bb = stencil_base<T, D>(g, xn, c);
+                  cf[C::XE + c] = xn[c] - (g.lo[c] + ((T)bb + (T)0.5) * g.dx[c]);
+                }
+#pragma unroll
+                for (int j = 0; j < D; j++) {
+                  const T dinv = (T)4 * g.inv_dx[j] * g.inv_dx[j];  // D_p^-1 (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:152-158)
+#pragma unroll
+                  for (int c = 0; c < D; c++) cf[C::MC + c + D * j] = m * Bs[c + D * j] * dinv;
+                }
+                int q = 0;
+#pragma unroll
+                for (int c = 0; c < D; c++)
+#pragma unroll
+                  for (int j = c; j < D; j++) cf[C::Q + q++] = a.dt * tau[c + D * j];
+              }
+            }
+          }
+        }
+      }
+      // ---- stable ranks of the new base cells
+      s_wc[wv][lane] = 0;  // this wave's row only: the wave's own writes below come after it
+      const bool act = cell < 64;
+      uint64_t mm = __ballot(act);
+#pragma unroll
+      for (int bb = 0; bb < 6; bb++) {
+        const bool bit = (cell >> bb) & 1u;
+        const uint64_t bal = __ballot(bit);
+        mm &= bit ? bal : ~bal;
+      }
+      uint32_t rank = (uint32_t)__popcll(mm & lt_mask);
+      if (act && lane == __ffsll((unsigned long long)mm) - 1) s_wc[wv][cell] = (uint32_t)__popcll(mm);
+      __syncthreads();
+      if (threadIdx.x < 64) {  // wave 0: per-cell offsets over the waves, cell totals and the largest
+        uint32_t run = 0;
+#pragma unroll
+        for (int w2 = 0; w2 < NW; w2++) {
+          const uint32_t v = s_wc[w2][threadIdx.x];
+          s_wc[w2][threadIdx.x] = run;
+          run += v;
+        }
+        uint32_t mx = run;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o, 64));
+        s_cn[threadIdx.x] = run;
+        if (threadIdx.x == 0) s_max = mx;
+      }
+      __syncthreads();
+      if (act) rank += s_wc[wv][cell];
+      const uint32_t b_cnt = s_cn[lane];
+      const uint32_t rounds = (s_max + NW - 1) / NW;
+      for (uint32_t k = 0; k < rounds; k++) {
+        if (act && rank / NW == k) {
+          const int sl = (int)(rank % NW) * 64 + (int)cell;
+#pragma unroll
+          for (int q = 0; q < C::N; q++) s_c[q][sl] = cf[q];
+        }
+        __syncthreads();
+        if (k * NW + wv < b_cnt) {
+          int lcell[D];
+          cell_offsets<D>((uint32_t)lane, lcell);
+          const int lane_li = D == 3 ? (lcell[0] * R + lcell[1]) * R + lcell[D - 1] : lcell[0] * R + lcell[D - 1];
+          p2g_scatter_slot<T, D>(g, s_c, threadIdx.x, lane_li, s_reg[wv]);
+        }
+        __syncthreads();
+      }
+    }
+    for (int k = threadIdx.x; k < RN; k += BLOCK) {
+      T4 sum = s_reg[0][k];
+#pragma unroll
+      for (int w2 = 1; w2 < NW; w2++) sum += s_reg[w2][k];
+      if (sum.x != (T)0) {
+        int off[D], nc[D];
+        region_offsets<D>(k, off);
+#pragma unroll
+        for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
+        Node4<T>* nd = a.grid_next + node_slot<T, D>(nc, g);
+        atomic_add(&nd->m, sum.x);
+        atomic_add(&nd->x, sum.y);
+        atomic_add(&nd->y, sum.z);
+        if constexpr (D == 3) atomic_add(&nd->z, sum.w);
+      }
+    }
     __syncthreads();
   }
 }
@@ -4016,7 +4280,7 @@ class Engine final : public EngineBase {
     keys_.release(); skeys_.release(); order_.release(); ktmp_.release(); vtmp_.release();
     keys_.alloc(n); skeys_.alloc(n); order_.alloc(n); ktmp_.alloc(n); vtmp_.alloc(n);
     sort_scratch_.release();
-    sort_scratch_.alloc(sort_scratch_bytes((uint32_t)n, key_bits_));
+    sort_scratch_.alloc(sort_scratch_bytes((uint32_t)n, key_bits_ + 6));
     stage_.release();
     stage_.alloc(n * D * D);
     slow_list_.release();
@@ -4199,10 +4463,12 @@ class Engine final : public EngineBase {
     hipEvent_t e0 = timer_.begin(stream_);
     Args<T, D> a = make_args(0);
     key_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(a, err_.p + 1);
-    radix_sort_pairs(keys_.p, skeys_.p, order_.p, ktmp_.p, vtmp_.p, n_, key_bits_, sort_scratch_.p, stream_);
+    radix_sort_pairs(keys_.p, skeys_.p, vtmp_.p, ktmp_.p, order_.p, n_, key_bits_ + 6, sort_scratch_.p, stream_);
     bin_stamp_++;
     bounds_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(skeys_.p, n_, tile_begin_.p, tile_end_.p, tile_stamp_.p,
                                                           bin_stamp_);
+    stratify_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(skeys_.p, vtmp_.p, order_.p, tile_begin_.p, tile_end_.p,
+                                                            n_);
     const uint32_t nb = cdiv(ntotal_, BLOCK);
     gcount_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, 0u, gblock_.p, 0);
     glist_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, 0u, gblock_.p, ptiles_.p,
@@ -4596,9 +4862,9 @@ class Engine final : public EngineBase {
     hipEvent_t e3 = timer_.begin(stream_);
     hipEvent_t ek = timer_.begin(stream_);  // the fused kernel alone
     if (fric)
-      p2g_kernel<T, D, true, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+      g2p2g_kernel<T, D, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
     else
-      p2g_kernel<T, D, true, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+      g2p2g_kernel<T, D, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
     timer_.end(PH_FUSED, ek, stream_);
     if (fric)
       g2p_mover_kernel<T, D, true, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);

@@ -13,8 +13,8 @@ REPORT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
                       "mpmx_step.resources.txt")
 # kernel (mangled name prefix) -> largest scratch bytes per lane allowed
 HOT = {
-    "_ZN4mpmx10p2g_kernelIfLi3ELb1ELb0E": 0,   # fused G2P -> P2G, the step's main kernel
-    "_ZN4mpmx10p2g_kernelIfLi3ELb1ELb1E": 0,   # the same with wall friction
+    "_ZN4mpmx12g2p2g_kernelIfLi3ELb0E": 0,     # fused G2P -> P2G, the step's main kernel
+    "_ZN4mpmx12g2p2g_kernelIfLi3ELb1E": 0,     # the same with wall friction
     "_ZN4mpmx10p2g_kernelIfLi3ELb0ELb0E": 8,   # unfused P2G (first step of a run)
     "_ZN4mpmx10g2p_kernelIfLi3ELb0E": 0,
     "_ZN4mpmx10g2p_kernelIfLi3ELb1E": 0,
