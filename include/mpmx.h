@@ -147,7 +147,8 @@ int mpmx_synchronize(mpmx_ctx* ctx);
    each call must take the next stage (MPMX_ESTATE otherwise, and mpmx_step
    fails while a step is open) and returns after its GPU work.  Such a step
    runs the unfused kernels.  Between stages 3 and 4, mpmx_get_grid returns
-   P2G's velocities (momentum / mass), as example.velocity holds them there.
+   P2G's velocities (momentum / mass, before any force), as example.velocity
+   holds them there.
    mpmx_stage_name returns the stage's name, or NULL. */
 #define MPMX_STAGES 8
 int mpmx_step_stage(mpmx_ctx* ctx, double dt, int stage);
@@ -194,11 +195,14 @@ int mpmx_particle_totals(mpmx_ctx* ctx, mpmx_totals* out);
 
 /* Grid totals of the last step, recorded by every step while grid stats are
    enabled (mpmx_enable_grid_stats): which = 0 after Particle_To_Grid (node
-   velocities p/m; with reflection boundary conditions before the mass and
-   momentum reflection), which = 1 after the grid update (the velocity G2P
-   reads): Total_Grid_Linear_Momentum, Total_Grid_Angular_Momentum and
+   velocities p/m before any force, with the reflection boundary conditions'
+   mass and momentum reflection, as example.velocity holds them after the
+   "p2g" step), which = 1 after the grid update (the velocity G2P reads):
+   Total_Grid_Linear_Momentum, Total_Grid_Angular_Momentum and
    Total_Grid_Kinetic_Energy (MPM_EXAMPLE.cpp:226-241, :268-297) over the
-   nodes with nonzero mass, node locations the MAC cell centres. */
+   nodes with nonzero mass, node locations the MAC cell centres.  While grid
+   stats are enabled every step takes the unfused kernels and one more P2G
+   (without the elastic impulse) for the which = 0 totals. */
 int mpmx_grid_totals(mpmx_ctx* ctx, int which, mpmx_totals* out);
 
 /* Dense grid arrays of the last step, indexed like MPM_EXAMPLE::mass and

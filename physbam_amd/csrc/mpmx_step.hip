@@ -3603,7 +3603,9 @@ class Engine final : public EngineBase {
     if (nsteps < 0) throw Error{MPMX_EINVAL, "negative step count"};
     // recorded steps too (the reverse pass recomputes its own); plasticity
     // projects F between G2P and the next P2G, which the fused kernel does not
-    const bool fuse = n_ > 0 && !plastic_ && !limits_on() && !refl_on_;
+    // grid statistics read each step's P2G grid, which the fused kernel never
+    // holds on its own
+    const bool fuse = n_ > 0 && !plastic_ && !limits_on() && !refl_on_ && !stats_on_;
     for (int s = 0; s < nsteps; s++) {
       if (fuse)
         step_fused((T)dt);
@@ -3796,6 +3798,7 @@ class Engine final : public EngineBase {
       a.gtiles = gt_other_;
       a.counts = lastcnt_.p;
     }
+    if (momentum) a.grid = vsave_.p;  // between "p2g" and "forces": Particle_To_Grid's own (m, m v)
     export_grid_kernel<T, D><<<grid_blocks(), BLOCK, 0, stream_>>>(a, dm.p, dv.p, momentum ? 1 : 0);
     if (mass) HIPCHECK(hipMemcpyAsync(mass, dm.p, nn * sizeof(T), hipMemcpyDeviceToHost, stream_));
     if (velocity) HIPCHECK(hipMemcpyAsync(velocity, dv.p, nn * D * sizeof(T), hipMemcpyDeviceToHost, stream_));
@@ -4512,6 +4515,7 @@ class Engine final : public EngineBase {
   struct OpenStep {
     Args<T, D> a;
     bool fric = false, empty = false;
+    bool p2g_copy = false;  // vsave_ holds Particle_To_Grid's (m, m v) of this step
     T dt = 0, cfl = 0, min_dt = 0;
     hipEvent_t e_step = nullptr;
     uint32_t ms = 0;
@@ -4554,6 +4558,15 @@ class Engine final : public EngineBase {
         hipEvent_t e1 = timer_.begin(stream_);
         p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
         p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+        // The grid above holds P2G's momentum plus the elastic impulse of the
+        // forces step.  Grid statistics and a step run stage by stage need
+        // Particle_To_Grid's own result (MPM_DRIVER.cpp:245-306, reflection
+        // of m and m v included): the same P2G with dt = 0 into vsave_.
+        s.p2g_copy = stats_on_ || stage_next_ != 0;
+        if (s.p2g_copy) {
+          p2g_momentum_copy(a);
+          if (stats_on_) record_grid_totals(a, vsave_.p, 0);
+        }
         timer_.end(PH_P2G, e1, stream_);
         if (!replaying_) {  // the mover count for the rebin policy; a reverse pass's replay keeps the host free
           const int slot = (int)(step_count_ & 3);
@@ -4568,7 +4581,6 @@ class Engine final : public EngineBase {
         const Args<T, D>& a = s.a;
         const T dt = s.dt;
         hipEvent_t e2 = timer_.begin(stream_);
-        if (stats_on_) record_grid_totals(a, a.grid, 0);
         Args<T, D> ag = a;  // the grid update's arguments
         if (refl_on_ && !refl_full_) {
           // gravity into the P2G sums, then Reflect_Boundary_Mass_Momentum with
@@ -4741,6 +4753,26 @@ class Engine final : public EngineBase {
     std::memcpy(out, lim_host_, sizeof(T));
     std::memcpy(out + 1, lim_host_ + 1, sizeof(T));
   }
+  // Particle_To_Grid's (m, m v) of the step whose arguments are `a` into
+  // vsave_ (the P2G again with dt = 0: no elastic impulse), with
+  // Reflect_Boundary_Mass_Momentum applied as the reference applies it inside
+  // Particle_To_Grid (MPM_DRIVER.cpp:276-280).  The P2G rebuilds the slow and
+  // mover lists, so they start empty.
+  void p2g_momentum_copy(const Args<T, D>& a) {
+    vsave_.alloc((size_t)ntotal_ * 64);
+    clear_tiles_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(vsave_.p, a.gtiles, counts_.p + 1);
+    fill_u32_kernel<<<1, BLOCK, 0, stream_>>>(counts_.p + 3, 2u, 0u);
+    Args<T, D> b = a;
+    b.dt = 0;
+    b.grid = vsave_.p;
+    p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(b);
+    p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(b);
+    if (refl_on_ && !refl_full_)
+      for (int k2 = 0; k2 < D; k2++)
+        reflect_mass_momentum_kernel<T, D><<<reflect_blocks(k2), BLOCK, 0, stream_>>>(vsave_.p, scenes_dev_.p,
+                                                                                        reflect_axis(k2));
+    HIPCHECK(hipGetLastError());
+  }
   // MPM_DRIVER::Reduce_Dt (MPM_DRIVER.cpp:519-545) in T after the grid
   // update of a step with a.dt; returns the dt G2P takes and, when it is
   // lowered below 0.99 dt, rescales the valid nodes' velocities.
@@ -4850,12 +4882,10 @@ class Engine final : public EngineBase {
     a.next_stamp = ms_next;
 
     hipEvent_t e2 = timer_.begin(stream_);
-    if (stats_on_) record_grid_totals(a, a.grid, 0);
     if (fric)
       grid_kernel<T, D, true><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
     else
       grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
-    if (stats_on_) record_grid_totals(a, a.grid, 1);
     clear_tiles_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(gother_, gt_other_, lastcnt_.p + 1);
     timer_.end(PH_GRID, e2, stream_);
 

@@ -32,7 +32,10 @@ class MPMRollout(torch.autograd.Function):
         sc = sim.sc
         sim.ctx.set_particles(sc["X"], V0.detach().cpu().numpy(), sc["F"], sc["B"], sc["mass"], sc["volume"],
                               mu.detach().cpu().numpy(), lam.detach().cpu().numpy(), sc["valid"])
-        sim.ctx.tape_begin(sim.checkpoint)
+        # record a tape only when a gradient is wanted: a forward under
+        # torch.no_grad() (a finite-difference probe) leaves no open tape behind
+        if any(fctx.needs_input_grad):
+            sim.ctx.tape_begin(sim.checkpoint)
         sim.ctx.step(sc["dt"], sim.steps)
         fctx.sim = sim
         X = sim.ctx.get_particles()["X"]

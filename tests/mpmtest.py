@@ -98,6 +98,20 @@ def make_scene(dim, N, prec=8, model=COROTATED, shape=None, ppc_axis=2, seed=123
     return sc
 
 
+def b_velocity_error(gotB, refB, refV, dx, dim=3):
+    """B's error measured by the velocity it carries into the next P2G.  A
+    particle adds B D^-1 (x_i - x_p) to the velocity of each stencil node
+    (MPM_DRIVER.cpp:263-266), with D^-1 = 4/dx^2 and |x_i - x_p| <= 1.5 dx per
+    axis, so an error dB moves a node velocity by at most (6/dx) max_c
+    sum_j |dB_cj|; returned relative to the largest particle speed.  This is
+    the yardstick when the exact B is ~0 (a body in uniform motion), where
+    max|dB| / max|B| only measures rounding noise against noise."""
+    d = np.abs(np.asarray(gotB, np.float64) - np.asarray(refB, np.float64)).reshape(-1, dim, dim)
+    worst = d.sum(1).max() if d.size else 0.0  # column-major: entry (c, j) at j*dim + c, sum over j
+    vmax = max(np.abs(np.asarray(refV, np.float64)).max(), 1e-30)
+    return float(6.0 / dx * worst / vmax)
+
+
 def make_rotating(dim, N, prec=8, model=COROTATED, omega=None, center=None, radius=0.3, E=1e3, nu=0.3,
                   rho=2000.0, seed=1234, dt=1e-3, gravity=None, walls=None):
     """A rotating ball, as the reference's rotating-sphere and rotating-circle

@@ -81,7 +81,8 @@ def test_batched_workload_at_full_shape(batched_workload):
     one sphere each (the generator the benchmark times), fixed corotated,
     gravity, FP32, dt 1e-4, in one context.  After 3 steps the smallest and
     the largest scene of each grid size match their own FP64 oracle runs to
-    1e-4 (X, V, F, B; valid flags equal); after 10 steps no particle is lost
+    1e-4 (X, V, F; B by the velocity its error carries into the next P2G;
+    valid flags equal); after 10 steps no particle is lost
     and each scene's momentum is M g t (no walls: the internal forces and the
     APIC transfers add no momentum), to 1e-5 of M |g| t."""
     sizes, parts = batched_workload
@@ -117,7 +118,10 @@ def test_batched_workload_at_full_shape(batched_workload):
         part = {k: at3[k][offs[s]:offs[s + 1]] for k in ("X", "V", "F", "B", "valid")}
         assert np.array_equal(part["valid"], orc["valid"]), s
         errs = {k: float(np.abs(part[k] - orc[k]).max() / max(np.abs(orc[k]).max(), 1e-30))
-                for k in ("X", "V", "F", "B")}
+                for k in ("X", "V", "F")}
+        # the spheres start at rest and fall freely, so their exact B is ~0:
+        # B is judged by the velocity its error carries into the next P2G
+        errs["B"] = mt.b_velocity_error(part["B"], orc["B"], orc["V"], 1.0 / sizes[s])
         print(f"scene {s} (N {sizes[s]}, {counts[s]} particles):", {k: f"{v:.1e}" for k, v in errs.items()},
               flush=True)
         for k, v in errs.items():

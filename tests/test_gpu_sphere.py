@@ -6,8 +6,9 @@ inset 3 cells on all six sides, dt 5e-5 (MPM_DRIVER.cpp:141-157 with
 NEO_HOOKEAN.cpp:44-81 and the slip projection of MPM_OBJECTIVE.cpp:150-182).
 
 - Three fused FP32 steps match the FP64 oracle run of the same float inputs
-  (the exact answer for them): X, V, F, B to 1e-5 relative (max|a - b| /
-  max|b| per quantity), valid flags equal.
+  (the exact answer for them): X, V, F to 1e-5 relative (max|a - b| /
+  max|b| per quantity), B to 1e-5 by the velocity its error carries into the
+  next P2G (the uniformly moving sphere's exact B is ~0), valid flags equal.
 - The 500 steps of the workload in FP32 against the same 500 steps in FP64 on
   the GPU: the drift of each quantity is printed and bounded, particle mass is
   conserved and no particle is lost.
@@ -58,8 +59,12 @@ def test_three_fused_steps_match_fp64_oracle(sphere):
         got = ctx.get_particles()
     print("oracle: 3 FP64 steps of the full sphere", flush=True)
     ref = mt.oracle_run(sc, 3, prec=8, want_grid=False)
-    errs = {k: rel(got[k], ref[k]) for k in ("X", "V", "F", "B")}
-    print("FP32 GPU vs FP64 oracle after 3 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    errs = {k: rel(got[k], ref[k]) for k in ("X", "V", "F")}
+    # the sphere moves uniformly, so its exact B is ~0: B is judged by the
+    # velocity its error carries into the next P2G
+    errs["B"] = mt.b_velocity_error(got["B"], ref["B"], ref["V"], 1.0 / bench.SPHERE_CELLS)
+    print("FP32 GPU vs FP64 oracle after 3 steps:", {k: f"{v:.2e}" for k, v in errs.items()},
+          f"(B relative to max|B|: {rel(got['B'], ref['B']):.2e})", flush=True)
     assert np.array_equal(got["valid"], ref["valid"])
     for k, v in errs.items():
         assert v <= 1e-5, (k, v)
@@ -79,7 +84,9 @@ def test_500_steps_fp32_against_fp64(sphere):
             out[prec] = ctx.get_particles()
             tot[prec] = (t0, ctx.particle_totals())
         print(f"precision {prec}: {bench.SPHERE_STEPS} steps", flush=True)
-    errs = {k: rel(out[mpmx.FP32][k], out[mpmx.FP64][k]) for k in ("X", "V", "F", "B")}
+    errs = {k: rel(out[mpmx.FP32][k], out[mpmx.FP64][k]) for k in ("X", "V", "F")}
+    errs["B"] = mt.b_velocity_error(out[mpmx.FP32]["B"], out[mpmx.FP64]["B"], out[mpmx.FP64]["V"],
+                                    1.0 / bench.SPHERE_CELLS)
     print(f"FP32 vs FP64 after {bench.SPHERE_STEPS} steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     n = P["X"].shape[0]
     for prec in (mpmx.FP32, mpmx.FP64):
@@ -87,5 +94,5 @@ def test_500_steps_fp32_against_fp64(sphere):
         assert t1[11] == n  # no particle lost
         assert abs(t1[0] - t0[0]) <= 1e-12 * t0[0]
         assert np.all(out[prec]["valid"] == 1)
-    # X, V, F within 1e-4 and B within 1e-3 of the FP64 run after 500 steps
+    # X, V, F within 1e-4 and B (by the velocity it carries) within 1e-3 of the FP64 run after 500 steps
     assert errs["X"] <= 1e-5 and errs["V"] <= 1e-4 and errs["F"] <= 1e-4 and errs["B"] <= 1e-3, errs

@@ -107,16 +107,18 @@ ROTATING = [  # dim, prec, model, omega, cells
 @pytest.mark.parametrize("dim,prec,model,omega,N", ROTATING)
 def test_rotating_body_conserves_angular_momentum(dim, prec, model, omega, N, fused):
     """The reference's rotating-ball scenes (no gravity, no walls) over 40
-    steps: the particle angular momentum (with its -m B:eps term), the grid's
-    after P2G and after the update, mass and linear momentum stay constant --
-    to 1e-12 relative in FP64 and to FP32 rounding (1e-5) in FP32 -- through
-    the fused step and through the unfused kernels (which the reflection
-    boundary conditions take; all sides free here, far from the ball)."""
+    steps: the particle angular momentum (with its -m B:eps term), mass and
+    linear momentum stay constant -- to 1e-12 relative in FP64 and to FP32
+    rounding (1e-5) in FP32.  Through the fused step (the particle totals after
+    every step; the phase timers show the fused kernel ran) and through the
+    unfused kernels, which every step takes while grid stats are enabled (then
+    also the grid's totals after P2G and after the update)."""
     sc = mt.make_rotating(dim, N, prec=prec, model=model, omega=omega, dt=2e-3, seed=41)
     ctx = gpu_context(sc)
-    if not fused:
-        ctx.set_reflection([mpmx.BC_FREE] * (2 * dim))
-    ctx.enable_grid_stats()
+    if fused:
+        ctx.enable_timing(True)
+    else:
+        ctx.enable_grid_stats()
     t0 = ctx.particle_totals()
     tol = 1e-12 if prec == 8 else 1e-5
     tol_m = 1e-12 if prec == 8 else 1e-6  # FP32 node masses are float sums
@@ -128,12 +130,19 @@ def test_rotating_body_conserves_angular_momentum(dim, prec, model, omega, N, fu
     worst = 0.0
     for k in range(40):
         ctx.step(sc["dt"], 1)
-        for t in (ctx.grid_totals(0), ctx.grid_totals(1)):
+        tots = [ctx.particle_totals()]
+        if not fused:
+            tots += [ctx.grid_totals(0), ctx.grid_totals(1)]
+        for t in tots:
             worst = max(worst, np.abs(t[4:4 + ns] - t0[4:4 + ns]).max() / L)
             assert abs(t[0] - M) <= tol_m * M
             assert np.abs(t[1:1 + dim]).max() <= tol * P
     t1 = ctx.particle_totals()
     worst = max(worst, np.abs(t1[4:4 + ns] - t0[4:4 + ns]).max() / L)
+    if fused:
+        ctx.synchronize()
+        assert ctx.phase_time("g2p2g")[1] >= 39  # every step after the first ran the fused kernel
+        assert ctx.phase_time("g2p")[1] == 0
     ctx.close()
     print(f"rotating d{dim} p{prec} m{model} fused={fused}: angular momentum drift {worst:.2e}", flush=True)
     assert abs(t1[0] - M) <= 1e-12 * M
