@@ -111,12 +111,26 @@ MPMX_HD uint32_t tile_key(const Geom<T, D>& g, const T* X) {
   return g.tile0 + tile;
 }
 
-// Offsets of cell `l` (0..63) inside its tile, row-major.
+// Label (0..63) of the cell with offsets o inside its tile, and back.  The
+// kernels give lane l of a wavefront the cell labelled l, and each lane then
+// reads or writes 16-byte nodes of a 6x6x6 region at 36 x + 6 y + z plus an
+// offset common to all lanes.  3D labels therefore run z fastest, then x, then
+// y: 16 consecutive labels (one y, x = 0..3, z = 0..3) fall on nodes
+// 36 x + z apart, distinct modulo 16 since 36 = 4 (mod 16), so the 16 lanes
+// that one LDS pass serves hit distinct banks (row-major labels put lanes
+// 6 apart and collide in pairs).  2D labels are row-major.
+template<int D> MPMX_HD uint32_t cell_label(const int* o) {
+  if constexpr (D == 3) return (uint32_t)((o[1] * 4 + o[0]) * 4 + o[2]);
+  else return (uint32_t)(o[0] * Tile<D>::E + o[1]);
+}
 template<int D> MPMX_HD void cell_offsets(uint32_t l, int* o) {
-#pragma unroll
-  for (int a = D - 1; a >= 0; a--) {
-    o[a] = (int)(l % Tile<D>::E);
-    l /= Tile<D>::E;
+  if constexpr (D == 3) {
+    o[2] = (int)(l & 3u);
+    o[0] = (int)((l >> 2) & 3u);
+    o[1] = (int)(l >> 4);
+  } else {
+    o[1] = (int)(l % Tile<D>::E);
+    o[0] = (int)(l / Tile<D>::E);
   }
 }
 

@@ -191,10 +191,10 @@ __global__ __launch_bounds__(BLOCK) void key_kernel(Args<T, D> a, uint32_t* rang
     if (key == INVALID_KEY) {
       atomicAdd(range_err, 1u);
     } else {
-      uint32_t cell = 0;
+      int lc[D];
 #pragma unroll
-      for (int c = 0; c < D; c++) cell = cell * Tile<D>::E + (uint32_t)((stencil_base<T, D>(g, x, c) + 3) % Tile<D>::E);
-      key = key * 64u + cell;
+      for (int c = 0; c < D; c++) lc[c] = (stencil_base<T, D>(g, x, c) + 3) % Tile<D>::E;
+      key = key * 64u + cell_label<D>(lc);
     }
   }
   a.keys[p] = key;
@@ -603,13 +603,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
 #pragma unroll
             for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
             bool in = true;
-            uint32_t cl = 0;
+            int lc[D];
 #pragma unroll
             for (int c = 0; c < D; c++) {
               const int lb = stencil_base<T, D>(g, x, c) + 3 - org[c];
               in = in && lb >= 0 && lb < E;
-              cl = cl * E + (uint32_t)(lb & (E - 1));
+              lc[c] = lb & (E - 1);
             }
+            const uint32_t cl = cell_label<D>(lc);
             if (in) {
               cell = cl;
             } else if (!FUSED) {
@@ -1458,15 +1459,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
             } else {
               // the new base: in this tile (P2G here) or a mover of the next step
               bool stay = true;
-              uint32_t cl = 0, nt = 0;
+              uint32_t nt = 0;
+              int lc[D];
 #pragma unroll
               for (int c = 0; c < D; c++) {
                 const int nb = stencil_base<T, D>(g, xn, c) + 3;
                 nt = nt * (uint32_t)g.GT[c] + (uint32_t)(nb / E);
                 const int q = nb - org[c];
                 stay = stay && q >= 0 && q < E;
-                cl = cl * E + (uint32_t)(q & (E - 1));
+                lc[c] = q & (E - 1);
               }
+              const uint32_t cl = cell_label<D>(lc);
               if (!stay) {
                 nt += g.tile0;
                 if (nt < a.ntotal) a.mstamp[nt] = a.next_stamp;
@@ -1939,13 +1942,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
 #pragma unroll
             for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
             bool in = true;
-            uint32_t cl = 0;
+            int lc[D];
 #pragma unroll
             for (int c = 0; c < D; c++) {
               const int lb = stencil_base<T, D>(g, x, c) + 3 - org[c];
               in = in && lb >= 0 && lb < E;
-              cl = cl * E + (uint32_t)(lb & (E - 1));
+              lc[c] = lb & (E - 1);
             }
+            const uint32_t cl = cell_label<D>(lc);
             if (in) cell = cl;
           }
         }
