@@ -431,6 +431,21 @@ int mpmx_svd(mpmx_ctx* ctx, int64_t n, const void* F, void* U, void* s, void* V)
 int mpmx_sort_keys(mpmx_ctx* ctx, int64_t n, const uint32_t* keys, int key_bits, uint32_t* keys_out,
                    uint32_t* order_out);
 
+/* The binning sort without allocation.  No counterpart in the reference: it is
+   for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  keys,
+   keys_out and order_out are DEVICE arrays of n entries on the context's GPU;
+   the sort is queued on the context stream and the call returns without
+   waiting (mpmx_synchronize waits).  Results equal mpmx_sort_keys's bit for
+   bit.  Its scratch (ping-pong arrays, histograms) is one static device array
+   per GPU of about 8.5 MiB, kept until the process ends, so n is at most
+   MPMX_SORT_NOALLOC_MAX_KEYS (MPMX_EINVAL above it, before any GPU work);
+   every call on a GPU shares that array, so calls that overlap on one GPU
+   (two contexts' streams) race.  It calls no hipMalloc or hipFree. */
+#define MPMX_SORT_NOALLOC_MAX_KEYS (1 << 20)
+int mpmx_sort_keys_noalloc(mpmx_ctx* ctx, int64_t n, const uint32_t* keys, int key_bits, uint32_t* keys_out,
+                           uint32_t* order_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -437,4 +437,11 @@ int mpmx_sort_keys(mpmx_ctx* ctx, int64_t n, const uint32_t* keys, int key_bits,
   return guard([&] { ctx->engine->sort_keys(n, keys, key_bits, keys_out, order_out); });
 }
 
+int mpmx_sort_keys_noalloc(mpmx_ctx* ctx, int64_t n, const uint32_t* keys, int key_bits, uint32_t* keys_out,
+                           uint32_t* order_out) {
+  if (!ctx) return bad("null context");
+  if (n > 0 && (!keys || !keys_out || !order_out)) return bad("null array");
+  return guard([&] { ctx->engine->sort_keys_noalloc(n, keys, key_bits, keys_out, order_out); });
+}
+
 }  // extern "C"

@@ -70,6 +70,8 @@ struct EngineBase {
   virtual void svd(int64_t n, const void* F, void* U, void* s, void* V) = 0;
   virtual void sort_keys(int64_t n, const uint32_t* keys, int key_bits, uint32_t* keys_out,
                          uint32_t* order_out) = 0;
+  virtual void sort_keys_noalloc(int64_t n, const uint32_t* keys, int key_bits, uint32_t* keys_out,
+                                 uint32_t* order_out) = 0;
 };
 
 // Throws mpmx::Error on failure.

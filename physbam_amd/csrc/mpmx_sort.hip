@@ -8,6 +8,10 @@
 // equals std::stable_sort of the same keys.
 #include "mpmx_sort.h"
 
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
 namespace mpmx {
 
 namespace {
@@ -217,6 +221,68 @@ void radix_sort_pairs(const uint32_t* keys_in, uint32_t* keys_out, uint32_t* val
     vin = vout;
     shift += bits;
   }
+}
+
+// ------------------------------------------------- sort without allocation --
+namespace {
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+// Layout of the static array: keys_tmp and vals_tmp (SORT_NOALLOC_MAX_KEYS
+// uint32 each), then the histogram and scan partials of sort_scratch_bytes for
+// that count (recomputed here as constants: nblocks << MAX_DIGIT_BITS
+// histogram entries, one scan partial per SCAN_TILE of them plus one, and the
+// 256-byte tail).
+constexpr size_t NOALLOC_TMP_BYTES = align256((size_t)SORT_NOALLOC_MAX_KEYS * sizeof(uint32_t));
+constexpr size_t NOALLOC_HIST = ((size_t)(SORT_NOALLOC_MAX_KEYS + SORT_TILE - 1) / SORT_TILE) << MAX_DIGIT_BITS;
+constexpr size_t NOALLOC_SORT_BYTES =
+    sizeof(uint32_t) * NOALLOC_HIST + sizeof(uint32_t) * ((NOALLOC_HIST + SCAN_TILE - 1) / SCAN_TILE + 1) + 256;
+constexpr size_t NOALLOC_BYTES = 2 * NOALLOC_TMP_BYTES + align256(NOALLOC_SORT_BYTES);
+
+// One copy per GPU, held until the process ends.  In host code this name is
+// a stand-in, not a device address: only the pointer that hipGetSymbolAddress
+// returns for the current device is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `device`'s array, looked up once per device with that
+// device current (the caller's current device is restored).
+void* scratch_on(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return g_scratch_ptr[device];
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) throw std::runtime_error(std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (cur != device && (e = hipSetDevice(device)) != hipSuccess)
+    throw std::runtime_error(std::string("hipSetDevice: ") + hipGetErrorString(e));
+  void* ptr = nullptr;
+  e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (cur != device) (void)hipSetDevice(cur);
+  if (e != hipSuccess || !ptr)
+    throw std::runtime_error(std::string("hipGetSymbolAddress of the sort scratch array: ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = ptr;
+  return ptr;
+}
+
+}  // namespace
+
+size_t sort_noalloc_scratch_bytes() { return NOALLOC_BYTES; }
+
+void radix_sort_pairs_noalloc(const uint32_t* keys_in, uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
+                              int key_bits, int device, hipStream_t stream) {
+  if (n > SORT_NOALLOC_MAX_KEYS)
+    throw std::runtime_error("the sort without allocation takes at most " + std::to_string(SORT_NOALLOC_MAX_KEYS) +
+                             " keys, got " + std::to_string(n));
+  unsigned char* base = (unsigned char*)scratch_on(device);
+  if (n == 0) return;
+  if (sort_scratch_bytes(n, key_bits) > NOALLOC_SORT_BYTES) throw std::runtime_error("sort scratch layout mismatch");
+  uint32_t* keys_tmp = (uint32_t*)base;
+  uint32_t* vals_tmp = (uint32_t*)(base + NOALLOC_TMP_BYTES);
+  void* sort_scratch = (void*)(base + 2 * NOALLOC_TMP_BYTES);
+  radix_sort_pairs(keys_in, keys_out, vals_out, keys_tmp, vals_tmp, n, key_bits, sort_scratch, stream);
 }
 
 }  // namespace mpmx

@@ -3384,6 +3384,7 @@ class Engine final : public EngineBase {
     key_bits_ = 1;
     while ((1ull << key_bits_) < nt + 1) key_bits_++;
     for (int a = 0; a < D; a++) grav_[a] = 0;
+    HIPCHECK(hipGetDevice(&device_));
     HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     tile_begin_.alloc(nt);
     tile_end_.alloc(nt);
@@ -4260,6 +4261,19 @@ class Engine final : public EngineBase {
     HIPCHECK(hipMemcpyAsync(order_out, vo.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
     HIPCHECK(hipStreamSynchronize(stream_));
   }
+  static_assert(SORT_NOALLOC_MAX_KEYS == MPMX_SORT_NOALLOC_MAX_KEYS, "the header's limit is the sort's");
+  // The same sort on device arrays, queued on the context stream, with its
+  // scratch in the sort's static per-GPU array: no allocation, no wait.
+  void sort_keys_noalloc(int64_t n, const uint32_t* keys, int key_bits, uint32_t* keys_out,
+                         uint32_t* order_out) override {
+    if (n < 0) throw Error{MPMX_EINVAL, "negative key count"};
+    if (n > (int64_t)SORT_NOALLOC_MAX_KEYS)
+      throw Error{MPMX_EINVAL, "mpmx_sort_keys_noalloc takes at most " + std::to_string(SORT_NOALLOC_MAX_KEYS) +
+                                   " keys (got " + std::to_string(n) + ")"};
+    if (key_bits < 1 || key_bits > 32) throw Error{MPMX_EINVAL, "key_bits must be in [1, 32]"};
+    radix_sort_pairs_noalloc(keys, keys_out, order_out, (uint32_t)n, key_bits, device_, stream_);
+    HIPCHECK(hipGetLastError());
+  }
 
  private:
   void allocate_particles() {
@@ -4953,6 +4967,7 @@ class Engine final : public EngineBase {
   int nwalls_ = 0;
   WallDev<T> walls_[MAX_WALLS];
   int key_bits_ = 1;
+  int device_ = 0;  // the GPU current at creation: the stream's
   hipStream_t stream_ = nullptr;
   uint32_t n_ = 0;
   uint32_t stamp_ = 0;

@@ -36,7 +36,7 @@ EXPORTS = (
     "mpmx_set_plasticity", "mpmx_get_plastic_state", "mpmx_set_drucker_prager", "mpmx_get_drucker_prager_state",
     "mpmx_tape_begin", "mpmx_backward",
     "mpmx_enable_timing", "mpmx_reset_timing",
-    "mpmx_phase_time", "mpmx_stress", "mpmx_svd", "mpmx_sort_keys", "mpmx_seed_particles",
+    "mpmx_phase_time", "mpmx_stress", "mpmx_svd", "mpmx_sort_keys", "mpmx_sort_keys_noalloc", "mpmx_seed_particles",
     "mpmx_random_boxes", "mpmx_seed_particles_poisson",
 )
 
@@ -148,6 +148,7 @@ def lib():
         L.mpmx_stress.argtypes = [vp, i64] + [vp] * 5
         L.mpmx_svd.argtypes = [vp, i64] + [vp] * 4
         L.mpmx_sort_keys.argtypes = [vp, i64, vp, c_int, vp, vp]
+        L.mpmx_sort_keys_noalloc.argtypes = [vp, i64, vp, c_int, vp, vp]
         L.mpmx_seed_particles.argtypes = [ctypes.POINTER(SeedDesc), c_int, i64] + [vp] * 9
         L.mpmx_seed_particles.restype = i64
         L.mpmx_seed_particles_poisson.argtypes = [ctypes.POINTER(SeedDesc), c_int, dbl, i64] + [vp] * 9
@@ -520,6 +521,11 @@ class Context:
         oo = np.zeros(n, np.uint32)
         _check(lib().mpmx_sort_keys(self._h, n, _p(keys), key_bits, _p(ko), _p(oo)))
         return ko, oo
+
+    def sort_keys_noalloc(self, n, keys_dev, key_bits, keys_out_dev, order_out_dev):
+        """mpmx_sort_keys_noalloc on device addresses (ints); queued, not waited for."""
+        _check(lib().mpmx_sort_keys_noalloc(self._h, int(n), ctypes.c_void_p(keys_dev), int(key_bits),
+                                            ctypes.c_void_p(keys_out_dev), ctypes.c_void_p(order_out_dev)))
 
 
 def seed_particles(dim, cells, lo, hi, shapes, ppc_axis=2, seed=1234, precision=FP32, jitter=True, poisson=False,
