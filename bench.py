@@ -134,6 +134,59 @@ def make_context(P):
 
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+# kernel phase -> rocprof kernel-name prefix of its main kernel
+TRAFFIC_KERNELS = {"g2p2g": "void mpmx::g2p2g_kernel<float, 3, false>", "grid": "void mpmx::grid_kernel<float, 3, false>",
+                   "side": "void mpmx::fused_side_kernel<float, 3, false>",
+                   "glist": "void mpmx::glist_kernel<float, 3, true>"}
+
+
+def kernel_sources_digest():
+    """SHA-256 over the HIP sources and headers of the library: a traffic
+    record measured on other sources does not describe this build."""
+    import hashlib
+
+    h = hashlib.sha256()
+    cdir = os.path.join(ROOT, "physbam_amd", "csrc")
+    for f in sorted(os.listdir(cdir)):
+        if f.endswith((".hip", ".h")):
+            with open(os.path.join(cdir, f), "rb") as fh:
+                h.update(f.encode() + b"\0" + fh.read())
+    return h.hexdigest()
+
+
+def _pmc_means(path, counter):
+    """Mean per dispatch (after each kernel's first) of a rocprofv3 --pmc CSV."""
+    import csv
+    vals = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return {k: (sum(v[1:]) / (len(v) - 1) if len(v) > 1 else v[0]) for k, v in vals.items()}
+
+
+def write_traffic(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib):
+    """profiles/traffic.json from four rocprofv3 --pmc runs (FETCH_SIZE and
+    WRITE_SIZE of the benchmark, then of --copy-probe MIB): KiB per dispatch
+    scaled by what the counters read for a plain copy of known size, with the
+    digest of the sources measured."""
+    nbytes = copy_mib * (1 << 20)
+    cf = max(_pmc_means(copy_fetch_csv, "FETCH_SIZE").items(), key=lambda kv: kv[1])[1]
+    cw = max(_pmc_means(copy_write_csv, "WRITE_SIZE").items(), key=lambda kv: kv[1])[1]
+    fs, ws = nbytes / (cf * 1024), nbytes / (cw * 1024)
+    fk, wk = _pmc_means(fetch_csv, "FETCH_SIZE"), _pmc_means(write_csv, "WRITE_SIZE")
+    out = {"sources_sha256": kernel_sources_digest(),
+           "copy_probe": {"bytes": nbytes, "fetch_kib": cf, "write_kib": cw, "fetch_scale": fs, "write_scale": ws}}
+    for phase, prefix in TRAFFIC_KERNELS.items():
+        kf = [k for k in fk if k.startswith(prefix)]
+        kw = [k for k in wk if k.startswith(prefix)]
+        if kf and kw:
+            f, w = fk[kf[0]], wk[kw[0]]
+            out[phase] = {"fetch_kib": f, "write_kib": w, "bytes_per_launch": (f * fs + w * ws) * 1024}
+    with open(TRAFFIC, "w") as fh:
+        json.dump(out, fh, indent=1)
+    return out
 
 
 def cpu_threads():
@@ -387,8 +440,15 @@ def main():
                          "OpenMP thread counts and print the rows as JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibrates HBM byte counters)")
+    ap.add_argument("--write-traffic", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="only write profiles/traffic.json from rocprofv3 --pmc counter_collection CSVs of the "
+                         "benchmark and of --copy-probe 1024")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
+
+    if args.write_traffic:
+        print(json.dumps(write_traffic(*args.write_traffic, copy_mib=1024)), flush=True)
+        return
 
     if args.copy_probe:
         import torch
@@ -463,13 +523,16 @@ def main():
     achieved = kbytes / avg_s / 1e9
     step_bytes = BYTES_PER_PARTICLE["step"] * n + BYTES_PER_NODE["step"] * nodes
     step_gbs = step_bytes / (sec / args.steps) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.isfile(tpath):
-        with open(tpath) as f:
+    # HBM bytes per launch of the dominant kernel from the committed counter
+    # runs, only when they were measured on these very sources
+    traffic, traffic_note = None, "no counter record"
+    if os.path.isfile(TRAFFIC):
+        with open(TRAFFIC) as f:
             tr = json.load(f)
-        if kern in tr:
-            traffic = tr[kern]["bytes_per_launch"]
+        if tr.get("sources_sha256") != kernel_sources_digest():
+            traffic_note = "counter record of other sources: not used"
+        elif kern in tr:
+            traffic, traffic_note = tr[kern]["bytes_per_launch"], "rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated"
 
     out = {
         "metric": METRIC,
@@ -491,7 +554,7 @@ def main():
                    "rebin_interval": args.rebin_interval, "rebins_timed": rebins1 - rebins0,
                    "movers_last_step": movers_last},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "bytes_per_launch": kbytes, "avg_launch_ms": avg_s * 1e3},
         "step_roofline": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
                           "bytes_per_step": step_bytes},

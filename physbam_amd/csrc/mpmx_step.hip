@@ -78,6 +78,11 @@ template<typename T, int D> struct Args {
   Node4<T>* grid;
   Node4<T>* gridf;
   Node4<T>* gridu;  // if set, the grid update also stores (m, v before the collision objects)
+  // if set, the grid update also zeroes these tiles of this grid (the fused
+  // step's other grid, as its P2G of two steps back left it)
+  Node4<T>* clear_grid;
+  const uint32_t* clear_tiles;
+  const uint32_t* clear_count;
   // fused G2P -> P2G: P2G target of the next step and the next step's mover
   // list (movers are appended from the bottom by the tile kernel and from the
   // top, index mcap-1-k, by the mover kernel; counts[5] and counts[7] count
@@ -456,7 +461,10 @@ __device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*
   // Separable form: with per-axis factors w, dw and dz = o dx - xe,
   //   m_i   += w_x w_y w_z m
   //   p_i,c += w_x w_y w_z (mv_c + sum_j mC_cj dz_j) - sum_j Q_cj (grad w)_j
-  // is built axis by axis so that each node costs one FMA per value.
+  // is built axis by axis.  Per x plane, the z-dependent part
+  //   w_x w_y (w_z dz_z mC_c2 - dw_z Q_c2) = w_y H_c(z)
+  // is formed once for the three z offsets, so that a node costs one FMA for
+  // its mass and two per momentum component: w_z P_c(x, y) + w_y H_c(z).
   if constexpr (D == 3) {
     T wzd[3], wydz[3];
 #pragma unroll
@@ -467,42 +475,36 @@ __device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*
 #pragma unroll
     for (int ox = 0; ox < 3; ox++) {
       const T wx = w[ox][0], dwx = dw[ox][0], dzx = dz[ox][0];
-      T ax[3], bx[3], dx_[3], ex[3], fx[3];
+      T ax[3], bx[3], dx_[3], H[3][3];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         ax[c] = wx * (mv[c] + mC[c][0] * dzx) - Q[c][0] * dwx;  // coefficient of w_y w_z
         bx[c] = wx * mC[c][1];                                  // of w_y dz_y w_z
         dx_[c] = -wx * Q[c][1];                                 // of dw_y w_z
-        ex[c] = wx * mC[c][2];                                  // of w_y w_z dz_z
-        fx[c] = -wx * Q[c][2];                                  // of w_y dw_z
+        const T ex = wx * mC[c][2], fx = -wx * Q[c][2];         // of w_y w_z dz_z and of w_y dw_z
+#pragma unroll
+        for (int oz = 0; oz < 3; oz++) H[oz][c] = wzd[oz] * ex + dw[oz][2] * fx;  // coefficient of w_y
       }
       const T mx = wx * m;
 #pragma unroll
       for (int oy = 0; oy < 3; oy++) {
         const T wy = w[oy][1], dwy = dw[oy][1], wyz = wydz[oy];
-        T P[3], S[3], Tz[3];
+        T P[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-          P[c] = wy * ax[c] + wyz * bx[c] + dwy * dx_[c];  // coefficient of w_z
-          S[c] = wy * ex[c];                                // of w_z dz_z
-          Tz[c] = wy * fx[c];                               // of dw_z
-        }
+        for (int c = 0; c < 3; c++) P[c] = wy * ax[c] + wyz * bx[c] + dwy * dx_[c];  // coefficient of w_z
         const T mxy = wy * mx;
 #pragma unroll
         for (int oz = 0; oz < 3; oz++) {
-          const T wz = w[oz][2], dwz = dw[oz][2], wzz = wzd[oz];
+          const T wz = w[oz][2];
           const int li = base_li + (ox * R + oy) * R + oz;
           T4 cur = reg[li];
           if constexpr (C::MASS) cur.x += wz * mxy;
           cur.y += wz * P[0];
           cur.z += wz * P[1];
           cur.w += wz * P[2];
-          cur.y += wzz * S[0];
-          cur.z += wzz * S[1];
-          cur.w += wzz * S[2];
-          cur.y += dwz * Tz[0];
-          cur.z += dwz * Tz[1];
-          cur.w += dwz * Tz[2];
+          cur.y += wy * H[oz][0];
+          cur.z += wy * H[oz][1];
+          cur.w += wy * H[oz][2];
           reg[li] = cur;
           __asm__ __volatile__("" ::: "memory");
         }
@@ -776,10 +778,96 @@ template<typename T, int D> struct ScatterCoef {
   static constexpr int M = 0, MV = 1, MC = 1 + D, TAU = 1 + D + D * D, XE = 1 + D + 2 * D * D, N = XE + D;
 };
 
+// Pass 1 of the side-path P2G for one particle p: its transfer coefficients
+// (mover: m, m v, m B D^-1, dt tau, x - x_base; slow-list particle: only
+// dt tau) into column `slot` of s_c, its stencil base into s_b (s_b[0] = -1:
+// skipped), its scene into s_sc.
+template<typename T, int D, bool FUSED>
+__device__ __forceinline__ void side_coefficients(const Args<T, D>& a, uint32_t p, bool mover,
+                                                  T (*s_c)[BLOCK], int (*s_b)[BLOCK], uint16_t* s_sc, int slot) {
+  using C = ScatterCoef<T, D>;
+  const Geom<T, D> g = particle_geom(a, p);
+  T x[D], Fm[D * D], tau[D * D];
+#pragma unroll
+  for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+#pragma unroll
+  for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
+  bool in = true;
+  int base[D];
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    base[c] = stencil_base<T, D>(g, x, c) + 3;
+    in = in && base[c] >= 0 && base[c] + 2 < g.G[c];
+  }
+  if (FUSED && !a.valid[p]) {
+    base[0] = -1;  // left the domain in this step's G2P
+  } else if (!in) {
+    atomicOr(a.err, 2u);
+    base[0] = -1;  // skipped in pass 2
+  } else if (mover) {
+    kirchhoff_any<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
+  } else {
+    kirchhoff_slow<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
+  }
+  const T m = mover ? a.mass[p] : (T)0;
+  s_c[C::M][slot] = m;
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    s_c[C::MV + c][slot] = mover ? m * a.V[c * a.n + p] : (T)0;
+    s_c[C::XE + c][slot] = x[c] - (g.lo[c] + ((T)(base[c] - 3) + (T)0.5) * g.dx[c]);
+    s_b[c][slot] = base[c];
+  }
+#pragma unroll
+  for (int j = 0; j < D; j++)
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      s_c[C::MC + c + D * j][slot] = mover ? m * a.B[(c + D * j) * a.n + p] * ((T)4 * g.inv_dx[j] * g.inv_dx[j]) : (T)0;
+      s_c[C::TAU + c + D * j][slot] = a.dt * tau[c + D * j];
+    }
+  s_sc[slot] = a.pscene ? a.pscene[p] : 0;
+}
+
+// Pass 2 of the side-path P2G: one thread per (particle, stencil node) of the
+// cnt particles of s_c, atomics into target.
+template<typename T, int D>
+__device__ __forceinline__ void side_scatter(const Args<T, D>& a, Node4<T>* target, const T (*s_c)[BLOCK],
+                                             const int (*s_b)[BLOCK], const uint16_t* s_sc, uint32_t cnt) {
+  using C = ScatterCoef<T, D>;
+  for (uint32_t u = threadIdx.x; u < cnt * Tile<D>::K; u += BLOCK) {
+    const uint32_t sl = u / Tile<D>::K;
+    const int k = (int)(u - sl * Tile<D>::K);
+    if (s_b[0][sl] < 0) continue;
+    const Geom<T, D> g = a.scenes[s_sc[sl]];
+    T xe[D], wt, gr[D], dz[D];
+    int o[D], nc[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) xe[c] = s_c[C::XE + c][sl];
+    node_weight<T, D>(g, xe, k, o, wt, gr, dz);
+#pragma unroll
+    for (int c = 0; c < D; c++) nc[c] = s_b[c][sl] + o[c];
+    Node4<T>* nd = target + node_slot<T, D>(nc, g);
+    T f[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      T st = 0, ap = s_c[C::MV + c][sl];
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        st += s_c[C::TAU + c + D * j][sl] * gr[j];
+        ap += s_c[C::MC + c + D * j][sl] * dz[j];
+      }
+      f[c] = wt * ap - st;
+    }
+    const T m = s_c[C::M][sl];
+    if (m != (T)0) atomic_add(&nd->m, wt * m);
+    atomic_add(&nd->x, f[0]);
+    atomic_add(&nd->y, f[1]);
+    if constexpr (D == 3) atomic_add(&nd->z, f[D - 1]);
+  }
+}
+
 template<typename T, int D, bool FUSED>
 __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
-  using C = ScatterCoef<T, D>;
-  __shared__ T s_c[C::N][BLOCK];
+  __shared__ T s_c[ScatterCoef<T, D>::N][BLOCK];
   __shared__ int s_b[D][BLOCK];
   __shared__ uint16_t s_sc[BLOCK];
   // FUSED: the movers are the fused step's current list (bottom counts[4],
@@ -803,80 +891,11 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
       } else {
         p = a.slow_list[spread_index(q - nm, ns)];
       }
-      const Geom<T, D> g = particle_geom(a, p);
-      T x[D], Fm[D * D], tau[D * D];
-#pragma unroll
-      for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
-#pragma unroll
-      for (int k = 0; k < D * D; k++) Fm[k] = a.F[k * a.n + p];
-      bool in = true;
-      int base[D];
-#pragma unroll
-      for (int c = 0; c < D; c++) {
-        base[c] = stencil_base<T, D>(g, x, c) + 3;
-        in = in && base[c] >= 0 && base[c] + 2 < g.G[c];
-      }
-      if (FUSED && !a.valid[p]) {
-        base[0] = -1;  // left the domain in this step's G2P
-      } else if (!in) {
-        atomicOr(a.err, 2u);
-        base[0] = -1;  // skipped in pass 2
-      } else if (mover) {
-        kirchhoff_any<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
-      } else {
-        kirchhoff_slow<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.root, a.vol[p], Fm, tau);
-      }
-      const T m = mover ? a.mass[p] : (T)0;
-      s_c[C::M][threadIdx.x] = m;
-#pragma unroll
-      for (int c = 0; c < D; c++) {
-        s_c[C::MV + c][threadIdx.x] = mover ? m * a.V[c * a.n + p] : (T)0;
-        s_c[C::XE + c][threadIdx.x] = x[c] - (g.lo[c] + ((T)(base[c] - 3) + (T)0.5) * g.dx[c]);
-        s_b[c][threadIdx.x] = base[c];
-      }
-#pragma unroll
-      for (int j = 0; j < D; j++)
-#pragma unroll
-        for (int c = 0; c < D; c++) {
-          s_c[C::MC + c + D * j][threadIdx.x] =
-              mover ? m * a.B[(c + D * j) * a.n + p] * ((T)4 * g.inv_dx[j] * g.inv_dx[j]) : (T)0;
-          s_c[C::TAU + c + D * j][threadIdx.x] = a.dt * tau[c + D * j];
-        }
-      s_sc[threadIdx.x] = a.pscene ? a.pscene[p] : 0;
+      side_coefficients<T, D, FUSED>(a, p, mover, s_c, s_b, s_sc, (int)threadIdx.x);
     }
     __syncthreads();
     // ---- pass 2: one thread per (particle, stencil node)
-    const uint32_t cnt = min(per, ntot - c0);
-    for (uint32_t u = threadIdx.x; u < cnt * Tile<D>::K; u += BLOCK) {
-      const uint32_t sl = u / Tile<D>::K;
-      const int k = (int)(u - sl * Tile<D>::K);
-      if (s_b[0][sl] < 0) continue;
-      const Geom<T, D> g = a.scenes[s_sc[sl]];
-      T xe[D], wt, gr[D], dz[D];
-      int o[D], nc[D];
-#pragma unroll
-      for (int c = 0; c < D; c++) xe[c] = s_c[C::XE + c][sl];
-      node_weight<T, D>(g, xe, k, o, wt, gr, dz);
-#pragma unroll
-      for (int c = 0; c < D; c++) nc[c] = s_b[c][sl] + o[c];
-      Node4<T>* nd = target + node_slot<T, D>(nc, g);
-      T f[D];
-#pragma unroll
-      for (int c = 0; c < D; c++) {
-        T st = 0, ap = s_c[C::MV + c][sl];
-#pragma unroll
-        for (int j = 0; j < D; j++) {
-          st += s_c[C::TAU + c + D * j][sl] * gr[j];
-          ap += s_c[C::MC + c + D * j][sl] * dz[j];
-        }
-        f[c] = wt * ap - st;
-      }
-      const T m = s_c[C::M][sl];
-      if (m != (T)0) atomic_add(&nd->m, wt * m);
-      atomic_add(&nd->x, f[0]);
-      atomic_add(&nd->y, f[1]);
-      if constexpr (D == 3) atomic_add(&nd->z, f[D - 1]);
-    }
+    side_scatter<T, D>(a, target, s_c, s_b, s_sc, min(per, ntot - c0));
     __syncthreads();
   }
 }
@@ -928,6 +947,13 @@ __device__ __forceinline__ int walls_apply(const Args<T, D>& a, const T* X0, T* 
 // then Coulomb friction at colliding nodes (MPM_DRIVER.cpp:1064-1094).
 template<typename T, int D, bool FRICTION>
 __global__ __launch_bounds__(BLOCK) void grid_kernel(Args<T, D> a) {
+  if (a.clear_grid) {
+    const uint64_t tc = (uint64_t)(*a.clear_count) * 64u;
+    Node4<T> z;
+    z.m = z.x = z.y = z.z = 0;
+    for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < tc; idx += (uint64_t)gridDim.x * BLOCK)
+      a.clear_grid[(uint64_t)a.clear_tiles[idx >> 6] * 64u + (idx & 63u)] = z;
+  }
   const uint64_t total = (uint64_t)a.counts[1] * 64u;
   for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total; idx += (uint64_t)gridDim.x * BLOCK) {
     const uint32_t t = a.gtiles[idx >> 6];
@@ -1611,6 +1637,102 @@ __global__ void fused_counts_kernel(uint32_t* counts, uint32_t* last, int mode) 
     counts[3] = 0;
     counts[5] = 0;
     counts[7] = 0;
+  }
+}
+
+// The side path of a fused step in one launch (in place of g2p_mover_kernel,
+// p2g_extra_kernel and fused_counts_kernel): the current mover list (stencil
+// base outside the binned tile at the start of the step: counts[4] entries
+// from the bottom of mover_list, counts[6] from the top) is gathered from the
+// HBM grid, stored, and then scattered into grid_next together with the
+// movers the tile kernel appended (counts[5], bottom of mover_list2) and the
+// stress impulse of the slow list (counts[3]).  A current mover still outside
+// its binned tile goes to the top of the next list.  The block that finishes
+// last (ticket in counts[12]) does the list bookkeeping of the step.
+template<typename T, int D, bool FRICTION>
+__global__ __launch_bounds__(BLOCK) void fused_side_kernel(Args<T, D> a, uint32_t* last) {
+  __shared__ T s_c[ScatterCoef<T, D>::N][BLOCK];
+  __shared__ int s_b[D][BLOCK];
+  __shared__ uint16_t s_sc[BLOCK];
+  __shared__ uint32_t s_ticket;
+  const uint32_t n4 = a.counts[4], n6 = a.counts[6], n5 = a.counts[5];
+  const uint32_t nc = n4 + n6, nm = nc + n5, ns = a.counts[3], ntot = nm + ns;
+  const uint32_t per = min((uint32_t)BLOCK, max(1u, (ntot + gridDim.x - 1) / gridDim.x));
+  for (uint32_t c0 = blockIdx.x * per; c0 < ntot; c0 += gridDim.x * per) {
+    const uint32_t q = c0 + threadIdx.x;
+    if (threadIdx.x < per && q < ntot) {
+      const bool mover = q < nm;
+      uint32_t p;
+      if (mover) {
+        const uint32_t r = spread_index(q, nm);
+        if (r < nc) {
+          // G2P of a current mover from the HBM grid (MPM_DRIVER.cpp:450-515)
+          const uint32_t qq = r < n4 ? r : a.mcap - 1u - (r - n4);
+          p = a.mover_list[qq];
+          const Geom<T, D> g = particle_geom(a, p);
+          T x[D];
+#pragma unroll
+          for (int c = 0; c < D; c++) x[c] = a.X[c * a.n + p];
+          Wts<T, D> w;
+          weights<T, D>(g, x, w);
+          bool in = true;
+#pragma unroll
+          for (int c = 0; c < D; c++) in = in && w.base[c] >= 0 && w.base[c] + 2 < g.G[c];
+          if (in) {  // otherwise flagged by side_coefficients
+            auto slot = [&](int ox, int oy, int oz) -> uint32_t {
+              int ncd[D];
+              ncd[0] = w.base[0] + ox;
+              ncd[1] = w.base[1] + oy;
+              if constexpr (D == 3) ncd[D - 1] = w.base[D - 1] + oz;
+              return node_slot<T, D>(ncd, g);
+            };
+            auto comp = [](const Node4<T>& nd, int c) -> T { return c == 0 ? nd.x : (c == 1 ? nd.y : nd.z); };
+            auto nv = [&](int ox, int oy, int oz, int c) -> T { return comp(a.grid[slot(ox, oy, oz)], c); };
+            auto nf = [&](int ox, int oy, int oz, int c) -> T {
+              if constexpr (FRICTION) return comp(a.gridf[slot(ox, oy, oz)], c);
+              else return nv(ox, oy, oz, c);
+            };
+            T vb[D], vfb[D], G[D * D], Bs[D * D];
+            g2p_gather<T, D, FRICTION>(g, a.dt, w, nv, nf, vb, vfb, G, Bs);
+            const uint32_t bt = a.mover_tile[qq];
+            if (g2p_store<T, D, false>(a, g, p, x, vb, vfb, G, Bs, bt)) {
+              const uint32_t k = a.mcap - 1u - atomicAdd(&a.counts[7], 1u);
+              a.mover_list2[k] = p;
+              a.mover_tile2[k] = bt;
+            }
+          }
+        } else {
+          p = a.mover_list2[r - nc];
+        }
+      } else {
+        p = a.slow_list[spread_index(q - nm, ns)];
+      }
+      side_coefficients<T, D, true>(a, p, mover, s_c, s_b, s_sc, (int)threadIdx.x);
+    }
+    __syncthreads();
+    side_scatter<T, D>(a, a.grid_next, s_c, s_b, s_sc, min(per, ntot - c0));
+    __syncthreads();
+  }
+  // the last block to finish: bookkeeping of fused_counts_kernel mode 1
+  __threadfence();
+  if (threadIdx.x == 0) s_ticket = atomicAdd(&a.counts[12], 1u);
+  __syncthreads();
+  if (s_ticket == gridDim.x - 1 && threadIdx.x == 0) {
+    __threadfence();
+    uint32_t* counts = a.counts;
+    const uint32_t c4 = atomicAdd(&counts[4], 0u), c5 = atomicAdd(&counts[5], 0u);
+    const uint32_t c6 = atomicAdd(&counts[6], 0u), c7 = atomicAdd(&counts[7], 0u);
+    const uint32_t c3 = atomicAdd(&counts[3], 0u), c1 = atomicAdd(&counts[1], 0u);
+    counts[10] = c4 + c6;  // movers of the step just taken
+    counts[4] = c5;
+    counts[6] = c7;
+    counts[9] = c5 + c7;
+    last[1] = c1;
+    counts[11] = c3;  // FP64-stress particles of the P2G just done
+    counts[3] = 0;
+    counts[5] = 0;
+    counts[7] = 0;
+    counts[12] = 0;
   }
 }
 
@@ -4900,11 +5022,16 @@ class Engine final : public EngineBase {
     a.next_stamp = ms_next;
 
     hipEvent_t e2 = timer_.begin(stream_);
-    if (fric)
-      grid_kernel<T, D, true><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
-    else
-      grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(a);
-    clear_tiles_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(gother_, gt_other_, lastcnt_.p + 1);
+    {
+      Args<T, D> ag = a;  // the update also zeroes the other grid's tiles for the fused kernel's P2G
+      ag.clear_grid = gother_;
+      ag.clear_tiles = gt_other_;
+      ag.clear_count = lastcnt_.p + 1;
+      if (fric)
+        grid_kernel<T, D, true><<<grid_blocks(), BLOCK, 0, stream_>>>(ag);
+      else
+        grid_kernel<T, D, false><<<grid_blocks(), BLOCK, 0, stream_>>>(ag);
+    }
     timer_.end(PH_GRID, e2, stream_);
 
     hipEvent_t e3 = timer_.begin(stream_);
@@ -4915,11 +5042,9 @@ class Engine final : public EngineBase {
       g2p2g_kernel<T, D, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
     timer_.end(PH_FUSED, ek, stream_);
     if (fric)
-      g2p_mover_kernel<T, D, true, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+      fused_side_kernel<T, D, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a, lastcnt_.p);
     else
-      g2p_mover_kernel<T, D, false, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
-    p2g_extra_kernel<T, D, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
-    fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 1);
+      fused_side_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a, lastcnt_.p);
     timer_.end(PH_G2P2G, e3, stream_);
     const int slot = (int)(step_count_ & 3);
     HIPCHECK(hipMemcpyAsync(rb_host_ + slot, counts_.p + 9, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
