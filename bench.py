@@ -123,6 +123,51 @@ def sphere_throughput(log, steps=SPHERE_STEPS, warmup=10):
                         "one random velocity in [-0.5, 0.5]^3, gravity, slip walls inset 3 cells, dt 5e-5, FP32"}
 
 
+def deformed_throughput(log, steps=100, warmup=10):
+    """The 12-box scene deformed: per box an affine velocity v_k + A_k (x - c_k)
+    (A_k entries U[-20, 20] /s, B = A_k h^2/4) and a pre-stretched F = R
+    diag(s) R^T (s U[0.85, 1.2] per axis, R a random rotation), so that the
+    stress takes its large-strain branch in most boxes, the small-strain
+    series in the others; same grid, material and dt as the headline."""
+    shapes = headline_boxes()
+    P = mpmx.seed_particles(3, CELLS, [0.0] * 3, [1.0] * 3, shapes, ppc_axis=2, seed=SEED, precision=mpmx.FP32)
+    X = P["X"]
+    n = int(X.shape[0])
+    rng = np.random.default_rng(11)
+    h = 1.0 / CELLS
+    e = h / 8  # a float32 position may round just outside its box; boxes are at least half a cell apart
+    V = np.zeros_like(X)
+    F = P["F"].copy()
+    B = np.zeros_like(P["B"])
+    for sh in shapes:
+        m = np.all((X >= np.asarray(sh["a"]) - e) & (X <= np.asarray(sh["b"]) + e), axis=1)
+        A = rng.uniform(-20.0, 20.0, (3, 3))
+        c = 0.5 * (np.asarray(sh["a"]) + np.asarray(sh["b"]))
+        V[m] = (rng.uniform(-0.5, 0.5, 3) + (X[m].astype(np.float64) - c) @ A.T).astype(np.float32)
+        B[m] = (A * (h * h / 4)).T.reshape(9).astype(np.float32)  # column-major
+        Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        Fk = Q @ np.diag(rng.uniform(0.85, 1.2, 3)) @ Q.T
+        F[m] = Fk.T.reshape(9).astype(np.float32)
+    P = dict(P, V=V, F=F, B=B)
+    ctx = make_context(P)
+    del P, X, V, F, B
+    ctx.step_async(DT, warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step_async(DT, steps)
+    ctx.synchronize()
+    sec = time.perf_counter() - t0
+    _, movers, slow = ctx.bin_stats(with_slow=True)
+    _, _, nvalid = ctx.particle_stats()
+    ctx.close()
+    log(f"deformed: {n} particles, {steps} steps in {sec * 1e3:.1f} ms")
+    return {"value": n * steps / sec, "unit": "particle-steps/s", "ms_per_step": sec * 1e3 / steps, "particles": n,
+            "valid_particles_after": nvalid, "movers_last_step": movers, "fp64_stress_last_step": slow,
+            "steps": steps,
+            "workload": "12-box scene, 256^3, per-box affine velocity (A U[-20,20]/s) and pre-stretched F "
+                        "(stretches U[0.85,1.2]), fixed-corotated, sticky floor, dt 2e-5, FP32"}
+
+
 def make_context(P):
     ctx = mpmx.Context(3, CELLS, [0.0] * 3, [1.0] * 3, precision=mpmx.FP32)
     ctx.set_constitutive(mpmx.FIXED_COROTATED)
@@ -429,6 +474,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint / forward ratio")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes throughput")
     ap.add_argument("--no-sphere", action="store_true", help="skip the neo-Hookean sphere (128^3) throughput")
+    ap.add_argument("--no-deformed", action="store_true", help="skip the deformed 12-box scene throughput")
     ap.add_argument("--reorder-interval", type=int, default=16)
     ap.add_argument("--rebin-fraction", type=float, default=1.0 / 128,
                     help="rebuild the bins when more than this fraction of particles left their binned tile")
@@ -563,6 +609,11 @@ def main():
         "adjoint_ratio": None,
         "cpu_baseline": None,
     }
+    if not args.no_deformed:
+        try:
+            out["deformed"] = deformed_throughput(log)
+        except Exception as e:  # report, never hide the GPU number
+            log(f"deformed run failed: {e}")
     if not args.no_sphere:
         try:
             out["sphere"] = sphere_throughput(log)

@@ -225,4 +225,178 @@ __device__ bool kirchhoff_adjoint(int model, double mu, double la, double thr, d
   return true;
 }
 
+// The same adjoint in the simulation precision T (FP32 contexts), written on
+// the forms the forward evaluates (kirchhoff_fast): with C = F F^T =
+// I + E (E = G + G^T + G G^T, G = F - I, no cancellation) and S the symmetric
+// part of taubar,
+//   neo-Hookean      tau = vol (mu E + lambda ln J I)
+//     Fbar = vol (2 mu S F + lambda tr S cof F / J),  mubar = vol S:E,
+//     lambdabar = vol ln J tr S
+//   fixed corotated  tau = vol (2 mu (C - sqrt C) + lambda (J - 1) J I)
+//     Cbar = 2 mu vol (S - L(S)),  L(S) = Q [(Q^T S Q)_ij / (s_i + s_j)] Q^T
+//     (the adjoint of d sqrt C, from sqrt C X + X sqrt C = dC in the eigenbasis
+//     Q of E, s_i = sqrt(1 + e_i)),
+//     Fbar = 2 Cbar F + vol lambda (2J - 1) tr S cof F,
+//     mubar = 2 vol sum_i (Q^T S Q)_ii s_i (s_i - 1),  s_i - 1 = e_i / (1 + s_i),
+//     lambdabar = vol (J - 1) J tr S.
+// Returns 0 on success, 1 when F is inverted or nearly singular (J <= 0 or
+// 1 + e_min < 1e-4 (1 + e_max): the FP64 SVD form kirchhoff_adjoint takes
+// those), 2 for the neo-Hookean clamp branch (no adjoint).  mub and lab are
+// returned in FP64 for the per-particle FP64 accumulation.
+template<typename T, int D>
+__device__ __forceinline__ int kirchhoff_adjoint_fast(int model, T mu, T la, T thr, T vol, const T* F, const T* taubar,
+                                                      T* Fbar, double& mub, double& lab) {
+  T S[D][D], Fm[D][D], G[D][D];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      S[i][j] = (T)0.5 * (taubar[i + D * j] + taubar[j + D * i]);
+      Fm[i][j] = F[i + D * j];
+      G[i][j] = Fm[i][j] - (i == j ? (T)1 : (T)0);
+    }
+  T trS = 0;
+#pragma unroll
+  for (int i = 0; i < D; i++) trS += S[i][i];
+  T E[D][D];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = i; j < D; j++) {
+      T e = G[i][j] + G[j][i];
+#pragma unroll
+      for (int k = 0; k < D; k++) e += G[i][k] * G[j][k];
+      E[i][j] = e;
+      E[j][i] = e;
+    }
+  T cof[D][D], Jm1;
+  if constexpr (D == 3) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        cof[i][j] = Fm[i1][j1] * Fm[i2][j2] - Fm[i1][j2] * Fm[i2][j1];
+      }
+    const T tr = G[0][0] + G[1][1] + G[2][2];
+    const T m2 = (G[0][0] * G[1][1] - G[0][1] * G[1][0]) + (G[0][0] * G[2][2] - G[0][2] * G[2][0]) +
+                 (G[1][1] * G[2][2] - G[1][2] * G[2][1]);
+    const T dg = G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1]) - G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0]) +
+                 G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
+    Jm1 = tr + m2 + dg;
+  } else {
+    cof[0][0] = Fm[1][1];
+    cof[0][1] = -Fm[1][0];
+    cof[1][0] = -Fm[0][1];
+    cof[1][1] = Fm[0][0];
+    Jm1 = G[0][0] + G[1][1] + (G[0][0] * G[1][1] - G[0][1] * G[1][0]);
+  }
+  const T J = (T)1 + Jm1;
+  if (!(J > (T)0)) return 1;
+  T SF[D][D];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      T b = 0;
+#pragma unroll
+      for (int k = 0; k < D; k++) b += S[i][k] * Fm[k][j];
+      SF[i][j] = b;
+    }
+  if (model == 1) {
+    if (!(J >= thr)) return 2;
+    T trSE = 0;
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) trSE += S[i][j] * E[i][j];
+    mub = (double)(vol * trSE);
+    lab = (double)(vol * tlog1p<T>(Jm1) * trS);
+    const T cl = la * trS / J;
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) Fbar[i + D * j] = vol * ((T)2 * mu * SF[i][j] + cl * cof[i][j]);
+    return 0;
+  }
+  // fixed corotated: eigenbasis of E
+  T Ed[D][D], Q[D][D];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) Ed[i][j] = E[i][j];
+  sym_eigen<T, D>(Ed, Q);
+  T e[D], sg[D], emin = Ed[0][0], emax = Ed[0][0];
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    e[i] = Ed[i][i];
+    emin = e[i] < emin ? e[i] : emin;
+    emax = e[i] > emax ? e[i] : emax;
+  }
+  if ((T)1 + emin < (T)1e-4 * ((T)1 + emax)) return 1;
+#pragma unroll
+  for (int i = 0; i < D; i++) sg[i] = tsqrt<T>((T)1 + e[i]);
+  // W = Q^T S Q
+  T SQ[D][D], W[D][D];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      T a = 0;
+#pragma unroll
+      for (int k = 0; k < D; k++) a += S[i][k] * Q[k][j];
+      SQ[i][j] = a;
+    }
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      T a = 0;
+#pragma unroll
+      for (int k = 0; k < D; k++) a += Q[k][i] * SQ[k][j];
+      W[i][j] = a;
+    }
+  T mus = 0;
+#pragma unroll
+  for (int i = 0; i < D; i++) mus += W[i][i] * sg[i] * (e[i] / ((T)1 + sg[i]));
+  mub = (double)((T)2 * vol * mus);
+  lab = (double)(vol * Jm1 * J * trS);
+  // K = Q [W_ij / (s_i + s_j)] Q^T;  Cbar = 2 mu vol (S - K)
+  T Wd[D][D], KQ[D][D], Cb[D][D];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) Wd[i][j] = W[i][j] / (sg[i] + sg[j]);
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      T a = 0;
+#pragma unroll
+      for (int k = 0; k < D; k++) a += Q[i][k] * Wd[k][j];
+      KQ[i][j] = a;
+    }
+  const T c2 = (T)2 * mu * vol;
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      T a = 0;
+#pragma unroll
+      for (int k = 0; k < D; k++) a += KQ[i][k] * Q[j][k];
+      Cb[i][j] = c2 * (S[i][j] - a);
+    }
+  const T cj = vol * la * ((T)2 * J - (T)1) * trS;
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      T a = 0;
+#pragma unroll
+      for (int k = 0; k < D; k++) a += Cb[i][k] * Fm[k][j];
+      Fbar[i + D * j] = (T)2 * a + cj * cof[i][j];
+    }
+  return 0;
+}
+
 }  // namespace mpmx

@@ -2397,18 +2397,32 @@ __global__ __launch_bounds__(BLOCK) void p2g_adjoint_kernel(Args<T, D> a, AdjArg
     for (int c = 0; c < D; c++)
 #pragma unroll
       for (int j = 0; j < D; j++) q.aB[(c + D * j) * a.n + p] = Cbar[c + D * j] * Dinv[j];
-    double Fd[D * D], tb[D * D], Fbar[D * D], mub = 0, lab = 0;
-#pragma unroll
-    for (int k = 0; k < D * D; k++) {
-      Fd[k] = (double)Fm[k];
-      tb[k] = (double)tbar[k];
-    }
-    if (!kirchhoff_adjoint<D>(a.model, (double)mu, (double)la, (double)a.thr, (double)vol, Fd, tb, Fbar, mub, lab)) {
+    // the stress adjoint in the context precision; FP64 (the SVD form) only
+    // for an inverted or nearly singular F
+    double mub = 0, lab = 0;
+    T Fbt[D * D];
+    int st = 1;
+    if constexpr (sizeof(T) == 4) st = kirchhoff_adjoint_fast<T, D>(a.model, mu, la, a.thr, vol, Fm, tbar, Fbt, mub, lab);
+    if (st == 2) {
       atomicOr(a.err + 2, 1u);
       continue;
     }
+    if (st == 1) {
+      double Fd[D * D], tb[D * D], Fbar[D * D];
 #pragma unroll
-    for (int k = 0; k < D * D; k++) q.aF[k * a.n + p] += (T)Fbar[k];
+      for (int k = 0; k < D * D; k++) {
+        Fd[k] = (double)Fm[k];
+        tb[k] = (double)tbar[k];
+      }
+      if (!kirchhoff_adjoint<D>(a.model, (double)mu, (double)la, (double)a.thr, (double)vol, Fd, tb, Fbar, mub, lab)) {
+        atomicOr(a.err + 2, 1u);
+        continue;
+      }
+#pragma unroll
+      for (int k = 0; k < D * D; k++) Fbt[k] = (T)Fbar[k];
+    }
+#pragma unroll
+    for (int k = 0; k < D * D; k++) q.aF[k * a.n + p] += Fbt[k];
     q.amu[p] += mub;
     q.ala[p] += lab;
   }
