@@ -230,6 +230,37 @@ int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers, int64_t* slo
    `interval` steps (>=1) have passed since the last reorder. */
 int mpmx_set_reorder_interval(mpmx_ctx* ctx, int interval);
 
+/* Launch configuration of the fused step (no counterpart in the reference,
+   whose step runs on the host).  graph = 1: the fused steps of mpmx_step and
+   mpmx_step_async run four at a time from one captured hipGraph, re-captured
+   when the arguments change (used while timing and tapes are off; other
+   steps run as before); bins are rebuilt between groups of four when due.
+   tile_workgroups: workgroups the tile kernels launch, each striding over the
+   tile list (0 = one per tile up to 8192; the resident count of the fused
+   kernel or a small multiple of it makes them persistent).  Results do not
+   depend on the configuration beyond floating-point summation order.  For a
+   batch context all scenes run in these same launches. */
+typedef struct mpmx_launch_config {
+  int graph;
+  int tile_workgroups;
+} mpmx_launch_config;
+int mpmx_set_launch_config(mpmx_ctx* ctx, const mpmx_launch_config* cfg);
+int mpmx_get_launch_config(mpmx_ctx* ctx, mpmx_launch_config* cfg);
+
+/* Autotuner of the launch configuration (no counterpart in the reference):
+   times trial_steps (>= 8) fused steps of dt of the context's own particles
+   after 4 untimed ones, for each candidate in this order: graph off, then on,
+   each with tile_workgroups 0, then 1, 2 and 4 times the fused kernel's
+   resident workgroup count; applies the fastest by wall clock, returns it in
+   *chosen, and writes each candidate's milliseconds per step to times_ms
+   (up to max_times entries; *ntimes = number of candidates).  The particles,
+   the simulation time and the timing switch are restored afterwards; the
+   bins are rebuilt at the next step.  Needs particles and a step the fused
+   kernels take (no plasticity, dt limits, reflection, grid stats or tape:
+   MPMX_ESTATE). */
+int mpmx_autotune(mpmx_ctx* ctx, double dt, int trial_steps, mpmx_launch_config* chosen, double* times_ms,
+                  int max_times, int* ntimes);
+
 /* Adaptive time step.
    mpmx_grid_v_upper_bound replaces MPM_DRIVER::Grid_V_Upper_Bound
    (Hybrid_Methods/Examples_And_Drivers/MPM_DRIVER.cpp:1125-1138, affine CFL
@@ -406,7 +437,8 @@ int mpmx_backward(mpmx_ctx* ctx, const void* dX, const void* dV, const void* dF,
    grid-tile list and clearing), "p2g", "grid", "g2p", "reorder", "step",
    "adjoint" (a whole mpmx_backward pass), "dt" (the dt limits of a step),
    "g2p2g" (a fused step's G2P and next P2G with their mover side paths) and
-   "g2p2g_kernel" (the fused kernel of that phase alone).
+   "g2p2g_kernel" (the fused kernel of that phase alone); "graph" counts the
+   graph replays (groups of four fused steps) whether timing is on or not.
    mpmx_phase_time returns the accumulated milliseconds and launch count since
    the last reset. */
 int mpmx_enable_timing(mpmx_ctx* ctx, int enable);

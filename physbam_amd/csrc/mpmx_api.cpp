@@ -198,6 +198,23 @@ int mpmx_set_rebin_policy(mpmx_ctx* ctx, double mover_fraction, int max_interval
   return guard([&] { ctx->engine->set_rebin_policy(mover_fraction, max_interval); });
 }
 
+int mpmx_set_launch_config(mpmx_ctx* ctx, const mpmx_launch_config* cfg) {
+  if (!ctx || !cfg) return bad("null argument");
+  return guard([&] { ctx->engine->set_launch_config(*cfg); });
+}
+
+int mpmx_get_launch_config(mpmx_ctx* ctx, mpmx_launch_config* cfg) {
+  if (!ctx || !cfg) return bad("null argument");
+  return guard([&] { *cfg = ctx->engine->launch_config(); });
+}
+
+int mpmx_autotune(mpmx_ctx* ctx, double dt, int trial_steps, mpmx_launch_config* chosen, double* times_ms,
+                  int max_times, int* ntimes) {
+  if (!ctx) return bad("null context");
+  if (max_times > 0 && !times_ms) return bad("null times array");
+  return guard([&] { ctx->engine->autotune(dt, trial_steps, chosen, times_ms, max_times, ntimes); });
+}
+
 int mpmx_bin_stats(mpmx_ctx* ctx, int64_t* rebins, int64_t* movers, int64_t* slow) {
   if (!ctx) return bad("null context");
   return guard([&] { ctx->engine->bin_stats(rebins, movers, slow); });

@@ -45,6 +45,10 @@ struct EngineBase {
   virtual void get_grid(void* mass, void* velocity) = 0;
   virtual void set_reorder_interval(int interval) = 0;
   virtual void set_rebin_policy(double mover_fraction, int max_interval) = 0;
+  virtual void set_launch_config(const mpmx_launch_config& cfg) = 0;
+  virtual mpmx_launch_config launch_config() const = 0;
+  virtual void autotune(double dt, int trial_steps, mpmx_launch_config* chosen, double* times_ms, int max_times,
+                        int* ntimes) = 0;
   virtual void bin_stats(int64_t* rebins, int64_t* movers, int64_t* slow) = 0;
   virtual void grid_v_upper_bound(double* out) = 0;
   virtual void compute_dt(double cfl, double max_dt, double* dt) = 0;

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -3569,10 +3570,9 @@ class Engine final : public EngineBase {
     HIPCHECK(hipStreamSynchronize(stream_));
   }
   ~Engine() override {
-    if (stream_) {
-      (void)hipStreamSynchronize(stream_);
-      (void)hipStreamDestroy(stream_);
-    }
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    release_graph();
+    if (stream_) (void)hipStreamDestroy(stream_);
     for (int i = 0; i < 4; i++)
       if (rb_ev_[i]) (void)hipEventDestroy(rb_ev_[i]);
     for (hipEvent_t e : {seg_done_[0], seg_done_[1], seg_free_[0], seg_free_[1], rep_go_})
@@ -3747,11 +3747,15 @@ class Engine final : public EngineBase {
     // grid statistics read each step's P2G grid, which the fused kernel never
     // holds on its own
     const bool fuse = n_ > 0 && !plastic_ && !limits_on() && !refl_on_ && !stats_on_;
-    for (int s = 0; s < nsteps; s++) {
-      if (fuse)
-        step_fused((T)dt);
-      else
-        step_once((T)dt);
+    if (fuse && use_graph_ && !timer_.on && !taping_) {
+      step_graph((T)dt, nsteps);
+    } else {
+      for (int s = 0; s < nsteps; s++) {
+        if (fuse)
+          step_fused((T)dt);
+        else
+          step_once((T)dt);
+      }
     }
     if (wait) synchronize();
   }
@@ -3946,6 +3950,73 @@ class Engine final : public EngineBase {
     HIPCHECK(hipStreamSynchronize(stream_));
   }
 
+  void set_launch_config(const mpmx_launch_config& cfg) override {
+    if (cfg.graph != 0 && cfg.graph != 1) throw Error{MPMX_EINVAL, "graph must be 0 or 1"};
+    if (cfg.tile_workgroups < 0 || cfg.tile_workgroups > (1 << 20))
+      throw Error{MPMX_EINVAL, "tile_workgroups must be in [0, 2^20]"};
+    use_graph_ = cfg.graph != 0;
+    tile_wg_ = (uint32_t)cfg.tile_workgroups;
+    release_graph();
+  }
+  mpmx_launch_config launch_config() const override {
+    mpmx_launch_config c;
+    c.graph = use_graph_ ? 1 : 0;
+    c.tile_workgroups = (int)tile_wg_;
+    return c;
+  }
+  // Times each candidate configuration on the context's own particles (saved
+  // and restored), applies the fastest.
+  void autotune(double dt, int trial, mpmx_launch_config* chosen, double* times, int max_times,
+                int* ntimes) override {
+    if (!(dt > 0)) throw Error{MPMX_EINVAL, "dt must be positive"};
+    if (trial < 8) throw Error{MPMX_EINVAL, "trial_steps must be at least 8"};
+    if (n_ == 0) throw Error{MPMX_ESTATE, "autotuning needs particles"};
+    if (taping_ || stage_next_ != 0) throw Error{MPMX_ESTATE, "autotuning needs no tape and no open staged step"};
+    if (plastic_ || limits_on() || refl_on_ || stats_on_)
+      throw Error{MPMX_ESTATE, "autotuning times the fused step: no plasticity, dt limits, reflection or grid stats"};
+    int cus = 0, per = 0;
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, g2p2g_kernel<T, D, false>, BLOCK, 0));
+    const uint32_t resident = (uint32_t)std::max(1, per) * (uint32_t)std::max(1, cus);
+    std::vector<mpmx_launch_config> cand;
+    for (int gr = 0; gr < 2; gr++)
+      for (uint32_t w : {0u, resident, 2 * resident, 4 * resident}) cand.push_back(mpmx_launch_config{gr, (int)w});
+    const mpmx_launch_config before = launch_config();
+    const bool timing = timer_.on;
+    const double t_before = time_, last_dt_before = last_dt_;
+    StateBuf saved;
+    save_state(saved);
+    std::vector<double> ms(cand.size(), 0.0);
+    try {
+      timer_.on = false;
+      for (size_t c = 0; c < cand.size(); c++) {
+        set_launch_config(cand[c]);
+        load_state(saved);
+        step(dt, 4, true);
+        const auto t0 = std::chrono::steady_clock::now();
+        step(dt, trial, true);
+        const auto t1 = std::chrono::steady_clock::now();
+        ms[c] = std::chrono::duration<double, std::milli>(t1 - t0).count() / trial;
+      }
+    } catch (...) {
+      set_launch_config(before);
+      timer_.on = timing;
+      throw;
+    }
+    size_t best = 0;
+    for (size_t c = 1; c < cand.size(); c++)
+      if (ms[c] < ms[best]) best = c;
+    set_launch_config(cand[best]);
+    load_state(saved);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    time_ = t_before;
+    last_dt_ = last_dt_before;
+    timer_.on = timing;
+    if (chosen) *chosen = cand[best];
+    for (int i = 0; i < max_times && i < (int)cand.size(); i++) times[i] = ms[i];
+    if (ntimes) *ntimes = (int)cand.size();
+  }
+
   void set_reorder_interval(int k) override {
     if (k < 1) throw Error{MPMX_EINVAL, "reorder interval must be >= 1"};
     reorder_interval_ = k;
@@ -4068,6 +4139,11 @@ class Engine final : public EngineBase {
   }
   bool phase_time(const char* phase, double* total_ms, int64_t* count) override {
     synchronize();
+    if (phase && std::strcmp(phase, "graph") == 0) {  // graph replays (groups of four steps) so far; no time
+      if (total_ms) *total_ms = 0;
+      if (count) *count = graph_groups_;
+      return true;
+    }
     for (int i = 0; i < PhaseTimer::NPH; i++)
       if (phase && std::strcmp(phase, kPhaseNames[i]) == 0) {
         if (total_ms) *total_ms = timer_.total[i];
@@ -4548,7 +4624,8 @@ class Engine final : public EngineBase {
   }
 
   Args<T, D> make_args(T dt) {
-    Args<T, D> a{};
+    Args<T, D> a;
+    std::memset(&a, 0, sizeof(a));  // padding too: graph_signature compares the bytes
     a.g = g_;
     const bool batch = scenes_.size() > 1;
     a.scenes = scenes_dev_.p;
@@ -4598,7 +4675,11 @@ class Engine final : public EngineBase {
       if (walls_[i].friction != (T)0 && walls_[i].type != 0) return true;
     return false;
   }
-  uint32_t tile_blocks() const { return std::min<uint32_t>(std::max<uint32_t>(ntotal_, 1), 8192); }
+  // workgroups of the tile kernels (grid-stride over the tile list): the
+  // launch configuration's count, else one per tile up to 8192
+  uint32_t tile_blocks() const {
+    return tile_wg_ ? tile_wg_ : std::min<uint32_t>(std::max<uint32_t>(ntotal_, 1), 8192);
+  }
   uint32_t grid_blocks() const { return (uint32_t)stats_blocks_; }
 
   // Rebuild the bins when forced, when max_interval steps have passed, or when
@@ -5030,6 +5111,14 @@ class Engine final : public EngineBase {
       rebin();
       fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 0);
     }
+    fused_body(dt, fric, false);
+    timer_.end(PH_STEP, e_step, stream_);
+  }
+
+  // The GPU work of one fused step on the stream, then its host-side advance.
+  // While the stream is captured into a graph (capturing) no event is
+  // recorded; the replay records the readback events itself.
+  void fused_body(T dt, bool fric, bool capturing) {
     Args<T, D> a = make_args(dt);
     a.order = identity_order_ ? nullptr : order_.p;
     const uint32_t ms_next = (uint32_t)(step_count_ + 2);  // mover stamp of the next step's grid
@@ -5062,8 +5151,10 @@ class Engine final : public EngineBase {
     timer_.end(PH_G2P2G, e3, stream_);
     const int slot = (int)(step_count_ & 3);
     HIPCHECK(hipMemcpyAsync(rb_host_ + slot, counts_.p + 9, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
-    HIPCHECK(hipEventRecord(rb_ev_[slot], stream_));
-    rb_meta_[slot] = Readback{step_count_, bin_stamp_};
+    if (!capturing) {
+      HIPCHECK(hipEventRecord(rb_ev_[slot], stream_));
+      rb_meta_[slot] = Readback{step_count_, bin_stamp_};
+    }
 
     // tile list of the next step's grid (now in gother_): binned tiles and
     // the tiles marked by this step's movers; the friction grid is cleared
@@ -5074,6 +5165,12 @@ class Engine final : public EngineBase {
     glist_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms_next, gblock_.p, gt_other_,
                                                         counts_.p + 1, fric ? gridf_.p : nullptr, nullptr, nullptr);
     timer_.end(PH_BIN, e0, stream_);
+    HIPCHECK(hipGetLastError());
+    fused_advance(dt);
+  }
+  // Host-side state of one fused step: the two grids, their tile lists and
+  // the mover lists swap roles; counters and time advance.
+  void fused_advance(T dt) {
     std::swap(gcur_, gother_);
     std::swap(gt_cur_, gt_other_);
     std::swap(ml_cur_, ml_next_);
@@ -5087,12 +5184,82 @@ class Engine final : public EngineBase {
         save_state(next_checkpoint());
       }
     }
-    HIPCHECK(hipGetLastError());
     has_grid_ = true;
     fused_last_ = true;
     time_ += (double)dt;
     last_dt_ = (double)dt;
-    timer_.end(PH_STEP, e_step, stream_);
+  }
+
+  // ---------------------------------------------------------------- graphs --
+  // Fused steps replayed four at a time from one captured hipGraph (the
+  // launch configuration's `graph`): the four steps bring the swapped grids and
+  // lists back to their places and fill the four readback slots, so one graph
+  // serves every group that starts at a step count divisible by four with the
+  // same arguments.  The graph's mover stamps repeat every four steps: a tile
+  // marked four steps earlier and not since may join a grid-tile list again,
+  // which adds a zero tile (cleared, skipped by the update) and never drops
+  // one.  Bins are rebuilt between groups when due.
+  std::vector<unsigned char> graph_signature(T dt, bool fric) {
+    Args<T, D> a = make_args(dt);
+    a.order = identity_order_ ? nullptr : order_.p;
+    a.next_stamp = 0;
+    std::vector<unsigned char> sig(sizeof(a) + 64, 0);
+    std::memcpy(sig.data(), &a, sizeof(a));
+    unsigned char* q = sig.data() + sizeof(a);
+    const uint64_t extra[6] = {bin_stamp_, (uint64_t)fric, tile_blocks(), (uint64_t)(uintptr_t)gother_,
+                               (uint64_t)(uintptr_t)gt_other_, (uint64_t)(uintptr_t)gridf_.p};
+    std::memcpy(q, extra, sizeof(extra));
+    return sig;
+  }
+  void release_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    graph_sig_.clear();
+  }
+  // nsteps fused steps, groups of four from the graph, the rest one by one
+  void step_graph(T dt, int nsteps) {
+    while (nsteps > 0) {
+      if (!fused_ || fused_dt_ != dt || (step_count_ & 3) != 0 || nsteps < 4) {
+        step_fused(dt);
+        nsteps--;
+        continue;
+      }
+      const bool fric = any_friction();
+      if (fric) gridf_.alloc((size_t)ntotal_ * 64);
+      if (rebin_due()) {
+        rebin();
+        fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 0);
+      }
+      std::vector<unsigned char> sig = graph_signature(dt, fric);
+      const int64_t s0 = step_count_;
+      if (!graph_exec_ || sig != graph_sig_) {
+        release_graph();
+        HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+        try {
+          for (int k = 0; k < 4; k++) fused_body(dt, fric, true);
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(stream_, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIPCHECK(hipStreamEndCapture(stream_, &graph_));
+        HIPCHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        graph_sig_ = sig;
+      } else {
+        for (int k = 0; k < 4; k++) fused_advance(dt);
+      }
+      HIPCHECK(hipGraphLaunch(graph_exec_, stream_));
+      for (int k = 0; k < 4; k++) {
+        const int64_t st = s0 + k;
+        HIPCHECK(hipEventRecord(rb_ev_[st & 3], stream_));
+        rb_meta_[st & 3] = Readback{st, bin_stamp_};
+      }
+      graph_groups_++;
+      nsteps -= 4;
+    }
   }
 
   Geom<T, D> g_{};
@@ -5168,6 +5335,14 @@ class Engine final : public EngineBase {
   double last_dt_ = 0;
   bool has_grid_ = false;
   bool stats_on_ = false;
+  // launch configuration: fused steps replayed from a captured graph, and the
+  // tile kernels' workgroup count (0: one per tile up to 8192)
+  bool use_graph_ = false;
+  uint32_t tile_wg_ = 0;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  std::vector<unsigned char> graph_sig_;
+  int64_t graph_groups_ = 0;
   double time_ = 0;
   int stats_blocks_ = 1;
   ParticleBufs<T, D> cur_{}, alt_{};

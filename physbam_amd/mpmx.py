@@ -37,6 +37,7 @@ EXPORTS = (
     "mpmx_tape_begin", "mpmx_backward",
     "mpmx_enable_timing", "mpmx_reset_timing",
     "mpmx_phase_time", "mpmx_stress", "mpmx_svd", "mpmx_sort_keys", "mpmx_sort_keys_noalloc", "mpmx_seed_particles",
+    "mpmx_set_launch_config", "mpmx_get_launch_config", "mpmx_autotune",
     "mpmx_random_boxes", "mpmx_seed_particles_poisson",
 )
 
@@ -79,6 +80,11 @@ class SeedDesc(ctypes.Structure):
 
 
 _lib = None
+
+
+class LaunchConfig(ctypes.Structure):
+    """mpmx_launch_config: graph (0/1), tile_workgroups (0 = one per tile up to 8192)."""
+    _fields_ = [("graph", ctypes.c_int), ("tile_workgroups", ctypes.c_int)]
 
 
 def lib():
@@ -149,6 +155,9 @@ def lib():
         L.mpmx_svd.argtypes = [vp, i64] + [vp] * 4
         L.mpmx_sort_keys.argtypes = [vp, i64, vp, c_int, vp, vp]
         L.mpmx_sort_keys_noalloc.argtypes = [vp, i64, vp, c_int, vp, vp]
+        L.mpmx_set_launch_config.argtypes = [vp, ctypes.POINTER(LaunchConfig)]
+        L.mpmx_get_launch_config.argtypes = [vp, ctypes.POINTER(LaunchConfig)]
+        L.mpmx_autotune.argtypes = [vp, dbl, c_int, ctypes.POINTER(LaunchConfig), vp, c_int, ctypes.POINTER(c_int)]
         L.mpmx_seed_particles.argtypes = [ctypes.POINTER(SeedDesc), c_int, i64] + [vp] * 9
         L.mpmx_seed_particles.restype = i64
         L.mpmx_seed_particles_poisson.argtypes = [ctypes.POINTER(SeedDesc), c_int, dbl, i64] + [vp] * 9
@@ -373,6 +382,25 @@ class Context:
         v = np.zeros((nn, self.dim), self.dtype)
         _check(lib().mpmx_get_grid(self._h, _p(m), _p(v)))
         return m, v
+
+    def set_launch_config(self, graph=False, tile_workgroups=0):
+        c = LaunchConfig(1 if graph else 0, int(tile_workgroups))
+        _check(lib().mpmx_set_launch_config(self._h, ctypes.byref(c)))
+
+    def launch_config(self):
+        c = LaunchConfig()
+        _check(lib().mpmx_get_launch_config(self._h, ctypes.byref(c)))
+        return {"graph": bool(c.graph), "tile_workgroups": int(c.tile_workgroups)}
+
+    def autotune(self, dt, trial_steps=16):
+        """mpmx_autotune: returns (chosen config dict, list of (config dict, ms per step) per candidate)."""
+        c = LaunchConfig()
+        times = np.zeros(32)
+        nt = ctypes.c_int(0)
+        _check(lib().mpmx_autotune(self._h, float(dt), int(trial_steps), ctypes.byref(c), _p(times), 32,
+                                   ctypes.byref(nt)))
+        ch = {"graph": bool(c.graph), "tile_workgroups": int(c.tile_workgroups)}
+        return ch, [float(t) for t in times[:nt.value]]
 
     def set_reorder_interval(self, k):
         _check(lib().mpmx_set_reorder_interval(self._h, k))
