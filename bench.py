@@ -384,21 +384,42 @@ def batched(log, nscenes=256, steps=100, seed=BOX_SEED):
     ctx = mpmx.Context.batch(3, grids, precision=mpmx.FP32)
     ctx.set_constitutive(mpmx.FIXED_COROTATED)
     ctx.set_gravity([0.0, -9.8, 0.0])
-    ctx.set_batch_particles(counts, cat("X"), cat("V"), cat("F"), cat("B"), cat("mass"), cat("volume"), cat("mu"),
-                            cat("lam"), cat("valid"))
+    arrays = [cat(k) for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lam", "valid")]
     del parts
+
+    def load():
+        ctx.set_batch_particles(counts, *arrays)
+
     dt = 1e-4
-    ctx.step(dt, 5)
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    ctx.step(dt, steps)
-    sec = time.perf_counter() - t1
+    # the launch autotuner picks graph replay and the tile kernels' workgroup
+    # count on these scenes (the particles are restored afterwards)
+    load()
+    chosen, times = ctx.autotune(dt, 16)
+    log(f"batched: autotuner chose {chosen} ({', '.join(f'{t:.3f}' for t in times)} ms per step)")
+
+    def timed(graph):
+        """steps from the initial state, after 5 untimed ones"""
+        ctx.set_launch_config(graph=graph, tile_workgroups=chosen["tile_workgroups"])
+        load()
+        ctx.step(dt, 5)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        ctx.step(dt, steps)
+        return time.perf_counter() - t1
+
+    sec_off = timed(False)
+    sec_on = timed(True)
+    sec = sec_on if chosen["graph"] else sec_off
     ctx.close()
     return {"value": n * steps / sec, "unit": "particle-steps/s", "ms_per_step": sec * 1e3 / steps,
+            "launch_config": chosen, "autotune_ms_per_step": times,
+            "ms_per_step_graph_off": sec_off * 1e3 / steps, "ms_per_step_graph_on": sec_on * 1e3 / steps,
             "scenes": nscenes, "particles": n, "particles_min": min(counts), "particles_max": max(counts),
             "steps": steps,
             "workload": f"{nscenes} independent 3D scenes, N in {{32,48,64,96}}^3, one sphere r U[0.1,0.3], 8 ppc, "
-                        "fixed-corotated, E log-U[1e3,1e5], nu U[0.2,0.4], gravity, dt 1e-4, FP32, one context"}
+                        "fixed-corotated, E log-U[1e3,1e5], nu U[0.2,0.4], gravity, dt 1e-4, FP32, one context; "
+                        "autotuned launch configuration (candidate times in autotune_ms_per_step: graph off, then "
+                        "on, each with tile workgroups 0, 1x, 2x, 4x resident)"}
 
 
 def adjoint_ratio(log, steps=200, checkpoint=20):

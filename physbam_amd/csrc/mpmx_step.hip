@@ -3984,9 +3984,17 @@ class Engine final : public EngineBase {
     const mpmx_launch_config before = launch_config();
     const bool timing = timer_.on;
     const double t_before = time_, last_dt_before = last_dt_;
+    // storage in bin order first (as the steps keep it), then saved: the trials
+    // rebuild the bins but do not reorder, so the saved state stays in slot
+    // order and every trial reads nearly coalesced particles
+    force_reorder_ = true;
+    rebin();
+    fused_ = false;
+    fused_last_ = false;
     StateBuf saved;
     save_state(saved);
     std::vector<double> ms(cand.size(), 0.0);
+    tuning_ = true;
     try {
       timer_.on = false;
       for (size_t c = 0; c < cand.size(); c++) {
@@ -4001,6 +4009,7 @@ class Engine final : public EngineBase {
     } catch (...) {
       set_launch_config(before);
       timer_.on = timing;
+      tuning_ = false;
       throw;
     }
     size_t best = 0;
@@ -4012,6 +4021,7 @@ class Engine final : public EngineBase {
     time_ = t_before;
     last_dt_ = last_dt_before;
     timer_.on = timing;
+    tuning_ = false;
     if (chosen) *chosen = cand[best];
     for (int i = 0; i < max_times && i < (int)cand.size(); i++) times[i] = ms[i];
     if (ntimes) *ntimes = (int)cand.size();
@@ -4713,7 +4723,8 @@ class Engine final : public EngineBase {
                                                          counts_.p + 0, nullptr, nullptr, nullptr);
     timer_.end(PH_REBIN, e0, stream_);
     identity_order_ = false;
-    if (!taping_ && (force_reorder_ || since_reorder_ >= reorder_interval_)) {
+    // storage stays in place while a tape or the autotuner holds saved states
+    if (!taping_ && !tuning_ && (force_reorder_ || since_reorder_ >= reorder_interval_)) {
       hipEvent_t er = timer_.begin(stream_);
       reorder_kernel<T, D><<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(cur_, alt_, order_.p, n_);
       user_order_ = false;
@@ -5338,6 +5349,7 @@ class Engine final : public EngineBase {
   // launch configuration: fused steps replayed from a captured graph, and the
   // tile kernels' workgroup count (0: one per tile up to 8192)
   bool use_graph_ = false;
+  bool tuning_ = false;  // mpmx_autotune is timing candidates
   uint32_t tile_wg_ = 0;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;

@@ -76,14 +76,16 @@ def batched_workload():
 
 
 @pytest.mark.timeout(900)
-def test_batched_workload_at_full_shape(batched_workload):
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_workload_at_full_shape(batched_workload, graph):
     """The batched benchmark workload itself: 256 scenes of 32^3..96^3 cells,
     one sphere each (the generator the benchmark times), fixed corotated,
-    gravity, FP32, dt 1e-4, in one context.  After 3 steps the smallest and
-    the largest scene of each grid size match their own FP64 oracle runs to
-    1e-4 (X, V, F; B by the velocity its error carries into the next P2G;
-    valid flags equal); after 10 steps no particle is lost
-    and each scene's momentum is M g t (no walls: the internal forces and the
+    gravity, FP32, dt 1e-4, in one context.  After 3 steps (12 with graph:
+    four one by one, then two hipGraph replays of four fused steps) the
+    smallest and the largest scene of each grid size match their own FP64
+    oracle runs to 1e-4 (X, V, F; B by the velocity its error carries into the
+    next P2G; valid flags equal); after 10 steps (16) no particle is lost and
+    each scene's momentum is M g t (no walls: the internal forces and the
     APIC transfers add no momentum), to 1e-5 of M |g| t."""
     sizes, parts = batched_workload
     counts = [int(P["X"].shape[0]) for P in parts]
@@ -97,10 +99,19 @@ def test_batched_workload_at_full_shape(batched_workload):
         ctx.set_gravity(list(g))
         ctx.set_batch_particles(counts, cat("X"), cat("V"), cat("F"), cat("B"), cat("mass"), cat("volume"),
                                 cat("mu"), cat("lam"), cat("valid"))
-        ctx.step(dt, 3)
-        at3 = ctx.get_particles()
-        ctx.step(dt, 7)
-        at10 = ctx.get_particles()
+        ctx.set_launch_config(graph=graph)
+        if graph:  # four steps one by one, then two graph replays of four
+            ctx.step(dt, 4)
+            ctx.step(dt, 8)
+            ncmp = 12
+            assert ctx.phase_time("graph")[1] == 2
+        else:
+            ctx.step(dt, 3)
+            ncmp = 3
+        at_cmp = ctx.get_particles()
+        nend = ncmp + (4 if graph else 7)
+        ctx.step(dt, nend - ncmp)
+        at_end = ctx.get_particles()
     offs = np.concatenate([[0], np.cumsum(counts)])
     picks = set()
     for N in (32, 48, 64, 96):
@@ -114,8 +125,8 @@ def test_batched_workload_at_full_shape(batched_workload):
         sc = dict(dim=3, prec=4, model=mpmx.FIXED_COROTATED, N=[sizes[s]] * 3, lo=[0.0] * 3, hi=[1.0] * 3,
                   gravity=list(g), dt=dt, neo_threshold=0.25, walls=[], X=P["X"], V=P["V"], F=P["F"], B=P["B"],
                   mass=P["mass"], volume=P["volume"], mu=P["mu"], valid=P["valid"], **{"lambda": P["lam"]})
-        orc = mt.oracle_run(sc, 3, prec=8, want_grid=False)
-        part = {k: at3[k][offs[s]:offs[s + 1]] for k in ("X", "V", "F", "B", "valid")}
+        orc = mt.oracle_run(sc, ncmp, prec=8, want_grid=False)
+        part = {k: at_cmp[k][offs[s]:offs[s + 1]] for k in ("X", "V", "F", "B", "valid")}
         assert np.array_equal(part["valid"], orc["valid"]), s
         errs = {k: float(np.abs(part[k] - orc[k]).max() / max(np.abs(orc[k]).max(), 1e-30))
                 for k in ("X", "V", "F")}
@@ -126,11 +137,11 @@ def test_batched_workload_at_full_shape(batched_workload):
               flush=True)
         for k, v in errs.items():
             assert v <= 1e-4, (s, k, v)
-    assert np.all(at10["valid"] == 1)
-    t = 10 * dt
+    assert np.all(at_end["valid"] == 1)
+    t = nend * dt
     for s in range(len(parts)):
         m = parts[s]["mass"].astype(np.float64)
-        V = at10["V"][offs[s]:offs[s + 1]].astype(np.float64)
+        V = at_end["V"][offs[s]:offs[s + 1]].astype(np.float64)
         M = m.sum()
         err = np.abs((m[:, None] * V).sum(0) - M * g * t).max() / (M * 9.8 * t)
         assert err <= 1e-5, (s, err)
