@@ -622,7 +622,8 @@ template<> struct SeriesRadius<double> { static constexpr double r2 = 1.0 / 6553
 // symmetric: only the upper triangle is formed.
 // HENCKY: ln(I + E) = E - E^2/2 + E^3/3 - ... + E^7/7 instead (a template
 // flag, so that the coefficients are immediates).
-template<typename T, int D, bool HENCKY>
+// DEG < 7 truncates the series at x^DEG (Horner from c_DEG).
+template<typename T, int D, bool HENCKY, int DEG = 7>
 __device__ __forceinline__ void corot_series(const T (&E)[D][D], T (&f)[D][D]) {
   const T c[8] = {(T)0,
                   HENCKY ? (T)1 : (T)0.5,
@@ -636,9 +637,9 @@ __device__ __forceinline__ void corot_series(const T (&E)[D][D], T (&f)[D][D]) {
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
-    for (int j = 0; j < D; j++) P[i][j] = c[7] * E[i][j] + (i == j ? c[6] : (T)0);
+    for (int j = 0; j < D; j++) P[i][j] = c[DEG] * E[i][j] + (i == j ? c[DEG - 1] : (T)0);
 #pragma unroll
-  for (int k = 5; k >= 0; k--) {
+  for (int k = DEG - 2; k >= 0; k--) {
     T Q[D][D];
 #pragma unroll
     for (int i = 0; i < D; i++)
@@ -717,10 +718,23 @@ __device__ __forceinline__ bool kirchhoff_fast(int model, T mu, T la, T thr, T v
     // the log series converges more slowly: a quarter of the radius^2
     if (r2 <= (hencky ? SeriesRadius<T>::r2 * (T)0.25 : SeriesRadius<T>::r2)) {
       T f[D][D];
-      if (hencky)
+      if (hencky) {
         corot_series<T, D, true>(E, f);
-      else
+      } else if constexpr (sizeof(T) == 4) {
+        // FP32 fixed corotated: the series' relative truncation error after
+        // x^K is about 2 |c_(K+1)| |E|^K, below 2e-8 for K = 7 at
+        // |E|_F <= 1/8, for K = 5 at |E|_F <= 1/20 and for K = 3 at
+        // |E|_F <= 1/160; the degree is the smallest that covers every lane
+        // of the wavefront (a uniform branch)
+        if (__any(r2 > (T)(1.0 / 400)))
+          corot_series<T, D, false, 7>(E, f);
+        else if (__any(r2 > (T)(1.0 / 25600)))
+          corot_series<T, D, false, 5>(E, f);
+        else
+          corot_series<T, D, false, 3>(E, f);
+      } else {
         corot_series<T, D, false>(E, f);
+      }
       const T pj = hencky ? la * tlog1p<T>(Jm1) : la * Jm1 * J, m2 = hencky ? mu : (T)2 * mu;
 #pragma unroll
       for (int i = 0; i < D; i++)
