@@ -409,32 +409,32 @@ template<typename T, int D, bool FRICTION, bool FUSED>
 __device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, const Geom<T, D>& g, const int* org,
                                               const typename Vec4Type<T>::type* sv);
 
-// Phase B of the tile P2G: the particle in coefficient slot sl, whose base
-// node is region node base_li, is added into this wavefront's copy of the
-// tile's stencil region.  Lanes of a wavefront hold distinct cells, so the nodes of
-// one read-modify-write step are distinct; the compiler barrier after each
-// write keeps the steps in order (a later step of one lane may touch the node
-// an earlier step of another lane wrote).
+// The P2G of one particle with transfer coefficients cf (layout C), whose
+// base node is region node base_li, added into this wavefront's copy of the
+// tile's stencil region.  The active lanes of a wavefront hold distinct cells,
+// so the nodes of one read-modify-write step are distinct; the compiler
+// barrier after each write keeps the steps in order (a later step of one lane
+// may touch the node an earlier step of another lane wrote).
 template<typename T, int D, class C = P2GCoef<T, D>>
-__device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*s_c)[BLOCK], int sl, int base_li,
+__device__ __forceinline__ void p2g_scatter_vals(const Geom<T, D>& g, const T* cf, int base_li,
                                                  typename Vec4Type<T>::type* reg) {
   constexpr int R = Tile<D>::R;
   using T4 = typename Vec4Type<T>::type;
-  const T m = C::MASS ? s_c[C::M][sl] : (T)0;
+  const T m = C::MASS ? cf[C::M] : (T)0;
   T mv[D], mC[D][D], Q[D][D], w[3][D], dw[3][D], dz[3][D];
 #pragma unroll
-  for (int c = 0; c < D; c++) mv[c] = s_c[C::MV + c][sl];
+  for (int c = 0; c < D; c++) mv[c] = cf[C::MV + c];
 #pragma unroll
   for (int j = 0; j < D; j++)
 #pragma unroll
-    for (int c = 0; c < D; c++) mC[c][j] = s_c[C::MC + c + D * j][sl];
+    for (int c = 0; c < D; c++) mC[c][j] = cf[C::MC + c + D * j];
   if constexpr (C::SYM) {
     int qq = 0;
 #pragma unroll
     for (int c = 0; c < D; c++)
 #pragma unroll
       for (int j = c; j < D; j++) {
-        const T qv = s_c[C::Q + qq++][sl];
+        const T qv = cf[C::Q + qq++];
         Q[c][j] = qv;
         Q[j][c] = qv;
       }
@@ -442,11 +442,11 @@ __device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*
 #pragma unroll
     for (int j = 0; j < D; j++)
 #pragma unroll
-      for (int c = 0; c < D; c++) Q[c][j] = s_c[C::Q + c + D * j][sl];
+      for (int c = 0; c < D; c++) Q[c][j] = cf[C::Q + c + D * j];
   }
 #pragma unroll
   for (int c = 0; c < D; c++) {
-    const T xe = s_c[C::XE + c][sl];
+    const T xe = cf[C::XE + c];
     const T inv = g.inv_dx[c];
     const T xx = xe * inv;
     const T q0 = xx - (T)1.5, q1 = xx - (T)1, q2 = xx - (T)0.5;
@@ -543,6 +543,18 @@ __device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*
       }
     }
   }
+}
+
+// Phase B of the tile P2G: the particle in coefficient slot sl, whose base
+// node is region node base_li, is added into this wavefront's copy of the
+// tile's stencil region (p2g_scatter_vals on the slot's coefficients).
+template<typename T, int D, class C = P2GCoef<T, D>>
+__device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*s_c)[BLOCK], int sl, int base_li,
+                                                 typename Vec4Type<T>::type* reg) {
+  T cf[C::N];
+#pragma unroll
+  for (int q = 0; q < C::N; q++) cf[q] = s_c[q][sl];
+  p2g_scatter_vals<T, D, C>(g, cf, base_li, reg);
 }
 
 // FUSED (g2p2g): each tile first runs the G2P of the current step
@@ -1601,6 +1613,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
     __syncthreads();
   }
 }
+
 
 // Zeroes the grid tiles of a tile list (count at *count).
 template<typename T>
