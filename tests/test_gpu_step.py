@@ -211,6 +211,51 @@ def test_svd_kernel():
 
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("dim", [2, 3])
+def test_svd_factors_match_the_oracle(dim, prec):
+    """mpmx_svd in 2D and 3D, FP32 and FP64: the reference's convention
+    (MATRIX_3X3.cpp:198-236, MATRIX_2X2.h:313-322: U, V rotations, |s| sorted
+    descending, only the last singular value negative), reconstruction and
+    orthonormality to the precision, and U, s, V equal to the oracle's (the
+    reference's own algorithm, same precision input) where the singular values
+    are separated by more than 5 % (then the factors are unique; closer ones
+    leave U and V free inside the near-degenerate subspace)."""
+    rng = np.random.default_rng(17 + dim)
+    n = 4096
+    d = dim
+    F = np.tile(np.eye(d).reshape(-1), (n, 1)) + 0.8 * (rng.random((n, d * d)) - 0.5)
+    F[: n // 8] *= -1  # inverted ones (det < 0 in 3D)
+    eps = 1e-12 if prec == 8 else 3e-6
+    with mpmx.Context(d, 8, [0] * d, [1] * d, precision=prec) as ctx:
+        U, S, V = ctx.svd(F)
+    dt_np = np.float64 if prec == 8 else np.float32
+    Fin = F.astype(dt_np).astype(np.float64)
+    Um = U.astype(np.float64).reshape(n, d, d).transpose(0, 2, 1)
+    Vm = V.astype(np.float64).reshape(n, d, d).transpose(0, 2, 1)
+    Fm = Fin.reshape(n, d, d).transpose(0, 2, 1)
+    S = S.astype(np.float64)
+    rec = Um @ (S[:, :, None] * Vm.transpose(0, 2, 1))
+    assert np.abs(rec - Fm).max() <= eps * 4 * np.abs(Fm).max()
+    for Q in (Um, Vm):
+        assert np.abs(Q @ Q.transpose(0, 2, 1) - np.eye(d)).max() <= 4 * eps
+        assert np.abs(np.linalg.det(Q) - 1).max() <= 4 * eps
+    a = np.abs(S)
+    for i in range(d - 1):
+        assert (a[:, i] >= a[:, i + 1] - 4 * eps * a[:, 0]).all()
+        assert (S[:, i] >= 0).all()
+    Uo, So, Vo, _ = mt.oracle_svd(F, d, prec=prec)
+    So = So.astype(np.float64)
+    assert np.abs(S - So).max() <= 8 * eps * np.abs(So).max()
+    gaps = np.min(np.abs(np.diff(np.abs(So), axis=1)), axis=1) / np.abs(So[:, 0])
+    sep = (gaps > 0.05) & (np.abs(So[:, -1]) > 0.05 * np.abs(So[:, 0]))
+    assert sep.sum() > n // 4
+    for got, ref in ((U, Uo), (V, Vo)):
+        err = np.abs(got.astype(np.float64)[sep] - ref.astype(np.float64)[sep]).max()
+        print(f"dim {d} prec {prec}: max factor difference {err:.2e} over {int(sep.sum())} matrices", flush=True)
+        assert err <= (1e-10 if prec == 8 else 2e-4)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("dim", [2, 3])
 def test_stress_near_rotations(dim, prec):
     # F = Q (I + e S): strains on both sides of the small-strain series radius
     rng = np.random.default_rng(5)
