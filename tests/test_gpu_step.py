@@ -217,8 +217,10 @@ def test_svd_factors_match_the_oracle(dim, prec):
     descending, only the last singular value negative), reconstruction and
     orthonormality to the precision, and U, s, V equal to the oracle's (the
     reference's own algorithm, same precision input) where the singular values
-    are separated by more than 5 % (then the factors are unique; closer ones
-    leave U and V free inside the near-degenerate subspace)."""
+    are separated by more than 5 % (then the factors are unique up to the
+    convention's freedom of negating the same columns of U and V in pairs, or
+    both whole factors in 2D; closer singular values leave U and V free inside
+    the near-degenerate subspace)."""
     rng = np.random.default_rng(17 + dim)
     n = 4096
     d = dim
@@ -248,9 +250,17 @@ def test_svd_factors_match_the_oracle(dim, prec):
     gaps = np.min(np.abs(np.diff(np.abs(So), axis=1)), axis=1) / np.abs(So[:, 0])
     sep = (gaps > 0.05) & (np.abs(So[:, -1]) > 0.05 * np.abs(So[:, 0]))
     assert sep.sum() > n // 4
-    for got, ref in ((U, Uo), (V, Vo)):
-        err = np.abs(got.astype(np.float64)[sep] - ref.astype(np.float64)[sep]).max()
-        print(f"dim {d} prec {prec}: max factor difference {err:.2e} over {int(sep.sum())} matrices", flush=True)
+    # U diag(s) and V diag(s) with s = +-1 per column and prod s = 1 factor
+    # F the same way under the convention: align the columns' signs on U,
+    # then V must match with the same signs
+    Ug, Ur = Um[sep], Uo.astype(np.float64).reshape(n, d, d).transpose(0, 2, 1)[sep]
+    Vg, Vr = Vm[sep], Vo.astype(np.float64).reshape(n, d, d).transpose(0, 2, 1)[sep]
+    sg = np.sign(np.einsum("nij,nij->nj", Ug, Ur))
+    assert np.all(np.prod(sg, axis=1) == 1)
+    for got, ref in ((Ug * sg[:, None, :], Ur), (Vg * sg[:, None, :], Vr)):
+        err = np.abs(got - ref).max()
+        print(f"dim {d} prec {prec}: max factor difference {err:.2e} over {int(sep.sum())} matrices "
+              f"({int((sg < 0).any(axis=1).sum())} with flipped column pairs)", flush=True)
         assert err <= (1e-10 if prec == 8 else 2e-4)
 
 
