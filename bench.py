@@ -181,7 +181,7 @@ def make_context(P):
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 # kernel phase -> rocprof kernel-name prefix of its main kernel
-TRAFFIC_KERNELS = {"g2p2g": "void mpmx::g2p2g_kernel<float, 3, false>", "grid": "void mpmx::grid_kernel<float, 3, false>",
+TRAFFIC_KERNELS = {"g2p2g": "void mpmx::g2p2g_kernel<float, 3, false, ", "grid": "void mpmx::grid_kernel<float, 3, false>",
                    "side": "void mpmx::fused_side_kernel<float, 3, false>",
                    "glist": "void mpmx::glist_kernel<float, 3, true>"}
 
@@ -399,7 +399,8 @@ def batched(log, nscenes=256, steps=100, seed=BOX_SEED):
 
     def timed(graph):
         """steps from the initial state, after 5 untimed ones"""
-        ctx.set_launch_config(graph=graph, tile_workgroups=chosen["tile_workgroups"])
+        ctx.set_launch_config(graph=graph, tile_workgroups=chosen["tile_workgroups"],
+                              tile_threads=chosen["tile_threads"])
         load()
         ctx.step(dt, 5)
         ctx.synchronize()
@@ -419,7 +420,7 @@ def batched(log, nscenes=256, steps=100, seed=BOX_SEED):
             "workload": f"{nscenes} independent 3D scenes, N in {{32,48,64,96}}^3, one sphere r U[0.1,0.3], 8 ppc, "
                         "fixed-corotated, E log-U[1e3,1e5], nu U[0.2,0.4], gravity, dt 1e-4, FP32, one context; "
                         "autotuned launch configuration (candidate times in autotune_ms_per_step: graph off, then "
-                        "on, each with tile workgroups 0, 1x, 2x, 4x resident)"}
+                        "on; tile threads 128, then 256; tile workgroups 0, 1x, 2x, 4x resident)"}
 
 
 def adjoint_ratio(log, steps=200, checkpoint=20):

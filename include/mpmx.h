@@ -237,21 +237,26 @@ int mpmx_set_reorder_interval(mpmx_ctx* ctx, int interval);
    steps run as before); bins are rebuilt between groups of four when due.
    tile_workgroups: workgroups the tile kernels launch, each striding over the
    tile list (0 = one per tile up to 8192; the resident count of the fused
-   kernel or a small multiple of it makes them persistent).  Results do not
-   depend on the configuration beyond floating-point summation order.  For a
-   batch context all scenes run in these same launches. */
+   kernel or a small multiple of it makes them persistent).  tile_threads:
+   workgroup size of the fused tile kernel, 128 or 256 (0 = 128): the tile's
+   particles are taken that many at a time, and each cell's particles of a
+   batch are added in rounds of tile_threads / 64.  Results do not depend on
+   the configuration beyond floating-point summation order.  For a batch
+   context all scenes run in these same launches. */
 typedef struct mpmx_launch_config {
   int graph;
   int tile_workgroups;
+  int tile_threads;
 } mpmx_launch_config;
 int mpmx_set_launch_config(mpmx_ctx* ctx, const mpmx_launch_config* cfg);
 int mpmx_get_launch_config(mpmx_ctx* ctx, mpmx_launch_config* cfg);
 
 /* Autotuner of the launch configuration (no counterpart in the reference):
    times trial_steps (>= 8) fused steps of dt of the context's own particles
-   after 4 untimed ones, for each candidate in this order: graph off, then on,
-   each with tile_workgroups 0, then 1, 2 and 4 times the fused kernel's
-   resident workgroup count; applies the fastest by wall clock, returns it in
+   after 4 untimed ones, for each candidate in this order: graph off, then on;
+   within each, tile_threads 128, then 256; within each, tile_workgroups 0,
+   then 1, 2 and 4 times the fused kernel's resident workgroup count (16
+   candidates); applies the fastest by wall clock, returns it in
    *chosen, and writes each candidate's milliseconds per step to times_ms
    (up to max_times entries; *ntimes = number of candidates).  The particles,
    the simulation time and the timing switch are restored afterwards; the

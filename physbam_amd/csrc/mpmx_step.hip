@@ -402,7 +402,7 @@ template<typename T> struct Vec4Type;
 template<> struct Vec4Type<float> { typedef float type __attribute__((ext_vector_type(4))); };
 template<> struct Vec4Type<double> { typedef double type __attribute__((ext_vector_type(4))); };
 
-template<typename T, int D, bool FRICTION>
+template<typename T, int D, bool FRICTION, int NT = BLOCK>
 __device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>& g, const int* org,
                                           typename Vec4Type<T>::type* sv);
 template<typename T, int D, bool FRICTION, bool FUSED>
@@ -548,8 +548,8 @@ __device__ __forceinline__ void p2g_scatter_vals(const Geom<T, D>& g, const T* c
 // Phase B of the tile P2G: the particle in coefficient slot sl, whose base
 // node is region node base_li, is added into this wavefront's copy of the
 // tile's stencil region (p2g_scatter_vals on the slot's coefficients).
-template<typename T, int D, class C = P2GCoef<T, D>>
-__device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*s_c)[BLOCK], int sl, int base_li,
+template<typename T, int D, class C = P2GCoef<T, D>, int ROW = BLOCK>
+__device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*s_c)[ROW], int sl, int base_li,
                                                  typename Vec4Type<T>::type* reg) {
   T cf[C::N];
 #pragma unroll
@@ -1300,11 +1300,11 @@ __device__ __forceinline__ bool g2p_store(const Args<T, D>& a, const Geom<T, D>&
 // The tile's stencil region of the grid nodes (m, v) of a.grid and, with
 // FRICTION, of a.gridf (second half), staged in LDS as 4-vectors so that a
 // node is one 16-byte (FP32) LDS read.
-template<typename T, int D, bool FRICTION>
+template<typename T, int D, bool FRICTION, int NT>
 __device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>& g, const int* org,
                                           typename Vec4Type<T>::type* sv) {
   using T4 = typename Vec4Type<T>::type;
-  for (int k = threadIdx.x; k < Tile<D>::RN; k += BLOCK) {
+  for (int k = threadIdx.x; k < Tile<D>::RN; k += NT) {
     int off[D], nc[D];
     region_offsets<D>(k, off);
 #pragma unroll
@@ -1404,15 +1404,15 @@ __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
 // appended to the next mover list, and their P2G is left to
 // p2g_extra_kernel<FUSED>, which also adds the stress impulse of particles
 // whose stress needs the FP64 SVD (queued here with tau = 0).
-template<typename T, int D, bool FRICTION>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (FRICTION ? 2 : 3) : 1))) void g2p2g_kernel(
+template<typename T, int D, bool FRICTION, int FB = BLOCK>
+__global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (FRICTION ? 2 : 3) : 1))) void g2p2g_kernel(
     Args<T, D> a) {
-  constexpr int R = Tile<D>::R, RN = Tile<D>::RN, NW = BLOCK / 64, E = Tile<D>::E;
+  constexpr int R = Tile<D>::R, RN = Tile<D>::RN, NW = FB / 64, E = Tile<D>::E;
   using C = P2GCoef<T, D>;
   using T4 = typename Vec4Type<T>::type;
   __shared__ T4 s_reg[NW][RN];              // per-wave copies of the region: (m, p_x, p_y, p_z)
   __shared__ T4 s_v[FRICTION ? 2 : 1][RN];  // staged grid of the G2P half (and friction velocities)
-  __shared__ T s_c[C::N][BLOCK];            // coefficients of one round: slot (r % 4) * 64 + cell
+  __shared__ T s_c[C::N][FB];            // coefficients of one round: slot (r % 4) * 64 + cell
   __shared__ uint32_t s_wc[NW][64];         // per-wave cell counts, then offsets
   __shared__ uint32_t s_cn[64], s_max;
   const uint32_t ntile = a.counts[0];
@@ -1423,14 +1423,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
     const Geom<T, D> g = tile_geom(a, t);
     int org[D];
     tile_origin<T, D>(t, g, org);
-    g2p_stage<T, D, FRICTION>(a, g, org, &s_v[0][0]);
-    for (int k = threadIdx.x; k < NW * RN; k += BLOCK) {
+    g2p_stage<T, D, FRICTION, FB>(a, g, org, &s_v[0][0]);
+    for (int k = threadIdx.x; k < NW * RN; k += FB) {
       T4 z = {0, 0, 0, 0};
       s_reg[k / RN][k % RN] = z;
     }
     __syncthreads();
     const uint32_t tb = a.tile_begin[t], te = a.tile_end[t];
-    for (uint32_t c0 = tb; c0 < te; c0 += BLOCK) {
+    for (uint32_t c0 = tb; c0 < te; c0 += FB) {
       // ---- G2P of one particle per thread; P2G coefficients kept in registers
       uint32_t cell = 64;
       T cf[C::N];
@@ -1589,12 +1589,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
           int lcell[D];
           cell_offsets<D>((uint32_t)lane, lcell);
           const int lane_li = D == 3 ? (lcell[0] * R + lcell[1]) * R + lcell[D - 1] : lcell[0] * R + lcell[D - 1];
-          p2g_scatter_slot<T, D>(g, s_c, threadIdx.x, lane_li, s_reg[wv]);
+          p2g_scatter_slot<T, D, P2GCoef<T, D>, FB>(g, s_c, threadIdx.x, lane_li, s_reg[wv]);
         }
         __syncthreads();
       }
     }
-    for (int k = threadIdx.x; k < RN; k += BLOCK) {
+    for (int k = threadIdx.x; k < RN; k += FB) {
       T4 sum = s_reg[0][k];
 #pragma unroll
       for (int w2 = 1; w2 < NW; w2++) sum += s_reg[w2][k];
@@ -3967,15 +3967,33 @@ class Engine final : public EngineBase {
     if (cfg.graph != 0 && cfg.graph != 1) throw Error{MPMX_EINVAL, "graph must be 0 or 1"};
     if (cfg.tile_workgroups < 0 || cfg.tile_workgroups > (1 << 20))
       throw Error{MPMX_EINVAL, "tile_workgroups must be in [0, 2^20]"};
+    if (cfg.tile_threads != 0 && cfg.tile_threads != 128 && cfg.tile_threads != 256)
+      throw Error{MPMX_EINVAL, "tile_threads must be 0, 128 or 256"};
     use_graph_ = cfg.graph != 0;
     tile_wg_ = (uint32_t)cfg.tile_workgroups;
+    fused_threads_ = cfg.tile_threads ? cfg.tile_threads : 128;
     release_graph();
   }
   mpmx_launch_config launch_config() const override {
     mpmx_launch_config c;
     c.graph = use_graph_ ? 1 : 0;
     c.tile_workgroups = (int)tile_wg_;
+    c.tile_threads = fused_threads_;
     return c;
+  }
+  // the fused tile kernel at the configured workgroup size
+  void launch_fused_kernel(const Args<T, D>& a, bool fric) {
+    if (fused_threads_ == 256) {
+      if (fric)
+        g2p2g_kernel<T, D, true, 256><<<tile_blocks(), 256, 0, stream_>>>(a);
+      else
+        g2p2g_kernel<T, D, false, 256><<<tile_blocks(), 256, 0, stream_>>>(a);
+    } else {
+      if (fric)
+        g2p2g_kernel<T, D, true, 128><<<tile_blocks(), 128, 0, stream_>>>(a);
+      else
+        g2p2g_kernel<T, D, false, 128><<<tile_blocks(), 128, 0, stream_>>>(a);
+    }
   }
   // Times each candidate configuration on the context's own particles (saved
   // and restored), applies the fastest.
@@ -3987,13 +4005,17 @@ class Engine final : public EngineBase {
     if (taping_ || stage_next_ != 0) throw Error{MPMX_ESTATE, "autotuning needs no tape and no open staged step"};
     if (plastic_ || limits_on() || refl_on_ || stats_on_)
       throw Error{MPMX_ESTATE, "autotuning times the fused step: no plasticity, dt limits, reflection or grid stats"};
-    int cus = 0, per = 0;
+    int cus = 0, per128 = 0, per256 = 0;
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
-    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, g2p2g_kernel<T, D, false>, BLOCK, 0));
-    const uint32_t resident = (uint32_t)std::max(1, per) * (uint32_t)std::max(1, cus);
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per128, g2p2g_kernel<T, D, false, 128>, 128, 0));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per256, g2p2g_kernel<T, D, false, 256>, 256, 0));
     std::vector<mpmx_launch_config> cand;
     for (int gr = 0; gr < 2; gr++)
-      for (uint32_t w : {0u, resident, 2 * resident, 4 * resident}) cand.push_back(mpmx_launch_config{gr, (int)w});
+      for (int th : {128, 256}) {
+        const uint32_t resident = (uint32_t)std::max(1, th == 128 ? per128 : per256) * (uint32_t)std::max(1, cus);
+        for (uint32_t w : {0u, resident, 2 * resident, 4 * resident})
+          cand.push_back(mpmx_launch_config{gr, (int)w, th});
+      }
     const mpmx_launch_config before = launch_config();
     const bool timing = timer_.on;
     const double t_before = time_, last_dt_before = last_dt_;
@@ -5163,10 +5185,7 @@ class Engine final : public EngineBase {
 
     hipEvent_t e3 = timer_.begin(stream_);
     hipEvent_t ek = timer_.begin(stream_);  // the fused kernel alone
-    if (fric)
-      g2p2g_kernel<T, D, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-    else
-      g2p2g_kernel<T, D, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+    launch_fused_kernel(a, fric);
     timer_.end(PH_FUSED, ek, stream_);
     if (fric)
       fused_side_kernel<T, D, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a, lastcnt_.p);
@@ -5230,8 +5249,8 @@ class Engine final : public EngineBase {
     std::vector<unsigned char> sig(sizeof(a) + 64, 0);
     std::memcpy(sig.data(), &a, sizeof(a));
     unsigned char* q = sig.data() + sizeof(a);
-    const uint64_t extra[6] = {bin_stamp_, (uint64_t)fric, tile_blocks(), (uint64_t)(uintptr_t)gother_,
-                               (uint64_t)(uintptr_t)gt_other_, (uint64_t)(uintptr_t)gridf_.p};
+    const uint64_t extra[7] = {bin_stamp_, (uint64_t)fric, tile_blocks(), (uint64_t)(uintptr_t)gother_,
+                               (uint64_t)(uintptr_t)gt_other_, (uint64_t)(uintptr_t)gridf_.p, (uint64_t)fused_threads_};
     std::memcpy(q, extra, sizeof(extra));
     return sig;
   }
@@ -5362,6 +5381,7 @@ class Engine final : public EngineBase {
   // launch configuration: fused steps replayed from a captured graph, and the
   // tile kernels' workgroup count (0: one per tile up to 8192)
   bool use_graph_ = false;
+  int fused_threads_ = 128;  // workgroup size of the fused tile kernel (128 or 256)
   bool tuning_ = false;  // mpmx_autotune is timing candidates
   uint32_t tile_wg_ = 0;
   hipGraph_t graph_ = nullptr;

@@ -83,8 +83,8 @@ _lib = None
 
 
 class LaunchConfig(ctypes.Structure):
-    """mpmx_launch_config: graph (0/1), tile_workgroups (0 = one per tile up to 8192)."""
-    _fields_ = [("graph", ctypes.c_int), ("tile_workgroups", ctypes.c_int)]
+    """mpmx_launch_config: graph (0/1), tile_workgroups (0 = one per tile up to 8192), tile_threads (0/128/256)."""
+    _fields_ = [("graph", ctypes.c_int), ("tile_workgroups", ctypes.c_int), ("tile_threads", ctypes.c_int)]
 
 
 def lib():
@@ -383,14 +383,14 @@ class Context:
         _check(lib().mpmx_get_grid(self._h, _p(m), _p(v)))
         return m, v
 
-    def set_launch_config(self, graph=False, tile_workgroups=0):
-        c = LaunchConfig(1 if graph else 0, int(tile_workgroups))
+    def set_launch_config(self, graph=False, tile_workgroups=0, tile_threads=0):
+        c = LaunchConfig(1 if graph else 0, int(tile_workgroups), int(tile_threads))
         _check(lib().mpmx_set_launch_config(self._h, ctypes.byref(c)))
 
     def launch_config(self):
         c = LaunchConfig()
         _check(lib().mpmx_get_launch_config(self._h, ctypes.byref(c)))
-        return {"graph": bool(c.graph), "tile_workgroups": int(c.tile_workgroups)}
+        return {"graph": bool(c.graph), "tile_workgroups": int(c.tile_workgroups), "tile_threads": int(c.tile_threads)}
 
     def autotune(self, dt, trial_steps=16):
         """mpmx_autotune: returns (chosen config dict, list of (config dict, ms per step) per candidate)."""
@@ -399,7 +399,7 @@ class Context:
         nt = ctypes.c_int(0)
         _check(lib().mpmx_autotune(self._h, float(dt), int(trial_steps), ctypes.byref(c), _p(times), 32,
                                    ctypes.byref(nt)))
-        ch = {"graph": bool(c.graph), "tile_workgroups": int(c.tile_workgroups)}
+        ch = {"graph": bool(c.graph), "tile_workgroups": int(c.tile_workgroups), "tile_threads": int(c.tile_threads)}
         return ch, [float(t) for t in times[:nt.value]]
 
     def set_reorder_interval(self, k):

@@ -8,7 +8,8 @@ between replays: FP64 to 1e-12 and FP32 to 1e-5 relative (max|a - b| /
 max|b| for X, V, F; B by the velocity its difference carries into the next
 P2G; the grid sums' float atomics make the order of additions differ between
 any two runs).  Also with bins rebuilt every few
-steps (each rebuild re-captures the graph), with persistent tile workgroups,
+steps (each rebuild re-captures the graph), with persistent tile workgroups
+and both workgroup sizes of the fused kernel,
 with wall friction, and in a batch of scenes; the autotuner leaves the
 particles and the time as they were and picks one of its candidates.
 """
@@ -43,22 +44,22 @@ def check(a, b, tol, tag, dx=1.0 / 32):
     assert np.array_equal(a["valid"], b["valid"]), tag
 
 
-@pytest.mark.parametrize("prec,friction,tile_wg,rebin_every", [
-    (8, 0.0, 0, 0),
-    (4, 0.0, 0, 0),
-    (8, 0.3, 0, 0),
-    (8, 0.0, 0, 5),
-    (4, 0.0, 512, 3),
+@pytest.mark.parametrize("prec,friction,tile_wg,rebin_every,threads", [
+    (8, 0.0, 0, 0, 0),
+    (4, 0.0, 0, 0, 256),
+    (8, 0.3, 0, 0, 0),
+    (8, 0.0, 0, 5, 256),
+    (4, 0.0, 512, 3, 128),
 ])
-def test_graph_replays_match_eager_steps(prec, friction, tile_wg, rebin_every):
+def test_graph_replays_match_eager_steps(prec, friction, tile_wg, rebin_every, threads):
     sc = scene(prec, friction)
     tol = 1e-12 if prec == 8 else 1e-5
     eager, graph = gpu_context(sc), gpu_context(sc)
     try:
         for c in (eager, graph):
             c.set_rebin_policy(1.0 / 128, rebin_every)
-        graph.set_launch_config(graph=True, tile_workgroups=tile_wg)
-        assert graph.launch_config() == {"graph": True, "tile_workgroups": tile_wg}
+        graph.set_launch_config(graph=True, tile_workgroups=tile_wg, tile_threads=threads)
+        assert graph.launch_config() == {"graph": True, "tile_workgroups": tile_wg, "tile_threads": threads or 128}
         # the first four steps run one by one (the pipeline starts, the step
         # count reaches a multiple of four); every later call is whole groups
         for k, n in enumerate((4, 4, 8, 4, 8, 12)):
@@ -118,7 +119,7 @@ def test_autotune_restores_the_state_and_picks_a_candidate():
         before, t = ctx.get_particles(), ctx.time
         chosen, times = ctx.autotune(sc["dt"], 8)
         print("autotune:", chosen, [f"{x:.3f}" for x in times], flush=True)
-        assert len(times) == 8 and all(x > 0 for x in times)
+        assert len(times) == 16 and all(x > 0 for x in times)
         assert ctx.launch_config() == chosen
         assert chosen["tile_workgroups"] >= 0
         after = ctx.get_particles()
@@ -140,6 +141,8 @@ def test_launch_config_is_checked():
     try:
         with pytest.raises(mpmx.MpmxError):
             ctx.set_launch_config(graph=True, tile_workgroups=-1)
+        with pytest.raises(mpmx.MpmxError):
+            ctx.set_launch_config(tile_threads=192)
         with pytest.raises(mpmx.MpmxError) as e:
             ctx.autotune(sc["dt"], 4)  # fewer than 8 trial steps
         assert e.value.code == -1
