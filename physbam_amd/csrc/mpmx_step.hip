@@ -2499,7 +2499,8 @@ __global__ __launch_bounds__(BLOCK) void reorder_kernel(ParticleBufs<T, D> src, 
 // ---------------------------------------------------------- plasticity ----
 // Plasticity after G2P (MPM_PLASTICITY_MODEL::Update_Particles,
 // MPM_PLASTICITY_MODEL.cpp:34-48) for the particles simulated in the step
-// (valid at its start: `simulated`), in FP64: F = U S V^T, the model maps S
+// (valid at its start: `simulated`), in the particle precision (as the
+// reference templated on T): F = U S V^T, the model maps S
 // to Sc, and if it changed, F = U Sc V^T and Fp = V Sc^-1 U^T F Fp.
 //   clamp (MPM_PLASTICITY_CLAMP.cpp:31-45): Sc = clamp(S, 1 - theta_c,
 //     1 + theta_s); h = max(1e-8, exp(min(max_hardening, xi (1 - prod(S) /
@@ -2516,10 +2517,10 @@ template<typename T> struct PlasticDev {
 // MPM_DRUCKER_PRAGER::Update_Hardening (MPM_DRUCKER_PRAGER.cpp:123-131): the
 // friction angle phi = a0 + (a1 q - a4) exp(-a3 q) degrees at plastic
 // deformation q, rho_F = sqrt(2/3) 2 sin(phi) / (3 - sin(phi)).
-template<typename T> __device__ __forceinline__ double dp_rho_f(const PlasticDev<T>& pl, double q) {
-  const double phi = 3.14159265358979323846 / 180 * (pl.a0 + (pl.a1 * q - pl.a4) * exp(-pl.a3 * q));
-  const double sp = sin(phi);
-  return sqrt(2.0 / 3.0) * 2 * sp / (3 - sp);
+template<typename T, typename R = double> __device__ __forceinline__ R dp_rho_f(const PlasticDev<T>& pl, R q) {
+  const R phi = (R)(3.14159265358979323846 / 180) * ((R)pl.a0 + ((R)pl.a1 * q - (R)pl.a4) * exp(-(R)pl.a3 * q));
+  const R sp = sin(phi);
+  return sqrt((R)(2.0 / 3.0)) * 2 * sp / (3 - sp);
 }
 
 template<typename T, int D>
@@ -2527,67 +2528,68 @@ __global__ __launch_bounds__(BLOCK) void plasticity_kernel(ParticleBufs<T, D> b,
                                                            PlasticDev<T> pl) {
   for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < n; p += gridDim.x * BLOCK) {
     if (!simulated[p]) continue;
-    double Fe[D * D], U[D * D], S[D], V[D * D];
+    using R = T;  // the particle precision: FP32 contexts take the FP32 SVD
+    R Fe[D * D], U[D * D], S[D], V[D * D];
 #pragma unroll
-    for (int k = 0; k < D * D; k++) Fe[k] = (double)b.F[(size_t)k * n + p];
-    svd<double, D>(Fe, U, S, V);
-    double Sc[D];
-    double Fp[D * D];
+    for (int k = 0; k < D * D; k++) Fe[k] = (R)b.F[(size_t)k * n + p];
+    svd<R, D>(Fe, U, S, V);
+    R Sc[D];
+    R Fp[D * D];
 #pragma unroll
-    for (int k = 0; k < D * D; k++) Fp[k] = (double)b.Fp[(size_t)k * n + p];
+    for (int k = 0; k < D * D; k++) Fp[k] = (R)b.Fp[(size_t)k * n + p];
     if (pl.kind == 1) {
       bool same = true;
 #pragma unroll
       for (int a = 0; a < D; a++) {
-        Sc[a] = S[a] <= pl.lo ? pl.lo : (S[a] >= pl.hi ? pl.hi : S[a]);
+        Sc[a] = S[a] <= (R)pl.lo ? (R)pl.lo : (S[a] >= (R)pl.hi ? (R)pl.hi : S[a]);
         same = same && Sc[a] == S[a];
       }
       if (same) continue;
-      double ps = 1, pc = 1;
+      R ps = 1, pc = 1;
 #pragma unroll
       for (int a = 0; a < D; a++) {
         ps *= S[a];
         pc *= Sc[a];
       }
-      double detp;
+      R detp;
       if constexpr (D == 3)
         detp = Fp[0] * (Fp[4] * Fp[8] - Fp[7] * Fp[5]) + Fp[3] * (Fp[7] * Fp[2] - Fp[1] * Fp[8]) +
                Fp[6] * (Fp[1] * Fp[5] - Fp[4] * Fp[2]);
       else
         detp = Fp[0] * Fp[3] - Fp[1] * Fp[2];
-      const double det = ps / pc * detp;
-      double h = exp(fmin(pl.max_hardening, pl.xi * (1 - det)));
-      if (h < 1e-8) h = 1e-8;
-      b.mu[p] = (T)((double)b.mu0[p] * h);
-      b.la[p] = (T)((double)b.la0[p] * h);
+      const R det = ps / pc * detp;
+      R h = exp(fmin((R)pl.max_hardening, (R)pl.xi * (1 - det)));
+      if (h < (R)1e-8) h = (R)1e-8;
+      b.mu[p] = (T)((R)b.mu0[p] * h);
+      b.la[p] = (T)((R)b.la0[p] * h);
     } else {
       // MPM_DRUCKER_PRAGER::Compute (MPM_DRUCKER_PRAGER.cpp:64-94): return
       // mapping of the log strain onto the Drucker-Prager cone
-      const double mu = (double)b.mu[p], la = (double)b.la[p];
-      const double g = 1 / (D * la + 2 * mu), bb = 1 / (2 * mu);
-      const double rho = (double)b.rhoF[p], coh = (double)b.coh[p];
-      double vcp = (double)b.vc[p], pdef = (double)b.pdef[p];
-      const double beta = -coh * g / (rho * D);
-      double lf[D], sh[D], k = 0;
+      const R mu = (R)b.mu[p], la = (R)b.la[p];
+      const R g = 1 / (D * la + 2 * mu), bb = 1 / (2 * mu);
+      const R rho = (R)b.rhoF[p], coh = (R)b.coh[p];
+      R vcp = (R)b.vc[p], pdef = (R)b.pdef[p];
+      const R beta = -coh * g / (rho * D);
+      R lf[D], sh[D], k = 0;
 #pragma unroll
       for (int a = 0; a < D; a++) {
         lf[a] = log(fabs(S[a]));
         sh[a] = lf[a] + beta + vcp / D;
         k += sh[a];
       }
-      double q2 = 0;
+      R q2 = 0;
 #pragma unroll
       for (int a = 0; a < D; a++) {
         sh[a] -= k / D;
         q2 += sh[a] * sh[a];
       }
-      const double q = sqrt(q2), r = rho * k / g, dg = q + bb * r;
+      const R q = sqrt(q2), r = rho * k / g, dg = q + bb * r;
       if (dg <= 0) continue;
-      double inc;
+      R inc;
       if (q == 0 || k > 0) {  // tension: to the tip of the cone
 #pragma unroll
-        for (int a = 0; a < D; a++) Sc[a] = coh != 0 ? exp(-beta) : 1.0;
-        double ls = 0, l2 = 0;
+        for (int a = 0; a < D; a++) Sc[a] = coh != 0 ? exp(-beta) : (R)1;
+        R ls = 0, l2 = 0;
 #pragma unroll
         for (int a = 0; a < D; a++) {
           ls += lf[a];
@@ -2596,7 +2598,7 @@ __global__ __launch_bounds__(BLOCK) void plasticity_kernel(ParticleBufs<T, D> b,
         if (pl.use_vc) vcp += ls + beta * D;
         inc = sqrt(l2);
       } else {
-        const double pp = r * bb / q;
+        const R pp = r * bb / q;
 #pragma unroll
         for (int a = 0; a < D; a++) Sc[a] = exp(k / D - pp * sh[a] - beta);
         vcp = 0;
@@ -2605,19 +2607,19 @@ __global__ __launch_bounds__(BLOCK) void plasticity_kernel(ParticleBufs<T, D> b,
       pdef += inc;
       b.vc[p] = (T)vcp;
       b.pdef[p] = (T)pdef;
-      b.rhoF[p] = (T)dp_rho_f<T>(pl, pdef);
+      b.rhoF[p] = (T)dp_rho_f<T, R>(pl, pdef);
     }
     // F = U Sc V^T;  M = V Sc^-1 U^T F_e;  Fp = M Fp  (column-major)
-    double M[D * D];
+    R M[D * D];
 #pragma unroll
     for (int i = 0; i < D; i++)
 #pragma unroll
       for (int j = 0; j < D; j++) {
-        double f = 0, m = 0;
+        R f = 0, m = 0;
 #pragma unroll
         for (int k = 0; k < D; k++) {
           f += U[i + D * k] * Sc[k] * V[j + D * k];
-          double ut = 0;  // (U^T Fe)_kj
+          R ut = 0;  // (U^T Fe)_kj
 #pragma unroll
           for (int l = 0; l < D; l++) ut += U[l + D * k] * Fe[l + D * j];
           m += V[i + D * k] / Sc[k] * ut;
@@ -2629,7 +2631,7 @@ __global__ __launch_bounds__(BLOCK) void plasticity_kernel(ParticleBufs<T, D> b,
     for (int i = 0; i < D; i++)
 #pragma unroll
       for (int j = 0; j < D; j++) {
-        double f = 0;
+        R f = 0;
 #pragma unroll
         for (int k = 0; k < D; k++) f += M[i + D * k] * Fp[k + D * j];
         b.Fp[(size_t)(i + D * j) * n + p] = (T)f;

@@ -6,7 +6,11 @@ the oracle.
 FP64: X, V, F, B, Fp, mu, lambda (and plastic_def, rho_F, cohesion, vc for
 sand) to 1e-9 relative after one step and 1e-8 after ten.  FP32: against the
 FP64 oracle of the same float inputs, to 1e-5 and B to the bound the golden
-step test uses (B is measured against max(|B|, |V| h)).  Both return mappings
+step test uses (B is measured against max(|B|, |V| h)); the plastic update
+runs in FP32 there (the particle precision, as the reference templated on
+float), and the clamp model's hardening h = exp(xi (1 - det Fe det Fp))
+carries the FP32 error of the determinants into mu and lambda times xi, so
+those two get xi times the bound.  Both return mappings
 are continuous in F (the Drucker-Prager tension branch meets the cone branch
 at zero trace), so the plastic update keeps these tolerances.  plastic_def
 and vc are sums of log strains and get the same bounds as absolute errors
@@ -88,6 +92,8 @@ def test_plasticity_matches_reference(name):
                     scale = np.abs(r).max()
                 err = float(np.abs(got[k] - r).max() / scale)
                 tol = (1e-5 if k != "B" else 3e-4 * max(1, n / 2)) if fp32 else (1e-9 if n <= 1 else 1e-8)
+                if fp32 and not sand and k in ("mu", "lambda"):
+                    tol *= max(1.0, sc["plasticity"]["hardening_factor"])
                 print(f"  {k} {err:.2e} (tol {tol:.0e})", flush=True)
                 assert err <= tol, (name, n, k, err)
 
