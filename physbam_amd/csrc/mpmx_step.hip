@@ -1547,6 +1547,20 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
           }
         }
       }
+      // ---- a wavefront whose particles sit each in the cell of its lane's
+      // label (the stratified order of an undisturbed tile: one particle per
+      // cell, in label order) adds them from registers into its region copy,
+      // as the rounds below would with the same lanes
+      if (__all(cell == 64 || cell == (uint32_t)lane)) {
+        if (cell < 64) {
+          int lcell[D];
+          cell_offsets<D>((uint32_t)lane, lcell);
+          const int lane_li = D == 3 ? (lcell[0] * R + lcell[1]) * R + lcell[D - 1] : lcell[0] * R + lcell[D - 1];
+          p2g_scatter_vals<T, D, C>(g, cf, lane_li, s_reg[wv]);
+        }
+        cell = 64;
+      }
+      if (!__syncthreads_or(cell < 64)) continue;  // every wavefront done
       // ---- stable ranks of the new base cells
       s_wc[wv][lane] = 0;  // this wave's row only: the wave's own writes below come after it
       const bool act = cell < 64;
@@ -1727,12 +1741,15 @@ __global__ __launch_bounds__(BLOCK) void fused_side_kernel(Args<T, D> a, uint32_
     side_scatter<T, D>(a, a.grid_next, s_c, s_b, s_sc, min(per, ntot - c0));
     __syncthreads();
   }
-  // the last block to finish: bookkeeping of fused_counts_kernel mode 1
-  __threadfence();
-  if (threadIdx.x == 0) s_ticket = atomicAdd(&a.counts[12], 1u);
+  // the last block to finish: bookkeeping of fused_counts_kernel mode 1.  One
+  // acquire-release ticket per block, taken by one thread after the block's
+  // barrier (a device-scope fence in every wave writes the L2 back once per
+  // wave); the counts the last block reads are device-scope atomics.
+  __syncthreads();
+  if (threadIdx.x == 0)
+    s_ticket = __hip_atomic_fetch_add(&a.counts[12], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   if (s_ticket == gridDim.x - 1 && threadIdx.x == 0) {
-    __threadfence();
     uint32_t* counts = a.counts;
     const uint32_t c4 = atomicAdd(&counts[4], 0u), c5 = atomicAdd(&counts[5], 0u);
     const uint32_t c6 = atomicAdd(&counts[6], 0u), c7 = atomicAdd(&counts[7], 0u);
@@ -2499,14 +2516,17 @@ __global__ __launch_bounds__(BLOCK) void reorder_kernel(ParticleBufs<T, D> src, 
 // ---------------------------------------------------------- plasticity ----
 // Plasticity after G2P (MPM_PLASTICITY_MODEL::Update_Particles,
 // MPM_PLASTICITY_MODEL.cpp:34-48) for the particles simulated in the step
-// (valid at its start: `simulated`), in the particle precision (as the
-// reference templated on T): F = U S V^T, the model maps S
+// (valid at its start: `simulated`), computed in R: F = U S V^T, the model maps S
 // to Sc, and if it changed, F = U Sc V^T and Fp = V Sc^-1 U^T F Fp.
 //   clamp (MPM_PLASTICITY_CLAMP.cpp:31-45): Sc = clamp(S, 1 - theta_c,
 //     1 + theta_s); h = max(1e-8, exp(min(max_hardening, xi (1 - prod(S) /
 //     prod(Sc) det Fp)))), mu = mu0 h, lambda = lambda0 h;
 //   Drucker-Prager (MPM_DRUCKER_PRAGER.cpp:64-94): return mapping of ln S
 //     onto the cone, with hardening of the friction angle.
+// R is the particle precision for the clamp, which is continuous in F; the
+// Drucker-Prager mapping runs in FP64: with a volume correction vc != 0 it
+// jumps at the cone's edge (dg = 0: unprojected S against S exp(vc / d)), so
+// FP32 rounding of dg would move particles across the jump.
 template<typename T> struct PlasticDev {
   int kind;                           // 1 clamp, 2 Drucker-Prager
   double lo, hi, max_hardening, xi;   // clamp
@@ -2523,12 +2543,11 @@ template<typename T, typename R = double> __device__ __forceinline__ R dp_rho_f(
   return sqrt((R)(2.0 / 3.0)) * 2 * sp / (3 - sp);
 }
 
-template<typename T, int D>
+template<typename T, int D, typename R>
 __global__ __launch_bounds__(BLOCK) void plasticity_kernel(ParticleBufs<T, D> b, uint32_t n, const uint8_t* simulated,
                                                            PlasticDev<T> pl) {
   for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < n; p += gridDim.x * BLOCK) {
     if (!simulated[p]) continue;
-    using R = T;  // the particle precision: FP32 contexts take the FP32 SVD
     R Fe[D * D], U[D * D], S[D], V[D * D];
 #pragma unroll
     for (int k = 0; k < D * D; k++) Fe[k] = (R)b.F[(size_t)k * n + p];
@@ -4926,8 +4945,11 @@ class Engine final : public EngineBase {
         }
         if (plastic_) {
           hipEvent_t e5 = timer_.begin(stream_);
-          plasticity_kernel<T, D><<<std::min<uint32_t>(cdiv(n_, BLOCK), 4096), BLOCK, 0, stream_>>>(cur_, n_,
-                                                                                                  simmask_.p, pl_);
+          const uint32_t pb = std::min<uint32_t>(cdiv(n_, BLOCK), 4096);
+          if (pl_.kind == 1)
+            plasticity_kernel<T, D, T><<<pb, BLOCK, 0, stream_>>>(cur_, n_, simmask_.p, pl_);
+          else
+            plasticity_kernel<T, D, double><<<pb, BLOCK, 0, stream_>>>(cur_, n_, simmask_.p, pl_);
           timer_.end(PH_G2P, e5, stream_);
         }
         step_count_++;

@@ -6,13 +6,13 @@ the oracle.
 FP64: X, V, F, B, Fp, mu, lambda (and plastic_def, rho_F, cohesion, vc for
 sand) to 1e-9 relative after one step and 1e-8 after ten.  FP32: against the
 FP64 oracle of the same float inputs, to 1e-5 and B to the bound the golden
-step test uses (B is measured against max(|B|, |V| h)); the plastic update
-runs in FP32 there (the particle precision, as the reference templated on
-float), and the clamp model's hardening h = exp(xi (1 - det Fe det Fp))
+step test uses (B is measured against max(|B|, |V| h)).  The clamp runs in
+FP32 there (the particle precision, as the reference templated on float); it
+is continuous in F, and its hardening h = exp(xi (1 - det Fe det Fp))
 carries the FP32 error of the determinants into mu and lambda times xi, so
-those two get xi times the bound.  Both return mappings
-are continuous in F (the Drucker-Prager tension branch meets the cone branch
-at zero trace), so the plastic update keeps these tolerances.  plastic_def
+those two get xi times the bound.  The Drucker-Prager mapping jumps where
+dg crosses 0 with a volume correction vc != 0 (MPM_DRUCKER_PRAGER.cpp:64-94)
+and runs in FP64, so that no particle crosses a jump on FP32 rounding.  plastic_def
 and vc are sums of log strains and get the same bounds as absolute errors
 (each FP32 step adds the log of an F rounded to FP32).
 """
