@@ -445,7 +445,9 @@ int mpmx_backward(mpmx_ctx* ctx, const void* dX, const void* dV, const void* dF,
    "g2p2g_kernel" (the fused kernel of that phase alone); "graph" counts the
    graph replays (groups of four fused steps) whether timing is on or not.
    mpmx_phase_time returns the accumulated milliseconds and launch count since
-   the last reset. */
+   the last reset.  enable: 0 off, 1 every phase, 2 only "g2p2g_kernel" (two
+   events per fused step: the other phases' events cost stream time of their
+   own); other values are MPMX_EINVAL. */
 int mpmx_enable_timing(mpmx_ctx* ctx, int enable);
 int mpmx_reset_timing(mpmx_ctx* ctx);
 int mpmx_phase_time(mpmx_ctx* ctx, const char* phase, double* total_ms, int64_t* count);

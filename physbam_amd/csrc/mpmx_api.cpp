@@ -418,7 +418,8 @@ int mpmx_backward(mpmx_ctx* ctx, const void* dX, const void* dV, const void* dF,
 
 int mpmx_enable_timing(mpmx_ctx* ctx, int enable) {
   if (!ctx) return bad("null context");
-  return guard([&] { ctx->engine->enable_timing(enable != 0); });
+  if (enable < 0 || enable > 2) return MPMX_EINVAL;
+  return guard([&] { ctx->engine->enable_timing(enable); });
 }
 
 int mpmx_reset_timing(mpmx_ctx* ctx) {

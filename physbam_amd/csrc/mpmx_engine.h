@@ -66,7 +66,7 @@ struct EngineBase {
   virtual void tape_begin(int checkpoint_every) = 0;
   virtual void backward(const void* dX, const void* dV, const void* dF, const void* dB, void* gX, void* gV, void* gF,
                         void* gB, double* gmu, double* glambda) = 0;
-  virtual void enable_timing(bool on) = 0;
+  virtual void enable_timing(int mode) = 0;  // 0 off, 1 every phase, 2 the fused tile kernel only
   virtual void reset_timing() = 0;
   virtual bool phase_time(const char* phase, double* total_ms, int64_t* count) = 0;
   virtual void stress(int64_t n, const void* F, const void* volume, const void* mu, const void* lambda,

@@ -3466,7 +3466,8 @@ struct PhaseTimer {
   static constexpr int NPH = 11;
   double total[NPH] = {};
   int64_t count[NPH] = {};
-  bool on = false;
+  bool on = false;         // every phase
+  bool kernel_on = false;  // the fused tile kernel (begin_kernel), also when `on` is off
   hipEvent_t get() {
     if (pool.empty()) {
       hipEvent_t e;
@@ -3483,8 +3484,14 @@ struct PhaseTimer {
     HIPCHECK(hipEventRecord(e, s));
     return e;
   }
+  hipEvent_t begin_kernel(hipStream_t s) {
+    if (!kernel_on) return nullptr;
+    hipEvent_t e = get();
+    HIPCHECK(hipEventRecord(e, s));
+    return e;
+  }
   void end(int phase, hipEvent_t a, hipStream_t s) {
-    if (!on || !a) return;
+    if (!a) return;
     hipEvent_t b = get();
     HIPCHECK(hipEventRecord(b, s));
     pending.push_back({phase, a, b});
@@ -3781,7 +3788,7 @@ class Engine final : public EngineBase {
     // grid statistics read each step's P2G grid, which the fused kernel never
     // holds on its own
     const bool fuse = n_ > 0 && !plastic_ && !limits_on() && !refl_on_ && !stats_on_;
-    if (fuse && use_graph_ && !timer_.on && !taping_) {
+    if (fuse && use_graph_ && !timer_.on && !timer_.kernel_on && !taping_) {
       step_graph((T)dt, nsteps);
     } else {
       for (int s = 0; s < nsteps; s++) {
@@ -4038,7 +4045,7 @@ class Engine final : public EngineBase {
           cand.push_back(mpmx_launch_config{gr, (int)w, th});
       }
     const mpmx_launch_config before = launch_config();
-    const bool timing = timer_.on;
+    const bool timing = timer_.on, ktiming = timer_.kernel_on;
     const double t_before = time_, last_dt_before = last_dt_;
     // storage in bin order first (as the steps keep it), then saved: the trials
     // rebuild the bins but do not reorder, so the saved state stays in slot
@@ -4053,6 +4060,7 @@ class Engine final : public EngineBase {
     tuning_ = true;
     try {
       timer_.on = false;
+      timer_.kernel_on = false;
       for (size_t c = 0; c < cand.size(); c++) {
         set_launch_config(cand[c]);
         load_state(saved);
@@ -4065,6 +4073,7 @@ class Engine final : public EngineBase {
     } catch (...) {
       set_launch_config(before);
       timer_.on = timing;
+      timer_.kernel_on = ktiming;
       tuning_ = false;
       throw;
     }
@@ -4077,6 +4086,7 @@ class Engine final : public EngineBase {
     time_ = t_before;
     last_dt_ = last_dt_before;
     timer_.on = timing;
+    timer_.kernel_on = ktiming;
     tuning_ = false;
     if (chosen) *chosen = cand[best];
     for (int i = 0; i < max_times && i < (int)cand.size(); i++) times[i] = ms[i];
@@ -4192,9 +4202,10 @@ class Engine final : public EngineBase {
     if (movers) *movers = has_grid_ ? (int64_t)(fused_last_ ? c[10] : c[4]) : 0;
     if (slow) *slow = has_grid_ ? (int64_t)(fused_last_ ? c[11] : c[3]) : 0;
   }
-  void enable_timing(bool on) override {
+  void enable_timing(int mode) override {
     synchronize();
-    timer_.on = on;
+    timer_.on = mode == 1;
+    timer_.kernel_on = mode != 0;
   }
   void reset_timing() override {
     synchronize();
@@ -5208,7 +5219,7 @@ class Engine final : public EngineBase {
     timer_.end(PH_GRID, e2, stream_);
 
     hipEvent_t e3 = timer_.begin(stream_);
-    hipEvent_t ek = timer_.begin(stream_);  // the fused kernel alone
+    hipEvent_t ek = timer_.begin_kernel(stream_);  // the fused kernel alone
     launch_fused_kernel(a, fric);
     timer_.end(PH_FUSED, ek, stream_);
     if (fric)

@@ -512,7 +512,8 @@ class Context:
         return dict(X=outs[0], V=outs[1], F=outs[2], B=outs[3], mu=gmu, **{"lambda": gla})
 
     def enable_timing(self, on=True):
-        _check(lib().mpmx_enable_timing(self._h, 1 if on else 0))
+        """on: True/1 every phase, 2 the fused tile kernel only, False/0 off."""
+        _check(lib().mpmx_enable_timing(self._h, int(on)))
 
     def reset_timing(self):
         _check(lib().mpmx_reset_timing(self._h))

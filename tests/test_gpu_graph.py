@@ -152,3 +152,30 @@ def test_launch_config_is_checked():
         assert e.value.code == -3
     finally:
         ctx.close()
+
+
+def test_kernel_only_timing():
+    """Timing mode 2 records the fused tile kernel alone (the phases around it
+    stay untimed) and keeps the steps off the graph, whose replays record no
+    events; other modes are rejected."""
+    sc = scene(4, N=16)
+    ctx = gpu_context(sc)
+    try:
+        ctx.set_launch_config(graph=True)
+        ctx.enable_timing(2)
+        ctx.reset_timing()
+        ctx.step(sc["dt"], 6)
+        ms, cnt = ctx.phase_time("g2p2g_kernel")
+        assert cnt == 6 and ms > 0
+        for ph in ("grid", "bin", "g2p2g", "step"):
+            assert ctx.phase_time(ph)[1] == 0, ph
+        assert ctx.phase_time("graph")[1] == 0
+        ctx.enable_timing(False)
+        ctx.reset_timing()
+        ctx.step(sc["dt"], 8)
+        assert ctx.phase_time("g2p2g_kernel")[1] == 0
+        assert ctx.phase_time("graph")[1] >= 1
+        with pytest.raises(mpmx.MpmxError):
+            ctx.enable_timing(3)
+    finally:
+        ctx.close()
