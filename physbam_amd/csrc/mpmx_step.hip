@@ -1297,9 +1297,11 @@ __device__ __forceinline__ bool g2p_store(const Args<T, D>& a, const Geom<T, D>&
   return true;
 }
 
-// The tile's stencil region of the grid nodes (m, v) of a.grid and, with
-// FRICTION, of a.gridf (second half), staged in LDS as 4-vectors so that a
-// node is one 16-byte (FP32) LDS read.
+// The tile's stencil region of the grid nodes of a.grid and, with FRICTION,
+// of a.gridf (second half), staged in LDS as 4-vectors (v_x, v_y, v_z, m):
+// the velocity is the first 12 bytes of a 16-byte-aligned node, so a gather
+// reads it with one 12-byte LDS load (16-byte node stride across the lanes;
+// single-component loads at that stride would hit every bank four times).
 template<typename T, int D, bool FRICTION, int NT>
 __device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>& g, const int* org,
                                           typename Vec4Type<T>::type* sv) {
@@ -1311,10 +1313,10 @@ __device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>&
     for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
     const uint32_t slot = node_slot<T, D>(nc, g);
     const Node4<T> nd = a.grid[slot];
-    sv[k] = T4{nd.m, nd.x, nd.y, nd.z};
+    sv[k] = T4{nd.x, nd.y, nd.z, nd.m};
     if constexpr (FRICTION) {
       const Node4<T> nf = a.gridf[slot];
-      sv[Tile<D>::RN + k] = T4{nf.m, nf.x, nf.y, nf.z};
+      sv[Tile<D>::RN + k] = T4{nf.x, nf.y, nf.z, nf.m};
     }
   }
 }
@@ -1345,7 +1347,7 @@ __device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, c
     }
     if (!in) continue;  // mover
     const int li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
-    auto comp = [](const typename Vec4Type<T>::type& q, int c) -> T { return c == 0 ? q.y : (c == 1 ? q.z : q.w); };
+    auto comp = [](const typename Vec4Type<T>::type& q, int c) -> T { return c == 0 ? q.x : (c == 1 ? q.y : q.z); };
     auto nv = [&](int ox, int oy, int oz, int c) -> T {
       return comp(sv[li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)], c);
     };
@@ -1453,7 +1455,7 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
           if (in) {  // otherwise a mover of the current list
             const int li0 = D == 3 ? (lb[0] * R + lb[1]) * R + lb[D - 1] : lb[0] * R + lb[D - 1];
             const T4* sv = &s_v[0][0];
-            auto comp = [](const T4& q, int c) -> T { return c == 0 ? q.y : (c == 1 ? q.z : q.w); };
+            auto comp = [](const T4& q, int c) -> T { return c == 0 ? q.x : (c == 1 ? q.y : q.z); };
             auto nv = [&](int ox, int oy, int oz, int c) -> T {
               return comp(sv[li0 + (D == 3 ? (ox * R + oy) * R + oz : ox * R + oy)], c);
             };
