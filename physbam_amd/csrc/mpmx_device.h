@@ -550,6 +550,7 @@ __device__ __forceinline__ void kirchhoff(int model, T mu, T la, T thr, T root, 
 //                          (corot_series; truncation below 2e-8 relative);
 //     otherwise:           E = U diag(e) U^T by Jacobi; s_i = sqrt(1 + e_i);
 //                          tau = vol U diag(2 mu s_i (s_i - 1) + lambda (J-1) J) U^T
+//                          (FP32 3D: corot_closed_form, no eigenvectors)
 // Inverted or nearly degenerate F (J <= 0 or s_min < 1e-2 s_max) and the
 // neo-Hookean clamp branch (J < t) are declined (return false) and go through
 // the FP64 SVD of kirchhoff_typed<double>().
@@ -662,8 +663,65 @@ __device__ __forceinline__ void corot_series(const T (&E)[D][D], T (&f)[D][D]) {
     for (int j = 0; j < D; j++) f[i][j] = P[i][j];
 }
 
-// Returns false (tau untouched) when F needs the FP64 SVD path.
-template<typename T, int D>
+// Fixed-corotated tau in 3D past the series radius, without eigenvectors.
+// With b = F F^T = I + E and V = sqrt(b), the Cayley-Hamilton theorem gives
+// (Hoger and Carlson 1984, "Determination of the stretch and rotation in the
+// polar decomposition of the deformation gradient")
+//   V = (-b^2 + (i1^2 - i2) b + i1 i3 I) / (i1 i2 - i3),  i_k the invariants of V.
+// Written in the deviations d_k = s_k - 1 of the stretches (s_k^2 = 1 + e_k,
+// e_k the eigenvalues of E) through their elementary symmetric functions
+// q1, q2, q3, the corotated term is free of cancellation near b = I:
+//   b - V = (c0 I + c1 E + E^2) / den,   c0 = -q3 (4 + q1),
+//   c1 = 4 + 4 q1 + 3 q2 + q1^2 + q1 q2 - q3,
+//   den = 8 + 8 q1 + 2 q2 + 2 q1^2 + q1 q2 - q3.
+// The e_k come from the trigonometric solution of the characteristic cubic
+// of E (O. K. Smith 1961, "Eigenvalues of a symmetric 3x3 matrix"); only
+// symmetric functions of the d_k enter, so the accuracy lost in splitting a
+// nearly double eigenvalue cancels.  Returns false for nearly singular F
+// (s_min < 1e-2 s_max), as the eigen branch does.
+template<typename T>
+__device__ __forceinline__ bool corot_closed_form(const T (&E)[3][3], T mu, T pj, T vol, T* tau) {
+  const T mE = (E[0][0] + E[1][1] + E[2][2]) * (T)(1.0 / 3);
+  const T b00 = E[0][0] - mE, b11 = E[1][1] - mE, b22 = E[2][2] - mE;
+  const T b01 = E[0][1], b02 = E[0][2], b12 = E[1][2];
+  const T p2 = (b00 * b00 + b11 * b11 + b22 * b22 + (T)2 * (b01 * b01 + b02 * b02 + b12 * b12)) * (T)(1.0 / 6);
+  T e1 = mE, e2 = mE, e3 = mE;  // e1 >= e2 >= e3
+  if (p2 > (T)1e-30) {
+    const T p = sqrt(p2);
+    const T det = b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) + b02 * (b01 * b12 - b11 * b02);
+    T r = det / ((T)2 * p * p2);
+    r = r < (T)-1 ? (T)-1 : (r > (T)1 ? (T)1 : r);
+    const T phi = acos(r) * (T)(1.0 / 3);
+    const T c = cos(phi), sn = sin(phi);
+    e1 = mE + (T)2 * p * c;
+    e3 = mE - p * (c + (T)1.7320508075688772 * sn);
+    e2 = (T)3 * mE - e1 - e3;
+  }
+  if ((T)1 + e3 < (T)1e-4 * ((T)1 + e1)) return false;
+  const T d1 = e1 / ((T)1 + sqrt((T)1 + e1)), d2 = e2 / ((T)1 + sqrt((T)1 + e2)), d3 = e3 / ((T)1 + sqrt((T)1 + e3));
+  const T q1 = d1 + d2 + d3, q2 = d1 * d2 + d2 * d3 + d3 * d1, q3 = d1 * d2 * d3;
+  const T c0 = -q3 * ((T)4 + q1);
+  const T c1 = (T)4 + (T)4 * q1 + (T)3 * q2 + q1 * q1 + q1 * q2 - q3;
+  const T inv = (T)1 / ((T)8 + (T)8 * q1 + (T)2 * q2 + (T)2 * q1 * q1 + q1 * q2 - q3);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = i; j < 3; j++) {
+      T e2ij = 0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) e2ij += E[i][k] * E[k][j];
+      const T bv = ((i == j ? c0 : (T)0) + c1 * E[i][j] + e2ij) * inv;
+      const T t = vol * ((T)2 * mu * bv + (i == j ? pj : (T)0));
+      tau[i + 3 * j] = t;
+      tau[j + 3 * i] = t;
+    }
+  return true;
+}
+
+// Returns false (tau untouched) when F needs the FP64 SVD path.  CLOSED:
+// FP32 3D fixed corotated past the series radius by corot_closed_form (else
+// the Jacobi eigen branch, which needs fewer registers at once).
+template<typename T, int D, bool CLOSED = true>
 __device__ __forceinline__ bool kirchhoff_fast(int model, T mu, T la, T thr, T vol, const T* F, T* tau) {
   T G[D][D];
 #pragma unroll
@@ -741,6 +799,10 @@ __device__ __forceinline__ bool kirchhoff_fast(int model, T mu, T la, T thr, T v
 #pragma unroll
         for (int j = 0; j < D; j++) tau[i + D * j] = vol * (m2 * f[i][j] + (i == j ? pj : (T)0));
       return true;
+    }
+    if constexpr (CLOSED && D == 3 && sizeof(T) == 4) {
+      // FP32 fixed corotated: the closed form in place of the Jacobi sweeps
+      if (!hencky) return corot_closed_form<T>(E, mu, la * Jm1 * J, vol, tau);
     }
     T U[D][D];
     sym_eigen<T, D>(E, U);

@@ -697,7 +697,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
             s_c[C::XE + c][a_slot] = x[c] - cen;
           }
           T tau[D * D];
-          if (!kirchhoff_fast<T, D>(a.model, a.mu[p], a.la[p], a.thr, a.vol[p], Fm, tau)) {
+          // the unfused kernel runs at 128 registers: the Jacobi branch, which fits
+          if (!kirchhoff_fast<T, D, false>(a.model, a.mu[p], a.la[p], a.thr, a.vol[p], Fm, tau)) {
 #pragma unroll
             for (int q = 0; q < D * D; q++) tau[q] = 0;
             a.slow_list[atomicAdd(&a.counts[3], 1u)] = p;  // the compiler aggregates this per wavefront
