@@ -131,7 +131,7 @@ def test_batched_workload_at_full_shape(batched_workload, graph):
         errs = {k: float(np.abs(part[k] - orc[k]).max() / max(np.abs(orc[k]).max(), 1e-30))
                 for k in ("X", "V", "F")}
         # the spheres start at rest and fall freely, so their exact B is ~0:
-        # B is judged by the velocity its error carries into the next P2G
+        # B is measured by the velocity its error carries into the next P2G
         errs["B"] = mt.b_velocity_error(part["B"], orc["B"], orc["V"], 1.0 / sizes[s])
         print(f"scene {s} (N {sizes[s]}, {counts[s]} particles):", {k: f"{v:.1e}" for k, v in errs.items()},
               flush=True)

@@ -60,7 +60,7 @@ def test_three_fused_steps_match_fp64_oracle(sphere):
     print("oracle: 3 FP64 steps of the full sphere", flush=True)
     ref = mt.oracle_run(sc, 3, prec=8, want_grid=False)
     errs = {k: rel(got[k], ref[k]) for k in ("X", "V", "F")}
-    # the sphere moves uniformly, so its exact B is ~0: B is judged by the
+    # the sphere moves uniformly, so its exact B is ~0: B is measured by the
     # velocity its error carries into the next P2G
     errs["B"] = mt.b_velocity_error(got["B"], ref["B"], ref["V"], 1.0 / bench.SPHERE_CELLS)
     print("FP32 GPU vs FP64 oracle after 3 steps:", {k: f"{v:.2e}" for k, v in errs.items()},
