@@ -1688,6 +1688,9 @@ __global__ __launch_bounds__(BLOCK) void fused_side_kernel(Args<T, D> a, uint32_
   __shared__ uint32_t s_ticket;
   const uint32_t n4 = a.counts[4], n6 = a.counts[6], n5 = a.counts[5];
   const uint32_t nc = n4 + n6, nm = nc + n5, ns = a.counts[3], ntot = nm + ns;
+  // each workgroup takes consecutive list entries (the entries one wavefront
+  // of the tile kernel appended come from one tile); this measured a little
+  // faster than spreading the entries over the workgroups
   const uint32_t per = min((uint32_t)BLOCK, max(1u, (ntot + gridDim.x - 1) / gridDim.x));
   for (uint32_t c0 = blockIdx.x * per; c0 < ntot; c0 += gridDim.x * per) {
     const uint32_t q = c0 + threadIdx.x;
@@ -1695,7 +1698,7 @@ __global__ __launch_bounds__(BLOCK) void fused_side_kernel(Args<T, D> a, uint32_
       const bool mover = q < nm;
       uint32_t p;
       if (mover) {
-        const uint32_t r = spread_index(q, nm);
+        const uint32_t r = q;
         if (r < nc) {
           // G2P of a current mover from the HBM grid (MPM_DRIVER.cpp:450-515)
           const uint32_t qq = r < n4 ? r : a.mcap - 1u - (r - n4);
@@ -1736,7 +1739,7 @@ __global__ __launch_bounds__(BLOCK) void fused_side_kernel(Args<T, D> a, uint32_
           p = a.mover_list2[r - nc];
         }
       } else {
-        p = a.slow_list[spread_index(q - nm, ns)];
+        p = a.slow_list[q - nm];
       }
       side_coefficients<T, D, true>(a, p, mover, s_c, s_b, s_sc, (int)threadIdx.x);
     }
