@@ -1385,28 +1385,31 @@ __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
 }
 
 // ------------------------------------------------------------- fused step --
-// One workgroup per binned tile (grid-stride over the tile list): this step's
-// G2P and the next step's P2G of the tile's particles, whose state stays on
-// chip in between.  The particles are taken 256 at a time, one per thread, in
-// the stratified bin order (a chunk holds about four particles of each cell):
+// One workgroup of FB threads (W = FB / 64 wavefronts) per binned tile
+// (grid-stride over the tile list): this step's G2P and the next step's P2G
+// of the tile's particles, whose state stays on chip in between.  The
+// particles are taken FB at a time, one per thread, in the stratified bin
+// order (a chunk holds about W particles of each cell):
 //   G2P:      thread t gathers its particle from the tile's staged grid
 //             region, stores X, V, F, B and, if the new stencil base stays in
 //             the tile, computes the particle's P2G coefficients (stress from
 //             the new F, m v, m B D^-1, dt tau, x - x_base) in registers;
-//   grouping: wave-ballot ranks of the new base cells (stable), per-wave
-//             counts and offsets in LDS;
-//   rounds:   in round k the threads whose rank r in their cell has r / 4 = k
-//             write their coefficients to slot (r % 4) * 64 + cell, then lane
-//             l of wave w adds slot w * 64 + l into the wave's copy of the
-//             tile's region (p2g_scatter_slot: the 64 lanes of an instruction
-//             hold 64 distinct cells, so no two touch the same node);
+//   direct:   a wavefront whose lane l holds a particle of the cell labelled
+//             l (or none) adds its particles from registers into its copy of
+//             the tile's region;
+//   grouping: for the other particles, wave-ballot ranks of the new base
+//             cells (stable), per-wave counts and offsets in LDS;
+//   rounds:   in round k the threads whose rank r in their cell has r / W = k
+//             write their coefficients to slot (r % W) * 64 + cell, then lane
+//             l of wave w adds slot w * 64 + l into the wave's region copy
+//             (p2g_scatter_slot: the 64 lanes of an instruction hold 64
+//             distinct cells, so no two touch the same node);
 //   flush:    the copies are summed and added to grid_next, one atomic per
 //             touched node and component.
 // Particles of the current mover list (base outside the tile) are left to
-// g2p_mover_kernel<FUSED>; particles whose new base leaves the tile are
-// appended to the next mover list, and their P2G is left to
-// p2g_extra_kernel<FUSED>, which also adds the stress impulse of particles
-// whose stress needs the FP64 SVD (queued here with tau = 0).
+// fused_side_kernel; particles whose new base leaves the tile are appended to
+// the next mover list and particles whose stress needs the FP64 SVD to the
+// slow list (with tau = 0 here): fused_side_kernel adds their P2G.
 template<typename T, int D, bool FRICTION, int FB = BLOCK>
 __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (FRICTION ? 2 : 3) : 1))) void g2p2g_kernel(
     Args<T, D> a) {
