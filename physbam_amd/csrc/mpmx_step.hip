@@ -402,12 +402,6 @@ template<typename T> struct Vec4Type;
 template<> struct Vec4Type<float> { typedef float type __attribute__((ext_vector_type(4))); };
 template<> struct Vec4Type<double> { typedef double type __attribute__((ext_vector_type(4))); };
 
-template<typename T, int D, bool FRICTION, int NT = BLOCK>
-__device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>& g, const int* org,
-                                          typename Vec4Type<T>::type* sv);
-template<typename T, int D, bool FRICTION, bool FUSED>
-__device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, const Geom<T, D>& g, const int* org,
-                                              const typename Vec4Type<T>::type* sv);
 
 // The P2G of one particle with transfer coefficients cf (layout C), whose
 // base node is region node base_li, added into this wavefront's copy of the
@@ -557,14 +551,11 @@ __device__ __forceinline__ void p2g_scatter_slot(const Geom<T, D>& g, const T (*
   p2g_scatter_vals<T, D, C>(g, cf, base_li, reg);
 }
 
-// FUSED (g2p2g): each tile first runs the G2P of the current step
-// (g2p_tile_pass on a.grid), then the P2G of the next step of the same
-// particles into a.grid_next.  The P2G half reads the state the G2P half just
-// wrote (L2-resident) and skips particles whose base is outside the tile:
-// movers of the current list (still at the old position, left to
-// g2p_mover_kernel<FUSED>) and new movers (appended by the G2P half).
-template<typename T, int D, bool FUSED, bool FRICTION>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (FUSED ? 3 : 4) : 1))) void p2g_kernel(Args<T, D> a) {
+// The unfused P2G (first step of a run, adaptive substeps, taped steps):
+// one workgroup per binned tile; particles whose base is outside the tile go
+// to the mover list for p2g_extra_kernel.
+template<typename T, int D>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void p2g_kernel(Args<T, D> a) {
   constexpr int R = Tile<D>::R, RN = Tile<D>::RN, NW = BLOCK / 64, E = Tile<D>::E;
   constexpr int JT = 2, CH = JT * BLOCK;  // particles per chunk
   using C = P2GCoef<T, D>;
@@ -573,7 +564,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
   __shared__ union {
     T c[C::N][BLOCK];            // phase A -> phase B coefficients
     uint32_t wc[JT][NW][64];     // grouping: per (pass, wave) cell counts, then offsets
-    T4 v[FUSED ? (FRICTION ? 2 : 1) : 1][FUSED ? RN : 1];  // FUSED: staged grid of the G2P half
   } s_u;
   __shared__ uint32_t s_idx[CH];  // chunk's particle slots grouped by cell
   __shared__ uint32_t s_cb[64], s_cn[64], s_max;
@@ -591,12 +581,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
     for (int c = 0; c < D; c++) dinv[c] = (T)4 * g.inv_dx[c] * g.inv_dx[c];  // D_p^-1 (PARTICLE_GRID_WEIGHTS_SPLINE.cpp:152-158)
     int org[D];
     tile_origin<T, D>(t, g, org);
-    if constexpr (FUSED) {
-      g2p_stage<T, D, FRICTION>(a, g, org, &s_u.v[0][0]);
-      __syncthreads();
-      g2p_tile_pass<T, D, FRICTION, true>(a, t, g, org, &s_u.v[0][0]);
-      __syncthreads();
-    }
     for (int k = threadIdx.x; k < NW * RN; k += BLOCK) {
       T4 z = {0, 0, 0, 0};
       s_reg[k / RN][k % RN] = z;
@@ -628,7 +612,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
             const uint32_t cl = cell_label<D>(lc);
             if (in) {
               cell = cl;
-            } else if (!FUSED) {
+            } else {
               const uint32_t q = atomicAdd(&a.counts[4], 1u);  // aggregated per wavefront by the compiler
               a.mover_list[q] = p;
               a.mover_tile[q] = t;
@@ -736,7 +720,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T)
         region_offsets<D>(k, off);
 #pragma unroll
         for (int c = 0; c < D; c++) nc[c] = org[c] + off[c];
-        Node4<T>* nd = (FUSED ? a.grid_next : a.grid) + node_slot<T, D>(nc, g);
+        Node4<T>* nd = a.grid + node_slot<T, D>(nc, g);
         atomic_add(&nd->m, sum.x);
         atomic_add(&nd->x, sum.y);
         atomic_add(&nd->y, sum.z);
@@ -879,17 +863,15 @@ __device__ __forceinline__ void side_scatter(const Args<T, D>& a, Node4<T>* targ
   }
 }
 
-template<typename T, int D, bool FUSED>
+// The side path of the unfused P2G: the movers p2g_kernel listed (counts[4])
+// and the particles whose stress needs the FP64 SVD (counts[3]), with global
+// atomics into a.grid.
+template<typename T, int D>
 __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
   __shared__ T s_c[ScatterCoef<T, D>::N][BLOCK];
   __shared__ int s_b[D][BLOCK];
   __shared__ uint16_t s_sc[BLOCK];
-  // FUSED: the movers are the fused step's current list (bottom counts[4],
-  // top counts[6]; their new state is already stored) followed by the bottom
-  // of the next list (counts[5]); the target is grid_next
-  const uint32_t n4 = a.counts[4], n6 = FUSED ? a.counts[6] : 0u, n5 = FUSED ? a.counts[5] : 0u;
-  const uint32_t nm = n4 + n6 + n5, ns = a.counts[3], ntot = nm + ns;
-  Node4<T>* const target = FUSED ? a.grid_next : a.grid;
+  const uint32_t nm = a.counts[4], ns = a.counts[3], ntot = nm + ns;
   // particles per block and pass: short lists are spread over every block
   const uint32_t per = min((uint32_t)BLOCK, max(1u, (ntot + gridDim.x - 1) / gridDim.x));
   for (uint32_t c0 = blockIdx.x * per; c0 < ntot; c0 += gridDim.x * per) {
@@ -899,17 +881,15 @@ __global__ __launch_bounds__(BLOCK) void p2g_extra_kernel(Args<T, D> a) {
       const bool mover = q < nm;
       uint32_t p;
       if (mover) {
-        const uint32_t r = spread_index(q, nm);
-        p = r < n4 ? a.mover_list[r]
-                   : (r < n4 + n6 ? a.mover_list[a.mcap - 1u - (r - n4)] : a.mover_list2[r - n4 - n6]);
+        p = a.mover_list[spread_index(q, nm)];
       } else {
         p = a.slow_list[spread_index(q - nm, ns)];
       }
-      side_coefficients<T, D, FUSED>(a, p, mover, s_c, s_b, s_sc, (int)threadIdx.x);
+      side_coefficients<T, D, false>(a, p, mover, s_c, s_b, s_sc, (int)threadIdx.x);
     }
     __syncthreads();
     // ---- pass 2: one thread per (particle, stencil node)
-    side_scatter<T, D>(a, target, s_c, s_b, s_sc, min(per, ntot - c0));
+    side_scatter<T, D>(a, a.grid, s_c, s_b, s_sc, min(per, ntot - c0));
     __syncthreads();
   }
 }
@@ -1303,7 +1283,7 @@ __device__ __forceinline__ bool g2p_store(const Args<T, D>& a, const Geom<T, D>&
 // the velocity is the first 12 bytes of a 16-byte-aligned node, so a gather
 // reads it with one 12-byte LDS load (16-byte node stride across the lanes;
 // single-component loads at that stride would hit every bank four times).
-template<typename T, int D, bool FRICTION, int NT>
+template<typename T, int D, bool FRICTION, int NT = BLOCK>
 __device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>& g, const int* org,
                                           typename Vec4Type<T>::type* sv) {
   using T4 = typename Vec4Type<T>::type;
@@ -1323,10 +1303,8 @@ __device__ __forceinline__ void g2p_stage(const Args<T, D>& a, const Geom<T, D>&
 }
 
 // G2P of the particles binned to tile t from the staged region; movers
-// (base outside the tile) are left to g2p_mover_kernel.  FUSED: particles
-// whose new base leaves the tile are appended to the bottom of the next
-// mover list (counts[5]).
-template<typename T, int D, bool FRICTION, bool FUSED>
+// (base outside the tile) are left to g2p_mover_kernel.
+template<typename T, int D, bool FRICTION>
 __device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, const Geom<T, D>& g, const int* org,
                                               const typename Vec4Type<T>::type* sv) {
   constexpr int R = Tile<D>::R, E = Tile<D>::E, RN = Tile<D>::RN;
@@ -1358,11 +1336,7 @@ __device__ __forceinline__ void g2p_tile_pass(const Args<T, D>& a, uint32_t t, c
     };
     T vb[D], vfb[D], G[D * D], Bs[D * D];
     g2p_gather<T, D, FRICTION>(g, a.dt, w, nv, nf, vb, vfb, G, Bs);
-    if (g2p_store<T, D, !FUSED>(a, g, p, x, vb, vfb, G, Bs, t) && FUSED) {
-      const uint32_t k = atomicAdd(&a.counts[5], 1u);
-      a.mover_list2[k] = p;
-      a.mover_tile2[k] = t;
-    }
+    g2p_store<T, D, true>(a, g, p, x, vb, vfb, G, Bs, t);
   }
 }
 
@@ -1379,7 +1353,7 @@ __global__ __launch_bounds__(BLOCK) void g2p_kernel(Args<T, D> a) {
     tile_origin<T, D>(t, g, org);
     g2p_stage<T, D, FRICTION>(a, g, org, &s_v[0][0]);
     __syncthreads();
-    g2p_tile_pass<T, D, FRICTION, false>(a, t, g, org, &s_v[0][0]);
+    g2p_tile_pass<T, D, FRICTION>(a, t, g, org, &s_v[0][0]);
     __syncthreads();
   }
 }
@@ -1776,18 +1750,15 @@ __global__ __launch_bounds__(BLOCK) void fused_side_kernel(Args<T, D> a, uint32_
   }
 }
 
-// G2P of the movers listed by P2G this step, reading the grid from HBM.
-// FUSED: the current list of the fused step (counts[4] entries from the
-// bottom, counts[6] from the top); particles still outside their binned tile
-// are appended to the top of the next list.
-template<typename T, int D, bool FRICTION, bool FUSED>
+// G2P of the movers the unfused P2G listed this step (counts[4]), reading
+// the grid from HBM.
+template<typename T, int D, bool FRICTION>
 __global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
-  const uint32_t nb = a.counts[4], nm = nb + (FUSED ? a.counts[6] : 0u);
+  const uint32_t nm = a.counts[4];
   // consecutive list entries go to different blocks, so a short list still
   // occupies every compute unit
   for (uint32_t r = threadIdx.x * gridDim.x + blockIdx.x; r < nm; r += gridDim.x * BLOCK) {
-    const uint32_t s_ = spread_index(r, nm);
-    const uint32_t q = s_ < nb ? s_ : a.mcap - 1u - (s_ - nb);
+    const uint32_t q = spread_index(r, nm);
     const uint32_t p = a.mover_list[q];
     const Geom<T, D> g = particle_geom(a, p);
     T x[D];
@@ -1814,12 +1785,7 @@ __global__ __launch_bounds__(BLOCK) void g2p_mover_kernel(Args<T, D> a) {
     };
     T vb[D], vfb[D], G[D * D], Bs[D * D];
     g2p_gather<T, D, FRICTION>(g, a.dt, w, nv, nf, vb, vfb, G, Bs);
-    const uint32_t bt = a.mover_tile[q];
-    if (g2p_store<T, D, !FUSED>(a, g, p, x, vb, vfb, G, Bs, bt) && FUSED) {
-      const uint32_t k = a.mcap - 1u - atomicAdd(&a.counts[7], 1u);
-      a.mover_list2[k] = p;
-      a.mover_tile2[k] = bt;
-    }
+    g2p_store<T, D, true>(a, g, p, x, vb, vfb, G, Bs, a.mover_tile[q]);
   }
 }
 
@@ -4878,8 +4844,8 @@ class Engine final : public EngineBase {
         if (s.empty) break;
         const Args<T, D>& a = s.a;
         hipEvent_t e1 = timer_.begin(stream_);
-        p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-        p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+        p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+        p2g_extra_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
         // The grid above holds P2G's momentum plus the elastic impulse of the
         // forces step.  Grid statistics and a step run stage by stage need
         // Particle_To_Grid's own result (MPM_DRIVER.cpp:245-306, reflection
@@ -4948,10 +4914,10 @@ class Engine final : public EngineBase {
         }
         if (s.fric) {
           g2p_kernel<T, D, true><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-          g2p_mover_kernel<T, D, true, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+          g2p_mover_kernel<T, D, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
         } else {
           g2p_kernel<T, D, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-          g2p_mover_kernel<T, D, false, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+          g2p_mover_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
         }
         timer_.end(PH_G2P, e3, stream_);
         break;
@@ -5090,8 +5056,8 @@ class Engine final : public EngineBase {
     Args<T, D> b = a;
     b.dt = 0;
     b.grid = vsave_.p;
-    p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(b);
-    p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(b);
+    p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(b);
+    p2g_extra_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(b);
     if (refl_on_ && !refl_full_)
       for (int k2 = 0; k2 < D; k2++)
         reflect_mass_momentum_kernel<T, D><<<reflect_blocks(k2), BLOCK, 0, stream_>>>(vsave_.p, scenes_dev_.p,
@@ -5110,8 +5076,8 @@ class Engine final : public EngineBase {
     Args<T, D> b = a;
     b.dt = 0;
     b.grid = vsave_.p;
-    p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(b);
-    p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(b);
+    p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(b);
+    p2g_extra_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(b);
     normalize_nodes_kernel<T><<<grid_blocks(), BLOCK, 0, stream_>>>(vsave_.p, a.gtiles, counts_.p + 1);
     T v[2];
     dt_limit_values(a, lim_strong_, lim_sound_, cfl, lim_cfl_F_, v);
@@ -5180,8 +5146,8 @@ class Engine final : public EngineBase {
     a.order = identity_order_ ? nullptr : order_.p;
     a.next_stamp = ms + 1;
     hipEvent_t e1 = timer_.begin(stream_);
-    p2g_kernel<T, D, false, false><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
-    p2g_extra_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+    p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
+    p2g_extra_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
     fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 2);
     timer_.end(PH_P2G, e1, stream_);
     HIPCHECK(hipGetLastError());
@@ -5190,7 +5156,7 @@ class Engine final : public EngineBase {
   }
 
   // One step with the G2P of this step and the P2G of the next one fused
-  // (p2g_kernel<FUSED>).  The P2G uses this step's dt; a later step with another
+  // (g2p2g_kernel).  The P2G uses this step's dt; a later step with another
   // dt, or any step through step_once, starts the pipeline again.
   void step_fused(T dt) {
     const bool fric = any_friction();

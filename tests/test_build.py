@@ -15,14 +15,14 @@ REPORT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 HOT = {
     "_ZN4mpmx12g2p2g_kernelIfLi3ELb0E": 0,     # fused G2P -> P2G, the step's main kernel
     "_ZN4mpmx12g2p2g_kernelIfLi3ELb1E": 0,     # the same with wall friction
-    "_ZN4mpmx10p2g_kernelIfLi3ELb0ELb0E": 8,   # unfused P2G (first step of a run)
+    "_ZN4mpmx10p2g_kernelIfLi3EE": 8,         # unfused P2G (first step of a run)
     "_ZN4mpmx10g2p_kernelIfLi3ELb0E": 0,
     "_ZN4mpmx10g2p_kernelIfLi3ELb1E": 0,
     "_ZN4mpmx11grid_kernelIfLi3ELb0E": 0,
     "_ZN4mpmx11grid_kernelIfLi3ELb1E": 0,
-    "_ZN4mpmx16p2g_extra_kernelIfLi3ELb0E": 0,  # unfused mover / slow-list P2G
+    "_ZN4mpmx16p2g_extra_kernelIfLi3EE": 0,   # unfused mover / slow-list P2G
     "_ZN4mpmx17fused_side_kernelIfLi3ELb0E": 0,  # fused step's movers and slow list
-    "_ZN4mpmx16g2p_mover_kernelIfLi3ELb0ELb0E": 0,
+    "_ZN4mpmx16g2p_mover_kernelIfLi3ELb0EE": 0,
     "_ZN4mpmx28reflect_mass_momentum_kernelIfLi3E": 0,
     "_ZN4mpmx23reflect_velocity_kernelIfLi3E": 0,
 }
