@@ -568,6 +568,14 @@ template<typename T> struct EigenSweeps;
 template<> struct EigenSweeps<float> { static constexpr int n = 4; };
 template<> struct EigenSweeps<double> { static constexpr int n = 6; };
 
+// Off-diagonal entries below this are left alone: a rotation for them moves
+// nothing a strain of T's precision can hold, and 1 / a_pq of them would
+// overflow (with equal diagonal entries, as for repeated stretches, the angle
+// became 0 * inf = NaN).
+template<typename T> struct TinyOffDiag;
+template<> struct TinyOffDiag<float> { static constexpr float v = 1e-18f; };
+template<> struct TinyOffDiag<double> { static constexpr double v = 1e-150; };
+
 // Cyclic Jacobi on a symmetric D x D matrix A (in place); U collects the
 // rotations so that A_in = U diag(A_out) U^T.
 template<typename T, int D>
@@ -584,7 +592,7 @@ __device__ __forceinline__ void sym_eigen(T (&A)[D][D], T (&U)[D][D]) {
     for (int k = 0; k < NP; k++) {
       const int p = (D == 3 && k == 2) ? 1 : 0, q = (D == 3 && k != 0) ? 2 : 1;
       const T apq = A[p][q];
-      if (apq != (T)0) {
+      if (tabs(apq) > TinyOffDiag<T>::v) {
         const T theta = (A[q][q] - A[p][p]) * (T)0.5 * fast_rcp<T>(apq);
         T t = fast_rcp<T>(tabs(theta) + fast_sqrt<T>((T)1 + theta * theta));
         t = theta < 0 ? -t : t;
