@@ -301,33 +301,39 @@ def test_stress_near_rotations(dim, prec):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
-def test_stress_repeated_stretches(prec):
-    """Fixed-corotated stress past the series radius where the stretches of F
-    repeat: F = s R (all three equal: the characteristic cubic of F F^T - I
-    has a triple root), F = R diag(a, a, b) (a double one) and nearly double
-    ones, against the FP64 oracle; FP32 to 2e-5 of the largest entry, as the
-    other stress tests."""
+@pytest.mark.parametrize("model", [mt.COROTATED, mt.HENCKY])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_stress_repeated_stretches(dim, model, prec):
+    """Stress past the series radius where the stretches of F repeat: F = s R
+    (all equal: the characteristic polynomial of F F^T - I has a root of
+    multiplicity d), in 3D F = R diag(a, a, b) (a double root), and nearly
+    repeated stretches, against the FP64 oracle; FP32 to 2e-5 of the largest
+    entry, as the other stress tests."""
     rng = np.random.default_rng(17)
+    if dim == 3:
+        sets = ([1.3, 1.3, 1.3], [0.8, 0.8, 0.8], [1.25, 1.25, 0.9], [0.85, 1.2, 1.2],
+                [1.2, 1.2 * (1 + 1e-6), 0.9], [1.5, 0.7, 0.7 * (1 - 1e-5)])
+    else:
+        sets = ([1.3, 1.3], [0.8, 0.8], [1.2, 1.2 * (1 + 1e-6)], [0.7, 0.7 * (1 - 1e-5)])
     mats = []
     for _ in range(64):
-        R = np.linalg.qr(rng.standard_normal((3, 3)))[0]
+        R = np.linalg.qr(rng.standard_normal((dim, dim)))[0]
         R[:, -1] *= np.sign(np.linalg.det(R))
-        Q = np.linalg.qr(rng.standard_normal((3, 3)))[0]
-        for st in ([1.3, 1.3, 1.3], [0.8, 0.8, 0.8], [1.25, 1.25, 0.9], [0.85, 1.2, 1.2],
-                   [1.2, 1.2 * (1 + 1e-6), 0.9], [1.5, 0.7, 0.7 * (1 - 1e-5)]):
+        Q = np.linalg.qr(rng.standard_normal((dim, dim)))[0]
+        for st in sets:
             mats.append(R @ Q @ np.diag(st) @ Q.T)
-    F = np.stack(mats).transpose(0, 2, 1).reshape(len(mats), 9)  # column-major
+    F = np.stack(mats).transpose(0, 2, 1).reshape(len(mats), dim * dim)  # column-major
     n = F.shape[0]
     vol = np.full(n, 1.5e-6)
     mu, la = mt.lame(rng.uniform(1e3, 1e5, n), rng.uniform(0.2, 0.4, n))
     dt = np.float32 if prec == 4 else np.float64
     F, vol, mu, la = (np.asarray(x, dt).astype(np.float64) for x in (F, vol, mu, la))
-    with mpmx.Context(3, 8, [0] * 3, [1] * 3, precision=prec) as ctx:
-        ctx.set_constitutive(mt.COROTATED, 0.25)
+    with mpmx.Context(dim, 8, [0] * dim, [1] * dim, precision=prec) as ctx:
+        ctx.set_constitutive(model, 0.25)
         tau = ctx.stress(F, vol, mu, la)
-    ref = mt.oracle_stress(F, vol, mu, la, 3, mt.COROTATED, 0.25, prec=8)
+    ref = mt.oracle_stress(F, vol, mu, la, dim, model, 0.25, prec=8)
     err = (np.abs(tau - ref) / np.abs(ref).max(axis=1, keepdims=True)).max()
-    print("repeated stretches", prec, f"{err:.2e}", flush=True)
+    print("repeated stretches", dim, model, prec, f"{err:.2e}", flush=True)
     assert err < (1e-10 if prec == 8 else 2e-5)
 
 
