@@ -114,6 +114,25 @@ def test_ten_steps_match_oracle(dim, prec, model, wtype, fric):
         print(tag, "fp32 oracle vs fp64 oracle", {k: f"{v:.2e}" for k, v in r.items()}, flush=True)
 
 
+@pytest.mark.parametrize("threads,wg", [(256, 0), (128, 512), (256, 512)])
+@pytest.mark.parametrize("dim,prec,model,wtype,fric", [CASES[1], CASES[4], CASES[7]])
+def test_ten_steps_with_other_launch_configs(dim, prec, model, wtype, fric, threads, wg):
+    """Ten fused steps with the fused kernel at 256 threads per workgroup
+    and/or a fixed count of workgroups striding over the tile list, against
+    the FP64 oracle to the bounds of the default configuration (2D and 3D,
+    FP64 and FP32, friction)."""
+    sc = scene(dim, prec, model, wtype, fric)
+    tag = f"launch t{threads} w{wg} d{dim} p{prec} m{model} w{wtype} f{fric}"
+    print(tag, flush=True)
+    orc64 = mt.oracle_run(sc, 10, prec=8, want_grid=False)
+    with gpu_context(sc) as ctx:
+        ctx.set_launch_config(graph=False, tile_workgroups=wg, tile_threads=threads)
+        ctx.set_reorder_interval(3)
+        ctx.step(sc["dt"], 10)
+        gpu = ctx.get_particles()
+    compare(gpu, orc64, 1e-8 if prec == 8 else 1e-4, tag)
+
+
 # Bins are rebuilt only when enough particles have left their binned tile;
 # until then those particles go through the mover kernels.  Each policy must
 # reproduce the oracle: never rebuilding (every tile change is a mover),
