@@ -507,6 +507,8 @@ def main():
     ap.add_argument("--rebin-interval", type=int, default=0, help="also rebuild the bins every K steps (0: never)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the forward+adjoint / forward measurement and print its JSON")
+    ap.add_argument("--deformed-only", action="store_true",
+                    help="only run the deformed 12-box scene (autotuned) and print its JSON")
     ap.add_argument("--cpu-table", type=str, default="", metavar="THREADS",
                     help="only time the reference CPU step of every configuration at these comma-separated "
                          "OpenMP thread counts and print the rows as JSON")
@@ -540,6 +542,10 @@ def main():
 
     if args.adjoint_only:
         print(json.dumps(adjoint_ratio(log)), flush=True)
+        return
+
+    if args.deformed_only:
+        print(json.dumps(deformed_throughput(log)), flush=True)
         return
 
     if args.gpus != 1:
