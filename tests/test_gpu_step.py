@@ -377,3 +377,31 @@ def test_stress_kernel(dim, model, prec):
     err = (np.abs(tau - ref) / scale).max()
     print("stress", dim, model, prec, err, flush=True)
     assert err < (1e-10 if prec == 8 else 2e-5)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_no_particles_and_all_invalid(prec):
+    """A context without particles steps (time advances, nothing else), and a
+    scene whose particles are all invalid steps through the fused and unfused
+    paths with an empty tile list, leaving every particle where it was."""
+    sc = scene(3, prec, mt.COROTATED, mt.STICK, 0.0)
+    d = 3
+    with mpmx.Context(d, sc["N"], sc["lo"], sc["hi"], precision=prec) as ctx:
+        ctx.set_gravity(sc["gravity"])
+        z = lambda *s: np.zeros(s)
+        ctx.set_particles(z(0, 3), z(0, 3), z(0, 9), z(0, 9), z(0), z(0), z(0), z(0), np.zeros(0, np.uint8))
+        ctx.step(sc["dt"], 3)
+        assert ctx.time == pytest.approx(3 * sc["dt"])
+        assert ctx.get_particles()["X"].shape[0] == 0
+    sc0 = dict(sc, valid=np.zeros_like(sc["valid"]))
+    for fused in (True, False):
+        with gpu_context(sc0) as ctx:
+            if not fused:
+                ctx.enable_grid_stats()  # grid statistics run the unfused step
+            ctx.step(sc["dt"], 4)
+            got = ctx.get_particles()
+            assert not got["valid"].any()
+            for k in ("X", "V", "F", "B"):
+                assert np.array_equal(got[k], np.asarray(sc0[k], got[k].dtype)), (fused, k)
+            gm, _ = ctx.get_grid()
+            assert not np.any(gm), fused
