@@ -405,3 +405,19 @@ def test_no_particles_and_all_invalid(prec):
                 assert np.array_equal(got[k], np.asarray(sc0[k], got[k].dtype)), (fused, k)
             gm, _ = ctx.get_grid()
             assert not np.any(gm), fused
+
+
+@pytest.mark.parametrize("dim,prec", [(2, 8), (3, 8), (3, 4)])
+def test_single_particle_matches_oracle(dim, prec):
+    """One particle (a tile holding a single particle, no other tile active),
+    five steps against the oracle."""
+    sc = scene(dim, prec, mt.COROTATED, mt.SLIP, 0.2)
+    i = int(np.argmax(sc["X"][:, 1]))  # the highest particle: it falls freely for a while
+    sc1 = dict(sc)
+    for k in ("X", "V", "F", "B", "mass", "volume", "mu", "lambda", "valid"):
+        sc1[k] = np.ascontiguousarray(np.asarray(sc[k])[i:i + 1])
+    orc = mt.oracle_run(sc1, 5, prec=8, want_grid=False)
+    with gpu_context(sc1) as ctx:
+        ctx.step(sc["dt"], 5)
+        gpu = ctx.get_particles()
+    compare(gpu, orc, 1e-9 if prec == 8 else 1e-5, f"single particle d{dim} p{prec}")
