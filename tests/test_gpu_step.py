@@ -452,3 +452,27 @@ def test_crowded_cell_matches_oracle(prec, threads):
         ctx.step(sc["dt"], 3)
         gpu = ctx.get_particles()
     compare(gpu, orc, 1e-9 if prec == 8 else 1e-5, f"crowded cell p{prec} t{threads}")
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_fast_particles_cross_tiles(prec):
+    """A body translating by (1.5, -2, 1) cells per step and spinning at up
+    to two cells per step at its rim (no CFL limit, as the reference allows
+    with a fixed dt): with the bins never rebuilt, the particles cross whole
+    tiles (about five cells in three steps), most become movers and a few
+    leave the domain; three fused steps against the FP64 oracle."""
+    sc = scene(3, prec, mt.COROTATED, mt.SLIP, 0.0)
+    h, dt = 1.0 / sc["N"][0], sc["dt"]
+    X = np.asarray(sc["X"], np.float64)
+    r = X - X.mean(axis=0)
+    w = np.array([0.3, 1.0, -0.5])
+    w *= 2.0 * h / dt / (np.linalg.norm(w) * np.linalg.norm(r, axis=1).max())
+    V = np.array([1.5, -2.0, 1.0]) * h / dt + np.cross(w, r)
+    sc1 = dict(sc, V=np.asarray(V, np.asarray(sc["V"]).dtype))
+    orc = mt.oracle_run(sc1, 3, prec=8, want_grid=False)
+    print("fast particles: valid after 3 steps", int(orc["valid"].sum()), "of", X.shape[0], flush=True)
+    with gpu_context(sc1) as ctx:
+        ctx.set_rebin_policy(2.0, 0)  # never rebuild: every tile change stays a mover
+        ctx.step(dt, 3)
+        gpu = ctx.get_particles()
+    compare(gpu, orc, 1e-9 if prec == 8 else 1e-5, f"fast particles p{prec}")
