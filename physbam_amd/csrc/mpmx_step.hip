@@ -1648,21 +1648,21 @@ __global__ void fused_counts_kernel(uint32_t* counts, uint32_t* last, int mode) 
   }
 }
 
-// The side path of a fused step in one launch (in place of g2p_mover_kernel,
-// p2g_extra_kernel and fused_counts_kernel): the current mover list (stencil
+// The side path of a fused step in one launch (in place of g2p_mover_kernel
+// and p2g_extra_kernel): the current mover list (stencil
 // base outside the binned tile at the start of the step: counts[4] entries
 // from the bottom of mover_list, counts[6] from the top) is gathered from the
 // HBM grid, stored, and then scattered into grid_next together with the
 // movers the tile kernel appended (counts[5], bottom of mover_list2) and the
 // stress impulse of the slow list (counts[3]).  A current mover still outside
-// its binned tile goes to the top of the next list.  The block that finishes
-// last (ticket in counts[12]) does the list bookkeeping of the step.
+// its binned tile goes to the top of the next list.  fused_counts_kernel
+// (mode 1) does the list bookkeeping of the step after it: a separate launch
+// orders it after every workgroup's appends without device-scope fences.
 template<typename T, int D, bool FRICTION>
-__global__ __launch_bounds__(BLOCK) void fused_side_kernel(Args<T, D> a, uint32_t* last) {
+__global__ __launch_bounds__(BLOCK) void fused_side_kernel(Args<T, D> a) {
   __shared__ T s_c[ScatterCoef<T, D>::N][BLOCK];
   __shared__ int s_b[D][BLOCK];
   __shared__ uint16_t s_sc[BLOCK];
-  __shared__ uint32_t s_ticket;
   const uint32_t n4 = a.counts[4], n6 = a.counts[6], n5 = a.counts[5];
   const uint32_t nc = n4 + n6, nm = nc + n5, ns = a.counts[3], ntot = nm + ns;
   // each workgroup takes consecutive list entries (the entries one wavefront
@@ -1723,30 +1723,6 @@ __global__ __launch_bounds__(BLOCK) void fused_side_kernel(Args<T, D> a, uint32_
     __syncthreads();
     side_scatter<T, D>(a, a.grid_next, s_c, s_b, s_sc, min(per, ntot - c0));
     __syncthreads();
-  }
-  // the last block to finish: bookkeeping of fused_counts_kernel mode 1.  One
-  // acquire-release ticket per block, taken by one thread after the block's
-  // barrier (a device-scope fence in every wave writes the L2 back once per
-  // wave); the counts the last block reads are device-scope atomics.
-  __syncthreads();
-  if (threadIdx.x == 0)
-    s_ticket = __hip_atomic_fetch_add(&a.counts[12], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  if (s_ticket == gridDim.x - 1 && threadIdx.x == 0) {
-    uint32_t* counts = a.counts;
-    const uint32_t c4 = atomicAdd(&counts[4], 0u), c5 = atomicAdd(&counts[5], 0u);
-    const uint32_t c6 = atomicAdd(&counts[6], 0u), c7 = atomicAdd(&counts[7], 0u);
-    const uint32_t c3 = atomicAdd(&counts[3], 0u), c1 = atomicAdd(&counts[1], 0u);
-    counts[10] = c4 + c6;  // movers of the step just taken
-    counts[4] = c5;
-    counts[6] = c7;
-    counts[9] = c5 + c7;
-    last[1] = c1;
-    counts[11] = c3;  // FP64-stress particles of the P2G just done
-    counts[3] = 0;
-    counts[5] = 0;
-    counts[7] = 0;
-    counts[12] = 0;
   }
 }
 
@@ -5198,9 +5174,10 @@ class Engine final : public EngineBase {
     launch_fused_kernel(a, fric);
     timer_.end(PH_FUSED, ek, stream_);
     if (fric)
-      fused_side_kernel<T, D, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a, lastcnt_.p);
+      fused_side_kernel<T, D, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
     else
-      fused_side_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a, lastcnt_.p);
+      fused_side_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
+    fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 1);
     timer_.end(PH_G2P2G, e3, stream_);
     const int slot = (int)(step_count_ & 3);
     HIPCHECK(hipMemcpyAsync(rb_host_ + slot, counts_.p + 9, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
