@@ -423,24 +423,24 @@ def test_single_particle_matches_oracle(dim, prec):
     compare(gpu, orc, 1e-9 if prec == 8 else 1e-5, f"single particle d{dim} p{prec}")
 
 
-@pytest.mark.parametrize("threads", [128, 256])
-@pytest.mark.parametrize("prec", [4, 8])
-def test_crowded_cell_matches_oracle(prec, threads):
+@pytest.mark.parametrize("dim,prec,threads", [(3, 4, 128), (3, 4, 256), (3, 8, 128), (3, 8, 256), (2, 4, 128),
+                                             (2, 8, 256)])
+def test_crowded_cell_matches_oracle(dim, prec, threads):
     """1500 particles inside one grid cell plus a sparse cloud around it: the
     fused kernel's per-cell ranks run far past the wavefronts of a workgroup
     (hundreds of rounds), three steps against the FP64 oracle."""
-    sc = scene(3, prec, mt.COROTATED, mt.STICK, 0.0)
+    sc = scene(dim, prec, mt.COROTATED, mt.STICK, 0.0)
     rng = np.random.default_rng(23)
     h = 1.0 / sc["N"][0]
     m = 1500
-    Xc = np.array([0.5, 0.5, 0.5]) + (0.1 + 0.8 * rng.random((m, 3))) * h  # one cell
-    keep = np.arange(0, sc["X"].shape[0], 7)                               # a sparse cloud
+    Xc = np.full(dim, 0.5) + (0.1 + 0.8 * rng.random((m, dim))) * h  # one cell
+    keep = np.arange(0, sc["X"].shape[0], 7)                          # a sparse cloud
     cat = lambda a, b: np.concatenate([np.asarray(a), np.asarray(b)]).astype(np.asarray(a).dtype)
     sc1 = dict(sc)
     sc1["X"] = cat(sc["X"][keep], Xc)
-    sc1["V"] = cat(sc["V"][keep], 0.2 * rng.standard_normal((m, 3)))
-    sc1["F"] = cat(sc["F"][keep], np.tile(np.eye(3).reshape(-1), (m, 1)) + 0.01 * rng.standard_normal((m, 9)))
-    sc1["B"] = cat(sc["B"][keep], np.zeros((m, 9)))
+    sc1["V"] = cat(sc["V"][keep], 0.2 * rng.standard_normal((m, dim)))
+    sc1["F"] = cat(sc["F"][keep], np.tile(np.eye(dim).reshape(-1), (m, 1)) + 0.01 * rng.standard_normal((m, dim * dim)))
+    sc1["B"] = cat(sc["B"][keep], np.zeros((m, dim * dim)))
     for k in ("mass", "volume", "mu", "lambda"):
         sc1[k] = cat(sc[k][keep], np.full(m, np.asarray(sc[k]).mean()))
     sc1["valid"] = cat(sc["valid"][keep], np.ones(m, np.uint8))
@@ -451,23 +451,28 @@ def test_crowded_cell_matches_oracle(prec, threads):
         ctx.set_launch_config(tile_threads=threads)
         ctx.step(sc["dt"], 3)
         gpu = ctx.get_particles()
-    compare(gpu, orc, 1e-9 if prec == 8 else 1e-5, f"crowded cell p{prec} t{threads}")
+    compare(gpu, orc, 1e-9 if prec == 8 else 1e-5, f"crowded cell d{dim} p{prec} t{threads}")
 
 
-@pytest.mark.parametrize("prec", [4, 8])
-def test_fast_particles_cross_tiles(prec):
+@pytest.mark.parametrize("dim,prec", [(3, 4), (3, 8), (2, 4), (2, 8)])
+def test_fast_particles_cross_tiles(dim, prec):
     """A body translating by (1.5, -2, 1) cells per step and spinning at up
     to two cells per step at its rim (no CFL limit, as the reference allows
     with a fixed dt): with the bins never rebuilt, the particles cross whole
     tiles (about five cells in three steps), most become movers and a few
     leave the domain; three fused steps against the FP64 oracle."""
-    sc = scene(3, prec, mt.COROTATED, mt.SLIP, 0.0)
+    sc = scene(dim, prec, mt.COROTATED, mt.SLIP, 0.0)
     h, dt = 1.0 / sc["N"][0], sc["dt"]
     X = np.asarray(sc["X"], np.float64)
     r = X - X.mean(axis=0)
-    w = np.array([0.3, 1.0, -0.5])
-    w *= 2.0 * h / dt / (np.linalg.norm(w) * np.linalg.norm(r, axis=1).max())
-    V = np.array([1.5, -2.0, 1.0]) * h / dt + np.cross(w, r)
+    rmax = np.linalg.norm(r, axis=1).max()
+    if dim == 3:
+        w = np.array([0.3, 1.0, -0.5])
+        w *= 2.0 * h / dt / (np.linalg.norm(w) * rmax)
+        spin = np.cross(w, r)
+    else:
+        spin = (2.0 * h / dt / rmax) * np.stack([-r[:, 1], r[:, 0]], axis=1)
+    V = np.array([1.5, -2.0, 1.0][:dim]) * h / dt + spin
     sc1 = dict(sc, V=np.asarray(V, np.asarray(sc["V"]).dtype))
     orc = mt.oracle_run(sc1, 3, prec=8, want_grid=False)
     print("fast particles: valid after 3 steps", int(orc["valid"].sum()), "of", X.shape[0], flush=True)
@@ -475,4 +480,4 @@ def test_fast_particles_cross_tiles(prec):
         ctx.set_rebin_policy(2.0, 0)  # never rebuild: every tile change stays a mover
         ctx.step(dt, 3)
         gpu = ctx.get_particles()
-    compare(gpu, orc, 1e-9 if prec == 8 else 1e-5, f"fast particles p{prec}")
+    compare(gpu, orc, 1e-9 if prec == 8 else 1e-5, f"fast particles d{dim} p{prec}")
