@@ -289,16 +289,32 @@ __device__ __forceinline__ bool grid_tile_flag(const Args<T, D>& a, const uint32
 }
 
 // GRID: touched grid tiles (octant rule); otherwise binned tiles (stamp == bs).
+// lists 1: the slow and mover lists of the step about to run start empty;
+// lists 2: the list bookkeeping at the end of a fused step (fused_counts_kernel
+// mode 1, done here to save a launch: this kernel runs right after the side
+// kernel and reads no counter, and the tile count in counts[1] is replaced
+// only by glist_kernel after it).
 template<typename T, int D, bool GRID>
 __global__ __launch_bounds__(BLOCK) void gcount_kernel(Args<T, D> a, const uint32_t* bstamp, uint32_t bs, uint32_t ms,
-                                                       uint32_t* block_cnt, int reset_lists) {
+                                                       uint32_t* block_cnt, int lists, uint32_t* last) {
   const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
-  if (reset_lists && t == 0) {  // slow and mover lists of the step about to run
+  if (lists == 1 && t == 0) {
     a.counts[3] = 0;
     a.counts[4] = 0;
     a.counts[5] = 0;
     a.counts[6] = 0;
     a.counts[7] = 0;
+  } else if (lists == 2 && t == 0) {
+    uint32_t* counts = a.counts;
+    counts[10] = counts[4] + counts[6];  // movers of the step just taken
+    counts[4] = counts[5];
+    counts[6] = counts[7];
+    counts[9] = counts[4] + counts[6];
+    last[1] = counts[1];
+    counts[11] = counts[3];  // FP64-stress particles of the P2G just done
+    counts[3] = 0;
+    counts[5] = 0;
+    counts[7] = 0;
   }
   const bool f = t < a.ntotal && (GRID ? grid_tile_flag<T, D>(a, bstamp, bs, ms, t) : bstamp[t] == bs);
   const int cnt = __syncthreads_count(f ? 1 : 0);
@@ -4605,7 +4621,7 @@ class Engine final : public EngineBase {
   void build_grid_tiles(uint32_t ms, Node4<T>* z0, Node4<T>* z1, Node4<T>* z2) {
     const uint32_t nb = cdiv(ntotal_, BLOCK);
     const Args<T, D> a = make_args(0);
-    gcount_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms, gblock_.p, 1);
+    gcount_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms, gblock_.p, 1, nullptr);
     glist_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms, gblock_.p, gt_cur_,
                                                         counts_.p + 1, z0, z1, z2);
   }
@@ -4736,7 +4752,7 @@ class Engine final : public EngineBase {
     stratify_kernel<<<cdiv(n_, BLOCK), BLOCK, 0, stream_>>>(skeys_.p, vtmp_.p, order_.p, tile_begin_.p, tile_end_.p,
                                                             n_);
     const uint32_t nb = cdiv(ntotal_, BLOCK);
-    gcount_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, 0u, gblock_.p, 0);
+    gcount_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, 0u, gblock_.p, 0, nullptr);
     glist_kernel<T, D, false><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, 0u, gblock_.p, ptiles_.p,
                                                          counts_.p + 0, nullptr, nullptr, nullptr);
     timer_.end(PH_REBIN, e0, stream_);
@@ -5177,24 +5193,26 @@ class Engine final : public EngineBase {
       fused_side_kernel<T, D, true><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
     else
       fused_side_kernel<T, D, false><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
-    fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 1);
     timer_.end(PH_G2P2G, e3, stream_);
+
+    // tile list of the next step's grid (now in gother_): binned tiles and
+    // the tiles marked by this step's movers; the friction grid is cleared
+    // over it.  gcount_kernel also does the step's list bookkeeping.
+    hipEvent_t e0 = timer_.begin(stream_);
+    const uint32_t nb = cdiv(ntotal_, BLOCK);
+    gcount_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms_next, gblock_.p, 2,
+                                                         lastcnt_.p);
+    glist_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms_next, gblock_.p, gt_other_,
+                                                        counts_.p + 1, fric ? gridf_.p : nullptr, nullptr, nullptr);
+    timer_.end(PH_BIN, e0, stream_);
+    // the mover count of this step (counts[9], set by the bookkeeping) for
+    // the host's rebin policy, read without stalling the stream
     const int slot = (int)(step_count_ & 3);
     HIPCHECK(hipMemcpyAsync(rb_host_ + slot, counts_.p + 9, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
     if (!capturing) {
       HIPCHECK(hipEventRecord(rb_ev_[slot], stream_));
       rb_meta_[slot] = Readback{step_count_, bin_stamp_};
     }
-
-    // tile list of the next step's grid (now in gother_): binned tiles and
-    // the tiles marked by this step's movers; the friction grid is cleared
-    // over it
-    hipEvent_t e0 = timer_.begin(stream_);
-    const uint32_t nb = cdiv(ntotal_, BLOCK);
-    gcount_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms_next, gblock_.p, 0);
-    glist_kernel<T, D, true><<<nb, BLOCK, 0, stream_>>>(a, tile_stamp_.p, bin_stamp_, ms_next, gblock_.p, gt_other_,
-                                                        counts_.p + 1, fric ? gridf_.p : nullptr, nullptr, nullptr);
-    timer_.end(PH_BIN, e0, stream_);
     HIPCHECK(hipGetLastError());
     fused_advance(dt);
   }
