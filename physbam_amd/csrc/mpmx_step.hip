@@ -290,10 +290,13 @@ __device__ __forceinline__ bool grid_tile_flag(const Args<T, D>& a, const uint32
 
 // GRID: touched grid tiles (octant rule); otherwise binned tiles (stamp == bs).
 // lists 1: the slow and mover lists of the step about to run start empty;
-// lists 2: the list bookkeeping at the end of a fused step (fused_counts_kernel
-// mode 1, done here to save a launch: this kernel runs right after the side
-// kernel and reads no counter, and the tile count in counts[1] is replaced
-// only by glist_kernel after it).
+// lists 2: the list bookkeeping at the end of a fused step, done here to save
+// a launch (this kernel runs right after the side kernel and reads no
+// counter, and the tile count in counts[1] is replaced only by glist_kernel
+// after it): the next mover lists become current ([4] <- [5], [6] <- [7]),
+// the pending grid's tile count becomes the last grid's (last[1] <- [1]),
+// [9] <- movers for the host's rebin policy, [10] <- movers of the step just
+// taken, [11] <- the P2G's FP64-stress count, and the slow list is consumed.
 template<typename T, int D, bool GRID>
 __global__ __launch_bounds__(BLOCK) void gcount_kernel(Args<T, D> a, const uint32_t* bstamp, uint32_t bs, uint32_t ms,
                                                        uint32_t* block_cnt, int lists, uint32_t* last) {
@@ -1635,28 +1638,18 @@ __global__ __launch_bounds__(BLOCK) void clear_tiles_kernel(Node4<T>* grid, cons
     grid[(uint64_t)tiles[idx >> 6] * 64u + (idx & 63u)] = z;
 }
 
-// Counter updates of the fused step.  mode 0 (bins rebuilt): the current
-// mover list is empty.  mode 1 (end of step): the next lists become current
-// ([4] <- [5], [6] <- [7]), the pending grid's tile count becomes the last
-// grid's (last[1] <- [1]), [9] <- movers for the host's rebin policy and
-// [10] <- movers of the step just taken.  mode 2: end of the priming P2G.
-// Modes 1 and 2 keep the P2G's FP64-stress count in [11].
-__global__ void fused_counts_kernel(uint32_t* counts, uint32_t* last, int mode) {
+// Counter updates of the fused pipeline.  mode 0 (bins rebuilt): the current
+// mover list is empty.  mode 2 (end of the priming P2G): the P2G's
+// FP64-stress count goes to [11], the slow list is consumed and the next
+// mover list starts empty.  (The end-of-step update of a fused step is done
+// by the first thread of gcount_kernel.)
+__global__ void fused_counts_kernel(uint32_t* counts, int mode) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (mode == 0) {
     counts[4] = 0;
     counts[6] = 0;
     counts[9] = 0;
-  } else if (mode == 1) {
-    counts[10] = counts[4] + counts[6];  // movers of the step just taken
-    counts[4] = counts[5];
-    counts[6] = counts[7];
-    counts[9] = counts[4] + counts[6];
-    last[1] = counts[1];
-  }
-  // modes 1 and 2 (end of a fused step, end of the priming P2G): the slow
-  // list is consumed and the next mover list starts empty
-  if (mode != 0) {
+  } else {
     counts[11] = counts[3];  // FP64-stress particles of the P2G just done
     counts[3] = 0;
     counts[5] = 0;
@@ -1671,9 +1664,9 @@ __global__ void fused_counts_kernel(uint32_t* counts, uint32_t* last, int mode) 
 // HBM grid, stored, and then scattered into grid_next together with the
 // movers the tile kernel appended (counts[5], bottom of mover_list2) and the
 // stress impulse of the slow list (counts[3]).  A current mover still outside
-// its binned tile goes to the top of the next list.  fused_counts_kernel
-// (mode 1) does the list bookkeeping of the step after it: a separate launch
-// orders it after every workgroup's appends without device-scope fences.
+// its binned tile goes to the top of the next list.  The first thread of the
+// next kernel (gcount_kernel, lists 2) does the step's list bookkeeping: the
+// kernel boundary orders it after every workgroup's appends.
 template<typename T, int D, bool FRICTION>
 __global__ __launch_bounds__(BLOCK) void fused_side_kernel(Args<T, D> a) {
   __shared__ T s_c[ScatterCoef<T, D>::N][BLOCK];
@@ -5140,7 +5133,7 @@ class Engine final : public EngineBase {
     hipEvent_t e1 = timer_.begin(stream_);
     p2g_kernel<T, D><<<tile_blocks(), BLOCK, 0, stream_>>>(a);
     p2g_extra_kernel<T, D><<<EXTRA_BLOCKS, BLOCK, 0, stream_>>>(a);
-    fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 2);
+    fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, 2);
     timer_.end(PH_P2G, e1, stream_);
     HIPCHECK(hipGetLastError());
     fused_ = true;
@@ -5157,7 +5150,7 @@ class Engine final : public EngineBase {
     if (!fused_ || fused_dt_ != dt) prime_fused(dt, fric);
     if (rebin_due()) {
       rebin();
-      fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 0);
+      fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, 0);
     }
     fused_body(dt, fric, false);
     timer_.end(PH_STEP, e_step, stream_);
@@ -5278,7 +5271,7 @@ class Engine final : public EngineBase {
       if (fric) gridf_.alloc((size_t)ntotal_ * 64);
       if (rebin_due()) {
         rebin();
-        fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, lastcnt_.p, 0);
+        fused_counts_kernel<<<1, 64, 0, stream_>>>(counts_.p, 0);
       }
       std::vector<unsigned char> sig = graph_signature(dt, fric);
       const int64_t s0 = step_count_;
